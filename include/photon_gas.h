@@ -1,0 +1,169 @@
+/*
+ * photon_gas.h -- C ABI of the MI355X (gfx950) gas-phase pressure projection.
+ *
+ * The reference has no plugin or FFI layer: the path is a set of private
+ * members of photon::Solver (src/sim/solver.h:290-336), called only from
+ * Solver::step (src/sim/solver.cpp:226-269).  These entry points are what a
+ * Solver built on this library binds in place of those members; each
+ * declaration cites the reference function it replaces.
+ *
+ * Conventions
+ *   - cell c = i + nx*(j + ny*k); x faces (nx+1)*ny*nz, y faces nx*(ny+1)*nz,
+ *     z faces nx*ny*(nz+1)  (src/sim/solver.h:216-220).
+ *   - Geometry arguments (phg_grid_create) are HOST pointers; field and
+ *     right-hand-side arguments of the compute entry points are DEVICE
+ *     pointers (from phg_malloc or any HIP allocation on the context's
+ *     device).  Work is queued on the context's stream; entry points that
+ *     return statistics synchronise that stream before returning.
+ *   - Every entry point returns 0 on success or a negative PHG_E* code;
+ *     phg_last_error() then holds a message (per context, not thread-safe
+ *     across threads sharing one context).  The library keeps no global
+ *     mutable state: contexts are independent.
+ */
+#ifndef PHOTON_GAS_H
+#define PHOTON_GAS_H
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+enum {
+    PHG_OK = 0,
+    PHG_EINVAL = -1,   /* bad argument or shape */
+    PHG_EHIP = -2,     /* a HIP call failed */
+    PHG_ENOMEM = -3,   /* device allocation failed */
+    PHG_ESTATE = -4    /* unknown handle */
+};
+
+/* storage precision of the multigrid preconditioner */
+enum {
+    PHG_PREC_FP32 = 0,       /* all levels FP32 (the reference's precision) */
+    PHG_PREC_MIXED_F16 = 1,  /* V-cycle levels and level-0 coefficients FP16, compute FP32 */
+    PHG_PREC_MIXED_BF16 = 2  /* same with BF16 storage */
+};
+
+/* stop rules */
+enum {
+    PHG_STOP_REL_L2 = 0,   /* ||r||_2 / ||b||_2 <= tol (FP64 norms of the recursive residual) */
+    PHG_STOP_MAXNORM = 1   /* max|r| / inv <= tol in 1/s, the reference's rule (gasphase.cpp:592-605) */
+};
+
+/* solve status */
+enum { PHG_RUNNING = 0, PHG_CONVERGED = 1, PHG_MAXITER = 2, PHG_BREAKDOWN = 3 };
+
+typedef struct phg_ctx phg_ctx;
+
+typedef struct {
+    double tol;          /* stop tolerance (see stop rule) */
+    int max_iters;       /* iteration cap (FidelityProfile::pressureMaxIters, solver.h:89) */
+    int stop_rule;       /* PHG_STOP_* */
+    float inv;           /* dx^2/dt, for PHG_STOP_MAXNORM */
+    int demean;          /* remove the mean of x on closed zones at the end (gasphase.cpp:628-639) */
+    int check_every;     /* iterations between host convergence checks (0 = library default) */
+    int use_graph;       /* capture the iteration sequence in a hipGraph (1) or launch directly (0) */
+    int zero_x;          /* start from x = 0 (x is written, not read) instead of warm-starting */
+} phg_solve_params;
+
+typedef struct {
+    int iters;           /* CG iterations (Solver::pressureIters, solver.h:277) */
+    int status;          /* PHG_CONVERGED / PHG_MAXITER / PHG_BREAKDOWN */
+    double relres;       /* final recursive ||r||/||b|| */
+    double maxres;       /* final max|r| / inv (Solver::pressureResidual, solver.h:278) */
+} phg_solve_stats;
+
+/* ---- context --------------------------------------------------------- */
+/* Replaces the JobPool the reference Solver owns (solver.h:291): one HIP
+ * stream on `device`. */
+int phg_create(phg_ctx **out, int device);
+void phg_destroy(phg_ctx *ctx);
+const char *phg_last_error(phg_ctx *ctx);
+int phg_synchronize(phg_ctx *ctx);
+void *phg_stream(phg_ctx *ctx);   /* the hipStream_t work is queued on */
+
+/* ---- device memory helpers (for callers without a HIP runtime of their own) */
+int phg_malloc(phg_ctx *ctx, void **dptr, size_t bytes);
+int phg_free(phg_ctx *ctx, void *dptr);
+int phg_memcpy_h2d(phg_ctx *ctx, void *dst, const void *src, size_t bytes);
+int phg_memcpy_d2h(phg_ctx *ctx, void *dst, const void *src, size_t bytes);
+int phg_memset(phg_ctx *ctx, void *dst, int value, size_t bytes);
+
+/* ---- geometry: one grid and its multigrid hierarchy -------------------
+ * Replaces Solver::buildPressureLevels (pressure_mg.cpp:28-191, run when
+ * mgDirty_ is set) and computeZones (gasphase.cpp:111-158).  ax/ay/az are the
+ * matrix coefficients (Solver::ax_/ay_/az_, solver.h:296), fx/fy/fz the face
+ * open fractions, blocked the solid mask; wind_closed (4 bytes: x-low,
+ * x-high, y-low, y-high, or NULL) flags sides where windAt() prescribes
+ * inflow (gasphase.cpp:52-61).  precision: PHG_PREC_*.  */
+int phg_grid_create(phg_ctx *ctx, int nx, int ny, int nz, float dx, const float *fx, const float *fy,
+                    const float *fz, const float *ax, const float *ay, const float *az, const uint8_t *blocked,
+                    const uint8_t *wind_closed, int precision, int *grid_out);
+int phg_grid_destroy(phg_ctx *ctx, int grid);
+/* hierarchy shape: level count, sizes (nlevels entries), zones */
+int phg_grid_info(phg_ctx *ctx, int grid, int *nlevels, int *level_n, int *nzones, int *ref_levels);
+/* zone labels (N ints, -1 solid) and open flags (nzones bytes), host out */
+int phg_grid_zones(phg_ctx *ctx, int grid, int *zone, uint8_t *zone_open);
+/* Host-only: the hierarchy phg_grid_create would build (no device needed).
+ * level_n: up to 12 entries; agg0 (N ints, may be NULL): the level-0 -> 1 map. */
+int phg_hierarchy_shape(int nx, int ny, int nz, const float *ax, const float *ay, const float *az,
+                        const uint8_t *blocked, const uint8_t *wind_closed, int *nlevels, int *level_n,
+                        int *agg0);
+
+/* ---- level-0 operator pieces (tests and callers of single kernels) ----
+ * y = A x with the fused FP64 dot x.y  (the apply lambda, gasphase.cpp:551-570) */
+int phg_apply(phg_ctx *ctx, int grid, const float *x, float *y, double *dot);
+/* z = M^-1 r, one symmetric V-cycle (Solver::pressurePrecondition, pressure_mg.cpp:270-300) */
+int phg_precondition(phg_ctx *ctx, int grid, const float *r, float *z);
+
+/* ---- batched MG-PCG ------------------------------------------------------
+ * A batch is a fixed set of systems (each on a grid, grids may differ or
+ * repeat) that advance together, one kernel launch per step for all of
+ * them.  The batch owns every work vector; the caller owns b and x. */
+int phg_batch_create(phg_ctx *ctx, int nsys, const int *grids, int *batch_out);
+int phg_batch_destroy(phg_ctx *ctx, int batch);
+/* The CG loop of Solver::project (gasphase.cpp:571-627) for every system:
+ * x (warm start in, solution out) and b are arrays of nsys device pointers.
+ * stats: nsys entries (may be NULL). */
+int phg_batch_solve(phg_ctx *ctx, int batch, const float *const *b, float *const *x, const phg_solve_params *params,
+                    phg_solve_stats *stats);
+/* one system: convenience over a cached one-system batch of `grid` */
+int phg_solve(phg_ctx *ctx, int grid, const float *b, float *x, const phg_solve_params *params,
+              phg_solve_stats *stats);
+
+/* ---- the projection step (Solver::project, gasphase.cpp:505-677) ------
+ * u,v,w (faces) and p (cells) in/out, S in; all device pointers.  Builds the
+ * RHS (:528-549), solves (:571-627, warm start from p), removes closed-zone
+ * means (:628-639) and corrects the face velocities (:640-676).
+ * max_vel out: Solver::maxVel.  */
+int phg_project(phg_ctx *ctx, int grid, float dt, float *u, float *v, float *w, float *p, const float *S,
+                const phg_solve_params *params, phg_solve_stats *stats, float *max_vel);
+/* the pieces of phg_project: RHS + diagonal (b, diag device out; p zeroed in solids) */
+int phg_rhs(phg_ctx *ctx, int grid, float dt, const float *u, const float *v, const float *w, const float *S,
+            float *b, float *diag, float *p);
+int phg_zone_mean(phg_ctx *ctx, int grid, float *p);
+int phg_correct(phg_ctx *ctx, int grid, float dt, const float *p, float *u, float *v, float *w, float *max_vel);
+
+/* ---- divergence source (Solver::energyAndExpansion, gasphase.cpp:251-271)
+ * T in/out, S out, Pb/Q/massSrc in (cells, device). max_div out: Solver::maxDiv. */
+int phg_energy_expansion(phg_ctx *ctx, int grid, float dt, float *T, float *S, const float *Pb, const float *Q,
+                         const float *massSrc, float *max_div);
+
+/* ---- scalar transport (Solver::advectScalars, gasphase.cpp:685-784) ----
+ * fields_in / fields_out: 10 device pointers each, in the order
+ * T, Yo, Yf, Ys, Ya, Yp, Yc, Yw, Yco, Yai (gasphase.cpp:687).  Out must not
+ * alias in (the reference writes tmp_ and swaps). */
+int phg_advect_scalars(phg_ctx *ctx, int grid, float dt, const float *u, const float *v, const float *w,
+                       const float *Pb, const float *nu, const float *const *fields_in, float *const *fields_out);
+
+/* ---- profiling: HIP-event timing of each kernel class on the context stream */
+int phg_profile_enable(phg_ctx *ctx, int on);
+/* kernel class names are returned by index; returns PHG_EINVAL past the last */
+int phg_profile_get(phg_ctx *ctx, int index, const char **name, double *total_ms, long long *launches,
+                    double *bytes);
+int phg_profile_reset(phg_ctx *ctx);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

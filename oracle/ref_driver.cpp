@@ -1,0 +1,166 @@
+// ref_driver.cpp -- C entry points over the reference's own photon::Solver,
+// used to pin the oracle and as the CPU baseline.  TEST INFRASTRUCTURE ONLY.
+//
+// Built by oracle/Makefile together with the reference's unmodified sources
+// (src/sim/*.cpp, src/scene/*.cpp under the reference checkout) into
+// oracle/_ref/libphoton_ref.so.  Only this file sees the Solver's private
+// members (it is compiled with private mapped to public): the synthetic
+// grids of the tests cannot be made through Scene + discretizer
+// (src/sim/discretize.cpp:18-20,282 always add margins and a roof), so the
+// driver writes the solver's arrays directly and calls the private phases.
+#include <algorithm>
+#include <cmath>
+#include <cstdint>
+#include <cstring>
+#include <memory>
+#include <string>
+#include <vector>
+#define private public
+#include "sim/solver.h"
+#undef private
+
+using photon::Solver;
+
+namespace {
+std::vector<float>* field(Solver* s, const char* name) {
+    std::string n(name);
+    if (n == "u") return &s->u;
+    if (n == "v") return &s->v;
+    if (n == "w") return &s->w;
+    if (n == "fx") return &s->fx;
+    if (n == "fy") return &s->fy;
+    if (n == "fz") return &s->fz;
+    if (n == "ax") return &s->ax_;
+    if (n == "ay") return &s->ay_;
+    if (n == "az") return &s->az_;
+    if (n == "p") return &s->p;
+    if (n == "S") return &s->S;
+    if (n == "T") return &s->T;
+    if (n == "Yo") return &s->Yo;
+    if (n == "Yf") return &s->Yf;
+    if (n == "Ys") return &s->Ys;
+    if (n == "Ya") return &s->Ya;
+    if (n == "Yp") return &s->Yp;
+    if (n == "Yc") return &s->Yc;
+    if (n == "Yw") return &s->Yw;
+    if (n == "Yco") return &s->Yco;
+    if (n == "Yai") return &s->Yai;
+    if (n == "nu") return &s->nu;
+    if (n == "Pb") return &s->Pb;
+    if (n == "Q") return &s->Q;
+    if (n == "massSrc") return &s->massSrc;
+    if (n == "rhs") return &s->rhs_;
+    if (n == "diag") return &s->tmp_[9];
+    return nullptr;
+}
+}  // namespace
+
+extern "C" {
+
+void* ref_create(int threads) { return new Solver(threads); }
+void ref_destroy(void* h) { delete static_cast<Solver*>(h); }
+
+// Geometry straight into the solver's arrays; zones from the reference's own
+// computeZones (gasphase.cpp:111-158).  Returns the zone count.
+int ref_setup(void* h, int nx, int ny, int nz, float dx, const float* fx, const float* fy, const float* fz,
+              const float* ax, const float* ay, const float* az, const uint8_t* blocked) {
+    Solver* s = static_cast<Solver*>(h);
+    s->nx = nx; s->ny = ny; s->nz = nz; s->N = nx * ny * nz;
+    s->dx = dx; s->ox = s->oy = s->oz = 0;
+    s->wind.speed = 0;
+    const size_t N = (size_t)s->N, nfx = (size_t)(nx + 1) * ny * nz, nfy = (size_t)nx * (ny + 1) * nz,
+                 nfz = (size_t)nx * ny * (nz + 1);
+    s->fx.assign(fx, fx + nfx); s->fy.assign(fy, fy + nfy); s->fz.assign(fz, fz + nfz);
+    s->ax_.assign(ax, ax + nfx); s->ay_.assign(ay, ay + nfy); s->az_.assign(az, az + nfz);
+    s->lx_.assign(nfx, 0); s->ly_.assign(nfy, 0); s->lz_.assign(nfz, 0);
+    s->u.assign(nfx, 0.0f); s->v.assign(nfy, 0.0f); s->w.assign(nfz, 0.0f);
+    s->blocked.assign(blocked, blocked + N);
+    s->p.assign(N, 0.0f); s->S.assign(N, 0.0f);
+    s->T.assign(N, Solver::T0);
+    s->Yo.assign(N, Solver::YO2_AMB); s->Yf.assign(N, 0.0f); s->Ys.assign(N, 0.0f); s->Ya.assign(N, 0.0f);
+    s->Yp.assign(N, 0.0f); s->Yc.assign(N, Solver::YCO2_AMB); s->Yw.assign(N, Solver::YH2O_AMB);
+    s->Yco.assign(N, 0.0f); s->Yai.assign(N, 0.0f);
+    s->nu.assign(N, 0.0f); s->Q.assign(N, 0.0f); s->massSrc.assign(N, 0.0f);
+    s->Pb.assign(N, Solver::P0);
+    s->rhs_.assign(N, 0.0f);
+    for (auto& t : s->tmp_) t.assign(N, 0.0f);
+    s->cgR_.assign(N, 0.0f); s->cgZ_.assign(N, 0.0f); s->cgP_.assign(N, 0.0f); s->cgQ_.assign(N, 0.0f);
+    s->zone.assign(N, -1);
+    s->computeZones();
+    s->mgDirty_ = true;
+    return (int)s->zoneP.size();
+}
+
+void ref_get_zones(void* h, int* zone, uint8_t* zoneOpen) {
+    Solver* s = static_cast<Solver*>(h);
+    std::copy(s->zone.begin(), s->zone.end(), zone);
+    std::copy(s->zoneOpen.begin(), s->zoneOpen.end(), zoneOpen);
+}
+
+int ref_set_field(void* h, const char* name, const float* data) {
+    std::vector<float>* f = field(static_cast<Solver*>(h), name);
+    if (!f) return -1;
+    std::copy(data, data + f->size(), f->begin());
+    return (int)f->size();
+}
+
+int ref_get_field(void* h, const char* name, float* out) {
+    std::vector<float>* f = field(static_cast<Solver*>(h), name);
+    if (!f) return -1;
+    std::copy(f->begin(), f->end(), out);
+    return (int)f->size();
+}
+
+// project(dt) with the given stop rule parameters (solver.h:89-90): returns iterations.
+int ref_project(void* h, float dt, float tol, int maxIters, float* residual, float* maxVel) {
+    Solver* s = static_cast<Solver*>(h);
+    s->profile.pressureTol = tol;
+    s->profile.pressureMaxIters = maxIters;
+    s->project(dt);
+    if (residual) *residual = s->pressureResidual;
+    if (maxVel) *maxVel = s->maxVel;
+    return s->pressureIters;
+}
+
+void ref_precondition(void* h, const float* r, float* z) {
+    Solver* s = static_cast<Solver*>(h);
+    std::vector<float> rv(r, r + s->N), zv(s->N, 0.0f);
+    s->pressurePrecondition(rv, zv);
+    std::copy(zv.begin(), zv.end(), z);
+}
+
+static void ensure_levels(Solver* s) {
+    if (s->mgDirty_ || s->mg_.empty()) s->buildPressureLevels();
+}
+
+int ref_mg_nlevels(void* h) {
+    Solver* s = static_cast<Solver*>(h);
+    ensure_levels(s);
+    return (int)s->mg_.size();
+}
+int ref_mg_level_n(void* h, int l) { return static_cast<Solver*>(h)->mg_[l].n; }
+int ref_mg_level_nnz(void* h, int l) { return (int)static_cast<Solver*>(h)->mg_[l].col.size(); }
+int ref_mg_level_has_agg(void* h, int l) { return static_cast<Solver*>(h)->mg_[l].agg.empty() ? 0 : 1; }
+void ref_mg_level_get(void* h, int l, int* rowStart, int* col, float* val, float* diag, int* agg) {
+    const photon::PressureLevel& L = static_cast<Solver*>(h)->mg_[l];
+    if (rowStart && !L.rowStart.empty()) std::copy(L.rowStart.begin(), L.rowStart.end(), rowStart);
+    if (col) std::copy(L.col.begin(), L.col.end(), col);
+    if (val) std::copy(L.val.begin(), L.val.end(), val);
+    if (diag) std::copy(L.diag.begin(), L.diag.end(), diag);
+    if (agg) {
+        if (L.agg.empty()) std::fill(agg, agg + L.n, -1);
+        else std::copy(L.agg.begin(), L.agg.end(), agg);
+    }
+}
+
+float ref_energy(void* h, float dt) {
+    Solver* s = static_cast<Solver*>(h);
+    s->energyAndExpansion(dt);
+    return s->maxDiv;
+}
+
+void ref_advect(void* h, float dt) { static_cast<Solver*>(h)->advectScalars(dt); }
+
+int ref_threads(void* h) { return static_cast<Solver*>(h)->pool_->threadCount(); }
+
+}  // extern "C"

@@ -1,0 +1,142 @@
+// device.hpp -- storage types, device descriptors and reduction helpers shared
+// by the gfx950 kernels of the pressure projection path.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace phg {
+
+constexpr int TB = 256;          // threads per block (4 wave64)
+constexpr int MAXL = 12;         // multigrid levels (pressure_mg.cpp:141)
+constexpr int NPART = 4;         // FP64 partials per block: dq, rr, max|r|, rz
+
+// ---- storage types: FP32, FP16 and BF16 (BF16 kept as raw bits, RNE) ----
+struct bf16_t {
+    uint16_t u;
+};
+
+__device__ __forceinline__ float to_f(float v) { return v; }
+__device__ __forceinline__ float to_f(_Float16 v) { return (float)v; }
+__device__ __forceinline__ float to_f(bf16_t v) { return __uint_as_float((uint32_t)v.u << 16); }
+
+template <class T> __device__ __forceinline__ T from_f(float v);
+template <> __device__ __forceinline__ float from_f<float>(float v) { return v; }
+template <> __device__ __forceinline__ _Float16 from_f<_Float16>(float v) { return (_Float16)v; }
+template <> __device__ __forceinline__ bf16_t from_f<bf16_t>(float v) {
+    uint32_t b = __float_as_uint(v);
+    b += 0x7FFFu + ((b >> 16) & 1u);   // round to nearest even
+    bf16_t r;
+    r.u = (uint16_t)(b >> 16);
+    return r;
+}
+
+template <class T> __device__ __forceinline__ float ld(const void* p, long i) {
+    return to_f(static_cast<const T*>(p)[i]);
+}
+template <class T> __device__ __forceinline__ void st(void* p, long i, float v) {
+    static_cast<T*>(p)[i] = from_f<T>(v);
+}
+
+// ---- descriptors -----------------------------------------------------------
+// One multigrid level of one system.  Operator arrays belong to the grid and
+// are shared by every system on it; x, b, r, t are the system's own.
+struct DLevel {
+    int n, nx, ny, nz;            // nx.. level 0 only
+    int ellk, pad;
+    const float* ax32;            // level 0 masked face coefficients, FP32 (outer matvec)
+    const float* ay32;
+    const float* az32;
+    const void* axc;              // level 0 coefficients in the V-cycle storage type
+    const void* ayc;
+    const void* azc;
+    const int* col;               // coarse: ELL columns, column-major [ellk][n]
+    const void* val;              // coarse: ELL values (V-cycle coefficient type)
+    const void* diag;             // coarse: diagonal (V-cycle coefficient type)
+    const int* agg;               // unknown -> next-level unknown (-1: none)
+    const int* cptr;              // children of next-level unknowns into this level
+    const int* cidx;
+    void* x;                      // work vectors in the V-cycle vector type
+    void* b;
+    void* r;
+    void* t;
+};
+
+// scalar slots (FP64) and state slots (int) of one system
+enum { S_RZ = 0, S_DQ, S_ALPHA, S_BETA, S_RR, S_BB, S_TOL, S_MAXR, S_INV, S_NSCAL = 16 };
+enum { ST_ACTIVE = 0, ST_ITERS, ST_MAXIT, ST_STATUS, ST_RULE, ST_ZEROX, ST_NSTATE = 8 };
+
+struct DSys {
+    int nl;                       // levels; the coarsest is nl-1
+    int nc;                       // coarsest size
+    int tiles0;                   // blocks of level 0 (stride of the partials)
+    int grid;
+    const double* cinv;           // nc*nc FP64 inverse of the grounded coarsest operator
+    const float* b;               // outer CG vectors, FP32, level-0 cells
+    float* x;
+    float* r;
+    float* d;
+    float* q;
+    float* z;
+    double* scal;
+    int* state;
+    double* part;                 // [tiles0][NPART]
+    DLevel L[MAXL];
+};
+
+// ---- small helpers -------------------------------------------------------
+__device__ __forceinline__ void cell_ijk(int c, int nx, int ny, int& i, int& j, int& k) {
+    const int sxy = nx * ny;
+    k = c / sxy;
+    const int r = c - k * sxy;
+    j = r / nx;
+    i = r - j * nx;
+}
+
+__device__ __forceinline__ double wave_sum(double v) {
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+    return v;
+}
+__device__ __forceinline__ float wave_max(float v) {
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v = fmaxf(v, __shfl_xor(v, o, 64));
+    return v;
+}
+
+// Fixed-tree block sum of NV doubles; thread q < NV writes out[q].
+template <int NV>
+__device__ __forceinline__ void block_sums(double (&v)[NV], double* out) {
+    __shared__ double sh[NV][TB / 64];
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+#pragma unroll
+    for (int q = 0; q < NV; ++q) {
+        v[q] = wave_sum(v[q]);
+        if (lane == 0) sh[q][w] = v[q];
+    }
+    __syncthreads();
+    if (threadIdx.x < NV) {
+        double s = 0;
+#pragma unroll
+        for (int ww = 0; ww < TB / 64; ++ww) s += sh[threadIdx.x][ww];
+        out[threadIdx.x] = s;
+    }
+}
+
+__device__ __forceinline__ float block_max(float v) {
+    __shared__ float sh[TB / 64];
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    v = wave_max(v);
+    if (lane == 0) sh[w] = v;
+    __syncthreads();
+    float m = sh[0];
+#pragma unroll
+    for (int ww = 1; ww < TB / 64; ++ww) m = fmaxf(m, sh[ww]);
+    return m;
+}
+
+// max of non-negative floats through their bit patterns (order independent)
+__device__ __forceinline__ void atomic_max_nonneg(float* addr, float v) {
+    atomicMax(reinterpret_cast<unsigned int*>(addr), __float_as_uint(v));
+}
+
+}  // namespace phg

@@ -1,0 +1,396 @@
+// hierarchy.cpp -- host-side multigrid set-up for the HIP pressure solver.
+// Algorithm: src/sim/pressure_mg.cpp:28-191 (connected aggregation, Galerkin
+// coarse operators); zones: src/sim/gasphase.cpp:111-158.
+#include <algorithm>
+#include <cmath>
+#include <string>
+
+#include "hierarchy.hpp"
+
+namespace phg {
+namespace {
+
+// Disjoint sets whose representative is always the smallest member, so the
+// partition and the representative do not depend on the order of joins.
+struct MinSets {
+    std::vector<int> up;
+    explicit MinSets(int n) : up(n) {
+        for (int i = 0; i < n; ++i) up[i] = i;
+    }
+    int root(int a) {
+        int r = a;
+        while (up[r] != r) r = up[r];
+        while (up[a] != r) {
+            int nxt = up[a];
+            up[a] = r;
+            a = nxt;
+        }
+        return r;
+    }
+    void merge(int a, int b) {
+        a = root(a);
+        b = root(b);
+        if (a == b) return;
+        if (a < b) up[b] = a;
+        else up[a] = b;
+    }
+};
+
+// Row builder keeping couplings in first-insertion order (the order fixes the
+// FP32 summation of repeated couplings, as in the reference).
+struct Rows {
+    std::vector<std::vector<std::pair<int, float>>> r;
+    explicit Rows(int n) : r(n) {}
+    void add(int row, int col, float w) {
+        for (auto& e : r[row])
+            if (e.first == col) {
+                e.second += w;
+                return;
+            }
+        r[row].push_back({col, w});
+    }
+};
+
+void to_csr(HostLevel& C, Rows& rows, const std::vector<double>& dg) {
+    C.diag.resize(C.n);
+    C.rowStart.assign(C.n + 1, 0);
+    for (int I = 0; I < C.n; ++I) {
+        C.diag[I] = (float)std::max(dg[I], 0.0);
+        C.rowStart[I + 1] = C.rowStart[I] + (int)rows.r[I].size();
+    }
+    C.col.clear();
+    C.val.clear();
+    for (int I = 0; I < C.n; ++I)
+        for (auto& e : rows.r[I]) {
+            C.col.push_back(e.first);
+            C.val.push_back(e.second);
+        }
+}
+
+// Partition level F into connected pieces of 2x2x2 groups of its block
+// coordinates; returns the coarse count, fills agg and the coarse block coords.
+int group_pieces(const HostLevel& F, std::vector<int>& agg, HostLevel& C) {
+    MinSets s(F.n);
+    for (int I = 0; I < F.n; ++I)
+        for (int e = F.rowStart[I]; e < F.rowStart[I + 1]; ++e) {
+            int J = F.col[e];
+            if (J > I && (F.bi[I] >> 1) == (F.bi[J] >> 1) && (F.bj[I] >> 1) == (F.bj[J] >> 1) &&
+                (F.bk[I] >> 1) == (F.bk[J] >> 1))
+                s.merge(I, J);
+        }
+    agg.assign(F.n, -1);
+    std::vector<int> id(F.n, -1);
+    C = HostLevel();
+    for (int I = 0; I < F.n; ++I) {
+        int r = s.root(I);
+        if (id[r] < 0) {
+            id[r] = C.n++;
+            C.bi.push_back(F.bi[I] >> 1);
+            C.bj.push_back(F.bj[I] >> 1);
+            C.bk.push_back(F.bk[I] >> 1);
+        }
+        agg[I] = id[r];
+    }
+    return C.n;
+}
+
+void galerkin_sparse(const HostLevel& F, HostLevel& C) {
+    Rows rows(C.n);
+    std::vector<double> dg(C.n, 0.0);
+    for (int I = 0; I < F.n; ++I) dg[F.agg[I]] += F.diag[I];
+    for (int I = 0; I < F.n; ++I)
+        for (int e = F.rowStart[I]; e < F.rowStart[I + 1]; ++e) {
+            int A = F.agg[I], B = F.agg[F.col[e]];
+            if (A == B) dg[A] -= F.val[e];   // internal coupling: seen from both ends
+            else rows.add(A, B, F.val[e]);
+        }
+    to_csr(C, rows, dg);
+    C.leak.assign(C.n, 0);
+    for (int I = 0; I < F.n; ++I)
+        if (F.leak[I]) C.leak[F.agg[I]] = 1;
+}
+
+void make_ell(HostLevel& L) {
+    int k = 0;
+    for (int I = 0; I < L.n; ++I) k = std::max(k, L.rowStart[I + 1] - L.rowStart[I]);
+    L.ellk = k;
+    L.ell_col.assign((size_t)k * L.n, 0);
+    L.ell_val.assign((size_t)k * L.n, 0.0f);
+    for (int I = 0; I < L.n; ++I) {
+        int q = 0;
+        for (int e = L.rowStart[I]; e < L.rowStart[I + 1]; ++e, ++q) {
+            L.ell_col[(size_t)q * L.n + I] = L.col[e];
+            L.ell_val[(size_t)q * L.n + I] = L.val[e];
+        }
+        for (; q < k; ++q) L.ell_col[(size_t)q * L.n + I] = I;
+    }
+}
+
+void make_children(HostLevel& F, int ncoarse) {
+    F.cptr.assign(ncoarse + 1, 0);
+    for (int i = 0; i < F.n; ++i)
+        if (F.agg[i] >= 0) F.cptr[F.agg[i] + 1]++;
+    for (int I = 0; I < ncoarse; ++I) F.cptr[I + 1] += F.cptr[I];
+    F.cidx.assign(F.cptr[ncoarse], 0);
+    std::vector<int> fill(F.cptr.begin(), F.cptr.end() - 1);
+    for (int i = 0; i < F.n; ++i)   // ascending: the gather sums in the reference's serial order
+        if (F.agg[i] >= 0) F.cidx[fill[F.agg[i]]++] = i;
+}
+
+// Dense inverse of the coarsest operator.  A closed component (no member on
+// an open boundary) has the constant vector as null space; adding s*1*1^T on
+// it gives an SPD matrix whose inverse acts as the pseudo-inverse on vectors
+// of zero component sum.  Rows with diag <= 0 are isolated and held at zero.
+void dense_inverse(HostHierarchy& H) {
+    const HostLevel& L = H.L.back();
+    const int n = L.n;
+    H.nc = n;
+    H.cmask.assign(n, 0);
+    std::vector<double> A((size_t)n * n, 0.0);
+    MinSets comp(std::max(n, 1));
+    for (int I = 0; I < n; ++I) {
+        if (L.diag[I] <= 0) {
+            H.cmask[I] = 1;
+            A[(size_t)I * n + I] = 1.0;
+            continue;
+        }
+        A[(size_t)I * n + I] += L.diag[I];
+        for (int e = L.rowStart[I]; e < L.rowStart[I + 1]; ++e) {
+            A[(size_t)I * n + L.col[e]] -= L.val[e];
+            if (L.diag[L.col[e]] > 0) comp.merge(I, L.col[e]);
+        }
+    }
+    std::vector<int> open(n, 0), cnt(n, 0);
+    std::vector<double> dsum(n, 0.0);
+    for (int I = 0; I < n; ++I) {
+        if (H.cmask[I]) continue;
+        int r = comp.root(I);
+        open[r] |= L.leak[I];
+        cnt[r]++;
+        dsum[r] += L.diag[I];
+    }
+    std::vector<std::vector<int>> members(n);
+    for (int I = 0; I < n; ++I)
+        if (!H.cmask[I]) members[comp.root(I)].push_back(I);
+    for (int r = 0; r < n; ++r) {
+        if (members[r].empty() || open[r]) continue;
+        double s = dsum[r] / ((double)cnt[r] * cnt[r]);
+        for (int I : members[r])
+            for (int J : members[r]) A[(size_t)I * n + J] += s;
+    }
+    // Cholesky A = G G^T (lower G stored in A)
+    for (int j = 0; j < n; ++j) {
+        double d = A[(size_t)j * n + j];
+        for (int k = 0; k < j; ++k) d -= A[(size_t)j * n + k] * A[(size_t)j * n + k];
+        d = d > 0 ? std::sqrt(d) : 1e-300;
+        A[(size_t)j * n + j] = d;
+        for (int i = j + 1; i < n; ++i) {
+            double s = A[(size_t)i * n + j];
+            for (int k = 0; k < j; ++k) s -= A[(size_t)i * n + k] * A[(size_t)j * n + k];
+            A[(size_t)i * n + j] = s / d;
+        }
+    }
+    // G^-1 (lower), then A^-1 = G^-T G^-1
+    std::vector<double> Gi((size_t)n * n, 0.0);
+    for (int j = 0; j < n; ++j) {
+        Gi[(size_t)j * n + j] = 1.0 / A[(size_t)j * n + j];
+        for (int i = j + 1; i < n; ++i) {
+            double s = 0;
+            for (int k = j; k < i; ++k) s -= A[(size_t)i * n + k] * Gi[(size_t)k * n + j];
+            Gi[(size_t)i * n + j] = s / A[(size_t)i * n + i];
+        }
+    }
+    H.cinv.assign((size_t)n * n, 0.0);
+    for (int i = 0; i < n; ++i)
+        for (int j = 0; j <= i; ++j) {
+            double s = 0;
+            for (int k = i; k < n; ++k) s += Gi[(size_t)k * n + i] * Gi[(size_t)k * n + j];
+            H.cinv[(size_t)i * n + j] = s;
+            H.cinv[(size_t)j * n + i] = s;
+        }
+    for (int I = 0; I < n; ++I)
+        if (H.cmask[I])
+            for (int J = 0; J < n; ++J) H.cinv[(size_t)I * n + J] = H.cinv[(size_t)J * n + I] = 0.0;
+}
+
+}  // namespace
+
+bool build_hierarchy(int nx, int ny, int nz, const float* ax, const float* ay, const float* az,
+                     const uint8_t* blocked, const uint8_t* wind_closed, int dense_max, HostHierarchy& H,
+                     std::string* err) {
+    if (nx < 2 || ny < 2 || nz < 2) {
+        if (err) *err = "grid must be at least 2 cells in each direction";
+        return false;
+    }
+    H = HostHierarchy();
+    const int n0 = nx * ny * nz, sxy = nx * ny;
+    auto xf = [&](int i, int j, int k) { return i + (nx + 1) * (j + ny * k); };
+    auto yf = [&](int i, int j, int k) { return i + nx * (j + (ny + 1) * k); };
+    auto zf = [&](int i, int j, int k) { return i + nx * (j + ny * k); };
+    H.L.emplace_back();
+    HostLevel& L0 = H.L[0];
+    L0.n = n0;
+    L0.nx = nx;
+    L0.ny = ny;
+    L0.nz = nz;
+    L0.ax.assign(ax, ax + (size_t)(nx + 1) * ny * nz);
+    L0.ay.assign(ay, ay + (size_t)nx * (ny + 1) * nz);
+    L0.az.assign(az, az + (size_t)nx * ny * (nz + 1));
+    bool wc[4] = {false, false, false, false};
+    if (wind_closed)
+        for (int s = 0; s < 4; ++s) wc[s] = wind_closed[s] != 0;
+    for (int k = 0; k < nz; ++k) {
+        for (int j = 0; j < ny; ++j) {
+            if (wc[0]) L0.ax[xf(0, j, k)] = 0;
+            if (wc[1]) L0.ax[xf(nx, j, k)] = 0;
+        }
+        for (int i = 0; i < nx; ++i) {
+            if (wc[2]) L0.ay[yf(i, 0, k)] = 0;
+            if (wc[3]) L0.ay[yf(i, ny, k)] = 0;
+        }
+    }
+    L0.diag.assign(n0, 0.0f);
+    L0.leak.assign(n0, 0);
+    for (int k = 0; k < nz; ++k)
+        for (int j = 0; j < ny; ++j)
+            for (int i = 0; i < nx; ++i) {
+                int c = i + nx * (j + ny * k);
+                if (blocked[c]) continue;
+                float a0 = L0.ax[xf(i, j, k)], a1 = L0.ax[xf(i + 1, j, k)], a2 = L0.ay[yf(i, j, k)],
+                      a3 = L0.ay[yf(i, j + 1, k)], a4 = L0.az[zf(i, j, k)], a5 = L0.az[zf(i, j, k + 1)];
+                L0.diag[c] = a0 + a1 + a2 + a3 + a4 + a5;
+                bool bnd = (i == 0 && a0 > 0) || (i == nx - 1 && a1 > 0) || (j == 0 && a2 > 0) ||
+                           (j == ny - 1 && a3 > 0) || (k == 0 && a4 > 0) || (k == nz - 1 && a5 > 0);
+                L0.leak[c] = (L0.diag[c] > 0 && bnd) ? 1 : 0;
+            }
+    // ---- level 1: connected pieces of 2x2x2 cell blocks
+    {
+        MinSets s(n0);
+        for (int k = 0; k < nz; ++k)
+            for (int j = 0; j < ny; ++j)
+                for (int i = 0; i < nx; ++i) {
+                    int c = i + nx * (j + ny * k);
+                    if (L0.diag[c] <= 0) continue;
+                    if ((i & 1) == 0 && i + 1 < nx && L0.ax[xf(i + 1, j, k)] > 0 && L0.diag[c + 1] > 0) s.merge(c, c + 1);
+                    if ((j & 1) == 0 && j + 1 < ny && L0.ay[yf(i, j + 1, k)] > 0 && L0.diag[c + nx] > 0) s.merge(c, c + nx);
+                    if ((k & 1) == 0 && k + 1 < nz && L0.az[zf(i, j, k + 1)] > 0 && L0.diag[c + sxy] > 0)
+                        s.merge(c, c + sxy);
+                }
+        L0.agg.assign(n0, -1);
+        std::vector<int> id(n0, -1);
+        HostLevel C;
+        for (int c = 0; c < n0; ++c) {
+            if (L0.diag[c] <= 0) continue;
+            int r = s.root(c);
+            if (id[r] < 0) {
+                id[r] = C.n++;
+                int i = c % nx, j = (c / nx) % ny, k = c / sxy;
+                C.bi.push_back(i >> 1);
+                C.bj.push_back(j >> 1);
+                C.bk.push_back(k >> 1);
+            }
+            L0.agg[c] = id[r];
+        }
+        Rows rows(C.n);
+        std::vector<double> dg(C.n, 0.0);
+        for (int k = 0; k < nz; ++k)
+            for (int j = 0; j < ny; ++j)
+                for (int i = 0; i < nx; ++i) {
+                    int c = i + nx * (j + ny * k);
+                    if (L0.diag[c] <= 0) continue;
+                    const int nb[3] = {i + 1 < nx ? c + 1 : -1, j + 1 < ny ? c + nx : -1, k + 1 < nz ? c + sxy : -1};
+                    const float w[3] = {i + 1 < nx ? L0.ax[xf(i + 1, j, k)] : 0.0f,
+                                        j + 1 < ny ? L0.ay[yf(i, j + 1, k)] : 0.0f,
+                                        k + 1 < nz ? L0.az[zf(i, j, k + 1)] : 0.0f};
+                    for (int d = 0; d < 3; ++d) {
+                        if (nb[d] < 0 || w[d] <= 0) continue;
+                        int A = L0.agg[c], B = L0.agg[nb[d]];
+                        if (A < 0 || B < 0 || A == B) continue;
+                        rows.add(A, B, w[d]);
+                        rows.add(B, A, w[d]);
+                    }
+                }
+        for (int c = 0; c < n0; ++c)
+            if (L0.agg[c] >= 0) dg[L0.agg[c]] += L0.diag[c];
+        for (int k = 0; k < nz; ++k)
+            for (int j = 0; j < ny; ++j)
+                for (int i = 0; i < nx; ++i) {
+                    int c = i + nx * (j + ny * k);
+                    if (L0.diag[c] <= 0) continue;
+                    const int nb[3] = {i + 1 < nx ? c + 1 : -1, j + 1 < ny ? c + nx : -1, k + 1 < nz ? c + sxy : -1};
+                    const float w[3] = {i + 1 < nx ? L0.ax[xf(i + 1, j, k)] : 0.0f,
+                                        j + 1 < ny ? L0.ay[yf(i, j + 1, k)] : 0.0f,
+                                        k + 1 < nz ? L0.az[zf(i, j, k + 1)] : 0.0f};
+                    for (int d = 0; d < 3; ++d)
+                        if (nb[d] >= 0 && L0.diag[nb[d]] > 0 && w[d] > 0 && L0.agg[nb[d]] == L0.agg[c])
+                            dg[L0.agg[c]] -= 2.0 * w[d];   // internal face counted from both cells
+                }
+        to_csr(C, rows, dg);
+        C.leak.assign(C.n, 0);
+        for (int c = 0; c < n0; ++c)
+            if (L0.agg[c] >= 0 && L0.leak[c]) C.leak[L0.agg[c]] = 1;
+        H.L.push_back(std::move(C));
+    }
+    // ---- coarser levels (reference rule), then optional extension for the dense solve
+    bool reference_phase = true;
+    while (true) {
+        const int n = H.L.back().n;
+        if ((int)H.L.size() >= 12) break;
+        if (reference_phase && !(n > 2000)) {
+            reference_phase = false;
+            H.ref_levels = (int)H.L.size();
+        }
+        if (!reference_phase && !(n > dense_max)) break;
+        HostLevel C;
+        std::vector<int> agg;
+        int nc = group_pieces(H.L.back(), agg, C);
+        if (nc > 0.8 * n) break;
+        H.L.back().agg = std::move(agg);
+        galerkin_sparse(H.L.back(), C);
+        H.L.push_back(std::move(C));
+    }
+    if (H.ref_levels == 0) H.ref_levels = (int)H.L.size();
+    for (size_t l = 1; l < H.L.size(); ++l) make_ell(H.L[l]);
+    for (size_t l = 0; l + 1 < H.L.size(); ++l) make_children(H.L[l], H.L[l + 1].n);
+    dense_inverse(H);
+    return true;
+}
+
+int compute_zones(int nx, int ny, int nz, const float* ax, const float* ay, const float* az, const uint8_t* blocked,
+                  std::vector<int>& zone, std::vector<uint8_t>& zone_open) {
+    const int N = nx * ny * nz, sxy = nx * ny;
+    zone.assign(N, -1);
+    zone_open.clear();
+    std::vector<int> stack;
+    for (int seed = 0; seed < N; ++seed) {
+        if (blocked[seed] || zone[seed] >= 0) continue;
+        const int z = (int)zone_open.size();
+        uint8_t open = 0;
+        zone[seed] = z;
+        stack.assign(1, seed);
+        while (!stack.empty()) {
+            int c = stack.back();
+            stack.pop_back();
+            int i = c % nx, j = (c / nx) % ny, k = c / sxy;
+            const float a[6] = {ax[i + (nx + 1) * (j + ny * k)], ax[i + 1 + (nx + 1) * (j + ny * k)],
+                                ay[i + nx * (j + (ny + 1) * k)], ay[i + nx * (j + 1 + (ny + 1) * k)],
+                                az[i + nx * (j + ny * k)], az[i + nx * (j + ny * (k + 1))]};
+            const int nb[6] = {i > 0 ? c - 1 : -1, i < nx - 1 ? c + 1 : -1, j > 0 ? c - nx : -1,
+                               j < ny - 1 ? c + nx : -1, k > 0 ? c - sxy : -1, k < nz - 1 ? c + sxy : -1};
+            for (int d = 0; d < 6; ++d) {
+                if (a[d] <= 0) continue;
+                if (nb[d] < 0) {
+                    if (d != 4) open = 1;   // the ground never opens a zone
+                    continue;
+                }
+                if (blocked[nb[d]] || zone[nb[d]] >= 0) continue;
+                zone[nb[d]] = z;
+                stack.push_back(nb[d]);
+            }
+        }
+        zone_open.push_back(open);
+    }
+    return (int)zone_open.size();
+}
+
+}  // namespace phg

@@ -1,0 +1,58 @@
+// hierarchy.hpp -- host-side set-up of the pressure multigrid for one grid.
+//
+// Connected-aggregation hierarchy with the reference's rules
+// (src/sim/pressure_mg.cpp:28-191): level 0 is the structured 7-point grid
+// with boundary-masked face coefficients; each coarse unknown is one
+// connected piece of a 2x2x2 block of finer unknowns; Galerkin operator with
+// piecewise-constant interpolation; coarsen while n > 2000 (and < 12 levels),
+// stop when a level no longer shrinks below 0.8x.  For the GPU's dense
+// coarsest solve the hierarchy may continue past 2000 unknowns with the same
+// aggregation rule until the coarsest holds at most `dense_max` unknowns.
+//
+// The set-up runs once per geometry change (the reference's mgDirty_), on the
+// host; what the GPU needs is laid out here: ELL (column-major) coarse
+// operators, child lists for deterministic gather restriction, and the dense
+// FP64 inverse of the grounded coarsest operator.
+#pragma once
+#include <cstdint>
+#include <string>
+#include <vector>
+
+namespace phg {
+
+struct HostLevel {
+    int n = 0, nx = 0, ny = 0, nz = 0;
+    std::vector<float> ax, ay, az;          // level 0: masked face coefficients
+    std::vector<int> rowStart, col;         // coarse: off-diagonal couplings (positive values)
+    std::vector<float> val;
+    std::vector<int> bi, bj, bk;            // coarse: block coordinates
+    std::vector<int> agg;                   // unknown -> next level unknown (empty on the coarsest)
+    std::vector<float> diag;
+    std::vector<uint8_t> leak;              // touches an open (Dirichlet) boundary face
+    // derived device layouts
+    int ellk = 0;                           // max couplings per row
+    std::vector<int> ell_col;               // [ellk][n], padded with the row itself
+    std::vector<float> ell_val;             // [ellk][n], padded with 0
+    std::vector<int> cptr, cidx;            // children of each next-level unknown, ascending
+};
+
+struct HostHierarchy {
+    std::vector<HostLevel> L;
+    int ref_levels = 0;                     // levels produced by the reference's stopping rule
+    int nc = 0;                             // coarsest size
+    std::vector<double> cinv;               // nc*nc inverse of the grounded coarsest operator
+    std::vector<uint8_t> cmask;             // coarsest rows held at zero (diag <= 0)
+};
+
+// wind_closed: 4 flags (x-low, x-high, y-low, y-high) or nullptr.
+// Returns false with a message on inconsistent input.
+bool build_hierarchy(int nx, int ny, int nz, const float* ax, const float* ay, const float* az,
+                     const uint8_t* blocked, const uint8_t* wind_closed, int dense_max, HostHierarchy& H,
+                     std::string* err);
+
+// Pressure zones: gas cells joined through faces with a > 0; open when a
+// side or top boundary face with a > 0 is reached (src/sim/gasphase.cpp:111-158).
+int compute_zones(int nx, int ny, int nz, const float* ax, const float* ay, const float* az, const uint8_t* blocked,
+                  std::vector<int>& zone, std::vector<uint8_t>& zone_open);
+
+}  // namespace phg

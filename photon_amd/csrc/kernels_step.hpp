@@ -1,0 +1,283 @@
+// kernels_step.hpp -- the kernels around the pressure solve: RHS assembly,
+// closed-zone mean removal, velocity correction (Solver::project,
+// src/sim/gasphase.cpp:505-677), the divergence source
+// (energyAndExpansion, :251-271) and scalar transport (advectScalars,
+// :685-784).  One thread per cell or face; all of them are HBM-bound
+// elementwise / 7-point (13-point for transport) passes.
+#pragma once
+#include "device.hpp"
+
+namespace phg {
+
+struct DGrid {
+    int nx, ny, nz, n;
+    const float* ax;      // masked matrix coefficients (the reference's coefs(), gasphase.cpp:518-527)
+    const float* ay;
+    const float* az;
+    const float* fx;      // face open fractions
+    const float* fy;
+    const float* fz;
+    const uint8_t* blocked;
+};
+
+__device__ __forceinline__ float h_out(float vel, bool inflow) { return inflow ? -0.5f * vel * vel : 0.0f; }
+
+// b = (S - div(f u*)) dx^2/dt plus the known outside H on open boundary
+// faces, diag = sum of the masked coefficients, p = 0 in solids (gasphase.cpp:528-549)
+__global__ void __launch_bounds__(TB) k_rhs(DGrid g, float dx, float inv, const float* __restrict__ u,
+                                            const float* __restrict__ v, const float* __restrict__ w,
+                                            const float* __restrict__ S, float* __restrict__ b,
+                                            float* __restrict__ diag, float* __restrict__ p) {
+    const int c = blockIdx.x * TB + threadIdx.x;
+    if (c >= g.n) return;
+    const int nx = g.nx, ny = g.ny, nz = g.nz;
+    int i, j, k;
+    cell_ijk(c, nx, ny, i, j, k);
+    if (g.blocked[c]) {
+        b[c] = 0.0f;
+        if (diag) diag[c] = 0.0f;
+        p[c] = 0.0f;
+        return;
+    }
+    const int a0 = c + j + ny * k, a1 = a0 + 1, b0 = c + nx * k, b1 = b0 + nx, c0 = c, c1 = c + nx * ny;
+    const float div = (g.fx[a1] * u[a1] - g.fx[a0] * u[a0] + g.fy[b1] * v[b1] - g.fy[b0] * v[b0] +
+                       g.fz[c1] * w[c1] - g.fz[c0] * w[c0]) / dx;
+    float bb = (S[c] - div) * inv;
+    const float co[6] = {g.ax[a0], g.ax[a1], g.ay[b0], g.ay[b1], g.az[c0], g.az[c1]};
+    if (i == 0 && co[0] > 0) bb += co[0] * h_out(u[a0], u[a0] > 0);
+    if (i == nx - 1 && co[1] > 0) bb += co[1] * h_out(u[a1], u[a1] < 0);
+    if (j == 0 && co[2] > 0) bb += co[2] * h_out(v[b0], v[b0] > 0);
+    if (j == ny - 1 && co[3] > 0) bb += co[3] * h_out(v[b1], v[b1] < 0);
+    if (k == nz - 1 && co[5] > 0) bb += co[5] * h_out(w[c1], w[c1] < 0);
+    b[c] = bb;
+    if (diag) diag[c] = co[0] + co[1] + co[2] + co[3] + co[4] + co[5];
+}
+
+// closed-zone means of p (FP64, fixed order): one block per closed zone
+__global__ void __launch_bounds__(TB) k_zone_sum(const int* __restrict__ zptr, const int* __restrict__ zcells,
+                                                 const float* __restrict__ p, double* __restrict__ mean) {
+    const int z = blockIdx.x;
+    const int b = zptr[z], e = zptr[z + 1];
+    double v[1] = {0.0};
+    for (int q = b + threadIdx.x; q < e; q += TB) v[0] += (double)p[zcells[q]];
+    __shared__ double out[1];
+    block_sums<1>(v, out);
+    __syncthreads();
+    if (threadIdx.x == 0) mean[z] = e > b ? out[0] / (double)(e - b) : 0.0;
+}
+
+__global__ void __launch_bounds__(TB) k_zone_sub(int n, const int* __restrict__ zclosed, const double* __restrict__ mean,
+                                                 float* __restrict__ p) {
+    const int c = blockIdx.x * TB + threadIdx.x;
+    if (c >= n) return;
+    const int z = zclosed[c];
+    if (z >= 0) p[c] -= (float)mean[z];
+}
+
+__global__ void k_fill_f32(float* p, int n, float v) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = v;
+}
+
+// u -= dt/dx grad H on matrix faces, clamp +-40 m/s, max |u| f
+// (gasphase.cpp:640-676).  Faces are numbered x faces, then y faces, then the
+// z faces above each cell (k = 1 .. nz; the ground face is not touched).
+__global__ void __launch_bounds__(TB) k_correct(DGrid g, float sc, const float* __restrict__ p, float* __restrict__ u,
+                                                float* __restrict__ v, float* __restrict__ w, float* __restrict__ vmax) {
+    const int nx = g.nx, ny = g.ny, nz = g.nz;
+    const int nfx = (nx + 1) * ny * nz, nfy = nx * (ny + 1) * nz, nfz = nx * ny * nz;
+    const int t = blockIdx.x * TB + threadIdx.x;
+    const float VMAX = 40.0f;
+    float vm = 0.0f;
+    if (t < nfx) {
+        const int f = t, i = f % (nx + 1), jk = f / (nx + 1), j = jk % ny, k = jk / ny;
+        if (g.fx[f] <= 0) {
+            u[f] = 0.0f;
+        } else {
+            float uf = u[f];
+            if (g.ax[f] > 0) {
+                const int c = i + nx * (j + ny * k);
+                const float hr = i < nx ? p[c] : h_out(uf, uf < 0);
+                const float hl = i > 0 ? p[c - 1] : h_out(uf, uf > 0);
+                uf = fminf(fmaxf(uf - sc * (hr - hl), -VMAX), VMAX);
+                u[f] = uf;
+            }
+            vm = fabsf(uf) * g.fx[f];
+        }
+    } else if (t < nfx + nfy) {
+        const int f = t - nfx, i = f % nx, jk = f / nx, j = jk % (ny + 1), k = jk / (ny + 1);
+        if (g.fy[f] <= 0) {
+            v[f] = 0.0f;
+        } else {
+            float vf = v[f];
+            if (g.ay[f] > 0) {
+                const int c = i + nx * (j + ny * k);
+                const float hr = j < ny ? p[c] : h_out(vf, vf < 0);
+                const float hl = j > 0 ? p[c - nx] : h_out(vf, vf > 0);
+                vf = fminf(fmaxf(vf - sc * (hr - hl), -VMAX), VMAX);
+                v[f] = vf;
+            }
+            vm = fabsf(vf) * g.fy[f];
+        }
+    } else if (t < nfx + nfy + nfz) {
+        const int c = t - nfx - nfy;          // the cell below the face
+        const int f = c + nx * ny;            // zface(i, j, k + 1)
+        const int k = c / (nx * ny);
+        if (g.fz[f] <= 0) {
+            w[f] = 0.0f;
+        } else {
+            float wf = w[f];
+            if (g.az[f] > 0) {
+                const float hu = k + 1 < nz ? p[f] : h_out(wf, wf < 0);
+                wf = fminf(fmaxf(wf - sc * (hu - p[c]), -VMAX), VMAX);
+                w[f] = wf;
+            }
+            vm = fabsf(wf) * g.fz[f];
+        }
+    }
+    vm = block_max(vm);
+    if (threadIdx.x == 0) atomic_max_nonneg(vmax, vm);
+}
+
+// T from the heat budget, divergence source S (energyAndExpansion, gasphase.cpp:251-271)
+__global__ void __launch_bounds__(TB) k_energy(int n, float dx, float dt, const uint8_t* __restrict__ blocked,
+                                               const float* __restrict__ Pb, const float* __restrict__ Q,
+                                               const float* __restrict__ massSrc, float* __restrict__ T,
+                                               float* __restrict__ S, float* __restrict__ maxdiv) {
+    const float RS = 287.0f, CP = 1100.0f, T_MIN = 200.0f, T_MAX = 2300.0f, D_MAX = 50.0f;
+    const int c = blockIdx.x * TB + threadIdx.x;
+    float md = 0.0f;
+    if (c < n) {
+        if (blocked[c]) {
+            S[c] = 0.0f;
+        } else {
+            const float V = dx * dx * dx;
+            const float t = T[c], pb = Pb[c], q = Q[c];
+            const float rho = pb / (RS * t);
+            const float m = rho * V;
+            const float tn = fminf(fmaxf(t + q / (m * CP), T_MIN), T_MAX);
+            const float rhon = pb / (RS * tn);
+            const float D = q / (m * CP * t * dt) + massSrc[c] / (rhon * V * dt);
+            T[c] = tn;
+            const float sv = fminf(fmaxf(D, -D_MAX), D_MAX);
+            S[c] = sv;
+            md = fabsf(sv);
+        }
+    }
+    md = block_max(md);
+    if (threadIdx.x == 0) atomic_max_nonneg(maxdiv, md);
+}
+
+struct Fields10 {
+    const float* in[10];
+    float* out[10];
+};
+
+__device__ __forceinline__ float superbee(float r) {   // gasphase.cpp:35
+    return fmaxf(fmaxf(0.0f, fminf(2.0f * r, 1.0f)), fminf(r, 2.0f));
+}
+
+// Superbee-limited, mass-weighted transport of the 10 scalars with species
+// diffusion and a neighbour-bounded clamp (advectScalars, gasphase.cpp:685-784)
+__global__ void __launch_bounds__(TB) k_advect(DGrid g, float dx, float dt, const float* __restrict__ u,
+                                               const float* __restrict__ v, const float* __restrict__ w,
+                                               const float* __restrict__ Pb, const float* __restrict__ nu,
+                                               Fields10 F) {
+    const float T0 = 293.15f, P0 = 101325.0f, RS = 287.0f, T_MIN = 200.0f, SC_T = 0.5f;
+    const float amb[10] = {T0, 0.232f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0006f, 0.0073f, 0.0f, 0.0f};
+    const int c = blockIdx.x * TB + threadIdx.x;
+    if (c >= g.n) return;
+    const int nx = g.nx, ny = g.ny, nz = g.nz, sxy = nx * ny;
+    if (g.blocked[c]) {
+#pragma unroll
+        for (int q = 0; q < 10; ++q) F.out[q][c] = F.in[q][c];
+        return;
+    }
+    int i, j, k;
+    cell_ijk(c, nx, ny, i, j, k);
+    const float s = dt / dx;
+    const int fa0 = c + j + ny * k, fa1 = fa0 + 1, fb0 = c + nx * k, fb1 = fb0 + nx, fc0 = c, fc1 = c + sxy;
+    const float fo[6] = {g.fx[fa0], g.fx[fa1], g.fy[fb0], g.fy[fb1], g.fz[fc0], g.fz[fc1]};
+    const float un[6] = {-fo[0] * u[fa0], fo[1] * u[fa1], -fo[2] * v[fb0], fo[3] * v[fb1], -fo[4] * w[fc0],
+                         fo[5] * w[fc1]};
+    const int id[3] = {i, j, k}, dim[3] = {nx, ny, nz}, st3[3] = {1, nx, sxy};
+    int nb[6], nb2[6], opp[6];
+#pragma unroll
+    for (int d = 0; d < 6; ++d) {
+        const int a = d >> 1, sg = (d & 1) ? 1 : -1;
+        const int n1 = id[a] + sg, n2 = id[a] + 2 * sg, o1 = id[a] - sg;
+        nb[d] = (n1 >= 0 && n1 < dim[a]) ? c + sg * st3[a] : -1;
+        nb2[d] = (n2 >= 0 && n2 < dim[a]) ? c + 2 * sg * st3[a] : -1;
+        opp[d] = (o1 >= 0 && o1 < dim[a]) ? c - sg * st3[a] : -1;
+        if (nb[d] >= 0 && g.blocked[nb[d]]) nb[d] = -2;
+        if (nb2[d] >= 0 && g.blocked[nb2[d]]) nb2[d] = -2;
+        if (opp[d] >= 0 && g.blocked[opp[d]]) opp[d] = -2;
+    }
+    const float* T = F.in[0];
+    const float nuc = nu[c];
+    const float tc = T[c];
+    const float rc = Pb[c] / (RS * tc);
+    float rUp[6], mden = rc;
+#pragma unroll
+    for (int d = 0; d < 6; ++d) {
+        rUp[d] = 0.0f;
+        if (fo[d] <= 0 || nb[d] == -2) continue;
+        const float tn = nb[d] >= 0 ? T[nb[d]] : T0;
+        float tf;
+        if (un[d] > 0) {
+            const float tu2 = opp[d] >= 0 ? T[opp[d]] : (opp[d] == -1 ? T0 : tc);
+            const float den = tn - tc;
+            tf = tc + (fabsf(den) > 1e-6f ? 0.5f * superbee((tc - tu2) / den) * den : 0.0f);
+        } else {
+            const float tu2 = nb2[d] >= 0 ? T[nb2[d]] : (nb2[d] == -1 ? T0 : tn);
+            const float den = tc - tn;
+            tf = tn + (fabsf(den) > 1e-6f ? 0.5f * superbee((tn - tu2) / den) * den : 0.0f);
+        }
+        const float pu = un[d] > 0 ? Pb[c] : (nb[d] >= 0 ? Pb[nb[d]] : P0);
+        rUp[d] = pu / (RS * fmaxf(tf, T_MIN));
+        mden -= s * rUp[d] * un[d];
+    }
+    const bool conservative = mden > 0.25f * rc;
+    float Dd[6], rf[6];
+#pragma unroll
+    for (int d = 0; d < 6; ++d) {
+        const bool has = nb[d] >= 0;
+        const float nun = has ? nu[nb[d]] : nuc;
+        Dd[d] = 0.5f * (nuc + nun) / SC_T + 2.0e-5f;
+        rf[d] = has ? 0.5f * (rc + Pb[nb[d]] / (RS * T[nb[d]])) : rc;
+    }
+    for (int q = 0; q < 10; ++q) {
+        const float* Fq = F.in[q];
+        const float fc = Fq[c];
+        float lo = fc, hi = fc, acc = 0.0f, macc = 0.0f;
+#pragma unroll
+        for (int d = 0; d < 6; ++d) {
+            if (fo[d] <= 0 || nb[d] == -2) continue;
+            const float fn = nb[d] >= 0 ? Fq[nb[d]] : amb[q];
+            lo = fminf(lo, fn);
+            hi = fmaxf(hi, fn);
+            const float flux = un[d];
+            float face = fc;
+            if (flux > 0) {
+                const float fu2 = opp[d] >= 0 ? Fq[opp[d]] : (opp[d] == -1 ? amb[q] : fc);
+                const float den = fn - fc;
+                if (fabsf(den) > 1e-12f) face += 0.5f * superbee((fc - fu2) / den) * den;
+            } else if (flux < 0) {
+                const float fu2 = nb2[d] >= 0 ? Fq[nb2[d]] : (nb2[d] == -1 ? amb[q] : fn);
+                const float den = fc - fn;
+                face = fn;
+                if (fabsf(den) > 1e-12f) face += 0.5f * superbee((fn - fu2) / den) * den;
+            }
+            acc -= flux * (face - fc);
+            macc -= rUp[d] * flux * face;
+            if (q > 0) {
+                acc += fo[d] * Dd[d] * (fn - fc) / dx;
+                macc += fo[d] * rf[d] * Dd[d] * (fn - fc) / dx;
+            }
+        }
+        const float val = conservative ? (rc * fc + s * macc) / mden : fc + s * acc;
+        F.out[q][c] = fminf(fmaxf(val, lo), hi);
+    }
+}
+
+}  // namespace phg

@@ -1,0 +1,990 @@
+// phg.hip -- host side of the MI355X gas-phase pressure projection library:
+// contexts, grids (device copies of the operator and multigrid hierarchy),
+// batches of systems, the batched MG-PCG driver and the C ABI declared in
+// include/photon_gas.h.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <map>
+#include <string>
+#include <vector>
+
+#include "../../include/photon_gas.h"
+#include "device.hpp"
+#include "hierarchy.hpp"
+#include "kernels_pcg.hpp"
+#include "kernels_step.hpp"
+
+namespace phg {
+
+// ---------------------------------------------------------------------------
+// profiling: HIP events around each launch, accumulated per kernel class
+// ---------------------------------------------------------------------------
+enum KClass {
+    K_RESID_INIT, K_APPLY, K_AXPY, K_DOT_RZ, K_UPDATE_D, K_FIN, K_VC_SMOOTH0, K_VC_RB, K_VC_RESID0, K_VC_RESTRICT,
+    K_VC_JAC, K_VC_RESID, K_VC_PROLONG, K_VC_COARSE, K_RHS, K_ZONE, K_CORRECT, K_ENERGY, K_ADVECT, K_COUNT, K_NCLASS
+};
+static const char* kKName[K_NCLASS] = {"resid_init", "apply",      "axpy",       "dot_rz",     "update_d",
+                                       "finalize",   "vc_smooth0", "vc_rb",      "vc_resid0",  "vc_restrict",
+                                       "vc_jacobi",  "vc_resid",   "vc_prolong", "vc_coarse",  "rhs",
+                                       "zone_mean",  "correct",    "energy",     "advect",     "count_active"};
+
+struct GridDev {
+    int nx = 0, ny = 0, nz = 0, n = 0;
+    float dx = 0;
+    int prec = PHG_PREC_FP32;
+    HostHierarchy H;
+    std::vector<int> zone;
+    std::vector<uint8_t> zone_open;
+    float *ax = nullptr, *ay = nullptr, *az = nullptr, *fx = nullptr, *fy = nullptr, *fz = nullptr;
+    uint8_t* blocked = nullptr;
+    void *axc = nullptr, *ayc = nullptr, *azc = nullptr;   // V-cycle coefficient copies
+    struct Lev {
+        int n = 0, ellk = 0;
+        int* col = nullptr;
+        void* val = nullptr;
+        void* diag = nullptr;
+        int* agg = nullptr;
+        int* cptr = nullptr;
+        int* cidx = nullptr;
+    };
+    std::vector<Lev> lv;
+    double* cinv = nullptr;
+    int nclosed = 0;
+    int *zptr = nullptr, *zcells = nullptr, *zclosed = nullptr;
+    double* zmean = nullptr;
+    float* scratch_b = nullptr;   // RHS of phg_project
+    float* vmax = nullptr;        // one float for max reductions
+    std::vector<void*> allocs;
+    int single_batch = -1;
+    DGrid dg() const {
+        DGrid g;
+        g.nx = nx;
+        g.ny = ny;
+        g.nz = nz;
+        g.n = n;
+        g.ax = ax;
+        g.ay = ay;
+        g.az = az;
+        g.fx = fx;
+        g.fy = fy;
+        g.fz = fz;
+        g.blocked = blocked;
+        return g;
+    }
+};
+
+struct BatchDev {
+    int nsys = 0, prec = 0, maxl = 0, maxnc = 0;
+    std::vector<int> grids;
+    std::vector<DSys> h_sys;
+    DSys* d_sys = nullptr;
+    std::vector<int2*> tab;        // per level: (system, tile) of systems having that level
+    std::vector<int> tab_len;
+    int2* tab_sys = nullptr;       // one entry per system
+    int4* tab_coarse = nullptr;    // (first system, count, row tile): coarsest-solve blocks
+    int tab_coarse_len = 0;
+    int cg_sys = 1;                // systems per coarsest-solve block (64 KB of LDS)
+    double* scal = nullptr;
+    int* state = nullptr;
+    int* d_count = nullptr;
+    int* h_count = nullptr;        // pinned
+    std::vector<void*> allocs;
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t graph_exec = nullptr;
+};
+
+}  // namespace phg
+
+using namespace phg;
+
+struct phg_ctx {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    std::string err;
+    int next_id = 1;
+    std::map<int, GridDev*> grids;
+    std::map<int, BatchDev*> batches;
+    bool prof = false;
+    struct Pending {
+        int cls;
+        hipEvent_t a, b;
+    };
+    std::vector<Pending> pending;
+    std::vector<hipEvent_t> pool;
+    double prof_ms[K_NCLASS] = {};
+    long long prof_n[K_NCLASS] = {};
+};
+
+namespace {
+
+int fail(phg_ctx* ctx, int code, const char* fmt, ...) {
+    char buf[512];
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(buf, sizeof buf, fmt, ap);
+    va_end(ap);
+    if (ctx) ctx->err = buf;
+    return code;
+}
+
+#define HIP_OK(call)                                                                                    \
+    do {                                                                                                \
+        hipError_t e_ = (call);                                                                         \
+        if (e_ != hipSuccess) return fail(ctx, PHG_EHIP, "%s: %s (%s:%d)", #call, hipGetErrorString(e_), \
+                                          __FILE__, __LINE__);                                          \
+    } while (0)
+
+hipEvent_t take_event(phg_ctx* ctx) {
+    if (!ctx->pool.empty()) {
+        hipEvent_t e = ctx->pool.back();
+        ctx->pool.pop_back();
+        return e;
+    }
+    hipEvent_t e = nullptr;
+    (void)hipEventCreate(&e);
+    return e;
+}
+
+void prof_drain(phg_ctx* ctx) {
+    for (auto& p : ctx->pending) {
+        float ms = 0;
+        if (hipEventElapsedTime(&ms, p.a, p.b) == hipSuccess) {
+            ctx->prof_ms[p.cls] += ms;
+            ctx->prof_n[p.cls] += 1;
+        }
+        ctx->pool.push_back(p.a);
+        ctx->pool.push_back(p.b);
+    }
+    ctx->pending.clear();
+}
+
+template <typename... KArgs, typename... Args>
+void launch(phg_ctx* ctx, int cls, void (*kern)(KArgs...), int blocks, size_t smem, Args... args) {
+    if (blocks <= 0) return;
+    hipEvent_t a = nullptr, b = nullptr;
+    if (ctx->prof) {
+        a = take_event(ctx);
+        b = take_event(ctx);
+        (void)hipEventRecord(a, ctx->stream);
+    }
+    hipLaunchKernelGGL(kern, dim3(blocks), dim3(TB), smem, ctx->stream, args...);
+    if (ctx->prof) {
+        (void)hipEventRecord(b, ctx->stream);
+        ctx->pending.push_back({cls, a, b});
+    }
+}
+
+template <class T>
+int dalloc(phg_ctx* ctx, std::vector<void*>& owner, T** p, size_t count) {
+    *p = nullptr;
+    if (count == 0) count = 1;
+    void* q = nullptr;
+    hipError_t e = hipMalloc(&q, count * sizeof(T));
+    if (e != hipSuccess) return fail(ctx, PHG_ENOMEM, "hipMalloc(%zu bytes): %s", count * sizeof(T), hipGetErrorString(e));
+    owner.push_back(q);
+    *p = static_cast<T*>(q);
+    return 0;
+}
+
+template <class T>
+int upload(phg_ctx* ctx, std::vector<void*>& owner, T** p, const T* src, size_t count) {
+    int rc = dalloc(ctx, owner, p, count);
+    if (rc) return rc;
+    if (count) HIP_OK(hipMemcpy(*p, src, count * sizeof(T), hipMemcpyHostToDevice));
+    return 0;
+}
+
+uint16_t f32_to_f16_bits(float f) {
+    _Float16 h = (_Float16)f;
+    uint16_t u;
+    std::memcpy(&u, &h, 2);
+    return u;
+}
+uint16_t f32_to_bf16_bits(float f) {
+    uint32_t b;
+    std::memcpy(&b, &f, 4);
+    b += 0x7FFFu + ((b >> 16) & 1u);
+    return (uint16_t)(b >> 16);
+}
+
+// upload float data in the V-cycle coefficient type of the precision
+int upload_ct(phg_ctx* ctx, std::vector<void*>& owner, void** p, const std::vector<float>& v, int prec) {
+    if (prec == PHG_PREC_FP32) {
+        float* q;
+        int rc = upload(ctx, owner, &q, v.data(), v.size());
+        *p = q;
+        return rc;
+    }
+    std::vector<uint16_t> h(v.size());
+    for (size_t i = 0; i < v.size(); ++i) h[i] = f32_to_f16_bits(v[i]);
+    uint16_t* q;
+    int rc = upload(ctx, owner, &q, h.data(), h.size());
+    *p = q;
+    return rc;
+}
+
+size_t vt_size(int prec) { return prec == PHG_PREC_FP32 ? 4 : 2; }
+
+void free_all(std::vector<void*>& v) {
+    for (void* p : v) (void)hipFree(p);
+    v.clear();
+}
+
+// tile table of one level, XCD-ordered: block b runs on XCD b % 8, and gets
+// the (b / 8)-th entry of that XCD's contiguous share of the logical list
+std::vector<int2> xcd_order(const std::vector<int2>& logical) {
+    const int M = (int)logical.size();
+    std::vector<int2> phys(M);
+    const int q = M / 8, r = M % 8;
+    for (int b = 0; b < M; ++b) {
+        const int x = b % 8, idx = b / 8;
+        const int start = x * q + std::min(x, r);
+        phys[b] = logical[start + idx];
+    }
+    return phys;
+}
+
+int tiles(int n) { return (n + TB - 1) / TB; }
+
+// largest coarsest level solved densely: above the reference's 2000-unknown
+// stopping size, so the hierarchy keeps the reference's levels whenever it
+// coarsens at all; the LDS of the coarsest-solve block holds CG_SYS vectors
+constexpr int kDenseMax = 2048;
+
+// ---------------------------------------------------------------------------
+// the V-cycle and one CG iteration as launch sequences
+// ---------------------------------------------------------------------------
+template <class CT, class VT>
+void vcycle_t(phg_ctx* ctx, BatchDev* B) {
+    const DSys* S = B->d_sys;
+    launch(ctx, K_VC_SMOOTH0, k_vc_smooth0<CT>, B->tab_len[0], 0, S, B->tab[0]);
+    launch(ctx, K_VC_RB, k_vc_rb<CT>, B->tab_len[0], 0, S, B->tab[0], 1);
+    launch(ctx, K_VC_RESID0, k_vc_resid0<CT, VT>, B->tab_len[0], 0, S, B->tab[0]);
+    launch(ctx, K_VC_RESTRICT, k_vc_restrict<VT>, B->tab_len[1], 0, S, B->tab[1], 0);
+    for (int l = 1; l + 1 < B->maxl; ++l) {
+        launch(ctx, K_VC_JAC, k_vc_jac0<CT, VT>, B->tab_len[l], 0, S, B->tab[l], l);
+        launch(ctx, K_VC_JAC, k_vc_jac<CT, VT>, B->tab_len[l], 0, S, B->tab[l], l, 1);
+        launch(ctx, K_VC_RESID, k_vc_resid<CT, VT>, B->tab_len[l], 0, S, B->tab[l], l);
+        launch(ctx, K_VC_RESTRICT, k_vc_restrict<VT>, B->tab_len[l + 1], 0, S, B->tab[l + 1], l);
+    }
+    launch(ctx, K_VC_COARSE, k_vc_coarse<VT>, B->tab_coarse_len, (size_t)B->cg_sys * B->maxnc * sizeof(double), S,
+           (const int4*)B->tab_coarse);
+    for (int l = B->maxl - 2; l >= 1; --l) {
+        launch(ctx, K_VC_PROLONG, k_vc_prolong<VT>, B->tab_len[l], 0, S, B->tab[l], l);
+        launch(ctx, K_VC_JAC, k_vc_jac<CT, VT>, B->tab_len[l], 0, S, B->tab[l], l, 0);
+        launch(ctx, K_VC_JAC, k_vc_jac<CT, VT>, B->tab_len[l], 0, S, B->tab[l], l, 1);
+    }
+    launch(ctx, K_VC_PROLONG, k_vc_prolong0<VT>, B->tab_len[0], 0, S, B->tab[0]);
+    launch(ctx, K_VC_RB, k_vc_rb<CT>, B->tab_len[0], 0, S, B->tab[0], 1);
+    launch(ctx, K_VC_RB, k_vc_rb<CT>, B->tab_len[0], 0, S, B->tab[0], 0);
+}
+
+void vcycle(phg_ctx* ctx, BatchDev* B) {
+    if (B->prec == PHG_PREC_FP32) vcycle_t<float, float>(ctx, B);
+    else if (B->prec == PHG_PREC_MIXED_F16) vcycle_t<_Float16, _Float16>(ctx, B);
+    else vcycle_t<_Float16, bf16_t>(ctx, B);
+}
+
+void cg_iteration(phg_ctx* ctx, BatchDev* B) {
+    const DSys* S = B->d_sys;
+    launch(ctx, K_APPLY, k_apply, B->tab_len[0], 0, S, B->tab[0]);
+    launch(ctx, K_FIN, k_fin_alpha, B->nsys, 0, S, B->tab_sys);
+    launch(ctx, K_AXPY, k_axpy, B->tab_len[0], 0, S, B->tab[0]);
+    vcycle(ctx, B);
+    launch(ctx, K_DOT_RZ, k_dot_rz, B->tab_len[0], 0, S, B->tab[0]);
+    launch(ctx, K_FIN, k_fin_beta, B->nsys, 0, S, B->tab_sys);
+    launch(ctx, K_UPDATE_D, k_update_d, B->tab_len[0], 0, S, B->tab[0], 0);
+}
+
+int upload_sys(phg_ctx* ctx, BatchDev* B) {
+    HIP_OK(hipMemcpyAsync(B->d_sys, B->h_sys.data(), sizeof(DSys) * B->nsys, hipMemcpyHostToDevice, ctx->stream));
+    return 0;
+}
+
+// set every system active with the given rule; scal/state from the host
+int reset_state(phg_ctx* ctx, BatchDev* B, const phg_solve_params* P) {
+    std::vector<double> scal((size_t)B->nsys * S_NSCAL, 0.0);
+    std::vector<int> state((size_t)B->nsys * ST_NSTATE, 0);
+    for (int s = 0; s < B->nsys; ++s) {
+        double* sc = &scal[(size_t)s * S_NSCAL];
+        int* st = &state[(size_t)s * ST_NSTATE];
+        sc[S_TOL] = P ? P->tol : 0.0;
+        sc[S_INV] = P ? P->inv : 1.0f;
+        sc[S_RZ] = 1.0;
+        st[ST_ACTIVE] = 1;
+        st[ST_MAXIT] = P ? P->max_iters : 0;
+        st[ST_RULE] = P && P->stop_rule == PHG_STOP_MAXNORM ? 1 : 0;
+        st[ST_ZEROX] = P && P->zero_x ? 1 : 0;
+    }
+    HIP_OK(hipMemcpyAsync(B->scal, scal.data(), scal.size() * sizeof(double), hipMemcpyHostToDevice, ctx->stream));
+    HIP_OK(hipMemcpyAsync(B->state, state.data(), state.size() * sizeof(int), hipMemcpyHostToDevice, ctx->stream));
+    // pageable sources: wait so the host vectors may go out of scope
+    HIP_OK(hipStreamSynchronize(ctx->stream));
+    return 0;
+}
+
+int demean(phg_ctx* ctx, GridDev* G, float* x) {
+    if (G->nclosed == 0) return 0;
+    launch(ctx, K_ZONE, k_zone_sum, G->nclosed, 0, G->zptr, G->zcells, (const float*)x, G->zmean);
+    launch(ctx, K_ZONE, k_zone_sub, tiles(G->n), 0, G->n, (const int*)G->zclosed, (const double*)G->zmean, x);
+    return 0;
+}
+
+GridDev* get_grid(phg_ctx* ctx, int id) {
+    auto it = ctx->grids.find(id);
+    return it == ctx->grids.end() ? nullptr : it->second;
+}
+BatchDev* get_batch(phg_ctx* ctx, int id) {
+    auto it = ctx->batches.find(id);
+    return it == ctx->batches.end() ? nullptr : it->second;
+}
+
+void destroy_batch(BatchDev* B) {
+    if (B->graph_exec) (void)hipGraphExecDestroy(B->graph_exec);
+    if (B->graph) (void)hipGraphDestroy(B->graph);
+    if (B->h_count) (void)hipHostFree(B->h_count);
+    free_all(B->allocs);
+    delete B;
+}
+
+void destroy_grid(GridDev* G) {
+    free_all(G->allocs);
+    delete G;
+}
+
+int single_batch(phg_ctx* ctx, int grid, BatchDev** out) {
+    GridDev* G = get_grid(ctx, grid);
+    if (!G) return fail(ctx, PHG_ESTATE, "unknown grid %d", grid);
+    if (G->single_batch < 0) {
+        int b = -1;
+        int rc = phg_batch_create(ctx, 1, &grid, &b);
+        if (rc) return rc;
+        G->single_batch = b;
+    }
+    *out = get_batch(ctx, G->single_batch);
+    return 0;
+}
+
+}  // namespace
+
+// ===========================================================================
+// C ABI
+// ===========================================================================
+extern "C" {
+
+int phg_create(phg_ctx** out, int device) {
+    if (!out) return PHG_EINVAL;
+    *out = nullptr;
+    int n = 0;
+    if (hipGetDeviceCount(&n) != hipSuccess || n <= 0) return PHG_EHIP;
+    if (device < 0 || device >= n) return PHG_EINVAL;
+    phg_ctx* ctx = new phg_ctx();
+    ctx->device = device;
+    if (hipSetDevice(device) != hipSuccess || hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking) != hipSuccess) {
+        delete ctx;
+        return PHG_EHIP;
+    }
+    *out = ctx;
+    return PHG_OK;
+}
+
+void phg_destroy(phg_ctx* ctx) {
+    if (!ctx) return;
+    (void)hipSetDevice(ctx->device);
+    (void)hipStreamSynchronize(ctx->stream);
+    for (auto& kv : ctx->batches) destroy_batch(kv.second);
+    for (auto& kv : ctx->grids) destroy_grid(kv.second);
+    for (auto& p : ctx->pending) {
+        (void)hipEventDestroy(p.a);
+        (void)hipEventDestroy(p.b);
+    }
+    for (auto e : ctx->pool) (void)hipEventDestroy(e);
+    (void)hipStreamDestroy(ctx->stream);
+    delete ctx;
+}
+
+const char* phg_last_error(phg_ctx* ctx) { return ctx ? ctx->err.c_str() : "null context"; }
+
+int phg_synchronize(phg_ctx* ctx) {
+    if (!ctx) return PHG_EINVAL;
+    HIP_OK(hipStreamSynchronize(ctx->stream));
+    if (ctx->prof) prof_drain(ctx);
+    return PHG_OK;
+}
+
+void* phg_stream(phg_ctx* ctx) { return ctx ? (void*)ctx->stream : nullptr; }
+
+int phg_malloc(phg_ctx* ctx, void** dptr, size_t bytes) {
+    if (!ctx || !dptr) return PHG_EINVAL;
+    (void)hipSetDevice(ctx->device);
+    hipError_t e = hipMalloc(dptr, bytes ? bytes : 1);
+    if (e != hipSuccess) return fail(ctx, PHG_ENOMEM, "hipMalloc(%zu): %s", bytes, hipGetErrorString(e));
+    return PHG_OK;
+}
+int phg_free(phg_ctx* ctx, void* dptr) {
+    if (!ctx) return PHG_EINVAL;
+    HIP_OK(hipStreamSynchronize(ctx->stream));
+    HIP_OK(hipFree(dptr));
+    return PHG_OK;
+}
+int phg_memcpy_h2d(phg_ctx* ctx, void* dst, const void* src, size_t bytes) {
+    if (!ctx) return PHG_EINVAL;
+    HIP_OK(hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, ctx->stream));
+    HIP_OK(hipStreamSynchronize(ctx->stream));
+    return PHG_OK;
+}
+int phg_memcpy_d2h(phg_ctx* ctx, void* dst, const void* src, size_t bytes) {
+    if (!ctx) return PHG_EINVAL;
+    HIP_OK(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, ctx->stream));
+    HIP_OK(hipStreamSynchronize(ctx->stream));
+    return PHG_OK;
+}
+int phg_memset(phg_ctx* ctx, void* dst, int value, size_t bytes) {
+    if (!ctx) return PHG_EINVAL;
+    HIP_OK(hipMemsetAsync(dst, value, bytes, ctx->stream));
+    return PHG_OK;
+}
+
+// ---------------------------------------------------------------------------
+int phg_grid_create(phg_ctx* ctx, int nx, int ny, int nz, float dx, const float* fx, const float* fy, const float* fz,
+                    const float* ax, const float* ay, const float* az, const uint8_t* blocked,
+                    const uint8_t* wind_closed, int precision, int* grid_out) {
+    if (!ctx || !grid_out) return PHG_EINVAL;
+    if (nx < 2 || ny < 2 || nz < 2 || (long)nx * ny * nz > (1L << 30) || !(dx > 0))
+        return fail(ctx, PHG_EINVAL, "bad grid %dx%dx%d dx=%g", nx, ny, nz, dx);
+    if (!fx || !fy || !fz || !ax || !ay || !az || !blocked) return fail(ctx, PHG_EINVAL, "null geometry array");
+    if (precision < PHG_PREC_FP32 || precision > PHG_PREC_MIXED_BF16)
+        return fail(ctx, PHG_EINVAL, "unknown precision %d", precision);
+    HIP_OK(hipSetDevice(ctx->device));
+    GridDev* G = new GridDev();
+    G->nx = nx;
+    G->ny = ny;
+    G->nz = nz;
+    G->n = nx * ny * nz;
+    G->dx = dx;
+    G->prec = precision;
+    std::string err;
+    if (!build_hierarchy(nx, ny, nz, ax, ay, az, blocked, wind_closed, kDenseMax, G->H, &err)) {
+        delete G;
+        return fail(ctx, PHG_EINVAL, "%s", err.c_str());
+    }
+    if (G->H.nc > kDenseMax) {
+        int nc = G->H.nc;
+        delete G;
+        return fail(ctx, PHG_EINVAL, "coarsest level has %d unknowns (limit %d)", nc, kDenseMax);
+    }
+    compute_zones(nx, ny, nz, ax, ay, az, blocked, G->zone, G->zone_open);
+    const size_t nfx = (size_t)(nx + 1) * ny * nz, nfy = (size_t)nx * (ny + 1) * nz, nfz = (size_t)nx * ny * (nz + 1);
+    const HostLevel& L0 = G->H.L[0];
+    int rc = 0;
+#define UP(dst, src, cnt) \
+    if (!rc) rc = upload(ctx, G->allocs, &dst, src, cnt);
+    UP(G->ax, L0.ax.data(), nfx)
+    UP(G->ay, L0.ay.data(), nfy)
+    UP(G->az, L0.az.data(), nfz)
+    UP(G->fx, fx, nfx)
+    UP(G->fy, fy, nfy)
+    UP(G->fz, fz, nfz)
+    UP(G->blocked, blocked, (size_t)G->n)
+    if (!rc) {
+        if (precision == PHG_PREC_FP32) {
+            G->axc = G->ax;
+            G->ayc = G->ay;
+            G->azc = G->az;
+        } else {
+            rc = upload_ct(ctx, G->allocs, &G->axc, L0.ax, precision);
+            if (!rc) rc = upload_ct(ctx, G->allocs, &G->ayc, L0.ay, precision);
+            if (!rc) rc = upload_ct(ctx, G->allocs, &G->azc, L0.az, precision);
+        }
+    }
+    const int nl = (int)G->H.L.size();
+    G->lv.resize(nl);
+    for (int l = 0; l < nl && !rc; ++l) {
+        const HostLevel& L = G->H.L[l];
+        GridDev::Lev& D = G->lv[l];
+        D.n = L.n;
+        D.ellk = L.ellk;
+        if (l > 0) {
+            UP(D.col, L.ell_col.data(), L.ell_col.size())
+            if (!rc) rc = upload_ct(ctx, G->allocs, &D.val, L.ell_val, precision);
+            if (!rc) rc = upload_ct(ctx, G->allocs, &D.diag, L.diag, precision);
+        }
+        if (l + 1 < nl) {
+            UP(D.agg, L.agg.data(), L.agg.size())
+            UP(D.cptr, L.cptr.data(), L.cptr.size())
+            UP(D.cidx, L.cidx.data(), L.cidx.size())
+        }
+    }
+    UP(G->cinv, G->H.cinv.data(), G->H.cinv.size())
+    // closed zones: cell lists for the deterministic mean removal
+    {
+        std::vector<int> zid(G->zone_open.size(), -1);
+        int nclosed = 0;
+        for (size_t z = 0; z < G->zone_open.size(); ++z)
+            if (!G->zone_open[z]) zid[z] = nclosed++;
+        std::vector<int> zptr(nclosed + 1, 0), zclosed(G->n, -1);
+        for (int c = 0; c < G->n; ++c)
+            if (G->zone[c] >= 0 && zid[G->zone[c]] >= 0) {
+                zclosed[c] = zid[G->zone[c]];
+                zptr[zclosed[c] + 1]++;
+            }
+        for (int z = 0; z < nclosed; ++z) zptr[z + 1] += zptr[z];
+        std::vector<int> zcells(std::max(zptr[nclosed], 1)), fill(zptr.begin(), zptr.end() - 1);
+        for (int c = 0; c < G->n; ++c)
+            if (zclosed[c] >= 0) zcells[fill[zclosed[c]]++] = c;
+        G->nclosed = nclosed;
+        UP(G->zptr, zptr.data(), zptr.size())
+        UP(G->zcells, zcells.data(), zcells.size())
+        UP(G->zclosed, zclosed.data(), zclosed.size())
+        if (!rc) rc = dalloc(ctx, G->allocs, &G->zmean, (size_t)std::max(nclosed, 1));
+    }
+    if (!rc) rc = dalloc(ctx, G->allocs, &G->scratch_b, (size_t)G->n);
+    if (!rc) rc = dalloc(ctx, G->allocs, &G->vmax, 1);
+#undef UP
+    if (rc) {
+        destroy_grid(G);
+        return rc;
+    }
+    const int id = ctx->next_id++;
+    ctx->grids[id] = G;
+    *grid_out = id;
+    return PHG_OK;
+}
+
+int phg_grid_destroy(phg_ctx* ctx, int grid) {
+    if (!ctx) return PHG_EINVAL;
+    GridDev* G = get_grid(ctx, grid);
+    if (!G) return fail(ctx, PHG_ESTATE, "unknown grid %d", grid);
+    HIP_OK(hipStreamSynchronize(ctx->stream));
+    if (G->single_batch >= 0) phg_batch_destroy(ctx, G->single_batch);
+    destroy_grid(G);
+    ctx->grids.erase(grid);
+    return PHG_OK;
+}
+
+int phg_grid_info(phg_ctx* ctx, int grid, int* nlevels, int* level_n, int* nzones, int* ref_levels) {
+    GridDev* G = ctx ? get_grid(ctx, grid) : nullptr;
+    if (!G) return ctx ? fail(ctx, PHG_ESTATE, "unknown grid %d", grid) : PHG_EINVAL;
+    if (nlevels) *nlevels = (int)G->H.L.size();
+    if (level_n)
+        for (size_t l = 0; l < G->H.L.size(); ++l) level_n[l] = G->H.L[l].n;
+    if (nzones) *nzones = (int)G->zone_open.size();
+    if (ref_levels) *ref_levels = G->H.ref_levels;
+    return PHG_OK;
+}
+
+int phg_grid_zones(phg_ctx* ctx, int grid, int* zone, uint8_t* zone_open) {
+    GridDev* G = ctx ? get_grid(ctx, grid) : nullptr;
+    if (!G) return ctx ? fail(ctx, PHG_ESTATE, "unknown grid %d", grid) : PHG_EINVAL;
+    if (zone) std::copy(G->zone.begin(), G->zone.end(), zone);
+    if (zone_open) std::copy(G->zone_open.begin(), G->zone_open.end(), zone_open);
+    return PHG_OK;
+}
+
+int phg_hierarchy_shape(int nx, int ny, int nz, const float* ax, const float* ay, const float* az,
+                        const uint8_t* blocked, const uint8_t* wind_closed, int* nlevels, int* level_n, int* agg0) {
+    if (!ax || !ay || !az || !blocked || !nlevels) return PHG_EINVAL;
+    HostHierarchy H;
+    std::string err;
+    if (!build_hierarchy(nx, ny, nz, ax, ay, az, blocked, wind_closed, kDenseMax, H, &err)) return PHG_EINVAL;
+    *nlevels = (int)H.L.size();
+    if (level_n)
+        for (size_t l = 0; l < H.L.size(); ++l) level_n[l] = H.L[l].n;
+    if (agg0) std::copy(H.L[0].agg.begin(), H.L[0].agg.end(), agg0);
+    return PHG_OK;
+}
+
+// ---------------------------------------------------------------------------
+int phg_batch_create(phg_ctx* ctx, int nsys, const int* grids, int* batch_out) {
+    if (!ctx || nsys <= 0 || !grids || !batch_out) return PHG_EINVAL;
+    HIP_OK(hipSetDevice(ctx->device));
+    BatchDev* B = new BatchDev();
+    B->nsys = nsys;
+    B->grids.assign(grids, grids + nsys);
+    B->h_sys.resize(nsys);
+    int rc = 0;
+    for (int s = 0; s < nsys && !rc; ++s) {
+        GridDev* G = get_grid(ctx, grids[s]);
+        if (!G) {
+            rc = fail(ctx, PHG_ESTATE, "unknown grid %d", grids[s]);
+            break;
+        }
+        if (s == 0) B->prec = G->prec;
+        else if (G->prec != B->prec) {
+            rc = fail(ctx, PHG_EINVAL, "all grids of a batch must share one precision");
+            break;
+        }
+        B->maxl = std::max(B->maxl, (int)G->lv.size());
+        B->maxnc = std::max(B->maxnc, G->H.nc);
+    }
+    if (rc) {
+        delete B;
+        return rc;
+    }
+    const size_t vts = vt_size(B->prec);
+    if (!rc) rc = dalloc(ctx, B->allocs, &B->scal, (size_t)nsys * S_NSCAL);
+    if (!rc) rc = dalloc(ctx, B->allocs, &B->state, (size_t)nsys * ST_NSTATE);
+    if (!rc) rc = dalloc(ctx, B->allocs, &B->d_count, 1);
+    if (!rc && hipHostMalloc((void**)&B->h_count, sizeof(int), hipHostMallocDefault) != hipSuccess)
+        rc = fail(ctx, PHG_ENOMEM, "hipHostMalloc failed");
+    for (int s = 0; s < nsys && !rc; ++s) {
+        GridDev* G = get_grid(ctx, grids[s]);
+        DSys& S = B->h_sys[s];
+        std::memset(&S, 0, sizeof(DSys));
+        S.nl = (int)G->lv.size();
+        S.nc = G->H.nc;
+        S.tiles0 = tiles(G->n);
+        S.grid = grids[s];
+        S.cinv = G->cinv;
+        S.scal = B->scal + (size_t)s * S_NSCAL;
+        S.state = B->state + (size_t)s * ST_NSTATE;
+        const size_t n0 = G->n;
+        float* v4 = nullptr;
+        rc = dalloc(ctx, B->allocs, &v4, 4 * n0);
+        if (rc) break;
+        S.r = v4;
+        S.d = v4 + n0;
+        S.q = v4 + 2 * n0;
+        S.z = v4 + 3 * n0;
+        rc = dalloc(ctx, B->allocs, &S.part, (size_t)S.tiles0 * NPART);
+        if (rc) break;
+        // V-cycle work vectors: level 0 residual scratch, coarse x b r t
+        size_t total = n0;
+        for (int l = 1; l < S.nl; ++l) total += 4 * (size_t)G->lv[l].n;
+        char* slab = nullptr;
+        rc = dalloc(ctx, B->allocs, &slab, total * vts);
+        if (rc) break;
+        for (int l = 0; l < S.nl; ++l) {
+            DLevel& D = S.L[l];
+            const GridDev::Lev& GL = G->lv[l];
+            D.n = GL.n;
+            D.ellk = GL.ellk;
+            D.col = GL.col;
+            D.val = GL.val;
+            D.diag = GL.diag;
+            D.agg = GL.agg;
+            D.cptr = GL.cptr;
+            D.cidx = GL.cidx;
+            if (l == 0) {
+                D.nx = G->nx;
+                D.ny = G->ny;
+                D.nz = G->nz;
+                D.ax32 = G->ax;
+                D.ay32 = G->ay;
+                D.az32 = G->az;
+                D.axc = G->axc;
+                D.ayc = G->ayc;
+                D.azc = G->azc;
+                D.r = slab;
+                slab += n0 * vts;
+            } else {
+                D.x = slab;
+                slab += GL.n * vts;
+                D.b = slab;
+                slab += GL.n * vts;
+                D.r = slab;
+                slab += GL.n * vts;
+                D.t = slab;
+                slab += GL.n * vts;
+            }
+        }
+    }
+    if (!rc) rc = dalloc(ctx, B->allocs, &B->d_sys, (size_t)nsys);
+    // tile tables
+    B->tab.assign(std::max(B->maxl, 1), nullptr);
+    B->tab_len.assign(std::max(B->maxl, 1), 0);
+    for (int l = 0; l < B->maxl && !rc; ++l) {
+        std::vector<int2> logical;
+        for (int s = 0; s < nsys; ++s) {
+            GridDev* G = get_grid(ctx, grids[s]);
+            if ((int)G->lv.size() <= l) continue;
+            const int t = tiles(G->lv[l].n);
+            for (int k = 0; k < t; ++k) logical.push_back(make_int2(s, k));
+        }
+        std::vector<int2> phys = xcd_order(logical);
+        B->tab_len[l] = (int)phys.size();
+        rc = upload(ctx, B->allocs, &B->tab[l], phys.data(), phys.size());
+    }
+    if (!rc) {
+        std::vector<int2> ts(nsys);
+        for (int s = 0; s < nsys; ++s) ts[s] = make_int2(s, 0);
+        rc = upload(ctx, B->allocs, &B->tab_sys, ts.data(), ts.size());
+    }
+    if (!rc) {
+        // coarsest-solve blocks: runs of up to CG_SYS consecutive systems on one grid
+        std::vector<int4> tc;
+        B->cg_sys = std::max(1, std::min(CG_SYS, 65536 / (8 * std::max(B->maxnc, 1))));
+        for (int s = 0; s < nsys;) {
+            int cnt = 1;
+            while (s + cnt < nsys && cnt < B->cg_sys && grids[s + cnt] == grids[s]) ++cnt;
+            const int nc = get_grid(ctx, grids[s])->H.nc;
+            for (int t = 0; t < (nc + CG_ROWS - 1) / CG_ROWS; ++t) tc.push_back(make_int4(s, cnt, t, 0));
+            s += cnt;
+        }
+        B->tab_coarse_len = (int)tc.size();
+        rc = upload(ctx, B->allocs, &B->tab_coarse, tc.data(), tc.size());
+    }
+    if (!rc) rc = upload_sys(ctx, B);
+    if (!rc) rc = hipStreamSynchronize(ctx->stream) == hipSuccess ? 0 : fail(ctx, PHG_EHIP, "sync");
+    if (rc) {
+        destroy_batch(B);
+        return rc;
+    }
+    const int id = ctx->next_id++;
+    ctx->batches[id] = B;
+    *batch_out = id;
+    return PHG_OK;
+}
+
+int phg_batch_destroy(phg_ctx* ctx, int batch) {
+    if (!ctx) return PHG_EINVAL;
+    BatchDev* B = get_batch(ctx, batch);
+    if (!B) return fail(ctx, PHG_ESTATE, "unknown batch %d", batch);
+    HIP_OK(hipStreamSynchronize(ctx->stream));
+    destroy_batch(B);
+    ctx->batches.erase(batch);
+    for (auto& kv : ctx->grids)
+        if (kv.second->single_batch == batch) kv.second->single_batch = -1;
+    return PHG_OK;
+}
+
+int phg_batch_solve(phg_ctx* ctx, int batch, const float* const* b, float* const* x, const phg_solve_params* P,
+                    phg_solve_stats* stats) {
+    if (!ctx || !b || !x || !P) return PHG_EINVAL;
+    BatchDev* B = get_batch(ctx, batch);
+    if (!B) return fail(ctx, PHG_ESTATE, "unknown batch %d", batch);
+    if (!(P->tol >= 0) || P->max_iters < 0) return fail(ctx, PHG_EINVAL, "bad tolerance or iteration cap");
+    HIP_OK(hipSetDevice(ctx->device));
+    for (int s = 0; s < B->nsys; ++s) {
+        if (!b[s] || !x[s]) return fail(ctx, PHG_EINVAL, "null vector for system %d", s);
+        B->h_sys[s].b = b[s];
+        B->h_sys[s].x = x[s];
+    }
+    int rc = upload_sys(ctx, B);
+    if (!rc) rc = reset_state(ctx, B, P);
+    if (rc) return rc;
+    const DSys* S = B->d_sys;
+    launch(ctx, K_RESID_INIT, k_resid_init, B->tab_len[0], 0, S, B->tab[0]);
+    launch(ctx, K_FIN, k_fin_init, B->nsys, 0, S, B->tab_sys);
+    vcycle(ctx, B);
+    launch(ctx, K_DOT_RZ, k_dot_rz, B->tab_len[0], 0, S, B->tab[0]);
+    launch(ctx, K_FIN, k_fin_rz0, B->nsys, 0, S, B->tab_sys);
+    launch(ctx, K_UPDATE_D, k_update_d, B->tab_len[0], 0, S, B->tab[0], 1);
+    const int check = P->check_every > 0 ? P->check_every : 4;
+    const bool graph = P->use_graph && !ctx->prof;
+    if (graph && !B->graph_exec) {
+        HIP_OK(hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal));
+        cg_iteration(ctx, B);
+        HIP_OK(hipStreamEndCapture(ctx->stream, &B->graph));
+        HIP_OK(hipGraphInstantiate(&B->graph_exec, B->graph, nullptr, nullptr, 0));
+    }
+    int done = 0;
+    while (done < P->max_iters) {
+        const int k = std::min(check, P->max_iters - done);
+        for (int i = 0; i < k; ++i) {
+            if (graph) HIP_OK(hipGraphLaunch(B->graph_exec, ctx->stream));
+            else cg_iteration(ctx, B);
+        }
+        done += k;
+        launch(ctx, K_COUNT, k_count_active, 1, 0, S, B->nsys, B->d_count);
+        HIP_OK(hipMemcpyAsync(B->h_count, B->d_count, sizeof(int), hipMemcpyDeviceToHost, ctx->stream));
+        HIP_OK(hipStreamSynchronize(ctx->stream));
+        if (*B->h_count == 0) break;
+    }
+    HIP_OK(hipGetLastError());
+    if (P->demean)
+        for (int s = 0; s < B->nsys; ++s) demean(ctx, get_grid(ctx, B->grids[s]), x[s]);
+    if (stats) {
+        std::vector<double> scal((size_t)B->nsys * S_NSCAL);
+        std::vector<int> state((size_t)B->nsys * ST_NSTATE);
+        HIP_OK(hipMemcpyAsync(scal.data(), B->scal, scal.size() * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
+        HIP_OK(hipMemcpyAsync(state.data(), B->state, state.size() * sizeof(int), hipMemcpyDeviceToHost, ctx->stream));
+        HIP_OK(hipStreamSynchronize(ctx->stream));
+        for (int s = 0; s < B->nsys; ++s) {
+            const double* sc = &scal[(size_t)s * S_NSCAL];
+            const int* st = &state[(size_t)s * ST_NSTATE];
+            stats[s].iters = st[ST_ITERS];
+            stats[s].status = st[ST_STATUS];
+            stats[s].relres = sc[S_BB] > 0 ? std::sqrt(sc[S_RR]) / std::sqrt(sc[S_BB]) : 0.0;
+            stats[s].maxres = (double)((float)sc[S_MAXR] / (float)sc[S_INV]);
+        }
+    }
+    HIP_OK(hipStreamSynchronize(ctx->stream));
+    if (ctx->prof) prof_drain(ctx);
+    return PHG_OK;
+}
+
+int phg_solve(phg_ctx* ctx, int grid, const float* b, float* x, const phg_solve_params* P, phg_solve_stats* stats) {
+    if (!ctx) return PHG_EINVAL;
+    BatchDev* B = nullptr;
+    int rc = single_batch(ctx, grid, &B);
+    if (rc) return rc;
+    return phg_batch_solve(ctx, get_grid(ctx, grid)->single_batch, &b, &x, P, stats);
+}
+
+int phg_apply(phg_ctx* ctx, int grid, const float* x, float* y, double* dot) {
+    if (!ctx || !x || !y) return PHG_EINVAL;
+    BatchDev* B = nullptr;
+    int rc = single_batch(ctx, grid, &B);
+    if (rc) return rc;
+    DSys saved = B->h_sys[0];
+    B->h_sys[0].d = const_cast<float*>(x);
+    B->h_sys[0].q = y;
+    rc = upload_sys(ctx, B);
+    phg_solve_params P{};
+    if (!rc) rc = reset_state(ctx, B, &P);
+    B->h_sys[0] = saved;
+    if (rc) return rc;
+    launch(ctx, K_APPLY, k_apply, B->tab_len[0], 0, (const DSys*)B->d_sys, B->tab[0]);
+    launch(ctx, K_FIN, k_fin_alpha, 1, 0, (const DSys*)B->d_sys, B->tab_sys);
+    double sc[S_NSCAL];
+    HIP_OK(hipMemcpyAsync(sc, B->scal, sizeof sc, hipMemcpyDeviceToHost, ctx->stream));
+    HIP_OK(hipStreamSynchronize(ctx->stream));
+    HIP_OK(hipGetLastError());
+    if (dot) *dot = sc[S_DQ];
+    return upload_sys(ctx, B);
+}
+
+int phg_precondition(phg_ctx* ctx, int grid, const float* r, float* z) {
+    if (!ctx || !r || !z) return PHG_EINVAL;
+    BatchDev* B = nullptr;
+    int rc = single_batch(ctx, grid, &B);
+    if (rc) return rc;
+    DSys saved = B->h_sys[0];
+    B->h_sys[0].r = const_cast<float*>(r);
+    B->h_sys[0].z = z;
+    rc = upload_sys(ctx, B);
+    phg_solve_params P{};
+    if (!rc) rc = reset_state(ctx, B, &P);
+    B->h_sys[0] = saved;
+    if (rc) return rc;
+    vcycle(ctx, B);
+    HIP_OK(hipStreamSynchronize(ctx->stream));
+    HIP_OK(hipGetLastError());
+    if (ctx->prof) prof_drain(ctx);
+    return upload_sys(ctx, B);
+}
+
+// ---------------------------------------------------------------------------
+int phg_rhs(phg_ctx* ctx, int grid, float dt, const float* u, const float* v, const float* w, const float* S, float* b,
+            float* diag, float* p) {
+    if (!ctx || !u || !v || !w || !S || !b || !p) return PHG_EINVAL;
+    GridDev* G = get_grid(ctx, grid);
+    if (!G) return fail(ctx, PHG_ESTATE, "unknown grid %d", grid);
+    if (!(dt > 0)) return fail(ctx, PHG_EINVAL, "dt must be positive");
+    const float inv = G->dx * G->dx / dt;
+    launch(ctx, K_RHS, k_rhs, tiles(G->n), 0, G->dg(), G->dx, inv, u, v, w, S, b, diag, p);
+    HIP_OK(hipGetLastError());
+    return PHG_OK;
+}
+
+int phg_zone_mean(phg_ctx* ctx, int grid, float* p) {
+    if (!ctx || !p) return PHG_EINVAL;
+    GridDev* G = get_grid(ctx, grid);
+    if (!G) return fail(ctx, PHG_ESTATE, "unknown grid %d", grid);
+    demean(ctx, G, p);
+    HIP_OK(hipGetLastError());
+    return PHG_OK;
+}
+
+int phg_correct(phg_ctx* ctx, int grid, float dt, const float* p, float* u, float* v, float* w, float* max_vel) {
+    if (!ctx || !p || !u || !v || !w) return PHG_EINVAL;
+    GridDev* G = get_grid(ctx, grid);
+    if (!G) return fail(ctx, PHG_ESTATE, "unknown grid %d", grid);
+    const int nfaces = (G->nx + 1) * G->ny * G->nz + G->nx * (G->ny + 1) * G->nz + G->n;
+    hipLaunchKernelGGL(k_fill_f32, dim3(1), dim3(64), 0, ctx->stream, G->vmax, 1, 0.0f);
+    launch(ctx, K_CORRECT, k_correct, tiles(nfaces), 0, G->dg(), dt / G->dx, p, u, v, w, G->vmax);
+    if (max_vel) {
+        HIP_OK(hipMemcpyAsync(max_vel, G->vmax, sizeof(float), hipMemcpyDeviceToHost, ctx->stream));
+        HIP_OK(hipStreamSynchronize(ctx->stream));
+    }
+    HIP_OK(hipGetLastError());
+    return PHG_OK;
+}
+
+int phg_project(phg_ctx* ctx, int grid, float dt, float* u, float* v, float* w, float* p, const float* S,
+                const phg_solve_params* params, phg_solve_stats* stats, float* max_vel) {
+    if (!ctx || !params) return PHG_EINVAL;
+    GridDev* G = get_grid(ctx, grid);
+    if (!G) return fail(ctx, PHG_ESTATE, "unknown grid %d", grid);
+    int rc = phg_rhs(ctx, grid, dt, u, v, w, S, G->scratch_b, nullptr, p);
+    if (rc) return rc;
+    phg_solve_params P = *params;
+    P.demean = 1;
+    P.inv = G->dx * G->dx / dt;
+    rc = phg_solve(ctx, grid, G->scratch_b, p, &P, stats);
+    if (rc) return rc;
+    return phg_correct(ctx, grid, dt, p, u, v, w, max_vel);
+}
+
+int phg_energy_expansion(phg_ctx* ctx, int grid, float dt, float* T, float* S, const float* Pb, const float* Q,
+                         const float* massSrc, float* max_div) {
+    if (!ctx || !T || !S || !Pb || !Q || !massSrc) return PHG_EINVAL;
+    GridDev* G = get_grid(ctx, grid);
+    if (!G) return fail(ctx, PHG_ESTATE, "unknown grid %d", grid);
+    hipLaunchKernelGGL(k_fill_f32, dim3(1), dim3(64), 0, ctx->stream, G->vmax, 1, 0.0f);
+    launch(ctx, K_ENERGY, k_energy, tiles(G->n), 0, G->n, G->dx, dt, (const uint8_t*)G->blocked, Pb, Q, massSrc, T, S,
+           G->vmax);
+    if (max_div) {
+        HIP_OK(hipMemcpyAsync(max_div, G->vmax, sizeof(float), hipMemcpyDeviceToHost, ctx->stream));
+        HIP_OK(hipStreamSynchronize(ctx->stream));
+    }
+    HIP_OK(hipGetLastError());
+    return PHG_OK;
+}
+
+int phg_advect_scalars(phg_ctx* ctx, int grid, float dt, const float* u, const float* v, const float* w,
+                       const float* Pb, const float* nu, const float* const* fields_in, float* const* fields_out) {
+    if (!ctx || !u || !v || !w || !Pb || !nu || !fields_in || !fields_out) return PHG_EINVAL;
+    GridDev* G = get_grid(ctx, grid);
+    if (!G) return fail(ctx, PHG_ESTATE, "unknown grid %d", grid);
+    Fields10 F;
+    for (int q = 0; q < 10; ++q) {
+        if (!fields_in[q] || !fields_out[q]) return fail(ctx, PHG_EINVAL, "null field %d", q);
+        for (int r = 0; r < 10; ++r)
+            if ((const void*)fields_out[q] == (const void*)fields_in[r])
+                return fail(ctx, PHG_EINVAL, "output field %d aliases input field %d", q, r);
+        F.in[q] = fields_in[q];
+        F.out[q] = fields_out[q];
+    }
+    launch(ctx, K_ADVECT, k_advect, tiles(G->n), 0, G->dg(), G->dx, dt, u, v, w, Pb, nu, F);
+    HIP_OK(hipGetLastError());
+    return PHG_OK;
+}
+
+int phg_profile_enable(phg_ctx* ctx, int on) {
+    if (!ctx) return PHG_EINVAL;
+    HIP_OK(hipStreamSynchronize(ctx->stream));
+    prof_drain(ctx);
+    ctx->prof = on != 0;
+    return PHG_OK;
+}
+
+int phg_profile_get(phg_ctx* ctx, int index, const char** name, double* total_ms, long long* launches, double* bytes) {
+    if (!ctx || index < 0 || index >= K_NCLASS) return PHG_EINVAL;
+    HIP_OK(hipStreamSynchronize(ctx->stream));
+    prof_drain(ctx);
+    if (name) *name = kKName[index];
+    if (total_ms) *total_ms = ctx->prof_ms[index];
+    if (launches) *launches = ctx->prof_n[index];
+    if (bytes) *bytes = 0;
+    return PHG_OK;
+}
+
+int phg_profile_reset(phg_ctx* ctx) {
+    if (!ctx) return PHG_EINVAL;
+    HIP_OK(hipStreamSynchronize(ctx->stream));
+    prof_drain(ctx);
+    for (int i = 0; i < K_NCLASS; ++i) {
+        ctx->prof_ms[i] = 0;
+        ctx->prof_n[i] = 0;
+    }
+    return PHG_OK;
+}
+
+}  // extern "C"

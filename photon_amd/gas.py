@@ -1,0 +1,363 @@
+"""Python host side of the MI355X gas-phase path, over the C ABI of
+libphoton_gas.so (include/photon_gas.h).
+
+It mirrors the pieces of the reference's photon::Solver that the path
+replaces (src/sim/solver.h:156-347): a GasSolver owns one grid's geometry and
+its device-resident fields and exposes project(dt), advect_scalars(dt) and
+energy_and_expansion(dt) with the reference's names and meaning, plus the
+batched solve used for many scenarios at once.  No CPU fallback exists: if
+the HIP library is missing or no GPU is present the calls raise.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libphoton_gas.so")
+
+PREC_FP32, PREC_MIXED_F16, PREC_MIXED_BF16 = 0, 1, 2
+STOP_REL_L2, STOP_MAXNORM = 0, 1
+CONVERGED, MAXITER, BREAKDOWN = 1, 2, 3
+
+EXPORTS = (
+    "phg_create", "phg_destroy", "phg_last_error", "phg_synchronize", "phg_stream", "phg_malloc", "phg_free",
+    "phg_memcpy_h2d", "phg_memcpy_d2h", "phg_memset", "phg_grid_create", "phg_grid_destroy", "phg_grid_info",
+    "phg_grid_zones", "phg_apply", "phg_precondition", "phg_batch_create", "phg_batch_destroy",
+    "phg_batch_solve", "phg_solve", "phg_project", "phg_rhs", "phg_zone_mean", "phg_correct",
+    "phg_energy_expansion", "phg_advect_scalars", "phg_profile_enable", "phg_profile_get", "phg_profile_reset",
+    "phg_hierarchy_shape",
+)
+
+
+class PhgError(RuntimeError):
+    pass
+
+
+class SolveParams(C.Structure):
+    _fields_ = [("tol", C.c_double), ("max_iters", C.c_int), ("stop_rule", C.c_int), ("inv", C.c_float),
+                ("demean", C.c_int), ("check_every", C.c_int), ("use_graph", C.c_int), ("zero_x", C.c_int)]
+
+
+class SolveStats(C.Structure):
+    _fields_ = [("iters", C.c_int), ("status", C.c_int), ("relres", C.c_double), ("maxres", C.c_double)]
+
+
+_lib = None
+
+
+def load_library():
+    """Load libphoton_gas.so (raises if it has not been built)."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise PhgError(f"{LIB_PATH} is missing: build it with `python -m photon_amd.build`")
+    L = C.CDLL(LIB_PATH)
+    vp, i32, f32, f64, sz = C.c_void_p, C.c_int, C.c_float, C.c_double, C.c_size_t
+    for name in EXPORTS:
+        getattr(L, name).restype = C.c_int
+    L.phg_create.argtypes = [C.POINTER(vp), i32]
+    L.phg_destroy.argtypes = [vp]
+    L.phg_destroy.restype = None
+    L.phg_last_error.argtypes = [vp]
+    L.phg_last_error.restype = C.c_char_p
+    L.phg_synchronize.argtypes = [vp]
+    L.phg_stream.argtypes = [vp]
+    L.phg_stream.restype = vp
+    L.phg_malloc.argtypes = [vp, C.POINTER(vp), sz]
+    L.phg_free.argtypes = [vp, vp]
+    L.phg_memcpy_h2d.argtypes = [vp, vp, vp, sz]
+    L.phg_memcpy_d2h.argtypes = [vp, vp, vp, sz]
+    L.phg_memset.argtypes = [vp, vp, i32, sz]
+    L.phg_grid_create.argtypes = [vp, i32, i32, i32, f32] + [vp] * 8 + [i32, C.POINTER(i32)]
+    L.phg_grid_destroy.argtypes = [vp, i32]
+    L.phg_grid_info.argtypes = [vp, i32, vp, vp, vp, vp]
+    L.phg_grid_zones.argtypes = [vp, i32, vp, vp]
+    L.phg_apply.argtypes = [vp, i32, vp, vp, C.POINTER(f64)]
+    L.phg_precondition.argtypes = [vp, i32, vp, vp]
+    L.phg_batch_create.argtypes = [vp, i32, vp, C.POINTER(i32)]
+    L.phg_batch_destroy.argtypes = [vp, i32]
+    L.phg_batch_solve.argtypes = [vp, i32, vp, vp, C.POINTER(SolveParams), vp]
+    L.phg_solve.argtypes = [vp, i32, vp, vp, C.POINTER(SolveParams), vp]
+    L.phg_project.argtypes = [vp, i32, f32, vp, vp, vp, vp, vp, C.POINTER(SolveParams), vp, C.POINTER(f32)]
+    L.phg_rhs.argtypes = [vp, i32, f32, vp, vp, vp, vp, vp, vp, vp]
+    L.phg_zone_mean.argtypes = [vp, i32, vp]
+    L.phg_correct.argtypes = [vp, i32, f32, vp, vp, vp, vp, C.POINTER(f32)]
+    L.phg_energy_expansion.argtypes = [vp, i32, f32, vp, vp, vp, vp, vp, C.POINTER(f32)]
+    L.phg_advect_scalars.argtypes = [vp, i32, f32, vp, vp, vp, vp, vp, vp, vp]
+    L.phg_profile_enable.argtypes = [vp, i32]
+    L.phg_profile_get.argtypes = [vp, i32, C.POINTER(C.c_char_p), C.POINTER(f64), C.POINTER(C.c_longlong), C.POINTER(f64)]
+    L.phg_profile_reset.argtypes = [vp]
+    L.phg_hierarchy_shape.argtypes = [i32, i32, i32] + [vp] * 5 + [C.POINTER(i32), vp, vp]
+    _lib = L
+    return L
+
+
+class DeviceArray:
+    """A device allocation of the context (float32 / int32 / uint8 ...)."""
+
+    def __init__(self, ctx: "Context", shape, dtype=np.float32):
+        self.ctx = ctx
+        self.dtype = np.dtype(dtype)
+        self.shape = (shape,) if isinstance(shape, (int, np.integer)) else tuple(shape)
+        self.size = int(np.prod(self.shape))
+        self.nbytes = self.size * self.dtype.itemsize
+        p = C.c_void_p()
+        ctx._check(ctx.L.phg_malloc(ctx.h, C.byref(p), max(self.nbytes, 1)))
+        self.ptr = p.value
+
+    @classmethod
+    def from_host(cls, ctx, arr, dtype=None):
+        a = np.ascontiguousarray(arr, dtype=dtype or arr.dtype)
+        d = cls(ctx, a.shape, a.dtype)
+        d.upload(a)
+        return d
+
+    def upload(self, arr):
+        a = np.ascontiguousarray(arr, dtype=self.dtype)
+        assert a.size == self.size, (a.size, self.size)
+        self.ctx._check(self.ctx.L.phg_memcpy_h2d(self.ctx.h, self.ptr, a.ctypes.data, self.nbytes))
+
+    def download(self):
+        out = np.empty(self.shape, self.dtype)
+        self.ctx._check(self.ctx.L.phg_memcpy_d2h(self.ctx.h, out.ctypes.data, self.ptr, self.nbytes))
+        return out
+
+    def zero(self):
+        self.ctx._check(self.ctx.L.phg_memset(self.ctx.h, self.ptr, 0, self.nbytes))
+
+    def free(self):
+        if self.ptr is not None and self.ctx.h:
+            self.ctx.L.phg_free(self.ctx.h, self.ptr)
+            self.ptr = None
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass
+
+
+class Context:
+    """One device and one HIP stream (phg_create)."""
+
+    def __init__(self, device: int = 0):
+        self.L = load_library()
+        h = C.c_void_p()
+        rc = self.L.phg_create(C.byref(h), device)
+        if rc != 0:
+            raise PhgError(f"phg_create(device={device}) failed with {rc}: no usable GPU")
+        self.h = h.value
+
+    def _check(self, rc):
+        if rc != 0:
+            raise PhgError(f"photon_gas error {rc}: {self.L.phg_last_error(self.h).decode()}")
+        return rc
+
+    def close(self):
+        if self.h:
+            self.L.phg_destroy(self.h)
+            self.h = None
+
+    def synchronize(self):
+        self._check(self.L.phg_synchronize(self.h))
+
+    def array(self, shape, dtype=np.float32):
+        return DeviceArray(self, shape, dtype)
+
+    def to_device(self, arr, dtype=None):
+        return DeviceArray.from_host(self, arr, dtype)
+
+    # profiling
+    def profile(self, on=True):
+        self._check(self.L.phg_profile_enable(self.h, 1 if on else 0))
+
+    def profile_reset(self):
+        self._check(self.L.phg_profile_reset(self.h))
+
+    def profile_read(self):
+        out = {}
+        i = 0
+        while True:
+            name, ms, n, by = C.c_char_p(), C.c_double(), C.c_longlong(), C.c_double()
+            if self.L.phg_profile_get(self.h, i, C.byref(name), C.byref(ms), C.byref(n), C.byref(by)) != 0:
+                break
+            out[name.value.decode()] = (ms.value, n.value)
+            i += 1
+        return out
+
+
+def solve_params(tol=1e-6, max_iters=200, stop_rule=STOP_REL_L2, inv=1.0, demean=True, check_every=4,
+                 use_graph=False, zero_x=False):
+    return SolveParams(tol, max_iters, stop_rule, inv, 1 if demean else 0, check_every, 1 if use_graph else 0,
+                       1 if zero_x else 0)
+
+
+class Grid:
+    """Geometry of one scenario on the device with its multigrid hierarchy
+    (phg_grid_create: buildPressureLevels + computeZones)."""
+
+    def __init__(self, ctx: Context, scen, precision=PREC_FP32, wind_closed=None):
+        self.ctx = ctx
+        self.scen = scen
+        self.precision = precision
+        self.nx, self.ny, self.nz, self.dx = scen.nx, scen.ny, scen.nz, scen.dx
+        self.n = scen.nx * scen.ny * scen.nz
+        arrs = [np.ascontiguousarray(a, np.float32) for a in (scen.fx, scen.fy, scen.fz, scen.ax, scen.ay, scen.az)]
+        blk = np.ascontiguousarray(scen.blocked, np.uint8)
+        wc = None if wind_closed is None else np.ascontiguousarray(wind_closed, np.uint8)
+        gid = C.c_int()
+        ctx._check(ctx.L.phg_grid_create(ctx.h, scen.nx, scen.ny, scen.nz, scen.dx,
+                                         *[a.ctypes.data for a in arrs], blk.ctypes.data,
+                                         None if wc is None else wc.ctypes.data, precision, C.byref(gid)))
+        self.id = gid.value
+
+    def info(self):
+        nl, nz, rl = C.c_int(), C.c_int(), C.c_int()
+        sizes = np.zeros(16, np.int32)
+        self.ctx._check(self.ctx.L.phg_grid_info(self.ctx.h, self.id, C.addressof(nl), sizes.ctypes.data,
+                                                 C.addressof(nz), C.addressof(rl)))
+        return dict(levels=sizes[:nl.value].tolist(), nzones=nz.value, ref_levels=rl.value)
+
+    def zones(self):
+        nz = self.info()["nzones"]
+        zone = np.zeros(self.n, np.int32)
+        zo = np.zeros(max(nz, 1), np.uint8)
+        self.ctx._check(self.ctx.L.phg_grid_zones(self.ctx.h, self.id, zone.ctypes.data, zo.ctypes.data))
+        return zone, zo[:nz]
+
+    def apply(self, x: DeviceArray, y: DeviceArray):
+        dot = C.c_double()
+        self.ctx._check(self.ctx.L.phg_apply(self.ctx.h, self.id, x.ptr, y.ptr, C.byref(dot)))
+        return dot.value
+
+    def precondition(self, r: DeviceArray, z: DeviceArray):
+        self.ctx._check(self.ctx.L.phg_precondition(self.ctx.h, self.id, r.ptr, z.ptr))
+
+    def solve(self, b: DeviceArray, x: DeviceArray, params: SolveParams):
+        st = SolveStats()
+        self.ctx._check(self.ctx.L.phg_solve(self.ctx.h, self.id, b.ptr, x.ptr, C.byref(params), C.addressof(st)))
+        return dict(iters=st.iters, status=st.status, relres=st.relres, maxres=st.maxres)
+
+    def destroy(self):
+        if self.id is not None and self.ctx.h:
+            self.ctx.L.phg_grid_destroy(self.ctx.h, self.id)
+            self.id = None
+
+
+class Batch:
+    """A fixed set of systems solved together (phg_batch_create / _solve)."""
+
+    def __init__(self, ctx: Context, grids):
+        self.ctx = ctx
+        self.grids = list(grids)
+        ids = np.array([g.id for g in self.grids], np.int32)
+        bid = C.c_int()
+        ctx._check(ctx.L.phg_batch_create(ctx.h, len(ids), ids.ctypes.data, C.byref(bid)))
+        self.id = bid.value
+
+    def solve(self, bs, xs, params: SolveParams):
+        n = len(self.grids)
+        assert len(bs) == n and len(xs) == n
+        bp = (C.c_void_p * n)(*[b.ptr if isinstance(b, DeviceArray) else b for b in bs])
+        xp = (C.c_void_p * n)(*[x.ptr if isinstance(x, DeviceArray) else x for x in xs])
+        st = (SolveStats * n)()
+        self.ctx._check(self.ctx.L.phg_batch_solve(self.ctx.h, self.id, bp, xp, C.byref(params), st))
+        return [dict(iters=s.iters, status=s.status, relres=s.relres, maxres=s.maxres) for s in st]
+
+    def destroy(self):
+        if self.id is not None and self.ctx.h:
+            self.ctx.L.phg_batch_destroy(self.ctx.h, self.id)
+            self.id = None
+
+
+FIELD_NAMES = ("T", "Yo", "Yf", "Ys", "Ya", "Yp", "Yc", "Yw", "Yco", "Yai")
+
+
+def hierarchy_shape(scen, wind_closed=None):
+    """Level sizes and level-0 aggregates the library builds for a geometry
+    (host only, no device needed)."""
+    L = load_library()
+    arrs = [np.ascontiguousarray(a, np.float32) for a in (scen.ax, scen.ay, scen.az)]
+    blk = np.ascontiguousarray(scen.blocked, np.uint8)
+    wc = None if wind_closed is None else np.ascontiguousarray(wind_closed, np.uint8)
+    nl = C.c_int()
+    sizes = np.zeros(16, np.int32)
+    agg = np.zeros(scen.nx * scen.ny * scen.nz, np.int32)
+    rc = L.phg_hierarchy_shape(scen.nx, scen.ny, scen.nz, *[a.ctypes.data for a in arrs], blk.ctypes.data,
+                               None if wc is None else wc.ctypes.data, C.byref(nl), sizes.ctypes.data,
+                               agg.ctypes.data)
+    if rc != 0:
+        raise PhgError(f"phg_hierarchy_shape failed with {rc}")
+    return sizes[:nl.value].tolist(), agg
+
+
+class GasSolver:
+    """Device-resident gas-phase fields of one grid with the reference's
+    phase names: project(dt), advect_scalars(dt), energy_and_expansion(dt)
+    (src/sim/gasphase.cpp:251-271, :505-677, :685-784).  Host numpy copies
+    are read with get(name) and written with set(name, array)."""
+
+    CELL_FIELDS = ("p", "S", "Pb", "Q", "massSrc", "nu") + FIELD_NAMES
+
+    def __init__(self, ctx: Context, scen, precision=PREC_FP32, wind_closed=None, tol=1e-6, max_iters=400,
+                 stop_rule=STOP_REL_L2):
+        self.ctx = ctx
+        self.scen = scen
+        self.grid = Grid(ctx, scen, precision, wind_closed)
+        n = self.grid.n
+        self.d = {}
+        for name in ("u", "v", "w"):
+            size = {"u": len(scen.fx), "v": len(scen.fy), "w": len(scen.fz)}[name]
+            self.d[name] = ctx.array(size)
+            self.d[name].zero()
+        init = dict(T=293.15, Yo=0.232, Yc=0.0006, Yw=0.0073, Pb=101325.0)
+        for name in self.CELL_FIELDS:
+            self.d[name] = ctx.to_device(np.full(n, init.get(name, 0.0), np.float32))
+        self._tmp = [ctx.array(n) for _ in range(10)]
+        self.params = solve_params(tol=tol, max_iters=max_iters, stop_rule=stop_rule)
+        self.pressureIters = 0
+        self.pressureResidual = 0.0
+        self.pressureRelRes = 0.0
+        self.maxVel = 0.0
+        self.maxDiv = 0.0
+
+    def set(self, name, arr):
+        self.d[name].upload(np.asarray(arr, np.float32))
+
+    def get(self, name):
+        return self.d[name].download()
+
+    def project(self, dt):
+        st = SolveStats()
+        vm = C.c_float()
+        d = self.d
+        self.ctx._check(self.ctx.L.phg_project(self.ctx.h, self.grid.id, dt, d["u"].ptr, d["v"].ptr, d["w"].ptr,
+                                               d["p"].ptr, d["S"].ptr, C.byref(self.params), C.addressof(st),
+                                               C.byref(vm)))
+        self.pressureIters = st.iters
+        self.pressureResidual = st.maxres
+        self.pressureRelRes = st.relres
+        self.maxVel = vm.value
+        return st.iters
+
+    def energy_and_expansion(self, dt):
+        md = C.c_float()
+        d = self.d
+        self.ctx._check(self.ctx.L.phg_energy_expansion(self.ctx.h, self.grid.id, dt, d["T"].ptr, d["S"].ptr,
+                                                        d["Pb"].ptr, d["Q"].ptr, d["massSrc"].ptr, C.byref(md)))
+        self.maxDiv = md.value
+        return md.value
+
+    def advect_scalars(self, dt):
+        d = self.d
+        ins = (C.c_void_p * 10)(*[d[f].ptr for f in FIELD_NAMES])
+        outs = (C.c_void_p * 10)(*[t.ptr for t in self._tmp])
+        self.ctx._check(self.ctx.L.phg_advect_scalars(self.ctx.h, self.grid.id, dt, d["u"].ptr, d["v"].ptr,
+                                                      d["w"].ptr, d["Pb"].ptr, d["nu"].ptr, ins, outs))
+        # swap, as the reference swaps tmp_ into the fields (gasphase.cpp:783)
+        for k, f in enumerate(FIELD_NAMES):
+            d[f], self._tmp[k] = self._tmp[k], d[f]

@@ -1,0 +1,22 @@
+"""Shared test set-up: repo root on sys.path, the `gpu` marker, and a
+session-wide HIP context for the GPU tests."""
+import os
+import sys
+
+import pytest
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+if ROOT not in sys.path:
+    sys.path.insert(0, ROOT)
+
+
+def pytest_configure(config):
+    config.addinivalue_line("markers", "gpu: needs an MI355X (runs through the HIP library)")
+
+
+@pytest.fixture(scope="session")
+def ctx():
+    from photon_amd import gas
+    c = gas.Context(0)
+    yield c
+    c.close()

@@ -1,0 +1,82 @@
+"""CPU tests of the host-side logic: the scenario generator, the library's
+host-built hierarchy against the oracle, and the C ABI's exported symbols."""
+import ctypes
+import os
+import re
+
+import numpy as np
+import pytest
+
+from oracle import pyoracle as po
+from photon_amd import gas
+from photon_amd import scenarios as sc
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def test_splitmix64_known_answer():
+    # SplitMix64 from seed 0: first outputs of the published reference generator
+    out = sc.splitmix64(0, 3)
+    assert [int(v) for v in out] == [0xE220A8397B1DCDAF, 0x6E789E6AA1B965F4, 0x06C45D188009454F]
+
+
+def test_normal_field_moments_and_determinism():
+    a = sc.normal_field(42, 200001)
+    assert abs(a.mean()) < 0.01 and abs(a.std() - 1) < 0.01
+    assert np.array_equal(a, sc.normal_field(42, 200001))
+
+
+def test_generator_is_seeded_and_refines():
+    g1, g2 = sc.house(1), sc.house(1)
+    assert np.array_equal(g1.blocked, g2.blocked) and np.array_equal(g1.ax, g2.ax)
+    assert not np.array_equal(sc.house(2).blocked, g1.blocked)
+    fine = sc.house(1, 160, 100, 80, 0.1)
+    # a 0.2 m cell is solid iff its 8 children at 0.1 m are
+    c = g1.blocked.reshape(40, 50, 80)
+    f = fine.blocked.reshape(80, 100, 160).reshape(40, 2, 50, 2, 80, 2).min(axis=(1, 3, 5))
+    assert np.array_equal(c, f)
+
+
+def test_zero_mean_rhs_per_closed_zone():
+    g = sc.house(1)
+    S = sc.divergence_field(g, 5).astype(np.float64)
+    for z in range(len(g.zone_open)):
+        m = g.zone == z
+        assert abs(S[m].mean()) < 1e-6
+    assert np.all(S[g.blocked != 0] == 0)
+
+
+def test_zones_agree_with_oracle():
+    for g in (sc.room(), sc.house(3), sc.batch_grid(2)):
+        zone, zo = po.zones(g)
+        assert np.array_equal(zone, g.zone) and np.array_equal(zo, g.zone_open)
+
+
+def test_library_hierarchy_matches_oracle():
+    for g in (sc.room(), sc.house(7, 24, 16, 12), sc.house(1), sc.batch_grid(5)):
+        sizes, agg0 = gas.hierarchy_shape(g)
+        H = po.Hierarchy(g)
+        assert sizes == [H.level(l)["n"] for l in range(H.nlevels)]
+        assert np.array_equal(agg0, H.level(0)["agg"])
+
+
+def test_abi_exports_every_declared_symbol():
+    header = open(os.path.join(ROOT, "include", "photon_gas.h")).read()
+    declared = set(re.findall(r"\b(phg_\w+)\s*\(", header))
+    assert declared, "no declarations parsed"
+    lib = ctypes.CDLL(gas.LIB_PATH)
+    missing = [s for s in sorted(declared) if not hasattr(lib, s)]
+    assert not missing, missing
+    assert declared <= set(gas.EXPORTS)
+
+
+def test_no_gpu_gives_clear_error_not_fallback():
+    # in a container without a GPU the context must refuse, never compute on the CPU
+    import subprocess
+    import sys
+    code = ("import sys; sys.path.insert(0, %r)\n"
+            "from photon_amd import gas\n"
+            "try:\n    gas.Context(0)\nexcept gas.PhgError as e:\n    print('refused', e)\n"
+            "else:\n    print('context')\n") % ROOT
+    out = subprocess.run([sys.executable, "-c", code], capture_output=True, text=True, timeout=120).stdout
+    assert "refused" in out or "context" in out

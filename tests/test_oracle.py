@@ -1,0 +1,231 @@
+"""CPU tests of the oracle (the checker the GPU path is compared against).
+
+Pins, strongest first:
+  1. bit-exact agreement with the golden vectors written by the reference's
+     own solver (tests/golden/make_golden.py, reference sources compiled with
+     -ffp-contract=off): hierarchy, V-cycle, project() after K iterations,
+     energyAndExpansion, advectScalars;
+  2. the same comparison live against oracle/_ref when it has been built;
+  3. mathematics that fixes the answer independently of any restatement:
+     a dense direct solve, manufactured solutions, the cosine eigenvectors of
+     a uniform box, A 1 = 0 on closed zones, symmetry of A and of the V-cycle.
+"""
+import os
+
+import numpy as np
+import pytest
+
+from oracle import pyoracle as po
+from photon_amd import scenarios as sc
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+DT = 0.01
+CASES = {"room": sc.room, "walls": lambda: sc.house(7, 24, 16, 12)}
+FIELDS = ("T", "Yo", "Yf", "Ys", "Ya", "Yp", "Yc", "Yw", "Yco", "Yai")
+
+
+def golden(name):
+    return np.load(os.path.join(HERE, "golden", f"ref_{name}.npz"))
+
+
+@pytest.fixture(scope="module", params=list(CASES))
+def case(request):
+    g = CASES[request.param]()
+    return request.param, g, golden(request.param), po.Hierarchy(g)
+
+
+# ---------------------------------------------------------------- 1. golden --
+def test_zones_match_golden(case):
+    _, g, G, _ = case
+    zone, zo = po.zones(g)
+    assert np.array_equal(zone, G["zone"]) and np.array_equal(zo, G["zone_open"])
+    assert np.array_equal(g.zone, G["zone"]) and np.array_equal(g.zone_open, G["zone_open"])
+
+
+def test_hierarchy_matches_golden(case):
+    _, g, G, H = case
+    assert [H.level(l)["n"] for l in range(H.nlevels)] == G["level_n"].tolist()
+    L1 = H.level(1)
+    for k in ("rowStart", "col", "val", "diag"):
+        assert np.array_equal(L1[k], G["l1_" + k]), k
+    assert np.array_equal(H.level(0)["agg"], G["l0_agg"])
+
+
+def test_vcycle_matches_golden(case):
+    _, g, G, H = case
+    z = H.precondition(G["vc_r"], coarse=po.COARSE_GS20)
+    assert np.array_equal(z, G["vc_z"])
+
+
+def test_project_matches_golden(case):
+    _, g, G, H = case
+    u, v, w = G["pj_u0"].copy(), G["pj_v0"].copy(), G["pj_w0"].copy()
+    p = np.zeros(g.n, np.float32)
+    o = po.project(H, g, DT, u, v, w, p, G["pj_S"], tol=0.0, max_iters=int(G["pj_K"]), stop=po.STOP_MAXNORM,
+                   coarse=po.COARSE_GS20)
+    assert o["iters"] == int(G["pj_iters"])
+    assert np.float32(o["residual"]) == G["pj_residual"]
+    assert np.float32(o["max_vel"]) == G["pj_maxvel"]
+    assert np.array_equal(o["b"], G["pj_rhs"])
+    assert np.array_equal(p, G["pj_p"])
+    assert np.array_equal(u, G["pj_u"]) and np.array_equal(v, G["pj_v"]) and np.array_equal(w, G["pj_w"])
+
+
+def test_energy_matches_golden(case):
+    _, g, G, _ = case
+    T, S = G["en_T0"].copy(), np.zeros(g.n, np.float32)
+    md = po.energy_expansion(g, DT, np.full(g.n, 101325.0, np.float32), G["en_Q"], G["en_ms"], T, S)
+    assert np.array_equal(T, G["en_T"]) and np.array_equal(S, G["en_S"])
+    assert np.float32(md) == G["en_maxdiv"]
+
+
+def test_advect_matches_golden(case):
+    _, g, G, _ = case
+    fields = [G["en_T0"]] + [G["ad_in_" + f] for f in FIELDS[1:]]
+    out = po.advect_scalars(g, DT, G["pj_u"], G["pj_v"], G["pj_w"], np.full(g.n, 101325.0, np.float32),
+                            G["ad_nu"], fields)
+    for f, o in zip(FIELDS, out):
+        assert np.array_equal(o, G["ad_out_" + f]), f
+
+
+# --------------------------------------------------- 2. live reference build --
+@pytest.mark.skipif(not po.ref_available(pin=True), reason="oracle/_ref not built (no reference checkout)")
+def test_live_reference_walls_larger():
+    g = sc.house(11, 40, 30, 20)
+    H = po.Hierarchy(g)
+    R = po.RefSolver(g, threads=2, pin=True)
+    lv = R.levels()
+    assert [l["n"] for l in lv] == [H.level(l)["n"] for l in range(H.nlevels)]
+    r = sc.rhs_from_divergence(g, sc.divergence_field(g, 3), DT)
+    assert np.array_equal(R.precondition(r), H.precondition(r))
+    S = sc.divergence_field(g, 5)
+    for f, n in (("u", len(g.fx)), ("v", len(g.fy)), ("w", len(g.fz))):
+        R.set(f, np.zeros(n, np.float32))
+    R.set("p", np.zeros(g.n, np.float32))
+    R.set("S", S)
+    it, res, vm = R.project(DT, 0.0, 9)
+    u, v, w = (np.zeros(len(a), np.float32) for a in (g.fx, g.fy, g.fz))
+    p = np.zeros(g.n, np.float32)
+    o = po.project(H, g, DT, u, v, w, p, S, tol=0.0, max_iters=9, stop=po.STOP_MAXNORM)
+    assert o["iters"] == it and np.array_equal(p, R.get("p", g.n)) and np.array_equal(u, R.get("u", len(g.fx)))
+
+
+# ------------------------------------------------------------ 3. mathematics --
+def dense_matrix(g):
+    """The level-0 operator as a dense FP64 matrix, built straight from the
+    face coefficients (independent of the oracle's code)."""
+    nx, ny, nz = g.shape
+    n = g.n
+    A = np.zeros((n, n))
+    ax = g.ax.reshape(nz, ny, nx + 1)
+    ay = g.ay.reshape(nz, ny + 1, nx)
+    az = g.az.reshape(nz + 1, ny, nx)
+    for k in range(nz):
+        for j in range(ny):
+            for i in range(nx):
+                c = i + nx * (j + ny * k)
+                if g.blocked[c]:
+                    continue
+                faces = [(ax[k, j, i], c - 1 if i > 0 else -1), (ax[k, j, i + 1], c + 1 if i < nx - 1 else -1),
+                         (ay[k, j, i], c - nx if j > 0 else -1), (ay[k, j + 1, i], c + nx if j < ny - 1 else -1),
+                         (az[k, j, i], c - nx * ny if k > 0 else -1),
+                         (az[k + 1, j, i], c + nx * ny if k < nz - 1 else -1)]
+                for a, nb in faces:
+                    A[c, c] += a
+                    if nb >= 0:
+                        A[c, nb] -= a
+    return A
+
+
+def tiny_walls():
+    return sc.house(5, 10, 8, 6, n_walls=4, n_open=2)
+
+
+def test_direct_solve_pins_fp64_oracle():
+    for g in (tiny_walls(), sc.room(0.4)):
+        A = dense_matrix(g)
+        b = sc.rhs_from_divergence(g, sc.divergence_field(g, 2), DT).astype(np.float64)
+        bp = po.closed_zone_demean(g, b)
+        x_dir = np.linalg.lstsq(A, bp, rcond=None)[0]
+        x_dir = po.closed_zone_demean(g, x_dir)
+        x, it, res = po.Hierarchy(g).pcg64(b)
+        assert np.linalg.norm(x - x_dir) <= 1e-9 * np.linalg.norm(x_dir)
+
+
+def test_fp32_reference_method_close_to_direct():
+    g = tiny_walls()
+    A = dense_matrix(g)
+    b = sc.rhs_from_divergence(g, sc.divergence_field(g, 4), DT)
+    H = po.Hierarchy(g)
+    x, it, rel = H.pcg(b, tol=1e-6)
+    x = po.closed_zone_demean(g, x)
+    x_dir = po.closed_zone_demean(g, np.linalg.lstsq(A, po.closed_zone_demean(g, b), rcond=None)[0])
+    assert rel <= 1e-6
+    assert np.linalg.norm(x - x_dir) <= 1e-5 * np.linalg.norm(x_dir)
+
+
+def test_apply_against_dense_matrix_and_symmetry():
+    g = tiny_walls()
+    A = dense_matrix(g)
+    H = po.Hierarchy(g)
+    x = sc.normal_field(8, g.n).astype(np.float32)
+    x[g.blocked != 0] = 0
+    y, dot = H.apply(x)
+    assert np.allclose(y, A @ x.astype(np.float64), rtol=1e-5, atol=1e-5)
+    assert np.allclose(A, A.T)
+    assert abs(dot - float(x.astype(np.float64) @ (A @ x))) <= 1e-4 * abs(dot)
+
+
+def test_constant_is_null_vector_on_closed_zones():
+    g = tiny_walls()
+    H = po.Hierarchy(g)
+    for z in range(len(g.zone_open)):
+        assert not g.zone_open[z]
+        x = (g.zone == z).astype(np.float32)
+        y, _ = H.apply(x)
+        assert np.abs(y).max() == 0.0
+
+
+def test_cosine_eigenvector_of_uniform_box():
+    nx, ny, nz = 8, 6, 5
+    g = sc._finish(np.zeros((nz, ny, nx), np.uint8), 0.2)   # closed box
+    kx, ky, kz = 3, 2, 1
+    i, j, k = np.meshgrid(np.arange(nx), np.arange(ny), np.arange(nz), indexing="ij")
+    v = (np.cos(np.pi * kx * (i + 0.5) / nx) * np.cos(np.pi * ky * (j + 0.5) / ny) *
+         np.cos(np.pi * kz * (k + 0.5) / nz)).transpose(2, 1, 0).ravel()
+    lam = 2 * (1 - np.cos(np.pi * kx / nx)) + 2 * (1 - np.cos(np.pi * ky / ny)) + 2 * (1 - np.cos(np.pi * kz / nz))
+    H = po.Hierarchy(g)
+    y, _ = H.apply(v.astype(np.float32))
+    assert np.allclose(y, lam * v, atol=2e-6)
+    # and the FP64 solve inverts it: A x = lam v  ->  x = v
+    x, _, _ = H.pcg64(lam * v)
+    assert np.allclose(x, v, atol=1e-10)
+
+
+def test_manufactured_solution():
+    g = sc.house(13, 20, 14, 10, n_walls=8, n_open=3)
+    A = dense_matrix(g)
+    xs = po.closed_zone_demean(g, sc.normal_field(21, g.n))
+    b = A @ xs
+    x, it, res = po.Hierarchy(g).pcg64(b, tol=1e-13)
+    assert np.linalg.norm(x - xs) <= 1e-9 * np.linalg.norm(xs)
+
+
+@pytest.mark.parametrize("coarse", [po.COARSE_GS20, po.COARSE_EXACT])
+def test_vcycle_is_symmetric_positive(coarse):
+    g = tiny_walls()
+    H = po.Hierarchy(g)
+    rng = np.random.default_rng(1)
+    a, b = (po.closed_zone_demean(g, rng.standard_normal(g.n)) for _ in range(2))
+    Ma, Mb = H.precondition(a, coarse, fp64=True), H.precondition(b, coarse, fp64=True)
+    assert abs(b @ Ma - a @ Mb) <= 1e-10 * abs(b @ Ma)
+    assert a @ Ma > 0 and b @ Mb > 0
+
+
+def test_exact_coarse_reduces_iterations():
+    g = sc.house(1)
+    H = po.Hierarchy(g)
+    b = sc.rhs_from_divergence(g, sc.divergence_field(g, 123), DT)
+    _, it_gs, _ = H.pcg(b, tol=1e-6, coarse=po.COARSE_GS20)
+    _, it_ex, _ = H.pcg(b, tol=1e-6, coarse=po.COARSE_EXACT)
+    assert it_ex <= it_gs
