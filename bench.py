@@ -37,28 +37,26 @@ DT = 0.01
 TOL = 1e-6
 
 
-def kernel_bytes(name, n0, vt_bytes, coef_bytes):
-    """Algorithmic HBM bytes per level-0 cell of one system for the level-0
-    kernels (one pass per array touched); coefficients (3 face arrays) are
-    counted once per launch per grid, not per system."""
+def kernel_bytes(name, n0, vt):
+    """Algorithmic HBM bytes of the level-0 kernels: (bytes per cell of one
+    system, bytes per launch shared by all systems of the grid).  One pass per
+    array touched; a half sweep writes half of z; outer CG vectors are FP32,
+    z and the residual scratch are `vt` bytes; the grid's 0/1 face mask
+    (1 B/cell) is read once per launch."""
     per_cell = {
-        "apply": 8,                       # read d, write q
-        "axpy": 24,                       # read x d r q, write x r
-        "vc_rb": 10,                      # read r, read z, write half of z
-        "vc_smooth0": 8,                  # read r, write z
-        "vc_resid0": 8 + vt_bytes,        # read r z, write r0
-        "dot_rz": 8,
-        "update_d": 12,
-        "resid_init": 12,
+        "dir_apply": 20 + vt,          # read z d_old x, write x d_new q
+        "pre_red": 12 + vt,            # read r q, write r z
+        "rb_black": 4 + 1.5 * vt,      # read r z, write half of z
+        "resid0": 4 + 2 * vt,          # read r z, write r0
+        "post_black": 4 + 1.5 * vt,    # read r z (+ t1 of level 1), write half of z
+        "post_red": 4 + 1.5 * vt,      # read r z, write half of z
     }
-    return per_cell.get(name), 3 * n0 * coef_bytes
+    return per_cell.get(name), n0 * 1
 
 
 def launches_per_solve(name, iters):
     """Level-0 launches of a kernel class for one system solved in `iters` iterations."""
-    vcycles = iters + 1
-    return {"apply": iters, "axpy": iters, "vc_rb": 3 * vcycles, "vc_smooth0": vcycles,
-            "vc_resid0": vcycles, "dot_rz": vcycles, "update_d": vcycles, "resid_init": 1}.get(name, 0)
+    return iters if name == "dir_apply" else iters + 1
 
 
 def run_batch(ctx, gas, grid_scen, prec, bs_host, steps, warmup, profile):
@@ -91,11 +89,16 @@ def run_batch(ctx, gas, grid_scen, prec, bs_host, steps, warmup, profile):
         ctx.profile(False)
     iters = np.array([[s["iters"] for s in st] for st in all_stats])
     ok = all(s["status"] == gas.CONVERGED and s["relres"] <= TOL for st in all_stats for s in st)
+    status = {}
+    for st in all_stats:
+        for s in st:
+            key = {1: "converged", 2: "max_iters", 3: "breakdown"}.get(s["status"], "running")
+            status[key] = status.get(key, 0) + 1
     x0 = x_all.download()[0] if nsys else None
     B.destroy()
     G.destroy()
     del b_all, x_all
-    return dict(seconds=t1 - t0, iters=iters, ok=ok, prof=prof, x0=x0)
+    return dict(seconds=t1 - t0, iters=iters, ok=ok, prof=prof, x0=x0, status=status)
 
 
 def run_plain(ctx, gas, grid_scen, bs_host, nsolve):
@@ -161,11 +164,11 @@ def cpu_baseline(scen, bs_host, iters_ref, budget_s=15.0):
             H.pcg(b, tol=TOL)
             t_used += time.perf_counter() - t0
             done += 1
-    return {"value": done / t_used, "unit": "solves/s", "cores": threads, "kind": kind,
-            "sample": (f"{done} solves of the 80x50x40 house workload ({'Solver::project of the reference, '
-                       'its GCC flags' if kind == 'reference' else 'the C restatement'}, FP32 MG-PCG run to the "
-                       f"reference method's iteration count for ||r||/||b|| <= 1e-6, mean {np.mean(iters_ref[:done]):.1f}"
-                       f" iterations), {threads} threads on {cpu_model}")}
+    what = "Solver::project of the reference, its GCC flags" if kind == "reference" else "the C restatement"
+    sample = (f"{done} solves of the 80x50x40 house workload ({what}, FP32 MG-PCG run to the reference "
+              f"method's iteration count for ||r||/||b|| <= 1e-6, mean {np.mean(iters_ref[:done]):.1f} "
+              f"iterations), {threads} threads on {cpu_model}")
+    return {"value": done / t_used, "unit": "solves/s", "cores": threads, "kind": kind, "sample": sample}
 
 
 def main():
@@ -203,10 +206,10 @@ def main():
 
     # roofline of the dominant kernel, from HIP events over the timed steps
     prof = mixed["prof"]
-    level0 = ["apply", "axpy", "vc_rb", "vc_smooth0", "vc_resid0", "dot_rz", "update_d", "resid_init"]
+    level0 = ["dir_apply", "pre_red", "rb_black", "resid0", "post_black", "post_red"]
     dom = max(prof, key=lambda k: prof[k][0])
     dom0 = max(level0, key=lambda k: prof.get(k, (0, 0))[0])
-    per_cell, coef = kernel_bytes(dom0, n, 2, 2)
+    per_cell, coef = kernel_bytes(dom0, n, 2)
     total_launch = prof[dom0][1]
     units = sum(launches_per_solve(dom0, it) for it in mixed["iters"].ravel())   # system-launches
     total_bytes = units * n * per_cell + total_launch * coef
@@ -257,6 +260,8 @@ def main():
         "iters_fp32_mean": float(fp32["iters"].mean()),
         "iters_plain_mean": float(plain["iters"].mean()),
         "all_converged": bool(mixed["ok"] and fp32["ok"]),
+        "status_mixed": mixed["status"],
+        "status_fp32": fp32["status"],
         "roofline": roof,
         "dominant_kernel_by_time": dom,
         "kernel_time_share": time_share,

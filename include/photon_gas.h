@@ -121,6 +121,12 @@ int phg_precondition(phg_ctx *ctx, int grid, const float *r, float *z);
  * repeat) that advance together, one kernel launch per step for all of
  * them.  The batch owns every work vector; the caller owns b and x. */
 int phg_batch_create(phg_ctx *ctx, int nsys, const int *grids, int *batch_out);
+/* tile shape of a batch (chosen per size class by the autotuner) */
+typedef struct {
+    int kchunk;          /* level-0 planes per block along z (0 = from level0_blocks) */
+    int level0_blocks;   /* level-0 blocks per launch to aim for when kchunk = 0 (0 = 4096) */
+} phg_batch_opts;
+int phg_batch_create_ex(phg_ctx *ctx, int nsys, const int *grids, const phg_batch_opts *opts, int *batch_out);
 int phg_batch_destroy(phg_ctx *ctx, int batch);
 /* The CG loop of Solver::project (gasphase.cpp:571-627) for every system:
  * x (warm start in, solution out) and b are arrays of nsys device pointers.
@@ -155,6 +161,10 @@ int phg_energy_expansion(phg_ctx *ctx, int grid, float dt, float *T, float *S, c
  * alias in (the reference writes tmp_ and swaps). */
 int phg_advect_scalars(phg_ctx *ctx, int grid, float dt, const float *u, const float *v, const float *w,
                        const float *Pb, const float *nu, const float *const *fields_in, float *const *fields_out);
+
+/* ---- measurement aid: a plain device copy (16-byte vector loads/stores),
+ * the calibration kernel for HBM byte counters; bytes must be a multiple of 16 */
+int phg_copy(phg_ctx *ctx, void *dst, const void *src, size_t bytes);
 
 /* ---- profiling: HIP-event timing of each kernel class on the context stream */
 int phg_profile_enable(phg_ctx *ctx, int on);

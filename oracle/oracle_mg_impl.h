@@ -168,7 +168,10 @@ static double FN(apply_slice)(const orc_level *L, const REAL *x, REAL *y, int k)
             REAL a0 = L->ax[i + (nx + 1) * (j + ny * k)], a1 = L->ax[i + 1 + (nx + 1) * (j + ny * k)];
             REAL a2 = L->ay[i + nx * (j + (ny + 1) * k)], a3 = L->ay[i + nx * (j + 1 + (ny + 1) * k)];
             REAL a4 = L->az[i + nx * (j + ny * k)], a5 = L->az[i + nx * (j + ny * (k + 1))];
-            REAL s = dg * x[c];
+            /* the diagonal is the sum of the faces: in FP32 this is bitwise the
+             * stored diag (same order of float adds); in FP64 it is the exact sum,
+             * so A 1 = 0 holds exactly on closed zones with fractional faces */
+            REAL s = (a0 + a1 + a2 + a3 + a4 + a5) * x[c];
             if (i > 0) s -= a0 * x[c - 1];
             if (i < nx - 1) s -= a1 * x[c + 1];
             if (j > 0) s -= a2 * x[c - nx];
@@ -270,7 +273,10 @@ double FN(orc_true_relres)(const orc_mg *mg, const REAL *b, const REAL *x) {
                 int c = i + nx * (j + ny * k);
                 bb += (double)b[c] * (double)b[c];
                 if (L->diag[c] <= 0) continue;
-                double s = (double)L->diag[c] * (double)x[c];
+                const double dsum = (double)L->ax[i + (nx + 1) * (j + ny * k)] + (double)L->ax[i + 1 + (nx + 1) * (j + ny * k)] +
+                                    (double)L->ay[i + nx * (j + (ny + 1) * k)] + (double)L->ay[i + nx * (j + 1 + (ny + 1) * k)] +
+                                    (double)L->az[i + nx * (j + ny * k)] + (double)L->az[i + nx * (j + ny * (k + 1))];
+                double s = (sizeof(REAL) == sizeof(double) ? dsum : (double)L->diag[c]) * (double)x[c];
                 if (i > 0) s -= (double)L->ax[i + (nx + 1) * (j + ny * k)] * (double)x[c - 1];
                 if (i < nx - 1) s -= (double)L->ax[i + 1 + (nx + 1) * (j + ny * k)] * (double)x[c + 1];
                 if (j > 0) s -= (double)L->ay[i + nx * (j + (ny + 1) * k)] * (double)x[c - nx];

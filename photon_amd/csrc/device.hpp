@@ -14,6 +14,8 @@ constexpr int NPART = 4;         // FP64 partials per block: dq, rr, max|r|, rz
 struct bf16_t {
     uint16_t u;
 };
+// coefficient tag: level-0 face coefficients read from the 1-byte face mask
+struct Mask01 {};
 
 __device__ __forceinline__ float to_f(float v) { return v; }
 __device__ __forceinline__ float to_f(_Float16 v) { return (float)v; }
@@ -52,6 +54,8 @@ struct DLevel {
     const int* col;               // coarse: ELL columns, column-major [ellk][n]
     const void* val;              // coarse: ELL values (V-cycle coefficient type)
     const void* diag;             // coarse: diagonal (V-cycle coefficient type)
+    const uint8_t* mask;          // level 0, grids whose coefficients are all 0 or 1: bit d set when
+                                  // face d (-x,+x,-y,+y,-z,+z) has coefficient 1; else null
     const int* agg;               // unknown -> next-level unknown (-1: none)
     const int* cptr;              // children of next-level unknowns into this level
     const int* cidx;
@@ -62,24 +66,39 @@ struct DLevel {
 };
 
 // scalar slots (FP64) and state slots (int) of one system
-enum { S_RZ = 0, S_DQ, S_ALPHA, S_BETA, S_RR, S_BB, S_TOL, S_MAXR, S_INV, S_NSCAL = 16 };
-enum { ST_ACTIVE = 0, ST_ITERS, ST_MAXIT, ST_STATUS, ST_RULE, ST_ZEROX, ST_NSTATE = 8 };
+// S_ZS: power-of-two scale applied to the V-cycle input (its residual made
+// O(1) so FP16/BF16 storage stays in range; exact in FP32); S_ZSN: the next one
+enum { S_RZ = 0, S_DQ, S_ALPHA, S_BETA, S_RR, S_BB, S_TOL, S_MAXR, S_INV, S_ZS, S_ZSN, S_NSCAL = 16 };
+enum { ST_ACTIVE = 0, ST_ITERS, ST_MAXIT, ST_STATUS, ST_RULE, ST_ZEROX, ST_XPEND, ST_NSTATE = 8 };
 
+// One system of a batch.  Level 0 is swept by columns: a block takes TB
+// consecutive (i, j) columns of one k-chunk of `kchunk` planes, so that
+// tiles0 = ceil(nx*ny/TB) * ceil(nz/kchunk) blocks cover the grid.
 struct DSys {
     int nl;                       // levels; the coarsest is nl-1
     int nc;                       // coarsest size
-    int tiles0;                   // blocks of level 0 (stride of the partials)
-    int grid;
+    int tiles0;                   // level-0 blocks (stride of the partials)
+    int kchunk;                   // planes per level-0 block
     const double* cinv;           // nc*nc FP64 inverse of the grounded coarsest operator
-    const float* b;               // outer CG vectors, FP32, level-0 cells
+    const float* b;               // outer CG vectors, FP32 on level-0 cells
     float* x;
     float* r;
-    float* d;
     float* q;
-    float* z;
+    float* d0;                    // direction, double-buffered: d_k lives in d[k & 1]
+    float* d1;
+    void* z;                      // preconditioned residual, V-cycle vector type
+    const float* invd;            // 1 / diag of level 0 (0 where diag <= 0), FP32
     double* scal;
     int* state;
     double* part;                 // [tiles0][NPART]
+    // closed-zone mean removal (src/sim/gasphase.cpp:628-639)
+    int nclosed, nztask;
+    const int4* ztask;            // (zone, first, end) chunks of the zone cell list
+    const int* ztptr;             // tasks of each zone: ztptr[z] .. ztptr[z+1]
+    const int* zcells;            // cells sorted by closed zone
+    const int* zclosed;           // per cell: closed-zone index or -1
+    double* zpart;                // per task
+    double* zmean;                // per closed zone
     DLevel L[MAXL];
 };
 

@@ -1,17 +1,33 @@
 // kernels_pcg.hpp -- batched MG-PCG kernels for gfx950.
 //
-// Every kernel works on a batch of independent systems.  A block handles one
-// 256-unknown tile of one system; the (system, tile) pair comes from a tile
-// table built at batch creation and ordered so that consecutive tiles of a
-// system land on the same XCD (blocks are dispatched round-robin over the
-// 8 XCDs, each with its own L2).  Systems that have stopped leave at once.
+// Every kernel works on a batch of independent systems.  The (system, tile)
+// pair of a block comes from a tile table built at batch creation and
+// ordered so that consecutive tiles of a system land on the same XCD (blocks
+// are dispatched round-robin over the 8 XCDs, each with its own L2).
+// Systems that have stopped leave at once.
 //
-// Reference: the CG loop src/sim/gasphase.cpp:571-627 (apply :551-570, FP64
-// dot products, FP64 alpha/beta, FP32 vector updates) and the V-cycle
-// src/sim/pressure_mg.cpp:195-300.  Level 0 of the V-cycle works in place on
-// the outer vectors: its right-hand side is the CG residual r and its
+// Level 0 (the structured grid, where nearly all the bytes are) is swept by
+// columns: a thread owns one (i, j) column and walks the planes of its block's
+// k-chunk, so consecutive lanes read consecutive cells of a plane and the
+// index arithmetic is done once per column.  One CG iteration is five
+// level-0 passes (reference: the CG loop src/sim/gasphase.cpp:603-627 and the
+// V-cycle src/sim/pressure_mg.cpp:270-300):
+//   k_dir_apply   d = z + beta d_old, q = A d, fused d.q, and the deferred
+//                 x += alpha_prev d_old of the previous iteration
+//   k_pre_red     r -= alpha q, fused |r|^2 and max|r|; first pre-smoothing
+//                 half sweep from zero (red: r/diag, black: 0)
+//   k_rb_black    black half sweep                         (gsColor(1))
+//   k_resid0      level-0 residual into the V-cycle scratch (residualGrid)
+//   ... restriction, coarse levels, coarsest solve, prolongation ...
+//   k_post_black  prolongation fused into the black half sweep: a GS update
+//                 replaces the cell's own value, so only the red neighbours'
+//                 prolonged values are needed, and they are formed on the fly
+//   k_post_red    red half sweep fused with the r.z partial
+// The V-cycle's right-hand side on level 0 is the CG residual r and its
 // iterate is z, so no copy-in/copy-out (pressure_mg.cpp:273,299) is needed.
 #pragma once
+#include <type_traits>
+
 #include "device.hpp"
 
 namespace phg {
@@ -23,43 +39,65 @@ struct Cell {
     int c, i, j, k;
 };
 
-__device__ __forceinline__ Cell tile_cell(int tile, int nx, int ny) {
-    Cell q;
-    q.c = tile * TB + threadIdx.x;
-    cell_ijk(q.c, nx, ny, q.i, q.j, q.k);
-    return q;
-}
-
-// the six face coefficients of a cell: -x, +x, -y, +y, -z, +z
+// the six face coefficients of a cell (-x, +x, -y, +y, -z, +z): from face
+// arrays in the reference's layout (FP32 or FP16), or, for grids whose
+// coefficients are all 0 or 1, from the 1-byte face mask (1 B/cell instead of
+// 12 B/cell of FP32 faces, exact)
 template <class CT>
-__device__ __forceinline__ void coefs6(const void* ax, const void* ay, const void* az, const Cell& q, int nx, int ny,
+__device__ __forceinline__ void coefs6(const DLevel& L, const void* ax, const void* ay, const void* az, const Cell& q,
                                        float a[6]) {
-    const int xf = q.c + q.j + ny * q.k;   // i + (nx+1)(j + ny k)
-    const int yf = q.c + nx * q.k;         // i + nx (j + (ny+1) k)
-    a[0] = ld<CT>(ax, xf);
-    a[1] = ld<CT>(ax, xf + 1);
-    a[2] = ld<CT>(ay, yf);
-    a[3] = ld<CT>(ay, yf + nx);
-    a[4] = ld<CT>(az, q.c);
-    a[5] = ld<CT>(az, q.c + nx * ny);
-}
-
-// sum over existing neighbours of a_f * x_nb (a missing neighbour is the
-// Dirichlet boundary, value 0)
-template <class XT>
-__device__ __forceinline__ float nb_sum(const void* x, const Cell& q, int nx, int ny, int nz, const float a[6]) {
-    const int sxy = nx * ny;
-    float s = 0.0f;
-    if (q.i > 0) s += a[0] * ld<XT>(x, q.c - 1);
-    if (q.i < nx - 1) s += a[1] * ld<XT>(x, q.c + 1);
-    if (q.j > 0) s += a[2] * ld<XT>(x, q.c - nx);
-    if (q.j < ny - 1) s += a[3] * ld<XT>(x, q.c + nx);
-    if (q.k > 0) s += a[4] * ld<XT>(x, q.c - sxy);
-    if (q.k < nz - 1) s += a[5] * ld<XT>(x, q.c + sxy);
-    return s;
+    if constexpr (std::is_same<CT, Mask01>::value) {
+        const unsigned b = L.mask[q.c];
+#pragma unroll
+        for (int d = 0; d < 6; ++d) a[d] = (float)((b >> d) & 1u);
+    } else {
+        const int nx = L.nx, ny = L.ny;
+        const int xf = q.c + q.j + ny * q.k;   // i + (nx+1)(j + ny k)
+        const int yf = q.c + nx * q.k;         // i + nx (j + (ny+1) k)
+        a[0] = ld<CT>(ax, xf);
+        a[1] = ld<CT>(ax, xf + 1);
+        a[2] = ld<CT>(ay, yf);
+        a[3] = ld<CT>(ay, yf + nx);
+        a[4] = ld<CT>(az, q.c);
+        a[5] = ld<CT>(az, q.c + nx * ny);
+    }
 }
 
 __device__ __forceinline__ float diag6(const float a[6]) { return a[0] + a[1] + a[2] + a[3] + a[4] + a[5]; }
+
+// sum over existing neighbours of a_f * v(nb) (a missing neighbour is the
+// Dirichlet boundary, value 0); v is a functor of the neighbour index.
+// Branch-free: a missing neighbour reads the cell itself with weight 0, so
+// all six loads are issued together.
+template <class F>
+__device__ __forceinline__ float nb_sum_f(const Cell& q, int nx, int ny, int nz, const float a[6], F v) {
+    const int sxy = nx * ny;
+    const bool e0 = q.i > 0, e1 = q.i < nx - 1, e2 = q.j > 0, e3 = q.j < ny - 1, e4 = q.k > 0, e5 = q.k < nz - 1;
+    const float v0 = v(e0 ? q.c - 1 : q.c), v1 = v(e1 ? q.c + 1 : q.c), v2 = v(e2 ? q.c - nx : q.c),
+                v3 = v(e3 ? q.c + nx : q.c), v4 = v(e4 ? q.c - sxy : q.c), v5 = v(e5 ? q.c + sxy : q.c);
+    return (e0 ? a[0] : 0.0f) * v0 + (e1 ? a[1] : 0.0f) * v1 + (e2 ? a[2] : 0.0f) * v2 + (e3 ? a[3] : 0.0f) * v3 +
+           (e4 ? a[4] : 0.0f) * v4 + (e5 ? a[5] : 0.0f) * v5;
+}
+
+// the column and k-range of a level-0 block
+struct ColTile {
+    int i, j, c2, k0, k1;
+    bool valid;
+};
+__device__ __forceinline__ ColTile col_tile(const DSys& s, int tile) {
+    const DLevel& L = s.L[0];
+    const int sxy = L.nx * L.ny;
+    const int nct = (sxy + TB - 1) / TB;
+    const int ct = tile % nct, kc = tile / nct;
+    ColTile t;
+    t.c2 = ct * TB + threadIdx.x;
+    t.valid = t.c2 < sxy;
+    t.j = t.c2 / L.nx;
+    t.i = t.c2 - t.j * L.nx;
+    t.k0 = kc * s.kchunk;
+    t.k1 = min(L.nz, t.k0 + s.kchunk);
+    return t;
+}
 
 #define PHG_SYS_PROLOGUE                  \
     const int2 e = tab[blockIdx.x];       \
@@ -69,29 +107,36 @@ __device__ __forceinline__ float diag6(const float a[6]) { return a[0] + a[1] + 
 // ---------------------------------------------------------------------------
 // outer CG kernels (FP32 vectors, FP64 reductions)
 // ---------------------------------------------------------------------------
-// r = b - A x on gas cells (0 elsewhere); partials |b|^2 (slot 0), |r|^2 (1), max|r| (2)
+// r = b - A x on gas cells (0 elsewhere), or x = 0 and r = b when starting
+// from zero; partials |b|^2 (slot 0), |r|^2 (1), max|r| (2)
+template <class OT>
 __global__ void __launch_bounds__(TB) k_resid_init(const DSys* __restrict__ sys, const int2* __restrict__ tab) {
     PHG_SYS_PROLOGUE
     const DLevel& L = s.L[0];
-    const Cell q = tile_cell(e.y, L.nx, L.ny);
+    const ColTile t = col_tile(s, e.y);
+    const int sxy = L.nx * L.ny;
     double v[2] = {0.0, 0.0};
     float m = 0.0f;
-    if (q.c < L.n) {
-        float a[6];
-        coefs6<float>(L.ax32, L.ay32, L.az32, q, L.nx, L.ny, a);
-        const float dg = diag6(a);
-        const float b = s.b[q.c];
-        float r = 0.0f;
-        if (s.state[ST_ZEROX]) {
-            s.x[q.c] = 0.0f;
-            if (dg > 0) r = b;
-        } else if (dg > 0) {
-            r = b - (dg * s.x[q.c] - nb_sum<float>(s.x, q, L.nx, L.ny, L.nz, a));
+    if (t.valid) {
+        const bool zx = s.state[ST_ZEROX] != 0;
+        for (int k = t.k0; k < t.k1; ++k) {
+            const Cell q{t.c2 + sxy * k, t.i, t.j, k};
+            float a[6];
+            coefs6<OT>(L, L.ax32, L.ay32, L.az32, q, a);
+            const float dg = diag6(a);
+            const float b = s.b[q.c];
+            float r = 0.0f;
+            if (zx) {
+                s.x[q.c] = 0.0f;
+                if (dg > 0) r = b;
+            } else if (dg > 0) {
+                r = b - (dg * s.x[q.c] - nb_sum_f(q, L.nx, L.ny, L.nz, a, [&](int n) { return s.x[n]; }));
+            }
+            s.r[q.c] = r;
+            v[0] += (double)b * b;
+            v[1] += (double)r * r;
+            m = fmaxf(m, fabsf(r));
         }
-        s.r[q.c] = r;
-        v[0] = (double)b * b;
-        v[1] = (double)r * r;
-        m = fabsf(r);
     }
     double* out = s.part + (long)e.y * NPART;
     block_sums<2>(v, out);
@@ -99,84 +144,59 @@ __global__ void __launch_bounds__(TB) k_resid_init(const DSys* __restrict__ sys,
     if (threadIdx.x == 0) out[2] = m;
 }
 
-// q = A d and the partial d.q (slot 0)  (apply, gasphase.cpp:551-570)
-__global__ void __launch_bounds__(TB) k_apply(const DSys* __restrict__ sys, const int2* __restrict__ tab) {
+// d_k = z + beta d_{k-1} (d_0 = z), q = A d_k with the partial d.q (slot 0)
+// (gasphase.cpp:620-622 and the apply of :606), and the previous iteration's
+// x += alpha d_{k-1} (gasphase.cpp:612), deferred to this pass which reads
+// d_{k-1} anyway.  d is double-buffered because neighbours read d_{k-1}
+// while d_k is written.
+template <class OT, class ZT>
+__global__ void __launch_bounds__(TB) k_dir_apply(const DSys* __restrict__ sys, const int2* __restrict__ tab) {
     PHG_SYS_PROLOGUE
     const DLevel& L = s.L[0];
-    const Cell q = tile_cell(e.y, L.nx, L.ny);
+    const ColTile t = col_tile(s, e.y);
+    const int sxy = L.nx * L.ny;
+    const int it = s.state[ST_ITERS];
+    const bool first = it == 0;
+    float* __restrict__ dnew = (it & 1) ? s.d1 : s.d0;
+    const float* __restrict__ dold = (it & 1) ? s.d0 : s.d1;
+    const double beta = s.scal[S_BETA], alpha = s.scal[S_ALPHA];
+    const void* z = s.z;
+    const float izs = (float)(1.0 / s.scal[S_ZS]);   // z is stored scaled by S_ZS (a power of two)
+    auto dval = [&](int n) { return first ? ld<ZT>(z, n) * izs : ld<ZT>(z, n) * izs + (float)(beta * dold[n]); };
     double v[1] = {0.0};
-    if (q.c < L.n) {
-        float a[6];
-        coefs6<float>(L.ax32, L.ay32, L.az32, q, L.nx, L.ny, a);
-        const float dg = diag6(a);
-        float y = 0.0f;
-        const float dc = s.d[q.c];
-        if (dg > 0) y = dg * dc - nb_sum<float>(s.d, q, L.nx, L.ny, L.nz, a);
-        s.q[q.c] = y;
-        v[0] = (double)y * dc;
+    if (t.valid) {
+        for (int k = t.k0; k < t.k1; ++k) {
+            const Cell q{t.c2 + sxy * k, t.i, t.j, k};
+            float a[6];
+            coefs6<OT>(L, L.ax32, L.ay32, L.az32, q, a);
+            const float dg = diag6(a);
+            const float dc = dval(q.c);
+            float y = 0.0f;
+            if (dg > 0) y = dg * dc - nb_sum_f(q, L.nx, L.ny, L.nz, a, dval);
+            if (!first) s.x[q.c] += (float)(alpha * dold[q.c]);
+            dnew[q.c] = dc;
+            s.q[q.c] = y;
+            v[0] += (double)y * dc;
+        }
     }
     block_sums<1>(v, s.part + (long)e.y * NPART);
 }
 
-// x += alpha d, r -= alpha q (gasphase.cpp:609-615); partials |r|^2 (slot 1), max|r| (2).
-// Cells with diag <= 0 hold d = q = r = 0, so they need no test.
-__global__ void __launch_bounds__(TB) k_axpy(const DSys* __restrict__ sys, const int2* __restrict__ tab) {
-    PHG_SYS_PROLOGUE
-    const int n = s.L[0].n;
-    const int c = e.y * TB + threadIdx.x;
+// after the loop: the last x += alpha d of systems that stopped (ST_XPEND)
+__global__ void __launch_bounds__(TB) k_x_flush(const DSys* __restrict__ sys, const int2* __restrict__ tab) {
+    const int2 e = tab[blockIdx.x];
+    const DSys& s = sys[e.x];
+    if (s.state[ST_XPEND] == 0) return;
+    const ColTile t = col_tile(s, e.y);
+    if (!t.valid) return;
+    const int sxy = s.L[0].nx * s.L[0].ny;
+    const int it = s.state[ST_ITERS];
+    const float* __restrict__ dk = ((it - 1) & 1) ? s.d1 : s.d0;
     const double alpha = s.scal[S_ALPHA];
-    double v[1] = {0.0};
-    float m = 0.0f;
-    if (c < n) {
-        s.x[c] += (float)(alpha * s.d[c]);
-        const float r = s.r[c] - (float)(alpha * s.q[c]);
-        s.r[c] = r;
-        v[0] = (double)r * r;
-        m = fabsf(r);
+    for (int k = t.k0; k < t.k1; ++k) {
+        const int c = t.c2 + sxy * k;
+        s.x[c] += (float)(alpha * dk[c]);
     }
-    double* out = s.part + (long)e.y * NPART;
-    // slot 1 <- |r|^2, slot 2 <- max|r|
-    __shared__ double sh[TB / 64];
-    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
-    v[0] = wave_sum(v[0]);
-    if (lane == 0) sh[w] = v[0];
-    __syncthreads();
-    m = block_max(m);
-    if (threadIdx.x == 0) {
-        double acc = 0.0;
-#pragma unroll
-        for (int ww = 0; ww < TB / 64; ++ww) acc += sh[ww];
-        out[1] = acc;
-        out[2] = m;
-    }
-}
-
-// partial r.z (slot 3)
-__global__ void __launch_bounds__(TB) k_dot_rz(const DSys* __restrict__ sys, const int2* __restrict__ tab) {
-    PHG_SYS_PROLOGUE
-    const int c = e.y * TB + threadIdx.x;
-    double v[1] = {0.0};
-    if (c < s.L[0].n) v[0] = (double)s.r[c] * s.z[c];
-    __shared__ double sh[TB / 64];
-    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
-    v[0] = wave_sum(v[0]);
-    if (lane == 0) sh[w] = v[0];
-    __syncthreads();
-    if (threadIdx.x == 0) {
-        double acc = 0.0;
-#pragma unroll
-        for (int ww = 0; ww < TB / 64; ++ww) acc += sh[ww];
-        s.part[(long)e.y * NPART + 3] = acc;
-    }
-}
-
-// d = z + beta d (gasphase.cpp:620-622); first = 1 copies d = z
-__global__ void __launch_bounds__(TB) k_update_d(const DSys* __restrict__ sys, const int2* __restrict__ tab, int first) {
-    PHG_SYS_PROLOGUE
-    const int c = e.y * TB + threadIdx.x;
-    if (c >= s.L[0].n) return;
-    if (first) s.d[c] = s.z[c];
-    else s.d[c] = s.z[c] + (float)(s.scal[S_BETA] * s.d[c]);
 }
 
 // ---- per-system finalisers: fixed-order sums of the block partials --------
@@ -189,6 +209,14 @@ __device__ __forceinline__ float max_parts(const DSys& s) {
     float m = 0.0f;
     for (int t = threadIdx.x; t < s.tiles0; t += TB) m = fmaxf(m, (float)s.part[(long)t * NPART + 2]);
     return block_max(m);
+}
+
+// power-of-two scale that brings a residual of squared norm rr over n cells to rms ~1
+__device__ __forceinline__ double vcycle_scale(double rr, int n) {
+    if (!(rr > 0)) return 1.0;
+    double e = rint(0.5 * log2((double)n / rr));
+    e = fmin(fmax(e, -60.0), 60.0);
+    return exp2(e);
 }
 
 __device__ __forceinline__ bool stop_reached(const DSys& s, double rr, float maxr) {
@@ -211,9 +239,12 @@ __global__ void __launch_bounds__(TB) k_fin_init(const DSys* __restrict__ sys, c
         s.scal[S_BB] = res[0];
         s.scal[S_RR] = res[1];
         s.scal[S_MAXR] = m;
+        s.scal[S_ZS] = s.scal[S_ZSN] = vcycle_scale(res[1], s.L[0].n);
         s.state[ST_ITERS] = 0;
-        if (stop_reached(s, res[1], m) || s.state[ST_MAXIT] <= 0) {
-            s.state[ST_STATUS] = stop_reached(s, res[1], m) ? 1 : 2;
+        s.state[ST_XPEND] = 0;
+        const bool done = stop_reached(s, res[1], m);
+        if (done || s.state[ST_MAXIT] <= 0) {
+            s.state[ST_STATUS] = done ? 1 : 2;
             s.state[ST_ACTIVE] = 0;
         }
     }
@@ -226,7 +257,7 @@ __global__ void __launch_bounds__(TB) k_fin_rz0(const DSys* __restrict__ sys, co
     sum_parts(s, 3, &res[0]);
     __syncthreads();
     if (threadIdx.x == 0) {
-        s.scal[S_RZ] = res[0];
+        s.scal[S_RZ] = res[0] / s.scal[S_ZS];
         if (!(res[0] > 0)) {
             s.state[ST_STATUS] = 3;
             s.state[ST_ACTIVE] = 0;
@@ -234,7 +265,7 @@ __global__ void __launch_bounds__(TB) k_fin_rz0(const DSys* __restrict__ sys, co
     }
 }
 
-// after k_apply: alpha = rz / dq, or stop on dq <= 0 (gasphase.cpp:606-608)
+// after k_dir_apply: alpha = rz / dq, or stop on dq <= 0 (gasphase.cpp:606-608)
 __global__ void __launch_bounds__(TB) k_fin_alpha(const DSys* __restrict__ sys, const int2* __restrict__ tab) {
     PHG_SYS_PROLOGUE
     __shared__ double res[1];
@@ -248,11 +279,12 @@ __global__ void __launch_bounds__(TB) k_fin_alpha(const DSys* __restrict__ sys, 
             s.state[ST_ACTIVE] = 0;
         } else {
             s.scal[S_ALPHA] = s.scal[S_RZ] / dq;
+            s.scal[S_ZS] = s.scal[S_ZSN];   // z of the previous V-cycle has been consumed
         }
     }
 }
 
-// after k_dot_rz: beta, iteration count and the stop test (gasphase.cpp:617-625)
+// after k_post_red: beta, iteration count and the stop test (gasphase.cpp:617-625)
 __global__ void __launch_bounds__(TB) k_fin_beta(const DSys* __restrict__ sys, const int2* __restrict__ tab) {
     PHG_SYS_PROLOGUE
     __shared__ double res[2];
@@ -262,22 +294,22 @@ __global__ void __launch_bounds__(TB) k_fin_beta(const DSys* __restrict__ sys, c
     const float m = max_parts(s);
     __syncthreads();
     if (threadIdx.x == 0) {
-        const double rz_new = res[0];
+        const double rz_new = res[0] / s.scal[S_ZS];
+        s.scal[S_ZSN] = vcycle_scale(res[1], s.L[0].n);
         s.scal[S_BETA] = rz_new / s.scal[S_RZ];
         s.scal[S_RZ] = rz_new;
         s.scal[S_RR] = res[1];
         s.scal[S_MAXR] = m;
         const int it = s.state[ST_ITERS] + 1;
         s.state[ST_ITERS] = it;
-        if (stop_reached(s, res[1], m)) {
-            s.state[ST_STATUS] = 1;
+        int status = 0;
+        if (stop_reached(s, res[1], m)) status = 1;
+        else if (it >= s.state[ST_MAXIT]) status = 2;
+        else if (!(rz_new > 0)) status = 3;
+        if (status) {
+            s.state[ST_STATUS] = status;
             s.state[ST_ACTIVE] = 0;
-        } else if (it >= s.state[ST_MAXIT]) {
-            s.state[ST_STATUS] = 2;
-            s.state[ST_ACTIVE] = 0;
-        } else if (!(rz_new > 0)) {
-            s.state[ST_STATUS] = 3;
-            s.state[ST_ACTIVE] = 0;
+            s.state[ST_XPEND] = 1;   // this iteration's x += alpha d is still owed
         }
     }
 }
@@ -298,63 +330,184 @@ __global__ void k_count_active(const DSys* __restrict__ sys, int nsys, int* out)
 // ---------------------------------------------------------------------------
 // V-cycle, level 0 (in place on the outer vectors: b = r, x = z)
 // ---------------------------------------------------------------------------
-// first pre-smoothing half sweep from x = 0: colour 0 cells x = b / diag,
-// colour 1 cells x = 0 (gsColor(0) on a zero iterate, pressure_mg.cpp:276,282)
-template <class CT>
-__global__ void __launch_bounds__(TB) k_vc_smooth0(const DSys* __restrict__ sys, const int2* __restrict__ tab) {
+// r -= alpha q (gasphase.cpp:613; skipped on the first V-cycle) with the
+// partials |r|^2 (slot 1) and max|r| (slot 2), then the first pre-smoothing
+// half sweep from z = 0: red cells z = r / diag, black cells z = 0
+// (gsColor(0) on a zero iterate, pressure_mg.cpp:276,282)
+template <class CT, class ZT>
+__global__ void __launch_bounds__(TB) k_pre_red(const DSys* __restrict__ sys, const int2* __restrict__ tab, int first) {
     PHG_SYS_PROLOGUE
     const DLevel& L = s.L[0];
-    const Cell q = tile_cell(e.y, L.nx, L.ny);
-    if (q.c >= L.n) return;
-    float z = 0.0f;
-    if (((q.i + q.j + q.k) & 1) == 0) {
-        float a[6];
-        coefs6<CT>(L.axc, L.ayc, L.azc, q, L.nx, L.ny, a);
-        const float dg = diag6(a);
-        if (dg > 0) z = s.r[q.c] / dg;
+    const ColTile t = col_tile(s, e.y);
+    const int sxy = L.nx * L.ny;
+    const double alpha = s.scal[S_ALPHA];
+    const float zs = (float)s.scal[S_ZS];
+    double v[1] = {0.0};
+    float m = 0.0f;
+    if (t.valid) {
+        for (int k = t.k0; k < t.k1; ++k) {
+            const int c = t.c2 + sxy * k;
+            float r = s.r[c];
+            if (!first) {
+                r -= (float)(alpha * s.q[c]);
+                s.r[c] = r;
+                v[0] += (double)r * r;
+                m = fmaxf(m, fabsf(r));
+            }
+            float z = 0.0f;
+            if (((t.i + t.j + k) & 1) == 0) {
+                if constexpr (std::is_same<CT, Mask01>::value) {
+                    const int dg = __popc((unsigned)L.mask[c]);
+                    if (dg > 0) z = (r * zs) / (float)dg;
+                } else {
+                    z = (r * zs) * s.invd[c];
+                }
+            }
+            st<ZT>(s.z, c, z);
+        }
     }
-    s.z[q.c] = z;
+    if (first) return;
+    double* out = s.part + (long)e.y * NPART;
+    __shared__ double sh[TB / 64];
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    v[0] = wave_sum(v[0]);
+    if (lane == 0) sh[w] = v[0];
+    __syncthreads();
+    m = block_max(m);
+    if (threadIdx.x == 0) {
+        double acc = 0.0;
+#pragma unroll
+        for (int ww = 0; ww < TB / 64; ++ww) acc += sh[ww];
+        out[1] = acc;
+        out[2] = m;
+    }
 }
 
-// red-black Gauss-Seidel half sweep on cells of one colour (gsColor, pressure_mg.cpp:195-216)
-template <class CT>
-__global__ void __launch_bounds__(TB) k_vc_rb(const DSys* __restrict__ sys, const int2* __restrict__ tab, int color) {
+// black half sweep z = (r + sum a z_nb) / diag on cells with i+j+k odd (gsColor(1))
+template <class CT, class ZT>
+__global__ void __launch_bounds__(TB) k_rb_black(const DSys* __restrict__ sys, const int2* __restrict__ tab) {
     PHG_SYS_PROLOGUE
     const DLevel& L = s.L[0];
-    const Cell q = tile_cell(e.y, L.nx, L.ny);
-    if (q.c >= L.n || ((q.i + q.j + q.k + color) & 1) != 0) return;
-    float a[6];
-    coefs6<CT>(L.axc, L.ayc, L.azc, q, L.nx, L.ny, a);
-    const float dg = diag6(a);
-    float z = 0.0f;
-    if (dg > 0) z = (s.r[q.c] + nb_sum<float>(s.z, q, L.nx, L.ny, L.nz, a)) / dg;
-    s.z[q.c] = z;
+    const ColTile t = col_tile(s, e.y);
+    if (!t.valid) return;
+    const int sxy = L.nx * L.ny;
+    const void* z = s.z;
+    const float zs = (float)s.scal[S_ZS];
+    for (int k = t.k0 + ((t.i + t.j + t.k0 + 1) & 1); k < t.k1; k += 2) {
+        const Cell q{t.c2 + sxy * k, t.i, t.j, k};
+        float a[6];
+        coefs6<CT>(L, L.axc, L.ayc, L.azc, q, a);
+        const float dg = diag6(a);
+        float zc = 0.0f;
+        if (dg > 0) zc = (s.r[q.c] * zs + nb_sum_f(q, L.nx, L.ny, L.nz, a, [&](int n) { return ld<ZT>(z, n); })) / dg;
+        st<ZT>(s.z, q.c, zc);
+    }
 }
 
-// level-0 residual r0 = b - A z into the V-cycle scratch (residualGrid, pressure_mg.cpp:217-235)
+// level-0 residual r0 = r - A z into the V-cycle scratch (residualGrid, pressure_mg.cpp:217-235)
 template <class CT, class VT>
-__global__ void __launch_bounds__(TB) k_vc_resid0(const DSys* __restrict__ sys, const int2* __restrict__ tab) {
+__global__ void __launch_bounds__(TB) k_resid0(const DSys* __restrict__ sys, const int2* __restrict__ tab) {
     PHG_SYS_PROLOGUE
     const DLevel& L = s.L[0];
-    const Cell q = tile_cell(e.y, L.nx, L.ny);
-    if (q.c >= L.n) return;
-    float a[6];
-    coefs6<CT>(L.axc, L.ayc, L.azc, q, L.nx, L.ny, a);
-    const float dg = diag6(a);
-    float r = 0.0f;
-    if (dg > 0) r = s.r[q.c] - dg * s.z[q.c] + nb_sum<float>(s.z, q, L.nx, L.ny, L.nz, a);
-    st<VT>(L.r, q.c, r);
+    const ColTile t = col_tile(s, e.y);
+    if (!t.valid) return;
+    const int sxy = L.nx * L.ny;
+    const void* z = s.z;
+    const float zs = (float)s.scal[S_ZS];
+    for (int k = t.k0; k < t.k1; ++k) {
+        const Cell q{t.c2 + sxy * k, t.i, t.j, k};
+        float a[6];
+        coefs6<CT>(L, L.axc, L.ayc, L.azc, q, a);
+        const float dg = diag6(a);
+        float r = 0.0f;
+        if (dg > 0)
+            r = s.r[q.c] * zs - dg * ld<VT>(z, q.c) + nb_sum_f(q, L.nx, L.ny, L.nz, a, [&](int n) { return ld<VT>(z, n); });
+        st<VT>(L.r, q.c, r);
+    }
 }
 
-// prolongation into level 0 and nothing else: z += t1[agg] (pressure_mg.cpp:293-295)
-template <class VT>
-__global__ void __launch_bounds__(TB) k_vc_prolong0(const DSys* __restrict__ sys, const int2* __restrict__ tab) {
+// prolongation fused into the black post-smoothing half sweep: black cells
+// z = (r + sum a (z_nb + t1[agg_nb])) / diag, where the red neighbours' values
+// are formed as prolonged on the fly (pressure_mg.cpp:293-296).  The black
+// cells' own prolonged values are never read: the sweep replaces them.
+template <class CT, class VT>
+__global__ void __launch_bounds__(TB) k_post_black(const DSys* __restrict__ sys, const int2* __restrict__ tab) {
     PHG_SYS_PROLOGUE
     const DLevel& L = s.L[0];
-    const int c = e.y * TB + threadIdx.x;
-    if (c >= L.n) return;
-    const int a = L.agg[c];
-    if (a >= 0) s.z[c] += ld<VT>(s.L[1].t, a);
+    const ColTile t = col_tile(s, e.y);
+    if (!t.valid) return;
+    const int sxy = L.nx * L.ny;
+    const void* z = s.z;
+    const void* t1 = s.L[1].t;
+    const int* __restrict__ agg = L.agg;
+    auto zp = [&](int n) {
+        const int g = agg[n];
+        return ld<VT>(z, n) + (g >= 0 ? ld<VT>(t1, g) : 0.0f);
+    };
+    const float zs = (float)s.scal[S_ZS];
+    for (int k = t.k0 + ((t.i + t.j + t.k0 + 1) & 1); k < t.k1; k += 2) {
+        const Cell q{t.c2 + sxy * k, t.i, t.j, k};
+        float a[6];
+        coefs6<CT>(L, L.axc, L.ayc, L.azc, q, a);
+        const float dg = diag6(a);
+        float zc = 0.0f;
+        if (dg > 0) zc = (s.r[q.c] * zs + nb_sum_f(q, L.nx, L.ny, L.nz, a, zp)) / dg;
+        st<VT>(s.z, q.c, zc);
+    }
+}
+
+// red post-smoothing half sweep (gsColor(0)) fused with the partial r.z (slot 3)
+template <class CT, class VT>
+__global__ void __launch_bounds__(TB) k_post_red(const DSys* __restrict__ sys, const int2* __restrict__ tab) {
+    PHG_SYS_PROLOGUE
+    const DLevel& L = s.L[0];
+    const ColTile t = col_tile(s, e.y);
+    const int sxy = L.nx * L.ny;
+    const void* z = s.z;
+    const float zs = (float)s.scal[S_ZS];
+    double v[1] = {0.0};
+    if (t.valid) {
+        for (int k = t.k0; k < t.k1; ++k) {
+            const Cell q{t.c2 + sxy * k, t.i, t.j, k};
+            const float r = s.r[q.c];
+            float zc;
+            if (((t.i + t.j + k) & 1) == 0) {
+                float a[6];
+                coefs6<CT>(L, L.axc, L.ayc, L.azc, q, a);
+                const float dg = diag6(a);
+                zc = 0.0f;
+                if (dg > 0) zc = (r * zs + nb_sum_f(q, L.nx, L.ny, L.nz, a, [&](int n) { return ld<VT>(z, n); })) / dg;
+                st<VT>(s.z, q.c, zc);
+            } else {
+                zc = ld<VT>(z, q.c);
+            }
+            v[0] += (double)r * zc;
+        }
+    }
+    double* out = s.part + (long)e.y * NPART;
+    __shared__ double sh[TB / 64];
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    v[0] = wave_sum(v[0]);
+    if (lane == 0) sh[w] = v[0];
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double acc = 0.0;
+#pragma unroll
+        for (int ww = 0; ww < TB / 64; ++ww) acc += sh[ww];
+        out[3] = acc;
+    }
+}
+
+// z (V-cycle type) -> FP32 copy, for phg_precondition
+template <class VT>
+__global__ void __launch_bounds__(TB) k_z_out(const DSys* __restrict__ sys, const int2* __restrict__ tab, float* out) {
+    const int2 e = tab[blockIdx.x];
+    const DSys& s = sys[e.x];
+    const ColTile t = col_tile(s, e.y);
+    if (!t.valid) return;
+    const int sxy = s.L[0].nx * s.L[0].ny;
+    const float izs = (float)(1.0 / s.scal[S_ZS]);
+    for (int k = t.k0; k < t.k1; ++k) out[t.c2 + sxy * k] = ld<VT>(s.z, t.c2 + sxy * k) * izs;
 }
 
 // ---------------------------------------------------------------------------
@@ -492,4 +645,84 @@ __global__ void __launch_bounds__(TB) k_vc_coarse(const DSys* __restrict__ sys, 
     }
 }
 
+// ---------------------------------------------------------------------------
+// closed-zone mean removal for every system of a batch (gasphase.cpp:628-639):
+// fixed-order FP64 sums over chunks of each zone's cell list, then per zone
+// ---------------------------------------------------------------------------
+__global__ void __launch_bounds__(TB) k_zone_part(const DSys* __restrict__ sys, const int2* __restrict__ tab) {
+    const int2 e = tab[blockIdx.x];
+    const DSys& s = sys[e.x];
+    const int4 tk = s.ztask[e.y];
+    const float* x = s.x;
+    double v[1] = {0.0};
+    for (int p = tk.y + threadIdx.x; p < tk.z; p += TB) v[0] += (double)x[s.zcells[p]];
+    __shared__ double out[1];
+    block_sums<1>(v, out);
+    __syncthreads();
+    if (threadIdx.x == 0) s.zpart[e.y] = out[0];
+}
+
+__global__ void __launch_bounds__(TB) k_zone_fin(const DSys* __restrict__ sys, const int2* __restrict__ tab) {
+    const DSys& s = sys[tab[blockIdx.x].x];
+    for (int z = threadIdx.x; z < s.nclosed; z += TB) {
+        double acc = 0.0;
+        int cnt = 0;
+        for (int t = s.ztptr[z]; t < s.ztptr[z + 1]; ++t) {
+            acc += s.zpart[t];
+            cnt += s.ztask[t].z - s.ztask[t].y;
+        }
+        s.zmean[z] = cnt > 0 ? acc / (double)cnt : 0.0;
+    }
+}
+
+__global__ void __launch_bounds__(TB) k_zone_sub(const DSys* __restrict__ sys, const int2* __restrict__ tab) {
+    const int2 e = tab[blockIdx.x];
+    const DSys& s = sys[e.x];
+    if (s.nclosed == 0) return;
+    const ColTile t = col_tile(s, e.y);
+    if (!t.valid) return;
+    const int sxy = s.L[0].nx * s.L[0].ny;
+    for (int k = t.k0; k < t.k1; ++k) {
+        const int c = t.c2 + sxy * k;
+        const int z = s.zclosed[c];
+        if (z >= 0) s.x[c] -= (float)s.zmean[z];
+    }
+}
+
+}  // namespace phg
+
+namespace phg {
+// y = A x on level 0 for phg_apply (FP32 operator), partial x.y in slot 0
+template <class OT>
+__global__ void __launch_bounds__(TB) k_apply_plain(const DSys* __restrict__ sys, const int2* __restrict__ tab,
+                                                    const float* __restrict__ x, float* __restrict__ y) {
+    const int2 e = tab[blockIdx.x];
+    const DSys& s = sys[e.x];
+    const DLevel& L = s.L[0];
+    const ColTile t = col_tile(s, e.y);
+    const int sxy = L.nx * L.ny;
+    double v[1] = {0.0};
+    if (t.valid) {
+        for (int k = t.k0; k < t.k1; ++k) {
+            const Cell q{t.c2 + sxy * k, t.i, t.j, k};
+            float a[6];
+            coefs6<OT>(L, L.ax32, L.ay32, L.az32, q, a);
+            const float dg = diag6(a);
+            float yc = 0.0f;
+            if (dg > 0) yc = dg * x[q.c] - nb_sum_f(q, L.nx, L.ny, L.nz, a, [&](int n) { return x[n]; });
+            y[q.c] = yc;
+            v[0] += (double)yc * x[q.c];
+        }
+    }
+    block_sums<1>(v, s.part + (long)e.y * NPART);
+}
+
+// plain sum of partial slot 0 into scal[S_DQ] (no state change)
+__global__ void __launch_bounds__(TB) k_fin_dot(const DSys* __restrict__ sys, const int2* __restrict__ tab) {
+    const DSys& s = sys[tab[blockIdx.x].x];
+    __shared__ double res[1];
+    sum_parts(s, 0, &res[0]);
+    __syncthreads();
+    if (threadIdx.x == 0) s.scal[S_DQ] = res[0];
+}
 }  // namespace phg

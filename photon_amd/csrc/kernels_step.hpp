@@ -53,27 +53,6 @@ __global__ void __launch_bounds__(TB) k_rhs(DGrid g, float dx, float inv, const 
     if (diag) diag[c] = co[0] + co[1] + co[2] + co[3] + co[4] + co[5];
 }
 
-// closed-zone means of p (FP64, fixed order): one block per closed zone
-__global__ void __launch_bounds__(TB) k_zone_sum(const int* __restrict__ zptr, const int* __restrict__ zcells,
-                                                 const float* __restrict__ p, double* __restrict__ mean) {
-    const int z = blockIdx.x;
-    const int b = zptr[z], e = zptr[z + 1];
-    double v[1] = {0.0};
-    for (int q = b + threadIdx.x; q < e; q += TB) v[0] += (double)p[zcells[q]];
-    __shared__ double out[1];
-    block_sums<1>(v, out);
-    __syncthreads();
-    if (threadIdx.x == 0) mean[z] = e > b ? out[0] / (double)(e - b) : 0.0;
-}
-
-__global__ void __launch_bounds__(TB) k_zone_sub(int n, const int* __restrict__ zclosed, const double* __restrict__ mean,
-                                                 float* __restrict__ p) {
-    const int c = blockIdx.x * TB + threadIdx.x;
-    if (c >= n) return;
-    const int z = zclosed[c];
-    if (z >= 0) p[c] -= (float)mean[z];
-}
-
 __global__ void k_fill_f32(float* p, int n, float v) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i < n) p[i] = v;
@@ -280,4 +259,11 @@ __global__ void __launch_bounds__(TB) k_advect(DGrid g, float dx, float dt, cons
     }
 }
 
+}  // namespace phg
+
+namespace phg {
+// plain 16-byte-per-lane copy (calibration of the HBM byte counters)
+__global__ void __launch_bounds__(TB) k_copy16(const float4* __restrict__ src, float4* __restrict__ dst, long n) {
+    for (long i = (long)blockIdx.x * TB + threadIdx.x; i < n; i += (long)gridDim.x * TB) dst[i] = src[i];
+}
 }  // namespace phg

@@ -25,13 +25,14 @@ namespace phg {
 // profiling: HIP events around each launch, accumulated per kernel class
 // ---------------------------------------------------------------------------
 enum KClass {
-    K_RESID_INIT, K_APPLY, K_AXPY, K_DOT_RZ, K_UPDATE_D, K_FIN, K_VC_SMOOTH0, K_VC_RB, K_VC_RESID0, K_VC_RESTRICT,
-    K_VC_JAC, K_VC_RESID, K_VC_PROLONG, K_VC_COARSE, K_RHS, K_ZONE, K_CORRECT, K_ENERGY, K_ADVECT, K_COUNT, K_NCLASS
+    K_RESID_INIT, K_DIR_APPLY, K_PRE_RED, K_RB_BLACK, K_RESID0, K_POST_BLACK, K_POST_RED, K_FIN, K_VC_RESTRICT,
+    K_VC_JAC, K_VC_RESID, K_VC_PROLONG, K_VC_COARSE, K_XFLUSH, K_ZONE, K_RHS, K_CORRECT, K_ENERGY, K_ADVECT,
+    K_COUNT, K_NCLASS
 };
-static const char* kKName[K_NCLASS] = {"resid_init", "apply",      "axpy",       "dot_rz",     "update_d",
-                                       "finalize",   "vc_smooth0", "vc_rb",      "vc_resid0",  "vc_restrict",
-                                       "vc_jacobi",  "vc_resid",   "vc_prolong", "vc_coarse",  "rhs",
-                                       "zone_mean",  "correct",    "energy",     "advect",     "count_active"};
+static const char* kKName[K_NCLASS] = {"resid_init",  "dir_apply",  "pre_red",    "rb_black",  "resid0",
+                                       "post_black",  "post_red",   "finalize",   "vc_restrict", "vc_jacobi",
+                                       "vc_resid",    "vc_prolong", "vc_coarse",  "x_flush",   "zone_mean",
+                                       "rhs",         "correct",    "energy",     "advect",    "count_active"};
 
 struct GridDev {
     int nx = 0, ny = 0, nz = 0, n = 0;
@@ -54,9 +55,12 @@ struct GridDev {
     };
     std::vector<Lev> lv;
     double* cinv = nullptr;
-    int nclosed = 0;
-    int *zptr = nullptr, *zcells = nullptr, *zclosed = nullptr;
-    double* zmean = nullptr;
+    float* invd = nullptr;        // 1/diag of level 0, 0 where diag <= 0
+    uint8_t* mask = nullptr;      // level-0 face mask when every coefficient is 0 or 1
+    bool binary = false;
+    int nclosed = 0, nztask = 0;
+    int *zcells = nullptr, *zclosed = nullptr, *ztptr = nullptr;
+    int4* ztask = nullptr;
     float* scratch_b = nullptr;   // RHS of phg_project
     float* vmax = nullptr;        // one float for max reductions
     std::vector<void*> allocs;
@@ -80,12 +84,15 @@ struct GridDev {
 
 struct BatchDev {
     int nsys = 0, prec = 0, maxl = 0, maxnc = 0;
+    bool binary = false;           // every grid has 0/1 coefficients: level-0 kernels read the face mask
     std::vector<int> grids;
     std::vector<DSys> h_sys;
     DSys* d_sys = nullptr;
     std::vector<int2*> tab;        // per level: (system, tile) of systems having that level
     std::vector<int> tab_len;
     int2* tab_sys = nullptr;       // one entry per system
+    int2* tab_z = nullptr;         // (system, zone task) for the closed-zone means
+    int tab_z_len = 0;
     int4* tab_coarse = nullptr;    // (first system, count, row tile): coarsest-solve blocks
     int tab_coarse_len = 0;
     int cg_sys = 1;                // systems per coarsest-solve block (64 KB of LDS)
@@ -255,50 +262,80 @@ int tiles(int n) { return (n + TB - 1) / TB; }
 // stopping size, so the hierarchy keeps the reference's levels whenever it
 // coarsens at all; the LDS of the coarsest-solve block holds CG_SYS vectors
 constexpr int kDenseMax = 2048;
+// cells per block of the closed-zone sums
+constexpr int kZoneChunk = 8192;
+// level-0 blocks a launch should offer the chip (256 CUs)
+constexpr int kLevel0Blocks = 4096;
 
 // ---------------------------------------------------------------------------
 // the V-cycle and one CG iteration as launch sequences
 // ---------------------------------------------------------------------------
-template <class CT, class VT>
-void vcycle_t(phg_ctx* ctx, BatchDev* B) {
+// C0: level-0 coefficient type of the V-cycle (FP32/FP16 faces or the 0/1
+// mask), CC: coarse operator storage, VT: V-cycle vector storage
+template <class C0, class CC, class VT>
+void vcycle_t(phg_ctx* ctx, BatchDev* B, int first) {
     const DSys* S = B->d_sys;
-    launch(ctx, K_VC_SMOOTH0, k_vc_smooth0<CT>, B->tab_len[0], 0, S, B->tab[0]);
-    launch(ctx, K_VC_RB, k_vc_rb<CT>, B->tab_len[0], 0, S, B->tab[0], 1);
-    launch(ctx, K_VC_RESID0, k_vc_resid0<CT, VT>, B->tab_len[0], 0, S, B->tab[0]);
+    const int2* T0 = B->tab[0];
+    const int n0 = B->tab_len[0];
+    launch(ctx, K_PRE_RED, k_pre_red<C0, VT>, n0, 0, S, T0, first);
+    launch(ctx, K_RB_BLACK, k_rb_black<C0, VT>, n0, 0, S, T0);
+    launch(ctx, K_RESID0, k_resid0<C0, VT>, n0, 0, S, T0);
     launch(ctx, K_VC_RESTRICT, k_vc_restrict<VT>, B->tab_len[1], 0, S, B->tab[1], 0);
     for (int l = 1; l + 1 < B->maxl; ++l) {
-        launch(ctx, K_VC_JAC, k_vc_jac0<CT, VT>, B->tab_len[l], 0, S, B->tab[l], l);
-        launch(ctx, K_VC_JAC, k_vc_jac<CT, VT>, B->tab_len[l], 0, S, B->tab[l], l, 1);
-        launch(ctx, K_VC_RESID, k_vc_resid<CT, VT>, B->tab_len[l], 0, S, B->tab[l], l);
+        launch(ctx, K_VC_JAC, k_vc_jac0<CC, VT>, B->tab_len[l], 0, S, B->tab[l], l);
+        launch(ctx, K_VC_JAC, k_vc_jac<CC, VT>, B->tab_len[l], 0, S, B->tab[l], l, 1);
+        launch(ctx, K_VC_RESID, k_vc_resid<CC, VT>, B->tab_len[l], 0, S, B->tab[l], l);
         launch(ctx, K_VC_RESTRICT, k_vc_restrict<VT>, B->tab_len[l + 1], 0, S, B->tab[l + 1], l);
     }
     launch(ctx, K_VC_COARSE, k_vc_coarse<VT>, B->tab_coarse_len, (size_t)B->cg_sys * B->maxnc * sizeof(double), S,
            (const int4*)B->tab_coarse);
     for (int l = B->maxl - 2; l >= 1; --l) {
         launch(ctx, K_VC_PROLONG, k_vc_prolong<VT>, B->tab_len[l], 0, S, B->tab[l], l);
-        launch(ctx, K_VC_JAC, k_vc_jac<CT, VT>, B->tab_len[l], 0, S, B->tab[l], l, 0);
-        launch(ctx, K_VC_JAC, k_vc_jac<CT, VT>, B->tab_len[l], 0, S, B->tab[l], l, 1);
+        launch(ctx, K_VC_JAC, k_vc_jac<CC, VT>, B->tab_len[l], 0, S, B->tab[l], l, 0);
+        launch(ctx, K_VC_JAC, k_vc_jac<CC, VT>, B->tab_len[l], 0, S, B->tab[l], l, 1);
     }
-    launch(ctx, K_VC_PROLONG, k_vc_prolong0<VT>, B->tab_len[0], 0, S, B->tab[0]);
-    launch(ctx, K_VC_RB, k_vc_rb<CT>, B->tab_len[0], 0, S, B->tab[0], 1);
-    launch(ctx, K_VC_RB, k_vc_rb<CT>, B->tab_len[0], 0, S, B->tab[0], 0);
+    launch(ctx, K_POST_BLACK, k_post_black<C0, VT>, n0, 0, S, T0);
+    launch(ctx, K_POST_RED, k_post_red<C0, VT>, n0, 0, S, T0);
 }
 
-void vcycle(phg_ctx* ctx, BatchDev* B) {
-    if (B->prec == PHG_PREC_FP32) vcycle_t<float, float>(ctx, B);
-    else if (B->prec == PHG_PREC_MIXED_F16) vcycle_t<_Float16, _Float16>(ctx, B);
-    else vcycle_t<_Float16, bf16_t>(ctx, B);
+void vcycle(phg_ctx* ctx, BatchDev* B, int first) {
+    if (B->prec == PHG_PREC_FP32) {
+        if (B->binary) vcycle_t<Mask01, float, float>(ctx, B, first);
+        else vcycle_t<float, float, float>(ctx, B, first);
+    } else if (B->prec == PHG_PREC_MIXED_F16) {
+        if (B->binary) vcycle_t<Mask01, _Float16, _Float16>(ctx, B, first);
+        else vcycle_t<_Float16, _Float16, _Float16>(ctx, B, first);
+    } else {
+        if (B->binary) vcycle_t<Mask01, _Float16, bf16_t>(ctx, B, first);
+        else vcycle_t<_Float16, _Float16, bf16_t>(ctx, B, first);
+    }
+}
+
+template <class OT>
+void dir_apply(phg_ctx* ctx, BatchDev* B) {
+    const DSys* S = B->d_sys;
+    if (B->prec == PHG_PREC_FP32) launch(ctx, K_DIR_APPLY, k_dir_apply<OT, float>, B->tab_len[0], 0, S, B->tab[0]);
+    else if (B->prec == PHG_PREC_MIXED_F16)
+        launch(ctx, K_DIR_APPLY, k_dir_apply<OT, _Float16>, B->tab_len[0], 0, S, B->tab[0]);
+    else launch(ctx, K_DIR_APPLY, k_dir_apply<OT, bf16_t>, B->tab_len[0], 0, S, B->tab[0]);
 }
 
 void cg_iteration(phg_ctx* ctx, BatchDev* B) {
     const DSys* S = B->d_sys;
-    launch(ctx, K_APPLY, k_apply, B->tab_len[0], 0, S, B->tab[0]);
+    if (B->binary) dir_apply<Mask01>(ctx, B);
+    else dir_apply<float>(ctx, B);
     launch(ctx, K_FIN, k_fin_alpha, B->nsys, 0, S, B->tab_sys);
-    launch(ctx, K_AXPY, k_axpy, B->tab_len[0], 0, S, B->tab[0]);
-    vcycle(ctx, B);
-    launch(ctx, K_DOT_RZ, k_dot_rz, B->tab_len[0], 0, S, B->tab[0]);
+    vcycle(ctx, B, 0);
     launch(ctx, K_FIN, k_fin_beta, B->nsys, 0, S, B->tab_sys);
-    launch(ctx, K_UPDATE_D, k_update_d, B->tab_len[0], 0, S, B->tab[0], 0);
+}
+
+// closed-zone mean removal of every system's x (whatever its state)
+void demean_batch(phg_ctx* ctx, BatchDev* B) {
+    if (B->tab_z_len == 0) return;
+    const DSys* S = B->d_sys;
+    launch(ctx, K_ZONE, k_zone_part, B->tab_z_len, 0, S, (const int2*)B->tab_z);
+    launch(ctx, K_ZONE, k_zone_fin, B->nsys, 0, S, (const int2*)B->tab_sys);
+    launch(ctx, K_ZONE, k_zone_sub, B->tab_len[0], 0, S, (const int2*)B->tab[0]);
 }
 
 int upload_sys(phg_ctx* ctx, BatchDev* B) {
@@ -316,6 +353,7 @@ int reset_state(phg_ctx* ctx, BatchDev* B, const phg_solve_params* P) {
         sc[S_TOL] = P ? P->tol : 0.0;
         sc[S_INV] = P ? P->inv : 1.0f;
         sc[S_RZ] = 1.0;
+        sc[S_ZS] = sc[S_ZSN] = 1.0;
         st[ST_ACTIVE] = 1;
         st[ST_MAXIT] = P ? P->max_iters : 0;
         st[ST_RULE] = P && P->stop_rule == PHG_STOP_MAXNORM ? 1 : 0;
@@ -325,13 +363,6 @@ int reset_state(phg_ctx* ctx, BatchDev* B, const phg_solve_params* P) {
     HIP_OK(hipMemcpyAsync(B->state, state.data(), state.size() * sizeof(int), hipMemcpyHostToDevice, ctx->stream));
     // pageable sources: wait so the host vectors may go out of scope
     HIP_OK(hipStreamSynchronize(ctx->stream));
-    return 0;
-}
-
-int demean(phg_ctx* ctx, GridDev* G, float* x) {
-    if (G->nclosed == 0) return 0;
-    launch(ctx, K_ZONE, k_zone_sum, G->nclosed, 0, G->zptr, G->zcells, (const float*)x, G->zmean);
-    launch(ctx, K_ZONE, k_zone_sub, tiles(G->n), 0, G->n, (const int*)G->zclosed, (const double*)G->zmean, x);
     return 0;
 }
 
@@ -521,7 +552,35 @@ int phg_grid_create(phg_ctx* ctx, int nx, int ny, int nz, float dx, const float*
         }
     }
     UP(G->cinv, G->H.cinv.data(), G->H.cinv.size())
-    // closed zones: cell lists for the deterministic mean removal
+    {
+        std::vector<float> invd(G->n, 0.0f);
+        for (int c = 0; c < G->n; ++c)
+            if (L0.diag[c] > 0) invd[c] = 1.0f / L0.diag[c];
+        UP(G->invd, invd.data(), invd.size())
+        // face mask: exact when every level-0 coefficient is 0 or 1
+        bool binary = true;
+        for (const std::vector<float>* a : {&L0.ax, &L0.ay, &L0.az})
+            for (float v : *a)
+                if (v != 0.0f && v != 1.0f) binary = false;
+        if (binary) {
+            std::vector<uint8_t> m(G->n, 0);
+            for (int k = 0; k < nz; ++k)
+                for (int j = 0; j < ny; ++j)
+                    for (int i = 0; i < nx; ++i) {
+                        const int c = i + nx * (j + ny * k);
+                        const int xf = i + (nx + 1) * (j + ny * k), yf = i + nx * (j + (ny + 1) * k);
+                        const float a[6] = {L0.ax[xf], L0.ax[xf + 1], L0.ay[yf], L0.ay[yf + nx], L0.az[c],
+                                            L0.az[c + nx * ny]};
+                        uint8_t b = 0;
+                        for (int d = 0; d < 6; ++d)
+                            if (a[d] > 0) b |= (uint8_t)(1u << d);
+                        m[c] = b;
+                    }
+            UP(G->mask, m.data(), m.size())
+            G->binary = true;
+        }
+    }
+    // closed zones: cell lists, cut into chunks, for the deterministic mean removal
     {
         std::vector<int> zid(G->zone_open.size(), -1);
         int nclosed = 0;
@@ -538,10 +597,20 @@ int phg_grid_create(phg_ctx* ctx, int nx, int ny, int nz, float dx, const float*
         for (int c = 0; c < G->n; ++c)
             if (zclosed[c] >= 0) zcells[fill[zclosed[c]]++] = c;
         G->nclosed = nclosed;
-        UP(G->zptr, zptr.data(), zptr.size())
+        std::vector<int4> tasks;
+        std::vector<int> ztptr(nclosed + 1, 0);
+        for (int z = 0; z < nclosed; ++z) {
+            ztptr[z] = (int)tasks.size();
+            for (int b = zptr[z]; b < zptr[z + 1]; b += kZoneChunk)
+                tasks.push_back(make_int4(z, b, std::min(zptr[z + 1], b + kZoneChunk), 0));
+        }
+        ztptr[nclosed] = (int)tasks.size();
+        G->nztask = (int)tasks.size();
+        if (tasks.empty()) tasks.push_back(make_int4(0, 0, 0, 0));
+        UP(G->ztask, tasks.data(), tasks.size())
+        UP(G->ztptr, ztptr.data(), ztptr.size())
         UP(G->zcells, zcells.data(), zcells.size())
         UP(G->zclosed, zclosed.data(), zclosed.size())
-        if (!rc) rc = dalloc(ctx, G->allocs, &G->zmean, (size_t)std::max(nclosed, 1));
     }
     if (!rc) rc = dalloc(ctx, G->allocs, &G->scratch_b, (size_t)G->n);
     if (!rc) rc = dalloc(ctx, G->allocs, &G->vmax, 1);
@@ -601,13 +670,19 @@ int phg_hierarchy_shape(int nx, int ny, int nz, const float* ax, const float* ay
 
 // ---------------------------------------------------------------------------
 int phg_batch_create(phg_ctx* ctx, int nsys, const int* grids, int* batch_out) {
+    return phg_batch_create_ex(ctx, nsys, grids, nullptr, batch_out);
+}
+
+int phg_batch_create_ex(phg_ctx* ctx, int nsys, const int* grids, const phg_batch_opts* opts, int* batch_out) {
     if (!ctx || nsys <= 0 || !grids || !batch_out) return PHG_EINVAL;
+    if (opts && (opts->kchunk < 0 || opts->level0_blocks < 0)) return fail(ctx, PHG_EINVAL, "bad batch options");
     HIP_OK(hipSetDevice(ctx->device));
     BatchDev* B = new BatchDev();
     B->nsys = nsys;
     B->grids.assign(grids, grids + nsys);
     B->h_sys.resize(nsys);
     int rc = 0;
+    long col_tiles = 0;
     for (int s = 0; s < nsys && !rc; ++s) {
         GridDev* G = get_grid(ctx, grids[s]);
         if (!G) {
@@ -621,11 +696,17 @@ int phg_batch_create(phg_ctx* ctx, int nsys, const int* grids, int* batch_out) {
         }
         B->maxl = std::max(B->maxl, (int)G->lv.size());
         B->maxnc = std::max(B->maxnc, G->H.nc);
+        B->binary = s == 0 ? G->binary : (B->binary && G->binary);
+        col_tiles += tiles(G->nx * G->ny);
     }
     if (rc) {
         delete B;
         return rc;
     }
+    // planes per level-0 block: split the columns' k range until the launch
+    // offers about kLevel0Blocks blocks
+    const long want = opts && opts->level0_blocks > 0 ? opts->level0_blocks : kLevel0Blocks;
+    const int nkc_target = (int)std::max(1L, (want + col_tiles - 1) / col_tiles);
     const size_t vts = vt_size(B->prec);
     if (!rc) rc = dalloc(ctx, B->allocs, &B->scal, (size_t)nsys * S_NSCAL);
     if (!rc) rc = dalloc(ctx, B->allocs, &B->state, (size_t)nsys * ST_NSTATE);
@@ -638,27 +719,39 @@ int phg_batch_create(phg_ctx* ctx, int nsys, const int* grids, int* batch_out) {
         std::memset(&S, 0, sizeof(DSys));
         S.nl = (int)G->lv.size();
         S.nc = G->H.nc;
-        S.tiles0 = tiles(G->n);
-        S.grid = grids[s];
+        S.kchunk = opts && opts->kchunk > 0 ? std::min(opts->kchunk, G->nz)
+                                            : std::max(1, (G->nz + nkc_target - 1) / nkc_target);
+        S.tiles0 = tiles(G->nx * G->ny) * ((G->nz + S.kchunk - 1) / S.kchunk);
         S.cinv = G->cinv;
+        S.invd = G->invd;
         S.scal = B->scal + (size_t)s * S_NSCAL;
         S.state = B->state + (size_t)s * ST_NSTATE;
+        S.nclosed = G->nclosed;
+        S.nztask = G->nztask;
+        S.ztask = G->ztask;
+        S.ztptr = G->ztptr;
+        S.zcells = G->zcells;
+        S.zclosed = G->zclosed;
         const size_t n0 = G->n;
         float* v4 = nullptr;
         rc = dalloc(ctx, B->allocs, &v4, 4 * n0);
         if (rc) break;
         S.r = v4;
-        S.d = v4 + n0;
-        S.q = v4 + 2 * n0;
-        S.z = v4 + 3 * n0;
+        S.q = v4 + n0;
+        S.d0 = v4 + 2 * n0;
+        S.d1 = v4 + 3 * n0;
         rc = dalloc(ctx, B->allocs, &S.part, (size_t)S.tiles0 * NPART);
+        if (!rc) rc = dalloc(ctx, B->allocs, &S.zpart, (size_t)std::max(G->nztask, 1));
+        if (!rc) rc = dalloc(ctx, B->allocs, &S.zmean, (size_t)std::max(G->nclosed, 1));
         if (rc) break;
-        // V-cycle work vectors: level 0 residual scratch, coarse x b r t
-        size_t total = n0;
+        // V-cycle vectors: level-0 iterate z and residual scratch, coarse x b r t
+        size_t total = 2 * n0;
         for (int l = 1; l < S.nl; ++l) total += 4 * (size_t)G->lv[l].n;
         char* slab = nullptr;
         rc = dalloc(ctx, B->allocs, &slab, total * vts);
         if (rc) break;
+        S.z = slab;
+        slab += n0 * vts;
         for (int l = 0; l < S.nl; ++l) {
             DLevel& D = S.L[l];
             const GridDev::Lev& GL = G->lv[l];
@@ -680,6 +773,7 @@ int phg_batch_create(phg_ctx* ctx, int nsys, const int* grids, int* batch_out) {
                 D.axc = G->axc;
                 D.ayc = G->ayc;
                 D.azc = G->azc;
+                D.mask = G->mask;
                 D.r = slab;
                 slab += n0 * vts;
             } else {
@@ -695,7 +789,7 @@ int phg_batch_create(phg_ctx* ctx, int nsys, const int* grids, int* batch_out) {
         }
     }
     if (!rc) rc = dalloc(ctx, B->allocs, &B->d_sys, (size_t)nsys);
-    // tile tables
+    // tile tables: level 0 by column tiles x k-chunks, coarse levels by 256 unknowns
     B->tab.assign(std::max(B->maxl, 1), nullptr);
     B->tab_len.assign(std::max(B->maxl, 1), 0);
     for (int l = 0; l < B->maxl && !rc; ++l) {
@@ -703,7 +797,7 @@ int phg_batch_create(phg_ctx* ctx, int nsys, const int* grids, int* batch_out) {
         for (int s = 0; s < nsys; ++s) {
             GridDev* G = get_grid(ctx, grids[s]);
             if ((int)G->lv.size() <= l) continue;
-            const int t = tiles(G->lv[l].n);
+            const int t = l == 0 ? B->h_sys[s].tiles0 : tiles(G->lv[l].n);
             for (int k = 0; k < t; ++k) logical.push_back(make_int2(s, k));
         }
         std::vector<int2> phys = xcd_order(logical);
@@ -711,12 +805,17 @@ int phg_batch_create(phg_ctx* ctx, int nsys, const int* grids, int* batch_out) {
         rc = upload(ctx, B->allocs, &B->tab[l], phys.data(), phys.size());
     }
     if (!rc) {
-        std::vector<int2> ts(nsys);
-        for (int s = 0; s < nsys; ++s) ts[s] = make_int2(s, 0);
+        std::vector<int2> ts(nsys), tz;
+        for (int s = 0; s < nsys; ++s) {
+            ts[s] = make_int2(s, 0);
+            for (int k = 0; k < B->h_sys[s].nztask; ++k) tz.push_back(make_int2(s, k));
+        }
         rc = upload(ctx, B->allocs, &B->tab_sys, ts.data(), ts.size());
+        B->tab_z_len = (int)tz.size();
+        if (!rc && !tz.empty()) rc = upload(ctx, B->allocs, &B->tab_z, tz.data(), tz.size());
     }
     if (!rc) {
-        // coarsest-solve blocks: runs of up to CG_SYS consecutive systems on one grid
+        // coarsest-solve blocks: runs of up to cg_sys consecutive systems on one grid
         std::vector<int4> tc;
         B->cg_sys = std::max(1, std::min(CG_SYS, 65536 / (8 * std::max(B->maxnc, 1))));
         for (int s = 0; s < nsys;) {
@@ -769,12 +868,11 @@ int phg_batch_solve(phg_ctx* ctx, int batch, const float* const* b, float* const
     if (!rc) rc = reset_state(ctx, B, P);
     if (rc) return rc;
     const DSys* S = B->d_sys;
-    launch(ctx, K_RESID_INIT, k_resid_init, B->tab_len[0], 0, S, B->tab[0]);
+    if (B->binary) launch(ctx, K_RESID_INIT, k_resid_init<Mask01>, B->tab_len[0], 0, S, B->tab[0]);
+    else launch(ctx, K_RESID_INIT, k_resid_init<float>, B->tab_len[0], 0, S, B->tab[0]);
     launch(ctx, K_FIN, k_fin_init, B->nsys, 0, S, B->tab_sys);
-    vcycle(ctx, B);
-    launch(ctx, K_DOT_RZ, k_dot_rz, B->tab_len[0], 0, S, B->tab[0]);
+    vcycle(ctx, B, 1);
     launch(ctx, K_FIN, k_fin_rz0, B->nsys, 0, S, B->tab_sys);
-    launch(ctx, K_UPDATE_D, k_update_d, B->tab_len[0], 0, S, B->tab[0], 1);
     const int check = P->check_every > 0 ? P->check_every : 4;
     const bool graph = P->use_graph && !ctx->prof;
     if (graph && !B->graph_exec) {
@@ -796,9 +894,9 @@ int phg_batch_solve(phg_ctx* ctx, int batch, const float* const* b, float* const
         HIP_OK(hipStreamSynchronize(ctx->stream));
         if (*B->h_count == 0) break;
     }
+    launch(ctx, K_XFLUSH, k_x_flush, B->tab_len[0], 0, S, B->tab[0]);
+    if (P->demean) demean_batch(ctx, B);
     HIP_OK(hipGetLastError());
-    if (P->demean)
-        for (int s = 0; s < B->nsys; ++s) demean(ctx, get_grid(ctx, B->grids[s]), x[s]);
     if (stats) {
         std::vector<double> scal((size_t)B->nsys * S_NSCAL);
         std::vector<int> state((size_t)B->nsys * ST_NSTATE);
@@ -832,22 +930,17 @@ int phg_apply(phg_ctx* ctx, int grid, const float* x, float* y, double* dot) {
     BatchDev* B = nullptr;
     int rc = single_batch(ctx, grid, &B);
     if (rc) return rc;
-    DSys saved = B->h_sys[0];
-    B->h_sys[0].d = const_cast<float*>(x);
-    B->h_sys[0].q = y;
-    rc = upload_sys(ctx, B);
-    phg_solve_params P{};
-    if (!rc) rc = reset_state(ctx, B, &P);
-    B->h_sys[0] = saved;
-    if (rc) return rc;
-    launch(ctx, K_APPLY, k_apply, B->tab_len[0], 0, (const DSys*)B->d_sys, B->tab[0]);
-    launch(ctx, K_FIN, k_fin_alpha, 1, 0, (const DSys*)B->d_sys, B->tab_sys);
+    if (B->binary)
+        launch(ctx, K_DIR_APPLY, k_apply_plain<Mask01>, B->tab_len[0], 0, (const DSys*)B->d_sys, (const int2*)B->tab[0], x, y);
+    else
+        launch(ctx, K_DIR_APPLY, k_apply_plain<float>, B->tab_len[0], 0, (const DSys*)B->d_sys, (const int2*)B->tab[0], x, y);
+    launch(ctx, K_FIN, k_fin_dot, 1, 0, (const DSys*)B->d_sys, (const int2*)B->tab_sys);
     double sc[S_NSCAL];
     HIP_OK(hipMemcpyAsync(sc, B->scal, sizeof sc, hipMemcpyDeviceToHost, ctx->stream));
     HIP_OK(hipStreamSynchronize(ctx->stream));
     HIP_OK(hipGetLastError());
     if (dot) *dot = sc[S_DQ];
-    return upload_sys(ctx, B);
+    return PHG_OK;
 }
 
 int phg_precondition(phg_ctx* ctx, int grid, const float* r, float* z) {
@@ -855,15 +948,18 @@ int phg_precondition(phg_ctx* ctx, int grid, const float* r, float* z) {
     BatchDev* B = nullptr;
     int rc = single_batch(ctx, grid, &B);
     if (rc) return rc;
-    DSys saved = B->h_sys[0];
-    B->h_sys[0].r = const_cast<float*>(r);
-    B->h_sys[0].z = z;
+    float* saved = B->h_sys[0].r;
+    B->h_sys[0].r = const_cast<float*>(r);   // read only: the first V-cycle does not update r
     rc = upload_sys(ctx, B);
     phg_solve_params P{};
     if (!rc) rc = reset_state(ctx, B, &P);
-    B->h_sys[0] = saved;
+    B->h_sys[0].r = saved;
     if (rc) return rc;
-    vcycle(ctx, B);
+    vcycle(ctx, B, 1);
+    const DSys* S = B->d_sys;
+    if (B->prec == PHG_PREC_FP32) launch(ctx, K_POST_RED, k_z_out<float>, B->tab_len[0], 0, S, B->tab[0], z);
+    else if (B->prec == PHG_PREC_MIXED_F16) launch(ctx, K_POST_RED, k_z_out<_Float16>, B->tab_len[0], 0, S, B->tab[0], z);
+    else launch(ctx, K_POST_RED, k_z_out<bf16_t>, B->tab_len[0], 0, S, B->tab[0], z);
     HIP_OK(hipStreamSynchronize(ctx->stream));
     HIP_OK(hipGetLastError());
     if (ctx->prof) prof_drain(ctx);
@@ -885,9 +981,16 @@ int phg_rhs(phg_ctx* ctx, int grid, float dt, const float* u, const float* v, co
 
 int phg_zone_mean(phg_ctx* ctx, int grid, float* p) {
     if (!ctx || !p) return PHG_EINVAL;
-    GridDev* G = get_grid(ctx, grid);
-    if (!G) return fail(ctx, PHG_ESTATE, "unknown grid %d", grid);
-    demean(ctx, G, p);
+    BatchDev* B = nullptr;
+    int rc = single_batch(ctx, grid, &B);
+    if (rc) return rc;
+    float* saved = B->h_sys[0].x;
+    B->h_sys[0].x = p;
+    rc = upload_sys(ctx, B);
+    B->h_sys[0].x = saved;
+    if (rc) return rc;
+    demean_batch(ctx, B);
+    HIP_OK(hipStreamSynchronize(ctx->stream));
     HIP_OK(hipGetLastError());
     return PHG_OK;
 }
@@ -953,6 +1056,15 @@ int phg_advect_scalars(phg_ctx* ctx, int grid, float dt, const float* u, const f
         F.out[q] = fields_out[q];
     }
     launch(ctx, K_ADVECT, k_advect, tiles(G->n), 0, G->dg(), G->dx, dt, u, v, w, Pb, nu, F);
+    HIP_OK(hipGetLastError());
+    return PHG_OK;
+}
+
+int phg_copy(phg_ctx* ctx, void* dst, const void* src, size_t bytes) {
+    if (!ctx || !dst || !src || bytes % 16) return ctx ? fail(ctx, PHG_EINVAL, "bytes must be a multiple of 16") : PHG_EINVAL;
+    const long n = (long)(bytes / 16);
+    const int blocks = (int)std::min<long>(std::max<long>(1, (n + TB - 1) / TB), 8192);
+    hipLaunchKernelGGL(k_copy16, dim3(blocks), dim3(TB), 0, ctx->stream, (const float4*)src, (float4*)dst, n);
     HIP_OK(hipGetLastError());
     return PHG_OK;
 }

@@ -28,7 +28,7 @@ EXPORTS = (
     "phg_grid_zones", "phg_apply", "phg_precondition", "phg_batch_create", "phg_batch_destroy",
     "phg_batch_solve", "phg_solve", "phg_project", "phg_rhs", "phg_zone_mean", "phg_correct",
     "phg_energy_expansion", "phg_advect_scalars", "phg_profile_enable", "phg_profile_get", "phg_profile_reset",
-    "phg_hierarchy_shape",
+    "phg_hierarchy_shape", "phg_batch_create_ex", "phg_copy",
 )
 
 
@@ -39,6 +39,10 @@ class PhgError(RuntimeError):
 class SolveParams(C.Structure):
     _fields_ = [("tol", C.c_double), ("max_iters", C.c_int), ("stop_rule", C.c_int), ("inv", C.c_float),
                 ("demean", C.c_int), ("check_every", C.c_int), ("use_graph", C.c_int), ("zero_x", C.c_int)]
+
+
+class BatchOpts(C.Structure):
+    _fields_ = [("kchunk", C.c_int), ("level0_blocks", C.c_int)]
 
 
 class SolveStats(C.Structure):
@@ -79,6 +83,8 @@ def load_library():
     L.phg_apply.argtypes = [vp, i32, vp, vp, C.POINTER(f64)]
     L.phg_precondition.argtypes = [vp, i32, vp, vp]
     L.phg_batch_create.argtypes = [vp, i32, vp, C.POINTER(i32)]
+    L.phg_copy.argtypes = [vp, vp, vp, sz]
+    L.phg_batch_create_ex.argtypes = [vp, i32, vp, C.POINTER(BatchOpts), C.POINTER(i32)]
     L.phg_batch_destroy.argtypes = [vp, i32]
     L.phg_batch_solve.argtypes = [vp, i32, vp, vp, C.POINTER(SolveParams), vp]
     L.phg_solve.argtypes = [vp, i32, vp, vp, C.POINTER(SolveParams), vp]
@@ -251,12 +257,13 @@ class Grid:
 class Batch:
     """A fixed set of systems solved together (phg_batch_create / _solve)."""
 
-    def __init__(self, ctx: Context, grids):
+    def __init__(self, ctx: Context, grids, kchunk=0, level0_blocks=0):
         self.ctx = ctx
         self.grids = list(grids)
         ids = np.array([g.id for g in self.grids], np.int32)
         bid = C.c_int()
-        ctx._check(ctx.L.phg_batch_create(ctx.h, len(ids), ids.ctypes.data, C.byref(bid)))
+        opts = BatchOpts(kchunk, level0_blocks)
+        ctx._check(ctx.L.phg_batch_create_ex(ctx.h, len(ids), ids.ctypes.data, C.byref(opts), C.byref(bid)))
         self.id = bid.value
 
     def solve(self, bs, xs, params: SolveParams):

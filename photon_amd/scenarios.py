@@ -300,3 +300,20 @@ def rhs_from_divergence(grid: Grid, S: np.ndarray, dt: float) -> np.ndarray:
     b = (S.astype(np.float32) * inv).astype(np.float32)
     b[grid.blocked != 0] = 0
     return b
+
+
+def vented(grid: Grid, seed: int, frac: float = 0.15) -> Grid:
+    """The same geometry with a seeded share of its open interior faces made
+    partial openings (open fraction U[0.05, 0.95], as vents and partly open
+    doors are, src/sim/solver.cpp:82-100): a matrix with fractional
+    coefficients."""
+    def perturb(f, salt):
+        u = (splitmix64(seed ^ salt, len(f)) >> np.uint64(11)).astype(np.float64) / 9007199254740992.0
+        v = (splitmix64(seed ^ (salt * 7 + 1), len(f)) >> np.uint64(11)).astype(np.float64) / 9007199254740992.0
+        out = f.copy()
+        sel = (f > 0) & (u < frac)
+        out[sel] = (0.05 + 0.9 * v[sel]).astype(np.float32)
+        return out
+    fx, fy, fz = perturb(grid.fx, 11), perturb(grid.fy, 13), perturb(grid.fz, 17)
+    g = Grid(grid.nx, grid.ny, grid.nz, grid.dx, grid.blocked.copy(), fx, fy, fz, fx.copy(), fy.copy(), fz.copy())
+    return compute_zones(g)

@@ -155,3 +155,21 @@ def test_mixed_grids_batch(ctx):
     B.destroy()
     for G in Gs:
         G.destroy()
+
+
+@pytest.mark.parametrize("prec", [gas.PREC_FP32, gas.PREC_MIXED_F16])
+def test_fractional_coefficients(ctx, prec):
+    """Partial openings give fractional matrix coefficients: the face-array
+    path of the level-0 kernels (the 0/1 face mask does not apply)."""
+    g = sc.vented(sc.house(5, 40, 30, 20), 3)
+    assert np.any((g.ax > 0) & (g.ax < 1))
+    H = po.Hierarchy(g)
+    b = sc.rhs_from_divergence(g, sc.divergence_field(g, 8), DT)
+    x64, _, _ = H.pcg64(b)
+    G = gas.Grid(ctx, g, precision=prec)
+    db, dx = ctx.to_device(b), ctx.array(g.n)
+    st = G.solve(db, dx, gas.solve_params(tol=1e-6, max_iters=300, zero_x=True))
+    x = dx.download()
+    print(f"vented prec={prec} iters={st['iters']} err_fp64={rel(x, x64):.2e}")
+    assert st["status"] == gas.CONVERGED and rel(x, x64) < 1e-5
+    G.destroy()
