@@ -50,8 +50,8 @@ enum {
     PHG_STOP_MAXNORM = 1   /* max|r| / inv <= tol in 1/s, the reference's rule (gasphase.cpp:592-605) */
 };
 
-/* solve status */
-enum { PHG_RUNNING = 0, PHG_CONVERGED = 1, PHG_MAXITER = 2, PHG_BREAKDOWN = 3 };
+/* solve status: breakdown = d.Ad <= 0 (gasphase.cpp:607) or r.z <= 0 (:605) */
+enum { PHG_RUNNING = 0, PHG_CONVERGED = 1, PHG_MAXITER = 2, PHG_BREAKDOWN = 3, PHG_BREAKDOWN_RZ = 4 };
 
 typedef struct phg_ctx phg_ctx;
 

@@ -271,6 +271,7 @@ void orc_mg_free(orc_mg *mg) {
     }
     free(mg->chol);
     free(mg->cmask);
+    free(mg->ccomp);
     free(mg);
 }
 
@@ -342,6 +343,20 @@ static void ensure_chol(orc_mg *mg) {
         for (int J = 0; J < n; ++J)
             if (!mg->cmask[J] && uf_find(p, J) == r) A[(size_t)I * n + J] += s;
     }
+    /* closed-component ids, for the projection onto zero component sums */
+    mg->ccomp = (int *)malloc(sizeof(int) * (size_t)(n > 0 ? n : 1));
+    int *cid = (int *)malloc(sizeof(int) * (size_t)(n > 0 ? n : 1));
+    for (int I = 0; I < n; ++I) cid[I] = -1;
+    mg->nccomp = 0;
+    for (int I = 0; I < n; ++I) {
+        mg->ccomp[I] = -1;
+        if (mg->cmask[I]) continue;
+        int r = uf_find(p, I);
+        if (open[r]) continue;
+        if (cid[r] < 0) cid[r] = mg->nccomp++;
+        mg->ccomp[I] = cid[r];
+    }
+    free(cid);
     free(open); free(cnt); free(dsum); free(p);
     /* in-place Cholesky, lower triangle */
     for (int j = 0; j < n; ++j) {
@@ -356,6 +371,19 @@ static void ensure_chol(orc_mg *mg) {
         }
     }
     mg->chol = A;
+}
+
+/* remove the mean of y on every closed component of the coarsest level */
+static void project_closed(const orc_mg *mg, int n, double *y) {
+    if (mg->nccomp == 0) return;
+    double *sum = (double *)calloc((size_t)mg->nccomp, sizeof(double));
+    double *cnt = (double *)calloc((size_t)mg->nccomp, sizeof(double));
+    for (int I = 0; I < n; ++I)
+        if (mg->ccomp[I] >= 0) { sum[mg->ccomp[I]] += y[I]; cnt[mg->ccomp[I]] += 1; }
+    for (int I = 0; I < n; ++I)
+        if (mg->ccomp[I] >= 0) y[I] -= sum[mg->ccomp[I]] / cnt[mg->ccomp[I]];
+    free(sum);
+    free(cnt);
 }
 
 static void chol_solve(const double *Lm, int n, double *y) {

@@ -50,6 +50,8 @@ typedef struct orc_mg {
     /* exact coarsest solve (FP64 Cholesky of the grounded coarsest operator) */
     double *chol;                 /* n_c*n_c lower factor, NULL until first use */
     uint8_t *cmask;               /* coarsest rows forced to zero (diag <= 0) */
+    int *ccomp;                   /* coarsest rows: closed-component id, -1 if open or masked */
+    int nccomp;
 } orc_mg;
 
 enum { ORC_COARSE_GS20 = 0, ORC_COARSE_EXACT = 1 };

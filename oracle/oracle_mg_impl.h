@@ -91,7 +91,10 @@ static void FN(coarse_exact)(orc_mg *mg, const orc_level *L, REAL *x, const REAL
     const int n = L->n;
     double *y = (double *)malloc(sizeof(double) * (size_t)(n > 0 ? n : 1));
     for (int I = 0; I < n; ++I) y[I] = mg->cmask[I] ? 0.0 : (double)b[I];
+    /* P B^-1 P: the pseudo-inverse on closed components (no constant offset) */
+    project_closed(mg, n, y);
     chol_solve(mg->chol, n, y);
+    project_closed(mg, n, y);
     for (int I = 0; I < n; ++I) x[I] = mg->cmask[I] ? (REAL)0 : (REAL)y[I];
     free(y);
 }

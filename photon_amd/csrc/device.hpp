@@ -69,7 +69,8 @@ struct DLevel {
 // S_ZS: power-of-two scale applied to the V-cycle input (its residual made
 // O(1) so FP16/BF16 storage stays in range; exact in FP32); S_ZSN: the next one
 enum { S_RZ = 0, S_DQ, S_ALPHA, S_BETA, S_RR, S_BB, S_TOL, S_MAXR, S_INV, S_ZS, S_ZSN, S_NSCAL = 16 };
-enum { ST_ACTIVE = 0, ST_ITERS, ST_MAXIT, ST_STATUS, ST_RULE, ST_ZEROX, ST_XPEND, ST_NSTATE = 8 };
+// ST_FLEX: flexible beta (mixed precision); ST_VC: a CG iteration has run (q is defined)
+enum { ST_ACTIVE = 0, ST_ITERS, ST_MAXIT, ST_STATUS, ST_RULE, ST_ZEROX, ST_XPEND, ST_FLEX, ST_VC, ST_NSTATE = 12 };
 
 // One system of a batch.  Level 0 is swept by columns: a block takes TB
 // consecutive (i, j) columns of one k-chunk of `kchunk` planes, so that

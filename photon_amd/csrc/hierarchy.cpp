@@ -211,6 +211,27 @@ void dense_inverse(HostHierarchy& H) {
     for (int I = 0; I < n; ++I)
         if (H.cmask[I])
             for (int J = 0; J < n; ++J) H.cinv[(size_t)I * n + J] = H.cinv[(size_t)J * n + I] = 0.0;
+    // On a closed component project both sides onto vectors of zero component
+    // sum: P B^-1 P is the pseudo-inverse of A there, so the coarse correction
+    // carries no constant offset (A's null space) whatever rounding left in
+    // the restricted residual.
+    for (int r = 0; r < n; ++r) {
+        const std::vector<int>& m = members[r];
+        if (m.empty() || open[r]) continue;
+        const double inv_c = 1.0 / (double)m.size();
+        std::vector<double> rowmean(m.size(), 0.0);
+        double total = 0.0;
+        for (size_t a = 0; a < m.size(); ++a) {
+            double acc = 0.0;
+            for (int J : m) acc += H.cinv[(size_t)m[a] * n + J];
+            rowmean[a] = acc * inv_c;
+            total += acc;
+        }
+        total *= inv_c * inv_c;
+        for (size_t a = 0; a < m.size(); ++a)
+            for (size_t b2 = 0; b2 < m.size(); ++b2)
+                H.cinv[(size_t)m[a] * n + m[b2]] += total - rowmean[a] - rowmean[b2];   // symmetric: col mean = row mean
+    }
 }
 
 }  // namespace

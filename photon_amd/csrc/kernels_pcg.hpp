@@ -259,7 +259,7 @@ __global__ void __launch_bounds__(TB) k_fin_rz0(const DSys* __restrict__ sys, co
     if (threadIdx.x == 0) {
         s.scal[S_RZ] = res[0] / s.scal[S_ZS];
         if (!(res[0] > 0)) {
-            s.state[ST_STATUS] = 3;
+            s.state[ST_STATUS] = 4;
             s.state[ST_ACTIVE] = 0;
         }
     }
@@ -280,23 +280,31 @@ __global__ void __launch_bounds__(TB) k_fin_alpha(const DSys* __restrict__ sys, 
         } else {
             s.scal[S_ALPHA] = s.scal[S_RZ] / dq;
             s.scal[S_ZS] = s.scal[S_ZSN];   // z of the previous V-cycle has been consumed
+            s.state[ST_VC] = 1;             // q is defined from here on
         }
     }
 }
 
-// after k_post_red: beta, iteration count and the stop test (gasphase.cpp:617-625)
+// after k_post_red: beta, iteration count and the stop test (gasphase.cpp:617-625).
+// beta = rz_new / rz (the reference's); with a mixed-precision V-cycle, whose
+// rounding makes the preconditioner vary slightly between applications,
+// the flexible (Polak-Ribiere) form beta = z_new.(r_new - r_old) / rz
+// = -alpha q.z_new / rz, which equals it for a fixed preconditioner
 __global__ void __launch_bounds__(TB) k_fin_beta(const DSys* __restrict__ sys, const int2* __restrict__ tab) {
     PHG_SYS_PROLOGUE
-    __shared__ double res[2];
+    __shared__ double res[3];
     sum_parts(s, 3, &res[0]);
     __syncthreads();
     sum_parts(s, 1, &res[1]);
+    __syncthreads();
+    sum_parts(s, 0, &res[2]);
     const float m = max_parts(s);
     __syncthreads();
     if (threadIdx.x == 0) {
         const double rz_new = res[0] / s.scal[S_ZS];
         s.scal[S_ZSN] = vcycle_scale(res[1], s.L[0].n);
-        s.scal[S_BETA] = rz_new / s.scal[S_RZ];
+        if (s.state[ST_FLEX]) s.scal[S_BETA] = -s.scal[S_ALPHA] * (res[2] / s.scal[S_ZS]) / s.scal[S_RZ];
+        else s.scal[S_BETA] = rz_new / s.scal[S_RZ];
         s.scal[S_RZ] = rz_new;
         s.scal[S_RR] = res[1];
         s.scal[S_MAXR] = m;
@@ -305,7 +313,7 @@ __global__ void __launch_bounds__(TB) k_fin_beta(const DSys* __restrict__ sys, c
         int status = 0;
         if (stop_reached(s, res[1], m)) status = 1;
         else if (it >= s.state[ST_MAXIT]) status = 2;
-        else if (!(rz_new > 0)) status = 3;
+        else if (!(rz_new > 0)) status = 4;
         if (status) {
             s.state[ST_STATUS] = status;
             s.state[ST_ACTIVE] = 0;
@@ -405,7 +413,7 @@ __global__ void __launch_bounds__(TB) k_rb_black(const DSys* __restrict__ sys, c
 }
 
 // level-0 residual r0 = r - A z into the V-cycle scratch (residualGrid, pressure_mg.cpp:217-235)
-template <class CT, class VT>
+template <class CT, class ZT, class VT>
 __global__ void __launch_bounds__(TB) k_resid0(const DSys* __restrict__ sys, const int2* __restrict__ tab) {
     PHG_SYS_PROLOGUE
     const DLevel& L = s.L[0];
@@ -421,7 +429,7 @@ __global__ void __launch_bounds__(TB) k_resid0(const DSys* __restrict__ sys, con
         const float dg = diag6(a);
         float r = 0.0f;
         if (dg > 0)
-            r = s.r[q.c] * zs - dg * ld<VT>(z, q.c) + nb_sum_f(q, L.nx, L.ny, L.nz, a, [&](int n) { return ld<VT>(z, n); });
+            r = s.r[q.c] * zs - dg * ld<ZT>(z, q.c) + nb_sum_f(q, L.nx, L.ny, L.nz, a, [&](int n) { return ld<ZT>(z, n); });
         st<VT>(L.r, q.c, r);
     }
 }
@@ -430,7 +438,7 @@ __global__ void __launch_bounds__(TB) k_resid0(const DSys* __restrict__ sys, con
 // z = (r + sum a (z_nb + t1[agg_nb])) / diag, where the red neighbours' values
 // are formed as prolonged on the fly (pressure_mg.cpp:293-296).  The black
 // cells' own prolonged values are never read: the sweep replaces them.
-template <class CT, class VT>
+template <class CT, class ZT, class VT>
 __global__ void __launch_bounds__(TB) k_post_black(const DSys* __restrict__ sys, const int2* __restrict__ tab) {
     PHG_SYS_PROLOGUE
     const DLevel& L = s.L[0];
@@ -442,7 +450,7 @@ __global__ void __launch_bounds__(TB) k_post_black(const DSys* __restrict__ sys,
     const int* __restrict__ agg = L.agg;
     auto zp = [&](int n) {
         const int g = agg[n];
-        return ld<VT>(z, n) + (g >= 0 ? ld<VT>(t1, g) : 0.0f);
+        return ld<ZT>(z, n) + (g >= 0 ? ld<VT>(t1, g) : 0.0f);
     };
     const float zs = (float)s.scal[S_ZS];
     for (int k = t.k0 + ((t.i + t.j + t.k0 + 1) & 1); k < t.k1; k += 2) {
@@ -452,11 +460,12 @@ __global__ void __launch_bounds__(TB) k_post_black(const DSys* __restrict__ sys,
         const float dg = diag6(a);
         float zc = 0.0f;
         if (dg > 0) zc = (s.r[q.c] * zs + nb_sum_f(q, L.nx, L.ny, L.nz, a, zp)) / dg;
-        st<VT>(s.z, q.c, zc);
+        st<ZT>(s.z, q.c, zc);
     }
 }
 
-// red post-smoothing half sweep (gsColor(0)) fused with the partial r.z (slot 3)
+// red post-smoothing half sweep (gsColor(0)) fused with the partial r.z
+// (slot 3) and, for flexible CG, q.z (slot 0)
 template <class CT, class VT>
 __global__ void __launch_bounds__(TB) k_post_red(const DSys* __restrict__ sys, const int2* __restrict__ tab) {
     PHG_SYS_PROLOGUE
@@ -465,7 +474,8 @@ __global__ void __launch_bounds__(TB) k_post_red(const DSys* __restrict__ sys, c
     const int sxy = L.nx * L.ny;
     const void* z = s.z;
     const float zs = (float)s.scal[S_ZS];
-    double v[1] = {0.0};
+    const bool with_q = s.state[ST_FLEX] != 0 && s.state[ST_VC] != 0;   // q exists once an iteration has run
+    double v[1] = {0.0}, vq = 0.0;
     if (t.valid) {
         for (int k = t.k0; k < t.k1; ++k) {
             const Cell q{t.c2 + sxy * k, t.i, t.j, k};
@@ -482,19 +492,28 @@ __global__ void __launch_bounds__(TB) k_post_red(const DSys* __restrict__ sys, c
                 zc = ld<VT>(z, q.c);
             }
             v[0] += (double)r * zc;
+            if (with_q) vq += (double)s.q[q.c] * zc;
         }
     }
     double* out = s.part + (long)e.y * NPART;
-    __shared__ double sh[TB / 64];
+    __shared__ double sh[2][TB / 64];
     const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
     v[0] = wave_sum(v[0]);
-    if (lane == 0) sh[w] = v[0];
+    if (with_q) vq = wave_sum(vq);
+    if (lane == 0) {
+        sh[0][w] = v[0];
+        sh[1][w] = vq;
+    }
     __syncthreads();
     if (threadIdx.x == 0) {
-        double acc = 0.0;
+        double acc = 0.0, accq = 0.0;
 #pragma unroll
-        for (int ww = 0; ww < TB / 64; ++ww) acc += sh[ww];
+        for (int ww = 0; ww < TB / 64; ++ww) {
+            acc += sh[0][ww];
+            accq += sh[1][ww];
+        }
         out[3] = acc;
+        if (with_q) out[0] = accq;
     }
 }
 

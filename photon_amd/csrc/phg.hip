@@ -277,9 +277,9 @@ void vcycle_t(phg_ctx* ctx, BatchDev* B, int first) {
     const DSys* S = B->d_sys;
     const int2* T0 = B->tab[0];
     const int n0 = B->tab_len[0];
-    launch(ctx, K_PRE_RED, k_pre_red<C0, VT>, n0, 0, S, T0, first);
-    launch(ctx, K_RB_BLACK, k_rb_black<C0, VT>, n0, 0, S, T0);
-    launch(ctx, K_RESID0, k_resid0<C0, VT>, n0, 0, S, T0);
+    launch(ctx, K_PRE_RED, k_pre_red<C0, float>, n0, 0, S, T0, first);
+    launch(ctx, K_RB_BLACK, k_rb_black<C0, float>, n0, 0, S, T0);
+    launch(ctx, K_RESID0, k_resid0<C0, float, VT>, n0, 0, S, T0);
     launch(ctx, K_VC_RESTRICT, k_vc_restrict<VT>, B->tab_len[1], 0, S, B->tab[1], 0);
     for (int l = 1; l + 1 < B->maxl; ++l) {
         launch(ctx, K_VC_JAC, k_vc_jac0<CC, VT>, B->tab_len[l], 0, S, B->tab[l], l);
@@ -294,8 +294,8 @@ void vcycle_t(phg_ctx* ctx, BatchDev* B, int first) {
         launch(ctx, K_VC_JAC, k_vc_jac<CC, VT>, B->tab_len[l], 0, S, B->tab[l], l, 0);
         launch(ctx, K_VC_JAC, k_vc_jac<CC, VT>, B->tab_len[l], 0, S, B->tab[l], l, 1);
     }
-    launch(ctx, K_POST_BLACK, k_post_black<C0, VT>, n0, 0, S, T0);
-    launch(ctx, K_POST_RED, k_post_red<C0, VT>, n0, 0, S, T0);
+    launch(ctx, K_POST_BLACK, k_post_black<C0, float, VT>, n0, 0, S, T0);
+    launch(ctx, K_POST_RED, k_post_red<C0, float>, n0, 0, S, T0);
 }
 
 void vcycle(phg_ctx* ctx, BatchDev* B, int first) {
@@ -314,10 +314,7 @@ void vcycle(phg_ctx* ctx, BatchDev* B, int first) {
 template <class OT>
 void dir_apply(phg_ctx* ctx, BatchDev* B) {
     const DSys* S = B->d_sys;
-    if (B->prec == PHG_PREC_FP32) launch(ctx, K_DIR_APPLY, k_dir_apply<OT, float>, B->tab_len[0], 0, S, B->tab[0]);
-    else if (B->prec == PHG_PREC_MIXED_F16)
-        launch(ctx, K_DIR_APPLY, k_dir_apply<OT, _Float16>, B->tab_len[0], 0, S, B->tab[0]);
-    else launch(ctx, K_DIR_APPLY, k_dir_apply<OT, bf16_t>, B->tab_len[0], 0, S, B->tab[0]);
+    launch(ctx, K_DIR_APPLY, k_dir_apply<OT, float>, B->tab_len[0], 0, S, B->tab[0]);
 }
 
 void cg_iteration(phg_ctx* ctx, BatchDev* B) {
@@ -358,6 +355,7 @@ int reset_state(phg_ctx* ctx, BatchDev* B, const phg_solve_params* P) {
         st[ST_MAXIT] = P ? P->max_iters : 0;
         st[ST_RULE] = P && P->stop_rule == PHG_STOP_MAXNORM ? 1 : 0;
         st[ST_ZEROX] = P && P->zero_x ? 1 : 0;
+        st[ST_FLEX] = B->prec != PHG_PREC_FP32 ? 1 : 0;
     }
     HIP_OK(hipMemcpyAsync(B->scal, scal.data(), scal.size() * sizeof(double), hipMemcpyHostToDevice, ctx->stream));
     HIP_OK(hipMemcpyAsync(B->state, state.data(), state.size() * sizeof(int), hipMemcpyHostToDevice, ctx->stream));
@@ -734,24 +732,23 @@ int phg_batch_create_ex(phg_ctx* ctx, int nsys, const int* grids, const phg_batc
         S.zclosed = G->zclosed;
         const size_t n0 = G->n;
         float* v4 = nullptr;
-        rc = dalloc(ctx, B->allocs, &v4, 4 * n0);
+        rc = dalloc(ctx, B->allocs, &v4, 5 * n0);
         if (rc) break;
         S.r = v4;
         S.q = v4 + n0;
         S.d0 = v4 + 2 * n0;
         S.d1 = v4 + 3 * n0;
+        S.z = v4 + 4 * n0;   // the level-0 iterate stays FP32 in every precision: it feeds d directly
         rc = dalloc(ctx, B->allocs, &S.part, (size_t)S.tiles0 * NPART);
         if (!rc) rc = dalloc(ctx, B->allocs, &S.zpart, (size_t)std::max(G->nztask, 1));
         if (!rc) rc = dalloc(ctx, B->allocs, &S.zmean, (size_t)std::max(G->nclosed, 1));
         if (rc) break;
-        // V-cycle vectors: level-0 iterate z and residual scratch, coarse x b r t
-        size_t total = 2 * n0;
+        // V-cycle storage: level-0 residual scratch, coarse x b r t
+        size_t total = n0;
         for (int l = 1; l < S.nl; ++l) total += 4 * (size_t)G->lv[l].n;
         char* slab = nullptr;
         rc = dalloc(ctx, B->allocs, &slab, total * vts);
         if (rc) break;
-        S.z = slab;
-        slab += n0 * vts;
         for (int l = 0; l < S.nl; ++l) {
             DLevel& D = S.L[l];
             const GridDev::Lev& GL = G->lv[l];
@@ -957,9 +954,7 @@ int phg_precondition(phg_ctx* ctx, int grid, const float* r, float* z) {
     if (rc) return rc;
     vcycle(ctx, B, 1);
     const DSys* S = B->d_sys;
-    if (B->prec == PHG_PREC_FP32) launch(ctx, K_POST_RED, k_z_out<float>, B->tab_len[0], 0, S, B->tab[0], z);
-    else if (B->prec == PHG_PREC_MIXED_F16) launch(ctx, K_POST_RED, k_z_out<_Float16>, B->tab_len[0], 0, S, B->tab[0], z);
-    else launch(ctx, K_POST_RED, k_z_out<bf16_t>, B->tab_len[0], 0, S, B->tab[0], z);
+    launch(ctx, K_POST_RED, k_z_out<float>, B->tab_len[0], 0, S, B->tab[0], z);
     HIP_OK(hipStreamSynchronize(ctx->stream));
     HIP_OK(hipGetLastError());
     if (ctx->prof) prof_drain(ctx);
