@@ -99,6 +99,82 @@ __device__ __forceinline__ ColTile col_tile(const DSys& s, int tile) {
     return t;
 }
 
+// ---------------------------------------------------------------------------
+// plane march with LDS-staged lateral neighbours
+// ---------------------------------------------------------------------------
+// A block owns the TB columns [c0, c0 + TB) of one k-chunk.  For every plane
+// k it stages val() of its columns, and of one grid row on either side (the
+// -nx / +nx neighbours), in LDS; the four lateral neighbours are then read
+// from LDS and the two vertical ones come from a register pipeline along k.
+// Each staged value is loaded from global memory once per plane instead of
+// up to seven times.  lds: 2 * (TB + 2 nx) floats, the plane buffer being
+// double-buffered so that one barrier per plane suffices.
+// Loads are software-pipelined two planes ahead (own column and the first
+// MARCH_HALO * TB halo values), so a plane's barrier does not wait on a
+// global-memory round trip.  The halo of a plane is 2 nx values; any beyond the
+// pipelined part (nx > MARCH_HALO * TB / 2) are loaded in the plane itself.
+constexpr int MARCH_HALO = 2;
+constexpr int MARCH_MAX_NX = 2048;   // LDS plane buffers: 2 (TB + 2 nx) floats
+
+template <class VAL, class BODY>
+__device__ __forceinline__ void march(const DSys& s, const ColTile& t, float* lds, VAL val, BODY body) {
+    const DLevel& L = s.L[0];
+    const int nx = L.nx, nz = L.nz, sxy = nx * L.ny;
+    const int W = TB + 2 * nx;
+    const int c0 = t.c2 - (int)threadIdx.x;
+    auto own = [&](int k) { return (t.valid && k >= 0 && k < nz) ? val(t.c2 + sxy * k) : 0.0f; };
+    auto halo = [&](int k, float (&h)[MARCH_HALO]) {
+#pragma unroll
+        for (int m = 0; m < MARCH_HALO; ++m) {
+            const int hh = (int)threadIdx.x + m * TB;
+            const int col = hh < nx ? c0 - nx + hh : c0 + TB + (hh - nx);
+            h[m] = (hh < 2 * nx && k < nz && col >= 0 && col < sxy) ? val(col + sxy * k) : 0.0f;
+        }
+    };
+    float vprev = own(t.k0 - 1), vcur = own(t.k0), vnext = own(t.k0 + 1);
+    float hcur[MARCH_HALO], hnext[MARCH_HALO];
+    halo(t.k0, hcur);
+    halo(t.k0 + 1, hnext);
+    for (int k = t.k0; k < t.k1; ++k) {
+        float* buf = lds + (k & 1) * W;
+        buf[nx + threadIdx.x] = vcur;   // 0 for columns past the grid
+#pragma unroll
+        for (int m = 0; m < MARCH_HALO; ++m) {
+            const int hh = (int)threadIdx.x + m * TB;
+            if (hh < 2 * nx) buf[hh < nx ? hh : TB + hh] = hcur[m];
+        }
+        for (int hh = (int)threadIdx.x + MARCH_HALO * TB; hh < 2 * nx; hh += TB) {
+            const int col = hh < nx ? c0 - nx + hh : c0 + TB + (hh - nx);
+            buf[hh < nx ? hh : TB + hh] = (col >= 0 && col < sxy) ? val(col + sxy * k) : 0.0f;
+        }
+        __syncthreads();
+        // issue the loads of plane k + 2 before this plane's work
+        const bool more = k + 1 < t.k1;
+        const float vnn = more ? own(k + 2) : 0.0f;
+        float hnn[MARCH_HALO];
+        if (more) halo(k + 2, hnn);
+        if (t.valid) body(k, vprev, vcur, vnext, (const float*)(buf + nx + threadIdx.x));
+        vprev = vcur;
+        vcur = vnext;
+        vnext = vnn;
+#pragma unroll
+        for (int m = 0; m < MARCH_HALO; ++m) {
+            hcur[m] = hnext[m];
+            hnext[m] = more ? hnn[m] : 0.0f;
+        }
+    }
+}
+
+// a_f-weighted sum of the six neighbours: lateral ones at p[-1], p[+1],
+// p[-nx], p[+nx] (the staged plane), vertical ones vp (k-1) and vn (k+1);
+// missing neighbours (Dirichlet boundary) weigh 0
+__device__ __forceinline__ float lat_sum(const Cell& q, int nx, int ny, int nz, const float a[6], const float* p,
+                                         float vp, float vn) {
+    const bool e0 = q.i > 0, e1 = q.i < nx - 1, e2 = q.j > 0, e3 = q.j < ny - 1, e4 = q.k > 0, e5 = q.k < nz - 1;
+    return (e0 ? a[0] : 0.0f) * p[-1] + (e1 ? a[1] : 0.0f) * p[1] + (e2 ? a[2] : 0.0f) * p[-nx] +
+           (e3 ? a[3] : 0.0f) * p[nx] + (e4 ? a[4] : 0.0f) * vp + (e5 ? a[5] : 0.0f) * vn;
+}
+
 #define PHG_SYS_PROLOGUE                  \
     const int2 e = tab[blockIdx.x];       \
     const DSys& s = sys[e.x];             \
@@ -163,22 +239,21 @@ __global__ void __launch_bounds__(TB) k_dir_apply(const DSys* __restrict__ sys, 
     const void* z = s.z;
     const float izs = (float)(1.0 / s.scal[S_ZS]);   // z is stored scaled by S_ZS (a power of two)
     auto dval = [&](int n) { return first ? ld<ZT>(z, n) * izs : ld<ZT>(z, n) * izs + (float)(beta * dold[n]); };
-    double v[1] = {0.0};
-    if (t.valid) {
-        for (int k = t.k0; k < t.k1; ++k) {
-            const Cell q{t.c2 + sxy * k, t.i, t.j, k};
-            float a[6];
-            coefs6<OT>(L, L.ax32, L.ay32, L.az32, q, a);
-            const float dg = diag6(a);
-            const float dc = dval(q.c);
-            float y = 0.0f;
-            if (dg > 0) y = dg * dc - nb_sum_f(q, L.nx, L.ny, L.nz, a, dval);
-            if (!first) s.x[q.c] += (float)(alpha * dold[q.c]);
-            dnew[q.c] = dc;
-            s.q[q.c] = y;
-            v[0] += (double)y * dc;
-        }
-    }
+    extern __shared__ float lds[];
+    double acc = 0.0;
+    march(s, t, lds, dval, [&](int k, float vp, float dc, float vn, const float* p) {
+        const Cell q{t.c2 + sxy * k, t.i, t.j, k};
+        float a[6];
+        coefs6<OT>(L, L.ax32, L.ay32, L.az32, q, a);
+        const float dg = diag6(a);
+        float y = 0.0f;
+        if (dg > 0) y = dg * dc - lat_sum(q, L.nx, L.ny, L.nz, a, p, vp, vn);
+        if (!first) s.x[q.c] += (float)(alpha * dold[q.c]);
+        dnew[q.c] = dc;
+        s.q[q.c] = y;
+        acc += (double)y * dc;
+    });
+    double v[1] = {acc};
     block_sums<1>(v, s.part + (long)e.y * NPART);
 }
 
@@ -397,19 +472,20 @@ __global__ void __launch_bounds__(TB) k_rb_black(const DSys* __restrict__ sys, c
     PHG_SYS_PROLOGUE
     const DLevel& L = s.L[0];
     const ColTile t = col_tile(s, e.y);
-    if (!t.valid) return;
     const int sxy = L.nx * L.ny;
     const void* z = s.z;
     const float zs = (float)s.scal[S_ZS];
-    for (int k = t.k0 + ((t.i + t.j + t.k0 + 1) & 1); k < t.k1; k += 2) {
+    extern __shared__ float lds[];
+    march(s, t, lds, [&](int n) { return ld<ZT>(z, n); }, [&](int k, float vp, float vc, float vn, const float* p) {
+        if (((t.i + t.j + k) & 1) == 0) return;
         const Cell q{t.c2 + sxy * k, t.i, t.j, k};
         float a[6];
         coefs6<CT>(L, L.axc, L.ayc, L.azc, q, a);
         const float dg = diag6(a);
         float zc = 0.0f;
-        if (dg > 0) zc = (s.r[q.c] * zs + nb_sum_f(q, L.nx, L.ny, L.nz, a, [&](int n) { return ld<ZT>(z, n); })) / dg;
+        if (dg > 0) zc = (s.r[q.c] * zs + lat_sum(q, L.nx, L.ny, L.nz, a, p, vp, vn)) / dg;
         st<ZT>(s.z, q.c, zc);
-    }
+    });
 }
 
 // level-0 residual r0 = r - A z into the V-cycle scratch (residualGrid, pressure_mg.cpp:217-235)
@@ -418,20 +494,19 @@ __global__ void __launch_bounds__(TB) k_resid0(const DSys* __restrict__ sys, con
     PHG_SYS_PROLOGUE
     const DLevel& L = s.L[0];
     const ColTile t = col_tile(s, e.y);
-    if (!t.valid) return;
     const int sxy = L.nx * L.ny;
     const void* z = s.z;
     const float zs = (float)s.scal[S_ZS];
-    for (int k = t.k0; k < t.k1; ++k) {
+    extern __shared__ float lds[];
+    march(s, t, lds, [&](int n) { return ld<ZT>(z, n); }, [&](int k, float vp, float vc, float vn, const float* p) {
         const Cell q{t.c2 + sxy * k, t.i, t.j, k};
         float a[6];
         coefs6<CT>(L, L.axc, L.ayc, L.azc, q, a);
         const float dg = diag6(a);
         float r = 0.0f;
-        if (dg > 0)
-            r = s.r[q.c] * zs - dg * ld<ZT>(z, q.c) + nb_sum_f(q, L.nx, L.ny, L.nz, a, [&](int n) { return ld<ZT>(z, n); });
+        if (dg > 0) r = s.r[q.c] * zs - dg * vc + lat_sum(q, L.nx, L.ny, L.nz, a, p, vp, vn);
         st<VT>(L.r, q.c, r);
-    }
+    });
 }
 
 // prolongation fused into the black post-smoothing half sweep: black cells
@@ -443,7 +518,6 @@ __global__ void __launch_bounds__(TB) k_post_black(const DSys* __restrict__ sys,
     PHG_SYS_PROLOGUE
     const DLevel& L = s.L[0];
     const ColTile t = col_tile(s, e.y);
-    if (!t.valid) return;
     const int sxy = L.nx * L.ny;
     const void* z = s.z;
     const void* t1 = s.L[1].t;
@@ -453,15 +527,17 @@ __global__ void __launch_bounds__(TB) k_post_black(const DSys* __restrict__ sys,
         return ld<ZT>(z, n) + (g >= 0 ? ld<VT>(t1, g) : 0.0f);
     };
     const float zs = (float)s.scal[S_ZS];
-    for (int k = t.k0 + ((t.i + t.j + t.k0 + 1) & 1); k < t.k1; k += 2) {
+    extern __shared__ float lds[];
+    march(s, t, lds, zp, [&](int k, float vp, float vc, float vn, const float* p) {
+        if (((t.i + t.j + k) & 1) == 0) return;
         const Cell q{t.c2 + sxy * k, t.i, t.j, k};
         float a[6];
         coefs6<CT>(L, L.axc, L.ayc, L.azc, q, a);
         const float dg = diag6(a);
         float zc = 0.0f;
-        if (dg > 0) zc = (s.r[q.c] * zs + nb_sum_f(q, L.nx, L.ny, L.nz, a, zp)) / dg;
+        if (dg > 0) zc = (s.r[q.c] * zs + lat_sum(q, L.nx, L.ny, L.nz, a, p, vp, vn)) / dg;
         st<ZT>(s.z, q.c, zc);
-    }
+    });
 }
 
 // red post-smoothing half sweep (gsColor(0)) fused with the partial r.z
@@ -476,25 +552,22 @@ __global__ void __launch_bounds__(TB) k_post_red(const DSys* __restrict__ sys, c
     const float zs = (float)s.scal[S_ZS];
     const bool with_q = s.state[ST_FLEX] != 0 && s.state[ST_VC] != 0;   // q exists once an iteration has run
     double v[1] = {0.0}, vq = 0.0;
-    if (t.valid) {
-        for (int k = t.k0; k < t.k1; ++k) {
-            const Cell q{t.c2 + sxy * k, t.i, t.j, k};
-            const float r = s.r[q.c];
-            float zc;
-            if (((t.i + t.j + k) & 1) == 0) {
-                float a[6];
-                coefs6<CT>(L, L.axc, L.ayc, L.azc, q, a);
-                const float dg = diag6(a);
-                zc = 0.0f;
-                if (dg > 0) zc = (r * zs + nb_sum_f(q, L.nx, L.ny, L.nz, a, [&](int n) { return ld<VT>(z, n); })) / dg;
-                st<VT>(s.z, q.c, zc);
-            } else {
-                zc = ld<VT>(z, q.c);
-            }
-            v[0] += (double)r * zc;
-            if (with_q) vq += (double)s.q[q.c] * zc;
+    extern __shared__ float lds[];
+    march(s, t, lds, [&](int n) { return ld<VT>(z, n); }, [&](int k, float vp, float vc, float vn, const float* p) {
+        const Cell q{t.c2 + sxy * k, t.i, t.j, k};
+        const float r = s.r[q.c];
+        float zc = vc;   // black cells: final from the black sweep
+        if (((t.i + t.j + k) & 1) == 0) {
+            float a[6];
+            coefs6<CT>(L, L.axc, L.ayc, L.azc, q, a);
+            const float dg = diag6(a);
+            zc = 0.0f;
+            if (dg > 0) zc = (r * zs + lat_sum(q, L.nx, L.ny, L.nz, a, p, vp, vn)) / dg;
+            st<VT>(s.z, q.c, zc);
         }
-    }
+        v[0] += (double)r * zc;
+        if (with_q) vq += (double)s.q[q.c] * zc;
+    });
     double* out = s.part + (long)e.y * NPART;
     __shared__ double sh[2][TB / 64];
     const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;

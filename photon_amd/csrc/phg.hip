@@ -85,6 +85,8 @@ struct GridDev {
 struct BatchDev {
     int nsys = 0, prec = 0, maxl = 0, maxnc = 0;
     bool binary = false;           // every grid has 0/1 coefficients: level-0 kernels read the face mask
+    int maxnx = 0;                 // widest grid: sizes the LDS plane buffers of the level-0 kernels
+    size_t lds0() const { return 2 * (size_t)(TB + 2 * maxnx) * sizeof(float); }
     std::vector<int> grids;
     std::vector<DSys> h_sys;
     DSys* d_sys = nullptr;
@@ -278,8 +280,8 @@ void vcycle_t(phg_ctx* ctx, BatchDev* B, int first) {
     const int2* T0 = B->tab[0];
     const int n0 = B->tab_len[0];
     launch(ctx, K_PRE_RED, k_pre_red<C0, float>, n0, 0, S, T0, first);
-    launch(ctx, K_RB_BLACK, k_rb_black<C0, float>, n0, 0, S, T0);
-    launch(ctx, K_RESID0, k_resid0<C0, float, VT>, n0, 0, S, T0);
+    launch(ctx, K_RB_BLACK, k_rb_black<C0, float>, n0, B->lds0(), S, T0);
+    launch(ctx, K_RESID0, k_resid0<C0, float, VT>, n0, B->lds0(), S, T0);
     launch(ctx, K_VC_RESTRICT, k_vc_restrict<VT>, B->tab_len[1], 0, S, B->tab[1], 0);
     for (int l = 1; l + 1 < B->maxl; ++l) {
         launch(ctx, K_VC_JAC, k_vc_jac0<CC, VT>, B->tab_len[l], 0, S, B->tab[l], l);
@@ -294,8 +296,8 @@ void vcycle_t(phg_ctx* ctx, BatchDev* B, int first) {
         launch(ctx, K_VC_JAC, k_vc_jac<CC, VT>, B->tab_len[l], 0, S, B->tab[l], l, 0);
         launch(ctx, K_VC_JAC, k_vc_jac<CC, VT>, B->tab_len[l], 0, S, B->tab[l], l, 1);
     }
-    launch(ctx, K_POST_BLACK, k_post_black<C0, float, VT>, n0, 0, S, T0);
-    launch(ctx, K_POST_RED, k_post_red<C0, float>, n0, 0, S, T0);
+    launch(ctx, K_POST_BLACK, k_post_black<C0, float, VT>, n0, B->lds0(), S, T0);
+    launch(ctx, K_POST_RED, k_post_red<C0, float>, n0, B->lds0(), S, T0);
 }
 
 void vcycle(phg_ctx* ctx, BatchDev* B, int first) {
@@ -314,7 +316,7 @@ void vcycle(phg_ctx* ctx, BatchDev* B, int first) {
 template <class OT>
 void dir_apply(phg_ctx* ctx, BatchDev* B) {
     const DSys* S = B->d_sys;
-    launch(ctx, K_DIR_APPLY, k_dir_apply<OT, float>, B->tab_len[0], 0, S, B->tab[0]);
+    launch(ctx, K_DIR_APPLY, k_dir_apply<OT, float>, B->tab_len[0], B->lds0(), S, B->tab[0]);
 }
 
 void cg_iteration(phg_ctx* ctx, BatchDev* B) {
@@ -486,6 +488,7 @@ int phg_grid_create(phg_ctx* ctx, int nx, int ny, int nz, float dx, const float*
     if (!ctx || !grid_out) return PHG_EINVAL;
     if (nx < 2 || ny < 2 || nz < 2 || (long)nx * ny * nz > (1L << 30) || !(dx > 0))
         return fail(ctx, PHG_EINVAL, "bad grid %dx%dx%d dx=%g", nx, ny, nz, dx);
+    if (nx > MARCH_MAX_NX) return fail(ctx, PHG_EINVAL, "nx=%d exceeds the limit %d", nx, MARCH_MAX_NX);
     if (!fx || !fy || !fz || !ax || !ay || !az || !blocked) return fail(ctx, PHG_EINVAL, "null geometry array");
     if (precision < PHG_PREC_FP32 || precision > PHG_PREC_MIXED_BF16)
         return fail(ctx, PHG_EINVAL, "unknown precision %d", precision);
@@ -695,6 +698,7 @@ int phg_batch_create_ex(phg_ctx* ctx, int nsys, const int* grids, const phg_batc
         B->maxl = std::max(B->maxl, (int)G->lv.size());
         B->maxnc = std::max(B->maxnc, G->H.nc);
         B->binary = s == 0 ? G->binary : (B->binary && G->binary);
+        B->maxnx = std::max(B->maxnx, G->nx);
         col_tiles += tiles(G->nx * G->ny);
     }
     if (rc) {
