@@ -37,26 +37,59 @@ DT = 0.01
 TOL = 1e-6
 
 
-def kernel_bytes(name, n0, vt):
-    """Algorithmic HBM bytes of the level-0 kernels: (bytes per cell of one
-    system, bytes per launch shared by all systems of the grid).  One pass per
-    array touched; a half sweep writes half of z; outer CG vectors are FP32,
-    z and the residual scratch are `vt` bytes; the grid's 0/1 face mask
-    (1 B/cell) is read once per launch."""
-    per_cell = {
-        "dir_apply": 20 + vt,          # read z d_old x, write x d_new q
-        "pre_red": 12 + vt,            # read r q, write r z
-        "rb_black": 4 + 1.5 * vt,      # read r z, write half of z
-        "resid0": 4 + 2 * vt,          # read r z, write r0
-        "post_black": 4 + 1.5 * vt,    # read r z (+ t1 of level 1), write half of z
-        "post_red": 4 + 1.5 * vt,      # read r z, write half of z
-    }
-    return per_cell.get(name), n0 * 1
+LEVEL0 = ["dir_apply", "pre_red", "rb_black", "resid0", "post_black", "post_red"]
+# HBM bytes per launch of each level-0 kernel on this workload, from rocprofv3
+# FETCH_SIZE and WRITE_SIZE runs scaled by the copy-kernel calibration
+TRAFFIC_FILE = "profiles/level0_house80x50x40_1000rhs_mixed_f16_hbm_traffic.txt"
 
 
-def launches_per_solve(name, iters):
-    """Level-0 launches of a kernel class for one system solved in `iters` iterations."""
-    return iters if name == "dir_apply" else iters + 1
+def measured_traffic(kernel):
+    """HBM bytes per launch of `kernel` as measured by the counter runs
+    recorded in TRAFFIC_FILE (lines `traffic <kernel> <bytes per launch>`),
+    or None when no measurement is recorded."""
+    path = os.path.join(ROOT, TRAFFIC_FILE)
+    try:
+        with open(path) as f:
+            for line in f:
+                parts = line.split()
+                if len(parts) == 3 and parts[0] == "traffic" and parts[1] == "k_" + kernel:
+                    return float(parts[2])
+    except OSError:
+        pass
+    return None
+
+
+def algorithmic_bytes(name, iters, n0, n1, vt, flex):
+    """Algorithmic HBM bytes of all launches of a level-0 kernel class for a
+    batch of systems of one grid that stopped after `iters` iterations each:
+    one pass over every array the kernel touches.  Outer CG vectors
+    (x r q d) and the smoother iterate z are FP32; the V-cycle residual
+    scratch and level-1 correction are `vt` bytes; a half sweep writes half
+    of z.  Per launch, the grid's 0/1 face mask (1 B/cell) and, for the
+    prolongation, its level-0 aggregate map (4 B/cell) are read once, being
+    shared by every system of the batch.  One solve with k iterations runs
+    dir_apply k times (the first from d = z alone) and every V-cycle kernel
+    k + 1 times (the first with r unchanged)."""
+    it = np.asarray(iters, np.int64).ravel()
+    nvc = int((it + 1).sum())                        # system-launches of a V-cycle kernel
+    nd = int(np.maximum(it, 0).sum())                # system-launches of dir_apply
+    nfirst = int((it > 0).sum())
+    if name == "dir_apply":                          # read z d_old x, write x d_new q; first: read z, write d q
+        per_sys = nfirst * 12 * n0 + (nd - nfirst) * 24 * n0
+    elif name == "pre_red":                          # read r q, write r z; first: read r, write z
+        per_sys = len(it) * 8 * n0 + (nvc - len(it)) * 16 * n0
+    elif name == "rb_black":                         # read z r, write half of z
+        per_sys = nvc * 10 * n0
+    elif name == "resid0":                           # read z r, write r0
+        per_sys = nvc * (8 + vt) * n0
+    elif name == "post_black":                       # read z r and t1 (level 1), write half of z
+        per_sys = nvc * (10 * n0 + vt * n1)
+    elif name == "post_red":                         # read z r (and q when flexible), write half of z
+        per_sys = nvc * 10 * n0 + (nvc - len(it) if flex else 0) * 4 * n0
+    else:
+        raise KeyError(name)
+    shared = {"post_black": 5}.get(name, 1) * n0     # per launch
+    return per_sys, shared
 
 
 def run_batch(ctx, gas, grid_scen, prec, bs_host, steps, warmup, profile):
@@ -180,6 +213,10 @@ def main():
     ap.add_argument("--plain", type=int, default=100, help="plain FP32 solves timed one at a time")
     ap.add_argument("--cpu-budget", type=float, default=15.0)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--mixed-only", action="store_true",
+                    help="time the mixed-precision batch only (for counter runs under rocprofv3)")
+    ap.add_argument("--copy-gib", type=float, default=0.0,
+                    help="also run the plain copy kernel over this many GiB (counter calibration)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for --gpus {args.gpus}); scenarios shard across GPUs "
@@ -196,29 +233,48 @@ def main():
     ctx = gas.Context(0)
 
     mixed = run_batch(ctx, gas, scen, gas.PREC_MIXED_F16, bs, args.steps, args.warmup, profile=True)
-    fp32 = run_batch(ctx, gas, scen, gas.PREC_FP32, bs, args.steps, args.warmup, profile=False)
-    plain = run_plain(ctx, gas, scen, bs, args.plain)
-
+    if args.copy_gib > 0:
+        copy_bytes = int(args.copy_gib * (1 << 30)) // 16 * 16
+        src, dst = ctx.array(copy_bytes // 4), ctx.array(copy_bytes // 4)
+        ctx.copy(dst, src, copy_bytes)
+        ctx.synchronize()
+        del src, dst
     solves = args.steps * args.nsys
     v_mixed = solves / mixed["seconds"]
-    v_fp32 = solves / fp32["seconds"]
-    v_plain = args.plain / plain["seconds"]
+    if args.mixed_only:
+        fp32 = plain = None
+        v_fp32 = v_plain = float("nan")
+    else:
+        fp32 = run_batch(ctx, gas, scen, gas.PREC_FP32, bs, args.steps, args.warmup, profile=False)
+        plain = run_plain(ctx, gas, scen, bs, args.plain)
+        v_fp32 = solves / fp32["seconds"]
+        v_plain = args.plain / plain["seconds"]
 
     # roofline of the dominant kernel, from HIP events over the timed steps
+    # (recorded on the library's stream, around each launch)
     prof = mixed["prof"]
-    level0 = ["dir_apply", "pre_red", "rb_black", "resid0", "post_black", "post_red"]
     dom = max(prof, key=lambda k: prof[k][0])
-    dom0 = max(level0, key=lambda k: prof.get(k, (0, 0))[0])
-    per_cell, coef = kernel_bytes(dom0, n, 2)
-    total_launch = prof[dom0][1]
-    units = sum(launches_per_solve(dom0, it) for it in mixed["iters"].ravel())   # system-launches
-    total_bytes = units * n * per_cell + total_launch * coef
+    n1 = int(gas.hierarchy_shape(scen)[0][1])
+    level0_gbs = {}
+    for k in LEVEL0:
+        if prof.get(k, (0, 0))[1] == 0:
+            continue
+        per_sys, shared = algorithmic_bytes(k, mixed["iters"], n, n1, 2, flex=True)
+        level0_gbs[k] = (per_sys + prof[k][1] * shared) / (prof[k][0] * 1e-3) / 1e9
+    dom0 = max(LEVEL0, key=lambda k: prof.get(k, (0, 0))[0])
+    per_sys, shared = algorithmic_bytes(dom0, mixed["iters"], n, n1, 2, flex=True)
+    nlaunch = prof[dom0][1]
+    total_bytes = per_sys + nlaunch * shared
     ms = prof[dom0][0]
     achieved = total_bytes / (ms * 1e-3)
-    roof = {"bound": "hbm", "kernel": dom0, "achieved": achieved / 1e9, "peak": HBM_PEAK / 1e9, "unit": "GB/s",
-            "frac": achieved / HBM_PEAK, "traffic": None,
-            "bytes_per_launch": total_bytes / max(total_launch, 1), "avg_launch_us": 1e3 * ms / max(total_launch, 1),
-            "per_cell_bytes": per_cell}
+    bpl = total_bytes / max(nlaunch, 1)
+    traffic = measured_traffic(dom0)
+    roof = {"bound": "hbm", "kernel": dom0, "achieved": round(achieved / 1e9, 1), "peak": HBM_PEAK / 1e9,
+            "unit": "GB/s", "frac": round(achieved / HBM_PEAK, 4),
+            "traffic": None if traffic is None else round(traffic),
+            "traffic_source": None if traffic is None else TRAFFIC_FILE,
+            "algorithmic_bytes_per_launch": round(bpl), "avg_launch_us": round(1e3 * ms / max(nlaunch, 1), 2),
+            "launches": nlaunch, "level0_algorithmic_GBps": {k: round(v, 1) for k, v in level0_gbs.items()}}
     time_share = {k: round(v[0] / sum(x[0] for x in prof.values()), 4) for k, v in prof.items() if v[0] > 0}
 
     cpu = None
@@ -252,16 +308,16 @@ def main():
                                f"{args.nsys} independent RHS per step, ||r||/||b|| <= 1e-6, x0 = 0",
                    "cells": n, "solves_per_step": args.nsys, "precision": "FP16 V-cycle storage, FP32 CG vectors, "
                    "FP64 dots/alpha/beta and coarsest solve", "parallelism": "batch of independent solves"},
-        "fp32_batched_solves_per_s": round(v_fp32, 2),
-        "plain_fp32_solves_per_s": round(v_plain, 2),
-        "mixed_vs_fp32_batched": round(v_mixed / v_fp32, 3),
-        "mixed_vs_plain_fp32": round(v_mixed / v_plain, 3),
+        "fp32_batched_solves_per_s": None if fp32 is None else round(v_fp32, 2),
+        "plain_fp32_solves_per_s": None if plain is None else round(v_plain, 2),
+        "mixed_vs_fp32_batched": None if fp32 is None else round(v_mixed / v_fp32, 3),
+        "mixed_vs_plain_fp32": None if plain is None else round(v_mixed / v_plain, 3),
         "iters_mixed_mean": float(mixed["iters"].mean()),
-        "iters_fp32_mean": float(fp32["iters"].mean()),
-        "iters_plain_mean": float(plain["iters"].mean()),
-        "all_converged": bool(mixed["ok"] and fp32["ok"]),
+        "iters_fp32_mean": None if fp32 is None else float(fp32["iters"].mean()),
+        "iters_plain_mean": None if plain is None else float(plain["iters"].mean()),
+        "all_converged": bool(mixed["ok"] and (fp32 is None or fp32["ok"])),
         "status_mixed": mixed["status"],
-        "status_fp32": fp32["status"],
+        "status_fp32": None if fp32 is None else fp32["status"],
         "roofline": roof,
         "dominant_kernel_by_time": dom,
         "kernel_time_share": time_share,

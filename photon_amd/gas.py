@@ -177,6 +177,11 @@ class Context:
     def to_device(self, arr, dtype=None):
         return DeviceArray.from_host(self, arr, dtype)
 
+    def copy(self, dst, src, nbytes):
+        """Device-to-device copy by the library's streaming copy kernel (16 B
+        per lane); the reference point for bandwidth and counter calibration."""
+        self._check(self.L.phg_copy(self.h, dst.ptr, src.ptr, nbytes))
+
     # profiling
     def profile(self, on=True):
         self._check(self.L.phg_profile_enable(self.h, 1 if on else 0))
