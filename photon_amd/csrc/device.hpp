@@ -32,11 +32,31 @@ template <> __device__ __forceinline__ bf16_t from_f<bf16_t>(float v) {
     return r;
 }
 
+// Global-memory access.  Pointers read from a descriptor are generic, and a
+// generic access compiles to a flat instruction, which counts against both the
+// vector-memory and the LDS wait counters: every LDS wait would then also wait
+// for outstanding loads.  These casts give the global address space, so the
+// compiler emits global_load / global_store.
+#define PHG_GLOBAL __attribute__((address_space(1)))
+template <class T> __device__ __forceinline__ T gld(const T* p, long i) {
+    return ((const PHG_GLOBAL T*)p)[i];
+}
+template <class T> __device__ __forceinline__ void gst(T* p, long i, T v) { ((PHG_GLOBAL T*)p)[i] = v; }
+// BF16 is a struct of raw bits: access the bits
+template <> __device__ __forceinline__ bf16_t gld(const bf16_t* p, long i) {
+    bf16_t r;
+    r.u = ((const PHG_GLOBAL uint16_t*)p)[i];
+    return r;
+}
+template <> __device__ __forceinline__ void gst(bf16_t* p, long i, bf16_t v) {
+    ((PHG_GLOBAL uint16_t*)p)[i] = v.u;
+}
+
 template <class T> __device__ __forceinline__ float ld(const void* p, long i) {
-    return to_f(static_cast<const T*>(p)[i]);
+    return to_f(gld(static_cast<const T*>(p), i));
 }
 template <class T> __device__ __forceinline__ void st(void* p, long i, float v) {
-    static_cast<T*>(p)[i] = from_f<T>(v);
+    gst(static_cast<T*>(p), i, from_f<T>(v));
 }
 
 // ---- descriptors -----------------------------------------------------------

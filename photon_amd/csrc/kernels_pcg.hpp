@@ -47,7 +47,7 @@ template <class CT>
 __device__ __forceinline__ void coefs6(const DLevel& L, const void* ax, const void* ay, const void* az, const Cell& q,
                                        float a[6]) {
     if constexpr (std::is_same<CT, Mask01>::value) {
-        const unsigned b = L.mask[q.c];
+        const unsigned b = gld(L.mask, q.c);
 #pragma unroll
         for (int d = 0; d < 6; ++d) a[d] = (float)((b >> d) & 1u);
     } else {
@@ -64,6 +64,32 @@ __device__ __forceinline__ void coefs6(const DLevel& L, const void* ax, const vo
 }
 
 __device__ __forceinline__ float diag6(const float a[6]) { return a[0] + a[1] + a[2] + a[3] + a[4] + a[5]; }
+
+// a cell's coefficients held in registers between their load and their use
+// (the 1-byte mask, or the six values)
+template <class CT>
+struct CoefRegs {
+    float a[6];
+    __device__ __forceinline__ void load(const DLevel& L, const void* ax, const void* ay, const void* az,
+                                         const Cell& q) {
+        coefs6<CT>(L, ax, ay, az, q, a);
+    }
+    __device__ __forceinline__ void get(float out[6]) const {
+#pragma unroll
+        for (int d = 0; d < 6; ++d) out[d] = a[d];
+    }
+};
+template <>
+struct CoefRegs<Mask01> {
+    unsigned m;
+    __device__ __forceinline__ void load(const DLevel& L, const void*, const void*, const void*, const Cell& q) {
+        m = gld(L.mask, q.c);
+    }
+    __device__ __forceinline__ void get(float out[6]) const {
+#pragma unroll
+        for (int d = 0; d < 6; ++d) out[d] = (float)((m >> d) & 1u);
+    }
+};
 
 // sum over existing neighbours of a_f * v(nb) (a missing neighbour is the
 // Dirichlet boundary, value 0); v is a functor of the neighbour index.
@@ -103,65 +129,119 @@ __device__ __forceinline__ ColTile col_tile(const DSys& s, int tile) {
 // plane march with LDS-staged lateral neighbours
 // ---------------------------------------------------------------------------
 // A block owns the TB columns [c0, c0 + TB) of one k-chunk.  For every plane
-// k it stages val() of its columns, and of one grid row on either side (the
-// -nx / +nx neighbours), in LDS; the four lateral neighbours are then read
-// from LDS and the two vertical ones come from a register pipeline along k.
-// Each staged value is loaded from global memory once per plane instead of
-// up to seven times.  lds: 2 * (TB + 2 nx) floats, the plane buffer being
-// double-buffered so that one barrier per plane suffices.
-// Loads are software-pipelined two planes ahead (own column and the first
-// MARCH_HALO * TB halo values), so a plane's barrier does not wait on a
-// global-memory round trip.  The halo of a plane is 2 nx values; any beyond the
-// pipelined part (nx > MARCH_HALO * TB / 2) are loaded in the plane itself.
+// k it stages the value of its columns, and of one grid row on either side
+// (the -nx / +nx neighbours), in LDS; the four lateral neighbours are then
+// read from LDS and the two vertical ones come from a register pipeline along
+// k.  Each staged value is loaded from global memory once per plane instead of
+// up to seven times.  lds: 2 * (TB + 2 nx) floats, double-buffered so that
+// one barrier per plane suffices.
+//
+// The kernel supplies
+//   raw(n)  -> R   the global loads behind the value of cell n (no arithmetic),
+//   fin(R)  -> float  the value from them,
+//   pre(q)  -> P   the cell's own operands (coefficients, r, ...),
+//   body(k, v(k-1), v(k), v(k+1), &plane[own column], P).
+// Raw planes are loaded three planes ahead and operands two ahead, with
+// clamped in-bounds indices and no branch around any load, so the compiler's
+// waits let each plane's loads overlap the work of the planes before it.
+// Halo values beyond the first MARCH_HALO * TB (nx > MARCH_HALO * TB / 2) are
+// loaded in the plane itself.
 constexpr int MARCH_HALO = 2;
 constexpr int MARCH_MAX_NX = 2048;   // LDS plane buffers: 2 (TB + 2 nx) floats
 
-template <class VAL, class BODY>
-__device__ __forceinline__ void march(const DSys& s, const ColTile& t, float* lds, VAL val, BODY body) {
+template <class RAW, class FIN, class PRE, class BODY>
+__device__ __forceinline__ void march(const DSys& s, const ColTile& t, float* lds, RAW raw, FIN fin, PRE pre,
+                                      BODY body) {
     const DLevel& L = s.L[0];
     const int nx = L.nx, nz = L.nz, sxy = nx * L.ny;
     const int W = TB + 2 * nx;
     const int c0 = t.c2 - (int)threadIdx.x;
-    auto own = [&](int k) { return (t.valid && k >= 0 && k < nz) ? val(t.c2 + sxy * k) : 0.0f; };
-    auto halo = [&](int k, float (&h)[MARCH_HALO]) {
+    using R = decltype(raw(0));
+    using P = decltype(pre(Cell{}));
+    struct Plane {
+        R own;
+        R h[MARCH_HALO];
+    };
+    const int cown = t.valid ? t.c2 : sxy - 1;   // any in-bounds column for idle lanes
+    int hcol[MARCH_HALO];
+    bool hin[MARCH_HALO];
+#pragma unroll
+    for (int m = 0; m < MARCH_HALO; ++m) {
+        const int hh = (int)threadIdx.x + m * TB;
+        const int col = hh < nx ? c0 - nx + hh : c0 + TB + (hh - nx);
+        hin[m] = hh < 2 * nx && col >= 0 && col < sxy;
+        hcol[m] = hin[m] ? col : cown;
+    }
+    auto kcl = [&](int k) { return min(max(k, 0), nz - 1); };
+    auto load = [&](int k, Plane& pl) {
+        const long base = (long)sxy * kcl(k);
+        pl.own = raw(cown + base);
+#pragma unroll
+        for (int m = 0; m < MARCH_HALO; ++m) pl.h[m] = raw(hcol[m] + base);
+    };
+    auto fin_own = [&](int k, const R& r) {
+        const float v = fin(r);
+        return (t.valid && k >= 0 && k < nz) ? v : 0.0f;
+    };
+    auto fin_halo = [&](int k, const Plane& pl, float (&h)[MARCH_HALO]) {
 #pragma unroll
         for (int m = 0; m < MARCH_HALO; ++m) {
-            const int hh = (int)threadIdx.x + m * TB;
-            const int col = hh < nx ? c0 - nx + hh : c0 + TB + (hh - nx);
-            h[m] = (hh < 2 * nx && k < nz && col >= 0 && col < sxy) ? val(col + sxy * k) : 0.0f;
+            const float v = fin(pl.h[m]);
+            h[m] = (hin[m] && k < nz) ? v : 0.0f;
         }
     };
-    float vprev = own(t.k0 - 1), vcur = own(t.k0), vnext = own(t.k0 + 1);
-    float hcur[MARCH_HALO], hnext[MARCH_HALO];
-    halo(t.k0, hcur);
-    halo(t.k0 + 1, hnext);
-    for (int k = t.k0; k < t.k1; ++k) {
+    // idle lanes (columns past the grid) load the last column's operands
+    const int oi = t.valid ? t.i : nx - 1, oj = t.valid ? t.j : L.ny - 1;
+    auto opnd = [&](int k) { return pre(Cell{cown + sxy * kcl(k), oi, oj, kcl(k)}); };
+
+    // Register rings, unrolled by three so that no register that a load is
+    // still filling is ever copied (a copy would make the wave wait for it):
+    // raw planes X0..X2 hold k+1, k+2 and receive k+3; operands P0..P2 hold k,
+    // k+1 and receive k+2.
+    Plane X0, X1, X2;
+    float vprev, vcur, hfin[MARCH_HALO];
+    {
+        Plane Z;
+        load(t.k0 - 1, Z);
+        vprev = fin_own(t.k0 - 1, Z.own);
+        load(t.k0, Z);
+        vcur = fin_own(t.k0, Z.own);
+        fin_halo(t.k0, Z, hfin);
+    }
+    load(t.k0 + 1, X0);
+    load(t.k0 + 2, X1);
+    P P0 = opnd(t.k0), P1 = opnd(t.k0 + 1), P2{};
+    auto step = [&](int k, const Plane& A, Plane& Cdst, const P& pcur, P& pdst) {
         float* buf = lds + (k & 1) * W;
         buf[nx + threadIdx.x] = vcur;   // 0 for columns past the grid
 #pragma unroll
         for (int m = 0; m < MARCH_HALO; ++m) {
             const int hh = (int)threadIdx.x + m * TB;
-            if (hh < 2 * nx) buf[hh < nx ? hh : TB + hh] = hcur[m];
+            if (hh < 2 * nx) buf[hh < nx ? hh : TB + hh] = hfin[m];
         }
         for (int hh = (int)threadIdx.x + MARCH_HALO * TB; hh < 2 * nx; hh += TB) {
             const int col = hh < nx ? c0 - nx + hh : c0 + TB + (hh - nx);
-            buf[hh < nx ? hh : TB + hh] = (col >= 0 && col < sxy) ? val(col + sxy * k) : 0.0f;
+            buf[hh < nx ? hh : TB + hh] = (col >= 0 && col < sxy) ? fin(raw(col + (long)sxy * k)) : 0.0f;
         }
         __syncthreads();
-        // issue the loads of plane k + 2 before this plane's work
-        const bool more = k + 1 < t.k1;
-        const float vnn = more ? own(k + 2) : 0.0f;
-        float hnn[MARCH_HALO];
-        if (more) halo(k + 2, hnn);
-        if (t.valid) body(k, vprev, vcur, vnext, (const float*)(buf + nx + threadIdx.x));
+        const float vnext = fin_own(k + 1, A.own);
+        float hnext[MARCH_HALO];
+        fin_halo(k + 1, A, hnext);
+        // no branch around loads (a branch would merge the wait counters and
+        // force an early wait): past the chunk they repeat the last plane
+        // needed, which the caches serve
+        load(min(k + 3, t.k1), Cdst);
+        pdst = opnd(min(k + 2, t.k1 - 1));
+        if (t.valid) body(k, vprev, vcur, vnext, (const float*)(buf + nx + threadIdx.x), pcur);
         vprev = vcur;
         vcur = vnext;
-        vnext = vnn;
 #pragma unroll
-        for (int m = 0; m < MARCH_HALO; ++m) {
-            hcur[m] = hnext[m];
-            hnext[m] = more ? hnn[m] : 0.0f;
-        }
+        for (int m = 0; m < MARCH_HALO; ++m) hfin[m] = hnext[m];
+    };
+    for (int k = t.k0; k < t.k1; k += 3) {
+        step(k, X0, X2, P0, P2);
+        if (k + 1 < t.k1) step(k + 1, X1, X0, P1, P0);
+        if (k + 2 < t.k1) step(k + 2, X2, X1, P2, P1);
     }
 }
 
@@ -173,6 +253,22 @@ __device__ __forceinline__ float lat_sum(const Cell& q, int nx, int ny, int nz, 
     const bool e0 = q.i > 0, e1 = q.i < nx - 1, e2 = q.j > 0, e3 = q.j < ny - 1, e4 = q.k > 0, e5 = q.k < nz - 1;
     return (e0 ? a[0] : 0.0f) * p[-1] + (e1 ? a[1] : 0.0f) * p[1] + (e2 ? a[2] : 0.0f) * p[-nx] +
            (e3 ? a[3] : 0.0f) * p[nx] + (e4 ? a[4] : 0.0f) * vp + (e5 ? a[5] : 0.0f) * vn;
+}
+
+// a cell's V-cycle coefficients and CG residual, loaded ahead by the march
+template <class CT>
+struct CoefR {
+    CoefRegs<CT> cf;
+    float r;
+};
+template <class CT>
+__device__ __forceinline__ auto cr_pre(const DSys& s, const DLevel& L) {
+    return [&s, &L](const Cell& q) {
+        CoefR<CT> o;
+        o.cf.load(L, L.axc, L.ayc, L.azc, q);
+        o.r = gld(s.r, q.c);
+        return o;
+    };
 }
 
 #define PHG_SYS_PROLOGUE                  \
@@ -238,19 +334,32 @@ __global__ void __launch_bounds__(TB) k_dir_apply(const DSys* __restrict__ sys, 
     const double beta = s.scal[S_BETA], alpha = s.scal[S_ALPHA];
     const void* z = s.z;
     const float izs = (float)(1.0 / s.scal[S_ZS]);   // z is stored scaled by S_ZS (a power of two)
-    auto dval = [&](int n) { return first ? ld<ZT>(z, n) * izs : ld<ZT>(z, n) * izs + (float)(beta * dold[n]); };
+    // d = z + beta d_old (d = z on the first iteration, where d_old is not read)
+    auto raw = [&](long n) { return float2{ld<ZT>(z, n), gld(dold, n)}; };
+    auto fin = [&](const float2& v) { return first ? v.x * izs : v.x * izs + (float)(beta * v.y); };
     extern __shared__ float lds[];
     double acc = 0.0;
-    march(s, t, lds, dval, [&](int k, float vp, float dc, float vn, const float* p) {
+    struct Op {
+        CoefRegs<OT> cf;
+        float x, dold;
+    };
+    auto pre = [&](const Cell& q) {
+        Op o;
+        o.cf.load(L, L.ax32, L.ay32, L.az32, q);
+        o.x = gld(s.x, q.c);
+        o.dold = gld(dold, q.c);
+        return o;
+    };
+    march(s, t, lds, raw, fin, pre, [&](int k, float vp, float dc, float vn, const float* p, const Op& o) {
         const Cell q{t.c2 + sxy * k, t.i, t.j, k};
         float a[6];
-        coefs6<OT>(L, L.ax32, L.ay32, L.az32, q, a);
+        o.cf.get(a);
         const float dg = diag6(a);
         float y = 0.0f;
         if (dg > 0) y = dg * dc - lat_sum(q, L.nx, L.ny, L.nz, a, p, vp, vn);
-        if (!first) s.x[q.c] += (float)(alpha * dold[q.c]);
-        dnew[q.c] = dc;
-        s.q[q.c] = y;
+        if (!first) gst(s.x, q.c, o.x + (float)(alpha * o.dold));
+        gst(dnew, q.c, dc);
+        gst(s.q, q.c, y);
         acc += (double)y * dc;
     });
     double v[1] = {acc};
@@ -430,20 +539,20 @@ __global__ void __launch_bounds__(TB) k_pre_red(const DSys* __restrict__ sys, co
     if (t.valid) {
         for (int k = t.k0; k < t.k1; ++k) {
             const int c = t.c2 + sxy * k;
-            float r = s.r[c];
+            float r = gld(s.r, c);
             if (!first) {
-                r -= (float)(alpha * s.q[c]);
-                s.r[c] = r;
+                r -= (float)(alpha * gld(s.q, c));
+                gst(s.r, c, r);
                 v[0] += (double)r * r;
                 m = fmaxf(m, fabsf(r));
             }
             float z = 0.0f;
             if (((t.i + t.j + k) & 1) == 0) {
                 if constexpr (std::is_same<CT, Mask01>::value) {
-                    const int dg = __popc((unsigned)L.mask[c]);
+                    const int dg = __popc((unsigned)gld(L.mask, c));
                     if (dg > 0) z = (r * zs) / (float)dg;
                 } else {
-                    z = (r * zs) * s.invd[c];
+                    z = (r * zs) * gld(s.invd, c);
                 }
             }
             st<ZT>(s.z, c, z);
@@ -476,14 +585,16 @@ __global__ void __launch_bounds__(TB) k_rb_black(const DSys* __restrict__ sys, c
     const void* z = s.z;
     const float zs = (float)s.scal[S_ZS];
     extern __shared__ float lds[];
-    march(s, t, lds, [&](int n) { return ld<ZT>(z, n); }, [&](int k, float vp, float vc, float vn, const float* p) {
+    auto raw = [&](long n) { return ld<ZT>(z, n); };
+    march(s, t, lds, raw, [](float v) { return v; }, cr_pre<CT>(s, L), [&](int k, float vp, float vc, float vn, const float* p,
+                                                                  const CoefR<CT>& o) {
         if (((t.i + t.j + k) & 1) == 0) return;
         const Cell q{t.c2 + sxy * k, t.i, t.j, k};
         float a[6];
-        coefs6<CT>(L, L.axc, L.ayc, L.azc, q, a);
+        o.cf.get(a);
         const float dg = diag6(a);
         float zc = 0.0f;
-        if (dg > 0) zc = (s.r[q.c] * zs + lat_sum(q, L.nx, L.ny, L.nz, a, p, vp, vn)) / dg;
+        if (dg > 0) zc = (o.r * zs + lat_sum(q, L.nx, L.ny, L.nz, a, p, vp, vn)) / dg;
         st<ZT>(s.z, q.c, zc);
     });
 }
@@ -498,13 +609,15 @@ __global__ void __launch_bounds__(TB) k_resid0(const DSys* __restrict__ sys, con
     const void* z = s.z;
     const float zs = (float)s.scal[S_ZS];
     extern __shared__ float lds[];
-    march(s, t, lds, [&](int n) { return ld<ZT>(z, n); }, [&](int k, float vp, float vc, float vn, const float* p) {
+    auto raw = [&](long n) { return ld<ZT>(z, n); };
+    march(s, t, lds, raw, [](float v) { return v; }, cr_pre<CT>(s, L), [&](int k, float vp, float vc, float vn, const float* p,
+                                                                  const CoefR<CT>& o) {
         const Cell q{t.c2 + sxy * k, t.i, t.j, k};
         float a[6];
-        coefs6<CT>(L, L.axc, L.ayc, L.azc, q, a);
+        o.cf.get(a);
         const float dg = diag6(a);
         float r = 0.0f;
-        if (dg > 0) r = s.r[q.c] * zs - dg * vc + lat_sum(q, L.nx, L.ny, L.nz, a, p, vp, vn);
+        if (dg > 0) r = o.r * zs - dg * vc + lat_sum(q, L.nx, L.ny, L.nz, a, p, vp, vn);
         st<VT>(L.r, q.c, r);
     });
 }
@@ -522,20 +635,28 @@ __global__ void __launch_bounds__(TB) k_post_black(const DSys* __restrict__ sys,
     const void* z = s.z;
     const void* t1 = s.L[1].t;
     const int* __restrict__ agg = L.agg;
-    auto zp = [&](int n) {
-        const int g = agg[n];
-        return ld<ZT>(z, n) + (g >= 0 ? ld<VT>(t1, g) : 0.0f);
+    // prolonged value z + t1[agg]: the aggregate index is loaded with z, the
+    // level-1 value (a small, cache-resident vector) when the value is formed
+    struct ZG {
+        float z;
+        int g;
+    };
+    auto raw = [&](long n) { return ZG{ld<ZT>(z, n), gld(agg, n)}; };
+    auto fin = [&](const ZG& v) {
+        const float c = ld<VT>(t1, max(v.g, 0));
+        return v.z + (v.g >= 0 ? c : 0.0f);
     };
     const float zs = (float)s.scal[S_ZS];
     extern __shared__ float lds[];
-    march(s, t, lds, zp, [&](int k, float vp, float vc, float vn, const float* p) {
+    march(s, t, lds, raw, fin, cr_pre<CT>(s, L), [&](int k, float vp, float vc, float vn, const float* p,
+                                                     const CoefR<CT>& o) {
         if (((t.i + t.j + k) & 1) == 0) return;
         const Cell q{t.c2 + sxy * k, t.i, t.j, k};
         float a[6];
-        coefs6<CT>(L, L.axc, L.ayc, L.azc, q, a);
+        o.cf.get(a);
         const float dg = diag6(a);
         float zc = 0.0f;
-        if (dg > 0) zc = (s.r[q.c] * zs + lat_sum(q, L.nx, L.ny, L.nz, a, p, vp, vn)) / dg;
+        if (dg > 0) zc = (o.r * zs + lat_sum(q, L.nx, L.ny, L.nz, a, p, vp, vn)) / dg;
         st<ZT>(s.z, q.c, zc);
     });
 }
@@ -553,20 +674,32 @@ __global__ void __launch_bounds__(TB) k_post_red(const DSys* __restrict__ sys, c
     const bool with_q = s.state[ST_FLEX] != 0 && s.state[ST_VC] != 0;   // q exists once an iteration has run
     double v[1] = {0.0}, vq = 0.0;
     extern __shared__ float lds[];
-    march(s, t, lds, [&](int n) { return ld<VT>(z, n); }, [&](int k, float vp, float vc, float vn, const float* p) {
+    struct Op {
+        CoefRegs<CT> cf;
+        float r, q;
+    };
+    auto pre = [&](const Cell& q) {
+        Op o;
+        o.cf.load(L, L.axc, L.ayc, L.azc, q);
+        o.r = gld(s.r, q.c);
+        o.q = with_q ? gld(s.q, q.c) : 0.0f;
+        return o;
+    };
+    auto raw = [&](long n) { return ld<VT>(z, n); };
+    march(s, t, lds, raw, [](float v) { return v; }, pre, [&](int k, float vp, float vc, float vn, const float* p, const Op& o) {
         const Cell q{t.c2 + sxy * k, t.i, t.j, k};
-        const float r = s.r[q.c];
+        const float r = o.r;
         float zc = vc;   // black cells: final from the black sweep
         if (((t.i + t.j + k) & 1) == 0) {
             float a[6];
-            coefs6<CT>(L, L.axc, L.ayc, L.azc, q, a);
+            o.cf.get(a);
             const float dg = diag6(a);
             zc = 0.0f;
             if (dg > 0) zc = (r * zs + lat_sum(q, L.nx, L.ny, L.nz, a, p, vp, vn)) / dg;
             st<VT>(s.z, q.c, zc);
         }
         v[0] += (double)r * zc;
-        if (with_q) vq += (double)s.q[q.c] * zc;
+        if (with_q) vq += (double)o.q * zc;
     });
     double* out = s.part + (long)e.y * NPART;
     __shared__ double sh[2][TB / 64];
