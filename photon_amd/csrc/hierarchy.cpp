@@ -232,6 +232,13 @@ void dense_inverse(HostHierarchy& H) {
             for (size_t b2 = 0; b2 < m.size(); ++b2)
                 H.cinv[(size_t)m[a] * n + m[b2]] += total - rowmean[a] - rowmean[b2];   // symmetric: col mean = row mean
     }
+    // exactly symmetric (the update above rounds the two halves differently):
+    // the device reads column i as row i
+    for (int i = 0; i < n; ++i)
+        for (int j = 0; j < i; ++j) {
+            const double v = 0.5 * (H.cinv[(size_t)i * n + j] + H.cinv[(size_t)j * n + i]);
+            H.cinv[(size_t)i * n + j] = H.cinv[(size_t)j * n + i] = v;
+        }
 }
 
 }  // namespace

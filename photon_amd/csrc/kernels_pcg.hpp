@@ -138,6 +138,7 @@ __device__ __forceinline__ ColTile col_tile(const DSys& s, int tile) {
 //
 // The kernel supplies
 //   raw(n)  -> R   the global loads behind the value of cell n (no arithmetic),
+//   mid(R&)        loads that depend on raw's (issued a plane before use),
 //   fin(R)  -> float  the value from them,
 //   pre(q)  -> P   the cell's own operands (coefficients, r, ...),
 //   body(k, v(k-1), v(k), v(k+1), &plane[own column], P).
@@ -149,8 +150,8 @@ __device__ __forceinline__ ColTile col_tile(const DSys& s, int tile) {
 constexpr int MARCH_HALO = 2;
 constexpr int MARCH_MAX_NX = 2048;   // LDS plane buffers: 2 (TB + 2 nx) floats
 
-template <class RAW, class FIN, class PRE, class BODY>
-__device__ __forceinline__ void march(const DSys& s, const ColTile& t, float* lds, RAW raw, FIN fin, PRE pre,
+template <class RAW, class MID, class FIN, class PRE, class BODY>
+__device__ __forceinline__ void march(const DSys& s, const ColTile& t, float* lds, RAW raw, MID mid, FIN fin, PRE pre,
                                       BODY body) {
     const DLevel& L = s.L[0];
     const int nx = L.nx, nz = L.nz, sxy = nx * L.ny;
@@ -179,6 +180,11 @@ __device__ __forceinline__ void march(const DSys& s, const ColTile& t, float* ld
 #pragma unroll
         for (int m = 0; m < MARCH_HALO; ++m) pl.h[m] = raw(hcol[m] + base);
     };
+    auto mid_plane = [&](Plane& pl) {
+        mid(pl.own);
+#pragma unroll
+        for (int m = 0; m < MARCH_HALO; ++m) mid(pl.h[m]);
+    };
     auto fin_own = [&](int k, const R& r) {
         const float v = fin(r);
         return (t.valid && k >= 0 && k < nz) ? v : 0.0f;
@@ -203,15 +209,18 @@ __device__ __forceinline__ void march(const DSys& s, const ColTile& t, float* ld
     {
         Plane Z;
         load(t.k0 - 1, Z);
+        mid_plane(Z);
         vprev = fin_own(t.k0 - 1, Z.own);
         load(t.k0, Z);
+        mid_plane(Z);
         vcur = fin_own(t.k0, Z.own);
         fin_halo(t.k0, Z, hfin);
     }
     load(t.k0 + 1, X0);
     load(t.k0 + 2, X1);
+    mid_plane(X0);
     P P0 = opnd(t.k0), P1 = opnd(t.k0 + 1), P2{};
-    auto step = [&](int k, const Plane& A, Plane& Cdst, const P& pcur, P& pdst) {
+    auto step = [&](int k, const Plane& A, Plane& B, Plane& Cdst, const P& pcur, P& pdst) {
         float* buf = lds + (k & 1) * W;
         buf[nx + threadIdx.x] = vcur;   // 0 for columns past the grid
 #pragma unroll
@@ -221,7 +230,13 @@ __device__ __forceinline__ void march(const DSys& s, const ColTile& t, float* ld
         }
         for (int hh = (int)threadIdx.x + MARCH_HALO * TB; hh < 2 * nx; hh += TB) {
             const int col = hh < nx ? c0 - nx + hh : c0 + TB + (hh - nx);
-            buf[hh < nx ? hh : TB + hh] = (col >= 0 && col < sxy) ? fin(raw(col + (long)sxy * k)) : 0.0f;
+            float v = 0.0f;
+            if (col >= 0 && col < sxy) {
+                R rv = raw(col + (long)sxy * k);
+                mid(rv);
+                v = fin(rv);
+            }
+            buf[hh < nx ? hh : TB + hh] = v;
         }
         __syncthreads();
         const float vnext = fin_own(k + 1, A.own);
@@ -232,6 +247,7 @@ __device__ __forceinline__ void march(const DSys& s, const ColTile& t, float* ld
         // needed, which the caches serve
         load(min(k + 3, t.k1), Cdst);
         pdst = opnd(min(k + 2, t.k1 - 1));
+        mid_plane(B);   // plane k + 2
         if (t.valid) body(k, vprev, vcur, vnext, (const float*)(buf + nx + threadIdx.x), pcur);
         vprev = vcur;
         vcur = vnext;
@@ -239,9 +255,9 @@ __device__ __forceinline__ void march(const DSys& s, const ColTile& t, float* ld
         for (int m = 0; m < MARCH_HALO; ++m) hfin[m] = hnext[m];
     };
     for (int k = t.k0; k < t.k1; k += 3) {
-        step(k, X0, X2, P0, P2);
-        if (k + 1 < t.k1) step(k + 1, X1, X0, P1, P0);
-        if (k + 2 < t.k1) step(k + 2, X2, X1, P2, P1);
+        step(k, X0, X1, X2, P0, P2);
+        if (k + 1 < t.k1) step(k + 1, X1, X2, X0, P1, P0);
+        if (k + 2 < t.k1) step(k + 2, X2, X0, X1, P2, P1);
     }
 }
 
@@ -350,7 +366,7 @@ __global__ void __launch_bounds__(TB) k_dir_apply(const DSys* __restrict__ sys, 
         o.dold = gld(dold, q.c);
         return o;
     };
-    march(s, t, lds, raw, fin, pre, [&](int k, float vp, float dc, float vn, const float* p, const Op& o) {
+    march(s, t, lds, raw, [](auto&) {}, fin, pre, [&](int k, float vp, float dc, float vn, const float* p, const Op& o) {
         const Cell q{t.c2 + sxy * k, t.i, t.j, k};
         float a[6];
         o.cf.get(a);
@@ -586,7 +602,7 @@ __global__ void __launch_bounds__(TB) k_rb_black(const DSys* __restrict__ sys, c
     const float zs = (float)s.scal[S_ZS];
     extern __shared__ float lds[];
     auto raw = [&](long n) { return ld<ZT>(z, n); };
-    march(s, t, lds, raw, [](float v) { return v; }, cr_pre<CT>(s, L), [&](int k, float vp, float vc, float vn, const float* p,
+    march(s, t, lds, raw, [](auto&) {}, [](float v) { return v; }, cr_pre<CT>(s, L), [&](int k, float vp, float vc, float vn, const float* p,
                                                                   const CoefR<CT>& o) {
         if (((t.i + t.j + k) & 1) == 0) return;
         const Cell q{t.c2 + sxy * k, t.i, t.j, k};
@@ -610,7 +626,7 @@ __global__ void __launch_bounds__(TB) k_resid0(const DSys* __restrict__ sys, con
     const float zs = (float)s.scal[S_ZS];
     extern __shared__ float lds[];
     auto raw = [&](long n) { return ld<ZT>(z, n); };
-    march(s, t, lds, raw, [](float v) { return v; }, cr_pre<CT>(s, L), [&](int k, float vp, float vc, float vn, const float* p,
+    march(s, t, lds, raw, [](auto&) {}, [](float v) { return v; }, cr_pre<CT>(s, L), [&](int k, float vp, float vc, float vn, const float* p,
                                                                   const CoefR<CT>& o) {
         const Cell q{t.c2 + sxy * k, t.i, t.j, k};
         float a[6];
@@ -640,15 +656,14 @@ __global__ void __launch_bounds__(TB) k_post_black(const DSys* __restrict__ sys,
     struct ZG {
         float z;
         int g;
+        float c;
     };
-    auto raw = [&](long n) { return ZG{ld<ZT>(z, n), gld(agg, n)}; };
-    auto fin = [&](const ZG& v) {
-        const float c = ld<VT>(t1, max(v.g, 0));
-        return v.z + (v.g >= 0 ? c : 0.0f);
-    };
+    auto raw = [&](long n) { return ZG{ld<ZT>(z, n), gld(agg, n), 0.0f}; };
+    auto mid = [&](ZG& v) { v.c = ld<VT>(t1, max(v.g, 0)); };
+    auto fin = [&](const ZG& v) { return v.z + (v.g >= 0 ? v.c : 0.0f); };
     const float zs = (float)s.scal[S_ZS];
     extern __shared__ float lds[];
-    march(s, t, lds, raw, fin, cr_pre<CT>(s, L), [&](int k, float vp, float vc, float vn, const float* p,
+    march(s, t, lds, raw, mid, fin, cr_pre<CT>(s, L), [&](int k, float vp, float vc, float vn, const float* p,
                                                      const CoefR<CT>& o) {
         if (((t.i + t.j + k) & 1) == 0) return;
         const Cell q{t.c2 + sxy * k, t.i, t.j, k};
@@ -686,7 +701,7 @@ __global__ void __launch_bounds__(TB) k_post_red(const DSys* __restrict__ sys, c
         return o;
     };
     auto raw = [&](long n) { return ld<VT>(z, n); };
-    march(s, t, lds, raw, [](float v) { return v; }, pre, [&](int k, float vp, float vc, float vn, const float* p, const Op& o) {
+    march(s, t, lds, raw, [](auto&) {}, [](float v) { return v; }, pre, [&](int k, float vp, float vc, float vn, const float* p, const Op& o) {
         const Cell q{t.c2 + sxy * k, t.i, t.j, k};
         const float r = o.r;
         float zc = vc;   // black cells: final from the black sweep
@@ -822,16 +837,24 @@ __global__ void __launch_bounds__(TB) k_vc_prolong(const DSys* __restrict__ sys,
     st<VT>(L.t, I, ld<VT>(L.t, I) + ld<VT>(s.L[l + 1].t, L.agg[I]));
 }
 
-// coarsest level: t = (grounded A_c)^-1 b in FP64.  A block takes up to
-// CG_SYS systems that share one grid (so each row of the inverse is read once
-// for all of them) and CG_ROWS rows; each wave handles CG_ROWS/4 rows with its
-// 64 lanes striding the row, so the inverse is read coalesced.
+// coarsest level: t = B b in FP64, B the projected inverse of the grounded
+// coarsest operator (symmetric, so column i is read as row i).  A block takes
+// up to CGS systems that share one grid, so each entry of B is read once for
+// all of them, and CG_ROWS rows: lane = row, B read coalesced along a column,
+// the CGS right-hand sides broadcast from LDS ([n][CGS] floats: the V-cycle
+// vectors hold at most FP32, so the copy is exact).  The four waves split the
+// column range and their partials are added in a fixed order.
 // The reference approximates this solve with 20 forward/backward Gauss-Seidel
 // sweeps (pressure_mg.cpp:277-281); a serial sweep has no place on a GPU.
-constexpr int CG_SYS = 8;
-constexpr int CG_ROWS = 32;
+constexpr int CG_ROWS = 64;
+constexpr int CG_SYS_MAX = 16;
 
-template <class VT>
+// LDS bytes of k_vc_coarse for `cgs` systems of up to `n` unknowns
+__host__ __device__ constexpr size_t coarse_lds(int cgs, int n) {
+    return (size_t)n * cgs * 4 > (size_t)3 * CG_ROWS * cgs * 8 ? (size_t)n * cgs * 4 : (size_t)3 * CG_ROWS * cgs * 8;
+}
+
+template <class VT, int CGS>
 __global__ void __launch_bounds__(TB) k_vc_coarse(const DSys* __restrict__ sys, const int4* __restrict__ tab) {
     const int4 e = tab[blockIdx.x];   // first system, count, row tile
     const DSys& s0 = sys[e.x];
@@ -840,32 +863,52 @@ __global__ void __launch_bounds__(TB) k_vc_coarse(const DSys* __restrict__ sys, 
     int live = 0;
     for (int g = 0; g < e.y; ++g) live |= sys[e.x + g].state[ST_ACTIVE];
     if (!live) return;
-    extern __shared__ double bsh[];   // [count][n]
-    for (int g = 0; g < e.y; ++g) {
-        const DSys& s = sys[e.x + g];
-        const void* b = s.L[s.nl - 1].b;
-        for (int j = threadIdx.x; j < n; j += TB) bsh[g * n + j] = (double)ld<VT>(b, j);
+    extern __shared__ double dsh[];
+    float* bsh = reinterpret_cast<float*>(dsh);   // [n][CGS]
+    for (int g = 0; g < CGS; ++g) {
+        if (g < e.y) {
+            const DSys& s = sys[e.x + g];
+            const void* b = s.L[s.nl - 1].b;
+            for (int j = threadIdx.x; j < n; j += TB) bsh[j * CGS + g] = ld<VT>(b, j);
+        } else {
+            for (int j = threadIdx.x; j < n; j += TB) bsh[j * CGS + g] = 0.0f;
+        }
     }
     __syncthreads();
     const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
-    for (int rr = 0; rr < CG_ROWS / 4; ++rr) {
-        const int i = e.z * CG_ROWS + w * (CG_ROWS / 4) + rr;
-        if (i >= n) break;
-        double acc[CG_SYS];
+    const int i = e.z * CG_ROWS + lane;
+    const int ic = i < n ? i : n - 1;
+    const int jper = (n + 3) / 4, j0 = w * jper, j1 = min(n, j0 + jper);
+    double acc[CGS];
 #pragma unroll
-        for (int g = 0; g < CG_SYS; ++g) acc[g] = 0.0;
-        for (int j = lane; j < n; j += 64) {
-            const double a = A[(long)i * n + j];
+    for (int g = 0; g < CGS; ++g) acc[g] = 0.0;
+    for (int j = j0; j < j1; ++j) {
+        const double a = gld(A, (long)j * n + ic);
+        const float4* bv = reinterpret_cast<const float4*>(bsh + j * CGS);
 #pragma unroll
-            for (int g = 0; g < CG_SYS; ++g)
-                if (g < e.y) acc[g] += a * bsh[g * n + j];
+        for (int q = 0; q < CGS / 4; ++q) {
+            const float4 b4 = bv[q];
+            acc[4 * q + 0] += a * (double)b4.x;
+            acc[4 * q + 1] += a * (double)b4.y;
+            acc[4 * q + 2] += a * (double)b4.z;
+            acc[4 * q + 3] += a * (double)b4.w;
         }
+    }
+    __syncthreads();   // bsh is reused for the partials of waves 1..3
+    double* red = dsh;   // [3][CG_ROWS][CGS]
+    if (w > 0) {
 #pragma unroll
-        for (int g = 0; g < CG_SYS; ++g) {
+        for (int g = 0; g < CGS; ++g) red[((w - 1) * CG_ROWS + lane) * CGS + g] = acc[g];
+    }
+    __syncthreads();
+    if (w == 0 && i < n) {
+#pragma unroll
+        for (int g = 0; g < CGS; ++g) {
             if (g >= e.y) break;
-            const double v = wave_sum(acc[g]);
+            const double v = acc[g] + red[(0 * CG_ROWS + lane) * CGS + g] + red[(1 * CG_ROWS + lane) * CGS + g] +
+                             red[(2 * CG_ROWS + lane) * CGS + g];
             const DSys& s = sys[e.x + g];
-            if (lane == 0 && s.state[ST_ACTIVE]) st<VT>(s.L[s.nl - 1].t, i, (float)v);
+            if (s.state[ST_ACTIVE]) st<VT>(s.L[s.nl - 1].t, i, (float)v);
         }
     }
 }

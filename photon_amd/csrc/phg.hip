@@ -262,7 +262,7 @@ int tiles(int n) { return (n + TB - 1) / TB; }
 
 // largest coarsest level solved densely: above the reference's 2000-unknown
 // stopping size, so the hierarchy keeps the reference's levels whenever it
-// coarsens at all; the LDS of the coarsest-solve block holds CG_SYS vectors
+// coarsens at all; the LDS of a coarsest-solve block holds 8 such vectors
 constexpr int kDenseMax = 2048;
 // cells per block of the closed-zone sums
 constexpr int kZoneChunk = 8192;
@@ -289,8 +289,12 @@ void vcycle_t(phg_ctx* ctx, BatchDev* B, int first) {
         launch(ctx, K_VC_RESID, k_vc_resid<CC, VT>, B->tab_len[l], 0, S, B->tab[l], l);
         launch(ctx, K_VC_RESTRICT, k_vc_restrict<VT>, B->tab_len[l + 1], 0, S, B->tab[l + 1], l);
     }
-    launch(ctx, K_VC_COARSE, k_vc_coarse<VT>, B->tab_coarse_len, (size_t)B->cg_sys * B->maxnc * sizeof(double), S,
-           (const int4*)B->tab_coarse);
+    if (B->cg_sys == CG_SYS_MAX)
+        launch(ctx, K_VC_COARSE, k_vc_coarse<VT, CG_SYS_MAX>, B->tab_coarse_len, coarse_lds(CG_SYS_MAX, B->maxnc), S,
+               (const int4*)B->tab_coarse);
+    else
+        launch(ctx, K_VC_COARSE, k_vc_coarse<VT, CG_SYS_MAX / 2>, B->tab_coarse_len,
+               coarse_lds(CG_SYS_MAX / 2, B->maxnc), S, (const int4*)B->tab_coarse);
     for (int l = B->maxl - 2; l >= 1; --l) {
         launch(ctx, K_VC_PROLONG, k_vc_prolong<VT>, B->tab_len[l], 0, S, B->tab[l], l);
         launch(ctx, K_VC_JAC, k_vc_jac<CC, VT>, B->tab_len[l], 0, S, B->tab[l], l, 0);
@@ -818,7 +822,8 @@ int phg_batch_create_ex(phg_ctx* ctx, int nsys, const int* grids, const phg_batc
     if (!rc) {
         // coarsest-solve blocks: runs of up to cg_sys consecutive systems on one grid
         std::vector<int4> tc;
-        B->cg_sys = std::max(1, std::min(CG_SYS, 65536 / (8 * std::max(B->maxnc, 1))));
+        // 16 systems per block while their right-hand sides fit 64 KB of LDS, else 8
+        B->cg_sys = coarse_lds(CG_SYS_MAX, B->maxnc) <= 65536 ? CG_SYS_MAX : CG_SYS_MAX / 2;
         for (int s = 0; s < nsys;) {
             int cnt = 1;
             while (s + cnt < nsys && cnt < B->cg_sys && grids[s + cnt] == grids[s]) ++cnt;
