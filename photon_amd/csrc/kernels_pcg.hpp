@@ -753,88 +753,170 @@ __global__ void __launch_bounds__(TB) k_z_out(const DSys* __restrict__ sys, cons
 // ---------------------------------------------------------------------------
 // V-cycle, coarse levels (ELL operators)
 // ---------------------------------------------------------------------------
+// Coarse levels 1 .. L-2 (ELL operators).  A block takes up to CV_GRP systems
+// that share one grid and 256 unknowns: each thread reads its row of the
+// shared operator once and applies it to every system of the group, whose
+// own vectors it gathers.  The arithmetic of each system is the same as if it
+// were alone.  tab: (first system, count, row tile, 0).
+constexpr int CV_GRP = 4;
+
+struct Grp {
+    int first, count;
+    unsigned active;   // bit g: system first + g still iterates
+};
+__device__ __forceinline__ Grp grp_of(const DSys* __restrict__ sys, const int4& e) {
+    Grp G{e.x, e.y, 0u};
+    for (int g = 0; g < e.y; ++g) G.active |= (sys[e.x + g].state[ST_ACTIVE] != 0 ? 1u : 0u) << g;
+    return G;
+}
+
 // gather restriction: b_{l+1}[I] = sum over children of r_l, ascending
 // (the serial sum of pressure_mg.cpp:285-286 in the same order)
 template <class VT>
-__global__ void __launch_bounds__(TB) k_vc_restrict(const DSys* __restrict__ sys, const int2* __restrict__ tab, int l) {
-    PHG_SYS_PROLOGUE
-    if (l + 1 >= s.nl) return;
-    const DLevel& F = s.L[l];
-    const DLevel& C = s.L[l + 1];
-    const int I = e.y * TB + threadIdx.x;
-    if (I >= C.n) return;
-    float acc = 0.0f;
-    for (int p = F.cptr[I]; p < F.cptr[I + 1]; ++p) acc += ld<VT>(F.r, F.cidx[p]);
-    st<VT>(C.b, I, acc);
+__global__ void __launch_bounds__(TB) k_vc_restrict(const DSys* __restrict__ sys, const int4* __restrict__ tab, int l) {
+    const int4 e = tab[blockIdx.x];
+    const DSys& s0 = sys[e.x];
+    if (l + 1 >= s0.nl) return;
+    const Grp G = grp_of(sys, e);
+    if (!G.active) return;
+    const DLevel& F = s0.L[l];
+    const int I = e.z * TB + threadIdx.x;
+    if (I >= s0.L[l + 1].n) return;
+    const void* r[CV_GRP];
+    float acc[CV_GRP];
+#pragma unroll
+    for (int g = 0; g < CV_GRP; ++g) {
+        r[g] = g < G.count ? sys[G.first + g].L[l].r : nullptr;
+        acc[g] = 0.0f;
+    }
+    const int p0 = gld(F.cptr, I), p1 = gld(F.cptr, I + 1);
+    for (int p = p0; p < p1; ++p) {
+        const int c = gld(F.cidx, p);
+#pragma unroll
+        for (int g = 0; g < CV_GRP; ++g)
+            if (g < G.count) acc[g] += ld<VT>(r[g], c);
+    }
+#pragma unroll
+    for (int g = 0; g < CV_GRP; ++g)
+        if ((G.active >> g) & 1u) st<VT>(sys[G.first + g].L[l + 1].b, I, acc[g]);
 }
 
 // first Jacobi sweep from zero: x = omega b / diag (jacobi, pressure_mg.cpp:237-247)
 template <class CT, class VT>
-__global__ void __launch_bounds__(TB) k_vc_jac0(const DSys* __restrict__ sys, const int2* __restrict__ tab, int l) {
-    PHG_SYS_PROLOGUE
-    if (l >= s.nl - 1) return;
-    const DLevel& L = s.L[l];
-    const int I = e.y * TB + threadIdx.x;
+__global__ void __launch_bounds__(TB) k_vc_jac0(const DSys* __restrict__ sys, const int4* __restrict__ tab, int l) {
+    const int4 e = tab[blockIdx.x];
+    const DSys& s0 = sys[e.x];
+    if (l >= s0.nl - 1) return;
+    const Grp G = grp_of(sys, e);
+    if (!G.active) return;
+    const DLevel& L = s0.L[l];
+    const int I = e.z * TB + threadIdx.x;
     if (I >= L.n) return;
     const float dg = ld<CT>(L.diag, I);
-    float x = 0.0f;
-    if (dg > 0) x = (0.7f * ld<VT>(L.b, I)) / dg;
-    st<VT>(L.x, I, x);
+#pragma unroll
+    for (int g = 0; g < CV_GRP; ++g) {
+        if (!((G.active >> g) & 1u)) continue;
+        const DLevel& Lg = sys[G.first + g].L[l];
+        float x = 0.0f;
+        if (dg > 0) x = (0.7f * ld<VT>(Lg.b, I)) / dg;
+        st<VT>(Lg.x, I, x);
+    }
 }
 
 // damped Jacobi sweep src -> dst, omega = 0.7
 template <class CT, class VT>
-__global__ void __launch_bounds__(TB) k_vc_jac(const DSys* __restrict__ sys, const int2* __restrict__ tab, int l,
+__global__ void __launch_bounds__(TB) k_vc_jac(const DSys* __restrict__ sys, const int4* __restrict__ tab, int l,
                                                int src_is_x) {
-    PHG_SYS_PROLOGUE
-    if (l >= s.nl - 1) return;
-    const DLevel& L = s.L[l];
-    const int I = e.y * TB + threadIdx.x;
+    const int4 e = tab[blockIdx.x];
+    const DSys& s0 = sys[e.x];
+    if (l >= s0.nl - 1) return;
+    const Grp G = grp_of(sys, e);
+    if (!G.active) return;
+    const DLevel& L = s0.L[l];
+    const int I = e.z * TB + threadIdx.x;
     if (I >= L.n) return;
-    const void* src = src_is_x ? L.x : L.t;
-    void* dst = src_is_x ? L.t : L.x;
+    const void* src[CV_GRP];
+    float acc[CV_GRP];
+#pragma unroll
+    for (int g = 0; g < CV_GRP; ++g) {
+        const DLevel& Lg = sys[G.first + min(g, G.count - 1)].L[l];
+        src[g] = src_is_x ? Lg.x : Lg.t;
+        acc[g] = g < G.count ? ld<VT>(Lg.b, I) : 0.0f;
+    }
     const float dg = ld<CT>(L.diag, I);
-    float out = 0.0f;
     if (dg > 0) {
-        float acc = ld<VT>(L.b, I);
         for (int p = 0; p < L.ellk; ++p) {
             const long o = (long)p * L.n + I;
-            acc += ld<CT>(L.val, o) * ld<VT>(src, L.col[o]);
+            const int c = gld(L.col, o);
+            const float v = ld<CT>(L.val, o);
+#pragma unroll
+            for (int g = 0; g < CV_GRP; ++g)
+                if (g < G.count) acc[g] += v * ld<VT>(src[g], c);
         }
-        out = (1.0f - 0.7f) * ld<VT>(src, I) + (0.7f * acc) / dg;
     }
-    st<VT>(dst, I, out);
+#pragma unroll
+    for (int g = 0; g < CV_GRP; ++g) {
+        if (!((G.active >> g) & 1u)) continue;
+        const DLevel& Lg = sys[G.first + g].L[l];
+        float out = 0.0f;
+        if (dg > 0) out = (1.0f - 0.7f) * ld<VT>(src[g], I) + (0.7f * acc[g]) / dg;
+        st<VT>(src_is_x ? Lg.t : Lg.x, I, out);
+    }
 }
 
 // coarse residual of the current iterate t (residualSparse, pressure_mg.cpp:248-257)
 template <class CT, class VT>
-__global__ void __launch_bounds__(TB) k_vc_resid(const DSys* __restrict__ sys, const int2* __restrict__ tab, int l) {
-    PHG_SYS_PROLOGUE
-    if (l >= s.nl - 1) return;
-    const DLevel& L = s.L[l];
-    const int I = e.y * TB + threadIdx.x;
+__global__ void __launch_bounds__(TB) k_vc_resid(const DSys* __restrict__ sys, const int4* __restrict__ tab, int l) {
+    const int4 e = tab[blockIdx.x];
+    const DSys& s0 = sys[e.x];
+    if (l >= s0.nl - 1) return;
+    const Grp G = grp_of(sys, e);
+    if (!G.active) return;
+    const DLevel& L = s0.L[l];
+    const int I = e.z * TB + threadIdx.x;
     if (I >= L.n) return;
     const float dg = ld<CT>(L.diag, I);
-    float r = 0.0f;
+    const void* t[CV_GRP];
+    float r[CV_GRP];
+#pragma unroll
+    for (int g = 0; g < CV_GRP; ++g) {
+        const DLevel& Lg = sys[G.first + min(g, G.count - 1)].L[l];
+        t[g] = Lg.t;
+        r[g] = g < G.count ? ld<VT>(Lg.b, I) - dg * ld<VT>(Lg.t, I) : 0.0f;
+    }
     if (dg > 0) {
-        r = ld<VT>(L.b, I) - dg * ld<VT>(L.t, I);
         for (int p = 0; p < L.ellk; ++p) {
             const long o = (long)p * L.n + I;
-            r += ld<CT>(L.val, o) * ld<VT>(L.t, L.col[o]);
+            const int c = gld(L.col, o);
+            const float v = ld<CT>(L.val, o);
+#pragma unroll
+            for (int g = 0; g < CV_GRP; ++g)
+                if (g < G.count) r[g] += v * ld<VT>(t[g], c);
         }
     }
-    st<VT>(L.r, I, r);
+#pragma unroll
+    for (int g = 0; g < CV_GRP; ++g)
+        if ((G.active >> g) & 1u) st<VT>(sys[G.first + g].L[l].r, I, dg > 0 ? r[g] : 0.0f);
 }
 
 // t_l += t_{l+1}[agg]
 template <class VT>
-__global__ void __launch_bounds__(TB) k_vc_prolong(const DSys* __restrict__ sys, const int2* __restrict__ tab, int l) {
-    PHG_SYS_PROLOGUE
-    if (l >= s.nl - 1) return;
-    const DLevel& L = s.L[l];
-    const int I = e.y * TB + threadIdx.x;
+__global__ void __launch_bounds__(TB) k_vc_prolong(const DSys* __restrict__ sys, const int4* __restrict__ tab, int l) {
+    const int4 e = tab[blockIdx.x];
+    const DSys& s0 = sys[e.x];
+    if (l >= s0.nl - 1) return;
+    const Grp G = grp_of(sys, e);
+    if (!G.active) return;
+    const DLevel& L = s0.L[l];
+    const int I = e.z * TB + threadIdx.x;
     if (I >= L.n) return;
-    st<VT>(L.t, I, ld<VT>(L.t, I) + ld<VT>(s.L[l + 1].t, L.agg[I]));
+    const int a = gld(L.agg, I);
+#pragma unroll
+    for (int g = 0; g < CV_GRP; ++g) {
+        if (!((G.active >> g) & 1u)) continue;
+        const DSys& sg = sys[G.first + g];
+        st<VT>(sg.L[l].t, I, ld<VT>(sg.L[l].t, I) + ld<VT>(sg.L[l + 1].t, a));
+    }
 }
 
 // coarsest level: t = B b in FP64, B the projected inverse of the grounded

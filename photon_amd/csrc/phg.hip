@@ -90,8 +90,10 @@ struct BatchDev {
     std::vector<int> grids;
     std::vector<DSys> h_sys;
     DSys* d_sys = nullptr;
-    std::vector<int2*> tab;        // per level: (system, tile) of systems having that level
+    std::vector<int2*> tab;        // level 0: (system, tile); tab[0] only
     std::vector<int> tab_len;
+    std::vector<int4*> tabg;       // levels >= 1: (first system, count, tile) of same-grid groups
+    std::vector<int> tabg_len;
     int2* tab_sys = nullptr;       // one entry per system
     int2* tab_z = nullptr;         // (system, zone task) for the closed-zone means
     int tab_z_len = 0;
@@ -246,9 +248,10 @@ void free_all(std::vector<void*>& v) {
 
 // tile table of one level, XCD-ordered: block b runs on XCD b % 8, and gets
 // the (b / 8)-th entry of that XCD's contiguous share of the logical list
-std::vector<int2> xcd_order(const std::vector<int2>& logical) {
+template <class E>
+std::vector<E> xcd_order(const std::vector<E>& logical) {
     const int M = (int)logical.size();
-    std::vector<int2> phys(M);
+    std::vector<E> phys(M);
     const int q = M / 8, r = M % 8;
     for (int b = 0; b < M; ++b) {
         const int x = b % 8, idx = b / 8;
@@ -282,12 +285,14 @@ void vcycle_t(phg_ctx* ctx, BatchDev* B, int first) {
     launch(ctx, K_PRE_RED, k_pre_red<C0, float>, n0, 0, S, T0, first);
     launch(ctx, K_RB_BLACK, k_rb_black<C0, float>, n0, B->lds0(), S, T0);
     launch(ctx, K_RESID0, k_resid0<C0, float, VT>, n0, B->lds0(), S, T0);
-    launch(ctx, K_VC_RESTRICT, k_vc_restrict<VT>, B->tab_len[1], 0, S, B->tab[1], 0);
+    const std::vector<int4*>& TG = B->tabg;
+    const std::vector<int>& NG = B->tabg_len;
+    launch(ctx, K_VC_RESTRICT, k_vc_restrict<VT>, NG[1], 0, S, TG[1], 0);
     for (int l = 1; l + 1 < B->maxl; ++l) {
-        launch(ctx, K_VC_JAC, k_vc_jac0<CC, VT>, B->tab_len[l], 0, S, B->tab[l], l);
-        launch(ctx, K_VC_JAC, k_vc_jac<CC, VT>, B->tab_len[l], 0, S, B->tab[l], l, 1);
-        launch(ctx, K_VC_RESID, k_vc_resid<CC, VT>, B->tab_len[l], 0, S, B->tab[l], l);
-        launch(ctx, K_VC_RESTRICT, k_vc_restrict<VT>, B->tab_len[l + 1], 0, S, B->tab[l + 1], l);
+        launch(ctx, K_VC_JAC, k_vc_jac0<CC, VT>, NG[l], 0, S, TG[l], l);
+        launch(ctx, K_VC_JAC, k_vc_jac<CC, VT>, NG[l], 0, S, TG[l], l, 1);
+        launch(ctx, K_VC_RESID, k_vc_resid<CC, VT>, NG[l], 0, S, TG[l], l);
+        launch(ctx, K_VC_RESTRICT, k_vc_restrict<VT>, NG[l + 1], 0, S, TG[l + 1], l);
     }
     if (B->cg_sys == CG_SYS_MAX)
         launch(ctx, K_VC_COARSE, k_vc_coarse<VT, CG_SYS_MAX>, B->tab_coarse_len, coarse_lds(CG_SYS_MAX, B->maxnc), S,
@@ -296,9 +301,9 @@ void vcycle_t(phg_ctx* ctx, BatchDev* B, int first) {
         launch(ctx, K_VC_COARSE, k_vc_coarse<VT, CG_SYS_MAX / 2>, B->tab_coarse_len,
                coarse_lds(CG_SYS_MAX / 2, B->maxnc), S, (const int4*)B->tab_coarse);
     for (int l = B->maxl - 2; l >= 1; --l) {
-        launch(ctx, K_VC_PROLONG, k_vc_prolong<VT>, B->tab_len[l], 0, S, B->tab[l], l);
-        launch(ctx, K_VC_JAC, k_vc_jac<CC, VT>, B->tab_len[l], 0, S, B->tab[l], l, 0);
-        launch(ctx, K_VC_JAC, k_vc_jac<CC, VT>, B->tab_len[l], 0, S, B->tab[l], l, 1);
+        launch(ctx, K_VC_PROLONG, k_vc_prolong<VT>, NG[l], 0, S, TG[l], l);
+        launch(ctx, K_VC_JAC, k_vc_jac<CC, VT>, NG[l], 0, S, TG[l], l, 0);
+        launch(ctx, K_VC_JAC, k_vc_jac<CC, VT>, NG[l], 0, S, TG[l], l, 1);
     }
     launch(ctx, K_POST_BLACK, k_post_black<C0, float, VT>, n0, B->lds0(), S, T0);
     launch(ctx, K_POST_RED, k_post_red<C0, float>, n0, B->lds0(), S, T0);
@@ -794,20 +799,33 @@ int phg_batch_create_ex(phg_ctx* ctx, int nsys, const int* grids, const phg_batc
         }
     }
     if (!rc) rc = dalloc(ctx, B->allocs, &B->d_sys, (size_t)nsys);
-    // tile tables: level 0 by column tiles x k-chunks, coarse levels by 256 unknowns
-    B->tab.assign(std::max(B->maxl, 1), nullptr);
-    B->tab_len.assign(std::max(B->maxl, 1), 0);
-    for (int l = 0; l < B->maxl && !rc; ++l) {
+    // tile tables: level 0 by column tiles x k-chunks; coarse levels by 256
+    // unknowns of a group of up to CV_GRP consecutive systems on one grid
+    B->tab.assign(1, nullptr);
+    B->tab_len.assign(1, 0);
+    B->tabg.assign(std::max(B->maxl, 1), nullptr);
+    B->tabg_len.assign(std::max(B->maxl, 1), 0);
+    if (!rc) {
         std::vector<int2> logical;
-        for (int s = 0; s < nsys; ++s) {
-            GridDev* G = get_grid(ctx, grids[s]);
-            if ((int)G->lv.size() <= l) continue;
-            const int t = l == 0 ? B->h_sys[s].tiles0 : tiles(G->lv[l].n);
-            for (int k = 0; k < t; ++k) logical.push_back(make_int2(s, k));
-        }
+        for (int s = 0; s < nsys; ++s)
+            for (int k = 0; k < B->h_sys[s].tiles0; ++k) logical.push_back(make_int2(s, k));
         std::vector<int2> phys = xcd_order(logical);
-        B->tab_len[l] = (int)phys.size();
-        rc = upload(ctx, B->allocs, &B->tab[l], phys.data(), phys.size());
+        B->tab_len[0] = (int)phys.size();
+        rc = upload(ctx, B->allocs, &B->tab[0], phys.data(), phys.size());
+    }
+    for (int l = 1; l < B->maxl && !rc; ++l) {
+        std::vector<int4> logical;
+        for (int s = 0; s < nsys;) {
+            GridDev* G = get_grid(ctx, grids[s]);
+            int cnt = 1;
+            while (s + cnt < nsys && cnt < CV_GRP && grids[s + cnt] == grids[s]) ++cnt;
+            if ((int)G->lv.size() > l)
+                for (int k = 0; k < tiles(G->lv[l].n); ++k) logical.push_back(make_int4(s, cnt, k, 0));
+            s += cnt;
+        }
+        std::vector<int4> phys = xcd_order(logical);
+        B->tabg_len[l] = (int)phys.size();
+        if (!phys.empty()) rc = upload(ctx, B->allocs, &B->tabg[l], phys.data(), phys.size());
     }
     if (!rc) {
         std::vector<int2> ts(nsys), tz;
