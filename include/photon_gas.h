@@ -125,6 +125,8 @@ int phg_batch_create(phg_ctx *ctx, int nsys, const int *grids, int *batch_out);
 typedef struct {
     int kchunk;          /* level-0 planes per block along z (0 = from level0_blocks) */
     int level0_blocks;   /* level-0 blocks per launch to aim for when kchunk = 0 (0 = 4096) */
+    int unfused;         /* 1: pre-smoothing as three separate passes instead of the fused one
+                            (same results bit for bit; for comparison) */
 } phg_batch_opts;
 int phg_batch_create_ex(phg_ctx *ctx, int nsys, const int *grids, const phg_batch_opts *opts, int *batch_out);
 int phg_batch_destroy(phg_ctx *ctx, int batch);

@@ -103,7 +103,8 @@ struct DSys {
     const double* cinv;           // nc*nc FP64 inverse of the grounded coarsest operator
     const float* b;               // outer CG vectors, FP32 on level-0 cells
     float* x;
-    float* r;
+    float* r;                     // residual, double-buffered: r_k lives in r (k even) or r1 (k odd),
+    float* r1;                    // so a pass that updates it never overwrites what a neighbour reads
     float* q;
     float* d0;                    // direction, double-buffered: d_k lives in d[k & 1]
     float* d1;

@@ -105,6 +105,14 @@ __device__ __forceinline__ float nb_sum_f(const Cell& q, int nx, int ny, int nz,
            (e4 ? a[4] : 0.0f) * v4 + (e5 ? a[5] : 0.0f) * v5;
 }
 
+// CG residual r_k: buffer k & 1.  The pre-smoothing of iteration k reads r_k
+// and writes r_{k+1}; the rest of that V-cycle reads r_{k+1}; the V-cycle
+// before the first iteration reads r_0 (ST_VC is still 0 then).
+__device__ __forceinline__ float* rbuf(const DSys& s, int k) { return (k & 1) ? s.r1 : s.r; }
+__device__ __forceinline__ const float* r_vc(const DSys& s) {
+    return s.state[ST_VC] ? rbuf(s, s.state[ST_ITERS] + 1) : s.r;
+}
+
 // the column and k-range of a level-0 block
 struct ColTile {
     int i, j, c2, k0, k1;
@@ -279,10 +287,11 @@ struct CoefR {
 };
 template <class CT>
 __device__ __forceinline__ auto cr_pre(const DSys& s, const DLevel& L) {
-    return [&s, &L](const Cell& q) {
+    const float* rv = r_vc(s);
+    return [&L, rv](const Cell& q) {
         CoefR<CT> o;
         o.cf.load(L, L.axc, L.ayc, L.azc, q);
-        o.r = gld(s.r, q.c);
+        o.r = gld(rv, q.c);
         return o;
     };
 }
@@ -550,15 +559,18 @@ __global__ void __launch_bounds__(TB) k_pre_red(const DSys* __restrict__ sys, co
     const int sxy = L.nx * L.ny;
     const double alpha = s.scal[S_ALPHA];
     const float zs = (float)s.scal[S_ZS];
+    const int it = s.state[ST_ITERS];
+    const float* __restrict__ rin = rbuf(s, it);   // r_it (r_0 on the first V-cycle)
+    float* __restrict__ rout = rbuf(s, it + 1);
     double v[1] = {0.0};
     float m = 0.0f;
     if (t.valid) {
         for (int k = t.k0; k < t.k1; ++k) {
             const int c = t.c2 + sxy * k;
-            float r = gld(s.r, c);
+            float r = gld(rin, c);
             if (!first) {
                 r -= (float)(alpha * gld(s.q, c));
-                gst(s.r, c, r);
+                gst(rout, c, r);
                 v[0] += (double)r * r;
                 m = fmaxf(m, fabsf(r));
             }
@@ -687,6 +699,7 @@ __global__ void __launch_bounds__(TB) k_post_red(const DSys* __restrict__ sys, c
     const void* z = s.z;
     const float zs = (float)s.scal[S_ZS];
     const bool with_q = s.state[ST_FLEX] != 0 && s.state[ST_VC] != 0;   // q exists once an iteration has run
+    const float* rv = r_vc(s);
     double v[1] = {0.0}, vq = 0.0;
     extern __shared__ float lds[];
     struct Op {
@@ -696,7 +709,7 @@ __global__ void __launch_bounds__(TB) k_post_red(const DSys* __restrict__ sys, c
     auto pre = [&](const Cell& q) {
         Op o;
         o.cf.load(L, L.axc, L.ayc, L.azc, q);
-        o.r = gld(s.r, q.c);
+        o.r = gld(rv, q.c);
         o.q = with_q ? gld(s.q, q.c) : 0.0f;
         return o;
     };
@@ -735,6 +748,213 @@ __global__ void __launch_bounds__(TB) k_post_red(const DSys* __restrict__ sys, c
         }
         out[3] = acc;
         if (with_q) out[0] = accq;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// fused pre-smoothing: k_pre_red + k_rb_black + k_resid0 in one pass
+// ---------------------------------------------------------------------------
+// For grids with 0/1 coefficients.  The black sweep at a cell needs the red
+// values of its neighbours, and the residual at a cell the smoothed values of
+// its neighbours, so the residual of plane k needs red values two cells away.
+// A block owns TB columns and marches its k-chunk with a lead of two planes:
+//   stage plane k+2   the new residual rn = r - alpha q and, over the block's
+//                     columns +- 2 grid rows (W2 = TB + 4 nx), the red value
+//                     rn zs / diag (black cells: rn zs) and the face mask
+//   smooth plane k+1  over the columns +- 1 row (W1 = TB + 2 nx): red cells
+//                     keep their value, black cells take the Gauss-Seidel update
+//   emit plane k      z, the updated r and the residual r0 = rn zs - A z of
+//                     the block's own columns, with |r|^2 and max|r|.
+// Every value is formed by the same operations as in the three separate
+// kernels, so the results are bitwise the same; the pass reads r and q and
+// writes r, z and r0 once (18 B/cell in place of 36).  LDS rings of four
+// planes: Rb [W2] floats, Zb [W1] floats, Mb [W2] bytes, Ob [TB] floats.
+constexpr int FUSE_M2 = 4;                              // W2 slots per thread
+constexpr int FUSE_M1 = 3;                              // W1 slots per thread
+constexpr int FUSE_MAX_NX = (FUSE_M2 * TB - TB) / 4;    // 192: W2 <= 4 TB, W1 <= 2.5 TB
+
+__host__ __device__ constexpr size_t fuse_lds(int nx) {
+    return (size_t)4 * (TB + 4 * nx) * 4 + (size_t)4 * (TB + 2 * nx) * 4 + (size_t)4 * TB * 4 +
+           (((size_t)4 * (TB + 4 * nx) + 3) / 4) * 4;
+}
+
+template <class VT, bool FIRST>
+__global__ void __launch_bounds__(TB) k_pre_fused(const DSys* __restrict__ sys, const int2* __restrict__ tab) {
+    PHG_SYS_PROLOGUE
+    const DLevel& L = s.L[0];
+    const ColTile t = col_tile(s, e.y);
+    const int nx = L.nx, ny = L.ny, nz = L.nz, sxy = nx * ny;
+    const int W1 = TB + 2 * nx, W2 = TB + 4 * nx;
+    const int c0 = t.c2 - (int)threadIdx.x;
+    const double alpha = s.scal[S_ALPHA];
+    const float zs = (float)s.scal[S_ZS];
+    const int it = s.state[ST_ITERS];
+    const float* __restrict__ rin = rbuf(s, it);   // r_it: read here, also in the halo rows of other blocks
+    float* __restrict__ rout = rbuf(s, it + 1);    // r_{it+1}: written for the own columns only
+    extern __shared__ float lds[];
+    float* Rb = lds;                     // [4][W2]
+    float* Zb = Rb + 4 * W2;             // [4][W1]
+    float* Ob = Zb + 4 * W1;             // [4][TB]: rn of the own columns
+    uint8_t* Mb = reinterpret_cast<uint8_t*>(Ob + 4 * TB);   // [4][W2]
+    auto slot = [](int p) { return (p + 4) & 3; };           // planes >= -4
+
+    // W2 slots of this thread: clamped column, in-grid flag, parity of i + j
+    int col2[FUSE_M2], par2[FUSE_M2];
+    bool in2[FUSE_M2];
+#pragma unroll
+    for (int m = 0; m < FUSE_M2; ++m) {
+        const int h = (int)threadIdx.x + m * TB;
+        const int col = c0 - 2 * nx + h;
+        in2[m] = h < W2 && col >= 0 && col < sxy;
+        const int cc = min(max(col, 0), sxy - 1);
+        col2[m] = cc;
+        par2[m] = (cc % nx + cc / nx) & 1;
+    }
+    // W1 slots: in-grid flag and (i, j)
+    int i1[FUSE_M1], j1[FUSE_M1];
+    bool in1[FUSE_M1];
+#pragma unroll
+    for (int m = 0; m < FUSE_M1; ++m) {
+        const int h = (int)threadIdx.x + m * TB;
+        const int col = c0 - nx + h;
+        in1[m] = h < W1 && col >= 0 && col < sxy;
+        const int cc = min(max(col, 0), sxy - 1);
+        j1[m] = cc / nx;
+        i1[m] = cc - j1[m] * nx;
+    }
+    struct Raw {
+        float r[FUSE_M2], q[FUSE_M2];
+        unsigned m[FUSE_M2];
+    };
+    auto load = [&](int p, Raw& X) {
+        const long base = (long)sxy * min(max(p, 0), nz - 1);
+#pragma unroll
+        for (int m = 0; m < FUSE_M2; ++m) {
+            X.r[m] = gld(rin, col2[m] + base);
+            X.q[m] = FIRST ? 0.0f : gld(s.q, col2[m] + base);
+            X.m[m] = gld(L.mask, col2[m] + base);
+        }
+    };
+    // stage plane p from its raw loads
+    auto stage = [&](int p, const Raw& X) {
+        const bool pin = p >= 0 && p < nz;
+        float* R = Rb + slot(p) * W2;
+        uint8_t* M = Mb + slot(p) * W2;
+        float* O = Ob + slot(p) * TB;
+#pragma unroll
+        for (int m = 0; m < FUSE_M2; ++m) {
+            const int h = (int)threadIdx.x + m * TB;
+            if (h >= W2) break;
+            float rn = X.r[m];
+            if (!FIRST) rn -= (float)(alpha * X.q[m]);
+            const bool ok = pin && in2[m];
+            const unsigned mk = X.m[m];
+            const int dg = __popc(mk);
+            const float rs = rn * zs;
+            float v = rs;   // black cells: rn zs
+            if (((par2[m] + p) & 1) == 0) v = dg > 0 ? rs / (float)dg : 0.0f;
+            R[h] = ok ? v : 0.0f;
+            M[h] = ok ? (uint8_t)mk : (uint8_t)0;
+            const int o = h - 2 * nx;
+            if (o >= 0 && o < TB) O[o] = ok ? rn : 0.0f;
+        }
+    };
+    // smoothed values of plane p over W1
+    auto smooth = [&](int p) {
+        const bool pin = p >= 0 && p < nz;
+        const float* R = Rb + slot(p) * W2;
+        const float* Rm = Rb + slot(p - 1) * W2;
+        const float* Rp = Rb + slot(p + 1) * W2;
+        const uint8_t* M = Mb + slot(p) * W2;
+        float* Z = Zb + slot(p) * W1;
+#pragma unroll
+        for (int m = 0; m < FUSE_M1; ++m) {
+            const int h = (int)threadIdx.x + m * TB;
+            if (h >= W1) break;
+            const int h2 = h + nx;
+            float z = 0.0f;
+            if (pin && in1[m]) {
+                const float v = R[h2];
+                if (((i1[m] + j1[m] + p) & 1) == 0) {
+                    z = v;
+                } else {
+                    const CoefRegs<Mask01> cf{(unsigned)M[h2]};
+                    float a[6];
+                    cf.get(a);
+                    const float dg = diag6(a);
+                    if (dg > 0) z = (v + lat_sum(Cell{0, i1[m], j1[m], p}, nx, ny, nz, a, R + h2, Rm[h2], Rp[h2])) / dg;
+                }
+            }
+            Z[h] = z;
+        }
+    };
+    double acc = 0.0;
+    float mx = 0.0f;
+    // emit plane k of the own column
+    auto emit = [&](int k) {
+        if (!t.valid) return;
+        const int c = t.c2 + sxy * k;
+        const float rn = Ob[slot(k) * TB + threadIdx.x];
+        const float* Z = Zb + slot(k) * W1 + nx + threadIdx.x;
+        const float zc = *Z;
+        const CoefRegs<Mask01> cf{(unsigned)Mb[slot(k) * W2 + 2 * nx + threadIdx.x]};
+        float a[6];
+        cf.get(a);
+        const float dg = diag6(a);
+        float r0 = 0.0f;
+        if (dg > 0)
+            r0 = rn * zs - dg * zc +
+                 lat_sum(Cell{c, t.i, t.j, k}, nx, ny, nz, a, Z, Zb[slot(k - 1) * W1 + nx + threadIdx.x],
+                         Zb[slot(k + 1) * W1 + nx + threadIdx.x]);
+        gst(reinterpret_cast<float*>(s.z), c, zc);
+        st<VT>(L.r, c, r0);
+        if (!FIRST) {
+            gst(rout, c, rn);
+            acc += (double)rn * rn;
+            mx = fmaxf(mx, fabsf(rn));
+        }
+    };
+    // one plane of the march: stage k+2, smooth k+1, emit k (for k >= k0).
+    // X holds the raw loads of plane k+2 and is refilled with plane k+5.
+    const int ks = t.k0 - 2;
+    auto step = [&](int k, Raw& X) {
+        stage(k + 2, X);
+        load(min(k + 5, t.k1 + 1), X);   // past the chunk: repeats the last plane needed
+        __syncthreads();
+        smooth(k + 1);
+        __syncthreads();
+        if (k >= t.k0) emit(k);
+    };
+    Raw X0, X1, X2;
+    {
+        Raw Y;
+        load(ks, Y);
+        stage(ks, Y);
+        load(ks + 1, Y);
+        stage(ks + 1, Y);
+    }
+    load(ks + 2, X0);
+    load(ks + 3, X1);
+    load(ks + 4, X2);
+    for (int k = ks; k < t.k1; k += 3) {
+        step(k, X0);
+        if (k + 1 < t.k1) step(k + 1, X1);
+        if (k + 2 < t.k1) step(k + 2, X2);
+    }
+    if (FIRST) return;
+    double* out = s.part + (long)e.y * NPART;
+    __shared__ double sh[TB / 64];
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    const double v = wave_sum(acc);
+    if (lane == 0) sh[w] = v;
+    __syncthreads();
+    mx = block_max(mx);
+    if (threadIdx.x == 0) {
+        double tot = 0.0;
+#pragma unroll
+        for (int ww = 0; ww < TB / 64; ++ww) tot += sh[ww];
+        out[1] = tot;
+        out[2] = mx;
     }
 }
 

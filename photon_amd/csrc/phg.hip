@@ -27,12 +27,13 @@ namespace phg {
 enum KClass {
     K_RESID_INIT, K_DIR_APPLY, K_PRE_RED, K_RB_BLACK, K_RESID0, K_POST_BLACK, K_POST_RED, K_FIN, K_VC_RESTRICT,
     K_VC_JAC, K_VC_RESID, K_VC_PROLONG, K_VC_COARSE, K_XFLUSH, K_ZONE, K_RHS, K_CORRECT, K_ENERGY, K_ADVECT,
-    K_COUNT, K_NCLASS
+    K_COUNT, K_PRE_SMOOTH, K_NCLASS
 };
 static const char* kKName[K_NCLASS] = {"resid_init",  "dir_apply",  "pre_red",    "rb_black",  "resid0",
                                        "post_black",  "post_red",   "finalize",   "vc_restrict", "vc_jacobi",
                                        "vc_resid",    "vc_prolong", "vc_coarse",  "x_flush",   "zone_mean",
-                                       "rhs",         "correct",    "energy",     "advect",    "count_active"};
+                                       "rhs",         "correct",    "energy",     "advect",    "count_active",
+                                       "pre_smooth"};
 
 struct GridDev {
     int nx = 0, ny = 0, nz = 0, n = 0;
@@ -86,6 +87,7 @@ struct BatchDev {
     int nsys = 0, prec = 0, maxl = 0, maxnc = 0;
     bool binary = false;           // every grid has 0/1 coefficients: level-0 kernels read the face mask
     int maxnx = 0;                 // widest grid: sizes the LDS plane buffers of the level-0 kernels
+    bool unfused = false;          // run the separate level-0 smoothing kernels (comparison runs)
     size_t lds0() const { return 2 * (size_t)(TB + 2 * maxnx) * sizeof(float); }
     std::vector<int> grids;
     std::vector<DSys> h_sys;
@@ -282,9 +284,15 @@ void vcycle_t(phg_ctx* ctx, BatchDev* B, int first) {
     const DSys* S = B->d_sys;
     const int2* T0 = B->tab[0];
     const int n0 = B->tab_len[0];
-    launch(ctx, K_PRE_RED, k_pre_red<C0, float>, n0, 0, S, T0, first);
-    launch(ctx, K_RB_BLACK, k_rb_black<C0, float>, n0, B->lds0(), S, T0);
-    launch(ctx, K_RESID0, k_resid0<C0, float, VT>, n0, B->lds0(), S, T0);
+    if (std::is_same<C0, Mask01>::value && B->maxnx <= FUSE_MAX_NX && !B->unfused) {
+        // pre-smoothing and residual in one pass
+        if (first) launch(ctx, K_PRE_SMOOTH, k_pre_fused<VT, true>, n0, fuse_lds(B->maxnx), S, T0);
+        else launch(ctx, K_PRE_SMOOTH, k_pre_fused<VT, false>, n0, fuse_lds(B->maxnx), S, T0);
+    } else {
+        launch(ctx, K_PRE_RED, k_pre_red<C0, float>, n0, 0, S, T0, first);
+        launch(ctx, K_RB_BLACK, k_rb_black<C0, float>, n0, B->lds0(), S, T0);
+        launch(ctx, K_RESID0, k_resid0<C0, float, VT>, n0, B->lds0(), S, T0);
+    }
     const std::vector<int4*>& TG = B->tabg;
     const std::vector<int>& NG = B->tabg_len;
     launch(ctx, K_VC_RESTRICT, k_vc_restrict<VT>, NG[1], 0, S, TG[1], 0);
@@ -685,11 +693,13 @@ int phg_batch_create(phg_ctx* ctx, int nsys, const int* grids, int* batch_out) {
 
 int phg_batch_create_ex(phg_ctx* ctx, int nsys, const int* grids, const phg_batch_opts* opts, int* batch_out) {
     if (!ctx || nsys <= 0 || !grids || !batch_out) return PHG_EINVAL;
-    if (opts && (opts->kchunk < 0 || opts->level0_blocks < 0)) return fail(ctx, PHG_EINVAL, "bad batch options");
+    if (opts && (opts->kchunk < 0 || opts->level0_blocks < 0 || opts->unfused < 0 || opts->unfused > 1))
+        return fail(ctx, PHG_EINVAL, "bad batch options");
     HIP_OK(hipSetDevice(ctx->device));
     BatchDev* B = new BatchDev();
     B->nsys = nsys;
     B->grids.assign(grids, grids + nsys);
+    B->unfused = opts && opts->unfused;
     B->h_sys.resize(nsys);
     int rc = 0;
     long col_tiles = 0;
@@ -745,13 +755,14 @@ int phg_batch_create_ex(phg_ctx* ctx, int nsys, const int* grids, const phg_batc
         S.zclosed = G->zclosed;
         const size_t n0 = G->n;
         float* v4 = nullptr;
-        rc = dalloc(ctx, B->allocs, &v4, 5 * n0);
+        rc = dalloc(ctx, B->allocs, &v4, 6 * n0);
         if (rc) break;
         S.r = v4;
         S.q = v4 + n0;
         S.d0 = v4 + 2 * n0;
         S.d1 = v4 + 3 * n0;
         S.z = v4 + 4 * n0;   // the level-0 iterate stays FP32 in every precision: it feeds d directly
+        S.r1 = v4 + 5 * n0;
         rc = dalloc(ctx, B->allocs, &S.part, (size_t)S.tiles0 * NPART);
         if (!rc) rc = dalloc(ctx, B->allocs, &S.zpart, (size_t)std::max(G->nztask, 1));
         if (!rc) rc = dalloc(ctx, B->allocs, &S.zmean, (size_t)std::max(G->nclosed, 1));

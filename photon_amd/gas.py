@@ -42,7 +42,7 @@ class SolveParams(C.Structure):
 
 
 class BatchOpts(C.Structure):
-    _fields_ = [("kchunk", C.c_int), ("level0_blocks", C.c_int)]
+    _fields_ = [("kchunk", C.c_int), ("level0_blocks", C.c_int), ("unfused", C.c_int)]
 
 
 class SolveStats(C.Structure):
@@ -262,12 +262,12 @@ class Grid:
 class Batch:
     """A fixed set of systems solved together (phg_batch_create / _solve)."""
 
-    def __init__(self, ctx: Context, grids, kchunk=0, level0_blocks=0):
+    def __init__(self, ctx: Context, grids, kchunk=0, level0_blocks=0, unfused=False):
         self.ctx = ctx
         self.grids = list(grids)
         ids = np.array([g.id for g in self.grids], np.int32)
         bid = C.c_int()
-        opts = BatchOpts(kchunk, level0_blocks)
+        opts = BatchOpts(kchunk, level0_blocks, 1 if unfused else 0)
         ctx._check(ctx.L.phg_batch_create_ex(ctx.h, len(ids), ids.ctypes.data, C.byref(opts), C.byref(bid)))
         self.id = bid.value
 
