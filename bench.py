@@ -37,7 +37,7 @@ DT = 0.01
 TOL = 1e-6
 
 
-LEVEL0 = ["dir_apply", "pre_red", "rb_black", "resid0", "post_black", "post_red"]
+LEVEL0 = ["dir_apply", "pre_smooth", "pre_red", "rb_black", "resid0", "post_black", "post_red"]
 # HBM bytes per launch of each level-0 kernel on this workload, from rocprofv3
 # FETCH_SIZE and WRITE_SIZE runs scaled by the copy-kernel calibration
 TRAFFIC_FILE = "profiles/level0_house80x50x40_1000rhs_mixed_f16_hbm_traffic.txt"
@@ -76,6 +76,8 @@ def algorithmic_bytes(name, iters, n0, n1, vt, flex):
     nfirst = int((it > 0).sum())
     if name == "dir_apply":                          # read z d_old x, write x d_new q; first: read z, write d q
         per_sys = nfirst * 12 * n0 + (nd - nfirst) * 24 * n0
+    elif name == "pre_smooth":                       # read r q, write r z r0; first: read r, write z r0
+        per_sys = len(it) * (8 + vt) * n0 + (nvc - len(it)) * (16 + vt) * n0
     elif name == "pre_red":                          # read r q, write r z; first: read r, write z
         per_sys = len(it) * 8 * n0 + (nvc - len(it)) * 16 * n0
     elif name == "rb_black":                         # read z r, write half of z
@@ -154,6 +156,83 @@ def run_plain(ctx, gas, grid_scen, bs_host, nsolve):
     t1 = time.perf_counter()
     G.destroy()
     return dict(seconds=t1 - t0, iters=np.array(iters))
+
+
+def run_precision(ctx, gas, sc, nrhs=16):
+    """The precision workload: the house refined to 0.1 m (160x100x80 =
+    1.28M cells), `nrhs` right-hand sides as one batch, per storage
+    precision: solves/s and iterations (errors against the FP64 oracle are
+    measured by the GPU tests)."""
+    g = sc.house(1, 160, 100, 80, 0.1)
+    bs = [sc.rhs_from_divergence(g, sc.divergence_field(g, 5000 + s), DT) for s in range(nrhs)]
+    out = {}
+    for name, prec in (("fp32", gas.PREC_FP32), ("mixed_f16", gas.PREC_MIXED_F16), ("mixed_bf16", gas.PREC_MIXED_BF16)):
+        r = run_batch(ctx, gas, g, prec, bs, 1, 1, profile=False)
+        out[name] = {"solves_per_s": round(nrhs / r["seconds"], 2), "iters_mean": float(r["iters"].mean()),
+                     "iters_max": int(r["iters"].max()), "all_converged": bool(r["ok"])}
+    out["workload"] = f"house 160x100x80 cells at 0.1 m (seed 1), {nrhs} RHS as one batch, ||r||/||b|| <= 1e-6"
+    return out
+
+
+def run_batching(ctx, gas, sc, nscen=64, steps=5):
+    """The batching workload: `nscen` scenarios with grids uniform between
+    20x15x12 and 80x50x40 (independent walls), each with a seeded 10-500 kW
+    burner; one timestep's pressure solves (FP16 V-cycle) as one batch whose
+    set-up and iterations replay as one captured graph.  Timed with the
+    library's default block shape and with the autotuner's per-size-class
+    block depth and smoother sweeps."""
+    from photon_amd import autotune as at
+    grids = [sc.batch_grid(s) for s in range(nscen)]
+    rhs = [sc.rhs_from_divergence(g, sc.burner_field(g, s), DT) for s, g in enumerate(grids)]
+    Gs = [gas.Grid(ctx, g, precision=gas.PREC_MIXED_F16) for g in grids]
+    dbs = [ctx.to_device(b) for b in rhs]
+    dxs = [ctx.array(g.n) for g in grids]
+    params = gas.solve_params(tol=TOL, max_iters=400, zero_x=True, use_graph=2, check_every=48)
+
+    def timed(kcs, sws):
+        B = gas.Batch(ctx, Gs, sys_kchunk=kcs, sys_sweeps0=sws)
+        B.solve(dbs, dxs, params)
+        ctx.synchronize()
+        lat = []
+        st = None
+        for _ in range(steps):
+            t0 = time.perf_counter()
+            st = B.solve(dbs, dxs, params)
+            ctx.synchronize()
+            lat.append(time.perf_counter() - t0)
+        B.destroy()
+        it = np.array([s["iters"] for s in st])
+        ok = all(s["status"] == gas.CONVERGED for s in st)
+        return {"solves_per_s": round(nscen / float(np.median(lat)), 1), "step_ms_median": round(1e3 * float(np.median(lat)), 3),
+                "iters_mean": float(it.mean()), "iters_max": int(it.max()), "all_converged": bool(ok)}
+
+    default = timed(None, None)
+    t0 = time.perf_counter()
+    table = at.tune(ctx, Gs, dbs, dxs, params, repeats=2)
+    tune_s = time.perf_counter() - t0
+    kcs, sws = at.settings(Gs, table)
+    tuned = timed(kcs, sws)
+    for G in Gs:
+        G.destroy()
+    sizes = np.array([g.n for g in grids])
+    return {"workload": f"{nscen} scenarios, grids uniform 20x15x12..80x50x40 ({sizes.min()}..{sizes.max()} cells, "
+                        f"{sizes.sum()} total), 10-500 kW burner each, one timestep's solves as one batch, FP16 V-cycle",
+            "default": default, "autotuned": tuned, "autotune_table": at.table_to_json(table),
+            "autotune_seconds": round(tune_s, 2)}
+
+
+def run_serving(ctx, window_s=2.0, steps=30):
+    """The serving workload: Poisson arrivals at 20 scenarios/s over
+    `window_s` seconds, sizes log-uniform 3.6k..1.28M cells (room aspect),
+    10-500 kW burners, each scenario run for `steps` steps of dt = 0.01 s
+    (energyAndExpansion, projection, advectScalars) with its state resident in
+    HBM; continuous batching under a 64 GB footprint budget."""
+    from photon_amd import serving as sv
+    specs = sv.poisson_stream(7, rate_per_s=20.0, window_s=window_s, steps=steps)
+    out = sv.serve(ctx, specs, hbm_budget_bytes=64e9, max_active=64)
+    out["workload"] = (f"Poisson 20/s over {window_s} s ({len(specs)} scenarios), 3.6k..1.28M cells log-uniform, "
+                       f"{steps} steps each at dt = 0.01 s, FP32, 64 GB admission budget")
+    return out
 
 
 def cpu_baseline(scen, bs_host, iters_ref, budget_s=15.0):
@@ -249,6 +328,9 @@ def main():
         plain = run_plain(ctx, gas, scen, bs, args.plain)
         v_fp32 = solves / fp32["seconds"]
         v_plain = args.plain / plain["seconds"]
+    precision = None if args.mixed_only else run_precision(ctx, gas, sc)
+    batching = None if args.mixed_only else run_batching(ctx, gas, sc)
+    serving = None if args.mixed_only else run_serving(ctx)
 
     # roofline of the dominant kernel, from HIP events over the timed steps
     # (recorded on the library's stream, around each launch)
@@ -321,6 +403,9 @@ def main():
         "roofline": roof,
         "dominant_kernel_by_time": dom,
         "kernel_time_share": time_share,
+        "precision_1p28M": precision,
+        "batching_64_scenarios": batching,
+        "serving_poisson": serving,
         "cpu_baseline": cpu,
     }
     ctx.close()

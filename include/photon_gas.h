@@ -62,8 +62,13 @@ typedef struct {
     float inv;           /* dx^2/dt, for PHG_STOP_MAXNORM */
     int demean;          /* remove the mean of x on closed zones at the end (gasphase.cpp:628-639) */
     int check_every;     /* iterations between host convergence checks (0 = library default) */
-    int use_graph;       /* capture the iteration sequence in a hipGraph (1) or launch directly (0) */
+    int use_graph;       /* 0: launch directly; 1: replay one captured CG iteration; 2: the set-up and
+                            the first check_every iterations as one captured graph (one launch per
+                            solve when every system stops within them), then as 1 */
     int zero_x;          /* start from x = 0 (x is written, not read) instead of warm-starting */
+    int refine;          /* restarts allowed from the recomputed residual b - A x: in FP32 the
+                            recursive residual can reach tol while the true one stalls higher; a
+                            system whose true residual misses tol is restarted from its x (0 = none) */
 } phg_solve_params;
 
 typedef struct {
@@ -127,6 +132,10 @@ typedef struct {
     int level0_blocks;   /* level-0 blocks per launch to aim for when kchunk = 0 (0 = 4096) */
     int unfused;         /* 1: pre-smoothing as three separate passes instead of the fused one
                             (same results bit for bit; for comparison) */
+    int sweeps0;         /* level-0 red-black sweeps before and after the coarse correction
+                            (0 = 1, the reference's gsColor pair, pressure_mg.cpp:276,296; at most 4) */
+    const int *sys_kchunk;   /* per system (nsys entries, 0 = batch value) or NULL */
+    const int *sys_sweeps0;  /* per system (nsys entries, 0 = batch value) or NULL */
 } phg_batch_opts;
 int phg_batch_create_ex(phg_ctx *ctx, int nsys, const int *grids, const phg_batch_opts *opts, int *batch_out);
 int phg_batch_destroy(phg_ctx *ctx, int batch);

@@ -88,9 +88,15 @@ struct DLevel {
 // scalar slots (FP64) and state slots (int) of one system
 // S_ZS: power-of-two scale applied to the V-cycle input (its residual made
 // O(1) so FP16/BF16 storage stays in range; exact in FP32); S_ZSN: the next one
-enum { S_RZ = 0, S_DQ, S_ALPHA, S_BETA, S_RR, S_BB, S_TOL, S_MAXR, S_INV, S_ZS, S_ZSN, S_NSCAL = 16 };
+// S_RPREV: ||r|| at the start of the current round (restarts must make progress)
+enum { S_RZ = 0, S_DQ, S_ALPHA, S_BETA, S_RR, S_BB, S_TOL, S_MAXR, S_INV, S_ZS, S_ZSN, S_RPREV, S_NSCAL = 16 };
 // ST_FLEX: flexible beta (mixed precision); ST_VC: a CG iteration has run (q is defined)
-enum { ST_ACTIVE = 0, ST_ITERS, ST_MAXIT, ST_STATUS, ST_RULE, ST_ZEROX, ST_XPEND, ST_FLEX, ST_VC, ST_NSTATE = 12 };
+// ST_ITOT: iterations of earlier rounds (restarts from the true residual)
+// ST_RESTART: this round restarts from the recomputed residual (see k_fin_init)
+enum {
+    ST_ACTIVE = 0, ST_ITERS, ST_MAXIT, ST_STATUS, ST_RULE, ST_ZEROX, ST_XPEND, ST_FLEX, ST_VC, ST_ITOT, ST_RESTART,
+    ST_NSTATE = 12
+};
 
 // One system of a batch.  Level 0 is swept by columns: a block takes TB
 // consecutive (i, j) columns of one k-chunk of `kchunk` planes, so that
@@ -100,6 +106,7 @@ struct DSys {
     int nc;                       // coarsest size
     int tiles0;                   // level-0 blocks (stride of the partials)
     int kchunk;                   // planes per level-0 block
+    int nu0;                      // level-0 red-black sweeps before and after the coarse correction
     const double* cinv;           // nc*nc FP64 inverse of the grounded coarsest operator
     const float* b;               // outer CG vectors, FP32 on level-0 cells
     float* x;

@@ -65,6 +65,15 @@ __device__ __forceinline__ void coefs6(const DLevel& L, const void* ax, const vo
 
 __device__ __forceinline__ float diag6(const float a[6]) { return a[0] + a[1] + a[2] + a[3] + a[4] + a[5]; }
 
+// x / diag on level 0: a true division (the reference's gsColor divides;
+// replacing it by a product with the reciprocal face count, one unit in the
+// last place away, made the FP32 iteration diverge on point-source problems
+// with several closed zones)
+template <class CT>
+__device__ __forceinline__ float div_diag(float x, float dg) {
+    return x / dg;
+}
+
 // a cell's coefficients held in registers between their load and their use
 // (the 1-byte mask, or the six values)
 template <class CT>
@@ -132,6 +141,14 @@ __device__ __forceinline__ ColTile col_tile(const DSys& s, int tile) {
     t.k1 = min(L.nz, t.k0 + s.kchunk);
     return t;
 }
+
+// Coarse right-hand sides are restricted by summing children, so each level's
+// is up to 8 times the one above (a point source) while the Galerkin operator
+// grows 4 times: 2-byte storage would overflow a few levels down.  Level l+1
+// therefore holds its vectors scaled by 1/4 (restriction multiplies by 1/4,
+// prolongation by 4).  The scale is a power of two, so FP32 results are
+// bitwise those of the unscaled cycle.
+constexpr float VC_LEVEL_SCALE = 4.0f;
 
 // ---------------------------------------------------------------------------
 // plane march with LDS-staged lateral neighbours
@@ -307,12 +324,16 @@ __device__ __forceinline__ auto cr_pre(const DSys& s, const DLevel& L) {
 // r = b - A x on gas cells (0 elsewhere), or x = 0 and r = b when starting
 // from zero; partials |b|^2 (slot 0), |r|^2 (1), max|r| (2)
 template <class OT>
-__global__ void __launch_bounds__(TB) k_resid_init(const DSys* __restrict__ sys, const int2* __restrict__ tab) {
+// hi: the residual of a restart, with every product of the FP32 x accumulated
+// in FP64 and rounded once (iterative refinement: a residual formed in FP32
+// is rounding noise once it falls to eps |A| |x|)
+__global__ void __launch_bounds__(TB) k_resid_init(const DSys* __restrict__ sys, const int2* __restrict__ tab, int hi) {
     PHG_SYS_PROLOGUE
     const DLevel& L = s.L[0];
     const ColTile t = col_tile(s, e.y);
     const int sxy = L.nx * L.ny;
     double v[2] = {0.0, 0.0};
+    double mag2[1] = {0.0};   // hi: sum of (|b| + |dg x| + sum |a x_nb|)^2, the size of the terms
     float m = 0.0f;
     if (t.valid) {
         const bool zx = s.state[ST_ZEROX] != 0;
@@ -326,6 +347,20 @@ __global__ void __launch_bounds__(TB) k_resid_init(const DSys* __restrict__ sys,
             if (zx) {
                 s.x[q.c] = 0.0f;
                 if (dg > 0) r = b;
+            } else if (dg > 0 && hi) {
+                const int sx = L.nx * L.ny;
+                const bool e0 = q.i > 0, e1 = q.i < L.nx - 1, e2 = q.j > 0, e3 = q.j < L.ny - 1, e4 = q.k > 0,
+                           e5 = q.k < L.nz - 1;
+                const double xc = (double)dg * (double)gld(s.x, q.c);
+                double acc = (double)b - xc, mg = fabs((double)b) + fabs(xc), tm;
+                if (e0) { tm = (double)a[0] * (double)gld(s.x, q.c - 1); acc += tm; mg += fabs(tm); }
+                if (e1) { tm = (double)a[1] * (double)gld(s.x, q.c + 1); acc += tm; mg += fabs(tm); }
+                if (e2) { tm = (double)a[2] * (double)gld(s.x, q.c - L.nx); acc += tm; mg += fabs(tm); }
+                if (e3) { tm = (double)a[3] * (double)gld(s.x, q.c + L.nx); acc += tm; mg += fabs(tm); }
+                if (e4) { tm = (double)a[4] * (double)gld(s.x, q.c - sx); acc += tm; mg += fabs(tm); }
+                if (e5) { tm = (double)a[5] * (double)gld(s.x, q.c + sx); acc += tm; mg += fabs(tm); }
+                r = (float)acc;
+                mag2[0] += mg * mg;
             } else if (dg > 0) {
                 r = b - (dg * s.x[q.c] - nb_sum_f(q, L.nx, L.ny, L.nz, a, [&](int n) { return s.x[n]; }));
             }
@@ -339,6 +374,10 @@ __global__ void __launch_bounds__(TB) k_resid_init(const DSys* __restrict__ sys,
     block_sums<2>(v, out);
     m = block_max(m);
     if (threadIdx.x == 0) out[2] = m;
+    if (hi) {
+        __syncthreads();
+        block_sums<1>(mag2, out + 3);
+    }
 }
 
 // d_k = z + beta d_{k-1} (d_0 = z), q = A d_k with the partial d.q (slot 0)
@@ -438,20 +477,46 @@ __device__ __forceinline__ bool stop_reached(const DSys& s, double rr, float max
 // after k_resid_init: |b|^2, |r|^2, max|r|; stop at iteration 0 if already converged
 __global__ void __launch_bounds__(TB) k_fin_init(const DSys* __restrict__ sys, const int2* __restrict__ tab) {
     PHG_SYS_PROLOGUE
-    __shared__ double res[2];
+    __shared__ double res[3];
     sum_parts(s, 0, &res[0]);
     __syncthreads();
     sum_parts(s, 1, &res[1]);
+    __syncthreads();
+    const bool restart = s.state[ST_RESTART] != 0;
+    if (restart) sum_parts(s, 3, &res[2]);
     const float m = max_parts(s);
     __syncthreads();
     if (threadIdx.x == 0) {
+        // A restart only pays while the true residual stands clear of the noise
+        // floor of an FP32 solution (eps_32 times the size of the terms of
+        // b - A x) and has fallen well below the residual the last round
+        // started from: otherwise it is rounding noise, and iterating on it can
+        // only damage x.  Then the system counts as converged.
+        bool floor_hit = false;
+        const double rn = sqrt(res[1]);
+        if (restart) {
+            s.state[ST_RESTART] = 0;
+            // the term sizes add linearly here, the rounding errors about in
+            // quadrature: a quarter of eps times their norm marks the noise
+            floor_hit = s.state[ST_RULE] == 0 &&
+                        (rn <= 0.25 * 5.9604644775390625e-8 * sqrt(res[2]) || rn > 0.25 * s.scal[S_RPREV]);
+        }
+        if (floor_hit) {
+            // no restart: the solve ends with the previous round's figures
+            s.state[ST_ITERS] = 0;
+            s.state[ST_XPEND] = 0;
+            s.state[ST_STATUS] = 1;
+            s.state[ST_ACTIVE] = 0;
+            return;
+        }
+        s.scal[S_RPREV] = rn;
         s.scal[S_BB] = res[0];
         s.scal[S_RR] = res[1];
         s.scal[S_MAXR] = m;
         s.scal[S_ZS] = s.scal[S_ZSN] = vcycle_scale(res[1], s.L[0].n);
         s.state[ST_ITERS] = 0;
         s.state[ST_XPEND] = 0;
-        const bool done = stop_reached(s, res[1], m);
+        const bool done = floor_hit || stop_reached(s, res[1], m);
         if (done || s.state[ST_MAXIT] <= 0) {
             s.state[ST_STATUS] = done ? 1 : 2;
             s.state[ST_ACTIVE] = 0;
@@ -531,6 +596,25 @@ __global__ void __launch_bounds__(TB) k_fin_beta(const DSys* __restrict__ sys, c
     }
 }
 
+// between rounds of a solve: the owed x += alpha d has been applied (x_flush);
+// a system that converged is started again from its x, with its true
+// residual b - A x (k_resid_init, then k_fin_init stops it at once if that
+// already meets the tolerance); one that hit the cap or broke down stays done
+__global__ void k_refine_reset(const DSys* __restrict__ sys, int nsys) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= nsys) return;
+    int* st = sys[i].state;
+    st[ST_XPEND] = 0;
+    if (st[ST_STATUS] != 1) return;
+    st[ST_ITOT] += st[ST_ITERS];
+    st[ST_ITERS] = 0;
+    st[ST_STATUS] = 0;
+    st[ST_ZEROX] = 0;
+    st[ST_VC] = 0;
+    st[ST_RESTART] = 1;
+    st[ST_ACTIVE] = 1;
+}
+
 __global__ void k_count_active(const DSys* __restrict__ sys, int nsys, int* out) {
     int n = 0;
     for (int i = threadIdx.x; i < nsys; i += blockDim.x) n += sys[i].state[ST_ACTIVE] != 0;
@@ -605,8 +689,9 @@ __global__ void __launch_bounds__(TB) k_pre_red(const DSys* __restrict__ sys, co
 
 // black half sweep z = (r + sum a z_nb) / diag on cells with i+j+k odd (gsColor(1))
 template <class CT, class ZT>
-__global__ void __launch_bounds__(TB) k_rb_black(const DSys* __restrict__ sys, const int2* __restrict__ tab) {
+__global__ void __launch_bounds__(TB) k_rb_black(const DSys* __restrict__ sys, const int2* __restrict__ tab, int sweep) {
     PHG_SYS_PROLOGUE
+    if (sweep >= s.nu0) return;   // sweep of a system that smooths fewer times
     const DLevel& L = s.L[0];
     const ColTile t = col_tile(s, e.y);
     const int sxy = L.nx * L.ny;
@@ -622,7 +707,7 @@ __global__ void __launch_bounds__(TB) k_rb_black(const DSys* __restrict__ sys, c
         o.cf.get(a);
         const float dg = diag6(a);
         float zc = 0.0f;
-        if (dg > 0) zc = (o.r * zs + lat_sum(q, L.nx, L.ny, L.nz, a, p, vp, vn)) / dg;
+        if (dg > 0) zc = div_diag<CT>(o.r * zs + lat_sum(q, L.nx, L.ny, L.nz, a, p, vp, vn), dg);
         st<ZT>(s.z, q.c, zc);
     });
 }
@@ -672,7 +757,7 @@ __global__ void __launch_bounds__(TB) k_post_black(const DSys* __restrict__ sys,
     };
     auto raw = [&](long n) { return ZG{ld<ZT>(z, n), gld(agg, n), 0.0f}; };
     auto mid = [&](ZG& v) { v.c = ld<VT>(t1, max(v.g, 0)); };
-    auto fin = [&](const ZG& v) { return v.z + (v.g >= 0 ? v.c : 0.0f); };
+    auto fin = [&](const ZG& v) { return v.z + (v.g >= 0 ? VC_LEVEL_SCALE * v.c : 0.0f); };
     const float zs = (float)s.scal[S_ZS];
     extern __shared__ float lds[];
     march(s, t, lds, raw, mid, fin, cr_pre<CT>(s, L), [&](int k, float vp, float vc, float vn, const float* p,
@@ -683,7 +768,7 @@ __global__ void __launch_bounds__(TB) k_post_black(const DSys* __restrict__ sys,
         o.cf.get(a);
         const float dg = diag6(a);
         float zc = 0.0f;
-        if (dg > 0) zc = (o.r * zs + lat_sum(q, L.nx, L.ny, L.nz, a, p, vp, vn)) / dg;
+        if (dg > 0) zc = div_diag<CT>(o.r * zs + lat_sum(q, L.nx, L.ny, L.nz, a, p, vp, vn), dg);
         st<ZT>(s.z, q.c, zc);
     });
 }
@@ -691,8 +776,9 @@ __global__ void __launch_bounds__(TB) k_post_black(const DSys* __restrict__ sys,
 // red post-smoothing half sweep (gsColor(0)) fused with the partial r.z
 // (slot 3) and, for flexible CG, q.z (slot 0)
 template <class CT, class VT>
-__global__ void __launch_bounds__(TB) k_post_red(const DSys* __restrict__ sys, const int2* __restrict__ tab) {
+__global__ void __launch_bounds__(TB) k_post_red(const DSys* __restrict__ sys, const int2* __restrict__ tab, int sweep) {
     PHG_SYS_PROLOGUE
+    if (sweep >= s.nu0) return;   // sweep of a system that smooths fewer times
     const DLevel& L = s.L[0];
     const ColTile t = col_tile(s, e.y);
     const int sxy = L.nx * L.ny;
@@ -723,7 +809,7 @@ __global__ void __launch_bounds__(TB) k_post_red(const DSys* __restrict__ sys, c
             o.cf.get(a);
             const float dg = diag6(a);
             zc = 0.0f;
-            if (dg > 0) zc = (r * zs + lat_sum(q, L.nx, L.ny, L.nz, a, p, vp, vn)) / dg;
+            if (dg > 0) zc = div_diag<CT>(r * zs + lat_sum(q, L.nx, L.ny, L.nz, a, p, vp, vn), dg);
             st<VT>(s.z, q.c, zc);
         }
         v[0] += (double)r * zc;
@@ -778,7 +864,7 @@ __host__ __device__ constexpr size_t fuse_lds(int nx) {
            (((size_t)4 * (TB + 4 * nx) + 3) / 4) * 4;
 }
 
-template <class VT, bool FIRST>
+template <class VT, bool FIRST, int M2 = FUSE_M2, int M1 = FUSE_M1>
 __global__ void __launch_bounds__(TB) k_pre_fused(const DSys* __restrict__ sys, const int2* __restrict__ tab) {
     PHG_SYS_PROLOGUE
     const DLevel& L = s.L[0];
@@ -799,10 +885,10 @@ __global__ void __launch_bounds__(TB) k_pre_fused(const DSys* __restrict__ sys, 
     auto slot = [](int p) { return (p + 4) & 3; };           // planes >= -4
 
     // W2 slots of this thread: clamped column, in-grid flag, parity of i + j
-    int col2[FUSE_M2], par2[FUSE_M2];
-    bool in2[FUSE_M2];
+    int col2[M2], par2[M2];
+    bool in2[M2];
 #pragma unroll
-    for (int m = 0; m < FUSE_M2; ++m) {
+    for (int m = 0; m < M2; ++m) {
         const int h = (int)threadIdx.x + m * TB;
         const int col = c0 - 2 * nx + h;
         in2[m] = h < W2 && col >= 0 && col < sxy;
@@ -811,10 +897,10 @@ __global__ void __launch_bounds__(TB) k_pre_fused(const DSys* __restrict__ sys, 
         par2[m] = (cc % nx + cc / nx) & 1;
     }
     // W1 slots: in-grid flag and (i, j)
-    int i1[FUSE_M1], j1[FUSE_M1];
-    bool in1[FUSE_M1];
+    int i1[M1], j1[M1];
+    bool in1[M1];
 #pragma unroll
-    for (int m = 0; m < FUSE_M1; ++m) {
+    for (int m = 0; m < M1; ++m) {
         const int h = (int)threadIdx.x + m * TB;
         const int col = c0 - nx + h;
         in1[m] = h < W1 && col >= 0 && col < sxy;
@@ -823,13 +909,13 @@ __global__ void __launch_bounds__(TB) k_pre_fused(const DSys* __restrict__ sys, 
         i1[m] = cc - j1[m] * nx;
     }
     struct Raw {
-        float r[FUSE_M2], q[FUSE_M2];
-        unsigned m[FUSE_M2];
+        float r[M2], q[M2];
+        unsigned m[M2];
     };
     auto load = [&](int p, Raw& X) {
         const long base = (long)sxy * min(max(p, 0), nz - 1);
 #pragma unroll
-        for (int m = 0; m < FUSE_M2; ++m) {
+        for (int m = 0; m < M2; ++m) {
             X.r[m] = gld(rin, col2[m] + base);
             X.q[m] = FIRST ? 0.0f : gld(s.q, col2[m] + base);
             X.m[m] = gld(L.mask, col2[m] + base);
@@ -842,7 +928,7 @@ __global__ void __launch_bounds__(TB) k_pre_fused(const DSys* __restrict__ sys, 
         uint8_t* M = Mb + slot(p) * W2;
         float* O = Ob + slot(p) * TB;
 #pragma unroll
-        for (int m = 0; m < FUSE_M2; ++m) {
+        for (int m = 0; m < M2; ++m) {
             const int h = (int)threadIdx.x + m * TB;
             if (h >= W2) break;
             float rn = X.r[m];
@@ -868,7 +954,7 @@ __global__ void __launch_bounds__(TB) k_pre_fused(const DSys* __restrict__ sys, 
         const uint8_t* M = Mb + slot(p) * W2;
         float* Z = Zb + slot(p) * W1;
 #pragma unroll
-        for (int m = 0; m < FUSE_M1; ++m) {
+        for (int m = 0; m < M1; ++m) {
             const int h = (int)threadIdx.x + m * TB;
             if (h >= W1) break;
             const int h2 = h + nx;
@@ -882,7 +968,9 @@ __global__ void __launch_bounds__(TB) k_pre_fused(const DSys* __restrict__ sys, 
                     float a[6];
                     cf.get(a);
                     const float dg = diag6(a);
-                    if (dg > 0) z = (v + lat_sum(Cell{0, i1[m], j1[m], p}, nx, ny, nz, a, R + h2, Rm[h2], Rp[h2])) / dg;
+                    if (dg > 0)
+                        z = div_diag<Mask01>(v + lat_sum(Cell{0, i1[m], j1[m], p}, nx, ny, nz, a, R + h2, Rm[h2], Rp[h2]),
+                                             dg);
                 }
             }
             Z[h] = z;
@@ -1018,7 +1106,7 @@ __global__ void __launch_bounds__(TB) k_vc_restrict(const DSys* __restrict__ sys
     }
 #pragma unroll
     for (int g = 0; g < CV_GRP; ++g)
-        if ((G.active >> g) & 1u) st<VT>(sys[G.first + g].L[l + 1].b, I, acc[g]);
+        if ((G.active >> g) & 1u) st<VT>(sys[G.first + g].L[l + 1].b, I, acc[g] * (1.0f / VC_LEVEL_SCALE));
 }
 
 // first Jacobi sweep from zero: x = omega b / diag (jacobi, pressure_mg.cpp:237-247)
@@ -1135,7 +1223,7 @@ __global__ void __launch_bounds__(TB) k_vc_prolong(const DSys* __restrict__ sys,
     for (int g = 0; g < CV_GRP; ++g) {
         if (!((G.active >> g) & 1u)) continue;
         const DSys& sg = sys[G.first + g];
-        st<VT>(sg.L[l].t, I, ld<VT>(sg.L[l].t, I) + ld<VT>(sg.L[l + 1].t, a));
+        st<VT>(sg.L[l].t, I, ld<VT>(sg.L[l].t, I) + VC_LEVEL_SCALE * ld<VT>(sg.L[l + 1].t, a));
     }
 }
 
@@ -1219,11 +1307,16 @@ __global__ void __launch_bounds__(TB) k_vc_coarse(const DSys* __restrict__ sys, 
 // closed-zone mean removal for every system of a batch (gasphase.cpp:628-639):
 // fixed-order FP64 sums over chunks of each zone's cell list, then per zone
 // ---------------------------------------------------------------------------
-__global__ void __launch_bounds__(TB) k_zone_part(const DSys* __restrict__ sys, const int2* __restrict__ tab) {
+// `which`: 0 the solution x (after the solve), 1 the residual r_0 of a system
+// restarted from its true residual, whose rounding can leave a component along
+// the null space (a constant per closed zone) that CG cannot reduce
+__global__ void __launch_bounds__(TB) k_zone_part(const DSys* __restrict__ sys, const int2* __restrict__ tab,
+                                                  int which) {
     const int2 e = tab[blockIdx.x];
     const DSys& s = sys[e.x];
+    if (which == 1 && s.state[ST_ACTIVE] == 0) return;
     const int4 tk = s.ztask[e.y];
-    const float* x = s.x;
+    const float* x = which == 1 ? s.r : s.x;
     double v[1] = {0.0};
     for (int p = tk.y + threadIdx.x; p < tk.z; p += TB) v[0] += (double)x[s.zcells[p]];
     __shared__ double out[1];
@@ -1232,8 +1325,10 @@ __global__ void __launch_bounds__(TB) k_zone_part(const DSys* __restrict__ sys, 
     if (threadIdx.x == 0) s.zpart[e.y] = out[0];
 }
 
-__global__ void __launch_bounds__(TB) k_zone_fin(const DSys* __restrict__ sys, const int2* __restrict__ tab) {
+__global__ void __launch_bounds__(TB) k_zone_fin(const DSys* __restrict__ sys, const int2* __restrict__ tab,
+                                                 int which) {
     const DSys& s = sys[tab[blockIdx.x].x];
+    if (which == 1 && s.state[ST_ACTIVE] == 0) return;
     for (int z = threadIdx.x; z < s.nclosed; z += TB) {
         double acc = 0.0;
         int cnt = 0;
@@ -1245,17 +1340,20 @@ __global__ void __launch_bounds__(TB) k_zone_fin(const DSys* __restrict__ sys, c
     }
 }
 
-__global__ void __launch_bounds__(TB) k_zone_sub(const DSys* __restrict__ sys, const int2* __restrict__ tab) {
+__global__ void __launch_bounds__(TB) k_zone_sub(const DSys* __restrict__ sys, const int2* __restrict__ tab,
+                                                 int which) {
     const int2 e = tab[blockIdx.x];
     const DSys& s = sys[e.x];
     if (s.nclosed == 0) return;
+    if (which == 1 && s.state[ST_ACTIVE] == 0) return;
+    float* x = which == 1 ? s.r : s.x;
     const ColTile t = col_tile(s, e.y);
     if (!t.valid) return;
     const int sxy = s.L[0].nx * s.L[0].ny;
     for (int k = t.k0; k < t.k1; ++k) {
         const int c = t.c2 + sxy * k;
         const int z = s.zclosed[c];
-        if (z >= 0) s.x[c] -= (float)s.zmean[z];
+        if (z >= 0) x[c] -= (float)s.zmean[z];
     }
 }
 

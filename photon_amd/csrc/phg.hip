@@ -88,6 +88,7 @@ struct BatchDev {
     bool binary = false;           // every grid has 0/1 coefficients: level-0 kernels read the face mask
     int maxnx = 0;                 // widest grid: sizes the LDS plane buffers of the level-0 kernels
     bool unfused = false;          // run the separate level-0 smoothing kernels (comparison runs)
+    int maxnu0 = 1;                // most level-0 sweeps of any system
     size_t lds0() const { return 2 * (size_t)(TB + 2 * maxnx) * sizeof(float); }
     std::vector<int> grids;
     std::vector<DSys> h_sys;
@@ -107,8 +108,11 @@ struct BatchDev {
     int* d_count = nullptr;
     int* h_count = nullptr;        // pinned
     std::vector<void*> allocs;
-    hipGraph_t graph = nullptr;
+    hipGraph_t graph = nullptr;          // one CG iteration
     hipGraphExec_t graph_exec = nullptr;
+    hipGraph_t graph_full = nullptr;     // set-up and graph_full_n iterations
+    hipGraphExec_t graph_full_exec = nullptr;
+    int graph_full_n = 0;
 };
 
 }  // namespace phg
@@ -273,6 +277,8 @@ constexpr int kDenseMax = 2048;
 constexpr int kZoneChunk = 8192;
 // level-0 blocks a launch should offer the chip (256 CUs)
 constexpr int kLevel0Blocks = 4096;
+// most level-0 red-black sweeps per side of the V-cycle
+constexpr int kMaxSweeps0 = 4;
 
 // ---------------------------------------------------------------------------
 // the V-cycle and one CG iteration as launch sequences
@@ -284,13 +290,26 @@ void vcycle_t(phg_ctx* ctx, BatchDev* B, int first) {
     const DSys* S = B->d_sys;
     const int2* T0 = B->tab[0];
     const int n0 = B->tab_len[0];
-    if (std::is_same<C0, Mask01>::value && B->maxnx <= FUSE_MAX_NX && !B->unfused) {
+    if (std::is_same<C0, Mask01>::value && B->maxnx <= FUSE_MAX_NX && !B->unfused && B->maxnu0 == 1) {
         // pre-smoothing and residual in one pass
-        if (first) launch(ctx, K_PRE_SMOOTH, k_pre_fused<VT, true>, n0, fuse_lds(B->maxnx), S, T0);
-        else launch(ctx, K_PRE_SMOOTH, k_pre_fused<VT, false>, n0, fuse_lds(B->maxnx), S, T0);
+        // slots per thread for the widest grid: 3 and 2 up to nx = 128, else 4 and 3
+        const size_t lds = fuse_lds(B->maxnx);
+        if (B->maxnx <= 128) {
+            if (first) launch(ctx, K_PRE_SMOOTH, k_pre_fused<VT, true, 3, 2>, n0, lds, S, T0);
+            else launch(ctx, K_PRE_SMOOTH, k_pre_fused<VT, false, 3, 2>, n0, lds, S, T0);
+        } else {
+            if (first) launch(ctx, K_PRE_SMOOTH, k_pre_fused<VT, true>, n0, lds, S, T0);
+            else launch(ctx, K_PRE_SMOOTH, k_pre_fused<VT, false>, n0, lds, S, T0);
+        }
     } else {
+        // red from zero, black, then (red, black) for each further sweep of
+        // the systems that make it, then the residual
         launch(ctx, K_PRE_RED, k_pre_red<C0, float>, n0, 0, S, T0, first);
-        launch(ctx, K_RB_BLACK, k_rb_black<C0, float>, n0, B->lds0(), S, T0);
+        launch(ctx, K_RB_BLACK, k_rb_black<C0, float>, n0, B->lds0(), S, T0, 0);
+        for (int v = 1; v < B->maxnu0; ++v) {
+            launch(ctx, K_POST_RED, k_post_red<C0, float>, n0, B->lds0(), S, T0, v);
+            launch(ctx, K_RB_BLACK, k_rb_black<C0, float>, n0, B->lds0(), S, T0, v);
+        }
         launch(ctx, K_RESID0, k_resid0<C0, float, VT>, n0, B->lds0(), S, T0);
     }
     const std::vector<int4*>& TG = B->tabg;
@@ -313,8 +332,14 @@ void vcycle_t(phg_ctx* ctx, BatchDev* B, int first) {
         launch(ctx, K_VC_JAC, k_vc_jac<CC, VT>, NG[l], 0, S, TG[l], l, 0);
         launch(ctx, K_VC_JAC, k_vc_jac<CC, VT>, NG[l], 0, S, TG[l], l, 1);
     }
+    // post-smoothing mirrors the pre-smoothing (black, red per sweep), so M
+    // stays symmetric; the last red sweep of each system leaves its r.z partials
     launch(ctx, K_POST_BLACK, k_post_black<C0, float, VT>, n0, B->lds0(), S, T0);
-    launch(ctx, K_POST_RED, k_post_red<C0, float>, n0, B->lds0(), S, T0);
+    launch(ctx, K_POST_RED, k_post_red<C0, float>, n0, B->lds0(), S, T0, 0);
+    for (int v = 1; v < B->maxnu0; ++v) {
+        launch(ctx, K_RB_BLACK, k_rb_black<C0, float>, n0, B->lds0(), S, T0, v);
+        launch(ctx, K_POST_RED, k_post_red<C0, float>, n0, B->lds0(), S, T0, v);
+    }
 }
 
 void vcycle(phg_ctx* ctx, BatchDev* B, int first) {
@@ -346,12 +371,14 @@ void cg_iteration(phg_ctx* ctx, BatchDev* B) {
 }
 
 // closed-zone mean removal of every system's x (whatever its state)
-void demean_batch(phg_ctx* ctx, BatchDev* B) {
+// remove the closed-zone means of x (which = 0) or of the residual of the
+// restarted systems (which = 1)
+void demean_batch(phg_ctx* ctx, BatchDev* B, int which = 0) {
     if (B->tab_z_len == 0) return;
     const DSys* S = B->d_sys;
-    launch(ctx, K_ZONE, k_zone_part, B->tab_z_len, 0, S, (const int2*)B->tab_z);
-    launch(ctx, K_ZONE, k_zone_fin, B->nsys, 0, S, (const int2*)B->tab_sys);
-    launch(ctx, K_ZONE, k_zone_sub, B->tab_len[0], 0, S, (const int2*)B->tab[0]);
+    launch(ctx, K_ZONE, k_zone_part, B->tab_z_len, 0, S, (const int2*)B->tab_z, which);
+    launch(ctx, K_ZONE, k_zone_fin, B->nsys, 0, S, (const int2*)B->tab_sys, which);
+    launch(ctx, K_ZONE, k_zone_sub, B->tab_len[0], 0, S, (const int2*)B->tab[0], which);
 }
 
 int upload_sys(phg_ctx* ctx, BatchDev* B) {
@@ -395,6 +422,8 @@ BatchDev* get_batch(phg_ctx* ctx, int id) {
 void destroy_batch(BatchDev* B) {
     if (B->graph_exec) (void)hipGraphExecDestroy(B->graph_exec);
     if (B->graph) (void)hipGraphDestroy(B->graph);
+    if (B->graph_full_exec) (void)hipGraphExecDestroy(B->graph_full_exec);
+    if (B->graph_full) (void)hipGraphDestroy(B->graph_full);
     if (B->h_count) (void)hipHostFree(B->h_count);
     free_all(B->allocs);
     delete B;
@@ -693,8 +722,15 @@ int phg_batch_create(phg_ctx* ctx, int nsys, const int* grids, int* batch_out) {
 
 int phg_batch_create_ex(phg_ctx* ctx, int nsys, const int* grids, const phg_batch_opts* opts, int* batch_out) {
     if (!ctx || nsys <= 0 || !grids || !batch_out) return PHG_EINVAL;
-    if (opts && (opts->kchunk < 0 || opts->level0_blocks < 0 || opts->unfused < 0 || opts->unfused > 1))
+    if (opts && (opts->kchunk < 0 || opts->level0_blocks < 0 || opts->unfused < 0 || opts->unfused > 1 ||
+                 opts->sweeps0 < 0 || opts->sweeps0 > kMaxSweeps0))
         return fail(ctx, PHG_EINVAL, "bad batch options");
+    if (opts)
+        for (int s = 0; s < nsys; ++s) {
+            if (opts->sys_kchunk && opts->sys_kchunk[s] < 0) return fail(ctx, PHG_EINVAL, "bad kchunk of system %d", s);
+            if (opts->sys_sweeps0 && (opts->sys_sweeps0[s] < 0 || opts->sys_sweeps0[s] > kMaxSweeps0))
+                return fail(ctx, PHG_EINVAL, "bad sweep count of system %d", s);
+        }
     HIP_OK(hipSetDevice(ctx->device));
     BatchDev* B = new BatchDev();
     B->nsys = nsys;
@@ -740,8 +776,15 @@ int phg_batch_create_ex(phg_ctx* ctx, int nsys, const int* grids, const phg_batc
         std::memset(&S, 0, sizeof(DSys));
         S.nl = (int)G->lv.size();
         S.nc = G->H.nc;
-        S.kchunk = opts && opts->kchunk > 0 ? std::min(opts->kchunk, G->nz)
-                                            : std::max(1, (G->nz + nkc_target - 1) / nkc_target);
+        const int kc = opts && opts->sys_kchunk && opts->sys_kchunk[s] > 0 ? opts->sys_kchunk[s]
+                       : opts && opts->kchunk > 0                        ? opts->kchunk
+                                                                         : 0;
+        S.kchunk = kc > 0 ? std::min(kc, G->nz) : std::max(1, (G->nz + nkc_target - 1) / nkc_target);
+        const int nu = opts && opts->sys_sweeps0 && opts->sys_sweeps0[s] > 0 ? opts->sys_sweeps0[s]
+                       : opts && opts->sweeps0 > 0                          ? opts->sweeps0
+                                                                            : 1;
+        S.nu0 = nu;
+        B->maxnu0 = std::max(B->maxnu0, nu);
         S.tiles0 = tiles(G->nx * G->ny) * ((G->nz + S.kchunk - 1) / S.kchunk);
         S.cinv = G->cinv;
         S.invd = G->invd;
@@ -903,33 +946,80 @@ int phg_batch_solve(phg_ctx* ctx, int batch, const float* const* b, float* const
     if (!rc) rc = reset_state(ctx, B, P);
     if (rc) return rc;
     const DSys* S = B->d_sys;
-    if (B->binary) launch(ctx, K_RESID_INIT, k_resid_init<Mask01>, B->tab_len[0], 0, S, B->tab[0]);
-    else launch(ctx, K_RESID_INIT, k_resid_init<float>, B->tab_len[0], 0, S, B->tab[0]);
-    launch(ctx, K_FIN, k_fin_init, B->nsys, 0, S, B->tab_sys);
-    vcycle(ctx, B, 1);
-    launch(ctx, K_FIN, k_fin_rz0, B->nsys, 0, S, B->tab_sys);
+    // r_0 = b - A x_0, the stop test at iteration 0, z_0 = M^-1 r_0, r.z;
+    // a restart projects its residual onto the range of A first
+    auto setup = [&](bool restart) {
+        const int hi = restart ? 1 : 0;
+        if (B->binary) launch(ctx, K_RESID_INIT, k_resid_init<Mask01>, B->tab_len[0], 0, S, B->tab[0], hi);
+        else launch(ctx, K_RESID_INIT, k_resid_init<float>, B->tab_len[0], 0, S, B->tab[0], hi);
+        if (restart) demean_batch(ctx, B, 1);
+        launch(ctx, K_FIN, k_fin_init, B->nsys, 0, S, B->tab_sys);
+        vcycle(ctx, B, 1);
+        launch(ctx, K_FIN, k_fin_rz0, B->nsys, 0, S, B->tab_sys);
+    };
     const int check = P->check_every > 0 ? P->check_every : 4;
     const bool graph = P->use_graph && !ctx->prof;
-    if (graph && !B->graph_exec) {
-        HIP_OK(hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal));
-        cg_iteration(ctx, B);
-        HIP_OK(hipStreamEndCapture(ctx->stream, &B->graph));
-        HIP_OK(hipGraphInstantiate(&B->graph_exec, B->graph, nullptr, nullptr, 0));
-    }
-    int done = 0;
-    while (done < P->max_iters) {
-        const int k = std::min(check, P->max_iters - done);
-        for (int i = 0; i < k; ++i) {
-            if (graph) HIP_OK(hipGraphLaunch(B->graph_exec, ctx->stream));
-            else cg_iteration(ctx, B);
+    const bool whole = P->use_graph == 2 && !ctx->prof;
+    // rounds: the first from x_0; each later one restarts the systems that
+    // converged from their x and the true residual b - A x (those already
+    // within tol stop at once); the cap applies per round
+    for (int round = 0;; ++round) {
+        int done = 0;
+        bool live = true;
+        if (whole && round == 0) {
+            // the set-up and the first `check` iterations as one graph: one launch
+            // per solve when every system stops within them (a system that has
+            // stopped leaves each later kernel at once)
+            const int N = std::min(check, P->max_iters);
+            if (!B->graph_full_exec || B->graph_full_n != N) {
+                if (B->graph_full_exec) (void)hipGraphExecDestroy(B->graph_full_exec);
+                if (B->graph_full) (void)hipGraphDestroy(B->graph_full);
+                B->graph_full_exec = nullptr;
+                B->graph_full = nullptr;
+                HIP_OK(hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal));
+                setup(false);
+                for (int i = 0; i < N; ++i) cg_iteration(ctx, B);
+                HIP_OK(hipStreamEndCapture(ctx->stream, &B->graph_full));
+                HIP_OK(hipGraphInstantiate(&B->graph_full_exec, B->graph_full, nullptr, nullptr, 0));
+                B->graph_full_n = N;
+            }
+            HIP_OK(hipGraphLaunch(B->graph_full_exec, ctx->stream));
+            done = N;
+            launch(ctx, K_COUNT, k_count_active, 1, 0, S, B->nsys, B->d_count);
+            HIP_OK(hipMemcpyAsync(B->h_count, B->d_count, sizeof(int), hipMemcpyDeviceToHost, ctx->stream));
+            HIP_OK(hipStreamSynchronize(ctx->stream));
+            live = *B->h_count != 0;
+        } else {
+            setup(round > 0);
+            if (round > 0) {   // a restart round: iterate only if some true residual missed tol
+                launch(ctx, K_COUNT, k_count_active, 1, 0, S, B->nsys, B->d_count);
+                HIP_OK(hipMemcpyAsync(B->h_count, B->d_count, sizeof(int), hipMemcpyDeviceToHost, ctx->stream));
+                HIP_OK(hipStreamSynchronize(ctx->stream));
+                live = *B->h_count != 0;
+            }
         }
-        done += k;
-        launch(ctx, K_COUNT, k_count_active, 1, 0, S, B->nsys, B->d_count);
-        HIP_OK(hipMemcpyAsync(B->h_count, B->d_count, sizeof(int), hipMemcpyDeviceToHost, ctx->stream));
-        HIP_OK(hipStreamSynchronize(ctx->stream));
-        if (*B->h_count == 0) break;
+        if (graph && live && !B->graph_exec) {
+            HIP_OK(hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal));
+            cg_iteration(ctx, B);
+            HIP_OK(hipStreamEndCapture(ctx->stream, &B->graph));
+            HIP_OK(hipGraphInstantiate(&B->graph_exec, B->graph, nullptr, nullptr, 0));
+        }
+        while (live && done < P->max_iters) {
+            const int k = std::min(check, P->max_iters - done);
+            for (int i = 0; i < k; ++i) {
+                if (graph) HIP_OK(hipGraphLaunch(B->graph_exec, ctx->stream));
+                else cg_iteration(ctx, B);
+            }
+            done += k;
+            launch(ctx, K_COUNT, k_count_active, 1, 0, S, B->nsys, B->d_count);
+            HIP_OK(hipMemcpyAsync(B->h_count, B->d_count, sizeof(int), hipMemcpyDeviceToHost, ctx->stream));
+            HIP_OK(hipStreamSynchronize(ctx->stream));
+            if (*B->h_count == 0) break;
+        }
+        launch(ctx, K_XFLUSH, k_x_flush, B->tab_len[0], 0, S, B->tab[0]);
+        if (round >= P->refine) break;
+        launch(ctx, K_FIN, k_refine_reset, (B->nsys + TB - 1) / TB, 0, S, B->nsys);
     }
-    launch(ctx, K_XFLUSH, k_x_flush, B->tab_len[0], 0, S, B->tab[0]);
     if (P->demean) demean_batch(ctx, B);
     HIP_OK(hipGetLastError());
     if (stats) {
@@ -941,7 +1031,7 @@ int phg_batch_solve(phg_ctx* ctx, int batch, const float* const* b, float* const
         for (int s = 0; s < B->nsys; ++s) {
             const double* sc = &scal[(size_t)s * S_NSCAL];
             const int* st = &state[(size_t)s * ST_NSTATE];
-            stats[s].iters = st[ST_ITERS];
+            stats[s].iters = st[ST_ITOT] + st[ST_ITERS];
             stats[s].status = st[ST_STATUS];
             stats[s].relres = sc[S_BB] > 0 ? std::sqrt(sc[S_RR]) / std::sqrt(sc[S_BB]) : 0.0;
             stats[s].maxres = (double)((float)sc[S_MAXR] / (float)sc[S_INV]);

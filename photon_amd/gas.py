@@ -38,11 +38,13 @@ class PhgError(RuntimeError):
 
 class SolveParams(C.Structure):
     _fields_ = [("tol", C.c_double), ("max_iters", C.c_int), ("stop_rule", C.c_int), ("inv", C.c_float),
-                ("demean", C.c_int), ("check_every", C.c_int), ("use_graph", C.c_int), ("zero_x", C.c_int)]
+                ("demean", C.c_int), ("check_every", C.c_int), ("use_graph", C.c_int), ("zero_x", C.c_int),
+                ("refine", C.c_int)]
 
 
 class BatchOpts(C.Structure):
-    _fields_ = [("kchunk", C.c_int), ("level0_blocks", C.c_int), ("unfused", C.c_int)]
+    _fields_ = [("kchunk", C.c_int), ("level0_blocks", C.c_int), ("unfused", C.c_int), ("sweeps0", C.c_int),
+                ("sys_kchunk", C.c_void_p), ("sys_sweeps0", C.c_void_p)]
 
 
 class SolveStats(C.Structure):
@@ -202,9 +204,12 @@ class Context:
 
 
 def solve_params(tol=1e-6, max_iters=200, stop_rule=STOP_REL_L2, inv=1.0, demean=True, check_every=4,
-                 use_graph=False, zero_x=False):
-    return SolveParams(tol, max_iters, stop_rule, inv, 1 if demean else 0, check_every, 1 if use_graph else 0,
-                       1 if zero_x else 0)
+                 use_graph=False, zero_x=False, refine=0):
+    """use_graph: False/0 direct launches, True/1 one captured iteration
+    replayed, 2 the whole solve (set-up + check_every iterations) as one graph."""
+    g = int(use_graph) if not isinstance(use_graph, bool) else (1 if use_graph else 0)
+    return SolveParams(tol, max_iters, stop_rule, inv, 1 if demean else 0, check_every, g, 1 if zero_x else 0,
+                       refine)
 
 
 class Grid:
@@ -262,12 +267,18 @@ class Grid:
 class Batch:
     """A fixed set of systems solved together (phg_batch_create / _solve)."""
 
-    def __init__(self, ctx: Context, grids, kchunk=0, level0_blocks=0, unfused=False):
+    def __init__(self, ctx: Context, grids, kchunk=0, level0_blocks=0, unfused=False, sweeps0=0, sys_kchunk=None,
+                 sys_sweeps0=None):
         self.ctx = ctx
         self.grids = list(grids)
         ids = np.array([g.id for g in self.grids], np.int32)
         bid = C.c_int()
-        opts = BatchOpts(kchunk, level0_blocks, 1 if unfused else 0)
+        kc = None if sys_kchunk is None else np.ascontiguousarray(sys_kchunk, np.int32)
+        sw = None if sys_sweeps0 is None else np.ascontiguousarray(sys_sweeps0, np.int32)
+        for a in (kc, sw):
+            assert a is None or len(a) == len(ids)
+        opts = BatchOpts(kchunk, level0_blocks, 1 if unfused else 0, sweeps0,
+                         None if kc is None else kc.ctypes.data, None if sw is None else sw.ctypes.data)
         ctx._check(ctx.L.phg_batch_create_ex(ctx.h, len(ids), ids.ctypes.data, C.byref(opts), C.byref(bid)))
         self.id = bid.value
 
@@ -330,6 +341,8 @@ class GasSolver:
         for name in self.CELL_FIELDS:
             self.d[name] = ctx.to_device(np.full(n, init.get(name, 0.0), np.float32))
         self._tmp = [ctx.array(n) for _ in range(10)]
+        self.d["b"] = ctx.array(n)       # right-hand side of the split projection (rhs / correct)
+        self.d["diag"] = ctx.array(n)
         self.params = solve_params(tol=tol, max_iters=max_iters, stop_rule=stop_rule)
         self.pressureIters = 0
         self.pressureResidual = 0.0
@@ -355,6 +368,22 @@ class GasSolver:
         self.pressureRelRes = st.relres
         self.maxVel = vm.value
         return st.iters
+
+    def rhs(self, dt):
+        """The right-hand side and diagonal of project() into d["b"], d["diag"]
+        (src/sim/gasphase.cpp:528-549); p is zeroed in solids."""
+        d = self.d
+        self.ctx._check(self.ctx.L.phg_rhs(self.ctx.h, self.grid.id, dt, d["u"].ptr, d["v"].ptr, d["w"].ptr,
+                                           d["S"].ptr, d["b"].ptr, d["diag"].ptr, d["p"].ptr))
+
+    def correct(self, dt):
+        """Face-velocity correction from p (src/sim/gasphase.cpp:640-676); sets maxVel."""
+        vm = C.c_float()
+        d = self.d
+        self.ctx._check(self.ctx.L.phg_correct(self.ctx.h, self.grid.id, dt, d["p"].ptr, d["u"].ptr, d["v"].ptr,
+                                               d["w"].ptr, C.byref(vm)))
+        self.maxVel = vm.value
+        return vm.value
 
     def energy_and_expansion(self, dt):
         md = C.c_float()

@@ -272,6 +272,35 @@ def batch_grid(seed: int) -> Grid:
     return house(seed, nx, ny, nz, 0.2, nw, no)
 
 
+def burner_field(grid: Grid, seed: int, rho: float = 1.2, cp: float = 1005.0, T: float = 293.15) -> np.ndarray:
+    """Divergence source of a square gas burner: a 2x2-cell burner on a
+    seeded gas floor cell, heat release uniform in [10, 500] kW, expanding the
+    gas of its cells at S = HRR / (cells rho cp T V) (the heat term of
+    energyAndExpansion, src/sim/gasphase.cpp:251-271, for one step), then zero
+    mean per closed zone (a closed room's pressure rises uniformly), FP32."""
+    rng = Rng(seed ^ 0xB0B0)
+    nx, ny = grid.nx, grid.ny
+    blk = grid.blocked.reshape(grid.nz, ny, nx)
+    floor = [(i, j) for j in range(ny - 1) for i in range(nx - 1)
+             if not blk[0, j:j + 2, i:i + 2].any()]
+    i0, j0 = floor[rng.randint(0, len(floor) - 1)]
+    hrr = 10e3 + (500e3 - 10e3) * rng.uniform()
+    V = grid.dx ** 3
+    s = np.zeros(grid.n, np.float64)
+    for j in (j0, j0 + 1):
+        for i in (i0, i0 + 1):
+            s[i + nx * j] = hrr / (4 * rho * cp * T * V)
+    if grid.zone is not None and len(grid.zone_open):
+        z = grid.zone
+        sel = (grid.blocked == 0) & (z >= 0)
+        nzn = len(grid.zone_open)
+        sums = np.bincount(z[sel], weights=s[sel], minlength=nzn)
+        cnts = np.bincount(z[sel], minlength=nzn)
+        mean = np.where(cnts > 0, sums / np.maximum(cnts, 1), 0.0)
+        s[sel] -= np.where(grid.zone_open == 0, mean, 0.0)[z[sel]]
+    return s.astype(np.float32)
+
+
 def divergence_field(grid: Grid, seed: int) -> np.ndarray:
     """S ~ N(0,1) on gas cells, zero mean per closed zone (FP64), 0 in solids;
     returned as float32.  This is the right-hand side of the solves up to the

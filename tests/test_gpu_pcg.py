@@ -199,3 +199,31 @@ def test_fractional_coefficients(ctx, prec):
     print(f"vented prec={prec} iters={st['iters']} err_fp64={rel(x, x64):.2e}")
     assert st["status"] == gas.CONVERGED and rel(x, x64) < 1e-5
     G.destroy()
+
+
+def test_precision_config_1p28M_cells(ctx):
+    """The precision workload: the house generator refined to 0.1 m
+    (160x100x80 = 1.28M cells, 5 levels).  FP16 and BF16 V-cycle storage
+    against FP32 storage: iterations and error against the FP64 oracle
+    (solved to 1e-12), per right-hand side."""
+    g = sc.house(1, 160, 100, 80, 0.1)
+    H = po.Hierarchy(g)
+    rows = []
+    for seed in (5, 6):
+        b = sc.rhs_from_divergence(g, sc.divergence_field(g, seed), DT)
+        x64, it64, _ = H.pcg64(b)
+        for prec in (gas.PREC_FP32, gas.PREC_MIXED_F16, gas.PREC_MIXED_BF16):
+            G = gas.Grid(ctx, g, precision=prec)
+            db, dx = ctx.to_device(b), ctx.array(g.n)
+            st = G.solve(db, dx, gas.solve_params(tol=1e-6, max_iters=300, zero_x=True))
+            x = dx.download()
+            err = rel(x, x64)
+            rows.append((seed, prec, st["iters"], st["relres"], err))
+            print(f"1.28M seed={seed} prec={prec} iters={st['iters']} relres={st['relres']:.3e} err_fp64={err:.3e}")
+            assert st["status"] == gas.CONVERGED and st["relres"] <= 1e-6
+            assert err < 1e-5
+            G.destroy()
+    for seed in (5, 6):
+        its = {p: it for s, p, it, _, _ in rows if s == seed}
+        assert its[gas.PREC_MIXED_F16] <= its[gas.PREC_FP32] + 2
+        assert its[gas.PREC_MIXED_BF16] <= its[gas.PREC_FP32] + 3

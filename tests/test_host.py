@@ -80,3 +80,23 @@ def test_no_gpu_gives_clear_error_not_fallback():
             "else:\n    print('context')\n") % ROOT
     out = subprocess.run([sys.executable, "-c", code], capture_output=True, text=True, timeout=120).stdout
     assert "refused" in out or "context" in out
+
+
+def test_serving_stream_and_sizes():
+    from photon_amd import serving as sv
+    specs = sv.poisson_stream(5, rate_per_s=20.0, window_s=30.0, steps=10)
+    # Poisson arrivals: about rate * window requests, sorted arrival times
+    assert 480 < len(specs) < 720
+    t = np.array([s.arrival_s for s in specs])
+    assert np.all(np.diff(t) > 0) and t[-1] <= 30.0
+    # sizes log-uniform in [3.6k, 1.28M]: the log-size median sits mid-range
+    logc = np.log([s.cells for s in specs])
+    assert np.log(3600) - 0.01 <= logc.min() and logc.max() <= np.log(1_280_000) + 0.01
+    assert abs(np.median(logc) - 0.5 * (np.log(3600) + np.log(1_280_000))) < 0.6
+    assert all(10.0 <= s.hrr_kw <= 500.0 for s in specs)
+    # the grid keeps the room's 4 : 3 : 2.4 aspect and the requested size
+    g = sv.grid_for(sv.JobSpec(0, 1, 3600, 1, 100.0))
+    assert (g.nx, g.ny, g.nz) == (20, 15, 12)
+    g = sv.grid_for(sv.JobSpec(0, 1, 200_000, 1, 100.0))
+    assert abs(g.n / 200_000 - 1) < 0.1
+    assert sv.footprint_bytes(g) > sv.footprint_bytes(sv.grid_for(sv.JobSpec(0, 1, 3600, 1, 1.0)))
