@@ -68,7 +68,9 @@ typedef struct {
     int zero_x;          /* start from x = 0 (x is written, not read) instead of warm-starting */
     int refine;          /* restarts allowed from the recomputed residual b - A x: in FP32 the
                             recursive residual can reach tol while the true one stalls higher; a
-                            system whose true residual misses tol is restarted from its x (0 = none) */
+                            system whose true residual misses tol (and stands clear of FP32 noise)
+                            is restarted from its x (0 = none).  With refine > 0 every residual
+                            formed from x, a warm start's included, is accumulated in FP64 */
 } phg_solve_params;
 
 typedef struct {
@@ -159,6 +161,13 @@ int phg_project(phg_ctx *ctx, int grid, float dt, float *u, float *v, float *w, 
 int phg_rhs(phg_ctx *ctx, int grid, float dt, const float *u, const float *v, const float *w, const float *S,
             float *b, float *diag, float *p);
 int phg_zone_mean(phg_ctx *ctx, int grid, float *p);
+/* The compatible divergence source of Solver::leakage (gasphase.cpp:487-498)
+ * for grids without crack faces: on each closed zone S -= (integral of S) /
+ * zone volume, i.e. its mean, so the pressure equation of a sealed room is
+ * solvable (the zone's pressure rise rate is gamma P times that shift).  S:
+ * device, in/out.  shift (host, one float per zone, 0 on open zones) may be
+ * NULL; when given, the call waits for the device. */
+int phg_leakage(phg_ctx *ctx, int grid, float *S, float *shift);
 int phg_correct(phg_ctx *ctx, int grid, float dt, const float *p, float *u, float *v, float *w, float *max_vel);
 
 /* ---- divergence source (Solver::energyAndExpansion, gasphase.cpp:251-271)

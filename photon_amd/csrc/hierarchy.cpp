@@ -2,13 +2,58 @@
 // Algorithm: src/sim/pressure_mg.cpp:28-191 (connected aggregation, Galerkin
 // coarse operators); zones: src/sim/gasphase.cpp:111-158.
 #include <algorithm>
+#include <atomic>
 #include <cmath>
 #include <string>
+#include <thread>
 
 #include "hierarchy.hpp"
 
 namespace phg {
 namespace {
+
+// Host threads for the dense coarsest inverse (O(n^3) with n up to 2048).
+// Every element is computed by one thread with the same operation order as a
+// serial loop, so the result does not depend on the thread count.
+int host_threads(int work) {
+    const int hw = (int)std::max(1u, std::thread::hardware_concurrency());
+    return std::max(1, std::min({hw, 16, work}));
+}
+
+// f(i) for i in [0, n), rows handed out one at a time (unbalanced triangles)
+template <class F>
+void parallel_rows(int n, F f) {
+    const int T = host_threads(n / 8);
+    if (T <= 1) {
+        for (int i = 0; i < n; ++i) f(i);
+        return;
+    }
+    std::atomic<int> next{0};
+    auto work = [&]() {
+        for (int i = next.fetch_add(1); i < n; i = next.fetch_add(1)) f(i);
+    };
+    std::vector<std::thread> pool;
+    for (int t = 1; t < T; ++t) pool.emplace_back(work);
+    work();
+    for (auto& th : pool) th.join();
+}
+
+// spinning barrier for a fixed team (one crossing per Cholesky column)
+struct SpinBarrier {
+    const int n;
+    std::atomic<int> count{0};
+    std::atomic<int> phase{0};
+    explicit SpinBarrier(int n_) : n(n_) {}
+    void wait() {
+        const int ph = phase.load(std::memory_order_acquire);
+        if (count.fetch_add(1, std::memory_order_acq_rel) + 1 == n) {
+            count.store(0, std::memory_order_relaxed);
+            phase.store(ph + 1, std::memory_order_release);
+        } else {
+            while (phase.load(std::memory_order_acquire) == ph) std::this_thread::yield();
+        }
+    }
+};
 
 // Disjoint sets whose representative is always the smallest member, so the
 // partition and the representative do not depend on the order of joins.
@@ -178,36 +223,50 @@ void dense_inverse(HostHierarchy& H) {
         for (int I : members[r])
             for (int J : members[r]) A[(size_t)I * n + J] += s;
     }
-    // Cholesky A = G G^T (lower G stored in A)
-    for (int j = 0; j < n; ++j) {
-        double d = A[(size_t)j * n + j];
-        for (int k = 0; k < j; ++k) d -= A[(size_t)j * n + k] * A[(size_t)j * n + k];
-        d = d > 0 ? std::sqrt(d) : 1e-300;
-        A[(size_t)j * n + j] = d;
-        for (int i = j + 1; i < n; ++i) {
-            double s = A[(size_t)i * n + j];
-            for (int k = 0; k < j; ++k) s -= A[(size_t)i * n + k] * A[(size_t)j * n + k];
-            A[(size_t)i * n + j] = s / d;
-        }
+    // Cholesky A = G G^T (lower G stored in A), column by column; the rows
+    // below the diagonal of a column are shared out among the team, and every
+    // thread forms the column's diagonal itself from the finished row j
+    {
+        const int T = host_threads(n / 64);
+        SpinBarrier bar(T);
+        auto team = [&](int t) {
+            for (int j = 0; j < n; ++j) {
+                double d = A[(size_t)j * n + j];
+                for (int k = 0; k < j; ++k) d -= A[(size_t)j * n + k] * A[(size_t)j * n + k];
+                d = d > 0 ? std::sqrt(d) : 1e-300;
+                for (int i = j + 1 + t; i < n; i += T) {
+                    double s = A[(size_t)i * n + j];
+                    for (int k = 0; k < j; ++k) s -= A[(size_t)i * n + k] * A[(size_t)j * n + k];
+                    A[(size_t)i * n + j] = s / d;
+                }
+                bar.wait();
+                if (t == 0) A[(size_t)j * n + j] = d;
+            }
+        };
+        std::vector<std::thread> pool;
+        for (int t = 1; t < T; ++t) pool.emplace_back(team, t);
+        team(0);
+        for (auto& th : pool) th.join();
     }
-    // G^-1 (lower), then A^-1 = G^-T G^-1
+    // G^-1 (lower, columns independent), then A^-1 = G^-T G^-1 (rows independent)
     std::vector<double> Gi((size_t)n * n, 0.0);
-    for (int j = 0; j < n; ++j) {
+    parallel_rows(n, [&](int j) {
         Gi[(size_t)j * n + j] = 1.0 / A[(size_t)j * n + j];
         for (int i = j + 1; i < n; ++i) {
             double s = 0;
             for (int k = j; k < i; ++k) s -= A[(size_t)i * n + k] * Gi[(size_t)k * n + j];
             Gi[(size_t)i * n + j] = s / A[(size_t)i * n + i];
         }
-    }
+    });
     H.cinv.assign((size_t)n * n, 0.0);
-    for (int i = 0; i < n; ++i)
+    parallel_rows(n, [&](int i) {
         for (int j = 0; j <= i; ++j) {
             double s = 0;
             for (int k = i; k < n; ++k) s += Gi[(size_t)k * n + i] * Gi[(size_t)k * n + j];
             H.cinv[(size_t)i * n + j] = s;
             H.cinv[(size_t)j * n + i] = s;
         }
+    });
     for (int I = 0; I < n; ++I)
         if (H.cmask[I])
             for (int J = 0; J < n; ++J) H.cinv[(size_t)I * n + J] = H.cinv[(size_t)J * n + I] = 0.0;

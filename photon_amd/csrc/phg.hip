@@ -949,7 +949,10 @@ int phg_batch_solve(phg_ctx* ctx, int batch, const float* const* b, float* const
     // r_0 = b - A x_0, the stop test at iteration 0, z_0 = M^-1 r_0, r.z;
     // a restart projects its residual onto the range of A first
     auto setup = [&](bool restart) {
-        const int hi = restart ? 1 : 0;
+        // with refinement on, every residual formed from x (a warm start's
+        // first one as well as a restart's) accumulates in FP64: an FP32 one
+        // is noise once it nears eps |A| |x|, where a warm start begins
+        const int hi = restart || P->refine > 0 ? 1 : 0;
         if (B->binary) launch(ctx, K_RESID_INIT, k_resid_init<Mask01>, B->tab_len[0], 0, S, B->tab[0], hi);
         else launch(ctx, K_RESID_INIT, k_resid_init<float>, B->tab_len[0], 0, S, B->tab[0], hi);
         if (restart) demean_batch(ctx, B, 1);
@@ -1098,6 +1101,31 @@ int phg_rhs(phg_ctx* ctx, int grid, float dt, const float* u, const float* v, co
     if (!(dt > 0)) return fail(ctx, PHG_EINVAL, "dt must be positive");
     const float inv = G->dx * G->dx / dt;
     launch(ctx, K_RHS, k_rhs, tiles(G->n), 0, G->dg(), G->dx, inv, u, v, w, S, b, diag, p);
+    HIP_OK(hipGetLastError());
+    return PHG_OK;
+}
+
+int phg_leakage(phg_ctx* ctx, int grid, float* S, float* shift) {
+    if (!ctx || !S) return PHG_EINVAL;
+    BatchDev* B = nullptr;
+    int rc = single_batch(ctx, grid, &B);
+    if (rc) return rc;
+    GridDev* G = get_grid(ctx, grid);
+    float* saved = B->h_sys[0].x;
+    B->h_sys[0].x = S;   // the closed-zone mean removal of x, applied to S
+    rc = upload_sys(ctx, B);
+    B->h_sys[0].x = saved;
+    if (rc) return rc;
+    demean_batch(ctx, B);
+    if (shift) {
+        std::vector<double> zm(std::max(G->nclosed, 1));
+        if (G->nclosed)
+            HIP_OK(hipMemcpyAsync(zm.data(), B->h_sys[0].zmean, G->nclosed * sizeof(double), hipMemcpyDeviceToHost,
+                                  ctx->stream));
+        HIP_OK(hipStreamSynchronize(ctx->stream));
+        int k = 0;
+        for (size_t z = 0; z < G->zone_open.size(); ++z) shift[z] = G->zone_open[z] ? 0.0f : (float)zm[k++];
+    }
     HIP_OK(hipGetLastError());
     return PHG_OK;
 }

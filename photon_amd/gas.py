@@ -28,7 +28,7 @@ EXPORTS = (
     "phg_grid_zones", "phg_apply", "phg_precondition", "phg_batch_create", "phg_batch_destroy",
     "phg_batch_solve", "phg_solve", "phg_project", "phg_rhs", "phg_zone_mean", "phg_correct",
     "phg_energy_expansion", "phg_advect_scalars", "phg_profile_enable", "phg_profile_get", "phg_profile_reset",
-    "phg_hierarchy_shape", "phg_batch_create_ex", "phg_copy",
+    "phg_hierarchy_shape", "phg_batch_create_ex", "phg_copy", "phg_leakage",
 )
 
 
@@ -93,6 +93,7 @@ def load_library():
     L.phg_project.argtypes = [vp, i32, f32, vp, vp, vp, vp, vp, C.POINTER(SolveParams), vp, C.POINTER(f32)]
     L.phg_rhs.argtypes = [vp, i32, f32, vp, vp, vp, vp, vp, vp, vp]
     L.phg_zone_mean.argtypes = [vp, i32, vp]
+    L.phg_leakage.argtypes = [vp, i32, vp, vp]
     L.phg_correct.argtypes = [vp, i32, f32, vp, vp, vp, vp, C.POINTER(f32)]
     L.phg_energy_expansion.argtypes = [vp, i32, f32, vp, vp, vp, vp, vp, C.POINTER(f32)]
     L.phg_advect_scalars.argtypes = [vp, i32, f32, vp, vp, vp, vp, vp, vp, vp]
@@ -369,6 +370,17 @@ class GasSolver:
         self.maxVel = vm.value
         return st.iters
 
+    def leakage(self, report=False, field="S"):
+        """The compatible source of Solver::leakage for grids without cracks
+        (src/sim/gasphase.cpp:487-498): S -= its mean on every closed zone.
+        With `report`, returns the per-zone shift (1/s; 0 on open zones).
+        `field` = "b" applies the same projection to the right-hand side."""
+        nz = len(self.scen.zone_open) if self.scen.zone_open is not None else 0
+        out = np.zeros(max(nz, 1), np.float32)
+        self.ctx._check(self.ctx.L.phg_leakage(self.ctx.h, self.grid.id, self.d[field].ptr,
+                                               out.ctypes.data if report else None))
+        return out[:nz] if report else None
+
     def rhs(self, dt):
         """The right-hand side and diagonal of project() into d["b"], d["diag"]
         (src/sim/gasphase.cpp:528-549); p is zeroed in solids."""
@@ -376,22 +388,27 @@ class GasSolver:
         self.ctx._check(self.ctx.L.phg_rhs(self.ctx.h, self.grid.id, dt, d["u"].ptr, d["v"].ptr, d["w"].ptr,
                                            d["S"].ptr, d["b"].ptr, d["diag"].ptr, d["p"].ptr))
 
-    def correct(self, dt):
-        """Face-velocity correction from p (src/sim/gasphase.cpp:640-676); sets maxVel."""
+    def correct(self, dt, report=True):
+        """Face-velocity correction from p (src/sim/gasphase.cpp:640-676); sets
+        maxVel when `report` (which waits for the device)."""
         vm = C.c_float()
         d = self.d
         self.ctx._check(self.ctx.L.phg_correct(self.ctx.h, self.grid.id, dt, d["p"].ptr, d["u"].ptr, d["v"].ptr,
-                                               d["w"].ptr, C.byref(vm)))
-        self.maxVel = vm.value
-        return vm.value
+                                               d["w"].ptr, C.byref(vm) if report else None))
+        if report:
+            self.maxVel = vm.value
+        return self.maxVel
 
-    def energy_and_expansion(self, dt):
+    def energy_and_expansion(self, dt, report=True):
+        """Sets maxDiv when `report` (which waits for the device)."""
         md = C.c_float()
         d = self.d
         self.ctx._check(self.ctx.L.phg_energy_expansion(self.ctx.h, self.grid.id, dt, d["T"].ptr, d["S"].ptr,
-                                                        d["Pb"].ptr, d["Q"].ptr, d["massSrc"].ptr, C.byref(md)))
-        self.maxDiv = md.value
-        return md.value
+                                                        d["Pb"].ptr, d["Q"].ptr, d["massSrc"].ptr,
+                                                        C.byref(md) if report else None))
+        if report:
+            self.maxDiv = md.value
+        return self.maxDiv
 
     def advect_scalars(self, dt):
         d = self.d

@@ -9,7 +9,8 @@ scenarios:
   footprint fits the budget and a slot is free, at any step boundary (so work
   that arrives while others run joins them at once);
 - every scenario keeps its fields resident on the device; a step advances all
-  admitted scenarios together: energyAndExpansion, the right-hand side, ONE
+  admitted scenarios together: energyAndExpansion, the closed-zone source
+  shift of leakage (sealed rooms), the right-hand side, ONE
   batched MG-PCG launch sequence for all their pressure solves, the velocity
   correction and advectScalars (src/sim/solver.cpp:252-257, the phases of
   Solver::step this library implements);
@@ -33,7 +34,8 @@ from . import scenarios as sc
 
 # device bytes per cell: fields (u v w, 16 cell fields, 10 advection
 # temporaries, b, diag), geometry and hierarchy, and the solver's work vectors
-BYTES_PER_CELL = 230
+# (the step batch and the grid's own one-system batch)
+BYTES_PER_CELL = 260
 # the dense coarsest inverse is at most 2048^2 FP64
 COARSE_BYTES_MAX = 2048 * 2048 * 8
 
@@ -87,14 +89,19 @@ class Server:
 
     def __init__(self, ctx: gas.Context, hbm_budget_bytes: float = 64e9, max_active: int = 64,
                  precision: int = gas.PREC_FP32, dt: float = 0.01, tol: float = 1e-6, max_iters: int = 400,
-                 refine: int = 1):
+                 refine: int = 1, warm_start: bool = False):
         self.ctx = ctx
         self.budget = hbm_budget_bytes
         self.max_active = max_active
         self.precision = precision
         self.dt = dt
-        self.params = gas.solve_params(tol=tol, max_iters=max_iters, demean=True, check_every=8, zero_x=False,
-                                       refine=refine)
+        # Without the momentum phase (outside this library) the velocity only
+        # changes by the projection itself, so a step's right-hand side is
+        # small, and a warm start from the last pressure sits far from its
+        # solution in relative terms; each step therefore solves from zero
+        # unless asked otherwise (the reference warm-starts, gasphase.cpp:576).
+        self.params = gas.solve_params(tol=tol, max_iters=max_iters, demean=True, check_every=8,
+                                       zero_x=not warm_start, refine=refine)
         self.queue: collections.deque = collections.deque()
         self.active: list[Job] = []
         self.finished: list[Job] = []
@@ -103,12 +110,23 @@ class Server:
         self._batch_ids = None
         self.step_latencies: list[float] = []   # one entry per job-step
         self.steps_run = 0
+        # host seconds spent admitting (geometry, hierarchy, device fields),
+        # rebuilding the batch after the active set changed, and stepping
+        self.t_admit = self.t_rebuild = self.t_step = self.t_solve = 0.0
+        self.steps = 0
 
     # ---- requests --------------------------------------------------------------
     def submit(self, spec: JobSpec):
         self.queue.append(spec)
 
     def _admit(self, now_s: float):
+        t0 = time.perf_counter()
+        try:
+            self._admit_inner(now_s)
+        finally:
+            self.t_admit += time.perf_counter() - t0
+
+    def _admit_inner(self, now_s: float):
         while self.queue and len(self.active) < self.max_active:
             spec = self.queue[0]
             g = grid_for(spec)
@@ -139,11 +157,13 @@ class Server:
     def _batch_for_active(self):
         ids = tuple(j.spec.job_id for j in self.active)
         if ids != self._batch_ids:
+            t0 = time.perf_counter()
             if self._batch is not None:
                 self._batch.destroy()
             Gs = [j.solver.grid for j in self.active]
             self._batch = gas.Batch(self.ctx, Gs, sys_kchunk=[kchunk_for(j.grid) for j in self.active])
             self._batch_ids = ids
+            self.t_rebuild += time.perf_counter() - t0
         return self._batch
 
     def step(self, now_s: float = 0.0) -> int:
@@ -155,17 +175,26 @@ class Server:
         t0 = time.perf_counter()
         dt = self.dt
         for j in self.active:
-            j.solver.energy_and_expansion(dt)
+            j.solver.energy_and_expansion(dt, report=False)   # queued: no wait for maxDiv
+            j.solver.leakage()                                # sealed rooms: a solvable pressure equation
             j.solver.rhs(dt)
+            # the divergence of the corrected velocity sums to zero over a sealed
+            # room only up to FP32 rounding; with no momentum phase that
+            # rounding is a large part of b, so b is projected the same way
+            j.solver.leakage(field="b")
         B = self._batch_for_active()
+        ts = time.perf_counter()
         stats = B.solve([j.solver.d["b"] for j in self.active], [j.solver.d["p"] for j in self.active], self.params)
+        self.t_solve += time.perf_counter() - ts
         for j, st in zip(self.active, stats):
-            j.solver.correct(dt)
+            j.solver.correct(dt, report=False)
             j.solver.advect_scalars(dt)
             j.iters.append(st["iters"])
             j.steps_done += 1
         self.ctx.synchronize()
         lat = time.perf_counter() - t0
+        self.t_step += lat
+        self.steps += 1
         n = len(self.active)
         self.step_latencies.extend([lat] * n)
         self.steps_run += n
@@ -218,6 +247,7 @@ def serve(ctx, specs: list[JobSpec], **server_kw) -> dict:
     t0 = time.perf_counter()
     i = 0
     waits = []
+    iters_all = []
     while i < len(pending) or srv.queue or srv.active:
         now = time.perf_counter() - t0
         while i < len(pending) and pending[i].arrival_s <= now:
@@ -230,14 +260,21 @@ def serve(ctx, specs: list[JobSpec], **server_kw) -> dict:
             if j.spec.job_id not in before:
                 waits.append(j.admitted_s - j.spec.arrival_s)
         for j in list(srv.finished):
+            iters_all.append(j.iters)
             srv.release(j)
             srv.finished.remove(j)
     wall = time.perf_counter() - t0
+    its = np.concatenate([np.asarray(v, np.int64) for v in iters_all]) if iters_all else np.zeros(1, np.int64)
     lat = np.array(srv.step_latencies) if srv.step_latencies else np.zeros(1)
     out = {"jobs": len(specs), "job_steps": srv.steps_run, "wall_s": round(wall, 3),
            "steps_per_s": round(srv.steps_run / wall, 1),
            "step_latency_ms_p50": round(1e3 * float(np.percentile(lat, 50)), 3),
            "step_latency_ms_p99": round(1e3 * float(np.percentile(lat, 99)), 3),
-           "admission_wait_s_mean": round(float(np.mean(waits)), 3) if waits else 0.0}
+           "admission_wait_s_mean": round(float(np.mean(waits)), 3) if waits else 0.0,
+           "loop_steps": srv.steps,
+           "pcg_iters_mean": round(float(its.mean()), 2), "pcg_iters_p99": int(np.percentile(its, 99)),
+           "pcg_iters_max": int(its.max()),
+           "host_s": {"admit": round(srv.t_admit, 3), "batch_rebuild": round(srv.t_rebuild, 3),
+                      "steps": round(srv.t_step, 3), "of_which_solve": round(srv.t_solve, 3)}}
     srv.close()
     return out

@@ -53,6 +53,30 @@ def test_rhs(ctx, scen):
     G.destroy()
 
 
+def test_leakage_makes_closed_zones_compatible(ctx, scen):
+    """leakage() without cracks (gasphase.cpp:487-498): on every closed zone S
+    loses its mean (the zone integral over the zone volume); open zones keep
+    S.  Pinned by the FP64 zone means and by the compatibility of the shifted
+    source (zero integral per closed zone)."""
+    g = scen
+    S = (sc.normal_field(41, g.n) + 3.0).astype(np.float32)   # a net source in every zone
+    S[g.blocked != 0] = 0
+    solver = gas.GasSolver(ctx, g)
+    solver.set("S", S)
+    shift = solver.leakage(report=True)
+    out = solver.get("S")
+    for z in range(len(g.zone_open)):
+        sel = g.zone == z
+        if g.zone_open[z]:
+            assert shift[z] == 0 and np.array_equal(out[sel], S[sel])
+        else:
+            mean = S[sel].astype(np.float64).mean()
+            assert abs(shift[z] - mean) <= 1e-6 * abs(mean)
+            np.testing.assert_allclose(out[sel], S[sel] - np.float32(mean), rtol=0, atol=2e-6)
+            assert abs(out[sel].astype(np.float64).sum()) <= 1e-5 * np.abs(S[sel]).sum()
+    assert np.array_equal(out[g.blocked != 0], S[g.blocked != 0])
+
+
 def test_zone_mean_and_correct(ctx, scen):
     g = scen
     u, v, w, _ = flow_state(g, 31)
