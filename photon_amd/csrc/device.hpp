@@ -37,25 +37,35 @@ template <> __device__ __forceinline__ bf16_t from_f<bf16_t>(float v) {
 // vector-memory and the LDS wait counters: every LDS wait would then also wait
 // for outstanding loads.  These casts give the global address space, so the
 // compiler emits global_load / global_store.
+// Indices are 32-bit unsigned (every array here has fewer than 2^31
+// elements): with the base pointer uniform across the wave, the compiler then
+// forms the address as a scalar base plus a 32-bit vector offset instead of
+// 64-bit vector arithmetic for every access.
+// The byte offset is formed in 32 bits, so the address is base + zero-extended
+// offset, which the scalar-base form of the load/store encodes directly.
 #define PHG_GLOBAL __attribute__((address_space(1)))
-template <class T> __device__ __forceinline__ T gld(const T* p, long i) {
-    return ((const PHG_GLOBAL T*)p)[i];
+template <class T> __device__ __forceinline__ const PHG_GLOBAL T* gptr(const T* p, unsigned i) {
+    return (const PHG_GLOBAL T*)((const PHG_GLOBAL char*)p + i * (unsigned)sizeof(T));
 }
-template <class T> __device__ __forceinline__ void gst(T* p, long i, T v) { ((PHG_GLOBAL T*)p)[i] = v; }
+template <class T> __device__ __forceinline__ PHG_GLOBAL T* gptr(T* p, unsigned i) {
+    return (PHG_GLOBAL T*)((PHG_GLOBAL char*)p + i * (unsigned)sizeof(T));
+}
+template <class T> __device__ __forceinline__ T gld(const T* p, unsigned i) { return *gptr(p, i); }
+template <class T> __device__ __forceinline__ void gst(T* p, unsigned i, T v) { *gptr(p, i) = v; }
 // BF16 is a struct of raw bits: access the bits
-template <> __device__ __forceinline__ bf16_t gld(const bf16_t* p, long i) {
+template <> __device__ __forceinline__ bf16_t gld(const bf16_t* p, unsigned i) {
     bf16_t r;
-    r.u = ((const PHG_GLOBAL uint16_t*)p)[i];
+    r.u = *gptr(reinterpret_cast<const uint16_t*>(p), i);
     return r;
 }
-template <> __device__ __forceinline__ void gst(bf16_t* p, long i, bf16_t v) {
-    ((PHG_GLOBAL uint16_t*)p)[i] = v.u;
+template <> __device__ __forceinline__ void gst(bf16_t* p, unsigned i, bf16_t v) {
+    *gptr(reinterpret_cast<uint16_t*>(p), i) = v.u;
 }
 
-template <class T> __device__ __forceinline__ float ld(const void* p, long i) {
+template <class T> __device__ __forceinline__ float ld(const void* p, unsigned i) {
     return to_f(gld(static_cast<const T*>(p), i));
 }
-template <class T> __device__ __forceinline__ void st(void* p, long i, float v) {
+template <class T> __device__ __forceinline__ void st(void* p, unsigned i, float v) {
     gst(static_cast<T*>(p), i, from_f<T>(v));
 }
 

@@ -117,9 +117,13 @@ __device__ __forceinline__ float nb_sum_f(const Cell& q, int nx, int ny, int nz,
 // CG residual r_k: buffer k & 1.  The pre-smoothing of iteration k reads r_k
 // and writes r_{k+1}; the rest of that V-cycle reads r_{k+1}; the V-cycle
 // before the first iteration reads r_0 (ST_VC is still 0 then).
+// A system's state is the same for the whole block: reading it through
+// readfirstlane keeps it (and the vector pointers chosen by it) in scalar
+// registers, so accesses through those pointers take the scalar-base form.
+__device__ __forceinline__ int st_uni(const DSys& s, int slot) { return __builtin_amdgcn_readfirstlane(s.state[slot]); }
 __device__ __forceinline__ float* rbuf(const DSys& s, int k) { return (k & 1) ? s.r1 : s.r; }
 __device__ __forceinline__ const float* r_vc(const DSys& s) {
-    return s.state[ST_VC] ? rbuf(s, s.state[ST_ITERS] + 1) : s.r;
+    return st_uni(s, ST_VC) ? rbuf(s, st_uni(s, ST_ITERS) + 1) : s.r;
 }
 
 // the column and k-range of a level-0 block
@@ -200,7 +204,7 @@ __device__ __forceinline__ void march(const DSys& s, const ColTile& t, float* ld
     }
     auto kcl = [&](int k) { return min(max(k, 0), nz - 1); };
     auto load = [&](int k, Plane& pl) {
-        const long base = (long)sxy * kcl(k);
+        const unsigned base = (unsigned)sxy * (unsigned)kcl(k);
         pl.own = raw(cown + base);
 #pragma unroll
         for (int m = 0; m < MARCH_HALO; ++m) pl.h[m] = raw(hcol[m] + base);
@@ -257,7 +261,7 @@ __device__ __forceinline__ void march(const DSys& s, const ColTile& t, float* ld
             const int col = hh < nx ? c0 - nx + hh : c0 + TB + (hh - nx);
             float v = 0.0f;
             if (col >= 0 && col < sxy) {
-                R rv = raw(col + (long)sxy * k);
+                R rv = raw((unsigned)col + (unsigned)sxy * (unsigned)k);
                 mid(rv);
                 v = fin(rv);
             }
@@ -391,7 +395,7 @@ __global__ void __launch_bounds__(TB) k_dir_apply(const DSys* __restrict__ sys, 
     const DLevel& L = s.L[0];
     const ColTile t = col_tile(s, e.y);
     const int sxy = L.nx * L.ny;
-    const int it = s.state[ST_ITERS];
+    const int it = st_uni(s, ST_ITERS);
     const bool first = it == 0;
     float* __restrict__ dnew = (it & 1) ? s.d1 : s.d0;
     const float* __restrict__ dold = (it & 1) ? s.d0 : s.d1;
@@ -399,7 +403,7 @@ __global__ void __launch_bounds__(TB) k_dir_apply(const DSys* __restrict__ sys, 
     const void* z = s.z;
     const float izs = (float)(1.0 / s.scal[S_ZS]);   // z is stored scaled by S_ZS (a power of two)
     // d = z + beta d_old (d = z on the first iteration, where d_old is not read)
-    auto raw = [&](long n) { return float2{ld<ZT>(z, n), gld(dold, n)}; };
+    auto raw = [&](unsigned n) { return float2{ld<ZT>(z, n), gld(dold, n)}; };
     auto fin = [&](const float2& v) { return first ? v.x * izs : v.x * izs + (float)(beta * v.y); };
     extern __shared__ float lds[];
     double acc = 0.0;
@@ -438,7 +442,7 @@ __global__ void __launch_bounds__(TB) k_x_flush(const DSys* __restrict__ sys, co
     const ColTile t = col_tile(s, e.y);
     if (!t.valid) return;
     const int sxy = s.L[0].nx * s.L[0].ny;
-    const int it = s.state[ST_ITERS];
+    const int it = st_uni(s, ST_ITERS);
     const float* __restrict__ dk = ((it - 1) & 1) ? s.d1 : s.d0;
     const double alpha = s.scal[S_ALPHA];
     for (int k = t.k0; k < t.k1; ++k) {
@@ -643,7 +647,7 @@ __global__ void __launch_bounds__(TB) k_pre_red(const DSys* __restrict__ sys, co
     const int sxy = L.nx * L.ny;
     const double alpha = s.scal[S_ALPHA];
     const float zs = (float)s.scal[S_ZS];
-    const int it = s.state[ST_ITERS];
+    const int it = st_uni(s, ST_ITERS);
     const float* __restrict__ rin = rbuf(s, it);   // r_it (r_0 on the first V-cycle)
     float* __restrict__ rout = rbuf(s, it + 1);
     double v[1] = {0.0};
@@ -698,7 +702,7 @@ __global__ void __launch_bounds__(TB) k_rb_black(const DSys* __restrict__ sys, c
     const void* z = s.z;
     const float zs = (float)s.scal[S_ZS];
     extern __shared__ float lds[];
-    auto raw = [&](long n) { return ld<ZT>(z, n); };
+    auto raw = [&](unsigned n) { return ld<ZT>(z, n); };
     march(s, t, lds, raw, [](auto&) {}, [](float v) { return v; }, cr_pre<CT>(s, L), [&](int k, float vp, float vc, float vn, const float* p,
                                                                   const CoefR<CT>& o) {
         if (((t.i + t.j + k) & 1) == 0) return;
@@ -722,7 +726,7 @@ __global__ void __launch_bounds__(TB) k_resid0(const DSys* __restrict__ sys, con
     const void* z = s.z;
     const float zs = (float)s.scal[S_ZS];
     extern __shared__ float lds[];
-    auto raw = [&](long n) { return ld<ZT>(z, n); };
+    auto raw = [&](unsigned n) { return ld<ZT>(z, n); };
     march(s, t, lds, raw, [](auto&) {}, [](float v) { return v; }, cr_pre<CT>(s, L), [&](int k, float vp, float vc, float vn, const float* p,
                                                                   const CoefR<CT>& o) {
         const Cell q{t.c2 + sxy * k, t.i, t.j, k};
@@ -755,7 +759,7 @@ __global__ void __launch_bounds__(TB) k_post_black(const DSys* __restrict__ sys,
         int g;
         float c;
     };
-    auto raw = [&](long n) { return ZG{ld<ZT>(z, n), gld(agg, n), 0.0f}; };
+    auto raw = [&](unsigned n) { return ZG{ld<ZT>(z, n), gld(agg, n), 0.0f}; };
     auto mid = [&](ZG& v) { v.c = ld<VT>(t1, max(v.g, 0)); };
     auto fin = [&](const ZG& v) { return v.z + (v.g >= 0 ? VC_LEVEL_SCALE * v.c : 0.0f); };
     const float zs = (float)s.scal[S_ZS];
@@ -784,7 +788,7 @@ __global__ void __launch_bounds__(TB) k_post_red(const DSys* __restrict__ sys, c
     const int sxy = L.nx * L.ny;
     const void* z = s.z;
     const float zs = (float)s.scal[S_ZS];
-    const bool with_q = s.state[ST_FLEX] != 0 && s.state[ST_VC] != 0;   // q exists once an iteration has run
+    const bool with_q = st_uni(s, ST_FLEX) != 0 && st_uni(s, ST_VC) != 0;   // q exists once an iteration has run
     const float* rv = r_vc(s);
     double v[1] = {0.0}, vq = 0.0;
     extern __shared__ float lds[];
@@ -799,7 +803,7 @@ __global__ void __launch_bounds__(TB) k_post_red(const DSys* __restrict__ sys, c
         o.q = with_q ? gld(s.q, q.c) : 0.0f;
         return o;
     };
-    auto raw = [&](long n) { return ld<VT>(z, n); };
+    auto raw = [&](unsigned n) { return ld<VT>(z, n); };
     march(s, t, lds, raw, [](auto&) {}, [](float v) { return v; }, pre, [&](int k, float vp, float vc, float vn, const float* p, const Op& o) {
         const Cell q{t.c2 + sxy * k, t.i, t.j, k};
         const float r = o.r;
@@ -874,7 +878,7 @@ __global__ void __launch_bounds__(TB) k_pre_fused(const DSys* __restrict__ sys, 
     const int c0 = t.c2 - (int)threadIdx.x;
     const double alpha = s.scal[S_ALPHA];
     const float zs = (float)s.scal[S_ZS];
-    const int it = s.state[ST_ITERS];
+    const int it = st_uni(s, ST_ITERS);
     const float* __restrict__ rin = rbuf(s, it);   // r_it: read here, also in the halo rows of other blocks
     float* __restrict__ rout = rbuf(s, it + 1);    // r_{it+1}: written for the own columns only
     extern __shared__ float lds[];
@@ -913,7 +917,7 @@ __global__ void __launch_bounds__(TB) k_pre_fused(const DSys* __restrict__ sys, 
         unsigned m[M2];
     };
     auto load = [&](int p, Raw& X) {
-        const long base = (long)sxy * min(max(p, 0), nz - 1);
+        const unsigned base = (unsigned)sxy * (unsigned)min(max(p, 0), nz - 1);
 #pragma unroll
         for (int m = 0; m < M2; ++m) {
             X.r[m] = gld(rin, col2[m] + base);
@@ -1154,7 +1158,7 @@ __global__ void __launch_bounds__(TB) k_vc_jac(const DSys* __restrict__ sys, con
     const float dg = ld<CT>(L.diag, I);
     if (dg > 0) {
         for (int p = 0; p < L.ellk; ++p) {
-            const long o = (long)p * L.n + I;
+            const unsigned o = (unsigned)p * (unsigned)L.n + (unsigned)I;
             const int c = gld(L.col, o);
             const float v = ld<CT>(L.val, o);
 #pragma unroll
@@ -1194,7 +1198,7 @@ __global__ void __launch_bounds__(TB) k_vc_resid(const DSys* __restrict__ sys, c
     }
     if (dg > 0) {
         for (int p = 0; p < L.ellk; ++p) {
-            const long o = (long)p * L.n + I;
+            const unsigned o = (unsigned)p * (unsigned)L.n + (unsigned)I;
             const int c = gld(L.col, o);
             const float v = ld<CT>(L.val, o);
 #pragma unroll
@@ -1273,7 +1277,7 @@ __global__ void __launch_bounds__(TB) k_vc_coarse(const DSys* __restrict__ sys, 
 #pragma unroll
     for (int g = 0; g < CGS; ++g) acc[g] = 0.0;
     for (int j = j0; j < j1; ++j) {
-        const double a = gld(A, (long)j * n + ic);
+        const double a = gld(A, (unsigned)j * (unsigned)n + (unsigned)ic);
         const float4* bv = reinterpret_cast<const float4*>(bsh + j * CGS);
 #pragma unroll
         for (int q = 0; q < CGS / 4; ++q) {
