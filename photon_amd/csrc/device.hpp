@@ -52,6 +52,8 @@ template <class T> __device__ __forceinline__ PHG_GLOBAL T* gptr(T* p, unsigned 
 }
 template <class T> __device__ __forceinline__ T gld(const T* p, unsigned i) { return *gptr(p, i); }
 template <class T> __device__ __forceinline__ void gst(T* p, unsigned i, T v) { *gptr(p, i) = v; }
+// two adjacent floats in one 8-byte access
+typedef float f32x2 __attribute__((ext_vector_type(2)));
 // BF16 is a struct of raw bits: access the bits
 template <> __device__ __forceinline__ bf16_t gld(const bf16_t* p, unsigned i) {
     bf16_t r;

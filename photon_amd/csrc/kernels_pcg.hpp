@@ -65,15 +65,6 @@ __device__ __forceinline__ void coefs6(const DLevel& L, const void* ax, const vo
 
 __device__ __forceinline__ float diag6(const float a[6]) { return a[0] + a[1] + a[2] + a[3] + a[4] + a[5]; }
 
-// x / diag on level 0: a true division (the reference's gsColor divides;
-// replacing it by a product with the reciprocal face count, one unit in the
-// last place away, made the FP32 iteration diverge on point-source problems
-// with several closed zones)
-template <class CT>
-__device__ __forceinline__ float div_diag(float x, float dg) {
-    return x / dg;
-}
-
 // a cell's coefficients held in registers between their load and their use
 // (the 1-byte mask, or the six values)
 template <class CT>
@@ -283,11 +274,17 @@ __device__ __forceinline__ void march(const DSys& s, const ColTile& t, float* ld
 #pragma unroll
         for (int m = 0; m < MARCH_HALO; ++m) hfin[m] = hnext[m];
     };
-    for (int k = t.k0; k < t.k1; k += 3) {
+    // whole groups of three steps in the loop, the rest after it: no step
+    // inside the loop is conditional, so the rings keep their registers
+    // around the loop and no load is waited on before its plane is used
+    int k = t.k0;
+    for (; k + 2 < t.k1; k += 3) {
         step(k, X0, X1, X2, P0, P2);
-        if (k + 1 < t.k1) step(k + 1, X1, X2, X0, P1, P0);
-        if (k + 2 < t.k1) step(k + 2, X2, X0, X1, P2, P1);
+        step(k + 1, X1, X2, X0, P1, P0);
+        step(k + 2, X2, X0, X1, P2, P1);
     }
+    if (k < t.k1) step(k, X0, X1, X2, P0, P2);
+    if (k + 1 < t.k1) step(k + 1, X1, X2, X0, P1, P0);
 }
 
 // a_f-weighted sum of the six neighbours: lateral ones at p[-1], p[+1],
@@ -298,6 +295,63 @@ __device__ __forceinline__ float lat_sum(const Cell& q, int nx, int ny, int nz, 
     const bool e0 = q.i > 0, e1 = q.i < nx - 1, e2 = q.j > 0, e3 = q.j < ny - 1, e4 = q.k > 0, e5 = q.k < nz - 1;
     return (e0 ? a[0] : 0.0f) * p[-1] + (e1 ? a[1] : 0.0f) * p[1] + (e2 ? a[2] : 0.0f) * p[-nx] +
            (e3 ? a[3] : 0.0f) * p[nx] + (e4 ? a[4] : 0.0f) * vp + (e5 ? a[5] : 0.0f) * vn;
+}
+
+// bit f set when face f of the cell at (i, j, k) has a neighbour cell
+__device__ __forceinline__ unsigned edge_bits(int i, int j, int k, int nx, int ny, int nz) {
+    return (i > 0 ? 1u : 0u) | (i < nx - 1 ? 2u : 0u) | (j > 0 ? 4u : 0u) | (j < ny - 1 ? 8u : 0u) |
+           (k > 0 ? 16u : 0u) | (k < nz - 1 ? 32u : 0u);
+}
+
+// lat_sum for 0/1 coefficients given as bits (face mask and edge bits
+// already combined).  The products are exact and the sum runs in the same
+// face order, so the result is bit-identical to lat_sum's.
+__device__ __forceinline__ float lat_bits(unsigned b, const float* p, int nx, float vp, float vn) {
+    return (float)(b & 1u) * p[-1] + (float)((b >> 1) & 1u) * p[1] + (float)((b >> 2) & 1u) * p[-nx] +
+           (float)((b >> 3) & 1u) * p[nx] + (float)((b >> 4) & 1u) * vp + (float)((b >> 5) & 1u) * vn;
+}
+
+// x / d for a face count d in 1..6, correctly rounded without a divide
+// sequence: the reciprocal of d rounded to nearest by one Newton step from
+// the hardware reciprocal (any start within two units in the last place
+// lands on it), then the quotient x (1/d) corrected by its exact FMA
+// residual (Markstein: a faithful quotient q plus (1/d)(x - q d) rounds to
+// x / d).  Equal to x / d for every float x whose quotient is normal
+// (checked exhaustively for d = 1..6).
+__device__ __forceinline__ float div_count(float x, float d) {
+    const float r0 = __builtin_amdgcn_rcpf(d);
+    const float r = fmaf(r0, fmaf(-d, r0, 1.0f), r0);
+    const float q0 = x * r;
+    return fmaf(fmaf(-q0, d, x), r, q0);
+}
+
+// a cell's diagonal, neighbour sum and division by the diagonal, from its
+// coefficient registers: the six values in general, the mask bits for a
+// 0/1 grid (same results, fewer instructions).  The division stays
+// correctly rounded: the reference's gsColor divides, and a plain product
+// with the reciprocal face count, one unit in the last place away, made the
+// FP32 iteration diverge on point-source problems with several closed zones.
+template <class CT>
+__device__ __forceinline__ float cell_diag(const CoefRegs<CT>& cf) {
+    if constexpr (std::is_same<CT, Mask01>::value)
+        return (float)__popc(cf.m);
+    else
+        return diag6(cf.a);
+}
+template <class CT>
+__device__ __forceinline__ float cell_lat(const CoefRegs<CT>& cf, const Cell& q, int nx, int ny, int nz,
+                                          const float* p, float vp, float vn) {
+    if constexpr (std::is_same<CT, Mask01>::value)
+        return lat_bits(cf.m & edge_bits(q.i, q.j, q.k, nx, ny, nz), p, nx, vp, vn);
+    else
+        return lat_sum(q, nx, ny, nz, cf.a, p, vp, vn);
+}
+template <class CT>
+__device__ __forceinline__ float cell_div(float x, float dg) {
+    if constexpr (std::is_same<CT, Mask01>::value)
+        return div_count(x, dg);
+    else
+        return x / dg;
 }
 
 // a cell's V-cycle coefficients and CG residual, loaded ahead by the march
@@ -399,12 +453,12 @@ __global__ void __launch_bounds__(TB) k_dir_apply(const DSys* __restrict__ sys, 
     const bool first = it == 0;
     float* __restrict__ dnew = (it & 1) ? s.d1 : s.d0;
     const float* __restrict__ dold = (it & 1) ? s.d0 : s.d1;
-    const double beta = s.scal[S_BETA], alpha = s.scal[S_ALPHA];
+    const float beta = (float)s.scal[S_BETA], alpha = (float)s.scal[S_ALPHA];
     const void* z = s.z;
     const float izs = (float)(1.0 / s.scal[S_ZS]);   // z is stored scaled by S_ZS (a power of two)
     // d = z + beta d_old (d = z on the first iteration, where d_old is not read)
     auto raw = [&](unsigned n) { return float2{ld<ZT>(z, n), gld(dold, n)}; };
-    auto fin = [&](const float2& v) { return first ? v.x * izs : v.x * izs + (float)(beta * v.y); };
+    auto fin = [&](const float2& v) { return first ? v.x * izs : fmaf(beta, v.y, v.x * izs); };
     extern __shared__ float lds[];
     double acc = 0.0;
     struct Op {
@@ -420,12 +474,10 @@ __global__ void __launch_bounds__(TB) k_dir_apply(const DSys* __restrict__ sys, 
     };
     march(s, t, lds, raw, [](auto&) {}, fin, pre, [&](int k, float vp, float dc, float vn, const float* p, const Op& o) {
         const Cell q{t.c2 + sxy * k, t.i, t.j, k};
-        float a[6];
-        o.cf.get(a);
-        const float dg = diag6(a);
+        const float dg = cell_diag(o.cf);
         float y = 0.0f;
-        if (dg > 0) y = dg * dc - lat_sum(q, L.nx, L.ny, L.nz, a, p, vp, vn);
-        if (!first) gst(s.x, q.c, o.x + (float)(alpha * o.dold));
+        if (dg > 0) y = dg * dc - cell_lat(o.cf, q, L.nx, L.ny, L.nz, p, vp, vn);
+        if (!first) gst(s.x, q.c, fmaf(alpha, o.dold, o.x));
         gst(dnew, q.c, dc);
         gst(s.q, q.c, y);
         acc += (double)y * dc;
@@ -444,10 +496,10 @@ __global__ void __launch_bounds__(TB) k_x_flush(const DSys* __restrict__ sys, co
     const int sxy = s.L[0].nx * s.L[0].ny;
     const int it = st_uni(s, ST_ITERS);
     const float* __restrict__ dk = ((it - 1) & 1) ? s.d1 : s.d0;
-    const double alpha = s.scal[S_ALPHA];
+    const float alpha = (float)s.scal[S_ALPHA];
     for (int k = t.k0; k < t.k1; ++k) {
         const int c = t.c2 + sxy * k;
-        s.x[c] += (float)(alpha * dk[c]);
+        s.x[c] = fmaf(alpha, dk[c], s.x[c]);
     }
 }
 
@@ -645,7 +697,7 @@ __global__ void __launch_bounds__(TB) k_pre_red(const DSys* __restrict__ sys, co
     const DLevel& L = s.L[0];
     const ColTile t = col_tile(s, e.y);
     const int sxy = L.nx * L.ny;
-    const double alpha = s.scal[S_ALPHA];
+    const float alpha = (float)s.scal[S_ALPHA];
     const float zs = (float)s.scal[S_ZS];
     const int it = st_uni(s, ST_ITERS);
     const float* __restrict__ rin = rbuf(s, it);   // r_it (r_0 on the first V-cycle)
@@ -657,7 +709,7 @@ __global__ void __launch_bounds__(TB) k_pre_red(const DSys* __restrict__ sys, co
             const int c = t.c2 + sxy * k;
             float r = gld(rin, c);
             if (!first) {
-                r -= (float)(alpha * gld(s.q, c));
+                r = fmaf(-alpha, gld(s.q, c), r);
                 gst(rout, c, r);
                 v[0] += (double)r * r;
                 m = fmaxf(m, fabsf(r));
@@ -666,7 +718,7 @@ __global__ void __launch_bounds__(TB) k_pre_red(const DSys* __restrict__ sys, co
             if (((t.i + t.j + k) & 1) == 0) {
                 if constexpr (std::is_same<CT, Mask01>::value) {
                     const int dg = __popc((unsigned)gld(L.mask, c));
-                    if (dg > 0) z = (r * zs) / (float)dg;
+                    if (dg > 0) z = div_count(r * zs, (float)dg);
                 } else {
                     z = (r * zs) * gld(s.invd, c);
                 }
@@ -707,11 +759,9 @@ __global__ void __launch_bounds__(TB) k_rb_black(const DSys* __restrict__ sys, c
                                                                   const CoefR<CT>& o) {
         if (((t.i + t.j + k) & 1) == 0) return;
         const Cell q{t.c2 + sxy * k, t.i, t.j, k};
-        float a[6];
-        o.cf.get(a);
-        const float dg = diag6(a);
+        const float dg = cell_diag(o.cf);
         float zc = 0.0f;
-        if (dg > 0) zc = div_diag<CT>(o.r * zs + lat_sum(q, L.nx, L.ny, L.nz, a, p, vp, vn), dg);
+        if (dg > 0) zc = cell_div<CT>(o.r * zs + cell_lat(o.cf, q, L.nx, L.ny, L.nz, p, vp, vn), dg);
         st<ZT>(s.z, q.c, zc);
     });
 }
@@ -730,11 +780,9 @@ __global__ void __launch_bounds__(TB) k_resid0(const DSys* __restrict__ sys, con
     march(s, t, lds, raw, [](auto&) {}, [](float v) { return v; }, cr_pre<CT>(s, L), [&](int k, float vp, float vc, float vn, const float* p,
                                                                   const CoefR<CT>& o) {
         const Cell q{t.c2 + sxy * k, t.i, t.j, k};
-        float a[6];
-        o.cf.get(a);
-        const float dg = diag6(a);
+        const float dg = cell_diag(o.cf);
         float r = 0.0f;
-        if (dg > 0) r = o.r * zs - dg * vc + lat_sum(q, L.nx, L.ny, L.nz, a, p, vp, vn);
+        if (dg > 0) r = o.r * zs - dg * vc + cell_lat(o.cf, q, L.nx, L.ny, L.nz, p, vp, vn);
         st<VT>(L.r, q.c, r);
     });
 }
@@ -768,11 +816,9 @@ __global__ void __launch_bounds__(TB) k_post_black(const DSys* __restrict__ sys,
                                                      const CoefR<CT>& o) {
         if (((t.i + t.j + k) & 1) == 0) return;
         const Cell q{t.c2 + sxy * k, t.i, t.j, k};
-        float a[6];
-        o.cf.get(a);
-        const float dg = diag6(a);
+        const float dg = cell_diag(o.cf);
         float zc = 0.0f;
-        if (dg > 0) zc = div_diag<CT>(o.r * zs + lat_sum(q, L.nx, L.ny, L.nz, a, p, vp, vn), dg);
+        if (dg > 0) zc = cell_div<CT>(o.r * zs + cell_lat(o.cf, q, L.nx, L.ny, L.nz, p, vp, vn), dg);
         st<ZT>(s.z, q.c, zc);
     });
 }
@@ -809,11 +855,9 @@ __global__ void __launch_bounds__(TB) k_post_red(const DSys* __restrict__ sys, c
         const float r = o.r;
         float zc = vc;   // black cells: final from the black sweep
         if (((t.i + t.j + k) & 1) == 0) {
-            float a[6];
-            o.cf.get(a);
-            const float dg = diag6(a);
+            const float dg = cell_diag(o.cf);
             zc = 0.0f;
-            if (dg > 0) zc = div_diag<CT>(r * zs + lat_sum(q, L.nx, L.ny, L.nz, a, p, vp, vn), dg);
+            if (dg > 0) zc = cell_div<CT>(r * zs + cell_lat(o.cf, q, L.nx, L.ny, L.nz, p, vp, vn), dg);
             st<VT>(s.z, q.c, zc);
         }
         v[0] += (double)r * zc;
@@ -857,147 +901,177 @@ __global__ void __launch_bounds__(TB) k_post_red(const DSys* __restrict__ sys, c
 //                     the block's own columns, with |r|^2 and max|r|.
 // Every value is formed by the same operations as in the three separate
 // kernels, so the results are bitwise the same; the pass reads r and q and
-// writes r, z and r0 once (18 B/cell in place of 36).  LDS rings of four
-// planes: Rb [W2] floats, Zb [W1] floats, Mb [W2] bytes, Ob [TB] floats.
-constexpr int FUSE_M2 = 4;                              // W2 slots per thread
-constexpr int FUSE_M1 = 3;                              // W1 slots per thread
-constexpr int FUSE_MAX_NX = (FUSE_M2 * TB - TB) / 4;    // 192: W2 <= 4 TB, W1 <= 2.5 TB
+// writes r, z and r0 once (18 B/cell in place of 36).
+// Staging and smoothing work on pairs of adjacent columns (2u and 2u+1 of
+// the W2 range, which starts at an even column).  The two cells of a pair
+// have opposite colours (along a row i changes by one; across the end of a
+// row, which only a pair of an odd nx straddles, i + j changes by nx - 2),
+// so a lane does one red and one black cell and no lane idles through the
+// other colour's work: one division per pair when staging, one stencil per
+// pair when smoothing.
+// LDS rings of four planes: Rb [W2] floats (after a two-float pad), Zb
+// [W1 + 2] floats, Ob [TB] floats, Mb [W2] bytes.
+constexpr int FUSE_MAX_NX = 192;   // staging pairs TB/2 + 2 nx <= 2 TB
 
 __host__ __device__ constexpr size_t fuse_lds(int nx) {
-    return (size_t)4 * (TB + 4 * nx) * 4 + (size_t)4 * (TB + 2 * nx) * 4 + (size_t)4 * TB * 4 +
-           (((size_t)4 * (TB + 4 * nx) + 3) / 4) * 4;
+    return (size_t)(2 + 4 * (TB + 4 * nx)) * 4 + (size_t)4 * (TB + 2 * nx + 2) * 4 + (size_t)4 * TB * 4 +
+           (size_t)4 * (TB + 4 * nx);
 }
 
-template <class VT, bool FIRST, int M2 = FUSE_M2, int M1 = FUSE_M1>
+// P2, P1: staging and smoothing pairs per thread, for TB/2 + 2 nx and at
+// most TB/2 + nx + 1 pairs
+template <class VT, bool FIRST, int P2, int P1>
 __global__ void __launch_bounds__(TB) k_pre_fused(const DSys* __restrict__ sys, const int2* __restrict__ tab) {
     PHG_SYS_PROLOGUE
     const DLevel& L = s.L[0];
     const ColTile t = col_tile(s, e.y);
     const int nx = L.nx, ny = L.ny, nz = L.nz, sxy = nx * ny;
-    const int W1 = TB + 2 * nx, W2 = TB + 4 * nx;
-    const int c0 = t.c2 - (int)threadIdx.x;
-    const double alpha = s.scal[S_ALPHA];
+    const int W1 = TB + 2 * nx, W2 = TB + 4 * nx, WZ = W1 + 2;
+    const int tid = (int)threadIdx.x;
+    const int c0 = t.c2 - tid;     // a multiple of TB, so even
+    const int zo = nx & ~1;        // Zb holds W2 index h at h - zo, so pairs stay 8-byte aligned
+    const float alpha = (float)s.scal[S_ALPHA];
     const float zs = (float)s.scal[S_ZS];
     const int it = st_uni(s, ST_ITERS);
     const float* __restrict__ rin = rbuf(s, it);   // r_it: read here, also in the halo rows of other blocks
     float* __restrict__ rout = rbuf(s, it + 1);    // r_{it+1}: written for the own columns only
     extern __shared__ float lds[];
-    float* Rb = lds;                     // [4][W2]
-    float* Zb = Rb + 4 * W2;             // [4][W1]
-    float* Ob = Zb + 4 * W1;             // [4][TB]: rn of the own columns
+    float* Rb = lds + 2;                 // [4][W2]; Rb[-1] is read (and ignored) by an edge pair of an odd nx
+    float* Zb = Rb + 4 * W2;             // [4][WZ]
+    float* Ob = Zb + 4 * WZ;             // [4][TB]: rn of the own columns
     uint8_t* Mb = reinterpret_cast<uint8_t*>(Ob + 4 * TB);   // [4][W2]
     auto slot = [](int p) { return (p + 4) & 3; };           // planes >= -4
+    auto ij_bits = [&](int c) {                              // the (i, j) part of edge_bits
+        const int j = c / nx, i = c - j * nx;
+        return (i > 0 ? 1u : 0u) | (i < nx - 1 ? 2u : 0u) | (j > 0 ? 4u : 0u) | (j < ny - 1 ? 8u : 0u);
+    };
 
-    // W2 slots of this thread: clamped column, in-grid flag, parity of i + j
-    int col2[M2], par2[M2];
-    bool in2[M2];
+    // staging pairs (W2 indices 2u, 2u+1): clamped column of the first
+    // cell, in-grid flags, colour (parity of i + j) of the first cell.  The
+    // range starts at an even column, so a pair is either wholly left of
+    // column 0 or starts inside the grid.
+    const int NP2 = W2 / 2;
+    int ca[P2], par2[P2];
+    bool ina[P2], inb[P2];
 #pragma unroll
-    for (int m = 0; m < M2; ++m) {
-        const int h = (int)threadIdx.x + m * TB;
-        const int col = c0 - 2 * nx + h;
-        in2[m] = h < W2 && col >= 0 && col < sxy;
-        const int cc = min(max(col, 0), sxy - 1);
-        col2[m] = cc;
-        par2[m] = (cc % nx + cc / nx) & 1;
+    for (int m = 0; m < P2; ++m) {
+        const int u = tid + m * TB;
+        const int col = c0 - 2 * nx + 2 * u;
+        ina[m] = u < NP2 && col >= 0 && col < sxy;
+        inb[m] = u < NP2 && col + 1 >= 0 && col + 1 < sxy;
+        ca[m] = min(max(col, 0), sxy - 1);
+        par2[m] = (ca[m] % nx + ca[m] / nx) & 1;
     }
-    // W1 slots: in-grid flag and (i, j)
-    int i1[M1], j1[M1];
-    bool in1[M1];
+    // smoothing pairs: W2 indices ha, ha + 1 over the W1 range [nx, nx + W1)
+    // (for an odd nx the first and last pair stick out by one cell)
+    const int u1 = nx >> 1, NP1 = ((TB + 3 * nx - 1) >> 1) - u1 + 1;
+    int ha1[P1], par1[P1];
+    unsigned ija[P1], ijb[P1];
+    bool ina1[P1], inb1[P1];
 #pragma unroll
-    for (int m = 0; m < M1; ++m) {
-        const int h = (int)threadIdx.x + m * TB;
-        const int col = c0 - nx + h;
-        in1[m] = h < W1 && col >= 0 && col < sxy;
-        const int cc = min(max(col, 0), sxy - 1);
-        j1[m] = cc / nx;
-        i1[m] = cc - j1[m] * nx;
+    for (int m = 0; m < P1; ++m) {
+        const bool use = tid + m * TB < NP1;
+        const int h = 2 * (u1 + tid + m * TB), col = c0 - 2 * nx + h;
+        ha1[m] = h;
+        ina1[m] = use && h >= nx && h < nx + W1 && col >= 0 && col < sxy;
+        inb1[m] = use && h + 1 >= nx && h + 1 < nx + W1 && col + 1 >= 0 && col + 1 < sxy;
+        const int cca = min(max(col, 0), sxy - 1), ccb = min(max(col + 1, 0), sxy - 1);
+        ija[m] = ij_bits(cca);
+        ijb[m] = ij_bits(ccb);
+        par1[m] = (cca % nx + cca / nx) & 1;
     }
+    // Every vector is 8-byte aligned and the plane size is even (the host
+    // runs this kernel only then), so a pair, which starts at an even index
+    // inside the grid, is one 8-byte load and its two mask bytes one 2-byte
+    // load: each memory instruction covers whole consecutive lines.  The
+    // loaded registers are not touched until the plane is staged three steps
+    // later, so the loads stay in flight.  Pairs outside the grid, and the
+    // unused slots of the last lanes, load a clamped pair that is not used.
     struct Raw {
-        float r[M2], q[M2];
-        unsigned m[M2];
+        f32x2 r[P2], q[P2];
+        uint16_t m[P2];   // mask bytes of the pair: first cell in bits 0-7
     };
     auto load = [&](int p, Raw& X) {
         const unsigned base = (unsigned)sxy * (unsigned)min(max(p, 0), nz - 1);
 #pragma unroll
-        for (int m = 0; m < M2; ++m) {
-            X.r[m] = gld(rin, col2[m] + base);
-            X.q[m] = FIRST ? 0.0f : gld(s.q, col2[m] + base);
-            X.m[m] = gld(L.mask, col2[m] + base);
+        for (int m = 0; m < P2; ++m) {
+            const unsigned i2 = (ca[m] + base) >> 1;
+            X.r[m] = gld(reinterpret_cast<const f32x2*>(rin), i2);
+            if (!FIRST) X.q[m] = gld(reinterpret_cast<const f32x2*>(s.q), i2);
+            X.m[m] = gld(reinterpret_cast<const uint16_t*>(L.mask), i2);
         }
     };
-    // stage plane p from its raw loads
+    // stage plane p from its raw loads: red cells rn zs / diag (0 without
+    // open faces), black cells rn zs, out-of-grid cells 0
     auto stage = [&](int p, const Raw& X) {
         const bool pin = p >= 0 && p < nz;
         float* R = Rb + slot(p) * W2;
         uint8_t* M = Mb + slot(p) * W2;
         float* O = Ob + slot(p) * TB;
 #pragma unroll
-        for (int m = 0; m < M2; ++m) {
-            const int h = (int)threadIdx.x + m * TB;
-            if (h >= W2) break;
-            float rn = X.r[m];
-            if (!FIRST) rn -= (float)(alpha * X.q[m]);
-            const bool ok = pin && in2[m];
-            const unsigned mk = X.m[m];
-            const int dg = __popc(mk);
-            const float rs = rn * zs;
-            float v = rs;   // black cells: rn zs
-            if (((par2[m] + p) & 1) == 0) v = dg > 0 ? rs / (float)dg : 0.0f;
-            R[h] = ok ? v : 0.0f;
-            M[h] = ok ? (uint8_t)mk : (uint8_t)0;
-            const int o = h - 2 * nx;
-            if (o >= 0 && o < TB) O[o] = ok ? rn : 0.0f;
+        for (int m = 0; m < P2; ++m) {
+            const int u = tid + m * TB;
+            if (u >= NP2) break;
+            float rna = X.r[m].x, rnb = X.r[m].y;
+            if (!FIRST) {
+                rna = fmaf(-alpha, X.q[m].x, rna);
+                rnb = fmaf(-alpha, X.q[m].y, rnb);
+            }
+            const unsigned ma = X.m[m] & 255u, mb = (unsigned)X.m[m] >> 8;
+            const bool oka = pin && ina[m], okb = pin && inb[m];
+            const bool ared = ((par2[m] + p) & 1) == 0;   // the first cell is red
+            const float rsa = rna * zs, rsb = rnb * zs;
+            const int dg = __popc(ared ? ma : mb);
+            const float vr = dg > 0 ? div_count(ared ? rsa : rsb, (float)dg) : 0.0f;
+            *reinterpret_cast<float2*>(R + 2 * u) =
+                float2{oka ? (ared ? vr : rsa) : 0.0f, okb ? (ared ? rsb : vr) : 0.0f};
+            *reinterpret_cast<uint16_t*>(M + 2 * u) = (uint16_t)((oka ? ma : 0u) | ((okb ? mb : 0u) << 8));
+            const int o = 2 * u - 2 * nx;
+            if (o >= 0 && o < TB) *reinterpret_cast<float2*>(O + o) = float2{oka ? rna : 0.0f, okb ? rnb : 0.0f};
         }
     };
-    // smoothed values of plane p over W1
+    // smoothed values of plane p over W1: red cells keep their value, black
+    // cells take the Gauss-Seidel update from their red neighbours
     auto smooth = [&](int p) {
         const bool pin = p >= 0 && p < nz;
+        const unsigned kb = (p > 0 ? 16u : 0u) | (p < nz - 1 ? 32u : 0u);
         const float* R = Rb + slot(p) * W2;
         const float* Rm = Rb + slot(p - 1) * W2;
         const float* Rp = Rb + slot(p + 1) * W2;
         const uint8_t* M = Mb + slot(p) * W2;
-        float* Z = Zb + slot(p) * W1;
+        float* Z = Zb + slot(p) * WZ - zo;
 #pragma unroll
-        for (int m = 0; m < M1; ++m) {
-            const int h = (int)threadIdx.x + m * TB;
-            if (h >= W1) break;
-            const int h2 = h + nx;
-            float z = 0.0f;
-            if (pin && in1[m]) {
-                const float v = R[h2];
-                if (((i1[m] + j1[m] + p) & 1) == 0) {
-                    z = v;
-                } else {
-                    const CoefRegs<Mask01> cf{(unsigned)M[h2]};
-                    float a[6];
-                    cf.get(a);
-                    const float dg = diag6(a);
-                    if (dg > 0)
-                        z = div_diag<Mask01>(v + lat_sum(Cell{0, i1[m], j1[m], p}, nx, ny, nz, a, R + h2, Rm[h2], Rp[h2]),
-                                             dg);
-                }
-            }
-            Z[h] = z;
+        for (int m = 0; m < P1; ++m) {
+            if (tid + m * TB >= NP1) break;
+            const bool ared = ((par1[m] + p) & 1) == 0;
+            const int hb = ha1[m] + (ared ? 1 : 0);    // the black cell of the pair
+            const float xm = R[hb - 1], xp = R[hb + 1];
+            const unsigned mk = M[hb];
+            const int dg = __popc(mk);
+            float zb = 0.0f;
+            if (pin && (ared ? inb1[m] : ina1[m]) && dg > 0)
+                zb = div_count(R[hb] + lat_bits(mk & ((ared ? ijb[m] : ija[m]) | kb), R + hb, nx, Rm[hb], Rp[hb]),
+                               (float)dg);
+            const float zr = pin && (ared ? ina1[m] : inb1[m]) ? (ared ? xm : xp) : 0.0f;
+            *reinterpret_cast<float2*>(Z + ha1[m]) = ared ? float2{zr, zb} : float2{zb, zr};
         }
     };
     double acc = 0.0;
     float mx = 0.0f;
     // emit plane k of the own column
+    const int hz = 2 * nx + tid - zo;   // the own column in Zb
     auto emit = [&](int k) {
         if (!t.valid) return;
         const int c = t.c2 + sxy * k;
-        const float rn = Ob[slot(k) * TB + threadIdx.x];
-        const float* Z = Zb + slot(k) * W1 + nx + threadIdx.x;
+        const float rn = Ob[slot(k) * TB + tid];
+        const float* Z = Zb + slot(k) * WZ + hz;
         const float zc = *Z;
-        const CoefRegs<Mask01> cf{(unsigned)Mb[slot(k) * W2 + 2 * nx + threadIdx.x]};
-        float a[6];
-        cf.get(a);
-        const float dg = diag6(a);
+        const CoefRegs<Mask01> cf{(unsigned)Mb[slot(k) * W2 + 2 * nx + tid]};
+        const float dg = cell_diag(cf);
         float r0 = 0.0f;
         if (dg > 0)
             r0 = rn * zs - dg * zc +
-                 lat_sum(Cell{c, t.i, t.j, k}, nx, ny, nz, a, Z, Zb[slot(k - 1) * W1 + nx + threadIdx.x],
-                         Zb[slot(k + 1) * W1 + nx + threadIdx.x]);
+                 cell_lat(cf, Cell{c, t.i, t.j, k}, nx, ny, nz, Z, Zb[slot(k - 1) * WZ + hz], Zb[slot(k + 1) * WZ + hz]);
         gst(reinterpret_cast<float*>(s.z), c, zc);
         st<VT>(L.r, c, r0);
         if (!FIRST) {
@@ -1028,11 +1102,17 @@ __global__ void __launch_bounds__(TB) k_pre_fused(const DSys* __restrict__ sys, 
     load(ks + 2, X0);
     load(ks + 3, X1);
     load(ks + 4, X2);
-    for (int k = ks; k < t.k1; k += 3) {
+    // whole groups of three steps in the loop, the rest after it: no step
+    // inside the loop is conditional, so the three raw buffers keep their
+    // registers around the loop and their loads are never waited on early
+    int k = ks;
+    for (; k + 2 < t.k1; k += 3) {
         step(k, X0);
-        if (k + 1 < t.k1) step(k + 1, X1);
-        if (k + 2 < t.k1) step(k + 2, X2);
+        step(k + 1, X1);
+        step(k + 2, X2);
     }
+    if (k < t.k1) step(k, X0);
+    if (k + 1 < t.k1) step(k + 1, X1);
     if (FIRST) return;
     double* out = s.part + (long)e.y * NPART;
     __shared__ double sh[TB / 64];

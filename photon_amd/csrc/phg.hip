@@ -87,6 +87,7 @@ struct BatchDev {
     int nsys = 0, prec = 0, maxl = 0, maxnc = 0;
     bool binary = false;           // every grid has 0/1 coefficients: level-0 kernels read the face mask
     int maxnx = 0;                 // widest grid: sizes the LDS plane buffers of the level-0 kernels
+    bool even_planes = true;       // every grid has an even nx * ny (the fused pre-smoothing loads pairs)
     bool unfused = false;          // run the separate level-0 smoothing kernels (comparison runs)
     int maxnu0 = 1;                // most level-0 sweeps of any system
     size_t lds0() const { return 2 * (size_t)(TB + 2 * maxnx) * sizeof(float); }
@@ -290,17 +291,19 @@ void vcycle_t(phg_ctx* ctx, BatchDev* B, int first) {
     const DSys* S = B->d_sys;
     const int2* T0 = B->tab[0];
     const int n0 = B->tab_len[0];
-    if (std::is_same<C0, Mask01>::value && B->maxnx <= FUSE_MAX_NX && !B->unfused && B->maxnu0 == 1) {
-        // pre-smoothing and residual in one pass
-        // slots per thread for the widest grid: 3 and 2 up to nx = 128, else 4 and 3
+    if (std::is_same<C0, Mask01>::value && B->maxnx <= FUSE_MAX_NX && B->even_planes && !B->unfused &&
+        B->maxnu0 == 1) {
+        // pre-smoothing and residual in one pass; column pairs per thread for
+        // the widest grid: staging TB/2 + 2 nx, smoothing up to TB/2 + nx + 1
+        static_assert(TB == 256, "pair counts below assume 256 threads");
         const size_t lds = fuse_lds(B->maxnx);
-        if (B->maxnx <= 128) {
-            if (first) launch(ctx, K_PRE_SMOOTH, k_pre_fused<VT, true, 3, 2>, n0, lds, S, T0);
-            else launch(ctx, K_PRE_SMOOTH, k_pre_fused<VT, false, 3, 2>, n0, lds, S, T0);
-        } else {
-            if (first) launch(ctx, K_PRE_SMOOTH, k_pre_fused<VT, true>, n0, lds, S, T0);
-            else launch(ctx, K_PRE_SMOOTH, k_pre_fused<VT, false>, n0, lds, S, T0);
-        }
+        auto go = [&](auto k1, auto k0) { launch(ctx, K_PRE_SMOOTH, first ? k1 : k0, n0, lds, S, T0); };
+        if (B->maxnx <= 64)
+            go(k_pre_fused<VT, true, 1, 1>, k_pre_fused<VT, false, 1, 1>);
+        else if (B->maxnx <= 127)
+            go(k_pre_fused<VT, true, 2, 1>, k_pre_fused<VT, false, 2, 1>);
+        else
+            go(k_pre_fused<VT, true, 2, 2>, k_pre_fused<VT, false, 2, 2>);
     } else {
         // red from zero, black, then (red, black) for each further sweep of
         // the systems that make it, then the residual
@@ -754,6 +757,7 @@ int phg_batch_create_ex(phg_ctx* ctx, int nsys, const int* grids, const phg_batc
         B->maxnc = std::max(B->maxnc, G->H.nc);
         B->binary = s == 0 ? G->binary : (B->binary && G->binary);
         B->maxnx = std::max(B->maxnx, G->nx);
+        if ((G->nx * G->ny) & 1) B->even_planes = false;
         col_tiles += tiles(G->nx * G->ny);
     }
     if (rc) {
@@ -1076,12 +1080,12 @@ int phg_precondition(phg_ctx* ctx, int grid, const float* r, float* z) {
     BatchDev* B = nullptr;
     int rc = single_batch(ctx, grid, &B);
     if (rc) return rc;
-    float* saved = B->h_sys[0].r;
-    B->h_sys[0].r = const_cast<float*>(r);   // read only: the first V-cycle does not update r
-    rc = upload_sys(ctx, B);
+    // the first V-cycle reads r_0 from the system's own residual buffer (the
+    // level-0 kernels rely on its alignment)
+    HIP_OK(hipMemcpyAsync(B->h_sys[0].r, r, get_grid(ctx, grid)->n * sizeof(float), hipMemcpyDeviceToDevice,
+                          ctx->stream));
     phg_solve_params P{};
-    if (!rc) rc = reset_state(ctx, B, &P);
-    B->h_sys[0].r = saved;
+    rc = reset_state(ctx, B, &P);
     if (rc) return rc;
     vcycle(ctx, B, 1);
     const DSys* S = B->d_sys;
@@ -1089,7 +1093,7 @@ int phg_precondition(phg_ctx* ctx, int grid, const float* r, float* z) {
     HIP_OK(hipStreamSynchronize(ctx->stream));
     HIP_OK(hipGetLastError());
     if (ctx->prof) prof_drain(ctx);
-    return upload_sys(ctx, B);
+    return PHG_OK;
 }
 
 // ---------------------------------------------------------------------------
