@@ -70,6 +70,18 @@ template <class T> __device__ __forceinline__ float ld(const void* p, unsigned i
 template <class T> __device__ __forceinline__ void st(void* p, unsigned i, float v) {
     gst(static_cast<T*>(p), i, from_f<T>(v));
 }
+// Streaming stores for the level-0 vectors: each pass writes them once and
+// the next kernel reads them back from HBM (a batch's level-0 vectors far
+// exceed L2 and the last-level cache), so they are marked evict-first.
+template <class T> __device__ __forceinline__ void gst_nt(T* p, unsigned i, T v) {
+    __builtin_nontemporal_store(v, gptr(p, i));
+}
+template <> __device__ __forceinline__ void gst_nt(bf16_t* p, unsigned i, bf16_t v) {
+    __builtin_nontemporal_store(v.u, gptr(reinterpret_cast<uint16_t*>(p), i));
+}
+template <class T> __device__ __forceinline__ void st_nt(void* p, unsigned i, float v) {
+    gst_nt(static_cast<T*>(p), i, from_f<T>(v));
+}
 
 // ---- descriptors -----------------------------------------------------------
 // One multigrid level of one system.  Operator arrays belong to the grid and

@@ -477,9 +477,9 @@ __global__ void __launch_bounds__(TB) k_dir_apply(const DSys* __restrict__ sys, 
         const float dg = cell_diag(o.cf);
         float y = 0.0f;
         if (dg > 0) y = dg * dc - cell_lat(o.cf, q, L.nx, L.ny, L.nz, p, vp, vn);
-        if (!first) gst(s.x, q.c, fmaf(alpha, o.dold, o.x));
-        gst(dnew, q.c, dc);
-        gst(s.q, q.c, y);
+        if (!first) gst_nt(s.x, q.c, fmaf(alpha, o.dold, o.x));
+        gst_nt(dnew, q.c, dc);
+        gst_nt(s.q, q.c, y);
         acc += (double)y * dc;
     });
     double v[1] = {acc};
@@ -710,7 +710,7 @@ __global__ void __launch_bounds__(TB) k_pre_red(const DSys* __restrict__ sys, co
             float r = gld(rin, c);
             if (!first) {
                 r = fmaf(-alpha, gld(s.q, c), r);
-                gst(rout, c, r);
+                gst_nt(rout, c, r);
                 v[0] += (double)r * r;
                 m = fmaxf(m, fabsf(r));
             }
@@ -723,7 +723,7 @@ __global__ void __launch_bounds__(TB) k_pre_red(const DSys* __restrict__ sys, co
                     z = (r * zs) * gld(s.invd, c);
                 }
             }
-            st<ZT>(s.z, c, z);
+            st_nt<ZT>(s.z, c, z);
         }
     }
     if (first) return;
@@ -762,7 +762,7 @@ __global__ void __launch_bounds__(TB) k_rb_black(const DSys* __restrict__ sys, c
         const float dg = cell_diag(o.cf);
         float zc = 0.0f;
         if (dg > 0) zc = cell_div<CT>(o.r * zs + cell_lat(o.cf, q, L.nx, L.ny, L.nz, p, vp, vn), dg);
-        st<ZT>(s.z, q.c, zc);
+        st_nt<ZT>(s.z, q.c, zc);
     });
 }
 
@@ -783,7 +783,7 @@ __global__ void __launch_bounds__(TB) k_resid0(const DSys* __restrict__ sys, con
         const float dg = cell_diag(o.cf);
         float r = 0.0f;
         if (dg > 0) r = o.r * zs - dg * vc + cell_lat(o.cf, q, L.nx, L.ny, L.nz, p, vp, vn);
-        st<VT>(L.r, q.c, r);
+        st_nt<VT>(L.r, q.c, r);
     });
 }
 
@@ -819,7 +819,7 @@ __global__ void __launch_bounds__(TB) k_post_black(const DSys* __restrict__ sys,
         const float dg = cell_diag(o.cf);
         float zc = 0.0f;
         if (dg > 0) zc = cell_div<CT>(o.r * zs + cell_lat(o.cf, q, L.nx, L.ny, L.nz, p, vp, vn), dg);
-        st<ZT>(s.z, q.c, zc);
+        st_nt<ZT>(s.z, q.c, zc);
     });
 }
 
@@ -858,7 +858,7 @@ __global__ void __launch_bounds__(TB) k_post_red(const DSys* __restrict__ sys, c
             const float dg = cell_diag(o.cf);
             zc = 0.0f;
             if (dg > 0) zc = cell_div<CT>(r * zs + cell_lat(o.cf, q, L.nx, L.ny, L.nz, p, vp, vn), dg);
-            st<VT>(s.z, q.c, zc);
+            st_nt<VT>(s.z, q.c, zc);
         }
         v[0] += (double)r * zc;
         if (with_q) vq += (double)o.q * zc;
@@ -1072,10 +1072,10 @@ __global__ void __launch_bounds__(TB) k_pre_fused(const DSys* __restrict__ sys, 
         if (dg > 0)
             r0 = rn * zs - dg * zc +
                  cell_lat(cf, Cell{c, t.i, t.j, k}, nx, ny, nz, Z, Zb[slot(k - 1) * WZ + hz], Zb[slot(k + 1) * WZ + hz]);
-        gst(reinterpret_cast<float*>(s.z), c, zc);
-        st<VT>(L.r, c, r0);
+        gst_nt(reinterpret_cast<float*>(s.z), c, zc);
+        st_nt<VT>(L.r, c, r0);
         if (!FIRST) {
-            gst(rout, c, rn);
+            gst_nt(rout, c, rn);
             acc += (double)rn * rn;
             mx = fmaxf(mx, fabsf(rn));
         }
