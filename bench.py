@@ -235,6 +235,47 @@ def run_serving(ctx, window_s=2.0, steps=30):
     return out
 
 
+def run_room(ctx, gas, sc, steps=300, warmup=20):
+    """The room case (20x15x12 at 0.2 m, door open to ambient) with a 100 kW
+    2x2-cell burner at the floor centre, stepped at dt = 0.01 s through the
+    phases this library implements, in Solver::step's order: energyAndExpansion,
+    the closed-zone source shift, project (||r||/||b|| <= 1e-6, warm start from
+    the previous pressure, iteration count read back every step as the
+    reference reads pressureIters), advectScalars."""
+    g = sc.room()
+    dt, hrr = 0.01, 100e3
+    Q = np.zeros(g.n, np.float32)
+    i0, j0 = g.nx // 2 - 1, g.ny // 2 - 1
+    for j in (j0, j0 + 1):
+        for i in (i0, i0 + 1):
+            Q[i + g.nx * j] = hrr * dt / 4.0
+    s = gas.GasSolver(ctx, g, tol=1e-6, max_iters=400)
+    s.set("Q", Q)
+    iters = []
+
+    def step():
+        s.energy_and_expansion(dt, report=False)
+        s.leakage()
+        iters.append(s.project(dt))
+        s.advect_scalars(dt)
+
+    for _ in range(warmup):
+        step()
+    ctx.synchronize()
+    iters.clear()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        step()
+    ctx.synchronize()
+    sec = time.perf_counter() - t0
+    T = s.get("T")
+    return {"workload": "room 20x15x12 at 0.2 m, door open, 100 kW burner, dt = 0.01 s, FP32, "
+                        f"{warmup} warm-up steps then {steps} timed",
+            "steps_per_s": round(steps / sec, 1), "ms_per_step": round(1e3 * sec / steps, 3),
+            "pcg_iters_mean": round(float(np.mean(iters)), 2), "pcg_iters_max": int(np.max(iters)),
+            "T_max_K": round(float(T.max()), 2)}
+
+
 def cpu_baseline(scen, bs_host, iters_ref, budget_s=15.0):
     """The reference's own Solver::project (oracle/_ref, its GCC flags) on the
     host cores, each solve run for the iteration count the reference method
@@ -331,6 +372,7 @@ def main():
     precision = None if args.mixed_only else run_precision(ctx, gas, sc)
     batching = None if args.mixed_only else run_batching(ctx, gas, sc)
     serving = None if args.mixed_only else run_serving(ctx)
+    room = None if args.mixed_only else run_room(ctx, gas, sc)
 
     # roofline of the dominant kernel, from HIP events over the timed steps
     # (recorded on the library's stream, around each launch)
@@ -406,6 +448,7 @@ def main():
         "precision_1p28M": precision,
         "batching_64_scenarios": batching,
         "serving_poisson": serving,
+        "room_c1": room,
         "cpu_baseline": cpu,
     }
     ctx.close()

@@ -121,14 +121,27 @@ def test_batch_equals_single_and_repeats(ctx, small):
     G.destroy()
 
 
+# walled grids with an odd nx and an even plane size: the fused pass's column
+# pairs then straddle row ends; one per pair-slot configuration (nx <= 64,
+# <= 127, <= 192)
+ODD_NX = {"odd37": (11, 37, 22, 10), "odd101": (12, 101, 14, 8), "odd131": (13, 131, 10, 6)}
+
+
 @pytest.mark.parametrize("prec", [gas.PREC_FP32, gas.PREC_MIXED_F16])
-@pytest.mark.parametrize("grid,kchunk", [("small", 3), ("small", 0), ("house", 7), ("house", 40)])
+@pytest.mark.parametrize("grid,kchunk", [("small", 3), ("small", 0), ("house", 7), ("house", 40), ("odd37", 3),
+                                         ("odd101", 0), ("odd131", 4)])
 def test_fused_presmoothing_is_bitwise_the_separate_passes(ctx, small, house, prec, grid, kchunk):
     """The fused pre-smoothing pass forms every value with the same operations
     as the three separate passes, so whole solves agree bit for bit, for
     k-chunks that cut the grid unevenly and for whole columns, on grids of
-    many column tiles (its halo rows belong to other blocks)."""
-    g, H = small if grid == "small" else house
+    many column tiles (its halo rows belong to other blocks), and on grids of
+    odd nx, where a column pair can hold the last cell of one row and the
+    first of the next."""
+    if grid in ODD_NX:
+        g = sc.house(*ODD_NX[grid])
+        assert g.nx % 2 == 1 and (g.nx * g.ny) % 2 == 0
+    else:
+        g, _ = small if grid == "small" else house
     G = gas.Grid(ctx, g, precision=prec)
     bs = [sc.rhs_from_divergence(g, sc.divergence_field(g, 300 + s), DT) for s in range(8)]
     p = gas.solve_params(tol=1e-6, max_iters=200, zero_x=True)
