@@ -43,16 +43,22 @@ LEVEL0 = ["dir_apply", "pre_smooth", "pre_red", "rb_black", "resid0", "post_blac
 TRAFFIC_FILE = "profiles/level0_house80x50x40_1000rhs_mixed_f16_hbm_traffic.txt"
 
 
+# kernel class (the library's profile name) -> the kernel that implements it
+# on the benchmarked grid
+KERNEL_OF_CLASS = {"pre_smooth": "k_pre_fused"}
+
+
 def measured_traffic(kernel):
-    """HBM bytes per launch of `kernel` as measured by the counter runs
-    recorded in TRAFFIC_FILE (lines `traffic <kernel> <bytes per launch>`),
-    or None when no measurement is recorded."""
+    """HBM bytes per launch of kernel class `kernel` as measured by the
+    counter runs recorded in TRAFFIC_FILE (lines `traffic <kernel function>
+    <bytes per launch>`), or None when no measurement is recorded."""
+    fn = KERNEL_OF_CLASS.get(kernel, "k_" + kernel)
     path = os.path.join(ROOT, TRAFFIC_FILE)
     try:
         with open(path) as f:
             for line in f:
                 parts = line.split()
-                if len(parts) == 3 and parts[0] == "traffic" and parts[1] == "k_" + kernel:
+                if len(parts) == 3 and parts[0] == "traffic" and parts[1] == fn:
                     return float(parts[2])
     except OSError:
         pass
