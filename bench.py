@@ -37,7 +37,7 @@ DT = 0.01
 TOL = 1e-6
 
 
-LEVEL0 = ["dir_apply", "pre_smooth", "pre_red", "rb_black", "resid0", "post_black", "post_red"]
+LEVEL0 = ["dir_apply", "pre_smooth", "pre_red", "rb_black", "resid0", "post_smooth", "post_black", "post_red"]
 # HBM bytes per launch of each level-0 kernel on this workload, from rocprofv3
 # FETCH_SIZE and WRITE_SIZE runs scaled by the copy-kernel calibration
 TRAFFIC_FILE = "profiles/level0_house80x50x40_1000rhs_mixed_f16_hbm_traffic.txt"
@@ -45,7 +45,7 @@ TRAFFIC_FILE = "profiles/level0_house80x50x40_1000rhs_mixed_f16_hbm_traffic.txt"
 
 # kernel class (the library's profile name) -> the kernel that implements it
 # on the benchmarked grid
-KERNEL_OF_CLASS = {"pre_smooth": "k_pre_fused"}
+KERNEL_OF_CLASS = {"pre_smooth": "k_pre_fused", "post_smooth": "k_post_fused"}
 
 
 def measured_traffic(kernel):
@@ -92,11 +92,13 @@ def algorithmic_bytes(name, iters, n0, n1, vt, flex):
         per_sys = nvc * (8 + vt) * n0
     elif name == "post_black":                       # read z r and t1 (level 1), write half of z
         per_sys = nvc * (10 * n0 + vt * n1)
+    elif name == "post_smooth":                      # read z r t1 (and q when flexible), write z
+        per_sys = nvc * (12 * n0 + vt * n1) + (nvc - len(it) if flex else 0) * 4 * n0
     elif name == "post_red":                         # read z r (and q when flexible), write half of z
         per_sys = nvc * 10 * n0 + (nvc - len(it) if flex else 0) * 4 * n0
     else:
         raise KeyError(name)
-    shared = {"post_black": 5}.get(name, 1) * n0     # per launch
+    shared = {"post_black": 5, "post_smooth": 5}.get(name, 1) * n0   # per launch
     return per_sys, shared
 
 

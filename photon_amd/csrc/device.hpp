@@ -140,6 +140,8 @@ struct DSys {
     float* d0;                    // direction, double-buffered: d_k lives in d[k & 1]
     float* d1;
     void* z;                      // preconditioned residual, V-cycle vector type
+    float* z2;                    // the pre-smoothed iterate of the fused passes: the fused
+                                  // post-smoothing reads it in its halo rows while it writes z
     const float* invd;            // 1 / diag of level 0 (0 where diag <= 0), FP32
     double* scal;
     int* state;

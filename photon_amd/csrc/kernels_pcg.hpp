@@ -901,7 +901,8 @@ __global__ void __launch_bounds__(TB) k_post_red(const DSys* __restrict__ sys, c
 //                     the block's own columns, with |r|^2 and max|r|.
 // Every value is formed by the same operations as in the three separate
 // kernels, so the results are bitwise the same; the pass reads r and q and
-// writes r, z and r0 once (18 B/cell in place of 36).
+// writes r, z and r0 once (18 B/cell in place of 36).  z goes to the system's
+// z2, from where the fused post-smoothing reads it.
 // Staging and smoothing work on pairs of adjacent columns (2u and 2u+1 of
 // the W2 range, which starts at an even column).  The two cells of a pair
 // have opposite colours (along a row i changes by one; across the end of a
@@ -1072,7 +1073,7 @@ __global__ void __launch_bounds__(TB) k_pre_fused(const DSys* __restrict__ sys, 
         if (dg > 0)
             r0 = rn * zs - dg * zc +
                  cell_lat(cf, Cell{c, t.i, t.j, k}, nx, ny, nz, Z, Zb[slot(k - 1) * WZ + hz], Zb[slot(k + 1) * WZ + hz]);
-        gst_nt(reinterpret_cast<float*>(s.z), c, zc);
+        gst_nt(s.z2, c, zc);   // read by the fused post-smoothing
         st_nt<VT>(L.r, c, r0);
         if (!FIRST) {
             gst_nt(rout, c, rn);
@@ -1127,6 +1128,247 @@ __global__ void __launch_bounds__(TB) k_pre_fused(const DSys* __restrict__ sys, 
         for (int ww = 0; ww < TB / 64; ++ww) tot += sh[ww];
         out[1] = tot;
         out[2] = mx;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// fused post-smoothing: k_post_black + k_post_red in one pass
+// ---------------------------------------------------------------------------
+// For the grids of k_pre_fused.  The black update of a cell needs the
+// prolonged values z + t1[agg] of its red neighbours, and the red update of a
+// cell the new values of its black neighbours, so the red update of plane k
+// needs prolonged values two cells away.  The march mirrors k_pre_fused:
+//   stage plane k+2   over the columns +- 2 rows, on column pairs: red cells
+//                     hold their prolonged value, black cells r zs; the
+//                     pair's one level-1 gather was issued a step earlier
+//   smooth plane k+1  over the columns +- 1 row: black cells take the
+//                     Gauss-Seidel update (the red cells' slots are unused)
+//   emit plane k      red cells of the own columns take their update from the
+//                     new black values; z of every own cell is written once,
+//                     with the partial sums r.z and (flexible CG) q.z.
+// Every value is formed by the same operations as in the two separate
+// kernels, and each thread sums its column's terms in the same order, so the
+// results are bitwise the same; the pass reads z and r once and writes z once
+// (12 B/cell, 16 with q, in place of about 27).
+__host__ __device__ constexpr size_t post_fuse_lds(int nx) { return fuse_lds(nx) + (size_t)4 * TB * 4; }
+
+template <class VT, int P2, int P1>
+__global__ void __launch_bounds__(TB) k_post_fused(const DSys* __restrict__ sys, const int2* __restrict__ tab) {
+    PHG_SYS_PROLOGUE
+    const DLevel& L = s.L[0];
+    const ColTile t = col_tile(s, e.y);
+    const int nx = L.nx, ny = L.ny, nz = L.nz, sxy = nx * ny;
+    const int W1 = TB + 2 * nx, W2 = TB + 4 * nx, WZ = W1 + 2;
+    const int tid = (int)threadIdx.x;
+    const int c0 = t.c2 - tid;     // a multiple of TB, so even
+    const int zo = nx & ~1;        // Zb holds W2 index h at h - zo, so pairs stay 8-byte aligned
+    const float zs = (float)s.scal[S_ZS];
+    const bool with_q = st_uni(s, ST_FLEX) != 0 && st_uni(s, ST_VC) != 0;   // q exists once an iteration has run
+    const float* __restrict__ rv = r_vc(s);
+    const float* __restrict__ z = s.z2;   // the pre-smoothed iterate, never written here
+    const VT* __restrict__ t1 = reinterpret_cast<const VT*>(s.L[1].t);
+    extern __shared__ float lds[];
+    float* Rb = lds + 2;                 // [4][W2]; Rb[-1] is read (and ignored) by an edge pair of an odd nx
+    float* Zb = Rb + 4 * W2;             // [4][WZ]
+    float* Ob = Zb + 4 * WZ;             // [4][TB]: r of the own columns
+    uint8_t* Mb = reinterpret_cast<uint8_t*>(Ob + 4 * TB);   // [4][W2]
+    float* Qb = reinterpret_cast<float*>(Mb + 4 * W2);       // [4][TB]: q of the own columns
+    auto slot = [](int p) { return (p + 4) & 3; };           // planes >= -4
+    auto ij_bits = [&](int c) {                              // the (i, j) part of edge_bits
+        const int j = c / nx, i = c - j * nx;
+        return (i > 0 ? 1u : 0u) | (i < nx - 1 ? 2u : 0u) | (j > 0 ? 4u : 0u) | (j < ny - 1 ? 8u : 0u);
+    };
+    // staging pairs (as in k_pre_fused)
+    const int NP2 = W2 / 2;
+    int ca[P2], par2[P2];
+    bool ina[P2], inb[P2];
+#pragma unroll
+    for (int m = 0; m < P2; ++m) {
+        const int u = tid + m * TB;
+        const int col = c0 - 2 * nx + 2 * u;
+        ina[m] = u < NP2 && col >= 0 && col < sxy;
+        inb[m] = u < NP2 && col + 1 >= 0 && col + 1 < sxy;
+        ca[m] = min(max(col, 0), sxy - 1);
+        par2[m] = (ca[m] % nx + ca[m] / nx) & 1;
+    }
+    // smoothing pairs (as in k_pre_fused)
+    const int u1 = nx >> 1, NP1 = ((TB + 3 * nx - 1) >> 1) - u1 + 1;
+    int ha1[P1], par1[P1];
+    unsigned ija[P1], ijb[P1];
+    bool ina1[P1], inb1[P1];
+#pragma unroll
+    for (int m = 0; m < P1; ++m) {
+        const bool use = tid + m * TB < NP1;
+        const int h = 2 * (u1 + tid + m * TB), col = c0 - 2 * nx + h;
+        ha1[m] = h;
+        ina1[m] = use && h >= nx && h < nx + W1 && col >= 0 && col < sxy;
+        inb1[m] = use && h + 1 >= nx && h + 1 < nx + W1 && col + 1 >= 0 && col + 1 < sxy;
+        const int cca = min(max(col, 0), sxy - 1), ccb = min(max(col + 1, 0), sxy - 1);
+        ija[m] = ij_bits(cca);
+        ijb[m] = ij_bits(ccb);
+        par1[m] = (cca % nx + cca / nx) & 1;
+    }
+    const unsigned cown = t.valid ? (unsigned)t.c2 : (unsigned)(sxy - 1);   // idle lanes read a valid column
+    // Raw loads of one plane, issued three steps before the plane is staged;
+    // the level-1 value of each pair's red cell is gathered one step before.
+    // Nothing touches a loaded register before then, so the loads stay in
+    // flight (pairs outside the grid load a clamped pair that is not used).
+    typedef int i32x2 __attribute__((ext_vector_type(2)));
+    struct Raw {
+        f32x2 z[P2], r[P2];
+        i32x2 a[P2];      // level-1 aggregates of the pair; a.x becomes the red cell's once gathered
+        uint16_t m[P2];   // mask bytes of the pair: first cell in bits 0-7
+        VT c[P2];         // level-1 value of the red cell's aggregate
+        float q;          // q of the own column
+    };
+    auto load = [&](int p, Raw& X) {
+        const unsigned base = (unsigned)sxy * (unsigned)min(max(p, 0), nz - 1);
+#pragma unroll
+        for (int m = 0; m < P2; ++m) {
+            const unsigned i2 = (ca[m] + base) >> 1;
+            X.z[m] = gld(reinterpret_cast<const f32x2*>(z), i2);
+            X.r[m] = gld(reinterpret_cast<const f32x2*>(rv), i2);
+            X.a[m] = gld(reinterpret_cast<const i32x2*>(L.agg), i2);
+            X.m[m] = gld(reinterpret_cast<const uint16_t*>(L.mask), i2);
+        }
+        X.q = gld(s.q, cown + base);
+    };
+    auto gather = [&](int p, Raw& X) {
+#pragma unroll
+        for (int m = 0; m < P2; ++m) {
+            const bool ared = ((par2[m] + p) & 1) == 0;
+            const int g = ared ? X.a[m].x : X.a[m].y;
+            X.a[m].x = g;
+            X.c[m] = gld(t1, (unsigned)max(g, 0));
+        }
+    };
+    // stage plane p: red cells z + t1[agg] (the prolongation), black cells r zs
+    auto stage = [&](int p, const Raw& X) {
+        const bool pin = p >= 0 && p < nz;
+        float* R = Rb + slot(p) * W2;
+        uint8_t* M = Mb + slot(p) * W2;
+        float* O = Ob + slot(p) * TB;
+#pragma unroll
+        for (int m = 0; m < P2; ++m) {
+            const int u = tid + m * TB;
+            if (u >= NP2) break;
+            const bool oka = pin && ina[m], okb = pin && inb[m];
+            const bool ared = ((par2[m] + p) & 1) == 0;   // the first cell is red
+            const float zr = (ared ? X.z[m].x : X.z[m].y) + (X.a[m].x >= 0 ? VC_LEVEL_SCALE * to_f(X.c[m]) : 0.0f);
+            const float va = ared ? zr : X.r[m].x * zs, vb = ared ? X.r[m].y * zs : zr;
+            *reinterpret_cast<float2*>(R + 2 * u) = float2{oka ? va : 0.0f, okb ? vb : 0.0f};
+            *reinterpret_cast<uint16_t*>(M + 2 * u) =
+                (uint16_t)((oka ? (X.m[m] & 255u) : 0u) | ((okb ? ((unsigned)X.m[m] >> 8) : 0u) << 8));
+            const int o = 2 * u - 2 * nx;
+            if (o >= 0 && o < TB) *reinterpret_cast<float2*>(O + o) = float2{X.r[m].x, X.r[m].y};
+        }
+        Qb[slot(p) * TB + tid] = X.q;
+    };
+    // black cells of plane p over W1 take the Gauss-Seidel update from the
+    // prolonged values of their red neighbours.  The red cells' slots are
+    // written too (as 0): a stencil reads every neighbour slot, weighting a
+    // missing neighbour by 0, and for an even nx the slot past a row's end
+    // belongs to a red cell of the next row; a stale value there could be a
+    // NaN, which a weight of 0 does not remove.
+    auto smooth = [&](int p) {
+        const bool pin = p >= 0 && p < nz;
+        const unsigned kb = (p > 0 ? 16u : 0u) | (p < nz - 1 ? 32u : 0u);
+        const float* R = Rb + slot(p) * W2;
+        const float* Rm = Rb + slot(p - 1) * W2;
+        const float* Rp = Rb + slot(p + 1) * W2;
+        const uint8_t* M = Mb + slot(p) * W2;
+        float* Z = Zb + slot(p) * WZ - zo;
+#pragma unroll
+        for (int m = 0; m < P1; ++m) {
+            if (tid + m * TB >= NP1) break;
+            const bool ared = ((par1[m] + p) & 1) == 0;
+            const int hb = ha1[m] + (ared ? 1 : 0);    // the black cell of the pair
+            const unsigned mk = M[hb];
+            const int dg = __popc(mk);
+            float zb = 0.0f;
+            if (pin && (ared ? inb1[m] : ina1[m]) && dg > 0)
+                zb = div_count(R[hb] + lat_bits(mk & ((ared ? ijb[m] : ija[m]) | kb), R + hb, nx, Rm[hb], Rp[hb]),
+                               (float)dg);
+            *reinterpret_cast<float2*>(Z + ha1[m]) = ared ? float2{0.0f, zb} : float2{zb, 0.0f};
+        }
+    };
+    double acc = 0.0, accq = 0.0;
+    const int hz = 2 * nx + tid - zo;   // the own column in Zb
+    auto emit = [&](int k) {
+        if (!t.valid) return;
+        const int c = t.c2 + sxy * k;
+        const float r = Ob[slot(k) * TB + tid];
+        const float* Z = Zb + slot(k) * WZ + hz;
+        float zc = *Z;   // black cells: the new value
+        if (((t.i + t.j + k) & 1) == 0) {
+            const CoefRegs<Mask01> cf{(unsigned)Mb[slot(k) * W2 + 2 * nx + tid]};
+            const float dg = cell_diag(cf);
+            zc = 0.0f;
+            if (dg > 0)
+                zc = cell_div<Mask01>(r * zs + cell_lat(cf, Cell{c, t.i, t.j, k}, nx, ny, nz, Z,
+                                                        Zb[slot(k - 1) * WZ + hz], Zb[slot(k + 1) * WZ + hz]),
+                                      dg);
+        }
+        gst_nt(reinterpret_cast<float*>(s.z), c, zc);
+        acc += (double)r * zc;
+        if (with_q) accq += (double)Qb[slot(k) * TB + tid] * zc;
+    };
+    const int ks = t.k0 - 2;
+    // stage k+2 (gathered a step ago), gather for plane k+3, refill the
+    // staged buffer with plane k+5, smooth k+1, emit k.  The gathers go out
+    // before the plane loads: memory operations complete in order for the
+    // wave's counter, so waiting for a gather at the next step then leaves
+    // the plane loads in flight.
+    auto step = [&](int k, Raw& X, Raw& Xn) {
+        stage(k + 2, X);
+        gather(min(k + 3, t.k1 + 1), Xn);
+        load(min(k + 5, t.k1 + 1), X);   // past the chunk: repeats the last plane needed
+        __syncthreads();
+        smooth(k + 1);
+        __syncthreads();
+        if (k >= t.k0) emit(k);
+    };
+    Raw X0, X1, X2;
+    {
+        Raw Y;
+        load(ks, Y);
+        gather(ks, Y);
+        stage(ks, Y);
+        load(ks + 1, Y);
+        gather(ks + 1, Y);
+        stage(ks + 1, Y);
+    }
+    load(ks + 2, X0);
+    load(ks + 3, X1);
+    load(ks + 4, X2);
+    gather(ks + 2, X0);
+    int k = ks;
+    for (; k + 2 < t.k1; k += 3) {
+        step(k, X0, X1);
+        step(k + 1, X1, X2);
+        step(k + 2, X2, X0);
+    }
+    if (k < t.k1) step(k, X0, X1);
+    if (k + 1 < t.k1) step(k + 1, X1, X2);
+    double* out = s.part + (long)e.y * NPART;
+    __shared__ double sh[2][TB / 64];
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    acc = wave_sum(acc);
+    if (with_q) accq = wave_sum(accq);
+    if (lane == 0) {
+        sh[0][w] = acc;
+        sh[1][w] = accq;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double a = 0.0, aq = 0.0;
+#pragma unroll
+        for (int ww = 0; ww < TB / 64; ++ww) {
+            a += sh[0][ww];
+            aq += sh[1][ww];
+        }
+        out[3] = a;
+        if (with_q) out[0] = aq;
     }
 }
 

@@ -27,13 +27,13 @@ namespace phg {
 enum KClass {
     K_RESID_INIT, K_DIR_APPLY, K_PRE_RED, K_RB_BLACK, K_RESID0, K_POST_BLACK, K_POST_RED, K_FIN, K_VC_RESTRICT,
     K_VC_JAC, K_VC_RESID, K_VC_PROLONG, K_VC_COARSE, K_XFLUSH, K_ZONE, K_RHS, K_CORRECT, K_ENERGY, K_ADVECT,
-    K_COUNT, K_PRE_SMOOTH, K_NCLASS
+    K_COUNT, K_PRE_SMOOTH, K_POST_SMOOTH, K_NCLASS
 };
 static const char* kKName[K_NCLASS] = {"resid_init",  "dir_apply",  "pre_red",    "rb_black",  "resid0",
                                        "post_black",  "post_red",   "finalize",   "vc_restrict", "vc_jacobi",
                                        "vc_resid",    "vc_prolong", "vc_coarse",  "x_flush",   "zone_mean",
                                        "rhs",         "correct",    "energy",     "advect",    "count_active",
-                                       "pre_smooth"};
+                                       "pre_smooth",  "post_smooth"};
 
 struct GridDev {
     int nx = 0, ny = 0, nz = 0, n = 0;
@@ -291,8 +291,11 @@ void vcycle_t(phg_ctx* ctx, BatchDev* B, int first) {
     const DSys* S = B->d_sys;
     const int2* T0 = B->tab[0];
     const int n0 = B->tab_len[0];
-    if (std::is_same<C0, Mask01>::value && B->maxnx <= FUSE_MAX_NX && B->even_planes && !B->unfused &&
-        B->maxnu0 == 1) {
+    // 0/1 grids of moderate width and even plane size smooth in two fused
+    // passes (one sweep per system); others in the separate half sweeps
+    const bool fused = std::is_same<C0, Mask01>::value && B->maxnx <= FUSE_MAX_NX && B->even_planes && !B->unfused &&
+                       B->maxnu0 == 1;
+    if (fused) {
         // pre-smoothing and residual in one pass; column pairs per thread for
         // the widest grid: staging TB/2 + 2 nx, smoothing up to TB/2 + nx + 1
         static_assert(TB == 256, "pair counts below assume 256 threads");
@@ -337,6 +340,16 @@ void vcycle_t(phg_ctx* ctx, BatchDev* B, int first) {
     }
     // post-smoothing mirrors the pre-smoothing (black, red per sweep), so M
     // stays symmetric; the last red sweep of each system leaves its r.z partials
+    if (fused) {
+        const size_t lds = post_fuse_lds(B->maxnx);
+        if (B->maxnx <= 64)
+            launch(ctx, K_POST_SMOOTH, k_post_fused<VT, 1, 1>, n0, lds, S, T0);
+        else if (B->maxnx <= 127)
+            launch(ctx, K_POST_SMOOTH, k_post_fused<VT, 2, 1>, n0, lds, S, T0);
+        else
+            launch(ctx, K_POST_SMOOTH, k_post_fused<VT, 2, 2>, n0, lds, S, T0);
+        return;
+    }
     launch(ctx, K_POST_BLACK, k_post_black<C0, float, VT>, n0, B->lds0(), S, T0);
     launch(ctx, K_POST_RED, k_post_red<C0, float>, n0, B->lds0(), S, T0, 0);
     for (int v = 1; v < B->maxnu0; ++v) {
@@ -802,7 +815,7 @@ int phg_batch_create_ex(phg_ctx* ctx, int nsys, const int* grids, const phg_batc
         S.zclosed = G->zclosed;
         const size_t n0 = G->n;
         float* v4 = nullptr;
-        rc = dalloc(ctx, B->allocs, &v4, 6 * n0);
+        rc = dalloc(ctx, B->allocs, &v4, 7 * n0);
         if (rc) break;
         S.r = v4;
         S.q = v4 + n0;
@@ -810,6 +823,7 @@ int phg_batch_create_ex(phg_ctx* ctx, int nsys, const int* grids, const phg_batc
         S.d1 = v4 + 3 * n0;
         S.z = v4 + 4 * n0;   // the level-0 iterate stays FP32 in every precision: it feeds d directly
         S.r1 = v4 + 5 * n0;
+        S.z2 = v4 + 6 * n0;
         rc = dalloc(ctx, B->allocs, &S.part, (size_t)S.tiles0 * NPART);
         if (!rc) rc = dalloc(ctx, B->allocs, &S.zpart, (size_t)std::max(G->nztask, 1));
         if (!rc) rc = dalloc(ctx, B->allocs, &S.zmean, (size_t)std::max(G->nclosed, 1));

@@ -35,7 +35,7 @@ from . import scenarios as sc
 # device bytes per cell: fields (u v w, 16 cell fields, 10 advection
 # temporaries, b, diag), geometry and hierarchy, and the solver's work vectors
 # (the step batch and the grid's own one-system batch)
-BYTES_PER_CELL = 260
+BYTES_PER_CELL = 268
 # the dense coarsest inverse is at most 2048^2 FP64
 COARSE_BYTES_MAX = 2048 * 2048 * 8
 
