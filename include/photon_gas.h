@@ -132,12 +132,14 @@ int phg_batch_create(phg_ctx *ctx, int nsys, const int *grids, int *batch_out);
 typedef struct {
     int kchunk;          /* level-0 planes per block along z (0 = from level0_blocks) */
     int level0_blocks;   /* level-0 blocks per launch to aim for when kchunk = 0 (0 = 4096) */
-    int unfused;         /* 1: pre-smoothing as three separate passes instead of the fused one
-                            (same results bit for bit; for comparison) */
+    int unfused;         /* 1: level-0 smoothing as separate half sweeps instead of the two fused
+                            passes (same results bit for bit; for comparison) */
     int sweeps0;         /* level-0 red-black sweeps before and after the coarse correction
                             (0 = 1, the reference's gsColor pair, pressure_mg.cpp:276,296; at most 4) */
     const int *sys_kchunk;   /* per system (nsys entries, 0 = batch value) or NULL */
     const int *sys_sweeps0;  /* per system (nsys entries, 0 = batch value) or NULL */
+    int fused_threads;   /* threads (and own columns) per block of the fused smoothing passes:
+                            0 = 512, or 256 (same results bit for bit) */
 } phg_batch_opts;
 int phg_batch_create_ex(phg_ctx *ctx, int nsys, const int *grids, const phg_batch_opts *opts, int *batch_out);
 int phg_batch_destroy(phg_ctx *ctx, int batch);

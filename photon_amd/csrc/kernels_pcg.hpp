@@ -912,24 +912,40 @@ __global__ void __launch_bounds__(TB) k_post_red(const DSys* __restrict__ sys, c
 // pair when smoothing.
 // LDS rings of four planes: Rb [W2] floats (after a two-float pad), Zb
 // [W1 + 2] floats, Ob [TB] floats, Mb [W2] bytes.
-constexpr int FUSE_MAX_NX = 192;   // staging pairs TB/2 + 2 nx <= 2 TB
+constexpr int FUSE_MAX_NX = 192;   // staging pairs NT/2 + 2 nx <= 2 NT
 
-__host__ __device__ constexpr size_t fuse_lds(int nx) {
-    return (size_t)(2 + 4 * (TB + 4 * nx)) * 4 + (size_t)4 * (TB + 2 * nx + 2) * 4 + (size_t)4 * TB * 4 +
-           (size_t)4 * (TB + 4 * nx);
+// A fused block has NT = TB or 2 TB threads and owns NT columns: the column
+// tile of its table entry and, for 2 TB, the next one (the table then lists
+// the even column tiles only).  A wider block stages fewer halo rows per own
+// column (TB + 4 nx against 2 TB + 4 nx for twice the columns).
+__host__ __device__ constexpr size_t fuse_lds(int nx, int nt) {
+    return (size_t)(2 + 4 * (nt + 4 * nx)) * 4 + (size_t)4 * (nt + 2 * nx + 2) * 4 + (size_t)4 * nt * 4 +
+           (size_t)4 * (nt + 4 * nx);
 }
 
-// P2, P1: staging and smoothing pairs per thread, for TB/2 + 2 nx and at
-// most TB/2 + nx + 1 pairs
-template <class VT, bool FIRST, int P2, int P1>
-__global__ void __launch_bounds__(TB) k_pre_fused(const DSys* __restrict__ sys, const int2* __restrict__ tab) {
+// The partial row of each TB-column tile of a fused block: lane 0 of the
+// tile's first wave adds the tile's wave sums in wave order, as a TB-thread
+// block adds them, so the partials are bitwise those of the separate passes.
+template <class F>
+__device__ __forceinline__ void tile_rows(const DSys& s, const int2& e, int c0, F write) {
+    const int tid = (int)threadIdx.x, lane = tid & 63, w = tid >> 6;
+    if (lane != 0 || (w % (TB / 64)) != 0) return;
+    const int h = w / (TB / 64);
+    if (h > 0 && c0 + h * TB >= s.L[0].nx * s.L[0].ny) return;   // no such column tile
+    write(w, s.part + (long)(e.y + h) * NPART);
+}
+
+// P2, P1: staging and smoothing pairs per thread, for NT/2 + 2 nx and at
+// most NT/2 + nx + 1 pairs
+template <class VT, bool FIRST, int NT, int P2, int P1>
+__global__ void __launch_bounds__(NT) k_pre_fused(const DSys* __restrict__ sys, const int2* __restrict__ tab) {
     PHG_SYS_PROLOGUE
     const DLevel& L = s.L[0];
     const ColTile t = col_tile(s, e.y);
     const int nx = L.nx, ny = L.ny, nz = L.nz, sxy = nx * ny;
-    const int W1 = TB + 2 * nx, W2 = TB + 4 * nx, WZ = W1 + 2;
+    const int W1 = NT + 2 * nx, W2 = NT + 4 * nx, WZ = W1 + 2;
     const int tid = (int)threadIdx.x;
-    const int c0 = t.c2 - tid;     // a multiple of TB, so even
+    const int c0 = t.c2 - tid;     // a multiple of TB, so even (NT = 2 TB: of 2 TB)
     const int zo = nx & ~1;        // Zb holds W2 index h at h - zo, so pairs stay 8-byte aligned
     const float alpha = (float)s.scal[S_ALPHA];
     const float zs = (float)s.scal[S_ZS];
@@ -939,8 +955,8 @@ __global__ void __launch_bounds__(TB) k_pre_fused(const DSys* __restrict__ sys, 
     extern __shared__ float lds[];
     float* Rb = lds + 2;                 // [4][W2]; Rb[-1] is read (and ignored) by an edge pair of an odd nx
     float* Zb = Rb + 4 * W2;             // [4][WZ]
-    float* Ob = Zb + 4 * WZ;             // [4][TB]: rn of the own columns
-    uint8_t* Mb = reinterpret_cast<uint8_t*>(Ob + 4 * TB);   // [4][W2]
+    float* Ob = Zb + 4 * WZ;             // [4][NT]: rn of the own columns
+    uint8_t* Mb = reinterpret_cast<uint8_t*>(Ob + 4 * NT);   // [4][W2]
     auto slot = [](int p) { return (p + 4) & 3; };           // planes >= -4
     auto ij_bits = [&](int c) {                              // the (i, j) part of edge_bits
         const int j = c / nx, i = c - j * nx;
@@ -956,7 +972,7 @@ __global__ void __launch_bounds__(TB) k_pre_fused(const DSys* __restrict__ sys, 
     bool ina[P2], inb[P2];
 #pragma unroll
     for (int m = 0; m < P2; ++m) {
-        const int u = tid + m * TB;
+        const int u = tid + m * NT;
         const int col = c0 - 2 * nx + 2 * u;
         ina[m] = u < NP2 && col >= 0 && col < sxy;
         inb[m] = u < NP2 && col + 1 >= 0 && col + 1 < sxy;
@@ -965,14 +981,14 @@ __global__ void __launch_bounds__(TB) k_pre_fused(const DSys* __restrict__ sys, 
     }
     // smoothing pairs: W2 indices ha, ha + 1 over the W1 range [nx, nx + W1)
     // (for an odd nx the first and last pair stick out by one cell)
-    const int u1 = nx >> 1, NP1 = ((TB + 3 * nx - 1) >> 1) - u1 + 1;
+    const int u1 = nx >> 1, NP1 = ((NT + 3 * nx - 1) >> 1) - u1 + 1;
     int ha1[P1], par1[P1];
     unsigned ija[P1], ijb[P1];
     bool ina1[P1], inb1[P1];
 #pragma unroll
     for (int m = 0; m < P1; ++m) {
-        const bool use = tid + m * TB < NP1;
-        const int h = 2 * (u1 + tid + m * TB), col = c0 - 2 * nx + h;
+        const bool use = tid + m * NT < NP1;
+        const int h = 2 * (u1 + tid + m * NT), col = c0 - 2 * nx + h;
         ha1[m] = h;
         ina1[m] = use && h >= nx && h < nx + W1 && col >= 0 && col < sxy;
         inb1[m] = use && h + 1 >= nx && h + 1 < nx + W1 && col + 1 >= 0 && col + 1 < sxy;
@@ -1008,10 +1024,10 @@ __global__ void __launch_bounds__(TB) k_pre_fused(const DSys* __restrict__ sys, 
         const bool pin = p >= 0 && p < nz;
         float* R = Rb + slot(p) * W2;
         uint8_t* M = Mb + slot(p) * W2;
-        float* O = Ob + slot(p) * TB;
+        float* O = Ob + slot(p) * NT;
 #pragma unroll
         for (int m = 0; m < P2; ++m) {
-            const int u = tid + m * TB;
+            const int u = tid + m * NT;
             if (u >= NP2) break;
             float rna = X.r[m].x, rnb = X.r[m].y;
             if (!FIRST) {
@@ -1028,7 +1044,7 @@ __global__ void __launch_bounds__(TB) k_pre_fused(const DSys* __restrict__ sys, 
                 float2{oka ? (ared ? vr : rsa) : 0.0f, okb ? (ared ? rsb : vr) : 0.0f};
             *reinterpret_cast<uint16_t*>(M + 2 * u) = (uint16_t)((oka ? ma : 0u) | ((okb ? mb : 0u) << 8));
             const int o = 2 * u - 2 * nx;
-            if (o >= 0 && o < TB) *reinterpret_cast<float2*>(O + o) = float2{oka ? rna : 0.0f, okb ? rnb : 0.0f};
+            if (o >= 0 && o < NT) *reinterpret_cast<float2*>(O + o) = float2{oka ? rna : 0.0f, okb ? rnb : 0.0f};
         }
     };
     // smoothed values of plane p over W1: red cells keep their value, black
@@ -1043,7 +1059,7 @@ __global__ void __launch_bounds__(TB) k_pre_fused(const DSys* __restrict__ sys, 
         float* Z = Zb + slot(p) * WZ - zo;
 #pragma unroll
         for (int m = 0; m < P1; ++m) {
-            if (tid + m * TB >= NP1) break;
+            if (tid + m * NT >= NP1) break;
             const bool ared = ((par1[m] + p) & 1) == 0;
             const int hb = ha1[m] + (ared ? 1 : 0);    // the black cell of the pair
             const float xm = R[hb - 1], xp = R[hb + 1];
@@ -1064,7 +1080,7 @@ __global__ void __launch_bounds__(TB) k_pre_fused(const DSys* __restrict__ sys, 
     auto emit = [&](int k) {
         if (!t.valid) return;
         const int c = t.c2 + sxy * k;
-        const float rn = Ob[slot(k) * TB + tid];
+        const float rn = Ob[slot(k) * NT + tid];
         const float* Z = Zb + slot(k) * WZ + hz;
         const float zc = *Z;
         const CoefRegs<Mask01> cf{(unsigned)Mb[slot(k) * W2 + 2 * nx + tid]};
@@ -1115,20 +1131,27 @@ __global__ void __launch_bounds__(TB) k_pre_fused(const DSys* __restrict__ sys, 
     if (k < t.k1) step(k, X0);
     if (k + 1 < t.k1) step(k + 1, X1);
     if (FIRST) return;
-    double* out = s.part + (long)e.y * NPART;
-    __shared__ double sh[TB / 64];
-    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    __shared__ double sh[NT / 64];
+    __shared__ float shm[NT / 64];
+    const int lane = tid & 63, w = tid >> 6;
     const double v = wave_sum(acc);
-    if (lane == 0) sh[w] = v;
-    __syncthreads();
-    mx = block_max(mx);
-    if (threadIdx.x == 0) {
-        double tot = 0.0;
-#pragma unroll
-        for (int ww = 0; ww < TB / 64; ++ww) tot += sh[ww];
-        out[1] = tot;
-        out[2] = mx;
+    const float wm = wave_max(mx);
+    if (lane == 0) {
+        sh[w] = v;
+        shm[w] = wm;
     }
+    __syncthreads();
+    tile_rows(s, e, c0, [&](int w0, double* out) {
+        double tot = 0.0;
+        float m = 0.0f;
+#pragma unroll
+        for (int ww = 0; ww < TB / 64; ++ww) {
+            tot += sh[w0 + ww];
+            m = fmaxf(m, shm[w0 + ww]);
+        }
+        out[1] = tot;
+        out[2] = m;
+    });
 }
 
 // ---------------------------------------------------------------------------
@@ -1150,17 +1173,17 @@ __global__ void __launch_bounds__(TB) k_pre_fused(const DSys* __restrict__ sys, 
 // kernels, and each thread sums its column's terms in the same order, so the
 // results are bitwise the same; the pass reads z and r once and writes z once
 // (12 B/cell, 16 with q, in place of about 27).
-__host__ __device__ constexpr size_t post_fuse_lds(int nx) { return fuse_lds(nx) + (size_t)4 * TB * 4; }
+__host__ __device__ constexpr size_t post_fuse_lds(int nx, int nt) { return fuse_lds(nx, nt) + (size_t)4 * nt * 4; }
 
-template <class VT, int P2, int P1>
-__global__ void __launch_bounds__(TB) k_post_fused(const DSys* __restrict__ sys, const int2* __restrict__ tab) {
+template <class VT, int NT, int P2, int P1>
+__global__ void __launch_bounds__(NT) k_post_fused(const DSys* __restrict__ sys, const int2* __restrict__ tab) {
     PHG_SYS_PROLOGUE
     const DLevel& L = s.L[0];
     const ColTile t = col_tile(s, e.y);
     const int nx = L.nx, ny = L.ny, nz = L.nz, sxy = nx * ny;
-    const int W1 = TB + 2 * nx, W2 = TB + 4 * nx, WZ = W1 + 2;
+    const int W1 = NT + 2 * nx, W2 = NT + 4 * nx, WZ = W1 + 2;
     const int tid = (int)threadIdx.x;
-    const int c0 = t.c2 - tid;     // a multiple of TB, so even
+    const int c0 = t.c2 - tid;     // a multiple of TB, so even (NT = 2 TB: of 2 TB)
     const int zo = nx & ~1;        // Zb holds W2 index h at h - zo, so pairs stay 8-byte aligned
     const float zs = (float)s.scal[S_ZS];
     const bool with_q = st_uni(s, ST_FLEX) != 0 && st_uni(s, ST_VC) != 0;   // q exists once an iteration has run
@@ -1170,9 +1193,9 @@ __global__ void __launch_bounds__(TB) k_post_fused(const DSys* __restrict__ sys,
     extern __shared__ float lds[];
     float* Rb = lds + 2;                 // [4][W2]; Rb[-1] is read (and ignored) by an edge pair of an odd nx
     float* Zb = Rb + 4 * W2;             // [4][WZ]
-    float* Ob = Zb + 4 * WZ;             // [4][TB]: r of the own columns
-    uint8_t* Mb = reinterpret_cast<uint8_t*>(Ob + 4 * TB);   // [4][W2]
-    float* Qb = reinterpret_cast<float*>(Mb + 4 * W2);       // [4][TB]: q of the own columns
+    float* Ob = Zb + 4 * WZ;             // [4][NT]: r of the own columns
+    uint8_t* Mb = reinterpret_cast<uint8_t*>(Ob + 4 * NT);   // [4][W2]
+    float* Qb = reinterpret_cast<float*>(Mb + 4 * W2);       // [4][NT]: q of the own columns
     auto slot = [](int p) { return (p + 4) & 3; };           // planes >= -4
     auto ij_bits = [&](int c) {                              // the (i, j) part of edge_bits
         const int j = c / nx, i = c - j * nx;
@@ -1184,7 +1207,7 @@ __global__ void __launch_bounds__(TB) k_post_fused(const DSys* __restrict__ sys,
     bool ina[P2], inb[P2];
 #pragma unroll
     for (int m = 0; m < P2; ++m) {
-        const int u = tid + m * TB;
+        const int u = tid + m * NT;
         const int col = c0 - 2 * nx + 2 * u;
         ina[m] = u < NP2 && col >= 0 && col < sxy;
         inb[m] = u < NP2 && col + 1 >= 0 && col + 1 < sxy;
@@ -1192,14 +1215,14 @@ __global__ void __launch_bounds__(TB) k_post_fused(const DSys* __restrict__ sys,
         par2[m] = (ca[m] % nx + ca[m] / nx) & 1;
     }
     // smoothing pairs (as in k_pre_fused)
-    const int u1 = nx >> 1, NP1 = ((TB + 3 * nx - 1) >> 1) - u1 + 1;
+    const int u1 = nx >> 1, NP1 = ((NT + 3 * nx - 1) >> 1) - u1 + 1;
     int ha1[P1], par1[P1];
     unsigned ija[P1], ijb[P1];
     bool ina1[P1], inb1[P1];
 #pragma unroll
     for (int m = 0; m < P1; ++m) {
-        const bool use = tid + m * TB < NP1;
-        const int h = 2 * (u1 + tid + m * TB), col = c0 - 2 * nx + h;
+        const bool use = tid + m * NT < NP1;
+        const int h = 2 * (u1 + tid + m * NT), col = c0 - 2 * nx + h;
         ha1[m] = h;
         ina1[m] = use && h >= nx && h < nx + W1 && col >= 0 && col < sxy;
         inb1[m] = use && h + 1 >= nx && h + 1 < nx + W1 && col + 1 >= 0 && col + 1 < sxy;
@@ -1247,10 +1270,10 @@ __global__ void __launch_bounds__(TB) k_post_fused(const DSys* __restrict__ sys,
         const bool pin = p >= 0 && p < nz;
         float* R = Rb + slot(p) * W2;
         uint8_t* M = Mb + slot(p) * W2;
-        float* O = Ob + slot(p) * TB;
+        float* O = Ob + slot(p) * NT;
 #pragma unroll
         for (int m = 0; m < P2; ++m) {
-            const int u = tid + m * TB;
+            const int u = tid + m * NT;
             if (u >= NP2) break;
             const bool oka = pin && ina[m], okb = pin && inb[m];
             const bool ared = ((par2[m] + p) & 1) == 0;   // the first cell is red
@@ -1260,9 +1283,9 @@ __global__ void __launch_bounds__(TB) k_post_fused(const DSys* __restrict__ sys,
             *reinterpret_cast<uint16_t*>(M + 2 * u) =
                 (uint16_t)((oka ? (X.m[m] & 255u) : 0u) | ((okb ? ((unsigned)X.m[m] >> 8) : 0u) << 8));
             const int o = 2 * u - 2 * nx;
-            if (o >= 0 && o < TB) *reinterpret_cast<float2*>(O + o) = float2{X.r[m].x, X.r[m].y};
+            if (o >= 0 && o < NT) *reinterpret_cast<float2*>(O + o) = float2{X.r[m].x, X.r[m].y};
         }
-        Qb[slot(p) * TB + tid] = X.q;
+        Qb[slot(p) * NT + tid] = X.q;
     };
     // black cells of plane p over W1 take the Gauss-Seidel update from the
     // prolonged values of their red neighbours.  The red cells' slots are
@@ -1280,7 +1303,7 @@ __global__ void __launch_bounds__(TB) k_post_fused(const DSys* __restrict__ sys,
         float* Z = Zb + slot(p) * WZ - zo;
 #pragma unroll
         for (int m = 0; m < P1; ++m) {
-            if (tid + m * TB >= NP1) break;
+            if (tid + m * NT >= NP1) break;
             const bool ared = ((par1[m] + p) & 1) == 0;
             const int hb = ha1[m] + (ared ? 1 : 0);    // the black cell of the pair
             const unsigned mk = M[hb];
@@ -1297,7 +1320,7 @@ __global__ void __launch_bounds__(TB) k_post_fused(const DSys* __restrict__ sys,
     auto emit = [&](int k) {
         if (!t.valid) return;
         const int c = t.c2 + sxy * k;
-        const float r = Ob[slot(k) * TB + tid];
+        const float r = Ob[slot(k) * NT + tid];
         const float* Z = Zb + slot(k) * WZ + hz;
         float zc = *Z;   // black cells: the new value
         if (((t.i + t.j + k) & 1) == 0) {
@@ -1311,7 +1334,7 @@ __global__ void __launch_bounds__(TB) k_post_fused(const DSys* __restrict__ sys,
         }
         gst_nt(reinterpret_cast<float*>(s.z), c, zc);
         acc += (double)r * zc;
-        if (with_q) accq += (double)Qb[slot(k) * TB + tid] * zc;
+        if (with_q) accq += (double)Qb[slot(k) * NT + tid] * zc;
     };
     const int ks = t.k0 - 2;
     // stage k+2 (gathered a step ago), gather for plane k+3, refill the
@@ -1350,9 +1373,8 @@ __global__ void __launch_bounds__(TB) k_post_fused(const DSys* __restrict__ sys,
     }
     if (k < t.k1) step(k, X0, X1);
     if (k + 1 < t.k1) step(k + 1, X1, X2);
-    double* out = s.part + (long)e.y * NPART;
-    __shared__ double sh[2][TB / 64];
-    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    __shared__ double sh[2][NT / 64];
+    const int lane = tid & 63, w = tid >> 6;
     acc = wave_sum(acc);
     if (with_q) accq = wave_sum(accq);
     if (lane == 0) {
@@ -1360,16 +1382,16 @@ __global__ void __launch_bounds__(TB) k_post_fused(const DSys* __restrict__ sys,
         sh[1][w] = accq;
     }
     __syncthreads();
-    if (threadIdx.x == 0) {
+    tile_rows(s, e, c0, [&](int w0, double* out) {
         double a = 0.0, aq = 0.0;
 #pragma unroll
         for (int ww = 0; ww < TB / 64; ++ww) {
-            a += sh[0][ww];
-            aq += sh[1][ww];
+            a += sh[0][w0 + ww];
+            aq += sh[1][w0 + ww];
         }
         out[3] = a;
         if (with_q) out[0] = aq;
-    }
+    });
 }
 
 // z (V-cycle type) -> FP32 copy, for phg_precondition

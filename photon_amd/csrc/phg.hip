@@ -95,6 +95,9 @@ struct BatchDev {
     std::vector<DSys> h_sys;
     DSys* d_sys = nullptr;
     std::vector<int2*> tab;        // level 0: (system, tile); tab[0] only
+    int2* tabw = nullptr;          // level 0 entries of even column tiles (blocks of 2 TB columns)
+    int tabw_len = 0;
+    int fuse_nt = 2 * TB;          // threads per block of the fused smoothing passes
     std::vector<int> tab_len;
     std::vector<int4*> tabg;       // levels >= 1: (first system, count, tile) of same-grid groups
     std::vector<int> tabg_len;
@@ -182,7 +185,7 @@ void prof_drain(phg_ctx* ctx) {
 }
 
 template <typename... KArgs, typename... Args>
-void launch(phg_ctx* ctx, int cls, void (*kern)(KArgs...), int blocks, size_t smem, Args... args) {
+void launch_nt(phg_ctx* ctx, int cls, void (*kern)(KArgs...), int blocks, int threads, size_t smem, Args... args) {
     if (blocks <= 0) return;
     hipEvent_t a = nullptr, b = nullptr;
     if (ctx->prof) {
@@ -190,11 +193,15 @@ void launch(phg_ctx* ctx, int cls, void (*kern)(KArgs...), int blocks, size_t sm
         b = take_event(ctx);
         (void)hipEventRecord(a, ctx->stream);
     }
-    hipLaunchKernelGGL(kern, dim3(blocks), dim3(TB), smem, ctx->stream, args...);
+    hipLaunchKernelGGL(kern, dim3(blocks), dim3(threads), smem, ctx->stream, args...);
     if (ctx->prof) {
         (void)hipEventRecord(b, ctx->stream);
         ctx->pending.push_back({cls, a, b});
     }
+}
+template <typename... KArgs, typename... Args>
+void launch(phg_ctx* ctx, int cls, void (*kern)(KArgs...), int blocks, size_t smem, Args... args) {
+    launch_nt(ctx, cls, kern, blocks, TB, smem, args...);
 }
 
 template <class T>
@@ -296,17 +303,29 @@ void vcycle_t(phg_ctx* ctx, BatchDev* B, int first) {
     const bool fused = std::is_same<C0, Mask01>::value && B->maxnx <= FUSE_MAX_NX && B->even_planes && !B->unfused &&
                        B->maxnu0 == 1;
     if (fused) {
-        // pre-smoothing and residual in one pass; column pairs per thread for
-        // the widest grid: staging TB/2 + 2 nx, smoothing up to TB/2 + nx + 1
+        // pre-smoothing and residual in one pass.  Column pairs per thread
+        // for the widest grid: staging NT/2 + 2 nx, smoothing up to
+        // NT/2 + nx + 1; 2 TB-thread blocks take the even column tiles.
         static_assert(TB == 256, "pair counts below assume 256 threads");
-        const size_t lds = fuse_lds(B->maxnx);
-        auto go = [&](auto k1, auto k0) { launch(ctx, K_PRE_SMOOTH, first ? k1 : k0, n0, lds, S, T0); };
-        if (B->maxnx <= 64)
-            go(k_pre_fused<VT, true, 1, 1>, k_pre_fused<VT, false, 1, 1>);
-        else if (B->maxnx <= 127)
-            go(k_pre_fused<VT, true, 2, 1>, k_pre_fused<VT, false, 2, 1>);
-        else
-            go(k_pre_fused<VT, true, 2, 2>, k_pre_fused<VT, false, 2, 2>);
+        if (B->fuse_nt == 2 * TB) {
+            const size_t lds = fuse_lds(B->maxnx, 2 * TB);
+            auto go = [&](auto k1, auto k0) {
+                launch_nt(ctx, K_PRE_SMOOTH, first ? k1 : k0, B->tabw_len, 2 * TB, lds, S, (const int2*)B->tabw);
+            };
+            if (B->maxnx <= 128)
+                go(k_pre_fused<VT, true, 2 * TB, 1, 1>, k_pre_fused<VT, false, 2 * TB, 1, 1>);
+            else
+                go(k_pre_fused<VT, true, 2 * TB, 2, 1>, k_pre_fused<VT, false, 2 * TB, 2, 1>);
+        } else {
+            const size_t lds = fuse_lds(B->maxnx, TB);
+            auto go = [&](auto k1, auto k0) { launch(ctx, K_PRE_SMOOTH, first ? k1 : k0, n0, lds, S, T0); };
+            if (B->maxnx <= 64)
+                go(k_pre_fused<VT, true, TB, 1, 1>, k_pre_fused<VT, false, TB, 1, 1>);
+            else if (B->maxnx <= 127)
+                go(k_pre_fused<VT, true, TB, 2, 1>, k_pre_fused<VT, false, TB, 2, 1>);
+            else
+                go(k_pre_fused<VT, true, TB, 2, 2>, k_pre_fused<VT, false, TB, 2, 2>);
+        }
     } else {
         // red from zero, black, then (red, black) for each further sweep of
         // the systems that make it, then the residual
@@ -341,13 +360,22 @@ void vcycle_t(phg_ctx* ctx, BatchDev* B, int first) {
     // post-smoothing mirrors the pre-smoothing (black, red per sweep), so M
     // stays symmetric; the last red sweep of each system leaves its r.z partials
     if (fused) {
-        const size_t lds = post_fuse_lds(B->maxnx);
-        if (B->maxnx <= 64)
-            launch(ctx, K_POST_SMOOTH, k_post_fused<VT, 1, 1>, n0, lds, S, T0);
-        else if (B->maxnx <= 127)
-            launch(ctx, K_POST_SMOOTH, k_post_fused<VT, 2, 1>, n0, lds, S, T0);
-        else
-            launch(ctx, K_POST_SMOOTH, k_post_fused<VT, 2, 2>, n0, lds, S, T0);
+        if (B->fuse_nt == 2 * TB) {
+            const size_t lds = post_fuse_lds(B->maxnx, 2 * TB);
+            const int2* TW = B->tabw;
+            if (B->maxnx <= 128)
+                launch_nt(ctx, K_POST_SMOOTH, k_post_fused<VT, 2 * TB, 1, 1>, B->tabw_len, 2 * TB, lds, S, TW);
+            else
+                launch_nt(ctx, K_POST_SMOOTH, k_post_fused<VT, 2 * TB, 2, 1>, B->tabw_len, 2 * TB, lds, S, TW);
+        } else {
+            const size_t lds = post_fuse_lds(B->maxnx, TB);
+            if (B->maxnx <= 64)
+                launch(ctx, K_POST_SMOOTH, k_post_fused<VT, TB, 1, 1>, n0, lds, S, T0);
+            else if (B->maxnx <= 127)
+                launch(ctx, K_POST_SMOOTH, k_post_fused<VT, TB, 2, 1>, n0, lds, S, T0);
+            else
+                launch(ctx, K_POST_SMOOTH, k_post_fused<VT, TB, 2, 2>, n0, lds, S, T0);
+        }
         return;
     }
     launch(ctx, K_POST_BLACK, k_post_black<C0, float, VT>, n0, B->lds0(), S, T0);
@@ -739,7 +767,8 @@ int phg_batch_create(phg_ctx* ctx, int nsys, const int* grids, int* batch_out) {
 int phg_batch_create_ex(phg_ctx* ctx, int nsys, const int* grids, const phg_batch_opts* opts, int* batch_out) {
     if (!ctx || nsys <= 0 || !grids || !batch_out) return PHG_EINVAL;
     if (opts && (opts->kchunk < 0 || opts->level0_blocks < 0 || opts->unfused < 0 || opts->unfused > 1 ||
-                 opts->sweeps0 < 0 || opts->sweeps0 > kMaxSweeps0))
+                 opts->sweeps0 < 0 || opts->sweeps0 > kMaxSweeps0 ||
+                 (opts->fused_threads != 0 && opts->fused_threads != TB && opts->fused_threads != 2 * TB)))
         return fail(ctx, PHG_EINVAL, "bad batch options");
     if (opts)
         for (int s = 0; s < nsys; ++s) {
@@ -752,6 +781,7 @@ int phg_batch_create_ex(phg_ctx* ctx, int nsys, const int* grids, const phg_batc
     B->nsys = nsys;
     B->grids.assign(grids, grids + nsys);
     B->unfused = opts && opts->unfused;
+    B->fuse_nt = opts && opts->fused_threads ? opts->fused_threads : 2 * TB;
     B->h_sys.resize(nsys);
     int rc = 0;
     long col_tiles = 0;
@@ -884,6 +914,14 @@ int phg_batch_create_ex(phg_ctx* ctx, int nsys, const int* grids, const phg_batc
         std::vector<int2> phys = xcd_order(logical);
         B->tab_len[0] = (int)phys.size();
         rc = upload(ctx, B->allocs, &B->tab[0], phys.data(), phys.size());
+        std::vector<int2> wide;
+        for (const int2& e : logical) {
+            const GridDev* G = get_grid(ctx, grids[e.x]);
+            if ((e.y % tiles(G->nx * G->ny)) % 2 == 0) wide.push_back(e);
+        }
+        std::vector<int2> wphys = xcd_order(wide);
+        B->tabw_len = (int)wphys.size();
+        if (!rc) rc = upload(ctx, B->allocs, &B->tabw, wphys.data(), wphys.size());
     }
     for (int l = 1; l < B->maxl && !rc; ++l) {
         std::vector<int4> logical;
