@@ -138,7 +138,8 @@ typedef struct {
                             (0 = 1, the reference's gsColor pair, pressure_mg.cpp:276,296; at most 4) */
     const int *sys_kchunk;   /* per system (nsys entries, 0 = batch value) or NULL */
     const int *sys_sweeps0;  /* per system (nsys entries, 0 = batch value) or NULL */
-    int fused_threads;   /* threads (and own columns) per block of the fused smoothing passes:
+    int fused_threads;   /* threads (and own columns) per block of the level-0 passes that stage
+                            halo rows (the fused smoothing passes, the direction update):
                             0 = 512, or 256 (same results bit for bit) */
 } phg_batch_opts;
 int phg_batch_create_ex(phg_ctx *ctx, int nsys, const int *grids, const phg_batch_opts *opts, int *batch_out);

@@ -137,6 +137,19 @@ __device__ __forceinline__ ColTile col_tile(const DSys& s, int tile) {
     return t;
 }
 
+// The partial row of each TB-column tile of a block of NT = TB or 2 TB
+// threads: lane 0 of the tile's first wave adds the tile's wave sums in wave
+// order, as a TB-thread block adds them, so the partials are bitwise those
+// of TB-thread blocks.
+template <class F>
+__device__ __forceinline__ void tile_rows(const DSys& s, const int2& e, int c0, F write) {
+    const int tid = (int)threadIdx.x, lane = tid & 63, w = tid >> 6;
+    if (lane != 0 || (w % (TB / 64)) != 0) return;
+    const int h = w / (TB / 64);
+    if (h > 0 && c0 + h * TB >= s.L[0].nx * s.L[0].ny) return;   // no such column tile
+    write(w, s.part + (long)(e.y + h) * NPART);
+}
+
 // Coarse right-hand sides are restricted by summing children, so each level's
 // is up to 8 times the one above (a point source) while the Galerkin operator
 // grows 4 times: 2-byte storage would overflow a few levels down.  Level l+1
@@ -170,26 +183,28 @@ constexpr float VC_LEVEL_SCALE = 4.0f;
 constexpr int MARCH_HALO = 2;
 constexpr int MARCH_MAX_NX = 2048;   // LDS plane buffers: 2 (TB + 2 nx) floats
 
-template <class RAW, class MID, class FIN, class PRE, class BODY>
+// NT: threads (own columns) of the block; HS: halo slots per thread, enough
+// for 2 nx <= HS NT (wider rows take the loop in `step`)
+template <int NT = TB, int HS = MARCH_HALO, class RAW, class MID, class FIN, class PRE, class BODY>
 __device__ __forceinline__ void march(const DSys& s, const ColTile& t, float* lds, RAW raw, MID mid, FIN fin, PRE pre,
                                       BODY body) {
     const DLevel& L = s.L[0];
     const int nx = L.nx, nz = L.nz, sxy = nx * L.ny;
-    const int W = TB + 2 * nx;
+    const int W = NT + 2 * nx;
     const int c0 = t.c2 - (int)threadIdx.x;
     using R = decltype(raw(0));
     using P = decltype(pre(Cell{}));
     struct Plane {
         R own;
-        R h[MARCH_HALO];
+        R h[HS];
     };
     const int cown = t.valid ? t.c2 : sxy - 1;   // any in-bounds column for idle lanes
-    int hcol[MARCH_HALO];
-    bool hin[MARCH_HALO];
+    int hcol[HS];
+    bool hin[HS];
 #pragma unroll
-    for (int m = 0; m < MARCH_HALO; ++m) {
-        const int hh = (int)threadIdx.x + m * TB;
-        const int col = hh < nx ? c0 - nx + hh : c0 + TB + (hh - nx);
+    for (int m = 0; m < HS; ++m) {
+        const int hh = (int)threadIdx.x + m * NT;
+        const int col = hh < nx ? c0 - nx + hh : c0 + NT + (hh - nx);
         hin[m] = hh < 2 * nx && col >= 0 && col < sxy;
         hcol[m] = hin[m] ? col : cown;
     }
@@ -198,20 +213,20 @@ __device__ __forceinline__ void march(const DSys& s, const ColTile& t, float* ld
         const unsigned base = (unsigned)sxy * (unsigned)kcl(k);
         pl.own = raw(cown + base);
 #pragma unroll
-        for (int m = 0; m < MARCH_HALO; ++m) pl.h[m] = raw(hcol[m] + base);
+        for (int m = 0; m < HS; ++m) pl.h[m] = raw(hcol[m] + base);
     };
     auto mid_plane = [&](Plane& pl) {
         mid(pl.own);
 #pragma unroll
-        for (int m = 0; m < MARCH_HALO; ++m) mid(pl.h[m]);
+        for (int m = 0; m < HS; ++m) mid(pl.h[m]);
     };
     auto fin_own = [&](int k, const R& r) {
         const float v = fin(r);
         return (t.valid && k >= 0 && k < nz) ? v : 0.0f;
     };
-    auto fin_halo = [&](int k, const Plane& pl, float (&h)[MARCH_HALO]) {
+    auto fin_halo = [&](int k, const Plane& pl, float (&h)[HS]) {
 #pragma unroll
-        for (int m = 0; m < MARCH_HALO; ++m) {
+        for (int m = 0; m < HS; ++m) {
             const float v = fin(pl.h[m]);
             h[m] = (hin[m] && k < nz) ? v : 0.0f;
         }
@@ -225,7 +240,7 @@ __device__ __forceinline__ void march(const DSys& s, const ColTile& t, float* ld
     // raw planes X0..X2 hold k+1, k+2 and receive k+3; operands P0..P2 hold k,
     // k+1 and receive k+2.
     Plane X0, X1, X2;
-    float vprev, vcur, hfin[MARCH_HALO];
+    float vprev, vcur, hfin[HS];
     {
         Plane Z;
         load(t.k0 - 1, Z);
@@ -244,23 +259,23 @@ __device__ __forceinline__ void march(const DSys& s, const ColTile& t, float* ld
         float* buf = lds + (k & 1) * W;
         buf[nx + threadIdx.x] = vcur;   // 0 for columns past the grid
 #pragma unroll
-        for (int m = 0; m < MARCH_HALO; ++m) {
-            const int hh = (int)threadIdx.x + m * TB;
-            if (hh < 2 * nx) buf[hh < nx ? hh : TB + hh] = hfin[m];
+        for (int m = 0; m < HS; ++m) {
+            const int hh = (int)threadIdx.x + m * NT;
+            if (hh < 2 * nx) buf[hh < nx ? hh : NT + hh] = hfin[m];
         }
-        for (int hh = (int)threadIdx.x + MARCH_HALO * TB; hh < 2 * nx; hh += TB) {
-            const int col = hh < nx ? c0 - nx + hh : c0 + TB + (hh - nx);
+        for (int hh = (int)threadIdx.x + HS * NT; hh < 2 * nx; hh += NT) {
+            const int col = hh < nx ? c0 - nx + hh : c0 + NT + (hh - nx);
             float v = 0.0f;
             if (col >= 0 && col < sxy) {
                 R rv = raw((unsigned)col + (unsigned)sxy * (unsigned)k);
                 mid(rv);
                 v = fin(rv);
             }
-            buf[hh < nx ? hh : TB + hh] = v;
+            buf[hh < nx ? hh : NT + hh] = v;
         }
         __syncthreads();
         const float vnext = fin_own(k + 1, A.own);
-        float hnext[MARCH_HALO];
+        float hnext[HS];
         fin_halo(k + 1, A, hnext);
         // no branch around loads (a branch would merge the wait counters and
         // force an early wait): past the chunk they repeat the last plane
@@ -272,7 +287,7 @@ __device__ __forceinline__ void march(const DSys& s, const ColTile& t, float* ld
         vprev = vcur;
         vcur = vnext;
 #pragma unroll
-        for (int m = 0; m < MARCH_HALO; ++m) hfin[m] = hnext[m];
+        for (int m = 0; m < HS; ++m) hfin[m] = hnext[m];
     };
     // whole groups of three steps in the loop, the rest after it: no step
     // inside the loop is conditional, so the rings keep their registers
@@ -443,8 +458,8 @@ __global__ void __launch_bounds__(TB) k_resid_init(const DSys* __restrict__ sys,
 // x += alpha d_{k-1} (gasphase.cpp:612), deferred to this pass which reads
 // d_{k-1} anyway.  d is double-buffered because neighbours read d_{k-1}
 // while d_k is written.
-template <class OT, class ZT>
-__global__ void __launch_bounds__(TB) k_dir_apply(const DSys* __restrict__ sys, const int2* __restrict__ tab) {
+template <class OT, class ZT, int NT = TB, int HS = MARCH_HALO>
+__global__ void __launch_bounds__(NT) k_dir_apply(const DSys* __restrict__ sys, const int2* __restrict__ tab) {
     PHG_SYS_PROLOGUE
     const DLevel& L = s.L[0];
     const ColTile t = col_tile(s, e.y);
@@ -472,7 +487,7 @@ __global__ void __launch_bounds__(TB) k_dir_apply(const DSys* __restrict__ sys, 
         o.dold = gld(dold, q.c);
         return o;
     };
-    march(s, t, lds, raw, [](auto&) {}, fin, pre, [&](int k, float vp, float dc, float vn, const float* p, const Op& o) {
+    march<NT, HS>(s, t, lds, raw, [](auto&) {}, fin, pre, [&](int k, float vp, float dc, float vn, const float* p, const Op& o) {
         const Cell q{t.c2 + sxy * k, t.i, t.j, k};
         const float dg = cell_diag(o.cf);
         float y = 0.0f;
@@ -482,8 +497,22 @@ __global__ void __launch_bounds__(TB) k_dir_apply(const DSys* __restrict__ sys, 
         gst_nt(s.q, q.c, y);
         acc += (double)y * dc;
     });
-    double v[1] = {acc};
-    block_sums<1>(v, s.part + (long)e.y * NPART);
+    if constexpr (NT == TB) {
+        double v[1] = {acc};
+        block_sums<1>(v, s.part + (long)e.y * NPART);
+    } else {
+        __shared__ double sh[NT / 64];
+        const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+        const double v = wave_sum(acc);
+        if (lane == 0) sh[w] = v;
+        __syncthreads();
+        tile_rows(s, e, t.c2 - (int)threadIdx.x, [&](int w0, double* out) {
+            double a = 0.0;
+#pragma unroll
+            for (int ww = 0; ww < TB / 64; ++ww) a += sh[w0 + ww];
+            out[0] = a;
+        });
+    }
 }
 
 // after the loop: the last x += alpha d of systems that stopped (ST_XPEND)
@@ -921,18 +950,6 @@ constexpr int FUSE_MAX_NX = 192;   // staging pairs NT/2 + 2 nx <= 2 NT
 __host__ __device__ constexpr size_t fuse_lds(int nx, int nt) {
     return (size_t)(2 + 4 * (nt + 4 * nx)) * 4 + (size_t)4 * (nt + 2 * nx + 2) * 4 + (size_t)4 * nt * 4 +
            (size_t)4 * (nt + 4 * nx);
-}
-
-// The partial row of each TB-column tile of a fused block: lane 0 of the
-// tile's first wave adds the tile's wave sums in wave order, as a TB-thread
-// block adds them, so the partials are bitwise those of the separate passes.
-template <class F>
-__device__ __forceinline__ void tile_rows(const DSys& s, const int2& e, int c0, F write) {
-    const int tid = (int)threadIdx.x, lane = tid & 63, w = tid >> 6;
-    if (lane != 0 || (w % (TB / 64)) != 0) return;
-    const int h = w / (TB / 64);
-    if (h > 0 && c0 + h * TB >= s.L[0].nx * s.L[0].ny) return;   // no such column tile
-    write(w, s.part + (long)(e.y + h) * NPART);
 }
 
 // P2, P1: staging and smoothing pairs per thread, for NT/2 + 2 nx and at

@@ -399,10 +399,23 @@ void vcycle(phg_ctx* ctx, BatchDev* B, int first) {
     }
 }
 
+// d = z + beta d_old, q = A d: on 2 TB-column blocks (the even column tiles)
+// unless the batch asks for TB; one halo slot per thread when the rows allow
 template <class OT>
 void dir_apply(phg_ctx* ctx, BatchDev* B) {
     const DSys* S = B->d_sys;
-    launch(ctx, K_DIR_APPLY, k_dir_apply<OT, float>, B->tab_len[0], B->lds0(), S, B->tab[0]);
+    if (B->fuse_nt == 2 * TB) {
+        const size_t lds = 2 * (size_t)(2 * TB + 2 * B->maxnx) * sizeof(float);
+        const int2* TW = B->tabw;
+        if (2 * B->maxnx <= 2 * TB)
+            launch_nt(ctx, K_DIR_APPLY, k_dir_apply<OT, float, 2 * TB, 1>, B->tabw_len, 2 * TB, lds, S, TW);
+        else
+            launch_nt(ctx, K_DIR_APPLY, k_dir_apply<OT, float, 2 * TB, MARCH_HALO>, B->tabw_len, 2 * TB, lds, S, TW);
+    } else if (2 * B->maxnx <= TB) {
+        launch(ctx, K_DIR_APPLY, k_dir_apply<OT, float, TB, 1>, B->tab_len[0], B->lds0(), S, B->tab[0]);
+    } else {
+        launch(ctx, K_DIR_APPLY, k_dir_apply<OT, float>, B->tab_len[0], B->lds0(), S, B->tab[0]);
+    }
 }
 
 void cg_iteration(phg_ctx* ctx, BatchDev* B) {
