@@ -1443,8 +1443,47 @@ __device__ __forceinline__ Grp grp_of(const DSys* __restrict__ sys, const int4& 
     return G;
 }
 
+// acc[g] += the sum over the row's ELL entries, in entry order, of
+// val * src_g[col], for the systems of a group.  Entries are taken eight at a
+// time with every load of the eight issued before the first is used: a loop
+// over single entries waits twice per entry (for the column, then for the
+// gather through it).  Past the row's width the loads repeat its last entry
+// and are not used (the width is the level's, the same for every thread).
+constexpr int ELL_UNROLL = 8;
+template <class CT, class VT>
+__device__ __forceinline__ void ell_accum(const DLevel& L, unsigned I, const void* const (&src)[CV_GRP], int count,
+                                          float (&acc)[CV_GRP]) {
+    const int K = L.ellk;
+    for (int p0 = 0; p0 < K; p0 += ELL_UNROLL) {
+        int c[ELL_UNROLL];
+        CT v[ELL_UNROLL];
+#pragma unroll
+        for (int q = 0; q < ELL_UNROLL; ++q) {
+            const unsigned o = (unsigned)min(p0 + q, K - 1) * (unsigned)L.n + I;
+            c[q] = gld(L.col, o);
+            v[q] = gld(static_cast<const CT*>(L.val), o);
+        }
+        VT x[CV_GRP][ELL_UNROLL];
+#pragma unroll
+        for (int g = 0; g < CV_GRP; ++g)
+            if (g < count)
+#pragma unroll
+                for (int q = 0; q < ELL_UNROLL; ++q) x[g][q] = gld(static_cast<const VT*>(src[g]), (unsigned)c[q]);
+#pragma unroll
+        for (int q = 0; q < ELL_UNROLL; ++q) {
+            if (p0 + q >= K) break;
+            const float vq = to_f(v[q]);
+#pragma unroll
+            for (int g = 0; g < CV_GRP; ++g)
+                if (g < count) acc[g] += vq * to_f(x[g][q]);
+        }
+    }
+}
+
 // gather restriction: b_{l+1}[I] = sum over children of r_l, ascending
-// (the serial sum of pressure_mg.cpp:285-286 in the same order)
+// (the serial sum of pressure_mg.cpp:285-286 in the same order).  A row has
+// at most 8 children at level 0 (a piece of a 2x2x2 block) and rarely more
+// further down; the first eight are gathered together.
 template <class VT>
 __global__ void __launch_bounds__(TB) k_vc_restrict(const DSys* __restrict__ sys, const int4* __restrict__ tab, int l) {
     const int4 e = tab[blockIdx.x];
@@ -1462,12 +1501,24 @@ __global__ void __launch_bounds__(TB) k_vc_restrict(const DSys* __restrict__ sys
         r[g] = g < G.count ? sys[G.first + g].L[l].r : nullptr;
         acc[g] = 0.0f;
     }
-    const int p0 = gld(F.cptr, I), p1 = gld(F.cptr, I + 1);
-    for (int p = p0; p < p1; ++p) {
-        const int c = gld(F.cidx, p);
+    const int p0 = gld(F.cptr, I), p1 = gld(F.cptr, I + 1);   // p1 > p0: an aggregate has a child
+    for (int pb = p0; pb < p1; pb += 8) {
+        int c[8];
+#pragma unroll
+        for (int q = 0; q < 8; ++q) c[q] = gld(F.cidx, (unsigned)min(pb + q, p1 - 1));
+        VT x[CV_GRP][8];
 #pragma unroll
         for (int g = 0; g < CV_GRP; ++g)
-            if (g < G.count) acc[g] += ld<VT>(r[g], c);
+            if (g < G.count)
+#pragma unroll
+                for (int q = 0; q < 8; ++q) x[g][q] = gld(static_cast<const VT*>(r[g]), (unsigned)c[q]);
+#pragma unroll
+        for (int q = 0; q < 8; ++q) {
+            if (pb + q >= p1) break;
+#pragma unroll
+            for (int g = 0; g < CV_GRP; ++g)
+                if (g < G.count) acc[g] += to_f(x[g][q]);
+        }
     }
 #pragma unroll
     for (int g = 0; g < CV_GRP; ++g)
@@ -1517,16 +1568,7 @@ __global__ void __launch_bounds__(TB) k_vc_jac(const DSys* __restrict__ sys, con
         acc[g] = g < G.count ? ld<VT>(Lg.b, I) : 0.0f;
     }
     const float dg = ld<CT>(L.diag, I);
-    if (dg > 0) {
-        for (int p = 0; p < L.ellk; ++p) {
-            const unsigned o = (unsigned)p * (unsigned)L.n + (unsigned)I;
-            const int c = gld(L.col, o);
-            const float v = ld<CT>(L.val, o);
-#pragma unroll
-            for (int g = 0; g < CV_GRP; ++g)
-                if (g < G.count) acc[g] += v * ld<VT>(src[g], c);
-        }
-    }
+    if (dg > 0) ell_accum<CT, VT>(L, (unsigned)I, src, G.count, acc);
 #pragma unroll
     for (int g = 0; g < CV_GRP; ++g) {
         if (!((G.active >> g) & 1u)) continue;
@@ -1557,16 +1599,7 @@ __global__ void __launch_bounds__(TB) k_vc_resid(const DSys* __restrict__ sys, c
         t[g] = Lg.t;
         r[g] = g < G.count ? ld<VT>(Lg.b, I) - dg * ld<VT>(Lg.t, I) : 0.0f;
     }
-    if (dg > 0) {
-        for (int p = 0; p < L.ellk; ++p) {
-            const unsigned o = (unsigned)p * (unsigned)L.n + (unsigned)I;
-            const int c = gld(L.col, o);
-            const float v = ld<CT>(L.val, o);
-#pragma unroll
-            for (int g = 0; g < CV_GRP; ++g)
-                if (g < G.count) r[g] += v * ld<VT>(t[g], c);
-        }
-    }
+    if (dg > 0) ell_accum<CT, VT>(L, (unsigned)I, t, G.count, r);
 #pragma unroll
     for (int g = 0; g < CV_GRP; ++g)
         if ((G.active >> g) & 1u) st<VT>(sys[G.first + g].L[l].r, I, dg > 0 ? r[g] : 0.0f);
