@@ -191,6 +191,10 @@ int phg_copy(phg_ctx *ctx, void *dst, const void *src, size_t bytes);
 
 /* ---- profiling: HIP-event timing of each kernel class on the context stream */
 int phg_profile_enable(phg_ctx *ctx, int on);
+/* time only the kernel classes whose bit (their index below) is set; all by
+ * default.  Each timed launch is bracketed by two events on the stream, so a
+ * run that times only the classes it reports stays close to an untimed one. */
+int phg_profile_select(phg_ctx *ctx, unsigned long long class_mask);
 /* kernel class names are returned by index; returns PHG_EINVAL past the last */
 int phg_profile_get(phg_ctx *ctx, int index, const char **name, double *total_ms, long long *launches,
                     double *bytes);

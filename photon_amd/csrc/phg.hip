@@ -131,6 +131,7 @@ struct phg_ctx {
     std::map<int, GridDev*> grids;
     std::map<int, BatchDev*> batches;
     bool prof = false;
+    unsigned long long prof_mask = ~0ull;   // kernel classes timed while profiling
     struct Pending {
         int cls;
         hipEvent_t a, b;
@@ -188,13 +189,14 @@ template <typename... KArgs, typename... Args>
 void launch_nt(phg_ctx* ctx, int cls, void (*kern)(KArgs...), int blocks, int threads, size_t smem, Args... args) {
     if (blocks <= 0) return;
     hipEvent_t a = nullptr, b = nullptr;
-    if (ctx->prof) {
+    const bool timed = ctx->prof && ((ctx->prof_mask >> cls) & 1ull);
+    if (timed) {
         a = take_event(ctx);
         b = take_event(ctx);
         (void)hipEventRecord(a, ctx->stream);
     }
     hipLaunchKernelGGL(kern, dim3(blocks), dim3(threads), smem, ctx->stream, args...);
-    if (ctx->prof) {
+    if (timed) {
         (void)hipEventRecord(b, ctx->stream);
         ctx->pending.push_back({cls, a, b});
     }
@@ -1294,6 +1296,12 @@ int phg_profile_enable(phg_ctx* ctx, int on) {
     HIP_OK(hipStreamSynchronize(ctx->stream));
     prof_drain(ctx);
     ctx->prof = on != 0;
+    return PHG_OK;
+}
+
+int phg_profile_select(phg_ctx* ctx, unsigned long long class_mask) {
+    if (!ctx) return PHG_EINVAL;
+    ctx->prof_mask = class_mask;
     return PHG_OK;
 }
 

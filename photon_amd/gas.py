@@ -27,7 +27,8 @@ EXPORTS = (
     "phg_memcpy_h2d", "phg_memcpy_d2h", "phg_memset", "phg_grid_create", "phg_grid_destroy", "phg_grid_info",
     "phg_grid_zones", "phg_apply", "phg_precondition", "phg_batch_create", "phg_batch_destroy",
     "phg_batch_solve", "phg_solve", "phg_project", "phg_rhs", "phg_zone_mean", "phg_correct",
-    "phg_energy_expansion", "phg_advect_scalars", "phg_profile_enable", "phg_profile_get", "phg_profile_reset",
+    "phg_energy_expansion", "phg_advect_scalars", "phg_profile_enable", "phg_profile_select", "phg_profile_get",
+    "phg_profile_reset",
     "phg_hierarchy_shape", "phg_batch_create_ex", "phg_copy", "phg_leakage",
 )
 
@@ -98,6 +99,7 @@ def load_library():
     L.phg_energy_expansion.argtypes = [vp, i32, f32, vp, vp, vp, vp, vp, C.POINTER(f32)]
     L.phg_advect_scalars.argtypes = [vp, i32, f32, vp, vp, vp, vp, vp, vp, vp]
     L.phg_profile_enable.argtypes = [vp, i32]
+    L.phg_profile_select.argtypes = [vp, C.c_ulonglong]
     L.phg_profile_get.argtypes = [vp, i32, C.POINTER(C.c_char_p), C.POINTER(f64), C.POINTER(C.c_longlong), C.POINTER(f64)]
     L.phg_profile_reset.argtypes = [vp]
     L.phg_hierarchy_shape.argtypes = [i32, i32, i32] + [vp] * 5 + [C.POINTER(i32), vp, vp]
@@ -186,8 +188,26 @@ class Context:
         self._check(self.L.phg_copy(self.h, dst.ptr, src.ptr, nbytes))
 
     # profiling
-    def profile(self, on=True):
+    def profile(self, on=True, classes=None):
+        """Time kernel launches by class with HIP events; `classes` (names)
+        limits the timing to those classes (None: every class)."""
+        mask = (1 << 64) - 1
+        if classes is not None:
+            names = self.profile_names()
+            mask = 0
+            for c in classes:
+                if c in names:
+                    mask |= 1 << names.index(c)
+        self._check(self.L.phg_profile_select(self.h, mask))
         self._check(self.L.phg_profile_enable(self.h, 1 if on else 0))
+
+    def profile_names(self):
+        out = []
+        while True:
+            name = C.c_char_p()
+            if self.L.phg_profile_get(self.h, len(out), C.byref(name), None, None, None) != 0:
+                return out
+            out.append(name.value.decode())
 
     def profile_reset(self):
         self._check(self.L.phg_profile_reset(self.h))
