@@ -9,16 +9,22 @@
 // Level 0 (the structured grid, where nearly all the bytes are) is swept by
 // columns: a thread owns one (i, j) column and walks the planes of its block's
 // k-chunk, so consecutive lanes read consecutive cells of a plane and the
-// index arithmetic is done once per column.  One CG iteration is five
-// level-0 passes (reference: the CG loop src/sim/gasphase.cpp:603-627 and the
-// V-cycle src/sim/pressure_mg.cpp:270-300):
+// index arithmetic is done once per column.  One CG iteration is three
+// level-0 passes on grids with 0/1 coefficients (reference: the CG loop
+// src/sim/gasphase.cpp:603-627 and the V-cycle src/sim/pressure_mg.cpp:270-300):
 //   k_dir_apply   d = z + beta d_old, q = A d, fused d.q, and the deferred
 //                 x += alpha_prev d_old of the previous iteration
+//   k_pre_fused   r -= alpha q with |r|^2 and max|r|, the red and black
+//                 pre-smoothing half sweeps from zero, and the level-0
+//                 residual into the V-cycle scratch
+//   ... restriction, coarse levels, coarsest solve, prolongation ...
+//   k_post_fused  prolongation, the black and red post-smoothing half sweeps,
+//                 and the r.z (and q.z) partials
+// Other grids run the half sweeps as separate passes with the same results:
 //   k_pre_red     r -= alpha q, fused |r|^2 and max|r|; first pre-smoothing
 //                 half sweep from zero (red: r/diag, black: 0)
 //   k_rb_black    black half sweep                         (gsColor(1))
 //   k_resid0      level-0 residual into the V-cycle scratch (residualGrid)
-//   ... restriction, coarse levels, coarsest solve, prolongation ...
 //   k_post_black  prolongation fused into the black half sweep: a GS update
 //                 replaces the cell's own value, so only the red neighbours'
 //                 prolonged values are needed, and they are formed on the fly
