@@ -945,17 +945,33 @@ __global__ void __launch_bounds__(TB) k_post_red(const DSys* __restrict__ sys, c
 // so a lane does one red and one black cell and no lane idles through the
 // other colour's work: one division per pair when staging, one stencil per
 // pair when smoothing.
-// LDS rings of four planes: Rb [W2] floats (after a two-float pad), Zb
-// [W1 + 2] floats, Ob [TB] floats, Mb [W2] bytes.
+// LDS rings of three planes: staged values (after a two-float pad), smoothed
+// values, the own columns' new residual, face masks.  Three slots suffice: a
+// plane's staged values are last read by the smoothing of the previous step
+// and its smoothed values by the previous emit, both behind a barrier of the
+// step that overwrites them; the own-column values are read before the
+// step's second barrier for the same reason.
 constexpr int FUSE_MAX_NX = 192;   // staging pairs NT/2 + 2 nx <= 2 NT
 
 // A fused block has NT = TB or 2 TB threads and owns NT columns: the column
 // tile of its table entry and, for 2 TB, the next one (the table then lists
 // the even column tiles only).  A wider block stages fewer halo rows per own
 // column (TB + 4 nx against 2 TB + 4 nx for twice the columns).
-__host__ __device__ constexpr size_t fuse_lds(int nx, int nt) {
-    return (size_t)(2 + 4 * (nt + 4 * nx)) * 4 + (size_t)4 * (nt + 2 * nx + 2) * 4 + (size_t)4 * nt * 4 +
-           (size_t)4 * (nt + 4 * nx);
+// The passes keep rings of three planes in LDS.  Their strides are those of
+// the configuration's widest grid, so every access is a per-thread base plus a
+// constant, and the ring slot of each step is known at compile time (the march
+// is unrolled by three).
+template <int V>
+using ic = std::integral_constant<int, V>;
+__host__ __device__ constexpr int fuse_nxmax(int nt, int p2, int p1) {   // widest nx of a configuration
+    return nt == 2 * TB ? (p2 == 1 ? 128 : FUSE_MAX_NX) : (p2 == 1 ? 64 : (p1 == 1 ? 127 : FUSE_MAX_NX));
+}
+__host__ __device__ constexpr int fuse_sr(int nt, int nxm) { return nt + 4 * nxm; }       // staged values, masks
+__host__ __device__ constexpr int fuse_sz(int nt, int nxm) { return nt + 2 * nxm + 2; }   // smoothed values
+__host__ __device__ constexpr size_t fuse_lds(int nt, int p2, int p1, bool post) {
+    const int m = fuse_nxmax(nt, p2, p1), sr = fuse_sr(nt, m), sz = fuse_sz(nt, m);
+    return (size_t)(2 + 3 * sr) * 4 + (size_t)3 * sz * 4 + (size_t)3 * nt * 4 + (size_t)3 * sr +
+           (post ? (size_t)3 * nt * 4 : 0);
 }
 
 // P2, P1: staging and smoothing pairs per thread, for NT/2 + 2 nx and at
@@ -966,7 +982,7 @@ __global__ void __launch_bounds__(NT) k_pre_fused(const DSys* __restrict__ sys, 
     const DLevel& L = s.L[0];
     const ColTile t = col_tile(s, e.y);
     const int nx = L.nx, ny = L.ny, nz = L.nz, sxy = nx * ny;
-    const int W1 = NT + 2 * nx, W2 = NT + 4 * nx, WZ = W1 + 2;
+    const int W1 = NT + 2 * nx, W2 = NT + 4 * nx;
     const int tid = (int)threadIdx.x;
     const int c0 = t.c2 - tid;     // a multiple of TB, so even (NT = 2 TB: of 2 TB)
     const int zo = nx & ~1;        // Zb holds W2 index h at h - zo, so pairs stay 8-byte aligned
@@ -975,12 +991,12 @@ __global__ void __launch_bounds__(NT) k_pre_fused(const DSys* __restrict__ sys, 
     const int it = st_uni(s, ST_ITERS);
     const float* __restrict__ rin = rbuf(s, it);   // r_it: read here, also in the halo rows of other blocks
     float* __restrict__ rout = rbuf(s, it + 1);    // r_{it+1}: written for the own columns only
+    constexpr int NXM = fuse_nxmax(NT, P2, P1), SR = fuse_sr(NT, NXM), SZ = fuse_sz(NT, NXM);
     extern __shared__ float lds[];
-    float* Rb = lds + 2;                 // [4][W2]; Rb[-1] is read (and ignored) by an edge pair of an odd nx
-    float* Zb = Rb + 4 * W2;             // [4][WZ]
-    float* Ob = Zb + 4 * WZ;             // [4][NT]: rn of the own columns
-    uint8_t* Mb = reinterpret_cast<uint8_t*>(Ob + 4 * NT);   // [4][W2]
-    auto slot = [](int p) { return (p + 4) & 3; };           // planes >= -4
+    float* Rb = lds + 2;                 // [3][SR]; Rb[-1] is read (and ignored) by an edge pair of an odd nx
+    float* Zb = Rb + 3 * SR;             // [3][SZ]
+    float* Ob = Zb + 3 * SZ;             // [3][NT]: rn of the own columns
+    uint8_t* Mb = reinterpret_cast<uint8_t*>(Ob + 3 * NT);   // [3][SR]
     auto ij_bits = [&](int c) {                              // the (i, j) part of edge_bits
         const int j = c / nx, i = c - j * nx;
         return (i > 0 ? 1u : 0u) | (i < nx - 1 ? 2u : 0u) | (j > 0 ? 4u : 0u) | (j < ny - 1 ? 8u : 0u);
@@ -1043,11 +1059,12 @@ __global__ void __launch_bounds__(NT) k_pre_fused(const DSys* __restrict__ sys, 
     };
     // stage plane p from its raw loads: red cells rn zs / diag (0 without
     // open faces), black cells rn zs, out-of-grid cells 0
-    auto stage = [&](int p, const Raw& X) {
+    auto stage = [&](int p, const Raw& X, auto S) {   // S: the ring slot of plane p
+        constexpr int sp = decltype(S)::value;
         const bool pin = p >= 0 && p < nz;
-        float* R = Rb + slot(p) * W2;
-        uint8_t* M = Mb + slot(p) * W2;
-        float* O = Ob + slot(p) * NT;
+        float* R = Rb + sp * SR;
+        uint8_t* M = Mb + sp * SR;
+        float* O = Ob + sp * NT;
 #pragma unroll
         for (int m = 0; m < P2; ++m) {
             const int u = tid + m * NT;
@@ -1072,14 +1089,15 @@ __global__ void __launch_bounds__(NT) k_pre_fused(const DSys* __restrict__ sys, 
     };
     // smoothed values of plane p over W1: red cells keep their value, black
     // cells take the Gauss-Seidel update from their red neighbours
-    auto smooth = [&](int p) {
+    auto smooth = [&](int p, auto S) {   // S: the ring slot of plane p
+        constexpr int sp = decltype(S)::value;
         const bool pin = p >= 0 && p < nz;
         const unsigned kb = (p > 0 ? 16u : 0u) | (p < nz - 1 ? 32u : 0u);
-        const float* R = Rb + slot(p) * W2;
-        const float* Rm = Rb + slot(p - 1) * W2;
-        const float* Rp = Rb + slot(p + 1) * W2;
-        const uint8_t* M = Mb + slot(p) * W2;
-        float* Z = Zb + slot(p) * WZ - zo;
+        const float* R = Rb + sp * SR;
+        const float* Rm = Rb + ((sp + 2) % 3) * SR;
+        const float* Rp = Rb + ((sp + 1) % 3) * SR;
+        const uint8_t* M = Mb + sp * SR;
+        float* Z = Zb + sp * SZ - zo;
 #pragma unroll
         for (int m = 0; m < P1; ++m) {
             if (tid + m * NT >= NP1) break;
@@ -1098,20 +1116,31 @@ __global__ void __launch_bounds__(NT) k_pre_fused(const DSys* __restrict__ sys, 
     };
     double acc = 0.0;
     float mx = 0.0f;
-    // emit plane k of the own column
+    // emit plane k of the own column: its rn and mask are read before the
+    // step's second barrier (the next step's staging reuses their slot)
     const int hz = 2 * nx + tid - zo;   // the own column in Zb
-    auto emit = [&](int k) {
+    struct Own {
+        float rn;
+        unsigned m;
+    };
+    auto own = [&](auto S) {
+        constexpr int sp = decltype(S)::value;
+        return Own{Ob[sp * NT + tid], (unsigned)Mb[sp * SR + 2 * nx + tid]};
+    };
+    auto emit = [&](int k, auto S, const Own& o) {
+        constexpr int sp = decltype(S)::value;
         if (!t.valid) return;
         const int c = t.c2 + sxy * k;
-        const float rn = Ob[slot(k) * NT + tid];
-        const float* Z = Zb + slot(k) * WZ + hz;
+        const float rn = o.rn;
+        const float* Z = Zb + sp * SZ + hz;
         const float zc = *Z;
-        const CoefRegs<Mask01> cf{(unsigned)Mb[slot(k) * W2 + 2 * nx + tid]};
+        const CoefRegs<Mask01> cf{o.m};
         const float dg = cell_diag(cf);
         float r0 = 0.0f;
         if (dg > 0)
             r0 = rn * zs - dg * zc +
-                 cell_lat(cf, Cell{c, t.i, t.j, k}, nx, ny, nz, Z, Zb[slot(k - 1) * WZ + hz], Zb[slot(k + 1) * WZ + hz]);
+                 cell_lat(cf, Cell{c, t.i, t.j, k}, nx, ny, nz, Z, Zb[((sp + 2) % 3) * SZ + hz],
+                          Zb[((sp + 1) % 3) * SZ + hz]);
         gst_nt(s.z2, c, zc);   // read by the fused post-smoothing
         st_nt<VT>(L.r, c, r0);
         if (!FIRST) {
@@ -1123,21 +1152,24 @@ __global__ void __launch_bounds__(NT) k_pre_fused(const DSys* __restrict__ sys, 
     // one plane of the march: stage k+2, smooth k+1, emit k (for k >= k0).
     // X holds the raw loads of plane k+2 and is refilled with plane k+5.
     const int ks = t.k0 - 2;
-    auto step = [&](int k, Raw& X) {
-        stage(k + 2, X);
+    // J: the ring slot of plane k, (k - ks) mod 3
+    auto step = [&](int k, Raw& X, auto J) {
+        constexpr int j = decltype(J)::value;
+        stage(k + 2, X, ic<(j + 2) % 3>());
         load(min(k + 5, t.k1 + 1), X);   // past the chunk: repeats the last plane needed
         __syncthreads();
-        smooth(k + 1);
+        smooth(k + 1, ic<(j + 1) % 3>());
+        const Own o = own(J);
         __syncthreads();
-        if (k >= t.k0) emit(k);
+        if (k >= t.k0) emit(k, J, o);
     };
     Raw X0, X1, X2;
     {
         Raw Y;
         load(ks, Y);
-        stage(ks, Y);
+        stage(ks, Y, ic<0>());
         load(ks + 1, Y);
-        stage(ks + 1, Y);
+        stage(ks + 1, Y, ic<1>());
     }
     load(ks + 2, X0);
     load(ks + 3, X1);
@@ -1147,12 +1179,12 @@ __global__ void __launch_bounds__(NT) k_pre_fused(const DSys* __restrict__ sys, 
     // registers around the loop and their loads are never waited on early
     int k = ks;
     for (; k + 2 < t.k1; k += 3) {
-        step(k, X0);
-        step(k + 1, X1);
-        step(k + 2, X2);
+        step(k, X0, ic<0>());
+        step(k + 1, X1, ic<1>());
+        step(k + 2, X2, ic<2>());
     }
-    if (k < t.k1) step(k, X0);
-    if (k + 1 < t.k1) step(k + 1, X1);
+    if (k < t.k1) step(k, X0, ic<0>());
+    if (k + 1 < t.k1) step(k + 1, X1, ic<1>());
     if (FIRST) return;
     __shared__ double sh[NT / 64];
     __shared__ float shm[NT / 64];
@@ -1196,7 +1228,6 @@ __global__ void __launch_bounds__(NT) k_pre_fused(const DSys* __restrict__ sys, 
 // kernels, and each thread sums its column's terms in the same order, so the
 // results are bitwise the same; the pass reads z and r once and writes z once
 // (12 B/cell, 16 with q, in place of about 27).
-__host__ __device__ constexpr size_t post_fuse_lds(int nx, int nt) { return fuse_lds(nx, nt) + (size_t)4 * nt * 4; }
 
 template <class VT, int NT, int P2, int P1>
 __global__ void __launch_bounds__(NT) k_post_fused(const DSys* __restrict__ sys, const int2* __restrict__ tab) {
@@ -1204,7 +1235,7 @@ __global__ void __launch_bounds__(NT) k_post_fused(const DSys* __restrict__ sys,
     const DLevel& L = s.L[0];
     const ColTile t = col_tile(s, e.y);
     const int nx = L.nx, ny = L.ny, nz = L.nz, sxy = nx * ny;
-    const int W1 = NT + 2 * nx, W2 = NT + 4 * nx, WZ = W1 + 2;
+    const int W1 = NT + 2 * nx, W2 = NT + 4 * nx;
     const int tid = (int)threadIdx.x;
     const int c0 = t.c2 - tid;     // a multiple of TB, so even (NT = 2 TB: of 2 TB)
     const int zo = nx & ~1;        // Zb holds W2 index h at h - zo, so pairs stay 8-byte aligned
@@ -1213,13 +1244,13 @@ __global__ void __launch_bounds__(NT) k_post_fused(const DSys* __restrict__ sys,
     const float* __restrict__ rv = r_vc(s);
     const float* __restrict__ z = s.z2;   // the pre-smoothed iterate, never written here
     const VT* __restrict__ t1 = reinterpret_cast<const VT*>(s.L[1].t);
+    constexpr int NXM = fuse_nxmax(NT, P2, P1), SR = fuse_sr(NT, NXM), SZ = fuse_sz(NT, NXM);
     extern __shared__ float lds[];
-    float* Rb = lds + 2;                 // [4][W2]; Rb[-1] is read (and ignored) by an edge pair of an odd nx
-    float* Zb = Rb + 4 * W2;             // [4][WZ]
-    float* Ob = Zb + 4 * WZ;             // [4][NT]: r of the own columns
-    uint8_t* Mb = reinterpret_cast<uint8_t*>(Ob + 4 * NT);   // [4][W2]
-    float* Qb = reinterpret_cast<float*>(Mb + 4 * W2);       // [4][NT]: q of the own columns
-    auto slot = [](int p) { return (p + 4) & 3; };           // planes >= -4
+    float* Rb = lds + 2;                 // [3][SR]; Rb[-1] is read (and ignored) by an edge pair of an odd nx
+    float* Zb = Rb + 3 * SR;             // [3][SZ]
+    float* Ob = Zb + 3 * SZ;             // [3][NT]: r of the own columns
+    uint8_t* Mb = reinterpret_cast<uint8_t*>(Ob + 3 * NT);   // [3][SR]
+    float* Qb = reinterpret_cast<float*>(Mb + 3 * SR);       // [3][NT]: q of the own columns
     auto ij_bits = [&](int c) {                              // the (i, j) part of edge_bits
         const int j = c / nx, i = c - j * nx;
         return (i > 0 ? 1u : 0u) | (i < nx - 1 ? 2u : 0u) | (j > 0 ? 4u : 0u) | (j < ny - 1 ? 8u : 0u);
@@ -1289,11 +1320,12 @@ __global__ void __launch_bounds__(NT) k_post_fused(const DSys* __restrict__ sys,
         }
     };
     // stage plane p: red cells z + t1[agg] (the prolongation), black cells r zs
-    auto stage = [&](int p, const Raw& X) {
+    auto stage = [&](int p, const Raw& X, auto S) {   // S: the ring slot of plane p
+        constexpr int sp = decltype(S)::value;
         const bool pin = p >= 0 && p < nz;
-        float* R = Rb + slot(p) * W2;
-        uint8_t* M = Mb + slot(p) * W2;
-        float* O = Ob + slot(p) * NT;
+        float* R = Rb + sp * SR;
+        uint8_t* M = Mb + sp * SR;
+        float* O = Ob + sp * NT;
 #pragma unroll
         for (int m = 0; m < P2; ++m) {
             const int u = tid + m * NT;
@@ -1308,7 +1340,7 @@ __global__ void __launch_bounds__(NT) k_post_fused(const DSys* __restrict__ sys,
             const int o = 2 * u - 2 * nx;
             if (o >= 0 && o < NT) *reinterpret_cast<float2*>(O + o) = float2{X.r[m].x, X.r[m].y};
         }
-        Qb[slot(p) * NT + tid] = X.q;
+        Qb[sp * NT + tid] = X.q;
     };
     // black cells of plane p over W1 take the Gauss-Seidel update from the
     // prolonged values of their red neighbours.  The red cells' slots are
@@ -1316,14 +1348,15 @@ __global__ void __launch_bounds__(NT) k_post_fused(const DSys* __restrict__ sys,
     // missing neighbour by 0, and for an even nx the slot past a row's end
     // belongs to a red cell of the next row; a stale value there could be a
     // NaN, which a weight of 0 does not remove.
-    auto smooth = [&](int p) {
+    auto smooth = [&](int p, auto S) {   // S: the ring slot of plane p
+        constexpr int sp = decltype(S)::value;
         const bool pin = p >= 0 && p < nz;
         const unsigned kb = (p > 0 ? 16u : 0u) | (p < nz - 1 ? 32u : 0u);
-        const float* R = Rb + slot(p) * W2;
-        const float* Rm = Rb + slot(p - 1) * W2;
-        const float* Rp = Rb + slot(p + 1) * W2;
-        const uint8_t* M = Mb + slot(p) * W2;
-        float* Z = Zb + slot(p) * WZ - zo;
+        const float* R = Rb + sp * SR;
+        const float* Rm = Rb + ((sp + 2) % 3) * SR;
+        const float* Rp = Rb + ((sp + 1) % 3) * SR;
+        const uint8_t* M = Mb + sp * SR;
+        float* Z = Zb + sp * SZ - zo;
 #pragma unroll
         for (int m = 0; m < P1; ++m) {
             if (tid + m * NT >= NP1) break;
@@ -1339,25 +1372,36 @@ __global__ void __launch_bounds__(NT) k_post_fused(const DSys* __restrict__ sys,
         }
     };
     double acc = 0.0, accq = 0.0;
+    // emit plane k of the own column: its r, mask and q are read before the
+    // step's second barrier (the next step's staging reuses their slot)
     const int hz = 2 * nx + tid - zo;   // the own column in Zb
-    auto emit = [&](int k) {
+    struct Own {
+        float r, q;
+        unsigned m;
+    };
+    auto own = [&](auto S) {
+        constexpr int sp = decltype(S)::value;
+        return Own{Ob[sp * NT + tid], Qb[sp * NT + tid], (unsigned)Mb[sp * SR + 2 * nx + tid]};
+    };
+    auto emit = [&](int k, auto S, const Own& o) {
+        constexpr int sp = decltype(S)::value;
         if (!t.valid) return;
         const int c = t.c2 + sxy * k;
-        const float r = Ob[slot(k) * NT + tid];
-        const float* Z = Zb + slot(k) * WZ + hz;
+        const float r = o.r;
+        const float* Z = Zb + sp * SZ + hz;
         float zc = *Z;   // black cells: the new value
         if (((t.i + t.j + k) & 1) == 0) {
-            const CoefRegs<Mask01> cf{(unsigned)Mb[slot(k) * W2 + 2 * nx + tid]};
+            const CoefRegs<Mask01> cf{o.m};
             const float dg = cell_diag(cf);
             zc = 0.0f;
             if (dg > 0)
                 zc = cell_div<Mask01>(r * zs + cell_lat(cf, Cell{c, t.i, t.j, k}, nx, ny, nz, Z,
-                                                        Zb[slot(k - 1) * WZ + hz], Zb[slot(k + 1) * WZ + hz]),
+                                                        Zb[((sp + 2) % 3) * SZ + hz], Zb[((sp + 1) % 3) * SZ + hz]),
                                       dg);
         }
         gst_nt(reinterpret_cast<float*>(s.z), c, zc);
         acc += (double)r * zc;
-        if (with_q) accq += (double)Qb[slot(k) * NT + tid] * zc;
+        if (with_q) accq += (double)o.q * zc;
     };
     const int ks = t.k0 - 2;
     // stage k+2 (gathered a step ago), gather for plane k+3, refill the
@@ -1365,24 +1409,27 @@ __global__ void __launch_bounds__(NT) k_post_fused(const DSys* __restrict__ sys,
     // before the plane loads: memory operations complete in order for the
     // wave's counter, so waiting for a gather at the next step then leaves
     // the plane loads in flight.
-    auto step = [&](int k, Raw& X, Raw& Xn) {
-        stage(k + 2, X);
+    // J: the ring slot of plane k, (k - ks) mod 3
+    auto step = [&](int k, Raw& X, Raw& Xn, auto J) {
+        constexpr int j = decltype(J)::value;
+        stage(k + 2, X, ic<(j + 2) % 3>());
         gather(min(k + 3, t.k1 + 1), Xn);
         load(min(k + 5, t.k1 + 1), X);   // past the chunk: repeats the last plane needed
         __syncthreads();
-        smooth(k + 1);
+        smooth(k + 1, ic<(j + 1) % 3>());
+        const Own o = own(J);
         __syncthreads();
-        if (k >= t.k0) emit(k);
+        if (k >= t.k0) emit(k, J, o);
     };
     Raw X0, X1, X2;
     {
         Raw Y;
         load(ks, Y);
         gather(ks, Y);
-        stage(ks, Y);
+        stage(ks, Y, ic<0>());
         load(ks + 1, Y);
         gather(ks + 1, Y);
-        stage(ks + 1, Y);
+        stage(ks + 1, Y, ic<1>());
     }
     load(ks + 2, X0);
     load(ks + 3, X1);
@@ -1390,12 +1437,12 @@ __global__ void __launch_bounds__(NT) k_post_fused(const DSys* __restrict__ sys,
     gather(ks + 2, X0);
     int k = ks;
     for (; k + 2 < t.k1; k += 3) {
-        step(k, X0, X1);
-        step(k + 1, X1, X2);
-        step(k + 2, X2, X0);
+        step(k, X0, X1, ic<0>());
+        step(k + 1, X1, X2, ic<1>());
+        step(k + 2, X2, X0, ic<2>());
     }
-    if (k < t.k1) step(k, X0, X1);
-    if (k + 1 < t.k1) step(k + 1, X1, X2);
+    if (k < t.k1) step(k, X0, X1, ic<0>());
+    if (k + 1 < t.k1) step(k + 1, X1, X2, ic<1>());
     __shared__ double sh[2][NT / 64];
     const int lane = tid & 63, w = tid >> 6;
     acc = wave_sum(acc);

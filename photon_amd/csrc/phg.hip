@@ -309,24 +309,26 @@ void vcycle_t(phg_ctx* ctx, BatchDev* B, int first) {
         // for the widest grid: staging NT/2 + 2 nx, smoothing up to
         // NT/2 + nx + 1; 2 TB-thread blocks take the even column tiles.
         static_assert(TB == 256, "pair counts below assume 256 threads");
+        // the LDS rings are sized for each configuration's widest grid
+        auto go = [&](auto k1, auto k0, int nt, int p2, int p1) {
+            const size_t lds = fuse_lds(nt, p2, p1, false);
+            if (nt == 2 * TB)
+                launch_nt(ctx, K_PRE_SMOOTH, first ? k1 : k0, B->tabw_len, nt, lds, S, (const int2*)B->tabw);
+            else
+                launch(ctx, K_PRE_SMOOTH, first ? k1 : k0, n0, lds, S, T0);
+        };
         if (B->fuse_nt == 2 * TB) {
-            const size_t lds = fuse_lds(B->maxnx, 2 * TB);
-            auto go = [&](auto k1, auto k0) {
-                launch_nt(ctx, K_PRE_SMOOTH, first ? k1 : k0, B->tabw_len, 2 * TB, lds, S, (const int2*)B->tabw);
-            };
             if (B->maxnx <= 128)
-                go(k_pre_fused<VT, true, 2 * TB, 1, 1>, k_pre_fused<VT, false, 2 * TB, 1, 1>);
+                go(k_pre_fused<VT, true, 2 * TB, 1, 1>, k_pre_fused<VT, false, 2 * TB, 1, 1>, 2 * TB, 1, 1);
             else
-                go(k_pre_fused<VT, true, 2 * TB, 2, 1>, k_pre_fused<VT, false, 2 * TB, 2, 1>);
+                go(k_pre_fused<VT, true, 2 * TB, 2, 1>, k_pre_fused<VT, false, 2 * TB, 2, 1>, 2 * TB, 2, 1);
         } else {
-            const size_t lds = fuse_lds(B->maxnx, TB);
-            auto go = [&](auto k1, auto k0) { launch(ctx, K_PRE_SMOOTH, first ? k1 : k0, n0, lds, S, T0); };
             if (B->maxnx <= 64)
-                go(k_pre_fused<VT, true, TB, 1, 1>, k_pre_fused<VT, false, TB, 1, 1>);
+                go(k_pre_fused<VT, true, TB, 1, 1>, k_pre_fused<VT, false, TB, 1, 1>, TB, 1, 1);
             else if (B->maxnx <= 127)
-                go(k_pre_fused<VT, true, TB, 2, 1>, k_pre_fused<VT, false, TB, 2, 1>);
+                go(k_pre_fused<VT, true, TB, 2, 1>, k_pre_fused<VT, false, TB, 2, 1>, TB, 2, 1);
             else
-                go(k_pre_fused<VT, true, TB, 2, 2>, k_pre_fused<VT, false, TB, 2, 2>);
+                go(k_pre_fused<VT, true, TB, 2, 2>, k_pre_fused<VT, false, TB, 2, 2>, TB, 2, 2);
         }
     } else {
         // red from zero, black, then (red, black) for each further sweep of
@@ -362,21 +364,25 @@ void vcycle_t(phg_ctx* ctx, BatchDev* B, int first) {
     // post-smoothing mirrors the pre-smoothing (black, red per sweep), so M
     // stays symmetric; the last red sweep of each system leaves its r.z partials
     if (fused) {
+        auto go = [&](auto kern, int nt, int p2, int p1) {
+            const size_t lds = fuse_lds(nt, p2, p1, true);
+            if (nt == 2 * TB)
+                launch_nt(ctx, K_POST_SMOOTH, kern, B->tabw_len, nt, lds, S, (const int2*)B->tabw);
+            else
+                launch(ctx, K_POST_SMOOTH, kern, n0, lds, S, T0);
+        };
         if (B->fuse_nt == 2 * TB) {
-            const size_t lds = post_fuse_lds(B->maxnx, 2 * TB);
-            const int2* TW = B->tabw;
             if (B->maxnx <= 128)
-                launch_nt(ctx, K_POST_SMOOTH, k_post_fused<VT, 2 * TB, 1, 1>, B->tabw_len, 2 * TB, lds, S, TW);
+                go(k_post_fused<VT, 2 * TB, 1, 1>, 2 * TB, 1, 1);
             else
-                launch_nt(ctx, K_POST_SMOOTH, k_post_fused<VT, 2 * TB, 2, 1>, B->tabw_len, 2 * TB, lds, S, TW);
+                go(k_post_fused<VT, 2 * TB, 2, 1>, 2 * TB, 2, 1);
         } else {
-            const size_t lds = post_fuse_lds(B->maxnx, TB);
             if (B->maxnx <= 64)
-                launch(ctx, K_POST_SMOOTH, k_post_fused<VT, TB, 1, 1>, n0, lds, S, T0);
+                go(k_post_fused<VT, TB, 1, 1>, TB, 1, 1);
             else if (B->maxnx <= 127)
-                launch(ctx, K_POST_SMOOTH, k_post_fused<VT, TB, 2, 1>, n0, lds, S, T0);
+                go(k_post_fused<VT, TB, 2, 1>, TB, 2, 1);
             else
-                launch(ctx, K_POST_SMOOTH, k_post_fused<VT, TB, 2, 2>, n0, lds, S, T0);
+                go(k_post_fused<VT, TB, 2, 2>, TB, 2, 2);
         }
         return;
     }

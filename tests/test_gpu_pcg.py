@@ -128,15 +128,16 @@ ODD_NX = {"odd37": (11, 37, 22, 10), "odd101": (12, 101, 14, 8), "odd131": (13, 
 
 
 @pytest.mark.parametrize("prec", [gas.PREC_FP32, gas.PREC_MIXED_F16])
-@pytest.mark.parametrize("grid,kchunk", [("small", 3), ("small", 0), ("house", 7), ("house", 40), ("odd37", 3),
-                                         ("odd101", 0), ("odd131", 4)])
-def test_fused_presmoothing_is_bitwise_the_separate_passes(ctx, small, house, prec, grid, kchunk):
-    """The fused pre-smoothing pass forms every value with the same operations
-    as the three separate passes, so whole solves agree bit for bit, for
+@pytest.mark.parametrize("grid,kchunk,width", [("small", 3, 0), ("small", 0, 256), ("house", 7, 0), ("house", 40, 0),
+                                               ("house", 40, 256), ("odd37", 3, 0), ("odd101", 0, 256),
+                                               ("odd131", 4, 0), ("odd131", 4, 256)])
+def test_fused_presmoothing_is_bitwise_the_separate_passes(ctx, small, house, prec, grid, kchunk, width):
+    """The fused smoothing passes form every value with the same operations
+    as the separate half sweeps, so whole solves agree bit for bit, for
     k-chunks that cut the grid unevenly and for whole columns, on grids of
-    many column tiles (its halo rows belong to other blocks), and on grids of
+    many column tiles (their halo rows belong to other blocks), on grids of
     odd nx, where a column pair can hold the last cell of one row and the
-    first of the next."""
+    first of the next, and for blocks of 512 (the default) and 256 threads."""
     if grid in ODD_NX:
         g = sc.house(*ODD_NX[grid])
         assert g.nx % 2 == 1 and (g.nx * g.ny) % 2 == 0
@@ -147,7 +148,7 @@ def test_fused_presmoothing_is_bitwise_the_separate_passes(ctx, small, house, pr
     p = gas.solve_params(tol=1e-6, max_iters=200, zero_x=True)
     res = []
     for unfused in (False, True):
-        B = gas.Batch(ctx, [G] * len(bs), kchunk=kchunk, unfused=unfused)
+        B = gas.Batch(ctx, [G] * len(bs), kchunk=kchunk, unfused=unfused, fused_threads=width)
         dbs = [ctx.to_device(b) for b in bs]
         dxs = [ctx.array(g.n) for _ in bs]
         stats = B.solve(dbs, dxs, p)
