@@ -87,9 +87,18 @@ struct BatchDev {
     int nsys = 0, prec = 0, maxl = 0, maxnc = 0;
     bool binary = false;           // every grid has 0/1 coefficients: level-0 kernels read the face mask
     int maxnx = 0;                 // widest grid: sizes the LDS plane buffers of the level-0 kernels
-    bool even_planes = true;       // every grid has an even nx * ny (the fused pre-smoothing loads pairs)
     bool unfused = false;          // run the separate level-0 smoothing kernels (comparison runs)
-    int maxnu0 = 1;                // most level-0 sweeps of any system
+    // Level-0 smoothing per system: the fused passes for 0/1 grids of
+    // nx <= FUSE_MAX_NX and even nx * ny smoothed once, the separate half
+    // sweeps for the others, each over its own tile tables
+    int2* tabf = nullptr;          // fused systems: every column tile (TB-thread blocks)
+    int tabf_len = 0;
+    int2* tabwf = nullptr;         // fused systems: even column tiles (2 TB-thread blocks)
+    int tabwf_len = 0;
+    int2* tabu = nullptr;          // the other systems: every column tile
+    int tabu_len = 0;
+    int maxnx_f = 0;               // widest grid among the fused systems
+    int maxnu0_u = 0;              // most sweeps among the others
     size_t lds0() const { return 2 * (size_t)(TB + 2 * maxnx) * sizeof(float); }
     std::vector<int> grids;
     std::vector<DSys> h_sys;
@@ -298,13 +307,12 @@ constexpr int kMaxSweeps0 = 4;
 template <class C0, class CC, class VT>
 void vcycle_t(phg_ctx* ctx, BatchDev* B, int first) {
     const DSys* S = B->d_sys;
-    const int2* T0 = B->tab[0];
-    const int n0 = B->tab_len[0];
-    // 0/1 grids of moderate width and even plane size smooth in two fused
-    // passes (one sweep per system); others in the separate half sweeps
-    const bool fused = std::is_same<C0, Mask01>::value && B->maxnx <= FUSE_MAX_NX && B->even_planes && !B->unfused &&
-                       B->maxnu0 == 1;
-    if (fused) {
+    // 0/1 grids of moderate width and even plane size smoothed once take the
+    // two fused passes; the other systems the separate half sweeps
+    const bool anyf = std::is_same<C0, Mask01>::value && B->tabf_len > 0;
+    const int2* TU = B->tabu;
+    const int nu = B->tabu_len;
+    if (anyf) {
         // pre-smoothing and residual in one pass.  Column pairs per thread
         // for the widest grid: staging NT/2 + 2 nx, smoothing up to
         // NT/2 + nx + 1; 2 TB-thread blocks take the even column tiles.
@@ -313,33 +321,35 @@ void vcycle_t(phg_ctx* ctx, BatchDev* B, int first) {
         auto go = [&](auto k1, auto k0, int nt, int p2, int p1) {
             const size_t lds = fuse_lds(nt, p2, p1, false);
             if (nt == 2 * TB)
-                launch_nt(ctx, K_PRE_SMOOTH, first ? k1 : k0, B->tabw_len, nt, lds, S, (const int2*)B->tabw);
+                launch_nt(ctx, K_PRE_SMOOTH, first ? k1 : k0, B->tabwf_len, nt, lds, S, (const int2*)B->tabwf);
             else
-                launch(ctx, K_PRE_SMOOTH, first ? k1 : k0, n0, lds, S, T0);
+                launch(ctx, K_PRE_SMOOTH, first ? k1 : k0, B->tabf_len, lds, S, (const int2*)B->tabf);
         };
+        const int mx = B->maxnx_f;
         if (B->fuse_nt == 2 * TB) {
-            if (B->maxnx <= 128)
+            if (mx <= 128)
                 go(k_pre_fused<VT, true, 2 * TB, 1, 1>, k_pre_fused<VT, false, 2 * TB, 1, 1>, 2 * TB, 1, 1);
             else
                 go(k_pre_fused<VT, true, 2 * TB, 2, 1>, k_pre_fused<VT, false, 2 * TB, 2, 1>, 2 * TB, 2, 1);
         } else {
-            if (B->maxnx <= 64)
+            if (mx <= 64)
                 go(k_pre_fused<VT, true, TB, 1, 1>, k_pre_fused<VT, false, TB, 1, 1>, TB, 1, 1);
-            else if (B->maxnx <= 127)
+            else if (mx <= 127)
                 go(k_pre_fused<VT, true, TB, 2, 1>, k_pre_fused<VT, false, TB, 2, 1>, TB, 2, 1);
             else
                 go(k_pre_fused<VT, true, TB, 2, 2>, k_pre_fused<VT, false, TB, 2, 2>, TB, 2, 2);
         }
-    } else {
+    }
+    if (nu > 0) {
         // red from zero, black, then (red, black) for each further sweep of
         // the systems that make it, then the residual
-        launch(ctx, K_PRE_RED, k_pre_red<C0, float>, n0, 0, S, T0, first);
-        launch(ctx, K_RB_BLACK, k_rb_black<C0, float>, n0, B->lds0(), S, T0, 0);
-        for (int v = 1; v < B->maxnu0; ++v) {
-            launch(ctx, K_POST_RED, k_post_red<C0, float>, n0, B->lds0(), S, T0, v);
-            launch(ctx, K_RB_BLACK, k_rb_black<C0, float>, n0, B->lds0(), S, T0, v);
+        launch(ctx, K_PRE_RED, k_pre_red<C0, float>, nu, 0, S, TU, first);
+        launch(ctx, K_RB_BLACK, k_rb_black<C0, float>, nu, B->lds0(), S, TU, 0);
+        for (int v = 1; v < B->maxnu0_u; ++v) {
+            launch(ctx, K_POST_RED, k_post_red<C0, float>, nu, B->lds0(), S, TU, v);
+            launch(ctx, K_RB_BLACK, k_rb_black<C0, float>, nu, B->lds0(), S, TU, v);
         }
-        launch(ctx, K_RESID0, k_resid0<C0, float, VT>, n0, B->lds0(), S, T0);
+        launch(ctx, K_RESID0, k_resid0<C0, float, VT>, nu, B->lds0(), S, TU);
     }
     const std::vector<int4*>& TG = B->tabg;
     const std::vector<int>& NG = B->tabg_len;
@@ -363,34 +373,36 @@ void vcycle_t(phg_ctx* ctx, BatchDev* B, int first) {
     }
     // post-smoothing mirrors the pre-smoothing (black, red per sweep), so M
     // stays symmetric; the last red sweep of each system leaves its r.z partials
-    if (fused) {
+    if (anyf) {
         auto go = [&](auto kern, int nt, int p2, int p1) {
             const size_t lds = fuse_lds(nt, p2, p1, true);
             if (nt == 2 * TB)
-                launch_nt(ctx, K_POST_SMOOTH, kern, B->tabw_len, nt, lds, S, (const int2*)B->tabw);
+                launch_nt(ctx, K_POST_SMOOTH, kern, B->tabwf_len, nt, lds, S, (const int2*)B->tabwf);
             else
-                launch(ctx, K_POST_SMOOTH, kern, n0, lds, S, T0);
+                launch(ctx, K_POST_SMOOTH, kern, B->tabf_len, lds, S, (const int2*)B->tabf);
         };
+        const int mx = B->maxnx_f;
         if (B->fuse_nt == 2 * TB) {
-            if (B->maxnx <= 128)
+            if (mx <= 128)
                 go(k_post_fused<VT, 2 * TB, 1, 1>, 2 * TB, 1, 1);
             else
                 go(k_post_fused<VT, 2 * TB, 2, 1>, 2 * TB, 2, 1);
         } else {
-            if (B->maxnx <= 64)
+            if (mx <= 64)
                 go(k_post_fused<VT, TB, 1, 1>, TB, 1, 1);
-            else if (B->maxnx <= 127)
+            else if (mx <= 127)
                 go(k_post_fused<VT, TB, 2, 1>, TB, 2, 1);
             else
                 go(k_post_fused<VT, TB, 2, 2>, TB, 2, 2);
         }
-        return;
     }
-    launch(ctx, K_POST_BLACK, k_post_black<C0, float, VT>, n0, B->lds0(), S, T0);
-    launch(ctx, K_POST_RED, k_post_red<C0, float>, n0, B->lds0(), S, T0, 0);
-    for (int v = 1; v < B->maxnu0; ++v) {
-        launch(ctx, K_RB_BLACK, k_rb_black<C0, float>, n0, B->lds0(), S, T0, v);
-        launch(ctx, K_POST_RED, k_post_red<C0, float>, n0, B->lds0(), S, T0, v);
+    if (nu > 0) {
+        launch(ctx, K_POST_BLACK, k_post_black<C0, float, VT>, nu, B->lds0(), S, TU);
+        launch(ctx, K_POST_RED, k_post_red<C0, float>, nu, B->lds0(), S, TU, 0);
+        for (int v = 1; v < B->maxnu0_u; ++v) {
+            launch(ctx, K_RB_BLACK, k_rb_black<C0, float>, nu, B->lds0(), S, TU, v);
+            launch(ctx, K_POST_RED, k_post_red<C0, float>, nu, B->lds0(), S, TU, v);
+        }
     }
 }
 
@@ -821,7 +833,6 @@ int phg_batch_create_ex(phg_ctx* ctx, int nsys, const int* grids, const phg_batc
         B->maxnc = std::max(B->maxnc, G->H.nc);
         B->binary = s == 0 ? G->binary : (B->binary && G->binary);
         B->maxnx = std::max(B->maxnx, G->nx);
-        if ((G->nx * G->ny) & 1) B->even_planes = false;
         col_tiles += tiles(G->nx * G->ny);
     }
     if (rc) {
@@ -852,7 +863,6 @@ int phg_batch_create_ex(phg_ctx* ctx, int nsys, const int* grids, const phg_batc
                        : opts && opts->sweeps0 > 0                          ? opts->sweeps0
                                                                             : 1;
         S.nu0 = nu;
-        B->maxnu0 = std::max(B->maxnu0, nu);
         S.tiles0 = tiles(G->nx * G->ny) * ((G->nz + S.kchunk - 1) / S.kchunk);
         S.cinv = G->cinv;
         S.invd = G->invd;
@@ -943,6 +953,28 @@ int phg_batch_create_ex(phg_ctx* ctx, int nsys, const int* grids, const phg_batc
         std::vector<int2> wphys = xcd_order(wide);
         B->tabw_len = (int)wphys.size();
         if (!rc) rc = upload(ctx, B->allocs, &B->tabw, wphys.data(), wphys.size());
+        // the smoothing split: which systems take the fused passes
+        std::vector<int2> lf, lwf, lu;
+        for (const int2& e : logical) {
+            const GridDev* G = get_grid(ctx, grids[e.x]);
+            const bool f = B->binary && !B->unfused && G->nx <= FUSE_MAX_NX && (G->nx * G->ny) % 2 == 0 &&
+                           B->h_sys[e.x].nu0 == 1;
+            if (f) {
+                lf.push_back(e);
+                if ((e.y % tiles(G->nx * G->ny)) % 2 == 0) lwf.push_back(e);
+                B->maxnx_f = std::max(B->maxnx_f, G->nx);
+            } else {
+                lu.push_back(e);
+                B->maxnu0_u = std::max(B->maxnu0_u, B->h_sys[e.x].nu0);
+            }
+        }
+        std::vector<int2> pf = xcd_order(lf), pwf = xcd_order(lwf), pu = xcd_order(lu);
+        B->tabf_len = (int)pf.size();
+        B->tabwf_len = (int)pwf.size();
+        B->tabu_len = (int)pu.size();
+        if (!rc && !pf.empty()) rc = upload(ctx, B->allocs, &B->tabf, pf.data(), pf.size());
+        if (!rc && !pwf.empty()) rc = upload(ctx, B->allocs, &B->tabwf, pwf.data(), pwf.size());
+        if (!rc && !pu.empty()) rc = upload(ctx, B->allocs, &B->tabu, pu.data(), pu.size());
     }
     for (int l = 1; l < B->maxl && !rc; ++l) {
         std::vector<int4> logical;

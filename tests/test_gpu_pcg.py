@@ -161,6 +161,37 @@ def test_fused_presmoothing_is_bitwise_the_separate_passes(ctx, small, house, pr
     G.destroy()
 
 
+@pytest.mark.parametrize("prec", [gas.PREC_FP32, gas.PREC_MIXED_F16])
+@pytest.mark.parametrize("width", [0, 256])
+def test_batch_splits_smoothing_per_system(ctx, prec, width):
+    """Fused passes are chosen per system: in one batch the grids of even
+    plane size and nx <= 192 smoothed once take them (in the configuration
+    the widest of them needs), while a grid of odd plane size, one wider than
+    192 cells and a system given two sweeps take the separate half sweeps.
+    Every system's solve is bit for bit the all-separate batch's."""
+    gs = [sc.house(21, 24, 18, 10), sc.house(22, 25, 19, 8), sc.house(23, 200, 12, 6),
+          sc.house(24, 131, 10, 6), sc.house(25, 24, 18, 10)]
+    assert (gs[1].nx * gs[1].ny) % 2 == 1 and gs[2].nx > 192
+    Gs = [gas.Grid(ctx, g, precision=prec) for g in gs]
+    sweeps = [1, 1, 1, 1, 2]
+    bs = [sc.rhs_from_divergence(g, sc.divergence_field(g, 500 + s), DT) for s, g in enumerate(gs)]
+    p = gas.solve_params(tol=1e-6, max_iters=200, zero_x=True)
+    res = []
+    for unfused in (False, True):
+        B = gas.Batch(ctx, Gs, kchunk=4, unfused=unfused, sys_sweeps0=sweeps, fused_threads=width)
+        dbs = [ctx.to_device(b) for b in bs]
+        dxs = [ctx.array(g.n) for g in gs]
+        stats = B.solve(dbs, dxs, p)
+        assert all(s["status"] == gas.CONVERGED for s in stats)
+        res.append(([d.download() for d in dxs], [s["iters"] for s in stats]))
+        B.destroy()
+    assert res[0][1] == res[1][1]
+    for k, (a, b) in enumerate(zip(res[0][0], res[1][0])):
+        assert np.array_equal(a, b), f"system {k}"
+    for G in Gs:
+        G.destroy()
+
+
 def test_graph_replay_matches_direct(ctx, small):
     g, H = small
     G = gas.Grid(ctx, g)
