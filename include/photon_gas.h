@@ -69,8 +69,11 @@ typedef struct {
     int refine;          /* restarts allowed from the recomputed residual b - A x: in FP32 the
                             recursive residual can reach tol while the true one stalls higher; a
                             system whose true residual misses tol (and stands clear of FP32 noise)
-                            is restarted from its x (0 = none).  With refine > 0 every residual
-                            formed from x, a warm start's included, is accumulated in FP64 */
+                            is restarted from its x (0 = none), and so is one that broke down
+                            (d.Ad <= 0 or r.z <= 0); it keeps PHG_BREAKDOWN* unless a restart
+                            converges or finds its residual at the FP32 noise floor.  With
+                            refine > 0 every residual formed from x, a warm start's included, is
+                            accumulated in FP64 */
 } phg_solve_params;
 
 typedef struct {

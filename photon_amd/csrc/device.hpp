@@ -117,9 +117,10 @@ enum { S_RZ = 0, S_DQ, S_ALPHA, S_BETA, S_RR, S_BB, S_TOL, S_MAXR, S_INV, S_ZS, 
 // ST_FLEX: flexible beta (mixed precision); ST_VC: a CG iteration has run (q is defined)
 // ST_ITOT: iterations of earlier rounds (restarts from the true residual)
 // ST_RESTART: this round restarts from the recomputed residual (see k_fin_init)
+// ST_LAST: how the round before a restart ended (converged or broke down)
 enum {
     ST_ACTIVE = 0, ST_ITERS, ST_MAXIT, ST_STATUS, ST_RULE, ST_ZEROX, ST_XPEND, ST_FLEX, ST_VC, ST_ITOT, ST_RESTART,
-    ST_NSTATE = 12
+    ST_LAST, ST_NSTATE = 12
 };
 
 // One system of a batch.  Level 0 is swept by columns: a block takes TB

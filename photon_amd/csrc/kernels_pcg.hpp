@@ -582,21 +582,25 @@ __global__ void __launch_bounds__(TB) k_fin_init(const DSys* __restrict__ sys, c
         // floor of an FP32 solution (eps_32 times the size of the terms of
         // b - A x) and has fallen well below the residual the last round
         // started from: otherwise it is rounding noise, and iterating on it can
-        // only damage x.  Then the system counts as converged.
-        bool floor_hit = false;
+        // only damage x.  Then a system whose last round converged counts as
+        // converged; one whose last round broke down does so only if its
+        // residual is at that noise floor, and keeps its breakdown status
+        // otherwise.
+        bool floor_hit = false, at_noise = false;
         const double rn = sqrt(res[1]);
         if (restart) {
             s.state[ST_RESTART] = 0;
             // the term sizes add linearly here, the rounding errors about in
             // quadrature: a quarter of eps times their norm marks the noise
-            floor_hit = s.state[ST_RULE] == 0 &&
-                        (rn <= 0.25 * 5.9604644775390625e-8 * sqrt(res[2]) || rn > 0.25 * s.scal[S_RPREV]);
+            at_noise = s.state[ST_RULE] == 0 && rn <= 0.25 * 5.9604644775390625e-8 * sqrt(res[2]);
+            floor_hit = s.state[ST_RULE] == 0 && (at_noise || rn > 0.25 * s.scal[S_RPREV]);
         }
         if (floor_hit) {
             // no restart: the solve ends with the previous round's figures
+            const int last = s.state[ST_LAST];
             s.state[ST_ITERS] = 0;
             s.state[ST_XPEND] = 0;
-            s.state[ST_STATUS] = 1;
+            s.state[ST_STATUS] = last == 1 || at_noise ? 1 : last;
             s.state[ST_ACTIVE] = 0;
             return;
         }
@@ -688,15 +692,19 @@ __global__ void __launch_bounds__(TB) k_fin_beta(const DSys* __restrict__ sys, c
 }
 
 // between rounds of a solve: the owed x += alpha d has been applied (x_flush);
-// a system that converged is started again from its x, with its true
-// residual b - A x (k_resid_init, then k_fin_init stops it at once if that
-// already meets the tolerance); one that hit the cap or broke down stays done
+// a system that converged or broke down (d.Ad <= 0 or r.z <= 0, which the
+// rounding of an FP16/BF16 V-cycle can bring about once the recursive
+// residual nears the accuracy an FP32 x can hold) is started again from its
+// x, with its true residual b - A x (k_resid_init, then k_fin_init stops it
+// at once if that already meets the tolerance); one that hit the cap stays done
 __global__ void k_refine_reset(const DSys* __restrict__ sys, int nsys) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= nsys) return;
     int* st = sys[i].state;
     st[ST_XPEND] = 0;
-    if (st[ST_STATUS] != 1) return;
+    const int last = st[ST_STATUS];
+    if (last != 1 && last != 3 && last != 4) return;
+    st[ST_LAST] = last;
     st[ST_ITOT] += st[ST_ITERS];
     st[ST_ITERS] = 0;
     st[ST_STATUS] = 0;

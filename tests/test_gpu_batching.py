@@ -69,6 +69,28 @@ def test_mixed_settings_match_oracle_and_solo(ctx, scenarios, prec):
         G.destroy()
 
 
+def test_refine_restarts_a_bf16_breakdown(ctx, scenarios):
+    """Scenario 4 is a point source in a closed zone of 65760 cells: its exact
+    solution rounded to FP32 leaves a true relative residual near 3e-5, so the
+    recursive residual of the last iterations runs below anything an FP32 x
+    can hold.  There the rounding of a BF16 V-cycle can make d.Ad <= 0 (a
+    breakdown); refine restarts such a system from b - A x accumulated in FP64,
+    and it ends converged within 1e-5 of the FP64 oracle."""
+    grids, rhs = scenarios
+    G = gas.Grid(ctx, grids[4], precision=gas.PREC_MIXED_BF16)
+    x64, _, _ = po.Hierarchy(grids[4]).pcg64(rhs[4])
+    p0 = gas.solve_params(tol=1e-6, max_iters=400, zero_x=True)
+    _, s0 = solve(ctx, [G], [rhs[4]], p0)
+    p = gas.solve_params(tol=1e-6, max_iters=400, zero_x=True, refine=2)
+    xs, st = solve(ctx, [G], [rhs[4]], p)
+    err = rel(xs[0], x64)
+    print(f"refine=0: status {s0[0]['status']} after {s0[0]['iters']}; refine=2: status {st[0]['status']} "
+          f"after {st[0]['iters']}, err_fp64={err:.2e}")
+    assert st[0]["status"] == gas.CONVERGED
+    assert err < 1e-5
+    G.destroy()
+
+
 def test_two_sweeps_take_fewer_iterations(ctx, scenarios):
     grids, rhs = scenarios
     G = gas.Grid(ctx, grids[5])
