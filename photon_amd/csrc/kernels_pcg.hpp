@@ -1692,15 +1692,37 @@ __global__ void __launch_bounds__(TB) k_vc_prolong(const DSys* __restrict__ sys,
 // all of them, and CG_ROWS rows: lane = row, B read coalesced along a column,
 // the CGS right-hand sides broadcast from LDS ([n][CGS] floats: the V-cycle
 // vectors hold at most FP32, so the copy is exact).  The four waves split the
-// column range and their partials are added in a fixed order.
+// column range and their partials are added in a fixed order.  A wave issues
+// the loads of CG_UNROLL columns before their products, which it adds in
+// column order, so the sums do not depend on CG_UNROLL or CGS.
 // The reference approximates this solve with 20 forward/backward Gauss-Seidel
 // sweeps (pressure_mg.cpp:277-281); a serial sweep has no place on a GPU.
 constexpr int CG_ROWS = 64;
 constexpr int CG_SYS_MAX = 16;
+constexpr int CG_UNROLL = 8;
 
 // LDS bytes of k_vc_coarse for `cgs` systems of up to `n` unknowns
 __host__ __device__ constexpr size_t coarse_lds(int cgs, int n) {
     return (size_t)n * cgs * 4 > (size_t)3 * CG_ROWS * cgs * 8 ? (size_t)n * cgs * 4 : (size_t)3 * CG_ROWS * cgs * 8;
+}
+
+// acc[g] += a b[g] for the CGS right-hand sides of one column
+template <int CGS>
+__device__ __forceinline__ void coarse_acc(double (&acc)[CGS], double a, const float* b) {
+    if constexpr (CGS % 4 == 0) {
+        const float4* bv = reinterpret_cast<const float4*>(b);
+#pragma unroll
+        for (int q = 0; q < CGS / 4; ++q) {
+            const float4 b4 = bv[q];
+            acc[4 * q + 0] += a * (double)b4.x;
+            acc[4 * q + 1] += a * (double)b4.y;
+            acc[4 * q + 2] += a * (double)b4.z;
+            acc[4 * q + 3] += a * (double)b4.w;
+        }
+    } else {
+#pragma unroll
+        for (int g = 0; g < CGS; ++g) acc[g] += a * (double)b[g];
+    }
 }
 
 template <class VT, int CGS>
@@ -1731,18 +1753,15 @@ __global__ void __launch_bounds__(TB) k_vc_coarse(const DSys* __restrict__ sys, 
     double acc[CGS];
 #pragma unroll
     for (int g = 0; g < CGS; ++g) acc[g] = 0.0;
-    for (int j = j0; j < j1; ++j) {
-        const double a = gld(A, (unsigned)j * (unsigned)n + (unsigned)ic);
-        const float4* bv = reinterpret_cast<const float4*>(bsh + j * CGS);
+    int j = j0;
+    for (; j + CG_UNROLL <= j1; j += CG_UNROLL) {
+        double a[CG_UNROLL];
 #pragma unroll
-        for (int q = 0; q < CGS / 4; ++q) {
-            const float4 b4 = bv[q];
-            acc[4 * q + 0] += a * (double)b4.x;
-            acc[4 * q + 1] += a * (double)b4.y;
-            acc[4 * q + 2] += a * (double)b4.z;
-            acc[4 * q + 3] += a * (double)b4.w;
-        }
+        for (int u = 0; u < CG_UNROLL; ++u) a[u] = gld(A, (unsigned)(j + u) * (unsigned)n + (unsigned)ic);
+#pragma unroll
+        for (int u = 0; u < CG_UNROLL; ++u) coarse_acc<CGS>(acc, a[u], bsh + (j + u) * CGS);
     }
+    for (; j < j1; ++j) coarse_acc<CGS>(acc, gld(A, (unsigned)j * (unsigned)n + (unsigned)ic), bsh + j * CGS);
     __syncthreads();   // bsh is reused for the partials of waves 1..3
     double* red = dsh;   // [3][CG_ROWS][CGS]
     if (w > 0) {

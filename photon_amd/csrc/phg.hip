@@ -360,12 +360,15 @@ void vcycle_t(phg_ctx* ctx, BatchDev* B, int first) {
         launch(ctx, K_VC_RESID, k_vc_resid<CC, VT>, NG[l], 0, S, TG[l], l);
         launch(ctx, K_VC_RESTRICT, k_vc_restrict<VT>, NG[l + 1], 0, S, TG[l + 1], l);
     }
-    if (B->cg_sys == CG_SYS_MAX)
-        launch(ctx, K_VC_COARSE, k_vc_coarse<VT, CG_SYS_MAX>, B->tab_coarse_len, coarse_lds(CG_SYS_MAX, B->maxnc), S,
-               (const int4*)B->tab_coarse);
-    else
-        launch(ctx, K_VC_COARSE, k_vc_coarse<VT, CG_SYS_MAX / 2>, B->tab_coarse_len,
-               coarse_lds(CG_SYS_MAX / 2, B->maxnc), S, (const int4*)B->tab_coarse);
+    auto coarse = [&](auto kern, int cgs) {
+        launch(ctx, K_VC_COARSE, kern, B->tab_coarse_len, coarse_lds(cgs, B->maxnc), S, (const int4*)B->tab_coarse);
+    };
+    switch (B->cg_sys) {
+        case 1: coarse(k_vc_coarse<VT, 1>, 1); break;
+        case 4: coarse(k_vc_coarse<VT, 4>, 4); break;
+        case CG_SYS_MAX / 2: coarse(k_vc_coarse<VT, CG_SYS_MAX / 2>, CG_SYS_MAX / 2); break;
+        default: coarse(k_vc_coarse<VT, CG_SYS_MAX>, CG_SYS_MAX); break;
+    }
     for (int l = B->maxl - 2; l >= 1; --l) {
         launch(ctx, K_VC_PROLONG, k_vc_prolong<VT>, NG[l], 0, S, TG[l], l);
         launch(ctx, K_VC_JAC, k_vc_jac<CC, VT>, NG[l], 0, S, TG[l], l, 0);
@@ -1003,8 +1006,17 @@ int phg_batch_create_ex(phg_ctx* ctx, int nsys, const int* grids, const phg_batc
     if (!rc) {
         // coarsest-solve blocks: runs of up to cg_sys consecutive systems on one grid
         std::vector<int4> tc;
-        // 16 systems per block while their right-hand sides fit 64 KB of LDS, else 8
-        B->cg_sys = coarse_lds(CG_SYS_MAX, B->maxnc) <= 65536 ? CG_SYS_MAX : CG_SYS_MAX / 2;
+        // systems per block: the longest run on one grid rounded up to 1, 4, 8
+        // or 16 (16 while their right-hand sides fit 64 KB of LDS, else 8)
+        int maxrun = 1;
+        for (int s = 0; s < nsys;) {
+            int cnt = 1;
+            while (s + cnt < nsys && grids[s + cnt] == grids[s]) ++cnt;
+            maxrun = std::max(maxrun, cnt);
+            s += cnt;
+        }
+        const int cap = coarse_lds(CG_SYS_MAX, B->maxnc) <= 65536 ? CG_SYS_MAX : CG_SYS_MAX / 2;
+        B->cg_sys = maxrun <= 1 ? 1 : maxrun <= 4 ? 4 : maxrun <= CG_SYS_MAX / 2 ? CG_SYS_MAX / 2 : cap;
         for (int s = 0; s < nsys;) {
             int cnt = 1;
             while (s + cnt < nsys && cnt < B->cg_sys && grids[s + cnt] == grids[s]) ++cnt;
