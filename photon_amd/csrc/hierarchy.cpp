@@ -225,19 +225,42 @@ void dense_inverse(HostHierarchy& H) {
     }
     // Cholesky A = G G^T (lower G stored in A), column by column; the rows
     // below the diagonal of a column are shared out among the team, and every
-    // thread forms the column's diagonal itself from the finished row j
+    // thread forms the column's diagonal itself from the finished row j.  A
+    // thread runs the sums of four of its rows side by side, each in the
+    // order of k, so they overlap without changing a bit.
     {
         const int T = host_threads(n / 64);
         SpinBarrier bar(T);
         auto team = [&](int t) {
             for (int j = 0; j < n; ++j) {
-                double d = A[(size_t)j * n + j];
-                for (int k = 0; k < j; ++k) d -= A[(size_t)j * n + k] * A[(size_t)j * n + k];
+                const double* aj = &A[(size_t)j * n];
+                double d = aj[j];
+                for (int k = 0; k < j; ++k) d -= aj[k] * aj[k];
                 d = d > 0 ? std::sqrt(d) : 1e-300;
-                for (int i = j + 1 + t; i < n; i += T) {
-                    double s = A[(size_t)i * n + j];
-                    for (int k = 0; k < j; ++k) s -= A[(size_t)i * n + k] * A[(size_t)j * n + k];
-                    A[(size_t)i * n + j] = s / d;
+                int i = j + 1 + t;
+                for (; i + 3 * T < n; i += 4 * T) {
+                    double* a0 = &A[(size_t)i * n];
+                    double* a1 = &A[(size_t)(i + T) * n];
+                    double* a2 = &A[(size_t)(i + 2 * T) * n];
+                    double* a3 = &A[(size_t)(i + 3 * T) * n];
+                    double s0 = a0[j], s1 = a1[j], s2 = a2[j], s3 = a3[j];
+                    for (int k = 0; k < j; ++k) {
+                        const double b = aj[k];
+                        s0 -= a0[k] * b;
+                        s1 -= a1[k] * b;
+                        s2 -= a2[k] * b;
+                        s3 -= a3[k] * b;
+                    }
+                    a0[j] = s0 / d;
+                    a1[j] = s1 / d;
+                    a2[j] = s2 / d;
+                    a3[j] = s3 / d;
+                }
+                for (; i < n; i += T) {
+                    double* ai = &A[(size_t)i * n];
+                    double s = ai[j];
+                    for (int k = 0; k < j; ++k) s -= ai[k] * aj[k];
+                    ai[j] = s / d;
                 }
                 bar.wait();
                 if (t == 0) A[(size_t)j * n + j] = d;
@@ -248,25 +271,79 @@ void dense_inverse(HostHierarchy& H) {
         team(0);
         for (auto& th : pool) th.join();
     }
-    // G^-1 (lower, columns independent), then A^-1 = G^-T G^-1 (rows independent)
-    std::vector<double> Gi((size_t)n * n, 0.0);
+    // G^-1 (lower, columns independent), then A^-1 = G^-T G^-1 (rows
+    // independent).  G^-1 is kept transposed (Gt[j][i] = G^-1[i][j]) so both
+    // loops run along contiguous rows; the sums keep the order of k.
+    std::vector<double> Gt((size_t)n * n, 0.0);
+    // four rows at a time: their sums over the finished k < i side by side,
+    // then the terms of the rows just finished, in the order of k
     parallel_rows(n, [&](int j) {
-        Gi[(size_t)j * n + j] = 1.0 / A[(size_t)j * n + j];
-        for (int i = j + 1; i < n; ++i) {
+        double* gj = &Gt[(size_t)j * n];
+        gj[j] = 1.0 / A[(size_t)j * n + j];
+        int i = j + 1;
+        for (; i + 3 < n; i += 4) {
+            const double* a0 = &A[(size_t)i * n];
+            const double* a1 = &A[(size_t)(i + 1) * n];
+            const double* a2 = &A[(size_t)(i + 2) * n];
+            const double* a3 = &A[(size_t)(i + 3) * n];
+            double s0 = 0, s1 = 0, s2 = 0, s3 = 0;
+            for (int k = j; k < i; ++k) {
+                const double g = gj[k];
+                s0 -= a0[k] * g;
+                s1 -= a1[k] * g;
+                s2 -= a2[k] * g;
+                s3 -= a3[k] * g;
+            }
+            gj[i] = s0 / a0[i];
+            s1 -= a1[i] * gj[i];
+            gj[i + 1] = s1 / a1[i + 1];
+            s2 -= a2[i] * gj[i];
+            s2 -= a2[i + 1] * gj[i + 1];
+            gj[i + 2] = s2 / a2[i + 2];
+            s3 -= a3[i] * gj[i];
+            s3 -= a3[i + 1] * gj[i + 1];
+            s3 -= a3[i + 2] * gj[i + 2];
+            gj[i + 3] = s3 / a3[i + 3];
+        }
+        for (; i < n; ++i) {
+            const double* ai = &A[(size_t)i * n];
             double s = 0;
-            for (int k = j; k < i; ++k) s -= A[(size_t)i * n + k] * Gi[(size_t)k * n + j];
-            Gi[(size_t)i * n + j] = s / A[(size_t)i * n + i];
+            for (int k = j; k < i; ++k) s -= ai[k] * gj[k];
+            gj[i] = s / ai[i];
         }
     });
     H.cinv.assign((size_t)n * n, 0.0);
     parallel_rows(n, [&](int i) {
-        for (int j = 0; j <= i; ++j) {
+        const double* gi = &Gt[(size_t)i * n];
+        double* ci = &H.cinv[(size_t)i * n];
+        int j = 0;
+        for (; j + 3 <= i; j += 4) {
+            const double* g0 = &Gt[(size_t)j * n];
+            const double* g1 = g0 + n;
+            const double* g2 = g1 + n;
+            const double* g3 = g2 + n;
+            double s0 = 0, s1 = 0, s2 = 0, s3 = 0;
+            for (int k = i; k < n; ++k) {
+                const double g = gi[k];
+                s0 += g * g0[k];
+                s1 += g * g1[k];
+                s2 += g * g2[k];
+                s3 += g * g3[k];
+            }
+            ci[j] = s0;
+            ci[j + 1] = s1;
+            ci[j + 2] = s2;
+            ci[j + 3] = s3;
+        }
+        for (; j <= i; ++j) {
+            const double* gj = &Gt[(size_t)j * n];
             double s = 0;
-            for (int k = i; k < n; ++k) s += Gi[(size_t)k * n + i] * Gi[(size_t)k * n + j];
-            H.cinv[(size_t)i * n + j] = s;
-            H.cinv[(size_t)j * n + i] = s;
+            for (int k = i; k < n; ++k) s += gi[k] * gj[k];
+            ci[j] = s;
         }
     });
+    for (int i = 0; i < n; ++i)
+        for (int j = 0; j < i; ++j) H.cinv[(size_t)j * n + i] = H.cinv[(size_t)i * n + j];
     for (int I = 0; I < n; ++I)
         if (H.cmask[I])
             for (int J = 0; J < n; ++J) H.cinv[(size_t)I * n + J] = H.cinv[(size_t)J * n + I] = 0.0;

@@ -91,9 +91,21 @@ class Grid:
     ax: np.ndarray               # float32 matrix coefficients (ax_, ay_, az_)
     ay: np.ndarray
     az: np.ndarray
-    zone: np.ndarray | None = None       # int32 [N]
-    zone_open: np.ndarray | None = None  # uint8 [nzones]
     burner_cells: np.ndarray | None = None
+    # (zone int32 [N], zone_open uint8 [nzones]), formed on first use
+    _zones: tuple | None = dataclasses.field(default=None, init=False, repr=False, compare=False)
+
+    @property
+    def zone(self) -> np.ndarray:
+        if self._zones is None:
+            compute_zones(self)
+        return self._zones[0]
+
+    @property
+    def zone_open(self) -> np.ndarray:
+        if self._zones is None:
+            compute_zones(self)
+        return self._zones[1]
 
     @property
     def n(self) -> int:
@@ -179,8 +191,7 @@ def compute_zones(grid: Grid):
                    (z3[-1], az[nz])):
         hit = (sel >= 0) & (a > 0)
         zopen[np.unique(sel[hit])] = 1
-    grid.zone = zone
-    grid.zone_open = zopen[:nzones] if nzones else zopen[:0]
+    grid._zones = (zone, zopen[:nzones] if nzones else zopen[:0])
     return grid
 
 
@@ -189,7 +200,7 @@ def _finish(blocked3, dx, open_sides=(), door=None) -> Grid:
     fx, fy, fz = faces_from_mask(blocked3, open_sides, door)
     g = Grid(nx, ny, nz, float(dx), blocked3.astype(np.uint8).ravel(), fx, fy, fz,
              fx.copy(), fy.copy(), fz.copy())
-    return compute_zones(g)
+    return g
 
 
 def room(dx: float = 0.2) -> Grid:
@@ -344,5 +355,4 @@ def vented(grid: Grid, seed: int, frac: float = 0.15) -> Grid:
         out[sel] = (0.05 + 0.9 * v[sel]).astype(np.float32)
         return out
     fx, fy, fz = perturb(grid.fx, 11), perturb(grid.fy, 13), perturb(grid.fz, 17)
-    g = Grid(grid.nx, grid.ny, grid.nz, grid.dx, grid.blocked.copy(), fx, fy, fz, fx.copy(), fy.copy(), fz.copy())
-    return compute_zones(g)
+    return Grid(grid.nx, grid.ny, grid.nz, grid.dx, grid.blocked.copy(), fx, fy, fz, fx.copy(), fy.copy(), fz.copy())
