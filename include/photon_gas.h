@@ -110,6 +110,24 @@ int phg_grid_create(phg_ctx *ctx, int nx, int ny, int nz, float dx, const float 
                     const float *fz, const float *ax, const float *ay, const float *az, const uint8_t *blocked,
                     const uint8_t *wind_closed, int precision, int *grid_out);
 int phg_grid_destroy(phg_ctx *ctx, int grid);
+/* The same set-up in two halves, so a server can build a grid's host half
+ * while the device runs other work (no counterpart in the reference, whose
+ * job runner builds one Solver at a time, src/app/cli.cpp:23-101):
+ * phg_grid_plan_build runs the host work of phg_grid_create (the hierarchy,
+ * its dense coarsest inverse and the zones) with no context and no device;
+ * calls on distinct plans may run on several threads at once.
+ * phg_grid_create_planned is phg_grid_create taking that work from the plan
+ * (which must come from the same nx, ny, nz, ax, ay, az, blocked and
+ * wind_closed; a plan serves one grid), and yields the same grid.  A plan is
+ * freed with phg_grid_plan_free, used or not. */
+typedef struct phg_grid_plan phg_grid_plan;
+int phg_grid_plan_build(int nx, int ny, int nz, const float *ax, const float *ay, const float *az,
+                        const uint8_t *blocked, const uint8_t *wind_closed, phg_grid_plan **plan_out);
+void phg_grid_plan_free(phg_grid_plan *plan);
+int phg_grid_create_planned(phg_ctx *ctx, int nx, int ny, int nz, float dx, const float *fx, const float *fy,
+                            const float *fz, const float *ax, const float *ay, const float *az,
+                            const uint8_t *blocked, const uint8_t *wind_closed, int precision,
+                            phg_grid_plan *plan, int *grid_out);
 /* hierarchy shape: level count, sizes (nlevels entries), zones */
 int phg_grid_info(phg_ctx *ctx, int grid, int *nlevels, int *level_n, int *nzones, int *ref_levels);
 /* zone labels (N ints, -1 solid) and open flags (nzones bytes), host out */

@@ -608,16 +608,91 @@ int phg_memset(phg_ctx* ctx, void* dst, int value, size_t bytes) {
 }
 
 // ---------------------------------------------------------------------------
+}  // extern "C"
+
+// the host half of a grid's set-up: hierarchy, dense coarsest inverse, zones
+struct phg_grid_plan {
+    int nx = 0, ny = 0, nz = 0;
+    bool used = false;
+    HostHierarchy H;
+    std::vector<int> zone;
+    std::vector<uint8_t> zone_open;
+};
+
+namespace {
+
+// the checks both halves make; the message is empty when the shape is fine
+std::string check_shape(int nx, int ny, int nz) {
+    char buf[128];
+    if (nx < 2 || ny < 2 || nz < 2 || (long)nx * ny * nz > (1L << 30)) {
+        snprintf(buf, sizeof buf, "bad grid %dx%dx%d", nx, ny, nz);
+        return buf;
+    }
+    if (nx > MARCH_MAX_NX) {
+        snprintf(buf, sizeof buf, "nx=%d exceeds the limit %d", nx, MARCH_MAX_NX);
+        return buf;
+    }
+    return "";
+}
+
+// build_hierarchy and compute_zones into P; false with a message on failure
+bool plan_geometry(int nx, int ny, int nz, const float* ax, const float* ay, const float* az, const uint8_t* blocked,
+                   const uint8_t* wind_closed, phg_grid_plan& P, std::string* err) {
+    P.nx = nx;
+    P.ny = ny;
+    P.nz = nz;
+    if (!build_hierarchy(nx, ny, nz, ax, ay, az, blocked, wind_closed, kDenseMax, P.H, err)) return false;
+    if (P.H.nc > kDenseMax) {
+        char buf[128];
+        snprintf(buf, sizeof buf, "coarsest level has %d unknowns (limit %d)", P.H.nc, kDenseMax);
+        *err = buf;
+        return false;
+    }
+    compute_zones(nx, ny, nz, ax, ay, az, blocked, P.zone, P.zone_open);
+    return true;
+}
+
+}  // namespace
+
+extern "C" {
+
+int phg_grid_plan_build(int nx, int ny, int nz, const float* ax, const float* ay, const float* az,
+                        const uint8_t* blocked, const uint8_t* wind_closed, phg_grid_plan** plan_out) {
+    if (!plan_out) return PHG_EINVAL;
+    *plan_out = nullptr;
+    if (!ax || !ay || !az || !blocked || !check_shape(nx, ny, nz).empty()) return PHG_EINVAL;
+    phg_grid_plan* P = new phg_grid_plan();
+    std::string err;
+    if (!plan_geometry(nx, ny, nz, ax, ay, az, blocked, wind_closed, *P, &err)) {
+        delete P;
+        return PHG_EINVAL;
+    }
+    *plan_out = P;
+    return PHG_OK;
+}
+
+void phg_grid_plan_free(phg_grid_plan* plan) { delete plan; }
+
 int phg_grid_create(phg_ctx* ctx, int nx, int ny, int nz, float dx, const float* fx, const float* fy, const float* fz,
                     const float* ax, const float* ay, const float* az, const uint8_t* blocked,
                     const uint8_t* wind_closed, int precision, int* grid_out) {
+    return phg_grid_create_planned(ctx, nx, ny, nz, dx, fx, fy, fz, ax, ay, az, blocked, wind_closed, precision,
+                                   nullptr, grid_out);
+}
+
+int phg_grid_create_planned(phg_ctx* ctx, int nx, int ny, int nz, float dx, const float* fx, const float* fy,
+                            const float* fz, const float* ax, const float* ay, const float* az,
+                            const uint8_t* blocked, const uint8_t* wind_closed, int precision, phg_grid_plan* plan,
+                            int* grid_out) {
     if (!ctx || !grid_out) return PHG_EINVAL;
-    if (nx < 2 || ny < 2 || nz < 2 || (long)nx * ny * nz > (1L << 30) || !(dx > 0))
-        return fail(ctx, PHG_EINVAL, "bad grid %dx%dx%d dx=%g", nx, ny, nz, dx);
-    if (nx > MARCH_MAX_NX) return fail(ctx, PHG_EINVAL, "nx=%d exceeds the limit %d", nx, MARCH_MAX_NX);
+    const std::string bad = check_shape(nx, ny, nz);
+    if (!bad.empty()) return fail(ctx, PHG_EINVAL, "%s", bad.c_str());
+    if (!(dx > 0)) return fail(ctx, PHG_EINVAL, "bad cell size dx=%g", dx);
     if (!fx || !fy || !fz || !ax || !ay || !az || !blocked) return fail(ctx, PHG_EINVAL, "null geometry array");
     if (precision < PHG_PREC_FP32 || precision > PHG_PREC_MIXED_BF16)
         return fail(ctx, PHG_EINVAL, "unknown precision %d", precision);
+    if (plan && (plan->used || plan->nx != nx || plan->ny != ny || plan->nz != nz))
+        return fail(ctx, PHG_EINVAL, "the plan is used or was built for another grid shape");
     HIP_OK(hipSetDevice(ctx->device));
     GridDev* G = new GridDev();
     G->nx = nx;
@@ -626,17 +701,22 @@ int phg_grid_create(phg_ctx* ctx, int nx, int ny, int nz, float dx, const float*
     G->n = nx * ny * nz;
     G->dx = dx;
     G->prec = precision;
-    std::string err;
-    if (!build_hierarchy(nx, ny, nz, ax, ay, az, blocked, wind_closed, kDenseMax, G->H, &err)) {
-        delete G;
-        return fail(ctx, PHG_EINVAL, "%s", err.c_str());
+    if (plan) {
+        G->H = std::move(plan->H);
+        G->zone = std::move(plan->zone);
+        G->zone_open = std::move(plan->zone_open);
+        plan->used = true;
+    } else {
+        phg_grid_plan P;
+        std::string err;
+        if (!plan_geometry(nx, ny, nz, ax, ay, az, blocked, wind_closed, P, &err)) {
+            delete G;
+            return fail(ctx, PHG_EINVAL, "%s", err.c_str());
+        }
+        G->H = std::move(P.H);
+        G->zone = std::move(P.zone);
+        G->zone_open = std::move(P.zone_open);
     }
-    if (G->H.nc > kDenseMax) {
-        int nc = G->H.nc;
-        delete G;
-        return fail(ctx, PHG_EINVAL, "coarsest level has %d unknowns (limit %d)", nc, kDenseMax);
-    }
-    compute_zones(nx, ny, nz, ax, ay, az, blocked, G->zone, G->zone_open);
     const size_t nfx = (size_t)(nx + 1) * ny * nz, nfy = (size_t)nx * (ny + 1) * nz, nfz = (size_t)nx * ny * (nz + 1);
     const HostLevel& L0 = G->H.L[0];
     int rc = 0;

@@ -30,6 +30,7 @@ EXPORTS = (
     "phg_energy_expansion", "phg_advect_scalars", "phg_profile_enable", "phg_profile_select", "phg_profile_get",
     "phg_profile_reset",
     "phg_hierarchy_shape", "phg_batch_create_ex", "phg_copy", "phg_leakage",
+    "phg_grid_plan_build", "phg_grid_plan_free", "phg_grid_create_planned",
 )
 
 
@@ -103,6 +104,10 @@ def load_library():
     L.phg_profile_get.argtypes = [vp, i32, C.POINTER(C.c_char_p), C.POINTER(f64), C.POINTER(C.c_longlong), C.POINTER(f64)]
     L.phg_profile_reset.argtypes = [vp]
     L.phg_hierarchy_shape.argtypes = [i32, i32, i32] + [vp] * 5 + [C.POINTER(i32), vp, vp]
+    L.phg_grid_plan_build.argtypes = [i32, i32, i32] + [vp] * 5 + [C.POINTER(vp)]
+    L.phg_grid_plan_free.argtypes = [vp]
+    L.phg_grid_plan_free.restype = None
+    L.phg_grid_create_planned.argtypes = [vp, i32, i32, i32, f32] + [vp] * 8 + [i32, vp, C.POINTER(i32)]
     _lib = L
     return L
 
@@ -233,11 +238,42 @@ def solve_params(tol=1e-6, max_iters=200, stop_rule=STOP_REL_L2, inv=1.0, demean
                        refine)
 
 
+class GridPlan:
+    """The host half of a grid's set-up (phg_grid_plan_build: the hierarchy,
+    its dense coarsest inverse and the zones), built without a device; safe
+    to build on a worker thread (ctypes releases the GIL for the call)."""
+
+    def __init__(self, scen, wind_closed=None):
+        L = load_library()
+        self.scen = scen
+        self.wind_closed = wind_closed
+        arrs = [np.ascontiguousarray(a, np.float32) for a in (scen.ax, scen.ay, scen.az)]
+        blk = np.ascontiguousarray(scen.blocked, np.uint8)
+        wc = None if wind_closed is None else np.ascontiguousarray(wind_closed, np.uint8)
+        h = C.c_void_p()
+        rc = L.phg_grid_plan_build(scen.nx, scen.ny, scen.nz, *[a.ctypes.data for a in arrs], blk.ctypes.data,
+                                   None if wc is None else wc.ctypes.data, C.byref(h))
+        if rc != 0:
+            raise PhgError(f"phg_grid_plan_build failed with {rc}")
+        self.h = h
+
+    def free(self):
+        if self.h:
+            load_library().phg_grid_plan_free(self.h)
+            self.h = None
+
+    def __del__(self):
+        if getattr(self, "h", None) and _lib is not None:
+            _lib.phg_grid_plan_free(self.h)
+            self.h = None
+
+
 class Grid:
     """Geometry of one scenario on the device with its multigrid hierarchy
-    (phg_grid_create: buildPressureLevels + computeZones)."""
+    (phg_grid_create: buildPressureLevels + computeZones).  `plan`: a GridPlan
+    of the same geometry whose host work is taken instead of redone."""
 
-    def __init__(self, ctx: Context, scen, precision=PREC_FP32, wind_closed=None):
+    def __init__(self, ctx: Context, scen, precision=PREC_FP32, wind_closed=None, plan: GridPlan | None = None):
         self.ctx = ctx
         self.scen = scen
         self.precision = precision
@@ -247,9 +283,21 @@ class Grid:
         blk = np.ascontiguousarray(scen.blocked, np.uint8)
         wc = None if wind_closed is None else np.ascontiguousarray(wind_closed, np.uint8)
         gid = C.c_int()
-        ctx._check(ctx.L.phg_grid_create(ctx.h, scen.nx, scen.ny, scen.nz, scen.dx,
-                                         *[a.ctypes.data for a in arrs], blk.ctypes.data,
-                                         None if wc is None else wc.ctypes.data, precision, C.byref(gid)))
+        if plan is None:
+            ctx._check(ctx.L.phg_grid_create(ctx.h, scen.nx, scen.ny, scen.nz, scen.dx,
+                                             *[a.ctypes.data for a in arrs], blk.ctypes.data,
+                                             None if wc is None else wc.ctypes.data, precision, C.byref(gid)))
+        else:
+            same_wc = (plan.wind_closed is None and wind_closed is None) or (
+                plan.wind_closed is not None and wind_closed is not None
+                and np.array_equal(np.asarray(plan.wind_closed, np.uint8), np.asarray(wind_closed, np.uint8)))
+            if plan.scen is not scen or not same_wc or not plan.h:
+                raise PhgError("the plan was built for another geometry or is used")
+            ctx._check(ctx.L.phg_grid_create_planned(ctx.h, scen.nx, scen.ny, scen.nz, scen.dx,
+                                                     *[a.ctypes.data for a in arrs], blk.ctypes.data,
+                                                     None if wc is None else wc.ctypes.data, precision, plan.h,
+                                                     C.byref(gid)))
+            plan.free()
         self.id = gid.value
 
     def info(self):
@@ -349,10 +397,10 @@ class GasSolver:
     CELL_FIELDS = ("p", "S", "Pb", "Q", "massSrc", "nu") + FIELD_NAMES
 
     def __init__(self, ctx: Context, scen, precision=PREC_FP32, wind_closed=None, tol=1e-6, max_iters=400,
-                 stop_rule=STOP_REL_L2):
+                 stop_rule=STOP_REL_L2, plan: GridPlan | None = None):
         self.ctx = ctx
         self.scen = scen
-        self.grid = Grid(ctx, scen, precision, wind_closed)
+        self.grid = Grid(ctx, scen, precision, wind_closed, plan=plan)
         n = self.grid.n
         self.d = {}
         for name in ("u", "v", "w"):

@@ -8,6 +8,11 @@ scenarios:
 - requests wait in a FIFO queue and are admitted when their estimated HBM
   footprint fits the budget and a slot is free, at any step boundary (so work
   that arrives while others run joins them at once);
+- a request's host set-up (geometry, multigrid hierarchy with its dense
+  coarsest inverse, zones: gas.GridPlan) starts on a worker thread when it
+  arrives, while the device steps the admitted scenarios; admission then only
+  installs the grid and fields on the device (it waits for the set-up only
+  when nothing else is running);
 - every scenario keeps its fields resident on the device; a step advances all
   admitted scenarios together: energyAndExpansion, the closed-zone source
   shift of leakage (sealed rooms), the right-hand side, ONE
@@ -23,6 +28,7 @@ scenario's results are the same bit for bit alone and in any mix.
 from __future__ import annotations
 
 import collections
+import concurrent.futures
 import math
 import time
 from dataclasses import dataclass, field
@@ -89,8 +95,11 @@ class Server:
 
     def __init__(self, ctx: gas.Context, hbm_budget_bytes: float = 64e9, max_active: int = 64,
                  precision: int = gas.PREC_FP32, dt: float = 0.01, tol: float = 1e-6, max_iters: int = 400,
-                 refine: int = 1, warm_start: bool = False):
+                 refine: int = 1, warm_start: bool = False, prep_workers: int = 2):
         self.ctx = ctx
+        # host set-up of queued requests (0: done at admission, on this thread)
+        self._pool = concurrent.futures.ThreadPoolExecutor(prep_workers) if prep_workers > 0 else None
+        self._prep: dict = {}
         self.budget = hbm_budget_bytes
         self.max_active = max_active
         self.precision = precision
@@ -110,14 +119,23 @@ class Server:
         self._batch_ids = None
         self.step_latencies: list[float] = []   # one entry per job-step
         self.steps_run = 0
-        # host seconds spent admitting (geometry, hierarchy, device fields),
-        # rebuilding the batch after the active set changed, and stepping
+        # host seconds of the loop's thread spent admitting (device grid and
+        # fields, plus geometry and hierarchy when prep_workers is 0, or the
+        # wait for them when nothing else runs), rebuilding the batch after
+        # the active set changed, and stepping
         self.t_admit = self.t_rebuild = self.t_step = self.t_solve = 0.0
         self.steps = 0
 
     # ---- requests --------------------------------------------------------------
     def submit(self, spec: JobSpec):
         self.queue.append(spec)
+        if self._pool is not None:
+            self._prep[spec.job_id] = self._pool.submit(self._prepare, spec)
+
+    def _prepare(self, spec: JobSpec):
+        """Host work of a request: its geometry, grid plan and burner field."""
+        g = grid_for(spec)
+        return g, gas.GridPlan(g), self._burner_heat(g, spec)
 
     def _admit(self, now_s: float):
         t0 = time.perf_counter()
@@ -129,16 +147,26 @@ class Server:
     def _admit_inner(self, now_s: float):
         while self.queue and len(self.active) < self.max_active:
             spec = self.queue[0]
-            g = grid_for(spec)
+            if self._pool is not None:
+                fut = self._prep[spec.job_id]
+                if not fut.done() and self.active:
+                    break   # set-up still running: admit at a later step boundary
+                g, plan, q = fut.result()
+            else:
+                g, plan, q = self._prepare_inline(spec)
             need = footprint_bytes(g)
             if self.used + need > self.budget and self.active:
                 break   # wait for memory to free up (a lone job is always admitted)
             self.queue.popleft()
-            solver = gas.GasSolver(self.ctx, g, precision=self.precision)
-            q = self._burner_heat(g, spec)
+            self._prep.pop(spec.job_id, None)
+            solver = gas.GasSolver(self.ctx, g, precision=self.precision, plan=plan)
             solver.set("Q", q)
             self.active.append(Job(spec, g, solver, need, now_s))
             self.used += need
+
+    def _prepare_inline(self, spec: JobSpec):
+        g = grid_for(spec)
+        return g, None, self._burner_heat(g, spec)
 
     def _burner_heat(self, g: sc.Grid, spec: JobSpec) -> np.ndarray:
         """Q (J per step) of a 2x2-cell floor burner of spec.hrr_kw at a seeded
@@ -214,6 +242,12 @@ class Server:
         s.grid.destroy()
 
     def close(self):
+        if self._pool is not None:
+            for fut in self._prep.values():
+                fut.cancel()
+            self._pool.shutdown(wait=True)
+            self._pool = None
+            self._prep.clear()
         if self._batch is not None:
             self._batch.destroy()
             self._batch = None

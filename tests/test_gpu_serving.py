@@ -82,3 +82,30 @@ def test_poisson_stream_serves_every_job(ctx):
     out = sv.serve(ctx, specs, hbm_budget_bytes=8e9, max_active=8)
     assert out["jobs"] == len(specs) and out["job_steps"] == 3 * len(specs)
     assert out["step_latency_ms_p99"] >= out["step_latency_ms_p50"] > 0
+
+
+def test_planned_grid_is_the_same_grid(ctx):
+    """A grid installed from a plan built on a worker thread is the grid
+    phg_grid_create builds: same hierarchy and zones, same solve bit for bit."""
+    import concurrent.futures
+
+    from photon_amd import scenarios as sc
+    g = sc.house(5, 40, 30, 20)
+    b = sc.rhs_from_divergence(g, sc.burner_field(g, 5), 0.01)
+    p = gas.solve_params(tol=1e-6, max_iters=200, zero_x=True)
+    with concurrent.futures.ThreadPoolExecutor(1) as pool:
+        plan = pool.submit(gas.GridPlan, g).result()
+    out = []
+    for pl in (None, plan):
+        G = gas.Grid(ctx, g, precision=gas.PREC_MIXED_F16, plan=pl)
+        db, dx = ctx.to_device(b), ctx.array(g.n)
+        st = G.solve(db, dx, p)
+        zone, zo = G.zones()
+        out.append((st["iters"], dx.download(), G.info(), zone, zo))
+        G.destroy()
+    assert not plan.h   # consumed
+    assert out[0][0] == out[1][0] and out[0][2] == out[1][2]
+    for k in (1, 3, 4):
+        assert np.array_equal(out[0][k], out[1][k])
+    with pytest.raises(gas.PhgError):
+        gas.Grid(ctx, g, plan=plan)   # a plan serves one grid

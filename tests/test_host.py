@@ -60,6 +60,27 @@ def test_library_hierarchy_matches_oracle():
         assert np.array_equal(agg0, H.level(0)["agg"])
 
 
+def test_grid_plan_builds_without_a_device():
+    """The host half of grid set-up runs with no context, on worker threads
+    at once, and refuses shapes phg_grid_create refuses."""
+    import concurrent.futures
+    gs = [sc.room(), sc.house(7, 24, 16, 12), sc.batch_grid(5)]
+    with concurrent.futures.ThreadPoolExecutor(3) as pool:
+        plans = list(pool.map(gas.GridPlan, gs))
+    for p in plans:
+        assert p.h
+        p.free()
+        assert not p.h
+    g = sc.room()
+    L = gas.load_library()
+    h = ctypes.c_void_p()
+    arrs = [np.ascontiguousarray(a, np.float32) for a in (g.ax, g.ay, g.az)]
+    blk = np.ascontiguousarray(g.blocked, np.uint8)
+    assert L.phg_grid_plan_build(1, g.ny, g.nz, *[a.ctypes.data for a in arrs], blk.ctypes.data, None,
+                                 ctypes.byref(h)) != 0
+    assert not h.value
+
+
 def test_abi_exports_every_declared_symbol():
     header = open(os.path.join(ROOT, "include", "photon_gas.h")).read()
     declared = set(re.findall(r"\b(phg_\w+)\s*\(", header))
