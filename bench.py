@@ -295,7 +295,10 @@ def cpu_baseline(scen, bs_host, iters_ref, budget_s=15.0):
     """The reference's own Solver::project (oracle/_ref, its GCC flags) on the
     host cores, each solve run for the iteration count the reference method
     needs to reach ||r||/||b|| <= 1e-6 on that RHS; falls back to the C
-    restatement (oracle/liboracle.so) when the reference build is absent."""
+    restatement (oracle/liboracle.so) when the reference build is absent.
+    `value` is for every core of the host share (at most 16); `by_threads`
+    adds the reference's default pool (cores - 2, src/sim/solver.cpp:41) and
+    one thread, on the same sample within the same time budget each."""
     from oracle import pyoracle as po
     from photon_amd import scenarios as sc
     threads = max(1, min(16, os.cpu_count() or 1))
@@ -310,19 +313,29 @@ def cpu_baseline(scen, bs_host, iters_ref, budget_s=15.0):
         pass
     kind = "reference" if po.ref_available(pin=False) else "port"
     done, t_used = 0, 0.0
+    by_threads = {}
     if kind == "reference":
-        R = po.RefSolver(scen, threads=threads, pin=False)
         zf = [np.zeros(len(a), np.float32) for a in (scen.fx, scen.fy, scen.fz)]
-        while t_used < budget_s and done < len(bs_host):
-            S = bs_host[done]
-            R.set("u", zf[0]); R.set("v", zf[1]); R.set("w", zf[2])
-            R.set("p", np.zeros(scen.n, np.float32))
-            R.set("S", S)
-            t0 = time.perf_counter()
-            R.project(DT, 0.0, int(iters_ref[done]))
-            t_used += time.perf_counter() - t0
-            done += 1
-        threads = R.threads
+
+        def run_ref(nthreads):
+            R = po.RefSolver(scen, threads=nthreads, pin=False)
+            n, used = 0, 0.0
+            while used < budget_s and n < len(bs_host):
+                R.set("u", zf[0]); R.set("v", zf[1]); R.set("w", zf[2])
+                R.set("p", np.zeros(scen.n, np.float32))
+                R.set("S", bs_host[n])
+                t0 = time.perf_counter()
+                R.project(DT, 0.0, int(iters_ref[n]))
+                used += time.perf_counter() - t0
+                n += 1
+            return R.threads, n, used
+
+        threads, done, t_used = run_ref(threads)
+        by_threads[str(threads)] = round(done / t_used, 3)
+        for t in sorted({max(1, threads - 2), 1}, reverse=True):
+            if str(t) not in by_threads:
+                tt, n, used = run_ref(t)
+                by_threads[str(tt)] = round(n / used, 3)
     else:
         H = po.Hierarchy(scen)
         threads = 1
@@ -336,7 +349,10 @@ def cpu_baseline(scen, bs_host, iters_ref, budget_s=15.0):
     sample = (f"{done} solves of the 80x50x40 house workload ({what}, FP32 MG-PCG run to the reference "
               f"method's iteration count for ||r||/||b|| <= 1e-6, mean {np.mean(iters_ref[:done]):.1f} "
               f"iterations), {threads} threads on {cpu_model}")
-    return {"value": done / t_used, "unit": "solves/s", "cores": threads, "kind": kind, "sample": sample}
+    out = {"value": done / t_used, "unit": "solves/s", "cores": threads, "kind": kind, "sample": sample}
+    if by_threads:
+        out["by_threads"] = by_threads
+    return out
 
 
 def main():
