@@ -202,7 +202,8 @@ def run_batching(ctx, gas, sc, nscen=64, steps=5):
     Gs = [gas.Grid(ctx, g, precision=gas.PREC_MIXED_F16) for g in grids]
     dbs = [ctx.to_device(b) for b in rhs]
     dxs = [ctx.array(g.n) for g in grids]
-    params = gas.solve_params(tol=TOL, max_iters=400, zero_x=True, use_graph=2, check_every=48)
+    # the set-up and the first 16 iterations as one graph, then a check every 4
+    params = gas.solve_params(tol=TOL, max_iters=400, zero_x=True, use_graph=2, check_every=16)
 
     def timed(kcs, sws):
         B = gas.Batch(ctx, Gs, sys_kchunk=kcs, sys_sweeps0=sws)

@@ -64,7 +64,8 @@ typedef struct {
     int check_every;     /* iterations between host convergence checks (0 = library default) */
     int use_graph;       /* 0: launch directly; 1: replay one captured CG iteration; 2: the set-up and
                             the first check_every iterations as one captured graph (one launch per
-                            solve when every system stops within them), then as 1 */
+                            solve when every system stops within them), then as 1 with a check every
+                            max(4, check_every / 4) iterations (at most check_every) */
     int zero_x;          /* start from x = 0 (x is written, not read) instead of warm-starting */
     int refine;          /* restarts allowed from the recomputed residual b - A x: in FP32 the
                             recursive residual can reach tol while the true one stalls higher; a

@@ -113,14 +113,17 @@ struct DLevel {
 // S_ZS: power-of-two scale applied to the V-cycle input (its residual made
 // O(1) so FP16/BF16 storage stays in range; exact in FP32); S_ZSN: the next one
 // S_RPREV: ||r|| at the start of the current round (restarts must make progress)
-enum { S_RZ = 0, S_DQ, S_ALPHA, S_BETA, S_RR, S_BB, S_TOL, S_MAXR, S_INV, S_ZS, S_ZSN, S_RPREV, S_NSCAL = 16 };
+// S_PNEXT: the ||r||^2 / ||b||^2 below which the residual's closed-zone means
+// are removed next (see k_proj_r)
+enum { S_RZ = 0, S_DQ, S_ALPHA, S_BETA, S_RR, S_BB, S_TOL, S_MAXR, S_INV, S_ZS, S_ZSN, S_RPREV, S_PNEXT, S_NSCAL = 16 };
 // ST_FLEX: flexible beta (mixed precision); ST_VC: a CG iteration has run (q is defined)
 // ST_ITOT: iterations of earlier rounds (restarts from the true residual)
 // ST_RESTART: this round restarts from the recomputed residual (see k_fin_init)
 // ST_LAST: how the round before a restart ended (converged or broke down)
+// ST_PROJ: the residual's closed-zone means are to be removed after this iteration
 enum {
     ST_ACTIVE = 0, ST_ITERS, ST_MAXIT, ST_STATUS, ST_RULE, ST_ZEROX, ST_XPEND, ST_FLEX, ST_VC, ST_ITOT, ST_RESTART,
-    ST_LAST, ST_NSTATE = 12
+    ST_LAST, ST_PROJ, ST_NSTATE = 16
 };
 
 // One system of a batch.  Level 0 is swept by columns: a block takes TB

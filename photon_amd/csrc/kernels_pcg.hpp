@@ -609,6 +609,8 @@ __global__ void __launch_bounds__(TB) k_fin_init(const DSys* __restrict__ sys, c
         s.scal[S_RR] = res[1];
         s.scal[S_MAXR] = m;
         s.scal[S_ZS] = s.scal[S_ZSN] = vcycle_scale(res[1], s.L[0].n);
+        s.scal[S_PNEXT] = 1e-6;
+        s.state[ST_PROJ] = 0;
         s.state[ST_ITERS] = 0;
         s.state[ST_XPEND] = 0;
         const bool done = floor_hit || stop_reached(s, res[1], m);
@@ -649,6 +651,7 @@ __global__ void __launch_bounds__(TB) k_fin_alpha(const DSys* __restrict__ sys, 
         } else {
             s.scal[S_ALPHA] = s.scal[S_RZ] / dq;
             s.scal[S_ZS] = s.scal[S_ZSN];   // z of the previous V-cycle has been consumed
+            s.state[ST_PROJ] = 0;           // r's closed-zone means, if due, were removed
             s.state[ST_VC] = 1;             // q is defined from here on
         }
     }
@@ -683,6 +686,12 @@ __global__ void __launch_bounds__(TB) k_fin_beta(const DSys* __restrict__ sys, c
         if (stop_reached(s, res[1], m)) status = 1;
         else if (it >= s.state[ST_MAXIT]) status = 2;
         else if (!(rz_new > 0)) status = 4;
+        // each further fall of ||r|| / ||b|| by 1e3 (from 1e-3 on) has k_proj_r
+        // clear the residual's closed-zone means once
+        if (!status && s.nclosed > 0 && res[1] <= s.scal[S_PNEXT] * s.scal[S_BB]) {
+            s.state[ST_PROJ] = 1;
+            s.scal[S_PNEXT] *= 1e-6;
+        }
         if (status) {
             s.state[ST_STATUS] = status;
             s.state[ST_ACTIVE] = 0;
@@ -1778,6 +1787,85 @@ __global__ void __launch_bounds__(TB) k_vc_coarse(const DSys* __restrict__ sys, 
             const DSys& s = sys[e.x + g];
             if (s.state[ST_ACTIVE]) st<VT>(s.L[s.nl - 1].t, i, (float)v);
         }
+    }
+}
+
+// ---------------------------------------------------------------------------
+// the residual kept in range(A).  On a closed zone A's null space holds the
+// constant vector, and b is free of it, so r is in exact arithmetic; but the
+// FP32 rounding of r -= alpha q adds a small constant per zone, taken on
+// mostly in the first iterations, when alpha q is large.  CG cannot reduce
+// it, and once the rest of r has fallen to its size the recursion loses
+// conjugacy: ||r|| grows by orders of magnitude until d.Ad <= 0 (a point
+// source in a closed room shows it near 1e-6).  The reference keeps r as it
+// comes (gasphase.cpp:603-627).  Here, when ||r|| / ||b|| first falls below
+// 1e-3, 1e-6, ... (k_fin_beta sets ST_PROJ; the next k_fin_alpha clears it),
+// r's mean on each closed zone is removed: fixed-order FP64 partial sums over
+// the chunks of the zone's cell list (the tasks of the final mean removal),
+// then the zone's mean formed from them in task order and subtracted in FP32,
+// as k_zone_sub does.  A block takes PROJ_TASKS tasks, so the launch of an
+// iteration without a projection stays small.
+// ---------------------------------------------------------------------------
+constexpr int PROJ_TASKS = 8;
+
+__global__ void __launch_bounds__(TB) k_proj_part(const DSys* __restrict__ sys, const int2* __restrict__ tab,
+                                                  int ntab) {
+    __shared__ double out[1];
+    for (int q = 0; q < PROJ_TASKS; ++q) {
+        const int idx = blockIdx.x * PROJ_TASKS + q;
+        if (idx >= ntab) break;
+        const int2 e = tab[idx];
+        const DSys& s = sys[e.x];
+        if (!st_uni(s, ST_PROJ)) continue;
+        const float* __restrict__ r = rbuf(s, st_uni(s, ST_ITERS));   // r of the iteration just finished
+        const int4 tk = s.ztask[e.y];
+        // a thread adds its cells in increasing order, eight loads issued before their use
+        double v[1] = {0.0};
+        int p = tk.y + threadIdx.x;
+        for (; p + 7 * TB < tk.z; p += 8 * TB) {
+            int c[8];
+            float x[8];
+#pragma unroll
+            for (int u = 0; u < 8; ++u) c[u] = s.zcells[p + u * TB];
+#pragma unroll
+            for (int u = 0; u < 8; ++u) x[u] = r[c[u]];
+#pragma unroll
+            for (int u = 0; u < 8; ++u) v[0] += (double)x[u];
+        }
+        for (; p < tk.z; p += TB) v[0] += (double)r[s.zcells[p]];
+        block_sums<1>(v, out);
+        __syncthreads();
+        if (threadIdx.x == 0) s.zpart[e.y] = out[0];
+        __syncthreads();
+    }
+}
+
+__global__ void __launch_bounds__(TB) k_proj_sub(const DSys* __restrict__ sys, const int2* __restrict__ tab,
+                                                 int ntab) {
+    for (int q = 0; q < PROJ_TASKS; ++q) {
+        const int idx = blockIdx.x * PROJ_TASKS + q;
+        if (idx >= ntab) break;
+        const int2 e = tab[idx];
+        const DSys& s = sys[e.x];
+        if (!st_uni(s, ST_PROJ)) continue;
+        float* __restrict__ r = rbuf(s, st_uni(s, ST_ITERS));
+        const int4 tk = s.ztask[e.y];
+        const int t0 = s.ztptr[tk.x], t1 = s.ztptr[tk.x + 1];
+        double acc = 0.0;
+        for (int t = t0; t < t1; ++t) acc += s.zpart[t];
+        const float mean = (float)(acc / (double)(s.ztask[t1 - 1].z - s.ztask[t0].y));
+        int p = tk.y + threadIdx.x;
+        for (; p + 7 * TB < tk.z; p += 8 * TB) {
+            int c[8];
+            float x[8];
+#pragma unroll
+            for (int u = 0; u < 8; ++u) c[u] = s.zcells[p + u * TB];
+#pragma unroll
+            for (int u = 0; u < 8; ++u) x[u] = r[c[u]];
+#pragma unroll
+            for (int u = 0; u < 8; ++u) r[c[u]] = x[u] - mean;
+        }
+        for (; p < tk.z; p += TB) r[s.zcells[p]] -= mean;
     }
 }
 

@@ -448,6 +448,11 @@ void cg_iteration(phg_ctx* ctx, BatchDev* B) {
     launch(ctx, K_FIN, k_fin_alpha, B->nsys, 0, S, B->tab_sys);
     vcycle(ctx, B, 0);
     launch(ctx, K_FIN, k_fin_beta, B->nsys, 0, S, B->tab_sys);
+    if (B->tab_z_len > 0) {
+        const int nb = (B->tab_z_len + PROJ_TASKS - 1) / PROJ_TASKS;
+        launch(ctx, K_ZONE, k_proj_part, nb, 0, S, (const int2*)B->tab_z, B->tab_z_len);
+        launch(ctx, K_ZONE, k_proj_sub, nb, 0, S, (const int2*)B->tab_z, B->tab_z_len);
+    }
 }
 
 // closed-zone mean removal of every system's x (whatever its state)
@@ -1164,6 +1169,9 @@ int phg_batch_solve(phg_ctx* ctx, int batch, const float* const* b, float* const
     const int check = P->check_every > 0 ? P->check_every : 4;
     const bool graph = P->use_graph && !ctx->prof;
     const bool whole = P->use_graph == 2 && !ctx->prof;
+    // after the whole-solve graph, the systems still running are the slow
+    // tail: check a quarter as far apart (at least 4 iterations)
+    const int tail_check = whole ? std::min(check, std::max(4, check / 4)) : check;
     // rounds: the first from x_0; each later one restarts the systems that
     // converged from their x and the true residual b - A x (those already
     // within tol stop at once); the cap applies per round
@@ -1209,7 +1217,7 @@ int phg_batch_solve(phg_ctx* ctx, int batch, const float* const* b, float* const
             HIP_OK(hipGraphInstantiate(&B->graph_exec, B->graph, nullptr, nullptr, 0));
         }
         while (live && done < P->max_iters) {
-            const int k = std::min(check, P->max_iters - done);
+            const int k = std::min(round == 0 ? tail_check : check, P->max_iters - done);
             for (int i = 0; i < k; ++i) {
                 if (graph) HIP_OK(hipGraphLaunch(B->graph_exec, ctx->stream));
                 else cg_iteration(ctx, B);
