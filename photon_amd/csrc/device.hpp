@@ -120,7 +120,7 @@ enum { S_RZ = 0, S_DQ, S_ALPHA, S_BETA, S_RR, S_BB, S_TOL, S_MAXR, S_INV, S_ZS, 
 // ST_ITOT: iterations of earlier rounds (restarts from the true residual)
 // ST_RESTART: this round restarts from the recomputed residual (see k_fin_init)
 // ST_LAST: how the round before a restart ended (converged or broke down)
-// ST_PROJ: the residual's closed-zone means are to be removed after this iteration
+// ST_PROJ: removal of the residual's closed-zone means (k_proj): 1 sums due, 2 subtraction due
 enum {
     ST_ACTIVE = 0, ST_ITERS, ST_MAXIT, ST_STATUS, ST_RULE, ST_ZEROX, ST_XPEND, ST_FLEX, ST_VC, ST_ITOT, ST_RESTART,
     ST_LAST, ST_PROJ, ST_NSTATE = 16

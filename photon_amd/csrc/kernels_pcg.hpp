@@ -651,7 +651,9 @@ __global__ void __launch_bounds__(TB) k_fin_alpha(const DSys* __restrict__ sys, 
         } else {
             s.scal[S_ALPHA] = s.scal[S_RZ] / dq;
             s.scal[S_ZS] = s.scal[S_ZSN];   // z of the previous V-cycle has been consumed
-            s.state[ST_PROJ] = 0;           // r's closed-zone means, if due, were removed
+            // k_proj: sums taken -> subtract this iteration; subtracted -> done
+            const int pj = s.state[ST_PROJ];
+            if (pj) s.state[ST_PROJ] = pj == 1 ? 2 : 0;
             s.state[ST_VC] = 1;             // q is defined from here on
         }
     }
@@ -688,7 +690,7 @@ __global__ void __launch_bounds__(TB) k_fin_beta(const DSys* __restrict__ sys, c
         else if (!(rz_new > 0)) status = 4;
         // each further fall of ||r|| / ||b|| by 1e3 (from 1e-3 on) has k_proj_r
         // clear the residual's closed-zone means once
-        if (!status && s.nclosed > 0 && res[1] <= s.scal[S_PNEXT] * s.scal[S_BB]) {
+        if (!status && s.nclosed > 0 && s.state[ST_PROJ] == 0 && res[1] <= s.scal[S_PNEXT] * s.scal[S_BB]) {
             s.state[ST_PROJ] = 1;
             s.scal[S_PNEXT] *= 1e-6;
         }
@@ -1799,73 +1801,66 @@ __global__ void __launch_bounds__(TB) k_vc_coarse(const DSys* __restrict__ sys, 
 // conjugacy: ||r|| grows by orders of magnitude until d.Ad <= 0 (a point
 // source in a closed room shows it near 1e-6).  The reference keeps r as it
 // comes (gasphase.cpp:603-627).  Here, when ||r|| / ||b|| first falls below
-// 1e-3, 1e-6, ... (k_fin_beta sets ST_PROJ; the next k_fin_alpha clears it),
-// r's mean on each closed zone is removed: fixed-order FP64 partial sums over
-// the chunks of the zone's cell list (the tasks of the final mean removal),
-// then the zone's mean formed from them in task order and subtracted in FP32,
-// as k_zone_sub does.  A block takes PROJ_TASKS tasks, so the launch of an
-// iteration without a projection stays small.
+// 1e-3, 1e-6, ..., r's mean on each closed zone is removed, over two
+// iterations with one launch of k_proj each (ST_PROJ: k_fin_beta sets 1 at
+// the crossing, k_fin_alpha turns 1 into 2 and 2 into 0):
+// - 1: fixed-order FP64 partial sums of r_k over the chunks of each zone's
+//   cell list (the tasks of the final mean removal);
+// - 2: each zone's mean formed from them in task order and subtracted in FP32
+//   from r_{k+1} = r_k - alpha q, as k_zone_sub does.  The mean of alpha q on
+//   a closed zone is rounding only, so this leaves one iteration's rounding.
+// A block takes PROJ_TASKS tasks, so the launch of an iteration without a
+// projection stays small.
 // ---------------------------------------------------------------------------
 constexpr int PROJ_TASKS = 8;
 
-__global__ void __launch_bounds__(TB) k_proj_part(const DSys* __restrict__ sys, const int2* __restrict__ tab,
-                                                  int ntab) {
+__global__ void __launch_bounds__(TB) k_proj(const DSys* __restrict__ sys, const int2* __restrict__ tab, int ntab) {
     __shared__ double out[1];
     for (int q = 0; q < PROJ_TASKS; ++q) {
         const int idx = blockIdx.x * PROJ_TASKS + q;
         if (idx >= ntab) break;
         const int2 e = tab[idx];
         const DSys& s = sys[e.x];
-        if (!st_uni(s, ST_PROJ)) continue;
-        const float* __restrict__ r = rbuf(s, st_uni(s, ST_ITERS));   // r of the iteration just finished
+        const int pj = st_uni(s, ST_PROJ);
+        if (!pj || !st_uni(s, ST_ACTIVE)) continue;
+        float* __restrict__ r = rbuf(s, st_uni(s, ST_ITERS));   // r of the iteration just finished
         const int4 tk = s.ztask[e.y];
-        // a thread adds its cells in increasing order, eight loads issued before their use
-        double v[1] = {0.0};
         int p = tk.y + threadIdx.x;
-        for (; p + 7 * TB < tk.z; p += 8 * TB) {
-            int c[8];
-            float x[8];
+        if (pj == 1) {
+            // a thread adds its cells in increasing order, eight loads issued before their use
+            double v[1] = {0.0};
+            for (; p + 7 * TB < tk.z; p += 8 * TB) {
+                int c[8];
+                float x[8];
 #pragma unroll
-            for (int u = 0; u < 8; ++u) c[u] = s.zcells[p + u * TB];
+                for (int u = 0; u < 8; ++u) c[u] = s.zcells[p + u * TB];
 #pragma unroll
-            for (int u = 0; u < 8; ++u) x[u] = r[c[u]];
+                for (int u = 0; u < 8; ++u) x[u] = r[c[u]];
 #pragma unroll
-            for (int u = 0; u < 8; ++u) v[0] += (double)x[u];
+                for (int u = 0; u < 8; ++u) v[0] += (double)x[u];
+            }
+            for (; p < tk.z; p += TB) v[0] += (double)r[s.zcells[p]];
+            block_sums<1>(v, out);
+            __syncthreads();
+            if (threadIdx.x == 0) s.zpart[e.y] = out[0];
+            __syncthreads();
+        } else {
+            const int t0 = s.ztptr[tk.x], t1 = s.ztptr[tk.x + 1];
+            double acc = 0.0;
+            for (int t = t0; t < t1; ++t) acc += s.zpart[t];
+            const float mean = (float)(acc / (double)(s.ztask[t1 - 1].z - s.ztask[t0].y));
+            for (; p + 7 * TB < tk.z; p += 8 * TB) {
+                int c[8];
+                float x[8];
+#pragma unroll
+                for (int u = 0; u < 8; ++u) c[u] = s.zcells[p + u * TB];
+#pragma unroll
+                for (int u = 0; u < 8; ++u) x[u] = r[c[u]];
+#pragma unroll
+                for (int u = 0; u < 8; ++u) r[c[u]] = x[u] - mean;
+            }
+            for (; p < tk.z; p += TB) r[s.zcells[p]] -= mean;
         }
-        for (; p < tk.z; p += TB) v[0] += (double)r[s.zcells[p]];
-        block_sums<1>(v, out);
-        __syncthreads();
-        if (threadIdx.x == 0) s.zpart[e.y] = out[0];
-        __syncthreads();
-    }
-}
-
-__global__ void __launch_bounds__(TB) k_proj_sub(const DSys* __restrict__ sys, const int2* __restrict__ tab,
-                                                 int ntab) {
-    for (int q = 0; q < PROJ_TASKS; ++q) {
-        const int idx = blockIdx.x * PROJ_TASKS + q;
-        if (idx >= ntab) break;
-        const int2 e = tab[idx];
-        const DSys& s = sys[e.x];
-        if (!st_uni(s, ST_PROJ)) continue;
-        float* __restrict__ r = rbuf(s, st_uni(s, ST_ITERS));
-        const int4 tk = s.ztask[e.y];
-        const int t0 = s.ztptr[tk.x], t1 = s.ztptr[tk.x + 1];
-        double acc = 0.0;
-        for (int t = t0; t < t1; ++t) acc += s.zpart[t];
-        const float mean = (float)(acc / (double)(s.ztask[t1 - 1].z - s.ztask[t0].y));
-        int p = tk.y + threadIdx.x;
-        for (; p + 7 * TB < tk.z; p += 8 * TB) {
-            int c[8];
-            float x[8];
-#pragma unroll
-            for (int u = 0; u < 8; ++u) c[u] = s.zcells[p + u * TB];
-#pragma unroll
-            for (int u = 0; u < 8; ++u) x[u] = r[c[u]];
-#pragma unroll
-            for (int u = 0; u < 8; ++u) r[c[u]] = x[u] - mean;
-        }
-        for (; p < tk.z; p += TB) r[s.zcells[p]] -= mean;
     }
 }
 
