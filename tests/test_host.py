@@ -91,6 +91,45 @@ def test_abi_exports_every_declared_symbol():
     assert declared <= set(gas.EXPORTS)
 
 
+def test_abi_builds_from_cpp_and_c(tmp_path):
+    """The header compiles as C and as C++, and a C++ caller (as the reference's
+    Solver would be) links the library and runs its host-only half of grid
+    set-up; without a GPU, phg_create refuses with an error code."""
+    import shutil
+    import subprocess
+    if not shutil.which("g++") or not shutil.which("gcc"):
+        pytest.skip("no host compiler")
+    inc = os.path.join(ROOT, "include")
+    src = tmp_path / "caller.cpp"
+    src.write_text(r'''
+#include "photon_gas.h"
+#include <cstdio>
+#include <vector>
+int main() {
+    const int nx = 6, ny = 5, nz = 4, n = nx * ny * nz;
+    std::vector<float> ax((nx + 1) * ny * nz, 1.0f), ay(nx * (ny + 1) * nz, 1.0f), az(nx * ny * (nz + 1), 1.0f);
+    std::vector<uint8_t> blocked(n, 0);
+    phg_grid_plan* plan = nullptr;
+    const int rc = phg_grid_plan_build(nx, ny, nz, ax.data(), ay.data(), az.data(), blocked.data(), nullptr, &plan);
+    std::printf("plan %d %d\n", rc, plan != nullptr);
+    phg_grid_plan_free(plan);
+    phg_ctx* ctx = nullptr;
+    std::printf("create %d\n", phg_create(&ctx, 0));
+    if (ctx) phg_destroy(ctx);
+    return 0;
+}
+''')
+    exe = tmp_path / "caller"
+    libdir = os.path.dirname(gas.LIB_PATH)
+    subprocess.run(["g++", "-std=c++17", "-I", inc, str(src), "-L", libdir, "-lphoton_gas",
+                    f"-Wl,-rpath,{libdir}", "-o", str(exe)], check=True, timeout=120)
+    out = subprocess.run([str(exe)], capture_output=True, text=True, timeout=120).stdout.split("\n")
+    assert out[0] == "plan 0 1"
+    assert out[1].startswith("create ")
+    subprocess.run(["gcc", "-x", "c", "-std=c99", "-fsyntax-only", "-I", inc, "-"], input='#include "photon_gas.h"\n',
+                   text=True, check=True, timeout=60)
+
+
 def test_no_gpu_gives_clear_error_not_fallback():
     # in a container without a GPU the context must refuse, never compute on the CPU
     import subprocess
