@@ -135,7 +135,10 @@ struct DSys {
     int tiles0;                   // level-0 blocks (stride of the partials)
     int kchunk;                   // planes per level-0 block
     int nu0;                      // level-0 red-black sweeps before and after the coarse correction
-    const double* cinv;           // nc*nc FP64 inverse of the grounded coarsest operator
+    const double* cinv;           // nc*nc FP64 inverse of the grounded coarsest operator (nc <= CP_MIN)
+    const double* cinvp;          // nc > CP_MIN: its upper triangle as 64x64 tiles (k_vc_coarse_p)
+    int cnt;                      // 64-row strips of a packed inverse (0: cinv is read whole)
+    double* cpart;                // [cnt][cnt][64]: block (R, S) of the inverse times strip S of b
     const float* b;               // outer CG vectors, FP32 on level-0 cells
     float* x;
     float* r;                     // residual, double-buffered: r_k lives in r (k even) or r1 (k odd),

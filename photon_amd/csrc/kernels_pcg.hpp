@@ -1792,6 +1792,123 @@ __global__ void __launch_bounds__(TB) k_vc_coarse(const DSys* __restrict__ sys, 
     }
 }
 
+// Coarsest level of a grid with more than CP_MIN unknowns: the inverse is
+// exactly symmetric, so only its upper triangle is stored, as 64x64 tiles
+// (I <= J, tile-major in row order, each column-major: element (r, c) of
+// tile (I, J) is B[64 I + r][64 J + c], zero past nc).  A block takes one tile
+// and up to CGS systems of its grid: it reads the tile once (lane = row,
+// coalesced along a column) and forms both of its products, B_IJ b_J for the
+// rows of strip I and, for I < J, B_IJ^T b_I for the rows of strip J, through
+// a transposed copy in LDS.  Each product sums the block's 64 terms in wave
+// order; k_vc_coarse_r then adds the 64-row products of a strip over all
+// strips S in order, so t_R = sum_S B_RS b_S.  Every element of the inverse
+// is read once per group instead of twice.
+constexpr int CP_MIN = 1024;
+
+__host__ __device__ constexpr size_t coarse_p_lds(int cgs) {
+    return (size_t)64 * 65 * 8 + (size_t)2 * 64 * cgs * 4 + (size_t)2 * 3 * 64 * cgs * 8;
+}
+
+template <class VT, int CGS>
+__global__ void __launch_bounds__(TB) k_vc_coarse_p(const DSys* __restrict__ sys, const int4* __restrict__ tab) {
+    const int4 e = tab[blockIdx.x];   // first system, count, strip I, strip J >= I
+    const DSys& s0 = sys[e.x];
+    int live = 0;
+    for (int g = 0; g < e.y; ++g) live |= sys[e.x + g].state[ST_ACTIVE];
+    if (!live) return;
+    const int n = s0.nc, nt = s0.cnt, I = e.z, J = e.w;
+    extern __shared__ double dsh[];
+    double* tl = dsh;                                           // [c][65]: column c of the tile
+    float* bI = reinterpret_cast<float*>(dsh + 64 * 65);       // [64][CGS]
+    float* bJ = bI + 64 * CGS;                                  // [64][CGS]
+    double* redr = reinterpret_cast<double*>(bJ + 64 * CGS);   // [3][64][CGS]
+    double* redc = redr + 3 * 64 * CGS;                         // [3][64][CGS]
+    for (int q = threadIdx.x; q < 64 * CGS; q += TB) {
+        const int r = q / CGS, g = q - r * CGS;
+        float vi = 0.0f, vj = 0.0f;
+        if (g < e.y) {
+            const DSys& s = sys[e.x + g];
+            const void* b = s.L[s.nl - 1].b;
+            if (64 * I + r < n) vi = ld<VT>(b, 64 * I + r);
+            if (64 * J + r < n) vj = ld<VT>(b, 64 * J + r);
+        }
+        bI[q] = vi;
+        bJ[q] = vj;
+    }
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    const size_t tix = (size_t)I * nt - (size_t)I * (I - 1) / 2 + (J - I);
+    const double* __restrict__ T = s0.cinvp + tix * 4096;
+    double a[16];   // row lane, columns 16 w .. 16 w + 15
+#pragma unroll
+    for (int i = 0; i < 16; ++i) a[i] = gld(T, (unsigned)((16 * w + i) * 64 + lane));
+#pragma unroll
+    for (int i = 0; i < 16; ++i) tl[(16 * w + i) * 65 + lane] = a[i];
+    __syncthreads();
+    double ar[CGS], ac[CGS];
+#pragma unroll
+    for (int g = 0; g < CGS; ++g) ar[g] = ac[g] = 0.0;
+#pragma unroll
+    for (int i = 0; i < 16; ++i) {
+        const float* bj = bJ + (16 * w + i) * CGS;
+#pragma unroll
+        for (int g = 0; g < CGS; ++g) ar[g] += a[i] * (double)bj[g];
+    }
+    if (I < J) {
+#pragma unroll
+        for (int i = 0; i < 16; ++i) {
+            const double t = tl[lane * 65 + 16 * w + i];   // element (16 w + i, lane)
+            const float* bi = bI + (16 * w + i) * CGS;
+#pragma unroll
+            for (int g = 0; g < CGS; ++g) ac[g] += t * (double)bi[g];
+        }
+    }
+    if (w > 0) {
+#pragma unroll
+        for (int g = 0; g < CGS; ++g) {
+            redr[((w - 1) * 64 + lane) * CGS + g] = ar[g];
+            redc[((w - 1) * 64 + lane) * CGS + g] = ac[g];
+        }
+    }
+    __syncthreads();
+    if (w == 0) {
+#pragma unroll
+        for (int g = 0; g < CGS; ++g) {
+            if (g >= e.y) break;
+            double* P = sys[e.x + g].cpart;
+            P[((size_t)I * nt + J) * 64 + lane] = ar[g] + redr[(0 * 64 + lane) * CGS + g] +
+                                                  redr[(1 * 64 + lane) * CGS + g] + redr[(2 * 64 + lane) * CGS + g];
+            if (I < J)
+                P[((size_t)J * nt + I) * 64 + lane] = ac[g] + redc[(0 * 64 + lane) * CGS + g] +
+                                                      redc[(1 * 64 + lane) * CGS + g] + redc[(2 * 64 + lane) * CGS + g];
+        }
+    }
+}
+
+// t_R = sum over strips S, in order, of the products k_vc_coarse_p left;
+// a wave per system of the group, lane = row
+template <class VT>
+__global__ void __launch_bounds__(TB) k_vc_coarse_r(const DSys* __restrict__ sys, const int4* __restrict__ tab) {
+    const int4 e = tab[blockIdx.x];   // first system, count, strip R
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    for (int g = w; g < e.y; g += TB / 64) {
+        const DSys& s = sys[e.x + g];
+        if (!s.state[ST_ACTIVE]) continue;
+        const int nt = s.cnt, i = 64 * e.z + lane;
+        const double* __restrict__ P = s.cpart + (size_t)e.z * nt * 64 + lane;
+        double y = 0.0;
+        int S = 0;
+        for (; S + 8 <= nt; S += 8) {
+            double p[8];
+#pragma unroll
+            for (int u = 0; u < 8; ++u) p[u] = P[(size_t)(S + u) * 64];
+#pragma unroll
+            for (int u = 0; u < 8; ++u) y += p[u];
+        }
+        for (; S < nt; ++S) y += P[(size_t)S * 64];
+        if (i < s.nc) st<VT>(s.L[s.nl - 1].t, i, (float)y);
+    }
+}
+
 // ---------------------------------------------------------------------------
 // the residual kept in range(A).  On a closed zone A's null space holds the
 // constant vector, and b is free of it, so r is in exact arithmetic; but the

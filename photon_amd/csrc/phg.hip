@@ -55,7 +55,9 @@ struct GridDev {
         int* cidx = nullptr;
     };
     std::vector<Lev> lv;
-    double* cinv = nullptr;
+    double* cinv = nullptr;       // the coarsest inverse whole (nc <= CP_MIN) ...
+    double* cinvp = nullptr;      // ... or its upper triangle as 64x64 tiles
+    int cnt = 0;                  // strips of cinvp (0: cinv)
     float* invd = nullptr;        // 1/diag of level 0, 0 where diag <= 0
     uint8_t* mask = nullptr;      // level-0 face mask when every coefficient is 0 or 1
     bool binary = false;
@@ -116,6 +118,11 @@ struct BatchDev {
     int4* tab_coarse = nullptr;    // (first system, count, row tile): coarsest-solve blocks
     int tab_coarse_len = 0;
     int cg_sys = 1;                // systems per coarsest-solve block (64 KB of LDS)
+    int4* tab_cp = nullptr;        // packed inverses: (first system, count, strip I, strip J >= I)
+    int tab_cp_len = 0;
+    int4* tab_cr = nullptr;        // packed inverses: (first system, count, strip)
+    int tab_cr_len = 0;
+    int cp_sys = 1;                // systems per packed-solve block
     double* scal = nullptr;
     int* state = nullptr;
     int* d_count = nullptr;
@@ -368,6 +375,17 @@ void vcycle_t(phg_ctx* ctx, BatchDev* B, int first) {
         case 4: coarse(k_vc_coarse<VT, 4>, 4); break;
         case CG_SYS_MAX / 2: coarse(k_vc_coarse<VT, CG_SYS_MAX / 2>, CG_SYS_MAX / 2); break;
         default: coarse(k_vc_coarse<VT, CG_SYS_MAX>, CG_SYS_MAX); break;
+    }
+    if (B->tab_cp_len > 0) {
+        auto packed = [&](auto kern, int cgs) {
+            launch(ctx, K_VC_COARSE, kern, B->tab_cp_len, coarse_p_lds(cgs), S, (const int4*)B->tab_cp);
+        };
+        switch (B->cp_sys) {
+            case 1: packed(k_vc_coarse_p<VT, 1>, 1); break;
+            case 4: packed(k_vc_coarse_p<VT, 4>, 4); break;
+            default: packed(k_vc_coarse_p<VT, 8>, 8); break;
+        }
+        launch(ctx, K_VC_COARSE, k_vc_coarse_r<VT>, B->tab_cr_len, 0, S, (const int4*)B->tab_cr);
     }
     for (int l = B->maxl - 2; l >= 1; --l) {
         launch(ctx, K_VC_PROLONG, k_vc_prolong<VT>, NG[l], 0, S, TG[l], l);
@@ -761,7 +779,21 @@ int phg_grid_create_planned(phg_ctx* ctx, int nx, int ny, int nz, float dx, cons
             UP(D.cidx, L.cidx.data(), L.cidx.size())
         }
     }
-    UP(G->cinv, G->H.cinv.data(), G->H.cinv.size())
+    if (G->H.nc > CP_MIN) {
+        // the upper triangle in tiles (I <= J) of 64x64, each column-major
+        const int nc = G->H.nc, nt = (nc + 63) / 64;
+        std::vector<double> tp((size_t)nt * (nt + 1) / 2 * 4096, 0.0);
+        size_t tix = 0;
+        for (int I = 0; I < nt; ++I)
+            for (int J = I; J < nt; ++J, ++tix)
+                for (int c = 0; c < 64 && 64 * J + c < nc; ++c)
+                    for (int r = 0; r < 64 && 64 * I + r < nc; ++r)
+                        tp[tix * 4096 + (size_t)c * 64 + r] = G->H.cinv[(size_t)(64 * I + r) * nc + 64 * J + c];
+        UP(G->cinvp, tp.data(), tp.size())
+        G->cnt = nt;
+    } else {
+        UP(G->cinv, G->H.cinv.data(), G->H.cinv.size())
+    }
     {
         std::vector<float> invd(G->n, 0.0f);
         for (int c = 0; c < G->n; ++c)
@@ -951,6 +983,8 @@ int phg_batch_create_ex(phg_ctx* ctx, int nsys, const int* grids, const phg_batc
         S.nu0 = nu;
         S.tiles0 = tiles(G->nx * G->ny) * ((G->nz + S.kchunk - 1) / S.kchunk);
         S.cinv = G->cinv;
+        S.cinvp = G->cinvp;
+        S.cnt = G->cnt;
         S.invd = G->invd;
         S.scal = B->scal + (size_t)s * S_NSCAL;
         S.state = B->state + (size_t)s * ST_NSTATE;
@@ -972,6 +1006,7 @@ int phg_batch_create_ex(phg_ctx* ctx, int nsys, const int* grids, const phg_batc
         S.r1 = v4 + 5 * n0;
         S.z2 = v4 + 6 * n0;
         rc = dalloc(ctx, B->allocs, &S.part, (size_t)S.tiles0 * NPART);
+        if (!rc && S.cnt) rc = dalloc(ctx, B->allocs, &S.cpart, (size_t)S.cnt * S.cnt * 64);
         if (!rc) rc = dalloc(ctx, B->allocs, &S.zpart, (size_t)std::max(G->nztask, 1));
         if (!rc) rc = dalloc(ctx, B->allocs, &S.zmean, (size_t)std::max(G->nclosed, 1));
         if (rc) break;
@@ -1090,25 +1125,41 @@ int phg_batch_create_ex(phg_ctx* ctx, int nsys, const int* grids, const phg_batc
         // coarsest-solve blocks: runs of up to cg_sys consecutive systems on one grid
         std::vector<int4> tc;
         // systems per block: the longest run on one grid rounded up to 1, 4, 8
-        // or 16 (16 while their right-hand sides fit 64 KB of LDS, else 8)
-        int maxrun = 1;
+        // or 16 (16 while their right-hand sides fit 64 KB of LDS, else 8), for
+        // the grids whose inverse is read whole and, up to 8, for the packed ones
+        int maxrun = 1, maxrun_p = 1;
         for (int s = 0; s < nsys;) {
             int cnt = 1;
             while (s + cnt < nsys && grids[s + cnt] == grids[s]) ++cnt;
-            maxrun = std::max(maxrun, cnt);
+            if (get_grid(ctx, grids[s])->cnt) maxrun_p = std::max(maxrun_p, cnt);
+            else maxrun = std::max(maxrun, cnt);
             s += cnt;
         }
         const int cap = coarse_lds(CG_SYS_MAX, B->maxnc) <= 65536 ? CG_SYS_MAX : CG_SYS_MAX / 2;
         B->cg_sys = maxrun <= 1 ? 1 : maxrun <= 4 ? 4 : maxrun <= CG_SYS_MAX / 2 ? CG_SYS_MAX / 2 : cap;
+        B->cp_sys = maxrun_p <= 1 ? 1 : maxrun_p <= 4 ? 4 : 8;
+        std::vector<int4> tp, tr;
         for (int s = 0; s < nsys;) {
+            const GridDev* G = get_grid(ctx, grids[s]);
+            const int lim = G->cnt ? B->cp_sys : B->cg_sys;
             int cnt = 1;
-            while (s + cnt < nsys && cnt < B->cg_sys && grids[s + cnt] == grids[s]) ++cnt;
-            const int nc = get_grid(ctx, grids[s])->H.nc;
-            for (int t = 0; t < (nc + CG_ROWS - 1) / CG_ROWS; ++t) tc.push_back(make_int4(s, cnt, t, 0));
+            while (s + cnt < nsys && cnt < lim && grids[s + cnt] == grids[s]) ++cnt;
+            if (G->cnt) {
+                for (int I = 0; I < G->cnt; ++I) {
+                    for (int J = I; J < G->cnt; ++J) tp.push_back(make_int4(s, cnt, I, J));
+                    tr.push_back(make_int4(s, cnt, I, 0));
+                }
+            } else {
+                for (int t = 0; t < (G->H.nc + CG_ROWS - 1) / CG_ROWS; ++t) tc.push_back(make_int4(s, cnt, t, 0));
+            }
             s += cnt;
         }
         B->tab_coarse_len = (int)tc.size();
-        rc = upload(ctx, B->allocs, &B->tab_coarse, tc.data(), tc.size());
+        B->tab_cp_len = (int)tp.size();
+        B->tab_cr_len = (int)tr.size();
+        if (!tc.empty()) rc = upload(ctx, B->allocs, &B->tab_coarse, tc.data(), tc.size());
+        if (!rc && !tp.empty()) rc = upload(ctx, B->allocs, &B->tab_cp, tp.data(), tp.size());
+        if (!rc && !tr.empty()) rc = upload(ctx, B->allocs, &B->tab_cr, tr.data(), tr.size());
     }
     if (!rc) rc = upload_sys(ctx, B);
     if (!rc) rc = hipStreamSynchronize(ctx->stream) == hipSuccess ? 0 : fail(ctx, PHG_EHIP, "sync");

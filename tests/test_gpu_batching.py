@@ -96,6 +96,27 @@ def test_point_source_in_closed_zone_converges_in_every_precision(ctx, scenarios
     assert max(its.values()) <= its[gas.PREC_FP32] + 2, its
 
 
+def test_packed_coarsest_inverse_shared_by_systems(ctx, scenarios):
+    """Scenario 1's coarsest level has 1441 unknowns, so its inverse is stored
+    as the upper triangle in tiles and each tile serves both of its products.
+    Five systems on that grid share every tile read (the 8-system variant);
+    each equals its solve alone bit for bit and the FP64 oracle within 1e-5."""
+    grids, _ = scenarios
+    g = grids[1]
+    assert gas.hierarchy_shape(g)[0][-1] > 1024
+    bs = [sc.rhs_from_divergence(g, sc.burner_field(g, 40 + s), DT) for s in range(5)]
+    G = gas.Grid(ctx, g, precision=gas.PREC_MIXED_F16)
+    p = gas.solve_params(tol=1e-6, max_iters=300, zero_x=True, refine=2)
+    xs, stats = solve(ctx, [G] * len(bs), bs, p)
+    H = po.Hierarchy(g)
+    for k, b in enumerate(bs):
+        solo, st = solve(ctx, [G], [b], p)
+        assert st[0]["iters"] == stats[k]["iters"] and np.array_equal(solo[0], xs[k]), k
+        x64, _, _ = H.pcg64(b)
+        assert stats[k]["status"] == gas.CONVERGED and rel(xs[k], x64) < 1e-5, k
+    G.destroy()
+
+
 def test_two_sweeps_take_fewer_iterations(ctx, scenarios):
     grids, rhs = scenarios
     G = gas.Grid(ctx, grids[5])
