@@ -303,6 +303,8 @@ constexpr int kDenseMax = 2048;
 constexpr int kZoneChunk = 8192;
 // level-0 blocks a launch should offer the chip (256 CUs)
 constexpr int kLevel0Blocks = 4096;
+// whole-inverse coarsest-solve blocks below which its groups are made smaller
+constexpr int kCoarseBlocks = 256;
 // most level-0 red-black sweeps per side of the V-cycle
 constexpr int kMaxSweeps0 = 4;
 
@@ -1126,7 +1128,9 @@ int phg_batch_create_ex(phg_ctx* ctx, int nsys, const int* grids, const phg_batc
         std::vector<int4> tc;
         // systems per block: the longest run on one grid rounded up to 1, 4, 8
         // or 16 (16 while their right-hand sides fit 64 KB of LDS, else 8), for
-        // the grids whose inverse is read whole and, up to 8, for the packed ones
+        // the grids whose inverse is read whole, then smaller while that launch
+        // would offer the chip fewer than kCoarseBlocks blocks (the grouping
+        // leaves every system's sums as they are); up to 8 for the packed ones
         int maxrun = 1, maxrun_p = 1;
         for (int s = 0; s < nsys;) {
             int cnt = 1;
@@ -1137,6 +1141,19 @@ int phg_batch_create_ex(phg_ctx* ctx, int nsys, const int* grids, const phg_batc
         }
         const int cap = coarse_lds(CG_SYS_MAX, B->maxnc) <= 65536 ? CG_SYS_MAX : CG_SYS_MAX / 2;
         B->cg_sys = maxrun <= 1 ? 1 : maxrun <= 4 ? 4 : maxrun <= CG_SYS_MAX / 2 ? CG_SYS_MAX / 2 : cap;
+        auto coarse_blocks = [&](int cgs) {
+            long nb = 0;
+            for (int s = 0; s < nsys;) {
+                int cnt = 1;
+                while (s + cnt < nsys && grids[s + cnt] == grids[s]) ++cnt;
+                const GridDev* G = get_grid(ctx, grids[s]);
+                if (!G->cnt) nb += (long)((cnt + cgs - 1) / cgs) * ((G->H.nc + CG_ROWS - 1) / CG_ROWS);
+                s += cnt;
+            }
+            return nb;
+        };
+        while (B->cg_sys > 1 && coarse_blocks(B->cg_sys) < kCoarseBlocks)
+            B->cg_sys = B->cg_sys > CG_SYS_MAX / 2 ? CG_SYS_MAX / 2 : B->cg_sys > 4 ? 4 : 1;
         B->cp_sys = maxrun_p <= 1 ? 1 : maxrun_p <= 4 ? 4 : 8;
         std::vector<int4> tp, tr;
         for (int s = 0; s < nsys;) {
