@@ -1100,15 +1100,28 @@ int phg_batch_create_ex(phg_ctx* ctx, int nsys, const int* grids, const phg_batc
         if (!rc && !pu.empty()) rc = upload(ctx, B->allocs, &B->tabu, pu.data(), pu.size());
     }
     for (int l = 1; l < B->maxl && !rc; ++l) {
+        // groups of CV_GRP, or of 2 or 1 while the level's launches would offer
+        // the chip fewer than kCoarseBlocks blocks (each system's arithmetic is
+        // the same in any group)
+        auto level_tiles = [&](int grp, std::vector<int4>* out) {
+            long nb = 0;
+            for (int s = 0; s < nsys;) {
+                GridDev* G = get_grid(ctx, grids[s]);
+                int cnt = 1;
+                while (s + cnt < nsys && cnt < grp && grids[s + cnt] == grids[s]) ++cnt;
+                if ((int)G->lv.size() > l) {
+                    nb += tiles(G->lv[l].n);
+                    if (out)
+                        for (int k = 0; k < tiles(G->lv[l].n); ++k) out->push_back(make_int4(s, cnt, k, 0));
+                }
+                s += cnt;
+            }
+            return nb;
+        };
+        int grp = CV_GRP;
+        while (grp > 1 && level_tiles(grp, nullptr) < kCoarseBlocks) grp /= 2;
         std::vector<int4> logical;
-        for (int s = 0; s < nsys;) {
-            GridDev* G = get_grid(ctx, grids[s]);
-            int cnt = 1;
-            while (s + cnt < nsys && cnt < CV_GRP && grids[s + cnt] == grids[s]) ++cnt;
-            if ((int)G->lv.size() > l)
-                for (int k = 0; k < tiles(G->lv[l].n); ++k) logical.push_back(make_int4(s, cnt, k, 0));
-            s += cnt;
-        }
+        level_tiles(grp, &logical);
         std::vector<int4> phys = xcd_order(logical);
         B->tabg_len[l] = (int)phys.size();
         if (!phys.empty()) rc = upload(ctx, B->allocs, &B->tabg[l], phys.data(), phys.size());
