@@ -162,7 +162,9 @@ typedef struct {
     const int *sys_sweeps0;  /* per system (nsys entries, 0 = batch value) or NULL */
     int fused_threads;   /* threads (and own columns) per block of the level-0 passes that stage
                             halo rows (the fused smoothing passes, the direction update):
-                            0 = 512, or 256 (same results bit for bit) */
+                            0 = 512 (1024 for the smoothing of grids wider than 128 cells), or
+                            256, 512 or 1024 for every such pass (1024: the direction update
+                            keeps 512); every choice gives the same results bit for bit */
 } phg_batch_opts;
 int phg_batch_create_ex(phg_ctx *ctx, int nsys, const int *grids, const phg_batch_opts *opts, int *batch_out);
 int phg_batch_destroy(phg_ctx *ctx, int batch);

@@ -972,10 +972,12 @@ __global__ void __launch_bounds__(TB) k_post_red(const DSys* __restrict__ sys, c
 // step's second barrier for the same reason.
 constexpr int FUSE_MAX_NX = 192;   // staging pairs NT/2 + 2 nx <= 2 NT
 
-// A fused block has NT = TB or 2 TB threads and owns NT columns: the column
-// tile of its table entry and, for 2 TB, the next one (the table then lists
-// the even column tiles only).  A wider block stages fewer halo rows per own
-// column (TB + 4 nx against 2 TB + 4 nx for twice the columns).
+// A fused block has NT = TB, 2 TB or 4 TB threads and owns NT columns: the
+// column tile of its table entry and the NT / TB - 1 after it (the table then
+// lists every second or fourth column tile only).  A wider block stages fewer
+// halo rows per own column (TB + 4 nx against 2 TB + 4 nx for twice the
+// columns); 4 TB takes one staging and one smoothing pair per thread up to
+// nx = FUSE_MAX_NX, where 2 TB needs two staging pairs past nx = 128.
 // The passes keep rings of three planes in LDS.  Their strides are those of
 // the configuration's widest grid, so every access is a per-thread base plus a
 // constant, and the ring slot of each step is known at compile time (the march
@@ -983,7 +985,9 @@ constexpr int FUSE_MAX_NX = 192;   // staging pairs NT/2 + 2 nx <= 2 NT
 template <int V>
 using ic = std::integral_constant<int, V>;
 __host__ __device__ constexpr int fuse_nxmax(int nt, int p2, int p1) {   // widest nx of a configuration
-    return nt == 2 * TB ? (p2 == 1 ? 128 : FUSE_MAX_NX) : (p2 == 1 ? 64 : (p1 == 1 ? 127 : FUSE_MAX_NX));
+    return nt == 4 * TB   ? FUSE_MAX_NX
+           : nt == 2 * TB ? (p2 == 1 ? 128 : FUSE_MAX_NX)
+                          : (p2 == 1 ? 64 : (p1 == 1 ? 127 : FUSE_MAX_NX));
 }
 __host__ __device__ constexpr int fuse_sr(int nt, int nxm) { return nt + 4 * nxm; }       // staged values, masks
 __host__ __device__ constexpr int fuse_sz(int nt, int nxm) { return nt + 2 * nxm + 2; }   // smoothed values

@@ -97,9 +97,11 @@ struct BatchDev {
     int tabf_len = 0;
     int2* tabwf = nullptr;         // fused systems: even column tiles (2 TB-thread blocks)
     int tabwf_len = 0;
+    int2* tabqf = nullptr;         // fused systems on 4 TB-thread blocks: every fourth column tile
+    int tabqf_len = 0;
     int2* tabu = nullptr;          // the other systems: every column tile
     int tabu_len = 0;
-    int maxnx_f = 0;               // widest grid among the fused systems
+    int maxnx_f = 0;               // widest grid among the fused systems on TB or 2 TB threads
     int maxnu0_u = 0;              // most sweeps among the others
     size_t lds0() const { return 2 * (size_t)(TB + 2 * maxnx) * sizeof(float); }
     std::vector<int> grids;
@@ -108,7 +110,7 @@ struct BatchDev {
     std::vector<int2*> tab;        // level 0: (system, tile); tab[0] only
     int2* tabw = nullptr;          // level 0 entries of even column tiles (blocks of 2 TB columns)
     int tabw_len = 0;
-    int fuse_nt = 2 * TB;          // threads per block of the fused smoothing passes
+    int fuse_nt = 0;               // threads per block of the fused passes (0: 2 TB up to nx = 128, 4 TB above)
     std::vector<int> tab_len;
     std::vector<int4*> tabg;       // levels >= 1: (first system, count, tile) of same-grid groups
     std::vector<int> tabg_len;
@@ -335,11 +337,17 @@ void vcycle_t(phg_ctx* ctx, BatchDev* B, int first) {
                 launch(ctx, K_PRE_SMOOTH, first ? k1 : k0, B->tabf_len, lds, S, (const int2*)B->tabf);
         };
         const int mx = B->maxnx_f;
-        if (B->fuse_nt == 2 * TB) {
-            if (mx <= 128)
-                go(k_pre_fused<VT, true, 2 * TB, 1, 1>, k_pre_fused<VT, false, 2 * TB, 1, 1>, 2 * TB, 1, 1);
-            else
-                go(k_pre_fused<VT, true, 2 * TB, 2, 1>, k_pre_fused<VT, false, 2 * TB, 2, 1>, 2 * TB, 2, 1);
+        if (B->fuse_nt != TB) {
+            if (B->tabwf_len > 0) {
+                if (mx <= 128)
+                    go(k_pre_fused<VT, true, 2 * TB, 1, 1>, k_pre_fused<VT, false, 2 * TB, 1, 1>, 2 * TB, 1, 1);
+                else
+                    go(k_pre_fused<VT, true, 2 * TB, 2, 1>, k_pre_fused<VT, false, 2 * TB, 2, 1>, 2 * TB, 2, 1);
+            }
+            if (B->tabqf_len)
+                launch_nt(ctx, K_PRE_SMOOTH,
+                          first ? k_pre_fused<VT, true, 4 * TB, 1, 1> : k_pre_fused<VT, false, 4 * TB, 1, 1>,
+                          B->tabqf_len, 4 * TB, fuse_lds(4 * TB, 1, 1, false), S, (const int2*)B->tabqf);
         } else {
             if (mx <= 64)
                 go(k_pre_fused<VT, true, TB, 1, 1>, k_pre_fused<VT, false, TB, 1, 1>, TB, 1, 1);
@@ -405,11 +413,16 @@ void vcycle_t(phg_ctx* ctx, BatchDev* B, int first) {
                 launch(ctx, K_POST_SMOOTH, kern, B->tabf_len, lds, S, (const int2*)B->tabf);
         };
         const int mx = B->maxnx_f;
-        if (B->fuse_nt == 2 * TB) {
-            if (mx <= 128)
-                go(k_post_fused<VT, 2 * TB, 1, 1>, 2 * TB, 1, 1);
-            else
-                go(k_post_fused<VT, 2 * TB, 2, 1>, 2 * TB, 2, 1);
+        if (B->fuse_nt != TB) {
+            if (B->tabwf_len > 0) {
+                if (mx <= 128)
+                    go(k_post_fused<VT, 2 * TB, 1, 1>, 2 * TB, 1, 1);
+                else
+                    go(k_post_fused<VT, 2 * TB, 2, 1>, 2 * TB, 2, 1);
+            }
+            if (B->tabqf_len)
+                launch_nt(ctx, K_POST_SMOOTH, k_post_fused<VT, 4 * TB, 1, 1>, B->tabqf_len, 4 * TB,
+                          fuse_lds(4 * TB, 1, 1, true), S, (const int2*)B->tabqf);
         } else {
             if (mx <= 64)
                 go(k_post_fused<VT, TB, 1, 1>, TB, 1, 1);
@@ -447,7 +460,7 @@ void vcycle(phg_ctx* ctx, BatchDev* B, int first) {
 template <class OT>
 void dir_apply(phg_ctx* ctx, BatchDev* B) {
     const DSys* S = B->d_sys;
-    if (B->fuse_nt == 2 * TB) {
+    if (B->fuse_nt != TB) {
         const size_t lds = 2 * (size_t)(2 * TB + 2 * B->maxnx) * sizeof(float);
         const int2* TW = B->tabw;
         if (2 * B->maxnx <= 2 * TB)
@@ -921,7 +934,8 @@ int phg_batch_create_ex(phg_ctx* ctx, int nsys, const int* grids, const phg_batc
     if (!ctx || nsys <= 0 || !grids || !batch_out) return PHG_EINVAL;
     if (opts && (opts->kchunk < 0 || opts->level0_blocks < 0 || opts->unfused < 0 || opts->unfused > 1 ||
                  opts->sweeps0 < 0 || opts->sweeps0 > kMaxSweeps0 ||
-                 (opts->fused_threads != 0 && opts->fused_threads != TB && opts->fused_threads != 2 * TB)))
+                 (opts->fused_threads != 0 && opts->fused_threads != TB && opts->fused_threads != 2 * TB &&
+                  opts->fused_threads != 4 * TB)))
         return fail(ctx, PHG_EINVAL, "bad batch options");
     if (opts)
         for (int s = 0; s < nsys; ++s) {
@@ -934,7 +948,7 @@ int phg_batch_create_ex(phg_ctx* ctx, int nsys, const int* grids, const phg_batc
     B->nsys = nsys;
     B->grids.assign(grids, grids + nsys);
     B->unfused = opts && opts->unfused;
-    B->fuse_nt = opts && opts->fused_threads ? opts->fused_threads : 2 * TB;
+    B->fuse_nt = opts ? opts->fused_threads : 0;
     B->h_sys.resize(nsys);
     int rc = 0;
     long col_tiles = 0;
@@ -1077,26 +1091,33 @@ int phg_batch_create_ex(phg_ctx* ctx, int nsys, const int* grids, const phg_batc
         B->tabw_len = (int)wphys.size();
         if (!rc) rc = upload(ctx, B->allocs, &B->tabw, wphys.data(), wphys.size());
         // the smoothing split: which systems take the fused passes
-        std::vector<int2> lf, lwf, lu;
+        std::vector<int2> lf, lwf, lqf, lu;
         for (const int2& e : logical) {
             const GridDev* G = get_grid(ctx, grids[e.x]);
             const bool f = B->binary && !B->unfused && G->nx <= FUSE_MAX_NX && (G->nx * G->ny) % 2 == 0 &&
                            B->h_sys[e.x].nu0 == 1;
             if (f) {
                 lf.push_back(e);
-                if ((e.y % tiles(G->nx * G->ny)) % 2 == 0) lwf.push_back(e);
-                B->maxnx_f = std::max(B->maxnx_f, G->nx);
+                const int h = e.y % tiles(G->nx * G->ny);
+                if (B->fuse_nt == 4 * TB || (B->fuse_nt == 0 && G->nx > 128)) {
+                    if (h % 4 == 0) lqf.push_back(e);
+                } else {
+                    if (h % 2 == 0) lwf.push_back(e);
+                    B->maxnx_f = std::max(B->maxnx_f, G->nx);
+                }
             } else {
                 lu.push_back(e);
                 B->maxnu0_u = std::max(B->maxnu0_u, B->h_sys[e.x].nu0);
             }
         }
-        std::vector<int2> pf = xcd_order(lf), pwf = xcd_order(lwf), pu = xcd_order(lu);
+        std::vector<int2> pf = xcd_order(lf), pwf = xcd_order(lwf), pqf = xcd_order(lqf), pu = xcd_order(lu);
         B->tabf_len = (int)pf.size();
-        B->tabwf_len = (int)pwf.size();
+        B->tabwf_len = B->fuse_nt == TB ? 0 : (int)pwf.size();
+        B->tabqf_len = (int)pqf.size();
         B->tabu_len = (int)pu.size();
         if (!rc && !pf.empty()) rc = upload(ctx, B->allocs, &B->tabf, pf.data(), pf.size());
-        if (!rc && !pwf.empty()) rc = upload(ctx, B->allocs, &B->tabwf, pwf.data(), pwf.size());
+        if (!rc && B->tabwf_len) rc = upload(ctx, B->allocs, &B->tabwf, pwf.data(), pwf.size());
+        if (!rc && !pqf.empty()) rc = upload(ctx, B->allocs, &B->tabqf, pqf.data(), pqf.size());
         if (!rc && !pu.empty()) rc = upload(ctx, B->allocs, &B->tabu, pu.data(), pu.size());
     }
     for (int l = 1; l < B->maxl && !rc; ++l) {

@@ -129,15 +129,17 @@ ODD_NX = {"odd37": (11, 37, 22, 10), "odd101": (12, 101, 14, 8), "odd131": (13, 
 
 @pytest.mark.parametrize("prec", [gas.PREC_FP32, gas.PREC_MIXED_F16])
 @pytest.mark.parametrize("grid,kchunk,width", [("small", 3, 0), ("small", 0, 256), ("house", 7, 0), ("house", 40, 0),
-                                               ("house", 40, 256), ("odd37", 3, 0), ("odd101", 0, 256),
-                                               ("odd131", 4, 0), ("odd131", 4, 256)])
+                                               ("house", 40, 256), ("house", 40, 1024), ("odd37", 3, 0),
+                                               ("odd101", 0, 256), ("odd101", 0, 1024), ("odd131", 4, 0),
+                                               ("odd131", 4, 256), ("odd131", 4, 512)])
 def test_fused_presmoothing_is_bitwise_the_separate_passes(ctx, small, house, prec, grid, kchunk, width):
     """The fused smoothing passes form every value with the same operations
     as the separate half sweeps, so whole solves agree bit for bit, for
     k-chunks that cut the grid unevenly and for whole columns, on grids of
     many column tiles (their halo rows belong to other blocks), on grids of
     odd nx, where a column pair can hold the last cell of one row and the
-    first of the next, and for blocks of 512 (the default) and 256 threads."""
+    first of the next, and for blocks of 256, 512 and 1024 threads (the
+    default: 512, and 1024 for grids wider than 128 cells)."""
     if grid in ODD_NX:
         g = sc.house(*ODD_NX[grid])
         assert g.nx % 2 == 1 and (g.nx * g.ny) % 2 == 0
@@ -165,9 +167,11 @@ def test_fused_presmoothing_is_bitwise_the_separate_passes(ctx, small, house, pr
 @pytest.mark.parametrize("width", [0, 256])
 def test_batch_splits_smoothing_per_system(ctx, prec, width):
     """Fused passes are chosen per system: in one batch the grids of even
-    plane size and nx <= 192 smoothed once take them (in the configuration
-    the widest of them needs), while a grid of odd plane size, one wider than
-    192 cells and a system given two sweeps take the separate half sweeps.
+    plane size and nx <= 192 smoothed once take them (by default on 512-thread
+    blocks up to 128 cells wide and 1024-thread blocks above, each group in
+    the configuration its widest grid needs), while a grid of odd plane size,
+    one wider than 192 cells and a system given two sweeps take the separate
+    half sweeps.
     Every system's solve is bit for bit the all-separate batch's."""
     gs = [sc.house(21, 24, 18, 10), sc.house(22, 25, 19, 8), sc.house(23, 200, 12, 6),
           sc.house(24, 131, 10, 6), sc.house(25, 24, 18, 10)]
