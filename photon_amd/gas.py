@@ -392,14 +392,21 @@ class GasSolver:
     """Device-resident gas-phase fields of one grid with the reference's
     phase names: project(dt), advect_scalars(dt), energy_and_expansion(dt)
     (src/sim/gasphase.cpp:251-271, :505-677, :685-784).  Host numpy copies
-    are read with get(name) and written with set(name, array)."""
+    are read with get(name) and written with set(name, array).
+
+    With `graph_steps` (the default), project() runs each solve's set-up and
+    iterations as one captured graph covering the previous step's iteration
+    count rounded up to a multiple of 4: consecutive steps' solves need
+    similar counts, so the host waits about once per solve.  Replays give
+    the same results bit for bit as direct launches."""
 
     CELL_FIELDS = ("p", "S", "Pb", "Q", "massSrc", "nu") + FIELD_NAMES
 
     def __init__(self, ctx: Context, scen, precision=PREC_FP32, wind_closed=None, tol=1e-6, max_iters=400,
-                 stop_rule=STOP_REL_L2, plan: GridPlan | None = None):
+                 stop_rule=STOP_REL_L2, plan: GridPlan | None = None, graph_steps=True):
         self.ctx = ctx
         self.scen = scen
+        self.graph_steps = graph_steps
         self.grid = Grid(ctx, scen, precision, wind_closed, plan=plan)
         n = self.grid.n
         self.d = {}
@@ -427,6 +434,9 @@ class GasSolver:
         return self.d[name].download()
 
     def project(self, dt):
+        if self.graph_steps:
+            self.params.use_graph = 2
+            self.params.check_every = max(4, -(-(self.pressureIters + 1) // 4) * 4) if self.pressureIters > 0 else 8
         st = SolveStats()
         vm = C.c_float()
         d = self.d
