@@ -7,10 +7,12 @@ V-cycle, src/sim/pressure_mg.cpp:270-300).  Here a batch mixes grids of very
 different sizes; the best block shape depends on the grid (short columns want
 whole-column blocks, tall ones a split so that the launch fills the chip),
 and one more smoothing sweep trades level-0 passes per iteration against
-iterations.  The tuner times each candidate on the systems of one class as
-their own batch, with the same right-hand sides and stop rule as the real
-solve, and keeps the fastest time to solution.  Every candidate solves the
-same linear systems to the same tolerance, so the choice changes speed only.
+iterations.  A batch's time is set by the mix (its slowest systems and the
+launches they keep alive), so a candidate is judged inside the whole batch,
+with the same right-hand sides and stop rule as the real solve: coordinate
+descent from the library's defaults, one class at a time, largest first.
+Every candidate solves the same linear systems to the same tolerance, so the
+choice changes speed only.
 """
 from __future__ import annotations
 
@@ -37,9 +39,9 @@ def size_class(n_cells: int) -> str:
 
 @dataclass
 class Choice:
-    kchunk: int      # 0 = whole columns
+    kchunk: int      # 0 = whole columns, -1 = the library's own choice
     sweeps0: int
-    seconds: float   # time to solution of the class's systems as one batch
+    seconds: float   # time to solution of the whole batch with the table's settings
 
 
 def _kc(kchunk: int, nz: int) -> int:
@@ -79,29 +81,34 @@ def _time_batch(ctx, grids, bs, xs, params, kcs, sws, repeats):
         B.destroy()
 
 
-def tune(ctx, grids, bs, xs, params, repeats: int = 2, kchunks=KCHUNKS, sweeps=SWEEPS) -> dict:
-    """Time every (kchunk, sweeps0) candidate on each size class present in
-    `grids` (gas.Grid objects; bs, xs: device pointers or arrays of the
-    right-hand sides and solutions, one per grid).  Returns class name ->
-    Choice.  Candidates that fail to converge are never chosen."""
-    table = {}
-    for _, name in SIZE_CLASSES:
-        idx = [k for k, G in enumerate(grids) if size_class(G.scen.n) == name]
-        if not idx:
-            continue
-        g = [grids[k] for k in idx]
-        b = [bs[k] for k in idx]
-        x = [xs[k] for k in idx]
-        best = None
+def tune(ctx, grids, bs, xs, params, repeats: int = 2, kchunks=KCHUNKS, sweeps=SWEEPS, margin: float = 0.01) -> dict:
+    """Per size class present in `grids` (gas.Grid objects; bs, xs: device
+    pointers or arrays of the right-hand sides and solutions, one per grid),
+    the (kchunk, sweeps0) that makes the whole batch fastest.  Starting from
+    the library's defaults, each class in turn, largest first, tries every
+    candidate with the other classes held at their current choice, timing the
+    whole batch, and keeps one that converges and is faster by more than
+    `margin`.  Returns class name -> Choice (seconds: the batch's time with the
+    final table)."""
+    present = [name for _, name in SIZE_CLASSES if any(size_class(G.scen.n) == name for G in grids)]
+    table = {name: Choice(-1, 1, float("nan")) for name in present}
+
+    def run(tab):
+        kcs, sws = settings(grids, tab)
+        return _time_batch(ctx, grids, bs, xs, params, kcs, sws, repeats)
+
+    best, ok = run(table)
+    if not ok:
+        best = float("inf")
+    for name in reversed(present):
         for kc in kchunks:
             for nu in sweeps:
-                t, ok = _time_batch(ctx, g, b, x, params, [_kc(kc, G.scen.nz) for G in g], [nu] * len(g), repeats)
-                if ok and (best is None or t < best.seconds):
-                    best = Choice(kc, nu, t)
-        if best is None:
-            best = Choice(0, 1, float("nan"))
-        table[name] = best
-    return table
+                trial = dict(table)
+                trial[name] = Choice(kc, nu, float("nan"))
+                t, ok = run(trial)
+                if ok and t < best * (1.0 - margin):
+                    best, table = t, trial
+    return {name: Choice(c.kchunk, c.sweeps0, best) for name, c in table.items()}
 
 
 def default_table() -> dict:
@@ -110,6 +117,8 @@ def default_table() -> dict:
 
 
 def table_to_json(table: dict) -> dict:
-    return {k: {"kchunk": ("whole column" if v.kchunk == 0 else v.kchunk), "sweeps0": v.sweeps0,
-                "class_seconds": None if np.isnan(v.seconds) else round(v.seconds, 6)}
+    def kc(v):
+        return "whole column" if v == 0 else "library default" if v < 0 else v
+    return {k: {"kchunk": kc(v.kchunk), "sweeps0": v.sweeps0,
+                "batch_seconds": None if np.isnan(v.seconds) else round(v.seconds, 6)}
             for k, v in table.items()}

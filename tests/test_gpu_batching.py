@@ -159,8 +159,9 @@ def test_autotuner_picks_converging_settings(ctx):
     table = at.tune(ctx, Gs, dbs, dxs, p, repeats=1, kchunks=(8, 0), sweeps=(1, 2))
     classes = {at.size_class(g.n) for g in grids}
     assert set(table) == classes
-    for c in table.values():
-        assert c.kchunk in (8, 0) and c.sweeps0 in (1, 2) and np.isfinite(c.seconds)
+    for c in table.values():   # a tried candidate, or the library's defaults (-1, 1) when none beat them
+        assert (c.kchunk in (8, 0) and c.sweeps0 in (1, 2)) or (c.kchunk, c.sweeps0) == (-1, 1)
+        assert np.isfinite(c.seconds)
     kcs, sws = at.settings(Gs, table)
     B = gas.Batch(ctx, Gs, sys_kchunk=kcs, sys_sweeps0=sws)
     st = B.solve(dbs, dxs, p)
