@@ -289,10 +289,39 @@ def run_room(ctx, gas, sc, steps=300, warmup=20):
                         f"{warmup} warm-up steps then {steps} timed",
             "steps_per_s": round(steps / sec, 1), "ms_per_step": round(1e3 * sec / steps, 3),
             "pcg_iters_mean": round(float(np.mean(iters)), 2), "pcg_iters_max": int(np.max(iters)),
-            "T_max_K": round(float(T.max()), 2)}
+            "T_max_K": round(float(T.max()), 2), "_iters": list(iters), "_Q": s.get("Q")}
 
 
-def cpu_baseline(scen, bs_host, iters_ref, budget_s=15.0):
+def cpu_room(po, iters, Q, budget_s):
+    """The room case through the reference's own Solver phases on the host
+    (energyAndExpansion, project, advectScalars; the door's open boundary
+    leaves leakage nothing to shift), each step's solve run for the iteration
+    count the GPU run needed on that step.  One thread: with more, the
+    reference's job pool has stalled on this 12-plane grid (in project and
+    advectScalars)."""
+    from photon_amd import scenarios as sc
+    g = sc.room()
+    R = po.RefSolver(g, threads=1, pin=False)
+    R.set("Q", Q)
+    dt = 0.01
+    for k in range(20):   # the GPU run's warm-up
+        R.energy(dt)
+        R.project(dt, 0.0, int(iters[0]))
+        R.advect(dt)
+    done, used = 0, 0.0
+    while used < budget_s and done < len(iters):
+        t0 = time.perf_counter()
+        R.energy(dt)
+        R.project(dt, 0.0, int(iters[done]))
+        R.advect(dt)
+        used += time.perf_counter() - t0
+        done += 1
+    return {"steps_per_s": round(done / used, 1), "threads": R.threads,
+            "sample": f"{done} steps after 20 warm-up steps, each solve run for the GPU run's iteration count on that "
+                      f"step (mean {np.mean(iters[:done]):.2f})"}
+
+
+def cpu_baseline(scen, bs_host, iters_ref, budget_s=15.0, room=None):
     """The reference's own Solver::project (oracle/_ref, its GCC flags) on the
     host cores, each solve run for the iteration count the reference method
     needs to reach ||r||/||b|| <= 1e-6 on that RHS; falls back to the C
@@ -353,6 +382,8 @@ def cpu_baseline(scen, bs_host, iters_ref, budget_s=15.0):
     out = {"value": done / t_used, "unit": "solves/s", "cores": threads, "kind": kind, "sample": sample}
     if by_threads:
         out["by_threads"] = by_threads
+    if room is not None and kind == "reference":
+        out["room_c1"] = cpu_room(po, room["_iters"], room["_Q"], min(budget_s, 10.0))
     return out
 
 
@@ -445,10 +476,13 @@ def main():
             for s in range(nsample):
                 _, it, _ = H.pcg(bs[s], tol=TOL)
                 iters_ref.append(it)
-            cpu = cpu_baseline(scen, S_list[:nsample], np.array(iters_ref), args.cpu_budget)
+            cpu = cpu_baseline(scen, S_list[:nsample], np.array(iters_ref), args.cpu_budget, room)
         except Exception as e:   # the baseline is reported, never the target
             cpu = {"value": None, "error": repr(e)}
 
+    if room is not None:   # the GPU run's per-step data served the CPU baseline only
+        room.pop("_iters", None)
+        room.pop("_Q", None)
     out = {
         "metric": METRIC,
         "value": round(v_mixed, 2),
