@@ -1559,8 +1559,11 @@ __device__ __forceinline__ void ell_accum(const DLevel& L, unsigned I, const voi
 // gather restriction: b_{l+1}[I] = sum over children of r_l, ascending
 // (the serial sum of pressure_mg.cpp:285-286 in the same order).  A row has
 // at most 8 children at level 0 (a piece of a 2x2x2 block) and rarely more
-// further down; the first eight are gathered together.
-template <class VT>
+// further down; the first eight are gathered together.  Where level l+1 is
+// not the system's coarsest, the same thread then takes the first Jacobi
+// sweep from zero there, x = omega b / diag (jacobi, pressure_mg.cpp:237-247,
+// from x = 0: pointwise), which saves a launch and a read of b.
+template <class CT, class VT>
 __global__ void __launch_bounds__(TB) k_vc_restrict(const DSys* __restrict__ sys, const int4* __restrict__ tab, int l) {
     const int4 e = tab[blockIdx.x];
     const DSys& s0 = sys[e.x];
@@ -1599,27 +1602,16 @@ __global__ void __launch_bounds__(TB) k_vc_restrict(const DSys* __restrict__ sys
 #pragma unroll
     for (int g = 0; g < CV_GRP; ++g)
         if ((G.active >> g) & 1u) st<VT>(sys[G.first + g].L[l + 1].b, I, acc[g] * (1.0f / VC_LEVEL_SCALE));
-}
-
-// first Jacobi sweep from zero: x = omega b / diag (jacobi, pressure_mg.cpp:237-247)
-template <class CT, class VT>
-__global__ void __launch_bounds__(TB) k_vc_jac0(const DSys* __restrict__ sys, const int4* __restrict__ tab, int l) {
-    const int4 e = tab[blockIdx.x];
-    const DSys& s0 = sys[e.x];
-    if (l >= s0.nl - 1) return;
-    const Grp G = grp_of(sys, e);
-    if (!G.active) return;
-    const DLevel& L = s0.L[l];
-    const int I = e.z * TB + threadIdx.x;
-    if (I >= L.n) return;
-    const float dg = ld<CT>(L.diag, I);
+    if (l + 2 < s0.nl) {
+        const float dg = ld<CT>(s0.L[l + 1].diag, I);
 #pragma unroll
-    for (int g = 0; g < CV_GRP; ++g) {
-        if (!((G.active >> g) & 1u)) continue;
-        const DLevel& Lg = sys[G.first + g].L[l];
-        float x = 0.0f;
-        if (dg > 0) x = (0.7f * ld<VT>(Lg.b, I)) / dg;
-        st<VT>(Lg.x, I, x);
+        for (int g = 0; g < CV_GRP; ++g) {
+            if (!((G.active >> g) & 1u)) continue;
+            const DLevel& Lg = sys[G.first + g].L[l + 1];
+            float x = 0.0f;
+            if (dg > 0) x = (0.7f * ld<VT>(Lg.b, I)) / dg;
+            st<VT>(Lg.x, I, x);
+        }
     }
 }
 
