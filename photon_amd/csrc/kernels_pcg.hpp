@@ -1513,9 +1513,12 @@ struct Grp {
     int first, count;
     unsigned active;   // bit g: system first + g still iterates
 };
-__device__ __forceinline__ Grp grp_of(const DSys* __restrict__ sys, const int4& e) {
+// The batch's state rows are one array with ST_NSTATE ints per system, so the
+// flags are read at an address known from the table entry alone, at the same
+// time as the system's descriptor rather than after it.
+__device__ __forceinline__ Grp grp_of(const int* __restrict__ states, const int4& e) {
     Grp G{e.x, e.y, 0u};
-    for (int g = 0; g < e.y; ++g) G.active |= (sys[e.x + g].state[ST_ACTIVE] != 0 ? 1u : 0u) << g;
+    for (int g = 0; g < e.y; ++g) G.active |= (states[(e.x + g) * ST_NSTATE + ST_ACTIVE] != 0 ? 1u : 0u) << g;
     return G;
 }
 
@@ -1564,11 +1567,12 @@ __device__ __forceinline__ void ell_accum(const DLevel& L, unsigned I, const voi
 // sweep from zero there, x = omega b / diag (jacobi, pressure_mg.cpp:237-247,
 // from x = 0: pointwise), which saves a launch and a read of b.
 template <class CT, class VT>
-__global__ void __launch_bounds__(TB) k_vc_restrict(const DSys* __restrict__ sys, const int4* __restrict__ tab, int l) {
+__global__ void __launch_bounds__(TB) k_vc_restrict(const DSys* __restrict__ sys, const int* __restrict__ states,
+                                                    const int4* __restrict__ tab, int l) {
     const int4 e = tab[blockIdx.x];
     const DSys& s0 = sys[e.x];
     if (l + 1 >= s0.nl) return;
-    const Grp G = grp_of(sys, e);
+    const Grp G = grp_of(states, e);
     if (!G.active) return;
     const DLevel& F = s0.L[l];
     const int I = e.z * TB + threadIdx.x;
@@ -1617,12 +1621,13 @@ __global__ void __launch_bounds__(TB) k_vc_restrict(const DSys* __restrict__ sys
 
 // damped Jacobi sweep src -> dst, omega = 0.7
 template <class CT, class VT>
-__global__ void __launch_bounds__(TB) k_vc_jac(const DSys* __restrict__ sys, const int4* __restrict__ tab, int l,
+__global__ void __launch_bounds__(TB) k_vc_jac(const DSys* __restrict__ sys, const int* __restrict__ states,
+                                               const int4* __restrict__ tab, int l,
                                                int src_is_x) {
     const int4 e = tab[blockIdx.x];
     const DSys& s0 = sys[e.x];
     if (l >= s0.nl - 1) return;
-    const Grp G = grp_of(sys, e);
+    const Grp G = grp_of(states, e);
     if (!G.active) return;
     const DLevel& L = s0.L[l];
     const int I = e.z * TB + threadIdx.x;
@@ -1636,7 +1641,7 @@ __global__ void __launch_bounds__(TB) k_vc_jac(const DSys* __restrict__ sys, con
         acc[g] = g < G.count ? ld<VT>(Lg.b, I) : 0.0f;
     }
     const float dg = ld<CT>(L.diag, I);
-    if (dg > 0) ell_accum<CT, VT>(L, (unsigned)I, src, G.count, acc);
+    ell_accum<CT, VT>(L, (unsigned)I, src, G.count, acc);   // padded rows too: their loads are in bounds
 #pragma unroll
     for (int g = 0; g < CV_GRP; ++g) {
         if (!((G.active >> g) & 1u)) continue;
@@ -1649,11 +1654,12 @@ __global__ void __launch_bounds__(TB) k_vc_jac(const DSys* __restrict__ sys, con
 
 // coarse residual of the current iterate t (residualSparse, pressure_mg.cpp:248-257)
 template <class CT, class VT>
-__global__ void __launch_bounds__(TB) k_vc_resid(const DSys* __restrict__ sys, const int4* __restrict__ tab, int l) {
+__global__ void __launch_bounds__(TB) k_vc_resid(const DSys* __restrict__ sys, const int* __restrict__ states,
+                                                 const int4* __restrict__ tab, int l) {
     const int4 e = tab[blockIdx.x];
     const DSys& s0 = sys[e.x];
     if (l >= s0.nl - 1) return;
-    const Grp G = grp_of(sys, e);
+    const Grp G = grp_of(states, e);
     if (!G.active) return;
     const DLevel& L = s0.L[l];
     const int I = e.z * TB + threadIdx.x;
@@ -1667,7 +1673,7 @@ __global__ void __launch_bounds__(TB) k_vc_resid(const DSys* __restrict__ sys, c
         t[g] = Lg.t;
         r[g] = g < G.count ? ld<VT>(Lg.b, I) - dg * ld<VT>(Lg.t, I) : 0.0f;
     }
-    if (dg > 0) ell_accum<CT, VT>(L, (unsigned)I, t, G.count, r);
+    ell_accum<CT, VT>(L, (unsigned)I, t, G.count, r);
 #pragma unroll
     for (int g = 0; g < CV_GRP; ++g)
         if ((G.active >> g) & 1u) st<VT>(sys[G.first + g].L[l].r, I, dg > 0 ? r[g] : 0.0f);
@@ -1675,11 +1681,12 @@ __global__ void __launch_bounds__(TB) k_vc_resid(const DSys* __restrict__ sys, c
 
 // t_l += t_{l+1}[agg]
 template <class VT>
-__global__ void __launch_bounds__(TB) k_vc_prolong(const DSys* __restrict__ sys, const int4* __restrict__ tab, int l) {
+__global__ void __launch_bounds__(TB) k_vc_prolong(const DSys* __restrict__ sys, const int* __restrict__ states,
+                                                   const int4* __restrict__ tab, int l) {
     const int4 e = tab[blockIdx.x];
     const DSys& s0 = sys[e.x];
     if (l >= s0.nl - 1) return;
-    const Grp G = grp_of(sys, e);
+    const Grp G = grp_of(states, e);
     if (!G.active) return;
     const DLevel& L = s0.L[l];
     const int I = e.z * TB + threadIdx.x;

@@ -370,11 +370,11 @@ void vcycle_t(phg_ctx* ctx, BatchDev* B, int first) {
     }
     const std::vector<int4*>& TG = B->tabg;
     const std::vector<int>& NG = B->tabg_len;
-    launch(ctx, K_VC_RESTRICT, k_vc_restrict<CC, VT>, NG[1], 0, S, TG[1], 0);
+    launch(ctx, K_VC_RESTRICT, k_vc_restrict<CC, VT>, NG[1], 0, S, (const int*)B->state, TG[1], 0);
     for (int l = 1; l + 1 < B->maxl; ++l) {
-        launch(ctx, K_VC_JAC, k_vc_jac<CC, VT>, NG[l], 0, S, TG[l], l, 1);
-        launch(ctx, K_VC_RESID, k_vc_resid<CC, VT>, NG[l], 0, S, TG[l], l);
-        launch(ctx, K_VC_RESTRICT, k_vc_restrict<CC, VT>, NG[l + 1], 0, S, TG[l + 1], l);
+        launch(ctx, K_VC_JAC, k_vc_jac<CC, VT>, NG[l], 0, S, (const int*)B->state, TG[l], l, 1);
+        launch(ctx, K_VC_RESID, k_vc_resid<CC, VT>, NG[l], 0, S, (const int*)B->state, TG[l], l);
+        launch(ctx, K_VC_RESTRICT, k_vc_restrict<CC, VT>, NG[l + 1], 0, S, (const int*)B->state, TG[l + 1], l);
     }
     auto coarse = [&](auto kern, int cgs) {
         launch(ctx, K_VC_COARSE, kern, B->tab_coarse_len, coarse_lds(cgs, B->maxnc), S, (const int4*)B->tab_coarse);
@@ -397,9 +397,9 @@ void vcycle_t(phg_ctx* ctx, BatchDev* B, int first) {
         launch(ctx, K_VC_COARSE, k_vc_coarse_r<VT>, B->tab_cr_len, 0, S, (const int4*)B->tab_cr);
     }
     for (int l = B->maxl - 2; l >= 1; --l) {
-        launch(ctx, K_VC_PROLONG, k_vc_prolong<VT>, NG[l], 0, S, TG[l], l);
-        launch(ctx, K_VC_JAC, k_vc_jac<CC, VT>, NG[l], 0, S, TG[l], l, 0);
-        launch(ctx, K_VC_JAC, k_vc_jac<CC, VT>, NG[l], 0, S, TG[l], l, 1);
+        launch(ctx, K_VC_PROLONG, k_vc_prolong<VT>, NG[l], 0, S, (const int*)B->state, TG[l], l);
+        launch(ctx, K_VC_JAC, k_vc_jac<CC, VT>, NG[l], 0, S, (const int*)B->state, TG[l], l, 0);
+        launch(ctx, K_VC_JAC, k_vc_jac<CC, VT>, NG[l], 0, S, (const int*)B->state, TG[l], l, 1);
     }
     // post-smoothing mirrors the pre-smoothing (black, red per sweep), so M
     // stays symmetric; the last red sweep of each system leaves its r.z partials
