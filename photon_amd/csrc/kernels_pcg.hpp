@@ -1506,7 +1506,7 @@ __global__ void __launch_bounds__(TB) k_z_out(const DSys* __restrict__ sys, cons
 // that share one grid and 256 unknowns: each thread reads its row of the
 // shared operator once and applies it to every system of the group, whose
 // own vectors it gathers.  The arithmetic of each system is the same as if it
-// were alone.  tab: (first system, count, row tile, 0).
+// were alone.  tab: (first system, count, row tile, levels of the systems' grid).
 constexpr int CV_GRP = 4;
 
 struct Grp {
@@ -1515,11 +1515,20 @@ struct Grp {
 };
 // The batch's state rows are one array with ST_NSTATE ints per system, so the
 // flags are read at an address known from the table entry alone, at the same
-// time as the system's descriptor rather than after it.
+// time as the system's descriptor rather than after it.  Each of these kernels
+// is a short chain of dependent loads per block, so every load of the CV_GRP
+// systems is issued without a test of the group's size: a slot past the group
+// reads its last system again (in bounds) and its result is never stored.  A
+// load under a per-slot test waits before the next slot's load is issued.
+// The flags are read by lanes 0..CV_GRP-1 of every wave with one vector load
+// and combined by a ballot (scalar loads of them were issued one at a time);
+// so the whole block calls this before any thread has left.
+__device__ __forceinline__ int slot(const int4& e, int g) { return e.x + min(g, e.y - 1); }
 __device__ __forceinline__ Grp grp_of(const int* __restrict__ states, const int4& e) {
-    Grp G{e.x, e.y, 0u};
-    for (int g = 0; g < e.y; ++g) G.active |= (states[(e.x + g) * ST_NSTATE + ST_ACTIVE] != 0 ? 1u : 0u) << g;
-    return G;
+    const int g = (int)(threadIdx.x & 63u);
+    const int a = states[slot(e, min(g, CV_GRP - 1)) * ST_NSTATE + ST_ACTIVE];
+    const unsigned long long m = __ballot(g < CV_GRP && g < e.y && a != 0);
+    return Grp{e.x, e.y, (unsigned)m & ((1u << CV_GRP) - 1u)};
 }
 
 // acc[g] += the sum over the row's ELL entries, in entry order, of
@@ -1570,8 +1579,8 @@ template <class CT, class VT>
 __global__ void __launch_bounds__(TB) k_vc_restrict(const DSys* __restrict__ sys, const int* __restrict__ states,
                                                     const int4* __restrict__ tab, int l) {
     const int4 e = tab[blockIdx.x];
+    if (l + 1 >= e.w) return;
     const DSys& s0 = sys[e.x];
-    if (l + 1 >= s0.nl) return;
     const Grp G = grp_of(states, e);
     if (!G.active) return;
     const DLevel& F = s0.L[l];
@@ -1581,9 +1590,11 @@ __global__ void __launch_bounds__(TB) k_vc_restrict(const DSys* __restrict__ sys
     float acc[CV_GRP];
 #pragma unroll
     for (int g = 0; g < CV_GRP; ++g) {
-        r[g] = g < G.count ? sys[G.first + g].L[l].r : nullptr;
+        r[g] = sys[slot(e, g)].L[l].r;
         acc[g] = 0.0f;
     }
+    const bool jac = l + 2 < e.w;
+    const float dg = ld<CT>(s0.L[l + 1].diag, I);   // level l+1 exists: read without a branch
     const int p0 = gld(F.cptr, I), p1 = gld(F.cptr, I + 1);   // p1 > p0: an aggregate has a child
     for (int pb = p0; pb < p1; pb += 8) {
         int c[8];
@@ -1604,18 +1615,13 @@ __global__ void __launch_bounds__(TB) k_vc_restrict(const DSys* __restrict__ sys
         }
     }
 #pragma unroll
-    for (int g = 0; g < CV_GRP; ++g)
-        if ((G.active >> g) & 1u) st<VT>(sys[G.first + g].L[l + 1].b, I, acc[g] * (1.0f / VC_LEVEL_SCALE));
-    if (l + 2 < s0.nl) {
-        const float dg = ld<CT>(s0.L[l + 1].diag, I);
-#pragma unroll
-        for (int g = 0; g < CV_GRP; ++g) {
-            if (!((G.active >> g) & 1u)) continue;
-            const DLevel& Lg = sys[G.first + g].L[l + 1];
-            float x = 0.0f;
-            if (dg > 0) x = (0.7f * ld<VT>(Lg.b, I)) / dg;
-            st<VT>(Lg.x, I, x);
-        }
+    for (int g = 0; g < CV_GRP; ++g) {
+        if (!((G.active >> g) & 1u)) continue;
+        const DLevel& Lg = sys[G.first + g].L[l + 1];
+        // b as stored: its value in VT, which the sweep below uses
+        const float b = to_f(from_f<VT>(acc[g] * (1.0f / VC_LEVEL_SCALE)));
+        st<VT>(Lg.b, I, b);
+        if (jac) st<VT>(Lg.x, I, dg > 0 ? (0.7f * b) / dg : 0.0f);
     }
 }
 
@@ -1625,20 +1631,21 @@ __global__ void __launch_bounds__(TB) k_vc_jac(const DSys* __restrict__ sys, con
                                                const int4* __restrict__ tab, int l,
                                                int src_is_x) {
     const int4 e = tab[blockIdx.x];
+    if (l >= e.w - 1) return;
     const DSys& s0 = sys[e.x];
-    if (l >= s0.nl - 1) return;
     const Grp G = grp_of(states, e);
     if (!G.active) return;
     const DLevel& L = s0.L[l];
     const int I = e.z * TB + threadIdx.x;
     if (I >= L.n) return;
     const void* src[CV_GRP];
-    float acc[CV_GRP];
+    float acc[CV_GRP], self[CV_GRP];
 #pragma unroll
     for (int g = 0; g < CV_GRP; ++g) {
-        const DLevel& Lg = sys[G.first + min(g, G.count - 1)].L[l];
+        const DLevel& Lg = sys[slot(e, g)].L[l];
         src[g] = src_is_x ? Lg.x : Lg.t;
-        acc[g] = g < G.count ? ld<VT>(Lg.b, I) : 0.0f;
+        acc[g] = ld<VT>(Lg.b, I);
+        self[g] = ld<VT>(src[g], I);
     }
     const float dg = ld<CT>(L.diag, I);
     ell_accum<CT, VT>(L, (unsigned)I, src, G.count, acc);   // padded rows too: their loads are in bounds
@@ -1647,7 +1654,7 @@ __global__ void __launch_bounds__(TB) k_vc_jac(const DSys* __restrict__ sys, con
         if (!((G.active >> g) & 1u)) continue;
         const DLevel& Lg = sys[G.first + g].L[l];
         float out = 0.0f;
-        if (dg > 0) out = (1.0f - 0.7f) * ld<VT>(src[g], I) + (0.7f * acc[g]) / dg;
+        if (dg > 0) out = (1.0f - 0.7f) * self[g] + (0.7f * acc[g]) / dg;
         st<VT>(src_is_x ? Lg.t : Lg.x, I, out);
     }
 }
@@ -1657,8 +1664,8 @@ template <class CT, class VT>
 __global__ void __launch_bounds__(TB) k_vc_resid(const DSys* __restrict__ sys, const int* __restrict__ states,
                                                  const int4* __restrict__ tab, int l) {
     const int4 e = tab[blockIdx.x];
+    if (l >= e.w - 1) return;
     const DSys& s0 = sys[e.x];
-    if (l >= s0.nl - 1) return;
     const Grp G = grp_of(states, e);
     if (!G.active) return;
     const DLevel& L = s0.L[l];
@@ -1669,9 +1676,9 @@ __global__ void __launch_bounds__(TB) k_vc_resid(const DSys* __restrict__ sys, c
     float r[CV_GRP];
 #pragma unroll
     for (int g = 0; g < CV_GRP; ++g) {
-        const DLevel& Lg = sys[G.first + min(g, G.count - 1)].L[l];
+        const DLevel& Lg = sys[slot(e, g)].L[l];
         t[g] = Lg.t;
-        r[g] = g < G.count ? ld<VT>(Lg.b, I) - dg * ld<VT>(Lg.t, I) : 0.0f;
+        r[g] = ld<VT>(Lg.b, I) - dg * ld<VT>(Lg.t, I);
     }
     ell_accum<CT, VT>(L, (unsigned)I, t, G.count, r);
 #pragma unroll
@@ -1684,20 +1691,23 @@ template <class VT>
 __global__ void __launch_bounds__(TB) k_vc_prolong(const DSys* __restrict__ sys, const int* __restrict__ states,
                                                    const int4* __restrict__ tab, int l) {
     const int4 e = tab[blockIdx.x];
+    if (l >= e.w - 1) return;
     const DSys& s0 = sys[e.x];
-    if (l >= s0.nl - 1) return;
     const Grp G = grp_of(states, e);
     if (!G.active) return;
     const DLevel& L = s0.L[l];
     const int I = e.z * TB + threadIdx.x;
     if (I >= L.n) return;
     const int a = gld(L.agg, I);
+    float t[CV_GRP];
 #pragma unroll
     for (int g = 0; g < CV_GRP; ++g) {
-        if (!((G.active >> g) & 1u)) continue;
-        const DSys& sg = sys[G.first + g];
-        st<VT>(sg.L[l].t, I, ld<VT>(sg.L[l].t, I) + VC_LEVEL_SCALE * ld<VT>(sg.L[l + 1].t, a));
+        const DSys& sg = sys[slot(e, g)];
+        t[g] = ld<VT>(sg.L[l].t, I) + VC_LEVEL_SCALE * ld<VT>(sg.L[l + 1].t, a);
     }
+#pragma unroll
+    for (int g = 0; g < CV_GRP; ++g)
+        if ((G.active >> g) & 1u) st<VT>(sys[G.first + g].L[l].t, I, t[g]);
 }
 
 // coarsest level: t = B b in FP64, B the projected inverse of the grounded
