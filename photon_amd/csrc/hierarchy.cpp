@@ -158,6 +158,7 @@ void galerkin_sparse(const HostLevel& F, HostLevel& C) {
 void make_ell(HostLevel& L) {
     int k = 0;
     for (int I = 0; I < L.n; ++I) k = std::max(k, L.rowStart[I + 1] - L.rowStart[I]);
+    k = std::max(k, 1);   // at least one (padding) entry: the kernels take the first eight without a test
     L.ellk = k;
     L.ell_col.assign((size_t)k * L.n, 0);
     L.ell_val.assign((size_t)k * L.n, 0.0f);

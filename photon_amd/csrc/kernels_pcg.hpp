@@ -1506,7 +1506,9 @@ __global__ void __launch_bounds__(TB) k_z_out(const DSys* __restrict__ sys, cons
 // that share one grid and 256 unknowns: each thread reads its row of the
 // shared operator once and applies it to every system of the group, whose
 // own vectors it gathers.  The arithmetic of each system is the same as if it
-// were alone.  tab: (first system, count, row tile, levels of the systems' grid).
+// were alone.  tab: (first system, count, row tile, L | R << 8) with L the
+// levels of the systems' grid and R the rows of the tile, so a block tests
+// both before it reads any descriptor.
 constexpr int CV_GRP = 4;
 
 struct Grp {
@@ -1541,8 +1543,9 @@ constexpr int ELL_UNROLL = 8;
 template <class CT, class VT>
 __device__ __forceinline__ void ell_accum(const DLevel& L, unsigned I, const void* const (&src)[CV_GRP], int count,
                                           float (&acc)[CV_GRP]) {
-    const int K = L.ellk;
-    for (int p0 = 0; p0 < K; p0 += ELL_UNROLL) {
+    const int K = L.ellk;   // >= 1
+    int p0 = 0;
+    do {
         int c[ELL_UNROLL];
         CT v[ELL_UNROLL];
 #pragma unroll
@@ -1565,7 +1568,8 @@ __device__ __forceinline__ void ell_accum(const DLevel& L, unsigned I, const voi
             for (int g = 0; g < CV_GRP; ++g)
                 if (g < count) acc[g] += vq * to_f(x[g][q]);
         }
-    }
+        p0 += ELL_UNROLL;
+    } while (p0 < K);
 }
 
 // gather restriction: b_{l+1}[I] = sum over children of r_l, ascending
@@ -1579,13 +1583,12 @@ template <class CT, class VT>
 __global__ void __launch_bounds__(TB) k_vc_restrict(const DSys* __restrict__ sys, const int* __restrict__ states,
                                                     const int4* __restrict__ tab, int l) {
     const int4 e = tab[blockIdx.x];
-    if (l + 1 >= e.w) return;
+    if (l + 1 >= (e.w & 255)) return;
     const DSys& s0 = sys[e.x];
     const Grp G = grp_of(states, e);
-    if (!G.active) return;
+    if (!G.active || (int)threadIdx.x >= e.w >> 8) return;
     const DLevel& F = s0.L[l];
     const int I = e.z * TB + threadIdx.x;
-    if (I >= s0.L[l + 1].n) return;
     const void* r[CV_GRP];
     float acc[CV_GRP];
 #pragma unroll
@@ -1593,10 +1596,11 @@ __global__ void __launch_bounds__(TB) k_vc_restrict(const DSys* __restrict__ sys
         r[g] = sys[slot(e, g)].L[l].r;
         acc[g] = 0.0f;
     }
-    const bool jac = l + 2 < e.w;
+    const bool jac = l + 2 < (e.w & 255);
     const float dg = ld<CT>(s0.L[l + 1].diag, I);   // level l+1 exists: read without a branch
     const int p0 = gld(F.cptr, I), p1 = gld(F.cptr, I + 1);   // p1 > p0: an aggregate has a child
-    for (int pb = p0; pb < p1; pb += 8) {
+    int pb = p0;
+    do {
         int c[8];
 #pragma unroll
         for (int q = 0; q < 8; ++q) c[q] = gld(F.cidx, (unsigned)min(pb + q, p1 - 1));
@@ -1613,7 +1617,8 @@ __global__ void __launch_bounds__(TB) k_vc_restrict(const DSys* __restrict__ sys
             for (int g = 0; g < CV_GRP; ++g)
                 if (g < G.count) acc[g] += to_f(x[g][q]);
         }
-    }
+        pb += 8;
+    } while (pb < p1);
 #pragma unroll
     for (int g = 0; g < CV_GRP; ++g) {
         if (!((G.active >> g) & 1u)) continue;
@@ -1631,13 +1636,12 @@ __global__ void __launch_bounds__(TB) k_vc_jac(const DSys* __restrict__ sys, con
                                                const int4* __restrict__ tab, int l,
                                                int src_is_x) {
     const int4 e = tab[blockIdx.x];
-    if (l >= e.w - 1) return;
+    if (l >= (e.w & 255) - 1) return;
     const DSys& s0 = sys[e.x];
     const Grp G = grp_of(states, e);
-    if (!G.active) return;
+    if (!G.active || (int)threadIdx.x >= e.w >> 8) return;
     const DLevel& L = s0.L[l];
     const int I = e.z * TB + threadIdx.x;
-    if (I >= L.n) return;
     const void* src[CV_GRP];
     float acc[CV_GRP], self[CV_GRP];
 #pragma unroll
@@ -1664,13 +1668,12 @@ template <class CT, class VT>
 __global__ void __launch_bounds__(TB) k_vc_resid(const DSys* __restrict__ sys, const int* __restrict__ states,
                                                  const int4* __restrict__ tab, int l) {
     const int4 e = tab[blockIdx.x];
-    if (l >= e.w - 1) return;
+    if (l >= (e.w & 255) - 1) return;
     const DSys& s0 = sys[e.x];
     const Grp G = grp_of(states, e);
-    if (!G.active) return;
+    if (!G.active || (int)threadIdx.x >= e.w >> 8) return;
     const DLevel& L = s0.L[l];
     const int I = e.z * TB + threadIdx.x;
-    if (I >= L.n) return;
     const float dg = ld<CT>(L.diag, I);
     const void* t[CV_GRP];
     float r[CV_GRP];
@@ -1691,13 +1694,12 @@ template <class VT>
 __global__ void __launch_bounds__(TB) k_vc_prolong(const DSys* __restrict__ sys, const int* __restrict__ states,
                                                    const int4* __restrict__ tab, int l) {
     const int4 e = tab[blockIdx.x];
-    if (l >= e.w - 1) return;
+    if (l >= (e.w & 255) - 1) return;
     const DSys& s0 = sys[e.x];
     const Grp G = grp_of(states, e);
-    if (!G.active) return;
+    if (!G.active || (int)threadIdx.x >= e.w >> 8) return;
     const DLevel& L = s0.L[l];
     const int I = e.z * TB + threadIdx.x;
-    if (I >= L.n) return;
     const int a = gld(L.agg, I);
     float t[CV_GRP];
 #pragma unroll

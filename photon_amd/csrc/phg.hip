@@ -112,7 +112,7 @@ struct BatchDev {
     int tabw_len = 0;
     int fuse_nt = 0;               // threads per block of the fused passes (0: 2 TB up to nx = 128, 4 TB above)
     std::vector<int> tab_len;
-    std::vector<int4*> tabg;       // levels >= 1: (first system, count, tile, levels) of same-grid groups
+    std::vector<int4*> tabg;       // levels >= 1: (first system, count, tile, levels | rows << 8) of same-grid groups
     std::vector<int> tabg_len;
     int2* tab_sys = nullptr;       // one entry per system
     int2* tab_z = nullptr;         // (system, zone task) for the closed-zone means
@@ -1132,7 +1132,8 @@ int phg_batch_create_ex(phg_ctx* ctx, int nsys, const int* grids, const phg_batc
                 if ((int)G->lv.size() > l) {
                     nb += tiles(G->lv[l].n);
                     if (out)
-                        for (int k = 0; k < tiles(G->lv[l].n); ++k) out->push_back(make_int4(s, cnt, k, (int)G->lv.size()));
+                        for (int k = 0; k < tiles(G->lv[l].n); ++k)
+                            out->push_back(make_int4(s, cnt, k, (int)G->lv.size() | std::min(TB, G->lv[l].n - k * TB) << 8));
                 }
                 s += cnt;
             }
