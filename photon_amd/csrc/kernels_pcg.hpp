@@ -1526,11 +1526,14 @@ struct Grp {
 // and combined by a ballot (scalar loads of them were issued one at a time);
 // so the whole block calls this before any thread has left.
 __device__ __forceinline__ int slot(const int4& e, int g) { return e.x + min(g, e.y - 1); }
-__device__ __forceinline__ Grp grp_of(const int* __restrict__ states, const int4& e) {
+// bit g: system first + g (g < count <= 64) still iterates
+__device__ __forceinline__ unsigned long long live_of(const int* __restrict__ states, int first, int count) {
     const int g = (int)(threadIdx.x & 63u);
-    const int a = states[slot(e, min(g, CV_GRP - 1)) * ST_NSTATE + ST_ACTIVE];
-    const unsigned long long m = __ballot(g < CV_GRP && g < e.y && a != 0);
-    return Grp{e.x, e.y, (unsigned)m & ((1u << CV_GRP) - 1u)};
+    const int a = states[(first + min(g, count - 1)) * ST_NSTATE + ST_ACTIVE];
+    return __ballot(g < count && a != 0);
+}
+__device__ __forceinline__ Grp grp_of(const int* __restrict__ states, const int4& e) {
+    return Grp{e.x, e.y, (unsigned)live_of(states, e.x, e.y)};   // e.y <= CV_GRP
 }
 
 // acc[g] += the sum over the row's ELL entries, in entry order, of
@@ -1751,26 +1754,46 @@ __device__ __forceinline__ void coarse_acc(double (&acc)[CGS], double a, const f
     }
 }
 
+// b of the group's systems into LDS, [rows][CGS] from row r0 (rows past n
+// and systems past the group read as 0), and their coarsest t (or, for
+// `tsel` 1, their product buffers) into tp.  Thread k takes system k % CGS
+// throughout (TB is a multiple of CGS), so it reads that system's pointer
+// once, and issues eight loads before it stores any: a loop of load, store
+// per row and system waits once per element.
 template <class VT, int CGS>
-__global__ void __launch_bounds__(TB) k_vc_coarse(const DSys* __restrict__ sys, const int4* __restrict__ tab) {
+__device__ __forceinline__ void coarse_stage(const DSys* __restrict__ sys, const int4& e, int top, int r0, int rows,
+                                             int n, float* bsh, void** tp, int tsel = 0) {
+    constexpr int JS = TB / CGS, SU = 8;
+    static_assert(TB % CGS == 0, "a thread keeps one system");
+    const int g = (int)threadIdx.x % CGS;
+    const DSys& sg = sys[e.x + min(g, e.y - 1)];
+    const void* b = sg.L[top].b;
+    if ((int)threadIdx.x < CGS) tp[g] = tsel ? (void*)sg.cpart : sg.L[top].t;
+    for (int j0 = (int)threadIdx.x / CGS; j0 < rows; j0 += SU * JS) {
+        float v[SU];
+#pragma unroll
+        for (int u = 0; u < SU; ++u) v[u] = ld<VT>(b, (unsigned)min(r0 + j0 + u * JS, n - 1));
+#pragma unroll
+        for (int u = 0; u < SU; ++u) {
+            const int j = j0 + u * JS;
+            if (j < rows) bsh[j * CGS + g] = g < e.y && r0 + j < n ? v[u] : 0.0f;
+        }
+    }
+}
+
+template <class VT, int CGS>
+__global__ void __launch_bounds__(TB) k_vc_coarse(const DSys* __restrict__ sys, const int* __restrict__ states,
+                                                  const int4* __restrict__ tab) {
     const int4 e = tab[blockIdx.x];   // first system, count, row tile
+    const unsigned long long live = live_of(states, e.x, e.y);
+    if (!live) return;
     const DSys& s0 = sys[e.x];
     const int n = s0.nc;
     const double* __restrict__ A = s0.cinv;
-    int live = 0;
-    for (int g = 0; g < e.y; ++g) live |= sys[e.x + g].state[ST_ACTIVE];
-    if (!live) return;
     extern __shared__ double dsh[];
+    __shared__ void* tsh[CGS];
     float* bsh = reinterpret_cast<float*>(dsh);   // [n][CGS]
-    for (int g = 0; g < CGS; ++g) {
-        if (g < e.y) {
-            const DSys& s = sys[e.x + g];
-            const void* b = s.L[s.nl - 1].b;
-            for (int j = threadIdx.x; j < n; j += TB) bsh[j * CGS + g] = ld<VT>(b, j);
-        } else {
-            for (int j = threadIdx.x; j < n; j += TB) bsh[j * CGS + g] = 0.0f;
-        }
-    }
+    coarse_stage<VT, CGS>(sys, e, s0.nl - 1, 0, n, n, bsh, tsh);
     __syncthreads();
     const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
     const int i = e.z * CG_ROWS + lane;
@@ -1801,8 +1824,7 @@ __global__ void __launch_bounds__(TB) k_vc_coarse(const DSys* __restrict__ sys, 
             if (g >= e.y) break;
             const double v = acc[g] + red[(0 * CG_ROWS + lane) * CGS + g] + red[(1 * CG_ROWS + lane) * CGS + g] +
                              red[(2 * CG_ROWS + lane) * CGS + g];
-            const DSys& s = sys[e.x + g];
-            if (s.state[ST_ACTIVE]) st<VT>(s.L[s.nl - 1].t, i, (float)v);
+            if ((live >> g) & 1u) st<VT>(tsh[g], i, (float)v);
         }
     }
 }
@@ -1825,31 +1847,21 @@ __host__ __device__ constexpr size_t coarse_p_lds(int cgs) {
 }
 
 template <class VT, int CGS>
-__global__ void __launch_bounds__(TB) k_vc_coarse_p(const DSys* __restrict__ sys, const int4* __restrict__ tab) {
+__global__ void __launch_bounds__(TB) k_vc_coarse_p(const DSys* __restrict__ sys, const int* __restrict__ states,
+                                                    const int4* __restrict__ tab) {
     const int4 e = tab[blockIdx.x];   // first system, count, strip I, strip J >= I
+    if (!live_of(states, e.x, e.y)) return;
     const DSys& s0 = sys[e.x];
-    int live = 0;
-    for (int g = 0; g < e.y; ++g) live |= sys[e.x + g].state[ST_ACTIVE];
-    if (!live) return;
     const int n = s0.nc, nt = s0.cnt, I = e.z, J = e.w;
     extern __shared__ double dsh[];
+    __shared__ void* psh[CGS];
     double* tl = dsh;                                           // [c][65]: column c of the tile
     float* bI = reinterpret_cast<float*>(dsh + 64 * 65);       // [64][CGS]
     float* bJ = bI + 64 * CGS;                                  // [64][CGS]
     double* redr = reinterpret_cast<double*>(bJ + 64 * CGS);   // [3][64][CGS]
     double* redc = redr + 3 * 64 * CGS;                         // [3][64][CGS]
-    for (int q = threadIdx.x; q < 64 * CGS; q += TB) {
-        const int r = q / CGS, g = q - r * CGS;
-        float vi = 0.0f, vj = 0.0f;
-        if (g < e.y) {
-            const DSys& s = sys[e.x + g];
-            const void* b = s.L[s.nl - 1].b;
-            if (64 * I + r < n) vi = ld<VT>(b, 64 * I + r);
-            if (64 * J + r < n) vj = ld<VT>(b, 64 * J + r);
-        }
-        bI[q] = vi;
-        bJ[q] = vj;
-    }
+    coarse_stage<VT, CGS>(sys, e, s0.nl - 1, 64 * I, 64, n, bI, psh, 1);
+    coarse_stage<VT, CGS>(sys, e, s0.nl - 1, 64 * J, 64, n, bJ, psh, 1);
     const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
     const size_t tix = (size_t)I * nt - (size_t)I * (I - 1) / 2 + (J - I);
     const double* __restrict__ T = s0.cinvp + tix * 4096;
@@ -1889,7 +1901,7 @@ __global__ void __launch_bounds__(TB) k_vc_coarse_p(const DSys* __restrict__ sys
 #pragma unroll
         for (int g = 0; g < CGS; ++g) {
             if (g >= e.y) break;
-            double* P = sys[e.x + g].cpart;
+            double* P = static_cast<double*>(psh[g]);
             P[((size_t)I * nt + J) * 64 + lane] = ar[g] + redr[(0 * 64 + lane) * CGS + g] +
                                                   redr[(1 * 64 + lane) * CGS + g] + redr[(2 * 64 + lane) * CGS + g];
             if (I < J)
@@ -1902,12 +1914,14 @@ __global__ void __launch_bounds__(TB) k_vc_coarse_p(const DSys* __restrict__ sys
 // t_R = sum over strips S, in order, of the products k_vc_coarse_p left;
 // a wave per system of the group, lane = row
 template <class VT>
-__global__ void __launch_bounds__(TB) k_vc_coarse_r(const DSys* __restrict__ sys, const int4* __restrict__ tab) {
+__global__ void __launch_bounds__(TB) k_vc_coarse_r(const DSys* __restrict__ sys, const int* __restrict__ states,
+                                                    const int4* __restrict__ tab) {
     const int4 e = tab[blockIdx.x];   // first system, count, strip R
     const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    const unsigned long long live = live_of(states, e.x, e.y);
     for (int g = w; g < e.y; g += TB / 64) {
         const DSys& s = sys[e.x + g];
-        if (!s.state[ST_ACTIVE]) continue;
+        if (!((live >> g) & 1u)) continue;
         const int nt = s.cnt, i = 64 * e.z + lane;
         const double* __restrict__ P = s.cpart + (size_t)e.z * nt * 64 + lane;
         double y = 0.0;

@@ -377,7 +377,8 @@ void vcycle_t(phg_ctx* ctx, BatchDev* B, int first) {
         launch(ctx, K_VC_RESTRICT, k_vc_restrict<CC, VT>, NG[l + 1], 0, S, (const int*)B->state, TG[l + 1], l);
     }
     auto coarse = [&](auto kern, int cgs) {
-        launch(ctx, K_VC_COARSE, kern, B->tab_coarse_len, coarse_lds(cgs, B->maxnc), S, (const int4*)B->tab_coarse);
+        launch(ctx, K_VC_COARSE, kern, B->tab_coarse_len, coarse_lds(cgs, B->maxnc), S, (const int*)B->state,
+               (const int4*)B->tab_coarse);
     };
     switch (B->cg_sys) {
         case 1: coarse(k_vc_coarse<VT, 1>, 1); break;
@@ -387,14 +388,15 @@ void vcycle_t(phg_ctx* ctx, BatchDev* B, int first) {
     }
     if (B->tab_cp_len > 0) {
         auto packed = [&](auto kern, int cgs) {
-            launch(ctx, K_VC_COARSE, kern, B->tab_cp_len, coarse_p_lds(cgs), S, (const int4*)B->tab_cp);
+            launch(ctx, K_VC_COARSE, kern, B->tab_cp_len, coarse_p_lds(cgs), S, (const int*)B->state,
+                   (const int4*)B->tab_cp);
         };
         switch (B->cp_sys) {
             case 1: packed(k_vc_coarse_p<VT, 1>, 1); break;
             case 4: packed(k_vc_coarse_p<VT, 4>, 4); break;
             default: packed(k_vc_coarse_p<VT, 8>, 8); break;
         }
-        launch(ctx, K_VC_COARSE, k_vc_coarse_r<VT>, B->tab_cr_len, 0, S, (const int4*)B->tab_cr);
+        launch(ctx, K_VC_COARSE, k_vc_coarse_r<VT>, B->tab_cr_len, 0, S, (const int*)B->state, (const int4*)B->tab_cr);
     }
     for (int l = B->maxl - 2; l >= 1; --l) {
         launch(ctx, K_VC_PROLONG, k_vc_prolong<VT>, NG[l], 0, S, (const int*)B->state, TG[l], l);
