@@ -1960,15 +1960,22 @@ __global__ void __launch_bounds__(TB) k_vc_coarse_r(const DSys* __restrict__ sys
 // ---------------------------------------------------------------------------
 constexpr int PROJ_TASKS = 8;
 
-__global__ void __launch_bounds__(TB) k_proj(const DSys* __restrict__ sys, const int2* __restrict__ tab, int ntab) {
+// Most iterations have no projection due: lane q of every wave reads task q's
+// flags in the batch's state array, and a ballot tells the block which tasks
+// to take, so an idle block leaves after two dependent loads.
+__global__ void __launch_bounds__(TB) k_proj(const DSys* __restrict__ sys, const int* __restrict__ states,
+                                             const int2* __restrict__ tab, int ntab) {
     __shared__ double out[1];
+    const int lane = (int)(threadIdx.x & 63u), base = blockIdx.x * PROJ_TASKS;
+    const int* sq = states + tab[min(base + min(lane, PROJ_TASKS - 1), ntab - 1)].x * ST_NSTATE;
+    const unsigned long long due =
+        __ballot(lane < PROJ_TASKS && base + lane < ntab && sq[ST_PROJ] != 0 && sq[ST_ACTIVE] != 0);
     for (int q = 0; q < PROJ_TASKS; ++q) {
-        const int idx = blockIdx.x * PROJ_TASKS + q;
-        if (idx >= ntab) break;
+        if (!((due >> q) & 1u)) continue;
+        const int idx = base + q;
         const int2 e = tab[idx];
         const DSys& s = sys[e.x];
         const int pj = st_uni(s, ST_PROJ);
-        if (!pj || !st_uni(s, ST_ACTIVE)) continue;
         float* __restrict__ r = rbuf(s, st_uni(s, ST_ITERS));   // r of the iteration just finished
         const int4 tk = s.ztask[e.y];
         int p = tk.y + threadIdx.x;

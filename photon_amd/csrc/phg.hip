@@ -483,8 +483,8 @@ void cg_iteration(phg_ctx* ctx, BatchDev* B) {
     vcycle(ctx, B, 0);
     launch(ctx, K_FIN, k_fin_beta, B->nsys, 0, S, B->tab_sys);
     if (B->tab_z_len > 0)
-        launch(ctx, K_ZONE, k_proj, (B->tab_z_len + PROJ_TASKS - 1) / PROJ_TASKS, 0, S, (const int2*)B->tab_z,
-               B->tab_z_len);
+        launch(ctx, K_ZONE, k_proj, (B->tab_z_len + PROJ_TASKS - 1) / PROJ_TASKS, 0, S, (const int*)B->state,
+               (const int2*)B->tab_z, B->tab_z_len);
 }
 
 // closed-zone mean removal of every system's x (whatever its state)
