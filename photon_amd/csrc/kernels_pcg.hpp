@@ -1522,9 +1522,9 @@ struct Grp {
 // systems is issued without a test of the group's size: a slot past the group
 // reads its last system again (in bounds) and its result is never stored.  A
 // load under a per-slot test waits before the next slot's load is issued.
-// The flags are read by lanes 0..CV_GRP-1 of every wave with one vector load
-// and combined by a ballot (scalar loads of them were issued one at a time);
-// so the whole block calls this before any thread has left.
+// The flags are read by lanes 0..count-1 of every wave with one vector load
+// and combined by a ballot (scalar loads of them were issued one at a time),
+// so the whole block calls live_of before any of its threads has left.
 __device__ __forceinline__ int slot(const int4& e, int g) { return e.x + min(g, e.y - 1); }
 // bit g: system first + g (g < count <= 64) still iterates
 __device__ __forceinline__ unsigned long long live_of(const int* __restrict__ states, int first, int count) {
