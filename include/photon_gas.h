@@ -50,8 +50,19 @@ enum {
     PHG_STOP_MAXNORM = 1   /* max|r| / inv <= tol in 1/s, the reference's rule (gasphase.cpp:592-605) */
 };
 
-/* solve status: breakdown = d.Ad <= 0 (gasphase.cpp:607) or r.z <= 0 (:605) */
-enum { PHG_RUNNING = 0, PHG_CONVERGED = 1, PHG_MAXITER = 2, PHG_BREAKDOWN = 3, PHG_BREAKDOWN_RZ = 4 };
+/* solve status: breakdown = d.Ad <= 0 (gasphase.cpp:607) or r.z <= 0 (:605);
+ * nonfinite = the residual norm became NaN or infinite (a non-finite
+ * right-hand side or warm start, or an overflow): the system stops there and
+ * is never restarted.  The reference's loop test `res > tol` also ends on a
+ * NaN residual, but reports nothing. */
+enum {
+    PHG_RUNNING = 0,
+    PHG_CONVERGED = 1,
+    PHG_MAXITER = 2,
+    PHG_BREAKDOWN = 3,
+    PHG_BREAKDOWN_RZ = 4,
+    PHG_NONFINITE = 5
+};
 
 typedef struct phg_ctx phg_ctx;
 
@@ -59,7 +70,7 @@ typedef struct {
     double tol;          /* stop tolerance (see stop rule) */
     int max_iters;       /* iteration cap (FidelityProfile::pressureMaxIters, solver.h:89) */
     int stop_rule;       /* PHG_STOP_* */
-    float inv;           /* dx^2/dt, for PHG_STOP_MAXNORM */
+    float inv;           /* dx^2/dt, for PHG_STOP_MAXNORM (must then be positive and finite) */
     int demean;          /* remove the mean of x on closed zones at the end (gasphase.cpp:628-639) */
     int check_every;     /* iterations between host convergence checks (0 = library default) */
     int use_graph;       /* 0: launch directly; 1: replay one captured CG iteration; 2: the set-up and
@@ -79,7 +90,7 @@ typedef struct {
 
 typedef struct {
     int iters;           /* CG iterations (Solver::pressureIters, solver.h:277) */
-    int status;          /* PHG_CONVERGED / PHG_MAXITER / PHG_BREAKDOWN */
+    int status;          /* PHG_CONVERGED / PHG_MAXITER / PHG_BREAKDOWN* / PHG_NONFINITE */
     double relres;       /* final recursive ||r||/||b|| */
     double maxres;       /* final max|r| / inv (Solver::pressureResidual, solver.h:278) */
 } phg_solve_stats;
@@ -129,6 +140,20 @@ int phg_grid_create_planned(phg_ctx *ctx, int nx, int ny, int nz, float dx, cons
                             const float *fz, const float *ax, const float *ay, const float *az,
                             const uint8_t *blocked, const uint8_t *wind_closed, int precision,
                             phg_grid_plan *plan, int *grid_out);
+/* How the V-cycle solves its coarsest level (pressure_mg.cpp:277-281).
+ * PHG_COARSE_EXACT (the default where it fits): the dense FP64 projected
+ * inverse, formed once per grid; a better preconditioner than the
+ * reference's, so CG needs fewer iterations but its iterates differ.
+ * PHG_COARSE_GS20: the reference's own 20 forward/backward Gauss-Seidel
+ * sweeps from zero, so the iterates, pressureIters and pressureResidual
+ * follow the reference's to rounding; also the only mode of grids whose
+ * coarsening stalls above 2048 coarsest unknowns (dense inverse refused;
+ * up to 16384).  The mode applies to batches created after the call; the
+ * grid's own cached one-system batch (phg_solve, phg_project, ...) follows
+ * it at once. */
+enum { PHG_COARSE_EXACT = 0, PHG_COARSE_GS20 = 1 };
+int phg_grid_set_coarse(phg_ctx *ctx, int grid, int mode);
+int phg_grid_get_coarse(phg_ctx *ctx, int grid, int *mode);
 /* hierarchy shape: level count, sizes (nlevels entries), zones */
 int phg_grid_info(phg_ctx *ctx, int grid, int *nlevels, int *level_n, int *nzones, int *ref_levels);
 /* zone labels (N ints, -1 solid) and open flags (nzones bytes), host out */

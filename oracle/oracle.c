@@ -415,6 +415,14 @@ static void chol_solve(const double *Lm, int n, double *y) {
 
 double orc_apply_f32(const orc_mg *mg, const float *x, float *y) { return apply_all_f32(mg, x, y); }
 
+/* The exact coarsest solve alone (the ORC_COARSE_EXACT mode of the V-cycle):
+ * x = P B^-1 P b on the coarsest level, B the grounded operator, P the
+ * removal of the mean on each closed component; rows with diag <= 0 give 0.
+ * Exposed so that tests can pin it against a pseudo-inverse directly. */
+void orc_coarse_solve_f64(orc_mg *mg, const double *b, double *x) {
+    coarse_exact_f64(mg, &mg->L[mg->nl - 1], x, b);
+}
+
 int orc_pcg_f32(orc_mg *mg, const float *b, float *x, float inv, float tol, int max_iters, int stop_mode,
                 int coarse_mode, double *res_out) {
     return pcg_core_f32(mg, b, x, inv, (double)tol, max_iters, stop_mode, coarse_mode, res_out, NULL, NULL);

@@ -74,6 +74,7 @@ void orc_precondition_f64(orc_mg *mg, const double *r, double *z, int coarse_mod
 
 /* ---- y = A x on level 0, returns sum y*x in per-slice FP64 (gasphase.cpp:551-570) */
 double orc_apply_f32(const orc_mg *mg, const float *x, float *y);
+void orc_coarse_solve_f64(orc_mg *mg, const double *b, double *x);
 
 /* ---- PCG on a given right-hand side (the CG loop of project(), gasphase.cpp:571-627)
  * x is the warm start on input and the solution on output.  Stop rules:

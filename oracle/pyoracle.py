@@ -58,6 +58,7 @@ def lib():
         L.orc_precondition_f64.argtypes = [C.c_void_p, _f64p, _f64p, C.c_int]
         L.orc_apply_f32.restype = C.c_double
         L.orc_apply_f32.argtypes = [C.c_void_p, _f32p, _f32p]
+        L.orc_coarse_solve_f64.argtypes = [C.c_void_p, _f64p, _f64p]
         L.orc_pcg_f32.argtypes = [C.c_void_p, _f32p, _f32p, C.c_float, C.c_float, C.c_int, C.c_int, C.c_int,
                                   C.POINTER(C.c_double)]
         L.orc_pcg_f64.argtypes = [C.c_void_p, _f64p, _f64p, C.c_double, C.c_int, C.c_int, C.POINTER(C.c_double)]
@@ -125,6 +126,13 @@ class Hierarchy:
             z = np.zeros_like(r)
             lib().orc_precondition_f32(self.h, r, z, coarse)
         return z
+
+    def coarse_solve(self, b):
+        """The exact coarsest solve alone (FP64) on a coarsest-level vector."""
+        b = np.ascontiguousarray(b, np.float64)
+        x = np.zeros_like(b)
+        lib().orc_coarse_solve_f64(self.h, b, x)
+        return x
 
     def apply(self, x):
         x = np.ascontiguousarray(x, np.float32)
@@ -279,6 +287,9 @@ class RefSolver:
             L.ref_energy.argtypes = [C.c_void_p, C.c_float]
             L.ref_advect.argtypes = [C.c_void_p, C.c_float]
             L.ref_threads.argtypes = [C.c_void_p]
+            L.ref_set_wind.argtypes = [C.c_void_p, C.c_float, C.c_float, C.c_float]
+            L.ref_wind_at.restype = C.c_float
+            L.ref_wind_at.argtypes = [C.c_void_p, C.c_int, C.c_int, C.c_float]
             cls._libs[path] = L
         return cls._libs[path]
 
@@ -338,6 +349,12 @@ class RefSolver:
                                _ptr(d["diag"]), _ptr(d["agg"]))
             out.append(d)
         return out
+
+    def set_wind(self, speed, from_deg, exponent=0.25):
+        self._L.ref_set_wind(self.h, speed, from_deg, exponent)
+
+    def wind_at(self, axis, side, z):
+        return self._L.ref_wind_at(self.h, axis, side, z)
 
     def energy(self, dt):
         return self._L.ref_energy(self.h, dt)

@@ -91,6 +91,20 @@ int ref_setup(void* h, int nx, int ny, int nz, float dx, const float* fx, const 
     return (int)s->zoneP.size();
 }
 
+// Wind at the open boundaries (Solver::wind, solver.h:106-111); the
+// hierarchy is rebuilt on the next project (its level 0 masks wind sides,
+// pressure_mg.cpp:31-46).
+void ref_set_wind(void* h, float speed, float fromDeg, float exponent) {
+    Solver* s = static_cast<Solver*>(h);
+    s->wind.speed = speed;
+    s->wind.fromDeg = fromDeg;
+    s->wind.exponent = exponent;
+    s->mgDirty_ = true;
+}
+
+// windAt (gasphase.cpp:52-61): the inflow velocity on a boundary face, NaN if none
+float ref_wind_at(void* h, int axis, int side, float z) { return static_cast<Solver*>(h)->windAt(axis, side, z); }
+
 void ref_get_zones(void* h, int* zone, uint8_t* zoneOpen) {
     Solver* s = static_cast<Solver*>(h);
     std::copy(s->zone.begin(), s->zone.end(), zone);

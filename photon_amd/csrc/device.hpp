@@ -161,6 +161,12 @@ struct DSys {
     const int* zclosed;           // per cell: closed-zone index or -1
     double* zpart;                // per task
     double* zmean;                // per closed zone
+    // the coarsest level by the reference's 20 forward/backward Gauss-Seidel
+    // sweeps instead of the dense inverse (k_vc_coarse_gs): the level schedule
+    // of each sweep (HostHierarchy::gs_ord / gs_lptr)
+    const int* gs_ord;
+    const int* gs_lptr;
+    int gs_nf, gs_nb;
     DLevel L[MAXL];
 };
 

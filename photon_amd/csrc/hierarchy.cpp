@@ -187,6 +187,37 @@ void make_children(HostLevel& F, int ncoarse) {
 // an open boundary) has the constant vector as null space; adding s*1*1^T on
 // it gives an SPD matrix whose inverse acts as the pseudo-inverse on vectors
 // of zero component sum.  Rows with diag <= 0 are isolated and held at zero.
+// forward sweep: rows in ascending order, a row after its lower neighbours;
+// backward sweep: descending, after its higher neighbours
+void gs_schedule(HostHierarchy& H) {
+    const HostLevel& C = H.L.back();
+    const int n = C.n;
+    H.gs_ord.assign(2 * (size_t)n, 0);
+    H.gs_lptr.clear();
+    std::vector<int> lev(n, 0);
+    for (int dir = 0; dir < 2; ++dir) {
+        int nlev = 0;
+        for (int t = 0; t < n; ++t) {
+            const int I = dir == 0 ? t : n - 1 - t;
+            int v = 0;
+            for (int e = C.rowStart[I]; e < C.rowStart[I + 1]; ++e) {
+                const int J = C.col[e];
+                if (dir == 0 ? J < I : J > I) v = std::max(v, lev[J] + 1);
+            }
+            lev[I] = v;
+            nlev = std::max(nlev, v + 1);
+        }
+        std::vector<int> start(nlev + 1, 0);
+        for (int I = 0; I < n; ++I) start[lev[I] + 1]++;
+        for (int l = 0; l < nlev; ++l) start[l + 1] += start[l];
+        std::vector<int> fill(start.begin(), start.end() - 1);
+        int* ord = H.gs_ord.data() + (size_t)dir * n;
+        for (int I = 0; I < n; ++I) ord[fill[lev[I]]++] = I;
+        H.gs_lptr.insert(H.gs_lptr.end(), start.begin(), start.end());
+        (dir == 0 ? H.gs_nf : H.gs_nb) = nlev;
+    }
+}
+
 void dense_inverse(HostHierarchy& H) {
     const HostLevel& L = H.L.back();
     const int n = L.n;
@@ -517,7 +548,18 @@ bool build_hierarchy(int nx, int ny, int nz, const float* ax, const float* ay, c
     if (H.ref_levels == 0) H.ref_levels = (int)H.L.size();
     for (size_t l = 1; l < H.L.size(); ++l) make_ell(H.L[l]);
     for (size_t l = 0; l + 1 < H.L.size(); ++l) make_children(H.L[l], H.L[l + 1].n);
-    dense_inverse(H);
+    // coarsening can stall (many small sealed pockets, each its own unknown):
+    // such a coarsest level is solved by the reference's Gauss-Seidel sweeps
+    // only, without the O(n^2) memory and O(n^3) work of the dense inverse
+    H.nc = H.L.back().n;
+    if (H.nc > kCoarseGsMax) {
+        if (err)
+            *err = "coarsest level has " + std::to_string(H.nc) + " unknowns (limit " + std::to_string(kCoarseGsMax) +
+                   "): coarsening stalled";
+        return false;
+    }
+    if (H.L.size() >= 2) gs_schedule(H);
+    if (H.nc <= dense_max) dense_inverse(H);
     return true;
 }
 

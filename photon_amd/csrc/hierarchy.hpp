@@ -40,11 +40,27 @@ struct HostHierarchy {
     std::vector<HostLevel> L;
     int ref_levels = 0;                     // levels produced by the reference's stopping rule
     int nc = 0;                             // coarsest size
-    std::vector<double> cinv;               // nc*nc inverse of the grounded coarsest operator
+    std::vector<double> cinv;               // nc*nc inverse of the grounded coarsest operator (nc <= dense_max)
     std::vector<uint8_t> cmask;             // coarsest rows held at zero (diag <= 0)
+    // Level schedule of the reference's Gauss-Seidel sweeps on the coarsest
+    // level (pressure_mg.cpp:258-266, coarsest >= level 1): gs_ord holds the
+    // rows of the forward sweep grouped by level, then those of the backward
+    // sweep; gs_lptr the gs_nf + 1 forward level starts, then the gs_nb + 1
+    // backward ones.  A row's level is one past the highest level of the
+    // neighbours the sweep updates before it, so the rows of one level are
+    // independent and a sweep level by level reads exactly the values the
+    // serial sweep reads.
+    std::vector<int> gs_ord, gs_lptr;
+    int gs_nf = 0, gs_nb = 0;
 };
 
+// largest coarsest level the library takes (the Gauss-Seidel sweeps keep
+// its vectors in one workgroup's LDS)
+constexpr int kCoarseGsMax = 16384;
+
 // wind_closed: 4 flags (x-low, x-high, y-low, y-high) or nullptr.
+// The dense coarsest inverse is formed only when the coarsest level has at
+// most dense_max unknowns; a coarsest level above kCoarseGsMax is refused.
 // Returns false with a message on inconsistent input.
 bool build_hierarchy(int nx, int ny, int nz, const float* ax, const float* ay, const float* az,
                      const uint8_t* blocked, const uint8_t* wind_closed, int dense_max, HostHierarchy& H,

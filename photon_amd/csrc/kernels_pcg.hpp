@@ -559,7 +559,8 @@ __device__ __forceinline__ double vcycle_scale(double rr, int n) {
 }
 
 __device__ __forceinline__ bool stop_reached(const DSys& s, double rr, float maxr) {
-    if (s.state[ST_RULE] == 1) return (double)(maxr / (float)s.scal[S_INV]) <= s.scal[S_TOL];
+    // the reference's test, in FP32: res = max|r| / inv against pressureTol (gasphase.cpp:601-604)
+    if (s.state[ST_RULE] == 1) return maxr / (float)s.scal[S_INV] <= (float)s.scal[S_TOL];
     const double bb = s.scal[S_BB];
     const double rel = bb > 0 ? sqrt(rr) / sqrt(bb) : 0.0;
     return rel <= s.scal[S_TOL];
@@ -613,6 +614,13 @@ __global__ void __launch_bounds__(TB) k_fin_init(const DSys* __restrict__ sys, c
         s.state[ST_PROJ] = 0;
         s.state[ST_ITERS] = 0;
         s.state[ST_XPEND] = 0;
+        // a right-hand side or warm start that is not finite leaves nothing to
+        // iterate on: the system stops at once with its own status
+        if (!isfinite(res[0]) || !isfinite(res[1])) {
+            s.state[ST_STATUS] = 5;
+            s.state[ST_ACTIVE] = 0;
+            return;
+        }
         const bool done = floor_hit || stop_reached(s, res[1], m);
         if (done || s.state[ST_MAXIT] <= 0) {
             s.state[ST_STATUS] = done ? 1 : 2;
@@ -685,7 +693,8 @@ __global__ void __launch_bounds__(TB) k_fin_beta(const DSys* __restrict__ sys, c
         const int it = s.state[ST_ITERS] + 1;
         s.state[ST_ITERS] = it;
         int status = 0;
-        if (stop_reached(s, res[1], m)) status = 1;
+        if (!isfinite(res[1])) status = 5;   // the reference's `res > tol` ends here too, unflagged
+        else if (stop_reached(s, res[1], m)) status = 1;
         else if (it >= s.state[ST_MAXIT]) status = 2;
         else if (!(rz_new > 0)) status = 4;
         // each further fall of ||r|| / ||b|| by 1e3 (from 1e-3 on) has k_proj_r
@@ -1725,7 +1734,9 @@ __global__ void __launch_bounds__(TB) k_vc_prolong(const DSys* __restrict__ sys,
 // the loads of CG_UNROLL columns before their products, which it adds in
 // column order, so the sums do not depend on CG_UNROLL or CGS.
 // The reference approximates this solve with 20 forward/backward Gauss-Seidel
-// sweeps (pressure_mg.cpp:277-281); a serial sweep has no place on a GPU.
+// sweeps (pressure_mg.cpp:277-281), a chain of dependent row updates that
+// leaves a GPU mostly idle; k_vc_coarse_gs runs them where a grid asks for
+// the reference's iterates.
 constexpr int CG_ROWS = 64;
 constexpr int CG_SYS_MAX = 16;
 constexpr int CG_UNROLL = 8;
@@ -1827,6 +1838,62 @@ __global__ void __launch_bounds__(TB) k_vc_coarse(const DSys* __restrict__ sys, 
             if ((live >> g) & 1u) st<VT>(tsh[g], i, (float)v);
         }
     }
+}
+
+// Coarsest level by the reference's own method (pressure_mg.cpp:258-281):
+// from x = 0, 20 times a forward then a backward Gauss-Seidel sweep,
+// x_I = (b_I + sum val x_col) / diag_I over the row's entries in order,
+// x_I = 0 where diag_I <= 0.  One block per system keeps x and b in LDS and
+// runs each sweep level by level (DSys::gs_ord): the rows of a level read
+// only rows of earlier levels (updated in this sweep) or of later ones (not
+// yet), exactly as the serial sweep does, so every x_I is formed by the same
+// operations on the same values.  No contraction into FMA, as in the
+// reference's arithmetic.  Used for grids asked to follow the reference's
+// iterates (phg_grid_set_coarse) and for coarsest levels too large for the
+// dense inverse.
+template <class CC, class VT>
+__global__ void __launch_bounds__(TB) k_vc_coarse_gs(const DSys* __restrict__ sys, const int* __restrict__ states,
+                                                     const int2* __restrict__ tab) {
+#pragma clang fp contract(off)
+    const int si = tab[blockIdx.x].x;
+    if (states[si * ST_NSTATE + ST_ACTIVE] == 0) return;
+    const DSys& s = sys[si];
+    const DLevel& L = s.L[s.nl - 1];
+    const int n = L.n, K = L.ellk;
+    extern __shared__ float gsh[];
+    float* x = gsh;        // [n]
+    float* b = gsh + n;    // [n]
+    for (int i = threadIdx.x; i < n; i += TB) {
+        x[i] = 0.0f;
+        b[i] = ld<VT>(L.b, (unsigned)i);
+    }
+    __syncthreads();
+    for (int it = 0; it < 20; ++it) {
+        for (int dir = 0; dir < 2; ++dir) {
+            const int* __restrict__ ord = s.gs_ord + (size_t)dir * n;
+            const int* __restrict__ lp = s.gs_lptr + (dir ? s.gs_nf + 1 : 0);
+            const int nlev = dir ? s.gs_nb : s.gs_nf;
+            for (int lv = 0; lv < nlev; ++lv) {
+                for (int t = lp[lv] + (int)threadIdx.x; t < lp[lv + 1]; t += TB) {
+                    const int I = ord[t];
+                    const float dg = ld<CC>(L.diag, (unsigned)I);
+                    float v = 0.0f;
+                    if (dg > 0) {
+                        // padded entries (col = I, val = 0) add an exact zero
+                        float acc = b[I];
+                        for (int q = 0; q < K; ++q) {
+                            const unsigned e = (unsigned)q * (unsigned)n + (unsigned)I;
+                            acc = acc + ld<CC>(L.val, e) * x[gld(L.col, e)];
+                        }
+                        v = acc / dg;
+                    }
+                    x[I] = v;
+                }
+                __syncthreads();
+            }
+        }
+    }
+    for (int i = threadIdx.x; i < n; i += TB) st<VT>(L.t, (unsigned)i, x[i]);
 }
 
 // Coarsest level of a grid with more than CP_MIN unknowns: the inverse is
@@ -2024,34 +2091,44 @@ __global__ void __launch_bounds__(TB) k_proj(const DSys* __restrict__ sys, const
 // `which`: 0 the solution x (after the solve), 1 the residual r_0 of a system
 // restarted from its true residual, whose rounding can leave a component along
 // the null space (a constant per closed zone) that CG cannot reduce
+// the steps of the closed-zone mean removal (partial sums of one task, the
+// means of every zone) as device functions
+__device__ __forceinline__ void zone_task_sum(const int4 tk, const int* __restrict__ zcells,
+                                              const float* __restrict__ x, double* __restrict__ slot) {
+    double v[1] = {0.0};
+    for (int p = tk.y + threadIdx.x; p < tk.z; p += TB) v[0] += (double)x[zcells[p]];
+    __shared__ double out[1];
+    block_sums<1>(v, out);
+    __syncthreads();
+    if (threadIdx.x == 0) *slot = out[0];
+}
+
+__device__ __forceinline__ void zone_means(int nclosed, const int* __restrict__ ztptr, const int4* __restrict__ ztask,
+                                           const double* __restrict__ zpart, double* __restrict__ zmean) {
+    for (int z = threadIdx.x; z < nclosed; z += TB) {
+        double acc = 0.0;
+        int cnt = 0;
+        for (int t = ztptr[z]; t < ztptr[z + 1]; ++t) {
+            acc += zpart[t];
+            cnt += ztask[t].z - ztask[t].y;
+        }
+        zmean[z] = cnt > 0 ? acc / (double)cnt : 0.0;
+    }
+}
+
 __global__ void __launch_bounds__(TB) k_zone_part(const DSys* __restrict__ sys, const int2* __restrict__ tab,
                                                   int which) {
     const int2 e = tab[blockIdx.x];
     const DSys& s = sys[e.x];
     if (which == 1 && s.state[ST_ACTIVE] == 0) return;
-    const int4 tk = s.ztask[e.y];
-    const float* x = which == 1 ? s.r : s.x;
-    double v[1] = {0.0};
-    for (int p = tk.y + threadIdx.x; p < tk.z; p += TB) v[0] += (double)x[s.zcells[p]];
-    __shared__ double out[1];
-    block_sums<1>(v, out);
-    __syncthreads();
-    if (threadIdx.x == 0) s.zpart[e.y] = out[0];
+    zone_task_sum(s.ztask[e.y], s.zcells, which == 1 ? s.r : s.x, &s.zpart[e.y]);
 }
 
 __global__ void __launch_bounds__(TB) k_zone_fin(const DSys* __restrict__ sys, const int2* __restrict__ tab,
                                                  int which) {
     const DSys& s = sys[tab[blockIdx.x].x];
     if (which == 1 && s.state[ST_ACTIVE] == 0) return;
-    for (int z = threadIdx.x; z < s.nclosed; z += TB) {
-        double acc = 0.0;
-        int cnt = 0;
-        for (int t = s.ztptr[z]; t < s.ztptr[z + 1]; ++t) {
-            acc += s.zpart[t];
-            cnt += s.ztask[t].z - s.ztask[t].y;
-        }
-        s.zmean[z] = cnt > 0 ? acc / (double)cnt : 0.0;
-    }
+    zone_means(s.nclosed, s.ztptr, s.ztask, s.zpart, s.zmean);
 }
 
 __global__ void __launch_bounds__(TB) k_zone_sub(const DSys* __restrict__ sys, const int2* __restrict__ tab,

@@ -10,6 +10,8 @@
 #include <cstdio>
 #include <cstring>
 #include <map>
+#include <new>
+#include <stdexcept>
 #include <string>
 #include <vector>
 
@@ -58,6 +60,10 @@ struct GridDev {
     double* cinv = nullptr;       // the coarsest inverse whole (nc <= CP_MIN) ...
     double* cinvp = nullptr;      // ... or its upper triangle as 64x64 tiles
     int cnt = 0;                  // strips of cinvp (0: cinv)
+    bool dense = false;           // the dense inverse exists (coarsest <= kDenseMax unknowns)
+    int coarse = PHG_COARSE_EXACT;   // how batches created from now on solve the coarsest level
+    int* gs_ord = nullptr;        // Gauss-Seidel level schedule of the coarsest level (HostHierarchy)
+    int* gs_lptr = nullptr;
     float* invd = nullptr;        // 1/diag of level 0, 0 where diag <= 0
     uint8_t* mask = nullptr;      // level-0 face mask when every coefficient is 0 or 1
     bool binary = false;
@@ -125,6 +131,9 @@ struct BatchDev {
     int4* tab_cr = nullptr;        // packed inverses: (first system, count, strip)
     int tab_cr_len = 0;
     int cp_sys = 1;                // systems per packed-solve block
+    int2* tab_gs = nullptr;        // systems whose coarsest level takes the Gauss-Seidel sweeps
+    int tab_gs_len = 0;
+    int maxnc_gs = 0;              // their largest coarsest level (LDS of k_vc_coarse_gs)
     double* scal = nullptr;
     int* state = nullptr;
     int* d_count = nullptr;
@@ -135,6 +144,7 @@ struct BatchDev {
     hipGraph_t graph_full = nullptr;     // set-up and graph_full_n iterations
     hipGraphExec_t graph_full_exec = nullptr;
     int graph_full_n = 0;
+    int graph_full_hi = -1;              // the refine > 0 flag graph_full was captured with
 };
 
 }  // namespace phg
@@ -170,6 +180,19 @@ int fail(phg_ctx* ctx, int code, const char* fmt, ...) {
     va_end(ap);
     if (ctx) ctx->err = buf;
     return code;
+}
+
+// Host set-up allocates with std::vector: an exception must not cross the C
+// ABI, so the entry points that do host work run inside this guard
+template <class F>
+int guarded(phg_ctx* ctx, F&& f) {
+    try {
+        return f();
+    } catch (const std::bad_alloc&) {
+        return fail(ctx, PHG_ENOMEM, "host memory exhausted");
+    } catch (const std::exception& e) {
+        return fail(ctx, PHG_EINVAL, "host exception: %s", e.what());
+    }
 }
 
 #define HIP_OK(call)                                                                                    \
@@ -386,6 +409,9 @@ void vcycle_t(phg_ctx* ctx, BatchDev* B, int first) {
         case CG_SYS_MAX / 2: coarse(k_vc_coarse<VT, CG_SYS_MAX / 2>, CG_SYS_MAX / 2); break;
         default: coarse(k_vc_coarse<VT, CG_SYS_MAX>, CG_SYS_MAX); break;
     }
+    if (B->tab_gs_len > 0)
+        launch(ctx, K_VC_COARSE, k_vc_coarse_gs<CC, VT>, B->tab_gs_len, 2 * (size_t)B->maxnc_gs * sizeof(float), S,
+               (const int*)B->state, (const int2*)B->tab_gs);
     if (B->tab_cp_len > 0) {
         auto packed = [&](auto kern, int cgs) {
             launch(ctx, K_VC_COARSE, kern, B->tab_cp_len, coarse_p_lds(cgs), S, (const int*)B->state,
@@ -679,12 +705,6 @@ bool plan_geometry(int nx, int ny, int nz, const float* ax, const float* ay, con
     P.ny = ny;
     P.nz = nz;
     if (!build_hierarchy(nx, ny, nz, ax, ay, az, blocked, wind_closed, kDenseMax, P.H, err)) return false;
-    if (P.H.nc > kDenseMax) {
-        char buf[128];
-        snprintf(buf, sizeof buf, "coarsest level has %d unknowns (limit %d)", P.H.nc, kDenseMax);
-        *err = buf;
-        return false;
-    }
     compute_zones(nx, ny, nz, ax, ay, az, blocked, P.zone, P.zone_open);
     return true;
 }
@@ -695,17 +715,19 @@ extern "C" {
 
 int phg_grid_plan_build(int nx, int ny, int nz, const float* ax, const float* ay, const float* az,
                         const uint8_t* blocked, const uint8_t* wind_closed, phg_grid_plan** plan_out) {
-    if (!plan_out) return PHG_EINVAL;
-    *plan_out = nullptr;
-    if (!ax || !ay || !az || !blocked || !check_shape(nx, ny, nz).empty()) return PHG_EINVAL;
-    phg_grid_plan* P = new phg_grid_plan();
-    std::string err;
-    if (!plan_geometry(nx, ny, nz, ax, ay, az, blocked, wind_closed, *P, &err)) {
-        delete P;
-        return PHG_EINVAL;
-    }
-    *plan_out = P;
-    return PHG_OK;
+    return guarded(nullptr, [&]() -> int {
+        if (!plan_out) return PHG_EINVAL;
+        *plan_out = nullptr;
+        if (!ax || !ay || !az || !blocked || !check_shape(nx, ny, nz).empty()) return PHG_EINVAL;
+        phg_grid_plan* P = new phg_grid_plan();
+        std::string err;
+        if (!plan_geometry(nx, ny, nz, ax, ay, az, blocked, wind_closed, *P, &err)) {
+            delete P;
+            return PHG_EINVAL;
+        }
+        *plan_out = P;
+        return PHG_OK;
+    });
 }
 
 void phg_grid_plan_free(phg_grid_plan* plan) { delete plan; }
@@ -721,165 +743,203 @@ int phg_grid_create_planned(phg_ctx* ctx, int nx, int ny, int nz, float dx, cons
                             const float* fz, const float* ax, const float* ay, const float* az,
                             const uint8_t* blocked, const uint8_t* wind_closed, int precision, phg_grid_plan* plan,
                             int* grid_out) {
-    if (!ctx || !grid_out) return PHG_EINVAL;
-    const std::string bad = check_shape(nx, ny, nz);
-    if (!bad.empty()) return fail(ctx, PHG_EINVAL, "%s", bad.c_str());
-    if (!(dx > 0)) return fail(ctx, PHG_EINVAL, "bad cell size dx=%g", dx);
-    if (!fx || !fy || !fz || !ax || !ay || !az || !blocked) return fail(ctx, PHG_EINVAL, "null geometry array");
-    if (precision < PHG_PREC_FP32 || precision > PHG_PREC_MIXED_BF16)
-        return fail(ctx, PHG_EINVAL, "unknown precision %d", precision);
-    if (plan && (plan->used || plan->nx != nx || plan->ny != ny || plan->nz != nz))
-        return fail(ctx, PHG_EINVAL, "the plan is used or was built for another grid shape");
-    HIP_OK(hipSetDevice(ctx->device));
-    GridDev* G = new GridDev();
-    G->nx = nx;
-    G->ny = ny;
-    G->nz = nz;
-    G->n = nx * ny * nz;
-    G->dx = dx;
-    G->prec = precision;
-    if (plan) {
-        G->H = std::move(plan->H);
-        G->zone = std::move(plan->zone);
-        G->zone_open = std::move(plan->zone_open);
-        plan->used = true;
-    } else {
-        phg_grid_plan P;
-        std::string err;
-        if (!plan_geometry(nx, ny, nz, ax, ay, az, blocked, wind_closed, P, &err)) {
-            delete G;
-            return fail(ctx, PHG_EINVAL, "%s", err.c_str());
-        }
-        G->H = std::move(P.H);
-        G->zone = std::move(P.zone);
-        G->zone_open = std::move(P.zone_open);
-    }
-    const size_t nfx = (size_t)(nx + 1) * ny * nz, nfy = (size_t)nx * (ny + 1) * nz, nfz = (size_t)nx * ny * (nz + 1);
-    const HostLevel& L0 = G->H.L[0];
-    int rc = 0;
-#define UP(dst, src, cnt) \
-    if (!rc) rc = upload(ctx, G->allocs, &dst, src, cnt);
-    UP(G->ax, L0.ax.data(), nfx)
-    UP(G->ay, L0.ay.data(), nfy)
-    UP(G->az, L0.az.data(), nfz)
-    UP(G->fx, fx, nfx)
-    UP(G->fy, fy, nfy)
-    UP(G->fz, fz, nfz)
-    UP(G->blocked, blocked, (size_t)G->n)
-    if (!rc) {
-        if (precision == PHG_PREC_FP32) {
-            G->axc = G->ax;
-            G->ayc = G->ay;
-            G->azc = G->az;
+    return guarded(ctx, [&]() -> int {
+        if (!ctx || !grid_out) return PHG_EINVAL;
+        const std::string bad = check_shape(nx, ny, nz);
+        if (!bad.empty()) return fail(ctx, PHG_EINVAL, "%s", bad.c_str());
+        if (!(dx > 0)) return fail(ctx, PHG_EINVAL, "bad cell size dx=%g", dx);
+        if (!fx || !fy || !fz || !ax || !ay || !az || !blocked) return fail(ctx, PHG_EINVAL, "null geometry array");
+        if (precision < PHG_PREC_FP32 || precision > PHG_PREC_MIXED_BF16)
+            return fail(ctx, PHG_EINVAL, "unknown precision %d", precision);
+        if (plan && (plan->used || plan->nx != nx || plan->ny != ny || plan->nz != nz))
+            return fail(ctx, PHG_EINVAL, "the plan is used or was built for another grid shape");
+        HIP_OK(hipSetDevice(ctx->device));
+        GridDev* G = new GridDev();
+        G->nx = nx;
+        G->ny = ny;
+        G->nz = nz;
+        G->n = nx * ny * nz;
+        G->dx = dx;
+        G->prec = precision;
+        if (plan) {
+            G->H = std::move(plan->H);
+            G->zone = std::move(plan->zone);
+            G->zone_open = std::move(plan->zone_open);
+            plan->used = true;
         } else {
-            rc = upload_ct(ctx, G->allocs, &G->axc, L0.ax, precision);
-            if (!rc) rc = upload_ct(ctx, G->allocs, &G->ayc, L0.ay, precision);
-            if (!rc) rc = upload_ct(ctx, G->allocs, &G->azc, L0.az, precision);
-        }
-    }
-    const int nl = (int)G->H.L.size();
-    G->lv.resize(nl);
-    for (int l = 0; l < nl && !rc; ++l) {
-        const HostLevel& L = G->H.L[l];
-        GridDev::Lev& D = G->lv[l];
-        D.n = L.n;
-        D.ellk = L.ellk;
-        if (l > 0) {
-            UP(D.col, L.ell_col.data(), L.ell_col.size())
-            if (!rc) rc = upload_ct(ctx, G->allocs, &D.val, L.ell_val, precision);
-            if (!rc) rc = upload_ct(ctx, G->allocs, &D.diag, L.diag, precision);
-        }
-        if (l + 1 < nl) {
-            UP(D.agg, L.agg.data(), L.agg.size())
-            UP(D.cptr, L.cptr.data(), L.cptr.size())
-            UP(D.cidx, L.cidx.data(), L.cidx.size())
-        }
-    }
-    if (G->H.nc > CP_MIN) {
-        // the upper triangle in tiles (I <= J) of 64x64, each column-major
-        const int nc = G->H.nc, nt = (nc + 63) / 64;
-        std::vector<double> tp((size_t)nt * (nt + 1) / 2 * 4096, 0.0);
-        size_t tix = 0;
-        for (int I = 0; I < nt; ++I)
-            for (int J = I; J < nt; ++J, ++tix)
-                for (int c = 0; c < 64 && 64 * J + c < nc; ++c)
-                    for (int r = 0; r < 64 && 64 * I + r < nc; ++r)
-                        tp[tix * 4096 + (size_t)c * 64 + r] = G->H.cinv[(size_t)(64 * I + r) * nc + 64 * J + c];
-        UP(G->cinvp, tp.data(), tp.size())
-        G->cnt = nt;
-    } else {
-        UP(G->cinv, G->H.cinv.data(), G->H.cinv.size())
-    }
-    {
-        std::vector<float> invd(G->n, 0.0f);
-        for (int c = 0; c < G->n; ++c)
-            if (L0.diag[c] > 0) invd[c] = 1.0f / L0.diag[c];
-        UP(G->invd, invd.data(), invd.size())
-        // face mask: exact when every level-0 coefficient is 0 or 1
-        bool binary = true;
-        for (const std::vector<float>* a : {&L0.ax, &L0.ay, &L0.az})
-            for (float v : *a)
-                if (v != 0.0f && v != 1.0f) binary = false;
-        if (binary) {
-            std::vector<uint8_t> m(G->n, 0);
-            for (int k = 0; k < nz; ++k)
-                for (int j = 0; j < ny; ++j)
-                    for (int i = 0; i < nx; ++i) {
-                        const int c = i + nx * (j + ny * k);
-                        const int xf = i + (nx + 1) * (j + ny * k), yf = i + nx * (j + (ny + 1) * k);
-                        const float a[6] = {L0.ax[xf], L0.ax[xf + 1], L0.ay[yf], L0.ay[yf + nx], L0.az[c],
-                                            L0.az[c + nx * ny]};
-                        uint8_t b = 0;
-                        for (int d = 0; d < 6; ++d)
-                            if (a[d] > 0) b |= (uint8_t)(1u << d);
-                        m[c] = b;
-                    }
-            UP(G->mask, m.data(), m.size())
-            G->binary = true;
-        }
-    }
-    // closed zones: cell lists, cut into chunks, for the deterministic mean removal
-    {
-        std::vector<int> zid(G->zone_open.size(), -1);
-        int nclosed = 0;
-        for (size_t z = 0; z < G->zone_open.size(); ++z)
-            if (!G->zone_open[z]) zid[z] = nclosed++;
-        std::vector<int> zptr(nclosed + 1, 0), zclosed(G->n, -1);
-        for (int c = 0; c < G->n; ++c)
-            if (G->zone[c] >= 0 && zid[G->zone[c]] >= 0) {
-                zclosed[c] = zid[G->zone[c]];
-                zptr[zclosed[c] + 1]++;
+            phg_grid_plan P;
+            std::string err;
+            if (!plan_geometry(nx, ny, nz, ax, ay, az, blocked, wind_closed, P, &err)) {
+                delete G;
+                return fail(ctx, PHG_EINVAL, "%s", err.c_str());
             }
-        for (int z = 0; z < nclosed; ++z) zptr[z + 1] += zptr[z];
-        std::vector<int> zcells(std::max(zptr[nclosed], 1)), fill(zptr.begin(), zptr.end() - 1);
-        for (int c = 0; c < G->n; ++c)
-            if (zclosed[c] >= 0) zcells[fill[zclosed[c]]++] = c;
-        G->nclosed = nclosed;
-        std::vector<int4> tasks;
-        std::vector<int> ztptr(nclosed + 1, 0);
-        for (int z = 0; z < nclosed; ++z) {
-            ztptr[z] = (int)tasks.size();
-            for (int b = zptr[z]; b < zptr[z + 1]; b += kZoneChunk)
-                tasks.push_back(make_int4(z, b, std::min(zptr[z + 1], b + kZoneChunk), 0));
+            G->H = std::move(P.H);
+            G->zone = std::move(P.zone);
+            G->zone_open = std::move(P.zone_open);
         }
-        ztptr[nclosed] = (int)tasks.size();
-        G->nztask = (int)tasks.size();
-        if (tasks.empty()) tasks.push_back(make_int4(0, 0, 0, 0));
-        UP(G->ztask, tasks.data(), tasks.size())
-        UP(G->ztptr, ztptr.data(), ztptr.size())
-        UP(G->zcells, zcells.data(), zcells.size())
-        UP(G->zclosed, zclosed.data(), zclosed.size())
+        const size_t nfx = (size_t)(nx + 1) * ny * nz, nfy = (size_t)nx * (ny + 1) * nz, nfz = (size_t)nx * ny * (nz + 1);
+        const HostLevel& L0 = G->H.L[0];
+        int rc = 0;
+    #define UP(dst, src, cnt) \
+        if (!rc) rc = upload(ctx, G->allocs, &dst, src, cnt);
+        UP(G->ax, L0.ax.data(), nfx)
+        UP(G->ay, L0.ay.data(), nfy)
+        UP(G->az, L0.az.data(), nfz)
+        UP(G->fx, fx, nfx)
+        UP(G->fy, fy, nfy)
+        UP(G->fz, fz, nfz)
+        UP(G->blocked, blocked, (size_t)G->n)
+        if (!rc) {
+            if (precision == PHG_PREC_FP32) {
+                G->axc = G->ax;
+                G->ayc = G->ay;
+                G->azc = G->az;
+            } else {
+                rc = upload_ct(ctx, G->allocs, &G->axc, L0.ax, precision);
+                if (!rc) rc = upload_ct(ctx, G->allocs, &G->ayc, L0.ay, precision);
+                if (!rc) rc = upload_ct(ctx, G->allocs, &G->azc, L0.az, precision);
+            }
+        }
+        const int nl = (int)G->H.L.size();
+        G->lv.resize(nl);
+        for (int l = 0; l < nl && !rc; ++l) {
+            const HostLevel& L = G->H.L[l];
+            GridDev::Lev& D = G->lv[l];
+            D.n = L.n;
+            D.ellk = L.ellk;
+            if (l > 0) {
+                UP(D.col, L.ell_col.data(), L.ell_col.size())
+                if (!rc) rc = upload_ct(ctx, G->allocs, &D.val, L.ell_val, precision);
+                if (!rc) rc = upload_ct(ctx, G->allocs, &D.diag, L.diag, precision);
+            }
+            if (l + 1 < nl) {
+                UP(D.agg, L.agg.data(), L.agg.size())
+                UP(D.cptr, L.cptr.data(), L.cptr.size())
+                UP(D.cidx, L.cidx.data(), L.cidx.size())
+            }
+        }
+        G->dense = !G->H.cinv.empty();
+        G->coarse = G->dense ? PHG_COARSE_EXACT : PHG_COARSE_GS20;
+        if (!G->H.gs_ord.empty()) {
+            UP(G->gs_ord, G->H.gs_ord.data(), G->H.gs_ord.size())
+            UP(G->gs_lptr, G->H.gs_lptr.data(), G->H.gs_lptr.size())
+        }
+        if (!G->dense) {
+            // the coarsest level is left to the Gauss-Seidel sweeps
+        } else if (G->H.nc > CP_MIN) {
+            // the upper triangle in tiles (I <= J) of 64x64, each column-major
+            const int nc = G->H.nc, nt = (nc + 63) / 64;
+            std::vector<double> tp((size_t)nt * (nt + 1) / 2 * 4096, 0.0);
+            size_t tix = 0;
+            for (int I = 0; I < nt; ++I)
+                for (int J = I; J < nt; ++J, ++tix)
+                    for (int c = 0; c < 64 && 64 * J + c < nc; ++c)
+                        for (int r = 0; r < 64 && 64 * I + r < nc; ++r)
+                            tp[tix * 4096 + (size_t)c * 64 + r] = G->H.cinv[(size_t)(64 * I + r) * nc + 64 * J + c];
+            UP(G->cinvp, tp.data(), tp.size())
+            G->cnt = nt;
+        } else {
+            UP(G->cinv, G->H.cinv.data(), G->H.cinv.size())
+        }
+        {
+            std::vector<float> invd(G->n, 0.0f);
+            for (int c = 0; c < G->n; ++c)
+                if (L0.diag[c] > 0) invd[c] = 1.0f / L0.diag[c];
+            UP(G->invd, invd.data(), invd.size())
+            // face mask: exact when every level-0 coefficient is 0 or 1
+            bool binary = true;
+            for (const std::vector<float>* a : {&L0.ax, &L0.ay, &L0.az})
+                for (float v : *a)
+                    if (v != 0.0f && v != 1.0f) binary = false;
+            if (binary) {
+                std::vector<uint8_t> m(G->n, 0);
+                for (int k = 0; k < nz; ++k)
+                    for (int j = 0; j < ny; ++j)
+                        for (int i = 0; i < nx; ++i) {
+                            const int c = i + nx * (j + ny * k);
+                            const int xf = i + (nx + 1) * (j + ny * k), yf = i + nx * (j + (ny + 1) * k);
+                            const float a[6] = {L0.ax[xf], L0.ax[xf + 1], L0.ay[yf], L0.ay[yf + nx], L0.az[c],
+                                                L0.az[c + nx * ny]};
+                            uint8_t b = 0;
+                            for (int d = 0; d < 6; ++d)
+                                if (a[d] > 0) b |= (uint8_t)(1u << d);
+                            m[c] = b;
+                        }
+                UP(G->mask, m.data(), m.size())
+                G->binary = true;
+            }
+        }
+        // closed zones: cell lists, cut into chunks, for the deterministic mean removal
+        {
+            std::vector<int> zid(G->zone_open.size(), -1);
+            int nclosed = 0;
+            for (size_t z = 0; z < G->zone_open.size(); ++z)
+                if (!G->zone_open[z]) zid[z] = nclosed++;
+            std::vector<int> zptr(nclosed + 1, 0), zclosed(G->n, -1);
+            for (int c = 0; c < G->n; ++c)
+                if (G->zone[c] >= 0 && zid[G->zone[c]] >= 0) {
+                    zclosed[c] = zid[G->zone[c]];
+                    zptr[zclosed[c] + 1]++;
+                }
+            for (int z = 0; z < nclosed; ++z) zptr[z + 1] += zptr[z];
+            std::vector<int> zcells(std::max(zptr[nclosed], 1)), fill(zptr.begin(), zptr.end() - 1);
+            for (int c = 0; c < G->n; ++c)
+                if (zclosed[c] >= 0) zcells[fill[zclosed[c]]++] = c;
+            G->nclosed = nclosed;
+            std::vector<int4> tasks;
+            std::vector<int> ztptr(nclosed + 1, 0);
+            for (int z = 0; z < nclosed; ++z) {
+                ztptr[z] = (int)tasks.size();
+                for (int b = zptr[z]; b < zptr[z + 1]; b += kZoneChunk)
+                    tasks.push_back(make_int4(z, b, std::min(zptr[z + 1], b + kZoneChunk), 0));
+            }
+            ztptr[nclosed] = (int)tasks.size();
+            G->nztask = (int)tasks.size();
+            if (tasks.empty()) tasks.push_back(make_int4(0, 0, 0, 0));
+            UP(G->ztask, tasks.data(), tasks.size())
+            UP(G->ztptr, ztptr.data(), ztptr.size())
+            UP(G->zcells, zcells.data(), zcells.size())
+            UP(G->zclosed, zclosed.data(), zclosed.size())
+        }
+        if (!rc) rc = dalloc(ctx, G->allocs, &G->scratch_b, (size_t)G->n);
+        if (!rc) rc = dalloc(ctx, G->allocs, &G->vmax, 1);
+    #undef UP
+        if (rc) {
+            destroy_grid(G);
+            return rc;
+        }
+        const int id = ctx->next_id++;
+        ctx->grids[id] = G;
+        *grid_out = id;
+        return PHG_OK;
+    });
+}
+
+int phg_grid_set_coarse(phg_ctx* ctx, int grid, int mode) {
+    if (!ctx) return PHG_EINVAL;
+    GridDev* G = get_grid(ctx, grid);
+    if (!G) return fail(ctx, PHG_ESTATE, "unknown grid %d", grid);
+    if (mode != PHG_COARSE_EXACT && mode != PHG_COARSE_GS20) return fail(ctx, PHG_EINVAL, "unknown coarse mode %d", mode);
+    if (mode == PHG_COARSE_EXACT && !G->dense)
+        return fail(ctx, PHG_EINVAL, "grid %d: the coarsest level has %d unknowns, too many for the dense inverse (%d)",
+                    grid, G->H.nc, kDenseMax);
+    if (mode == PHG_COARSE_GS20 && G->H.L.size() < 2)
+        return fail(ctx, PHG_EINVAL, "grid %d has no coarse level (its dense solve is exact)", grid);
+    if (mode != G->coarse && G->single_batch >= 0) {
+        // the cached one-system batch was laid out for the other mode
+        const int b = G->single_batch;
+        int rc = phg_batch_destroy(ctx, b);
+        if (rc) return rc;
     }
-    if (!rc) rc = dalloc(ctx, G->allocs, &G->scratch_b, (size_t)G->n);
-    if (!rc) rc = dalloc(ctx, G->allocs, &G->vmax, 1);
-#undef UP
-    if (rc) {
-        destroy_grid(G);
-        return rc;
-    }
-    const int id = ctx->next_id++;
-    ctx->grids[id] = G;
-    *grid_out = id;
+    G->coarse = mode;
+    return PHG_OK;
+}
+
+int phg_grid_get_coarse(phg_ctx* ctx, int grid, int* mode) {
+    if (!ctx || !mode) return PHG_EINVAL;
+    GridDev* G = get_grid(ctx, grid);
+    if (!G) return fail(ctx, PHG_ESTATE, "unknown grid %d", grid);
+    *mode = G->coarse;
     return PHG_OK;
 }
 
@@ -915,15 +975,17 @@ int phg_grid_zones(phg_ctx* ctx, int grid, int* zone, uint8_t* zone_open) {
 
 int phg_hierarchy_shape(int nx, int ny, int nz, const float* ax, const float* ay, const float* az,
                         const uint8_t* blocked, const uint8_t* wind_closed, int* nlevels, int* level_n, int* agg0) {
-    if (!ax || !ay || !az || !blocked || !nlevels) return PHG_EINVAL;
-    HostHierarchy H;
-    std::string err;
-    if (!build_hierarchy(nx, ny, nz, ax, ay, az, blocked, wind_closed, kDenseMax, H, &err)) return PHG_EINVAL;
-    *nlevels = (int)H.L.size();
-    if (level_n)
-        for (size_t l = 0; l < H.L.size(); ++l) level_n[l] = H.L[l].n;
-    if (agg0) std::copy(H.L[0].agg.begin(), H.L[0].agg.end(), agg0);
-    return PHG_OK;
+    return guarded(nullptr, [&]() -> int {
+        if (!ax || !ay || !az || !blocked || !nlevels) return PHG_EINVAL;
+        HostHierarchy H;
+        std::string err;
+        if (!build_hierarchy(nx, ny, nz, ax, ay, az, blocked, wind_closed, kDenseMax, H, &err)) return PHG_EINVAL;
+        *nlevels = (int)H.L.size();
+        if (level_n)
+            for (size_t l = 0; l < H.L.size(); ++l) level_n[l] = H.L[l].n;
+        if (agg0) std::copy(H.L[0].agg.begin(), H.L[0].agg.end(), agg0);
+        return PHG_OK;
+    });
 }
 
 // ---------------------------------------------------------------------------
@@ -932,298 +994,317 @@ int phg_batch_create(phg_ctx* ctx, int nsys, const int* grids, int* batch_out) {
 }
 
 int phg_batch_create_ex(phg_ctx* ctx, int nsys, const int* grids, const phg_batch_opts* opts, int* batch_out) {
-    if (!ctx || nsys <= 0 || !grids || !batch_out) return PHG_EINVAL;
-    if (opts && (opts->kchunk < 0 || opts->level0_blocks < 0 || opts->unfused < 0 || opts->unfused > 1 ||
-                 opts->sweeps0 < 0 || opts->sweeps0 > kMaxSweeps0 ||
-                 (opts->fused_threads != 0 && opts->fused_threads != TB && opts->fused_threads != 2 * TB &&
-                  opts->fused_threads != 4 * TB)))
-        return fail(ctx, PHG_EINVAL, "bad batch options");
-    if (opts)
-        for (int s = 0; s < nsys; ++s) {
-            if (opts->sys_kchunk && opts->sys_kchunk[s] < 0) return fail(ctx, PHG_EINVAL, "bad kchunk of system %d", s);
-            if (opts->sys_sweeps0 && (opts->sys_sweeps0[s] < 0 || opts->sys_sweeps0[s] > kMaxSweeps0))
-                return fail(ctx, PHG_EINVAL, "bad sweep count of system %d", s);
-        }
-    HIP_OK(hipSetDevice(ctx->device));
-    BatchDev* B = new BatchDev();
-    B->nsys = nsys;
-    B->grids.assign(grids, grids + nsys);
-    B->unfused = opts && opts->unfused;
-    B->fuse_nt = opts ? opts->fused_threads : 0;
-    B->h_sys.resize(nsys);
-    int rc = 0;
-    long col_tiles = 0;
-    for (int s = 0; s < nsys && !rc; ++s) {
-        GridDev* G = get_grid(ctx, grids[s]);
-        if (!G) {
-            rc = fail(ctx, PHG_ESTATE, "unknown grid %d", grids[s]);
-            break;
-        }
-        if (s == 0) B->prec = G->prec;
-        else if (G->prec != B->prec) {
-            rc = fail(ctx, PHG_EINVAL, "all grids of a batch must share one precision");
-            break;
-        }
-        B->maxl = std::max(B->maxl, (int)G->lv.size());
-        B->maxnc = std::max(B->maxnc, G->H.nc);
-        B->binary = s == 0 ? G->binary : (B->binary && G->binary);
-        B->maxnx = std::max(B->maxnx, G->nx);
-        col_tiles += tiles(G->nx * G->ny);
-    }
-    if (rc) {
-        delete B;
-        return rc;
-    }
-    // planes per level-0 block: split the columns' k range until the launch
-    // offers about kLevel0Blocks blocks
-    const long want = opts && opts->level0_blocks > 0 ? opts->level0_blocks : kLevel0Blocks;
-    const int nkc_target = (int)std::max(1L, (want + col_tiles - 1) / col_tiles);
-    const size_t vts = vt_size(B->prec);
-    if (!rc) rc = dalloc(ctx, B->allocs, &B->scal, (size_t)nsys * S_NSCAL);
-    if (!rc) rc = dalloc(ctx, B->allocs, &B->state, (size_t)nsys * ST_NSTATE);
-    if (!rc) rc = dalloc(ctx, B->allocs, &B->d_count, 1);
-    if (!rc && hipHostMalloc((void**)&B->h_count, sizeof(int), hipHostMallocDefault) != hipSuccess)
-        rc = fail(ctx, PHG_ENOMEM, "hipHostMalloc failed");
-    for (int s = 0; s < nsys && !rc; ++s) {
-        GridDev* G = get_grid(ctx, grids[s]);
-        DSys& S = B->h_sys[s];
-        std::memset(&S, 0, sizeof(DSys));
-        S.nl = (int)G->lv.size();
-        S.nc = G->H.nc;
-        const int kc = opts && opts->sys_kchunk && opts->sys_kchunk[s] > 0 ? opts->sys_kchunk[s]
-                       : opts && opts->kchunk > 0                        ? opts->kchunk
-                                                                         : 0;
-        S.kchunk = kc > 0 ? std::min(kc, G->nz) : std::max(1, (G->nz + nkc_target - 1) / nkc_target);
-        const int nu = opts && opts->sys_sweeps0 && opts->sys_sweeps0[s] > 0 ? opts->sys_sweeps0[s]
-                       : opts && opts->sweeps0 > 0                          ? opts->sweeps0
-                                                                            : 1;
-        S.nu0 = nu;
-        S.tiles0 = tiles(G->nx * G->ny) * ((G->nz + S.kchunk - 1) / S.kchunk);
-        S.cinv = G->cinv;
-        S.cinvp = G->cinvp;
-        S.cnt = G->cnt;
-        S.invd = G->invd;
-        S.scal = B->scal + (size_t)s * S_NSCAL;
-        S.state = B->state + (size_t)s * ST_NSTATE;
-        S.nclosed = G->nclosed;
-        S.nztask = G->nztask;
-        S.ztask = G->ztask;
-        S.ztptr = G->ztptr;
-        S.zcells = G->zcells;
-        S.zclosed = G->zclosed;
-        const size_t n0 = G->n;
-        float* v4 = nullptr;
-        rc = dalloc(ctx, B->allocs, &v4, 7 * n0);
-        if (rc) break;
-        S.r = v4;
-        S.q = v4 + n0;
-        S.d0 = v4 + 2 * n0;
-        S.d1 = v4 + 3 * n0;
-        S.z = v4 + 4 * n0;   // the level-0 iterate stays FP32 in every precision: it feeds d directly
-        S.r1 = v4 + 5 * n0;
-        S.z2 = v4 + 6 * n0;
-        rc = dalloc(ctx, B->allocs, &S.part, (size_t)S.tiles0 * NPART);
-        if (!rc && S.cnt) rc = dalloc(ctx, B->allocs, &S.cpart, (size_t)S.cnt * S.cnt * 64);
-        if (!rc) rc = dalloc(ctx, B->allocs, &S.zpart, (size_t)std::max(G->nztask, 1));
-        if (!rc) rc = dalloc(ctx, B->allocs, &S.zmean, (size_t)std::max(G->nclosed, 1));
-        if (rc) break;
-        // V-cycle storage: level-0 residual scratch, coarse x b r t
-        size_t total = n0;
-        for (int l = 1; l < S.nl; ++l) total += 4 * (size_t)G->lv[l].n;
-        char* slab = nullptr;
-        rc = dalloc(ctx, B->allocs, &slab, total * vts);
-        if (rc) break;
-        for (int l = 0; l < S.nl; ++l) {
-            DLevel& D = S.L[l];
-            const GridDev::Lev& GL = G->lv[l];
-            D.n = GL.n;
-            D.ellk = GL.ellk;
-            D.col = GL.col;
-            D.val = GL.val;
-            D.diag = GL.diag;
-            D.agg = GL.agg;
-            D.cptr = GL.cptr;
-            D.cidx = GL.cidx;
-            if (l == 0) {
-                D.nx = G->nx;
-                D.ny = G->ny;
-                D.nz = G->nz;
-                D.ax32 = G->ax;
-                D.ay32 = G->ay;
-                D.az32 = G->az;
-                D.axc = G->axc;
-                D.ayc = G->ayc;
-                D.azc = G->azc;
-                D.mask = G->mask;
-                D.r = slab;
-                slab += n0 * vts;
-            } else {
-                D.x = slab;
-                slab += GL.n * vts;
-                D.b = slab;
-                slab += GL.n * vts;
-                D.r = slab;
-                slab += GL.n * vts;
-                D.t = slab;
-                slab += GL.n * vts;
+    return guarded(ctx, [&]() -> int {
+        if (!ctx || nsys <= 0 || !grids || !batch_out) return PHG_EINVAL;
+        if (opts && (opts->kchunk < 0 || opts->level0_blocks < 0 || opts->unfused < 0 || opts->unfused > 1 ||
+                     opts->sweeps0 < 0 || opts->sweeps0 > kMaxSweeps0 ||
+                     (opts->fused_threads != 0 && opts->fused_threads != TB && opts->fused_threads != 2 * TB &&
+                      opts->fused_threads != 4 * TB)))
+            return fail(ctx, PHG_EINVAL, "bad batch options");
+        if (opts)
+            for (int s = 0; s < nsys; ++s) {
+                if (opts->sys_kchunk && opts->sys_kchunk[s] < 0) return fail(ctx, PHG_EINVAL, "bad kchunk of system %d", s);
+                if (opts->sys_sweeps0 && (opts->sys_sweeps0[s] < 0 || opts->sys_sweeps0[s] > kMaxSweeps0))
+                    return fail(ctx, PHG_EINVAL, "bad sweep count of system %d", s);
             }
+        HIP_OK(hipSetDevice(ctx->device));
+        BatchDev* B = new BatchDev();
+        B->nsys = nsys;
+        B->grids.assign(grids, grids + nsys);
+        B->unfused = opts && opts->unfused;
+        B->fuse_nt = opts ? opts->fused_threads : 0;
+        B->h_sys.resize(nsys);
+        int rc = 0;
+        long col_tiles = 0;
+        for (int s = 0; s < nsys && !rc; ++s) {
+            GridDev* G = get_grid(ctx, grids[s]);
+            if (!G) {
+                rc = fail(ctx, PHG_ESTATE, "unknown grid %d", grids[s]);
+                break;
+            }
+            if (s == 0) B->prec = G->prec;
+            else if (G->prec != B->prec) {
+                rc = fail(ctx, PHG_EINVAL, "all grids of a batch must share one precision");
+                break;
+            }
+            B->maxl = std::max(B->maxl, (int)G->lv.size());
+            if (G->coarse == PHG_COARSE_GS20) B->maxnc_gs = std::max(B->maxnc_gs, G->H.nc);
+            else B->maxnc = std::max(B->maxnc, G->H.nc);
+            B->binary = s == 0 ? G->binary : (B->binary && G->binary);
+            B->maxnx = std::max(B->maxnx, G->nx);
+            col_tiles += tiles(G->nx * G->ny);
         }
-    }
-    if (!rc) rc = dalloc(ctx, B->allocs, &B->d_sys, (size_t)nsys);
-    // tile tables: level 0 by column tiles x k-chunks; coarse levels by 256
-    // unknowns of a group of up to CV_GRP consecutive systems on one grid
-    B->tab.assign(1, nullptr);
-    B->tab_len.assign(1, 0);
-    B->tabg.assign(std::max(B->maxl, 1), nullptr);
-    B->tabg_len.assign(std::max(B->maxl, 1), 0);
-    if (!rc) {
-        std::vector<int2> logical;
-        for (int s = 0; s < nsys; ++s)
-            for (int k = 0; k < B->h_sys[s].tiles0; ++k) logical.push_back(make_int2(s, k));
-        std::vector<int2> phys = xcd_order(logical);
-        B->tab_len[0] = (int)phys.size();
-        rc = upload(ctx, B->allocs, &B->tab[0], phys.data(), phys.size());
-        std::vector<int2> wide;
-        for (const int2& e : logical) {
-            const GridDev* G = get_grid(ctx, grids[e.x]);
-            if ((e.y % tiles(G->nx * G->ny)) % 2 == 0) wide.push_back(e);
+        if (rc) {
+            delete B;
+            return rc;
         }
-        std::vector<int2> wphys = xcd_order(wide);
-        B->tabw_len = (int)wphys.size();
-        if (!rc) rc = upload(ctx, B->allocs, &B->tabw, wphys.data(), wphys.size());
-        // the smoothing split: which systems take the fused passes
-        std::vector<int2> lf, lwf, lqf, lu;
-        for (const int2& e : logical) {
-            const GridDev* G = get_grid(ctx, grids[e.x]);
-            const bool f = B->binary && !B->unfused && G->nx <= FUSE_MAX_NX && (G->nx * G->ny) % 2 == 0 &&
-                           B->h_sys[e.x].nu0 == 1;
-            if (f) {
-                lf.push_back(e);
-                const int h = e.y % tiles(G->nx * G->ny);
-                if (B->fuse_nt == 4 * TB || (B->fuse_nt == 0 && G->nx > 128)) {
-                    if (h % 4 == 0) lqf.push_back(e);
+        // planes per level-0 block: split the columns' k range until the launch
+        // offers about kLevel0Blocks blocks
+        const long want = opts && opts->level0_blocks > 0 ? opts->level0_blocks : kLevel0Blocks;
+        const int nkc_target = (int)std::max(1L, (want + col_tiles - 1) / col_tiles);
+        const size_t vts = vt_size(B->prec);
+        if (!rc) rc = dalloc(ctx, B->allocs, &B->scal, (size_t)nsys * S_NSCAL);
+        if (!rc) rc = dalloc(ctx, B->allocs, &B->state, (size_t)nsys * ST_NSTATE);
+        if (!rc) rc = dalloc(ctx, B->allocs, &B->d_count, 1);
+        if (!rc && hipHostMalloc((void**)&B->h_count, sizeof(int), hipHostMallocDefault) != hipSuccess)
+            rc = fail(ctx, PHG_ENOMEM, "hipHostMalloc failed");
+        for (int s = 0; s < nsys && !rc; ++s) {
+            GridDev* G = get_grid(ctx, grids[s]);
+            DSys& S = B->h_sys[s];
+            std::memset(&S, 0, sizeof(DSys));
+            S.nl = (int)G->lv.size();
+            S.nc = G->H.nc;
+            const int kc = opts && opts->sys_kchunk && opts->sys_kchunk[s] > 0 ? opts->sys_kchunk[s]
+                           : opts && opts->kchunk > 0                        ? opts->kchunk
+                                                                             : 0;
+            S.kchunk = kc > 0 ? std::min(kc, G->nz) : std::max(1, (G->nz + nkc_target - 1) / nkc_target);
+            const int nu = opts && opts->sys_sweeps0 && opts->sys_sweeps0[s] > 0 ? opts->sys_sweeps0[s]
+                           : opts && opts->sweeps0 > 0                          ? opts->sweeps0
+                                                                                : 1;
+            S.nu0 = nu;
+            S.tiles0 = tiles(G->nx * G->ny) * ((G->nz + S.kchunk - 1) / S.kchunk);
+            S.cinv = G->cinv;
+            S.cinvp = G->cinvp;
+            S.cnt = G->cnt;
+            S.gs_ord = G->gs_ord;
+            S.gs_lptr = G->gs_lptr;
+            S.gs_nf = G->H.gs_nf;
+            S.gs_nb = G->H.gs_nb;
+            S.invd = G->invd;
+            S.scal = B->scal + (size_t)s * S_NSCAL;
+            S.state = B->state + (size_t)s * ST_NSTATE;
+            S.nclosed = G->nclosed;
+            S.nztask = G->nztask;
+            S.ztask = G->ztask;
+            S.ztptr = G->ztptr;
+            S.zcells = G->zcells;
+            S.zclosed = G->zclosed;
+            const size_t n0 = G->n;
+            float* v4 = nullptr;
+            rc = dalloc(ctx, B->allocs, &v4, 7 * n0);
+            if (rc) break;
+            S.r = v4;
+            S.q = v4 + n0;
+            S.d0 = v4 + 2 * n0;
+            S.d1 = v4 + 3 * n0;
+            S.z = v4 + 4 * n0;   // the level-0 iterate stays FP32 in every precision: it feeds d directly
+            S.r1 = v4 + 5 * n0;
+            S.z2 = v4 + 6 * n0;
+            rc = dalloc(ctx, B->allocs, &S.part, (size_t)S.tiles0 * NPART);
+            if (!rc && S.cnt) rc = dalloc(ctx, B->allocs, &S.cpart, (size_t)S.cnt * S.cnt * 64);
+            if (!rc) rc = dalloc(ctx, B->allocs, &S.zpart, (size_t)std::max(G->nztask, 1));
+            if (!rc) rc = dalloc(ctx, B->allocs, &S.zmean, (size_t)std::max(G->nclosed, 1));
+            if (rc) break;
+            // V-cycle storage: level-0 residual scratch, coarse x b r t
+            size_t total = n0;
+            for (int l = 1; l < S.nl; ++l) total += 4 * (size_t)G->lv[l].n;
+            char* slab = nullptr;
+            rc = dalloc(ctx, B->allocs, &slab, total * vts);
+            if (rc) break;
+            for (int l = 0; l < S.nl; ++l) {
+                DLevel& D = S.L[l];
+                const GridDev::Lev& GL = G->lv[l];
+                D.n = GL.n;
+                D.ellk = GL.ellk;
+                D.col = GL.col;
+                D.val = GL.val;
+                D.diag = GL.diag;
+                D.agg = GL.agg;
+                D.cptr = GL.cptr;
+                D.cidx = GL.cidx;
+                if (l == 0) {
+                    D.nx = G->nx;
+                    D.ny = G->ny;
+                    D.nz = G->nz;
+                    D.ax32 = G->ax;
+                    D.ay32 = G->ay;
+                    D.az32 = G->az;
+                    D.axc = G->axc;
+                    D.ayc = G->ayc;
+                    D.azc = G->azc;
+                    D.mask = G->mask;
+                    D.r = slab;
+                    slab += n0 * vts;
                 } else {
-                    if (h % 2 == 0) lwf.push_back(e);
-                    B->maxnx_f = std::max(B->maxnx_f, G->nx);
+                    D.x = slab;
+                    slab += GL.n * vts;
+                    D.b = slab;
+                    slab += GL.n * vts;
+                    D.r = slab;
+                    slab += GL.n * vts;
+                    D.t = slab;
+                    slab += GL.n * vts;
                 }
-            } else {
-                lu.push_back(e);
-                B->maxnu0_u = std::max(B->maxnu0_u, B->h_sys[e.x].nu0);
             }
         }
-        std::vector<int2> pf = xcd_order(lf), pwf = xcd_order(lwf), pqf = xcd_order(lqf), pu = xcd_order(lu);
-        B->tabf_len = (int)pf.size();
-        B->tabwf_len = B->fuse_nt == TB ? 0 : (int)pwf.size();
-        B->tabqf_len = (int)pqf.size();
-        B->tabu_len = (int)pu.size();
-        if (!rc && !pf.empty()) rc = upload(ctx, B->allocs, &B->tabf, pf.data(), pf.size());
-        if (!rc && B->tabwf_len) rc = upload(ctx, B->allocs, &B->tabwf, pwf.data(), pwf.size());
-        if (!rc && !pqf.empty()) rc = upload(ctx, B->allocs, &B->tabqf, pqf.data(), pqf.size());
-        if (!rc && !pu.empty()) rc = upload(ctx, B->allocs, &B->tabu, pu.data(), pu.size());
-    }
-    for (int l = 1; l < B->maxl && !rc; ++l) {
-        // groups of CV_GRP, or of 2 or 1 while the level's launches would offer
-        // the chip fewer than kCoarseBlocks blocks (each system's arithmetic is
-        // the same in any group)
-        auto level_tiles = [&](int grp, std::vector<int4>* out) {
-            long nb = 0;
-            for (int s = 0; s < nsys;) {
-                GridDev* G = get_grid(ctx, grids[s]);
-                int cnt = 1;
-                while (s + cnt < nsys && cnt < grp && grids[s + cnt] == grids[s]) ++cnt;
-                if ((int)G->lv.size() > l) {
-                    nb += tiles(G->lv[l].n);
-                    if (out)
-                        for (int k = 0; k < tiles(G->lv[l].n); ++k)
-                            out->push_back(make_int4(s, cnt, k, (int)G->lv.size() | std::min(TB, G->lv[l].n - k * TB) << 8));
-                }
-                s += cnt;
+        if (!rc) rc = dalloc(ctx, B->allocs, &B->d_sys, (size_t)nsys);
+        // tile tables: level 0 by column tiles x k-chunks; coarse levels by 256
+        // unknowns of a group of up to CV_GRP consecutive systems on one grid
+        B->tab.assign(1, nullptr);
+        B->tab_len.assign(1, 0);
+        B->tabg.assign(std::max(B->maxl, 1), nullptr);
+        B->tabg_len.assign(std::max(B->maxl, 1), 0);
+        if (!rc) {
+            std::vector<int2> logical;
+            for (int s = 0; s < nsys; ++s)
+                for (int k = 0; k < B->h_sys[s].tiles0; ++k) logical.push_back(make_int2(s, k));
+            std::vector<int2> phys = xcd_order(logical);
+            B->tab_len[0] = (int)phys.size();
+            rc = upload(ctx, B->allocs, &B->tab[0], phys.data(), phys.size());
+            std::vector<int2> wide;
+            for (const int2& e : logical) {
+                const GridDev* G = get_grid(ctx, grids[e.x]);
+                if ((e.y % tiles(G->nx * G->ny)) % 2 == 0) wide.push_back(e);
             }
-            return nb;
-        };
-        int grp = CV_GRP;
-        while (grp > 1 && level_tiles(grp, nullptr) < kCoarseBlocks) grp /= 2;
-        std::vector<int4> logical;
-        level_tiles(grp, &logical);
-        std::vector<int4> phys = xcd_order(logical);
-        B->tabg_len[l] = (int)phys.size();
-        if (!phys.empty()) rc = upload(ctx, B->allocs, &B->tabg[l], phys.data(), phys.size());
-    }
-    if (!rc) {
-        std::vector<int2> ts(nsys), tz;
-        for (int s = 0; s < nsys; ++s) {
-            ts[s] = make_int2(s, 0);
-            for (int k = 0; k < B->h_sys[s].nztask; ++k) tz.push_back(make_int2(s, k));
+            std::vector<int2> wphys = xcd_order(wide);
+            B->tabw_len = (int)wphys.size();
+            if (!rc) rc = upload(ctx, B->allocs, &B->tabw, wphys.data(), wphys.size());
+            // the smoothing split: which systems take the fused passes
+            std::vector<int2> lf, lwf, lqf, lu;
+            for (const int2& e : logical) {
+                const GridDev* G = get_grid(ctx, grids[e.x]);
+                const bool f = B->binary && !B->unfused && G->nx <= FUSE_MAX_NX && (G->nx * G->ny) % 2 == 0 &&
+                               B->h_sys[e.x].nu0 == 1;
+                if (f) {
+                    lf.push_back(e);
+                    const int h = e.y % tiles(G->nx * G->ny);
+                    if (B->fuse_nt == 4 * TB || (B->fuse_nt == 0 && G->nx > 128)) {
+                        if (h % 4 == 0) lqf.push_back(e);
+                    } else {
+                        if (h % 2 == 0) lwf.push_back(e);
+                        B->maxnx_f = std::max(B->maxnx_f, G->nx);
+                    }
+                } else {
+                    lu.push_back(e);
+                    B->maxnu0_u = std::max(B->maxnu0_u, B->h_sys[e.x].nu0);
+                }
+            }
+            std::vector<int2> pf = xcd_order(lf), pwf = xcd_order(lwf), pqf = xcd_order(lqf), pu = xcd_order(lu);
+            B->tabf_len = (int)pf.size();
+            B->tabwf_len = B->fuse_nt == TB ? 0 : (int)pwf.size();
+            B->tabqf_len = (int)pqf.size();
+            B->tabu_len = (int)pu.size();
+            if (!rc && !pf.empty()) rc = upload(ctx, B->allocs, &B->tabf, pf.data(), pf.size());
+            if (!rc && B->tabwf_len) rc = upload(ctx, B->allocs, &B->tabwf, pwf.data(), pwf.size());
+            if (!rc && !pqf.empty()) rc = upload(ctx, B->allocs, &B->tabqf, pqf.data(), pqf.size());
+            if (!rc && !pu.empty()) rc = upload(ctx, B->allocs, &B->tabu, pu.data(), pu.size());
         }
-        rc = upload(ctx, B->allocs, &B->tab_sys, ts.data(), ts.size());
-        B->tab_z_len = (int)tz.size();
-        if (!rc && !tz.empty()) rc = upload(ctx, B->allocs, &B->tab_z, tz.data(), tz.size());
-    }
-    if (!rc) {
-        // coarsest-solve blocks: runs of up to cg_sys consecutive systems on one grid
-        std::vector<int4> tc;
-        // systems per block: the longest run on one grid rounded up to 1, 4, 8
-        // or 16 (16 while their right-hand sides fit 64 KB of LDS, else 8), for
-        // the grids whose inverse is read whole, then smaller while that launch
-        // would offer the chip fewer than kCoarseBlocks blocks (the grouping
-        // leaves every system's sums as they are); up to 8 for the packed ones
-        int maxrun = 1, maxrun_p = 1;
-        for (int s = 0; s < nsys;) {
-            int cnt = 1;
-            while (s + cnt < nsys && grids[s + cnt] == grids[s]) ++cnt;
-            if (get_grid(ctx, grids[s])->cnt) maxrun_p = std::max(maxrun_p, cnt);
-            else maxrun = std::max(maxrun, cnt);
-            s += cnt;
+        for (int l = 1; l < B->maxl && !rc; ++l) {
+            // groups of CV_GRP, or of 2 or 1 while the level's launches would offer
+            // the chip fewer than kCoarseBlocks blocks (each system's arithmetic is
+            // the same in any group)
+            auto level_tiles = [&](int grp, std::vector<int4>* out) {
+                long nb = 0;
+                for (int s = 0; s < nsys;) {
+                    GridDev* G = get_grid(ctx, grids[s]);
+                    int cnt = 1;
+                    while (s + cnt < nsys && cnt < grp && grids[s + cnt] == grids[s]) ++cnt;
+                    if ((int)G->lv.size() > l) {
+                        nb += tiles(G->lv[l].n);
+                        if (out)
+                            for (int k = 0; k < tiles(G->lv[l].n); ++k)
+                                out->push_back(make_int4(s, cnt, k, (int)G->lv.size() | std::min(TB, G->lv[l].n - k * TB) << 8));
+                    }
+                    s += cnt;
+                }
+                return nb;
+            };
+            int grp = CV_GRP;
+            while (grp > 1 && level_tiles(grp, nullptr) < kCoarseBlocks) grp /= 2;
+            std::vector<int4> logical;
+            level_tiles(grp, &logical);
+            std::vector<int4> phys = xcd_order(logical);
+            B->tabg_len[l] = (int)phys.size();
+            if (!phys.empty()) rc = upload(ctx, B->allocs, &B->tabg[l], phys.data(), phys.size());
         }
-        const int cap = coarse_lds(CG_SYS_MAX, B->maxnc) <= 65536 ? CG_SYS_MAX : CG_SYS_MAX / 2;
-        B->cg_sys = maxrun <= 1 ? 1 : maxrun <= 4 ? 4 : maxrun <= CG_SYS_MAX / 2 ? CG_SYS_MAX / 2 : cap;
-        auto coarse_blocks = [&](int cgs) {
-            long nb = 0;
+        if (!rc) {
+            std::vector<int2> ts(nsys), tz;
+            for (int s = 0; s < nsys; ++s) {
+                ts[s] = make_int2(s, 0);
+                for (int k = 0; k < B->h_sys[s].nztask; ++k) tz.push_back(make_int2(s, k));
+            }
+            rc = upload(ctx, B->allocs, &B->tab_sys, ts.data(), ts.size());
+            B->tab_z_len = (int)tz.size();
+            if (!rc && !tz.empty()) rc = upload(ctx, B->allocs, &B->tab_z, tz.data(), tz.size());
+        }
+        if (!rc) {
+            // coarsest-solve blocks: runs of up to cg_sys consecutive systems on one grid
+            std::vector<int4> tc;
+            // systems per block: the longest run on one grid rounded up to 1, 4, 8
+            // or 16 (16 while their right-hand sides fit 64 KB of LDS, else 8), for
+            // the grids whose inverse is read whole, then smaller while that launch
+            // would offer the chip fewer than kCoarseBlocks blocks (the grouping
+            // leaves every system's sums as they are); up to 8 for the packed ones
+            // (systems whose grid takes the Gauss-Seidel sweeps are left out: one
+            // block each, k_vc_coarse_gs)
+            auto gs_of = [&](int s) { return get_grid(ctx, grids[s])->coarse == PHG_COARSE_GS20; };
+            int maxrun = 1, maxrun_p = 1;
             for (int s = 0; s < nsys;) {
                 int cnt = 1;
                 while (s + cnt < nsys && grids[s + cnt] == grids[s]) ++cnt;
-                const GridDev* G = get_grid(ctx, grids[s]);
-                if (!G->cnt) nb += (long)((cnt + cgs - 1) / cgs) * ((G->H.nc + CG_ROWS - 1) / CG_ROWS);
+                if (gs_of(s)) {
+                } else if (get_grid(ctx, grids[s])->cnt) {
+                    maxrun_p = std::max(maxrun_p, cnt);
+                } else {
+                    maxrun = std::max(maxrun, cnt);
+                }
                 s += cnt;
             }
-            return nb;
-        };
-        while (B->cg_sys > 1 && coarse_blocks(B->cg_sys) < kCoarseBlocks)
-            B->cg_sys = B->cg_sys > CG_SYS_MAX / 2 ? CG_SYS_MAX / 2 : B->cg_sys > 4 ? 4 : 1;
-        B->cp_sys = maxrun_p <= 1 ? 1 : maxrun_p <= 4 ? 4 : 8;
-        std::vector<int4> tp, tr;
-        for (int s = 0; s < nsys;) {
-            const GridDev* G = get_grid(ctx, grids[s]);
-            const int lim = G->cnt ? B->cp_sys : B->cg_sys;
-            int cnt = 1;
-            while (s + cnt < nsys && cnt < lim && grids[s + cnt] == grids[s]) ++cnt;
-            if (G->cnt) {
-                for (int I = 0; I < G->cnt; ++I) {
-                    for (int J = I; J < G->cnt; ++J) tp.push_back(make_int4(s, cnt, I, J));
-                    tr.push_back(make_int4(s, cnt, I, 0));
+            const int cap = coarse_lds(CG_SYS_MAX, B->maxnc) <= 65536 ? CG_SYS_MAX : CG_SYS_MAX / 2;
+            B->cg_sys = maxrun <= 1 ? 1 : maxrun <= 4 ? 4 : maxrun <= CG_SYS_MAX / 2 ? CG_SYS_MAX / 2 : cap;
+            auto coarse_blocks = [&](int cgs) {
+                long nb = 0;
+                for (int s = 0; s < nsys;) {
+                    int cnt = 1;
+                    while (s + cnt < nsys && grids[s + cnt] == grids[s]) ++cnt;
+                    const GridDev* G = get_grid(ctx, grids[s]);
+                    if (!G->cnt && !gs_of(s)) nb += (long)((cnt + cgs - 1) / cgs) * ((G->H.nc + CG_ROWS - 1) / CG_ROWS);
+                    s += cnt;
                 }
-            } else {
-                for (int t = 0; t < (G->H.nc + CG_ROWS - 1) / CG_ROWS; ++t) tc.push_back(make_int4(s, cnt, t, 0));
+                return nb;
+            };
+            while (B->cg_sys > 1 && coarse_blocks(B->cg_sys) < kCoarseBlocks)
+                B->cg_sys = B->cg_sys > CG_SYS_MAX / 2 ? CG_SYS_MAX / 2 : B->cg_sys > 4 ? 4 : 1;
+            B->cp_sys = maxrun_p <= 1 ? 1 : maxrun_p <= 4 ? 4 : 8;
+            std::vector<int4> tp, tr;
+            std::vector<int2> tg;
+            for (int s = 0; s < nsys;) {
+                const GridDev* G = get_grid(ctx, grids[s]);
+                const int lim = G->cnt ? B->cp_sys : B->cg_sys;
+                int cnt = 1;
+                while (s + cnt < nsys && cnt < lim && grids[s + cnt] == grids[s]) ++cnt;
+                if (gs_of(s)) {
+                    for (int g = 0; g < cnt; ++g) tg.push_back(make_int2(s + g, 0));
+                } else if (G->cnt) {
+                    for (int I = 0; I < G->cnt; ++I) {
+                        for (int J = I; J < G->cnt; ++J) tp.push_back(make_int4(s, cnt, I, J));
+                        tr.push_back(make_int4(s, cnt, I, 0));
+                    }
+                } else {
+                    for (int t = 0; t < (G->H.nc + CG_ROWS - 1) / CG_ROWS; ++t) tc.push_back(make_int4(s, cnt, t, 0));
+                }
+                s += cnt;
             }
-            s += cnt;
+            B->tab_coarse_len = (int)tc.size();
+            B->tab_cp_len = (int)tp.size();
+            B->tab_cr_len = (int)tr.size();
+            B->tab_gs_len = (int)tg.size();
+            if (!tc.empty()) rc = upload(ctx, B->allocs, &B->tab_coarse, tc.data(), tc.size());
+            if (!rc && !tg.empty()) rc = upload(ctx, B->allocs, &B->tab_gs, tg.data(), tg.size());
+            if (!rc && !tp.empty()) rc = upload(ctx, B->allocs, &B->tab_cp, tp.data(), tp.size());
+            if (!rc && !tr.empty()) rc = upload(ctx, B->allocs, &B->tab_cr, tr.data(), tr.size());
         }
-        B->tab_coarse_len = (int)tc.size();
-        B->tab_cp_len = (int)tp.size();
-        B->tab_cr_len = (int)tr.size();
-        if (!tc.empty()) rc = upload(ctx, B->allocs, &B->tab_coarse, tc.data(), tc.size());
-        if (!rc && !tp.empty()) rc = upload(ctx, B->allocs, &B->tab_cp, tp.data(), tp.size());
-        if (!rc && !tr.empty()) rc = upload(ctx, B->allocs, &B->tab_cr, tr.data(), tr.size());
-    }
-    if (!rc) rc = upload_sys(ctx, B);
-    if (!rc) rc = hipStreamSynchronize(ctx->stream) == hipSuccess ? 0 : fail(ctx, PHG_EHIP, "sync");
-    if (rc) {
-        destroy_batch(B);
-        return rc;
-    }
-    const int id = ctx->next_id++;
-    ctx->batches[id] = B;
-    *batch_out = id;
-    return PHG_OK;
+        if (!rc) rc = upload_sys(ctx, B);
+        if (!rc) rc = hipStreamSynchronize(ctx->stream) == hipSuccess ? 0 : fail(ctx, PHG_EHIP, "sync");
+        if (rc) {
+            destroy_batch(B);
+            return rc;
+        }
+        const int id = ctx->next_id++;
+        ctx->batches[id] = B;
+        *batch_out = id;
+        return PHG_OK;
+    });
 }
 
 int phg_batch_destroy(phg_ctx* ctx, int batch) {
@@ -1240,120 +1321,129 @@ int phg_batch_destroy(phg_ctx* ctx, int batch) {
 
 int phg_batch_solve(phg_ctx* ctx, int batch, const float* const* b, float* const* x, const phg_solve_params* P,
                     phg_solve_stats* stats) {
-    if (!ctx || !b || !x || !P) return PHG_EINVAL;
-    BatchDev* B = get_batch(ctx, batch);
-    if (!B) return fail(ctx, PHG_ESTATE, "unknown batch %d", batch);
-    if (!(P->tol >= 0) || P->max_iters < 0) return fail(ctx, PHG_EINVAL, "bad tolerance or iteration cap");
-    HIP_OK(hipSetDevice(ctx->device));
-    for (int s = 0; s < B->nsys; ++s) {
-        if (!b[s] || !x[s]) return fail(ctx, PHG_EINVAL, "null vector for system %d", s);
-        B->h_sys[s].b = b[s];
-        B->h_sys[s].x = x[s];
-    }
-    int rc = upload_sys(ctx, B);
-    if (!rc) rc = reset_state(ctx, B, P);
-    if (rc) return rc;
-    const DSys* S = B->d_sys;
-    // r_0 = b - A x_0, the stop test at iteration 0, z_0 = M^-1 r_0, r.z;
-    // a restart projects its residual onto the range of A first
-    auto setup = [&](bool restart) {
-        // with refinement on, every residual formed from x (a warm start's
-        // first one as well as a restart's) accumulates in FP64: an FP32 one
-        // is noise once it nears eps |A| |x|, where a warm start begins
-        const int hi = restart || P->refine > 0 ? 1 : 0;
-        if (B->binary) launch(ctx, K_RESID_INIT, k_resid_init<Mask01>, B->tab_len[0], 0, S, B->tab[0], hi);
-        else launch(ctx, K_RESID_INIT, k_resid_init<float>, B->tab_len[0], 0, S, B->tab[0], hi);
-        if (restart) demean_batch(ctx, B, 1);
-        launch(ctx, K_FIN, k_fin_init, B->nsys, 0, S, B->tab_sys);
-        vcycle(ctx, B, 1);
-        launch(ctx, K_FIN, k_fin_rz0, B->nsys, 0, S, B->tab_sys);
-    };
-    const int check = P->check_every > 0 ? P->check_every : 4;
-    const bool graph = P->use_graph && !ctx->prof;
-    const bool whole = P->use_graph == 2 && !ctx->prof;
-    // after the whole-solve graph, the systems still running are the slow
-    // tail: check a quarter as far apart (at least 4 iterations)
-    const int tail_check = whole ? std::min(check, std::max(4, check / 4)) : check;
-    // rounds: the first from x_0; each later one restarts the systems that
-    // converged from their x and the true residual b - A x (those already
-    // within tol stop at once); the cap applies per round
-    for (int round = 0;; ++round) {
-        int done = 0;
-        bool live = true;
-        if (whole && round == 0) {
-            // the set-up and the first `check` iterations as one graph: one launch
-            // per solve when every system stops within them (a system that has
-            // stopped leaves each later kernel at once)
-            const int N = std::min(check, P->max_iters);
-            if (!B->graph_full_exec || B->graph_full_n != N) {
-                if (B->graph_full_exec) (void)hipGraphExecDestroy(B->graph_full_exec);
-                if (B->graph_full) (void)hipGraphDestroy(B->graph_full);
-                B->graph_full_exec = nullptr;
-                B->graph_full = nullptr;
-                HIP_OK(hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal));
-                setup(false);
-                for (int i = 0; i < N; ++i) cg_iteration(ctx, B);
-                HIP_OK(hipStreamEndCapture(ctx->stream, &B->graph_full));
-                HIP_OK(hipGraphInstantiate(&B->graph_full_exec, B->graph_full, nullptr, nullptr, 0));
-                B->graph_full_n = N;
-            }
-            HIP_OK(hipGraphLaunch(B->graph_full_exec, ctx->stream));
-            done = N;
-            launch(ctx, K_COUNT, k_count_active, 1, 0, S, B->nsys, B->d_count);
-            HIP_OK(hipMemcpyAsync(B->h_count, B->d_count, sizeof(int), hipMemcpyDeviceToHost, ctx->stream));
-            HIP_OK(hipStreamSynchronize(ctx->stream));
-            live = *B->h_count != 0;
-        } else {
-            setup(round > 0);
-            if (round > 0) {   // a restart round: iterate only if some true residual missed tol
+    return guarded(ctx, [&]() -> int {
+        if (!ctx || !b || !x || !P) return PHG_EINVAL;
+        BatchDev* B = get_batch(ctx, batch);
+        if (!B) return fail(ctx, PHG_ESTATE, "unknown batch %d", batch);
+        if (!(P->tol >= 0) || P->max_iters < 0) return fail(ctx, PHG_EINVAL, "bad tolerance or iteration cap");
+        if (P->stop_rule != PHG_STOP_REL_L2 && P->stop_rule != PHG_STOP_MAXNORM)
+            return fail(ctx, PHG_EINVAL, "unknown stop rule %d", P->stop_rule);
+        if (P->stop_rule == PHG_STOP_MAXNORM && !(P->inv > 0 && std::isfinite(P->inv)))
+            return fail(ctx, PHG_EINVAL, "PHG_STOP_MAXNORM needs inv = dx^2/dt > 0 (got %g)", (double)P->inv);
+        HIP_OK(hipSetDevice(ctx->device));
+        for (int s = 0; s < B->nsys; ++s) {
+            if (!b[s] || !x[s]) return fail(ctx, PHG_EINVAL, "null vector for system %d", s);
+            B->h_sys[s].b = b[s];
+            B->h_sys[s].x = x[s];
+        }
+        int rc = upload_sys(ctx, B);
+        if (!rc) rc = reset_state(ctx, B, P);
+        if (rc) return rc;
+        const DSys* S = B->d_sys;
+        // r_0 = b - A x_0, the stop test at iteration 0, z_0 = M^-1 r_0, r.z;
+        // a restart projects its residual onto the range of A first
+        auto setup = [&](bool restart) {
+            // with refinement on, every residual formed from x (a warm start's
+            // first one as well as a restart's) accumulates in FP64: an FP32 one
+            // is noise once it nears eps |A| |x|, where a warm start begins
+            const int hi = restart || P->refine > 0 ? 1 : 0;
+            if (B->binary) launch(ctx, K_RESID_INIT, k_resid_init<Mask01>, B->tab_len[0], 0, S, B->tab[0], hi);
+            else launch(ctx, K_RESID_INIT, k_resid_init<float>, B->tab_len[0], 0, S, B->tab[0], hi);
+            if (restart) demean_batch(ctx, B, 1);
+            launch(ctx, K_FIN, k_fin_init, B->nsys, 0, S, B->tab_sys);
+            vcycle(ctx, B, 1);
+            launch(ctx, K_FIN, k_fin_rz0, B->nsys, 0, S, B->tab_sys);
+        };
+        const int check = P->check_every > 0 ? P->check_every : 4;
+        const bool graph = P->use_graph && !ctx->prof;
+        const bool whole = P->use_graph == 2 && !ctx->prof;
+        // after the whole-solve graph, the systems still running are the slow
+        // tail: check a quarter as far apart (at least 4 iterations)
+        const int tail_check = whole ? std::min(check, std::max(4, check / 4)) : check;
+        // rounds: the first from x_0; each later one restarts the systems that
+        // converged from their x and the true residual b - A x (those already
+        // within tol stop at once); the cap applies per round
+        for (int round = 0;; ++round) {
+            int done = 0;
+            bool live = true;
+            if (whole && round == 0) {
+                // the set-up and the first `check` iterations as one graph: one launch
+                // per solve when every system stops within them (a system that has
+                // stopped leaves each later kernel at once)
+                const int N = std::min(check, P->max_iters);
+                // the set-up's residual precision (refine) is fixed at capture
+                const int hi0 = P->refine > 0 ? 1 : 0;
+                if (!B->graph_full_exec || B->graph_full_n != N || B->graph_full_hi != hi0) {
+                    if (B->graph_full_exec) (void)hipGraphExecDestroy(B->graph_full_exec);
+                    if (B->graph_full) (void)hipGraphDestroy(B->graph_full);
+                    B->graph_full_exec = nullptr;
+                    B->graph_full = nullptr;
+                    HIP_OK(hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal));
+                    setup(false);
+                    for (int i = 0; i < N; ++i) cg_iteration(ctx, B);
+                    HIP_OK(hipStreamEndCapture(ctx->stream, &B->graph_full));
+                    HIP_OK(hipGraphInstantiate(&B->graph_full_exec, B->graph_full, nullptr, nullptr, 0));
+                    B->graph_full_n = N;
+                    B->graph_full_hi = hi0;
+                }
+                HIP_OK(hipGraphLaunch(B->graph_full_exec, ctx->stream));
+                done = N;
                 launch(ctx, K_COUNT, k_count_active, 1, 0, S, B->nsys, B->d_count);
                 HIP_OK(hipMemcpyAsync(B->h_count, B->d_count, sizeof(int), hipMemcpyDeviceToHost, ctx->stream));
                 HIP_OK(hipStreamSynchronize(ctx->stream));
                 live = *B->h_count != 0;
+            } else {
+                setup(round > 0);
+                if (round > 0) {   // a restart round: iterate only if some true residual missed tol
+                    launch(ctx, K_COUNT, k_count_active, 1, 0, S, B->nsys, B->d_count);
+                    HIP_OK(hipMemcpyAsync(B->h_count, B->d_count, sizeof(int), hipMemcpyDeviceToHost, ctx->stream));
+                    HIP_OK(hipStreamSynchronize(ctx->stream));
+                    live = *B->h_count != 0;
+                }
             }
-        }
-        if (graph && live && !B->graph_exec) {
-            HIP_OK(hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal));
-            cg_iteration(ctx, B);
-            HIP_OK(hipStreamEndCapture(ctx->stream, &B->graph));
-            HIP_OK(hipGraphInstantiate(&B->graph_exec, B->graph, nullptr, nullptr, 0));
-        }
-        while (live && done < P->max_iters) {
-            const int k = std::min(round == 0 ? tail_check : check, P->max_iters - done);
-            for (int i = 0; i < k; ++i) {
-                if (graph) HIP_OK(hipGraphLaunch(B->graph_exec, ctx->stream));
-                else cg_iteration(ctx, B);
+            if (graph && live && !B->graph_exec) {
+                HIP_OK(hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal));
+                cg_iteration(ctx, B);
+                HIP_OK(hipStreamEndCapture(ctx->stream, &B->graph));
+                HIP_OK(hipGraphInstantiate(&B->graph_exec, B->graph, nullptr, nullptr, 0));
             }
-            done += k;
-            launch(ctx, K_COUNT, k_count_active, 1, 0, S, B->nsys, B->d_count);
-            HIP_OK(hipMemcpyAsync(B->h_count, B->d_count, sizeof(int), hipMemcpyDeviceToHost, ctx->stream));
+            while (live && done < P->max_iters) {
+                const int k = std::min(round == 0 ? tail_check : check, P->max_iters - done);
+                for (int i = 0; i < k; ++i) {
+                    if (graph) HIP_OK(hipGraphLaunch(B->graph_exec, ctx->stream));
+                    else cg_iteration(ctx, B);
+                }
+                done += k;
+                launch(ctx, K_COUNT, k_count_active, 1, 0, S, B->nsys, B->d_count);
+                HIP_OK(hipMemcpyAsync(B->h_count, B->d_count, sizeof(int), hipMemcpyDeviceToHost, ctx->stream));
+                HIP_OK(hipStreamSynchronize(ctx->stream));
+                if (*B->h_count == 0) break;
+            }
+            launch(ctx, K_XFLUSH, k_x_flush, B->tab_len[0], 0, S, B->tab[0]);
+            if (round >= P->refine) break;
+            launch(ctx, K_FIN, k_refine_reset, (B->nsys + TB - 1) / TB, 0, S, B->nsys);
+        }
+        if (P->demean) demean_batch(ctx, B);
+        HIP_OK(hipGetLastError());
+        if (stats) {
+            std::vector<double> scal((size_t)B->nsys * S_NSCAL);
+            std::vector<int> state((size_t)B->nsys * ST_NSTATE);
+            HIP_OK(hipMemcpyAsync(scal.data(), B->scal, scal.size() * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
+            HIP_OK(hipMemcpyAsync(state.data(), B->state, state.size() * sizeof(int), hipMemcpyDeviceToHost, ctx->stream));
             HIP_OK(hipStreamSynchronize(ctx->stream));
-            if (*B->h_count == 0) break;
+            for (int s = 0; s < B->nsys; ++s) {
+                const double* sc = &scal[(size_t)s * S_NSCAL];
+                const int* st = &state[(size_t)s * ST_NSTATE];
+                stats[s].iters = st[ST_ITOT] + st[ST_ITERS];
+                stats[s].status = st[ST_STATUS];
+                stats[s].relres = sc[S_BB] > 0 ? std::sqrt(sc[S_RR]) / std::sqrt(sc[S_BB]) : 0.0;
+                stats[s].maxres = (double)((float)sc[S_MAXR] / (float)sc[S_INV]);
+            }
         }
-        launch(ctx, K_XFLUSH, k_x_flush, B->tab_len[0], 0, S, B->tab[0]);
-        if (round >= P->refine) break;
-        launch(ctx, K_FIN, k_refine_reset, (B->nsys + TB - 1) / TB, 0, S, B->nsys);
-    }
-    if (P->demean) demean_batch(ctx, B);
-    HIP_OK(hipGetLastError());
-    if (stats) {
-        std::vector<double> scal((size_t)B->nsys * S_NSCAL);
-        std::vector<int> state((size_t)B->nsys * ST_NSTATE);
-        HIP_OK(hipMemcpyAsync(scal.data(), B->scal, scal.size() * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
-        HIP_OK(hipMemcpyAsync(state.data(), B->state, state.size() * sizeof(int), hipMemcpyDeviceToHost, ctx->stream));
         HIP_OK(hipStreamSynchronize(ctx->stream));
-        for (int s = 0; s < B->nsys; ++s) {
-            const double* sc = &scal[(size_t)s * S_NSCAL];
-            const int* st = &state[(size_t)s * ST_NSTATE];
-            stats[s].iters = st[ST_ITOT] + st[ST_ITERS];
-            stats[s].status = st[ST_STATUS];
-            stats[s].relres = sc[S_BB] > 0 ? std::sqrt(sc[S_RR]) / std::sqrt(sc[S_BB]) : 0.0;
-            stats[s].maxres = (double)((float)sc[S_MAXR] / (float)sc[S_INV]);
-        }
-    }
-    HIP_OK(hipStreamSynchronize(ctx->stream));
-    if (ctx->prof) prof_drain(ctx);
-    return PHG_OK;
+        if (ctx->prof) prof_drain(ctx);
+        return PHG_OK;
+    });
 }
 
 int phg_solve(phg_ctx* ctx, int grid, const float* b, float* x, const phg_solve_params* P, phg_solve_stats* stats) {

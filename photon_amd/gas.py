@@ -20,7 +20,8 @@ LIB_PATH = os.path.join(HERE, "libphoton_gas.so")
 
 PREC_FP32, PREC_MIXED_F16, PREC_MIXED_BF16 = 0, 1, 2
 STOP_REL_L2, STOP_MAXNORM = 0, 1
-CONVERGED, MAXITER, BREAKDOWN = 1, 2, 3
+CONVERGED, MAXITER, BREAKDOWN, BREAKDOWN_RZ, NONFINITE = 1, 2, 3, 4, 5
+COARSE_EXACT, COARSE_GS20 = 0, 1
 
 EXPORTS = (
     "phg_create", "phg_destroy", "phg_last_error", "phg_synchronize", "phg_stream", "phg_malloc", "phg_free",
@@ -31,6 +32,7 @@ EXPORTS = (
     "phg_profile_reset",
     "phg_hierarchy_shape", "phg_batch_create_ex", "phg_copy", "phg_leakage",
     "phg_grid_plan_build", "phg_grid_plan_free", "phg_grid_create_planned",
+    "phg_grid_set_coarse", "phg_grid_get_coarse",
 )
 
 
@@ -95,6 +97,8 @@ def load_library():
     L.phg_project.argtypes = [vp, i32, f32, vp, vp, vp, vp, vp, C.POINTER(SolveParams), vp, C.POINTER(f32)]
     L.phg_rhs.argtypes = [vp, i32, f32, vp, vp, vp, vp, vp, vp, vp]
     L.phg_zone_mean.argtypes = [vp, i32, vp]
+    L.phg_grid_set_coarse.argtypes = [vp, i32, i32]
+    L.phg_grid_get_coarse.argtypes = [vp, i32, C.POINTER(i32)]
     L.phg_leakage.argtypes = [vp, i32, vp, vp]
     L.phg_correct.argtypes = [vp, i32, f32, vp, vp, vp, vp, C.POINTER(f32)]
     L.phg_energy_expansion.argtypes = [vp, i32, f32, vp, vp, vp, vp, vp, C.POINTER(f32)]
@@ -307,6 +311,18 @@ class Grid:
                                                  C.addressof(nz), C.addressof(rl)))
         return dict(levels=sizes[:nl.value].tolist(), nzones=nz.value, ref_levels=rl.value)
 
+    @property
+    def coarse(self):
+        """How the V-cycle solves the coarsest level: COARSE_EXACT (dense
+        inverse) or COARSE_GS20 (the reference's 20 Gauss-Seidel sweeps)."""
+        m = C.c_int()
+        self.ctx._check(self.ctx.L.phg_grid_get_coarse(self.ctx.h, self.id, C.byref(m)))
+        return m.value
+
+    @coarse.setter
+    def coarse(self, mode):
+        self.ctx._check(self.ctx.L.phg_grid_set_coarse(self.ctx.h, self.id, int(mode)))
+
     def zones(self):
         nz = self.info()["nzones"]
         zone = np.zeros(self.n, np.int32)
@@ -403,11 +419,13 @@ class GasSolver:
     CELL_FIELDS = ("p", "S", "Pb", "Q", "massSrc", "nu") + FIELD_NAMES
 
     def __init__(self, ctx: Context, scen, precision=PREC_FP32, wind_closed=None, tol=1e-6, max_iters=400,
-                 stop_rule=STOP_REL_L2, plan: GridPlan | None = None, graph_steps=True):
+                 stop_rule=STOP_REL_L2, plan: GridPlan | None = None, graph_steps=True, coarse=None):
         self.ctx = ctx
         self.scen = scen
         self.graph_steps = graph_steps
         self.grid = Grid(ctx, scen, precision, wind_closed, plan=plan)
+        if coarse is not None:
+            self.grid.coarse = coarse
         n = self.grid.n
         self.d = {}
         for name in ("u", "v", "w"):

@@ -203,10 +203,12 @@ def _finish(blocked3, dx, open_sides=(), door=None) -> Grid:
     return g
 
 
-def room(dx: float = 0.2) -> Grid:
+def room(dx: float = 0.2, window: bool = False) -> Grid:
     """Single 4.0 x 3.0 x 2.4 m room: walls closed, a 0.8 x 2.0 m door in the
     x = 0 wall (j = 6..9 and k = 0..9 at 0.2 m) open to ambient, burner cells
-    (0.4 x 0.4 m) at the floor centre."""
+    (0.4 x 0.4 m) at the floor centre.  With `window`, also a 1.2 x 1.0 m
+    window open to ambient in the y = 3.0 m wall (x 1.4..2.6 m, z 1.0..2.0 m;
+    i = 7..12, k = 5..9 at 0.2 m)."""
     nx, ny, nz = round(4.0 / dx), round(3.0 / dx), round(2.4 / dx)
     blocked = np.zeros((nz, ny, nx), np.uint8)
     yc = (np.arange(ny) + 0.5) * dx
@@ -214,6 +216,11 @@ def room(dx: float = 0.2) -> Grid:
     door = (np.abs(zc[:, None] - 1.0) < 1.0) & (yc[None, :] > 1.2) & (yc[None, :] < 2.0)
     g = _finish(blocked, dx, door=door)
     xc = (np.arange(nx) + 0.5) * dx
+    if window:
+        win = (np.abs(zc[:, None] - 1.5) < 0.5) & (np.abs(xc[None, :] - 2.0) < 0.6)
+        fy = g.fy.reshape(nz, ny + 1, nx)
+        fy[:, ny, :] = np.where(win, 1.0, fy[:, ny, :])
+        g.ay = g.fy.copy()
     bi = np.nonzero(np.abs(xc - 2.0) < 0.2)[0]
     bj = np.nonzero(np.abs(yc - 1.4) < 0.2)[0]
     g.burner_cells = np.array([i + nx * j for j in bj for i in bi], np.int32)

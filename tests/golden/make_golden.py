@@ -26,7 +26,12 @@ DT = 0.01
 CASES = {
     "room": lambda: sc.room(),
     "walls": lambda: sc.house(7, 24, 16, 12),
+    # the room with a window in the y-high wall, wind from the north: the
+    # window side takes inflow, so windAt closes it for the pressure matrix
+    "wind": lambda: sc.room(window=True),
 }
+# Solver::wind (speed m/s at 10 m, degrees it blows from, profile exponent)
+WIND = {"wind": (3.0, 0.0, 0.25)}
 FIELDS = ("T", "Yo", "Yf", "Ys", "Ya", "Yp", "Yc", "Yw", "Yco", "Yai")
 
 
@@ -56,6 +61,13 @@ def scalar_inputs(g, seed):
 def make(name, g, K=6):
     out = {}
     R = po.RefSolver(g, threads=1, pin=True)
+    if name in WIND:
+        R.set_wind(*WIND[name])
+        out["wind"] = np.array(WIND[name], np.float32)
+        # sides (x-low, x-high, y-low, y-high) where windAt prescribes inflow;
+        # the power law never changes the sign, so one height decides a side
+        out["wind_closed"] = np.array([0 if np.isnan(R.wind_at(a, sd, 1.0)) else 1 for a in (0, 1) for sd in (0, 1)],
+                                      np.uint8)
     zone, zo = R.zones()
     out["zone"], out["zone_open"] = zone, zo
     lv = R.levels()
@@ -99,5 +111,6 @@ def make(name, g, K=6):
 if __name__ == "__main__":
     if not po.ref_available(pin=True):
         sys.exit("oracle/_ref/libphoton_ref_pin.so not built (needs the reference checkout): make -C oracle")
-    for name, fn in CASES.items():
-        make(name, fn())
+    names = sys.argv[1:] or list(CASES)
+    for name in names:
+        make(name, CASES[name]())

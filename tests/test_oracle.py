@@ -20,7 +20,7 @@ from photon_amd import scenarios as sc
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 DT = 0.01
-CASES = {"room": sc.room, "walls": lambda: sc.house(7, 24, 16, 12)}
+CASES = {"room": sc.room, "walls": lambda: sc.house(7, 24, 16, 12), "wind": lambda: sc.room(window=True)}
 FIELDS = ("T", "Yo", "Yf", "Ys", "Ya", "Yp", "Yc", "Yw", "Yco", "Yai")
 
 
@@ -28,10 +28,16 @@ def golden(name):
     return np.load(os.path.join(HERE, "golden", f"ref_{name}.npz"))
 
 
+def wind_closed(G):
+    """The sides the fixture's wind closes (x-low, x-high, y-low, y-high), or None."""
+    return G["wind_closed"] if "wind_closed" in G.files else None
+
+
 @pytest.fixture(scope="module", params=list(CASES))
 def case(request):
     g = CASES[request.param]()
-    return request.param, g, golden(request.param), po.Hierarchy(g)
+    G = golden(request.param)
+    return request.param, g, G, po.Hierarchy(g, wind_closed=wind_closed(G))
 
 
 # ---------------------------------------------------------------- 1. golden --
@@ -62,7 +68,7 @@ def test_project_matches_golden(case):
     u, v, w = G["pj_u0"].copy(), G["pj_v0"].copy(), G["pj_w0"].copy()
     p = np.zeros(g.n, np.float32)
     o = po.project(H, g, DT, u, v, w, p, G["pj_S"], tol=0.0, max_iters=int(G["pj_K"]), stop=po.STOP_MAXNORM,
-                   coarse=po.COARSE_GS20)
+                   coarse=po.COARSE_GS20, wind_closed=wind_closed(G))
     assert o["iters"] == int(G["pj_iters"])
     assert np.float32(o["residual"]) == G["pj_residual"]
     assert np.float32(o["max_vel"]) == G["pj_maxvel"]
@@ -220,6 +226,55 @@ def test_vcycle_is_symmetric_positive(coarse):
     Ma, Mb = H.precondition(a, coarse, fp64=True), H.precondition(b, coarse, fp64=True)
     assert abs(b @ Ma - a @ Mb) <= 1e-10 * abs(b @ Ma)
     assert a @ Ma > 0 and b @ Mb > 0
+
+
+def _coarsest_dense(H):
+    """The coarsest operator as a dense FP64 matrix from its CSR (diag minus
+    the positive couplings) and the rows that take part (diag > 0)."""
+    Lc = H.level(H.nlevels - 1)
+    n = Lc["n"]
+    A = np.zeros((n, n))
+    for I in range(n):
+        A[I, I] = Lc["diag"][I]
+        for q in range(Lc["rowStart"][I], Lc["rowStart"][I + 1]):
+            A[I, Lc["col"][q]] -= Lc["val"][q]
+    return A, Lc["diag"] > 0
+
+
+@pytest.mark.parametrize("name", ["walls", "room"])
+def test_exact_coarse_solve_is_the_pseudo_inverse(name):
+    # walls: a closed domain whose coarsest level (level 1, pinned to the
+    # reference's by the golden CSR) holds 7 closed components, so the
+    # operator is singular; room: open through its door, so it is regular
+    g = CASES[name]()
+    H = po.Hierarchy(g)
+    A, act = _coarsest_dense(H)
+    b = np.random.default_rng(3).standard_normal(len(act))
+    x = H.coarse_solve(b)
+    Aa = A[np.ix_(act, act)]
+    x_ref = np.zeros_like(b)
+    x_ref[act] = np.linalg.pinv(Aa) @ b[act]      # minimum-norm least-squares solution
+    assert np.abs(x - x_ref).max() <= 1e-10 * np.abs(x_ref).max()
+    assert np.all(x[~act] == 0)
+    # A x = P b: b with its component in the null space removed
+    Pb = Aa @ (np.linalg.pinv(Aa) @ b[act])
+    assert np.abs(Aa @ x[act] - Pb).max() <= 1e-9 * np.abs(Pb).max()
+    rank = np.linalg.matrix_rank(Aa)
+    assert rank == (act.sum() - 7 if name == "walls" else act.sum())
+
+
+def test_wind_sides_change_the_projection():
+    # the wind fixture differs from the same room solved without wind, so
+    # its bitwise match above does exercise the wind masking
+    g = CASES["wind"]()
+    G = golden("wind")
+    assert G["wind_closed"].tolist() == [0, 0, 0, 1]
+    H = po.Hierarchy(g)
+    u, v, w = G["pj_u0"].copy(), G["pj_v0"].copy(), G["pj_w0"].copy()
+    p = np.zeros(g.n, np.float32)
+    po.project(H, g, DT, u, v, w, p, G["pj_S"], tol=0.0, max_iters=int(G["pj_K"]), stop=po.STOP_MAXNORM,
+               coarse=po.COARSE_GS20)
+    assert not np.array_equal(p, G["pj_p"]) and not np.array_equal(v, G["pj_v"])
 
 
 def test_exact_coarse_reduces_iterations():
