@@ -17,8 +17,9 @@
  *     return statistics synchronise that stream before returning.
  *   - Every entry point returns 0 on success or a negative PHG_E* code;
  *     phg_last_error() then holds a message (per context, not thread-safe
- *     across threads sharing one context).  The library keeps no global
- *     mutable state: contexts are independent.
+ *     across threads sharing one context).  Contexts are independent: the
+ *     library's only global mutable state is the per-device workspace of
+ *     phg_zone_mean_noalloc.
  */
 #ifndef PHOTON_GAS_H
 #define PHOTON_GAS_H
@@ -213,6 +214,17 @@ int phg_project(phg_ctx *ctx, int grid, float dt, float *u, float *v, float *w, 
 int phg_rhs(phg_ctx *ctx, int grid, float dt, const float *u, const float *v, const float *w, const float *S,
             float *b, float *diag, float *p);
 int phg_zone_mean(phg_ctx *ctx, int grid, float *p);
+/* phg_zone_mean without any allocation.  No counterpart in the reference: it
+ * is for callers that cannot allocate device memory (code under HIP graph
+ * capture, tight serving loops).  Same result bit for bit as phg_zone_mean;
+ * p is a device pointer, in/out.  The work is queued on the context's stream
+ * and the call returns without waiting.  The partial sums and zone means go
+ * to a fixed workspace of 2048 doubles (16 KiB) that the library holds once
+ * per device: a grid needs one entry per closed-zone task (8192 cells) plus
+ * one per closed zone, and a larger need returns PHG_EINVAL.  Every call on a
+ * device shares that workspace, so calls that overlap on one device (two
+ * contexts' streams, or two threads) race. */
+int phg_zone_mean_noalloc(phg_ctx *ctx, int grid, float *p);
 /* The compatible divergence source of Solver::leakage (gasphase.cpp:487-498)
  * for grids without crack faces: on each closed zone S -= (integral of S) /
  * zone volume, i.e. its mean, so the pressure equation of a sealed room is

@@ -2091,8 +2091,8 @@ __global__ void __launch_bounds__(TB) k_proj(const DSys* __restrict__ sys, const
 // `which`: 0 the solution x (after the solve), 1 the residual r_0 of a system
 // restarted from its true residual, whose rounding can leave a component along
 // the null space (a constant per closed zone) that CG cannot reduce
-// the steps of the closed-zone mean removal (partial sums of one task, the
-// means of every zone) as device functions
+// the three steps shared by the batched kernels and the workspace kernels
+// of phg_zone_mean_noalloc (same sums in the same order)
 __device__ __forceinline__ void zone_task_sum(const int4 tk, const int* __restrict__ zcells,
                                               const float* __restrict__ x, double* __restrict__ slot) {
     double v[1] = {0.0};
@@ -2146,6 +2146,28 @@ __global__ void __launch_bounds__(TB) k_zone_sub(const DSys* __restrict__ sys, c
         const int z = s.zclosed[c];
         if (z >= 0) x[c] -= (float)s.zmean[z];
     }
+}
+
+// Workspace forms for one grid: the partials and the means live in a buffer
+// the caller names (ws: ntask partials, then nclosed means), so no batch and
+// no allocation is involved.  One block per task; one block for the means;
+// one thread per cell for the subtraction.
+__global__ void __launch_bounds__(TB) k_zone_part_ws(const int4* __restrict__ ztask, const int* __restrict__ zcells,
+                                                     const float* __restrict__ x, double* __restrict__ ws) {
+    zone_task_sum(ztask[blockIdx.x], zcells, x, &ws[blockIdx.x]);
+}
+
+__global__ void __launch_bounds__(TB) k_zone_fin_ws(int nclosed, int ntask, const int* __restrict__ ztptr,
+                                                    const int4* __restrict__ ztask, double* __restrict__ ws) {
+    zone_means(nclosed, ztptr, ztask, ws, ws + ntask);
+}
+
+__global__ void __launch_bounds__(TB) k_zone_sub_ws(int n, const int* __restrict__ zclosed,
+                                                    const double* __restrict__ zmean, float* __restrict__ x) {
+    const int c = blockIdx.x * TB + threadIdx.x;
+    if (c >= n) return;
+    const int z = zclosed[c];
+    if (z >= 0) x[c] -= (float)zmean[z];
 }
 
 }  // namespace phg

@@ -21,6 +21,15 @@
 #include "kernels_pcg.hpp"
 #include "kernels_step.hpp"
 
+// Device workspace of phg_zone_mean_noalloc, the entry point that allocates
+// nothing: the closed-zone partial sums (one per task) and then the means
+// (one per closed zone) of the call in progress.  Each device holds its own
+// copy until the process ends.  Host code never passes this name to a kernel
+// (on the host it is only a stand-in); phg_create looks up the copy's device
+// address with hipGetSymbolAddress and every use goes through that address.
+constexpr int kNoAllocDoubles = 2048;   // 16 KiB
+static __device__ __align__(256) double scratch[kNoAllocDoubles];
+
 namespace phg {
 
 // ---------------------------------------------------------------------------
@@ -168,6 +177,7 @@ struct phg_ctx {
     std::vector<hipEvent_t> pool;
     double prof_ms[K_NCLASS] = {};
     long long prof_n[K_NCLASS] = {};
+    double* scratch_dev = nullptr;   // this device's copy of `scratch`
 };
 
 namespace {
@@ -609,6 +619,14 @@ int phg_create(phg_ctx** out, int device) {
         delete ctx;
         return PHG_EHIP;
     }
+    // the device address of this device's workspace copy (device current)
+    void* sp = nullptr;
+    if (hipGetSymbolAddress(&sp, HIP_SYMBOL(scratch)) != hipSuccess || sp == nullptr) {
+        (void)hipStreamDestroy(ctx->stream);
+        delete ctx;
+        return PHG_EHIP;
+    }
+    ctx->scratch_dev = static_cast<double*>(sp);
     *out = ctx;
     return PHG_OK;
 }
@@ -1543,6 +1561,30 @@ int phg_zone_mean(phg_ctx* ctx, int grid, float* p) {
     if (rc) return rc;
     demean_batch(ctx, B);
     HIP_OK(hipStreamSynchronize(ctx->stream));
+    HIP_OK(hipGetLastError());
+    return PHG_OK;
+}
+
+int phg_zone_mean_noalloc(phg_ctx* ctx, int grid, float* p) {
+    if (!ctx || !p) return PHG_EINVAL;
+    GridDev* G = get_grid(ctx, grid);
+    if (!G) return fail(ctx, PHG_ESTATE, "unknown grid %d", grid);
+    const long need = (long)G->nztask + G->nclosed;
+    if (need > kNoAllocDoubles)
+        return fail(ctx, PHG_EINVAL,
+                    "phg_zone_mean_noalloc: grid %d needs %ld workspace entries (%d tasks + %d closed zones), "
+                    "limit %d",
+                    grid, need, G->nztask, G->nclosed, kNoAllocDoubles);
+    if (G->nclosed == 0) return PHG_OK;
+    // partials first, then the means; every entry read is written earlier in
+    // the same call, so nothing needs clearing
+    double* ws = ctx->scratch_dev;
+    hipLaunchKernelGGL(k_zone_part_ws, dim3(G->nztask), dim3(TB), 0, ctx->stream, (const int4*)G->ztask,
+                       (const int*)G->zcells, (const float*)p, ws);
+    hipLaunchKernelGGL(k_zone_fin_ws, dim3(1), dim3(TB), 0, ctx->stream, G->nclosed, G->nztask,
+                       (const int*)G->ztptr, (const int4*)G->ztask, ws);
+    hipLaunchKernelGGL(k_zone_sub_ws, dim3(tiles(G->n)), dim3(TB), 0, ctx->stream, G->n, (const int*)G->zclosed,
+                       (const double*)(ws + G->nztask), p);
     HIP_OK(hipGetLastError());
     return PHG_OK;
 }

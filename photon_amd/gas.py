@@ -31,7 +31,7 @@ EXPORTS = (
     "phg_energy_expansion", "phg_advect_scalars", "phg_profile_enable", "phg_profile_select", "phg_profile_get",
     "phg_profile_reset",
     "phg_hierarchy_shape", "phg_batch_create_ex", "phg_copy", "phg_leakage",
-    "phg_grid_plan_build", "phg_grid_plan_free", "phg_grid_create_planned",
+    "phg_grid_plan_build", "phg_grid_plan_free", "phg_grid_create_planned", "phg_zone_mean_noalloc",
     "phg_grid_set_coarse", "phg_grid_get_coarse",
 )
 
@@ -97,6 +97,7 @@ def load_library():
     L.phg_project.argtypes = [vp, i32, f32, vp, vp, vp, vp, vp, C.POINTER(SolveParams), vp, C.POINTER(f32)]
     L.phg_rhs.argtypes = [vp, i32, f32, vp, vp, vp, vp, vp, vp, vp]
     L.phg_zone_mean.argtypes = [vp, i32, vp]
+    L.phg_zone_mean_noalloc.argtypes = [vp, i32, vp]
     L.phg_grid_set_coarse.argtypes = [vp, i32, i32]
     L.phg_grid_get_coarse.argtypes = [vp, i32, C.POINTER(i32)]
     L.phg_leakage.argtypes = [vp, i32, vp, vp]
