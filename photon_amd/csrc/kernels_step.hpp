@@ -1,9 +1,8 @@
 // kernels_step.hpp -- the kernels around the pressure solve: RHS assembly,
-// closed-zone mean removal, velocity correction (Solver::project,
-// src/sim/gasphase.cpp:505-677), the divergence source
-// (energyAndExpansion, :251-271) and scalar transport (advectScalars,
-// :685-784).  One thread per cell or face; all of them are HBM-bound
-// elementwise / 7-point (13-point for transport) passes.
+// velocity correction (Solver::project, src/sim/gasphase.cpp:505-677) and
+// the divergence source (energyAndExpansion, :251-271).  One thread per cell
+// or face; all of them are HBM-bound elementwise / 7-point passes.  Scalar
+// transport (advectScalars) is in kernels_transport.hpp.
 #pragma once
 #include "device.hpp"
 
@@ -145,118 +144,6 @@ __global__ void __launch_bounds__(TB) k_energy(int n, float dx, float dt, const 
     }
     md = block_max(md);
     if (threadIdx.x == 0) atomic_max_nonneg(maxdiv, md);
-}
-
-struct Fields10 {
-    const float* in[10];
-    float* out[10];
-};
-
-__device__ __forceinline__ float superbee(float r) {   // gasphase.cpp:35
-    return fmaxf(fmaxf(0.0f, fminf(2.0f * r, 1.0f)), fminf(r, 2.0f));
-}
-
-// Superbee-limited, mass-weighted transport of the 10 scalars with species
-// diffusion and a neighbour-bounded clamp (advectScalars, gasphase.cpp:685-784)
-__global__ void __launch_bounds__(TB) k_advect(DGrid g, float dx, float dt, const float* __restrict__ u,
-                                               const float* __restrict__ v, const float* __restrict__ w,
-                                               const float* __restrict__ Pb, const float* __restrict__ nu,
-                                               Fields10 F) {
-    const float T0 = 293.15f, P0 = 101325.0f, RS = 287.0f, T_MIN = 200.0f, SC_T = 0.5f;
-    const float amb[10] = {T0, 0.232f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0006f, 0.0073f, 0.0f, 0.0f};
-    const int c = blockIdx.x * TB + threadIdx.x;
-    if (c >= g.n) return;
-    const int nx = g.nx, ny = g.ny, nz = g.nz, sxy = nx * ny;
-    if (g.blocked[c]) {
-#pragma unroll
-        for (int q = 0; q < 10; ++q) F.out[q][c] = F.in[q][c];
-        return;
-    }
-    int i, j, k;
-    cell_ijk(c, nx, ny, i, j, k);
-    const float s = dt / dx;
-    const int fa0 = c + j + ny * k, fa1 = fa0 + 1, fb0 = c + nx * k, fb1 = fb0 + nx, fc0 = c, fc1 = c + sxy;
-    const float fo[6] = {g.fx[fa0], g.fx[fa1], g.fy[fb0], g.fy[fb1], g.fz[fc0], g.fz[fc1]};
-    const float un[6] = {-fo[0] * u[fa0], fo[1] * u[fa1], -fo[2] * v[fb0], fo[3] * v[fb1], -fo[4] * w[fc0],
-                         fo[5] * w[fc1]};
-    const int id[3] = {i, j, k}, dim[3] = {nx, ny, nz}, st3[3] = {1, nx, sxy};
-    int nb[6], nb2[6], opp[6];
-#pragma unroll
-    for (int d = 0; d < 6; ++d) {
-        const int a = d >> 1, sg = (d & 1) ? 1 : -1;
-        const int n1 = id[a] + sg, n2 = id[a] + 2 * sg, o1 = id[a] - sg;
-        nb[d] = (n1 >= 0 && n1 < dim[a]) ? c + sg * st3[a] : -1;
-        nb2[d] = (n2 >= 0 && n2 < dim[a]) ? c + 2 * sg * st3[a] : -1;
-        opp[d] = (o1 >= 0 && o1 < dim[a]) ? c - sg * st3[a] : -1;
-        if (nb[d] >= 0 && g.blocked[nb[d]]) nb[d] = -2;
-        if (nb2[d] >= 0 && g.blocked[nb2[d]]) nb2[d] = -2;
-        if (opp[d] >= 0 && g.blocked[opp[d]]) opp[d] = -2;
-    }
-    const float* T = F.in[0];
-    const float nuc = nu[c];
-    const float tc = T[c];
-    const float rc = Pb[c] / (RS * tc);
-    float rUp[6], mden = rc;
-#pragma unroll
-    for (int d = 0; d < 6; ++d) {
-        rUp[d] = 0.0f;
-        if (fo[d] <= 0 || nb[d] == -2) continue;
-        const float tn = nb[d] >= 0 ? T[nb[d]] : T0;
-        float tf;
-        if (un[d] > 0) {
-            const float tu2 = opp[d] >= 0 ? T[opp[d]] : (opp[d] == -1 ? T0 : tc);
-            const float den = tn - tc;
-            tf = tc + (fabsf(den) > 1e-6f ? 0.5f * superbee((tc - tu2) / den) * den : 0.0f);
-        } else {
-            const float tu2 = nb2[d] >= 0 ? T[nb2[d]] : (nb2[d] == -1 ? T0 : tn);
-            const float den = tc - tn;
-            tf = tn + (fabsf(den) > 1e-6f ? 0.5f * superbee((tn - tu2) / den) * den : 0.0f);
-        }
-        const float pu = un[d] > 0 ? Pb[c] : (nb[d] >= 0 ? Pb[nb[d]] : P0);
-        rUp[d] = pu / (RS * fmaxf(tf, T_MIN));
-        mden -= s * rUp[d] * un[d];
-    }
-    const bool conservative = mden > 0.25f * rc;
-    float Dd[6], rf[6];
-#pragma unroll
-    for (int d = 0; d < 6; ++d) {
-        const bool has = nb[d] >= 0;
-        const float nun = has ? nu[nb[d]] : nuc;
-        Dd[d] = 0.5f * (nuc + nun) / SC_T + 2.0e-5f;
-        rf[d] = has ? 0.5f * (rc + Pb[nb[d]] / (RS * T[nb[d]])) : rc;
-    }
-    for (int q = 0; q < 10; ++q) {
-        const float* Fq = F.in[q];
-        const float fc = Fq[c];
-        float lo = fc, hi = fc, acc = 0.0f, macc = 0.0f;
-#pragma unroll
-        for (int d = 0; d < 6; ++d) {
-            if (fo[d] <= 0 || nb[d] == -2) continue;
-            const float fn = nb[d] >= 0 ? Fq[nb[d]] : amb[q];
-            lo = fminf(lo, fn);
-            hi = fmaxf(hi, fn);
-            const float flux = un[d];
-            float face = fc;
-            if (flux > 0) {
-                const float fu2 = opp[d] >= 0 ? Fq[opp[d]] : (opp[d] == -1 ? amb[q] : fc);
-                const float den = fn - fc;
-                if (fabsf(den) > 1e-12f) face += 0.5f * superbee((fc - fu2) / den) * den;
-            } else if (flux < 0) {
-                const float fu2 = nb2[d] >= 0 ? Fq[nb2[d]] : (nb2[d] == -1 ? amb[q] : fn);
-                const float den = fc - fn;
-                face = fn;
-                if (fabsf(den) > 1e-12f) face += 0.5f * superbee((fn - fu2) / den) * den;
-            }
-            acc -= flux * (face - fc);
-            macc -= rUp[d] * flux * face;
-            if (q > 0) {
-                acc += fo[d] * Dd[d] * (fn - fc) / dx;
-                macc += fo[d] * rf[d] * Dd[d] * (fn - fc) / dx;
-            }
-        }
-        const float val = conservative ? (rc * fc + s * macc) / mden : fc + s * acc;
-        F.out[q][c] = fminf(fmaxf(val, lo), hi);
-    }
 }
 
 }  // namespace phg

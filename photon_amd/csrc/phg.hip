@@ -20,6 +20,7 @@
 #include "hierarchy.hpp"
 #include "kernels_pcg.hpp"
 #include "kernels_step.hpp"
+#include "kernels_transport.hpp"
 
 // Device workspace of phg_zone_mean_noalloc, the entry point that allocates
 // nothing: the closed-zone partial sums (one per task) and then the means
@@ -1640,16 +1641,37 @@ int phg_advect_scalars(phg_ctx* ctx, int grid, float dt, const float* u, const f
     if (!ctx || !u || !v || !w || !Pb || !nu || !fields_in || !fields_out) return PHG_EINVAL;
     GridDev* G = get_grid(ctx, grid);
     if (!G) return fail(ctx, PHG_ESTATE, "unknown grid %d", grid);
-    Fields10 F;
+    TransportArgs A;
     for (int q = 0; q < 10; ++q) {
         if (!fields_in[q] || !fields_out[q]) return fail(ctx, PHG_EINVAL, "null field %d", q);
         for (int r = 0; r < 10; ++r)
             if ((const void*)fields_out[q] == (const void*)fields_in[r])
                 return fail(ctx, PHG_EINVAL, "output field %d aliases input field %d", q, r);
-        F.in[q] = fields_in[q];
-        F.out[q] = fields_out[q];
+        A.in[q] = fields_in[q];
+        A.out[q] = fields_out[q];
     }
-    launch(ctx, K_ADVECT, k_advect, tiles(G->n), 0, G->dg(), G->dx, dt, u, v, w, Pb, nu, F);
+    A.nx = G->nx;
+    A.ny = G->ny;
+    A.nz = G->nz;
+    A.dx = G->dx;
+    A.dt = dt;
+    A.u = u;
+    A.v = v;
+    A.w = w;
+    A.fx = G->fx;
+    A.fy = G->fy;
+    A.fz = G->fz;
+    A.blocked = G->blocked;
+    A.Pb = Pb;
+    A.nu = nu;
+    // column tiles x k-chunks of at least 4 planes, about 1024 blocks
+    A.tiles_x = (G->nx + TR_TX - 1) / TR_TX;
+    A.tiles_y = (G->ny + TR_TY - 1) / TR_TY;
+    const int ntile = A.tiles_x * A.tiles_y;
+    const int want = std::max(1, (1024 + ntile - 1) / ntile);
+    A.kchunk = std::max(4, (G->nz + want - 1) / want);
+    const int nchunk = (G->nz + A.kchunk - 1) / A.kchunk;
+    launch_nt(ctx, K_ADVECT, k_transport, ntile * nchunk, TR_THREADS, 0, A);
     HIP_OK(hipGetLastError());
     return PHG_OK;
 }
