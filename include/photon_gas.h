@@ -246,6 +246,70 @@ int phg_energy_expansion(phg_ctx *ctx, int grid, float dt, float *T, float *S, c
 int phg_advect_scalars(phg_ctx *ctx, int grid, float dt, const float *u, const float *v, const float *w,
                        const float *Pb, const float *nu, const float *const *fields_in, float *const *fields_out);
 
+/* ---- the rest of the gas step (Solver::step, solver.cpp:226-269) ----------
+ * The phases around the projection, each the device form of the reference
+ * member it names; fields are device pointers on the grid's cells or faces.
+ * Solver::step's order is: phg_step_clear, burners (phg_add_gas),
+ * phg_combustion, phg_turbulence, phg_heat_diffusion, phg_energy_expansion,
+ * phg_momentum, leakage (phg_zone_integrals + phg_zone_values), phg_project,
+ * phg_advect_scalars, then the closed zones' background pressure
+ * (phg_zone_values into Pb); photon_amd/host/gas_phase.hpp runs them. */
+
+/* Wind at the open boundaries (Solver::wind, solver.h:106-111) and the height
+ * of the grid's floor (Solver::oz); speed <= 0 means no wind.  The sides it
+ * enters through must match the wind_closed flags of the grid. */
+typedef struct {
+    float speed;      /* m/s at 10 m height */
+    float from_deg;   /* direction it blows from, degrees clockwise from north (+y) */
+    float exponent;   /* power-law profile exponent */
+    float oz;         /* m */
+} phg_wind;
+
+/* The gas state of one grid's cells that the source phases update (device
+ * pointers; Solver's members of the same names, solver.h:229-247). */
+typedef struct {
+    float *T, *Yo, *Yf, *Ys, *Ya, *Yp, *Yc, *Yw, *Yco, *Yai;
+    float *Pb;        /* background pressure of the cell's zone, Pa */
+    float *massSrc;   /* kg of gas added this step */
+    float *Q, *Qr;    /* gas heat and flame radiant energy this step, J */
+    float *hrr;       /* heat release rate, W/m3 */
+    uint8_t *pilot, *lit;
+} phg_gas;
+
+/* stableDt (gasphase.cpp:63-70): the step size from the last step's maxVel,
+ * maxNu and maxDiv and the profile's dtMax and cfl (host only). */
+int phg_stable_dt(float dt_max, float profile_dt_max, float cfl, float dx, float max_vel, float max_nu, float max_div,
+                  float *dt_out);
+/* Solver::turbulence (gasphase.cpp:163-218): Deardorff eddy viscosity nu from
+ * the face velocities and T; max_nu (host, may be NULL): Solver::maxNu. */
+int phg_turbulence(phg_ctx *ctx, int grid, const float *u, const float *v, const float *w, const float *T, float *nu,
+                   float *max_nu);
+/* Solver::heatDiffusion (gasphase.cpp:221-248): conduction into Q. */
+int phg_heat_diffusion(phg_ctx *ctx, int grid, float dt, const float *T, const float *nu, const float *Pb, float *Q);
+/* Solver::momentum (gasphase.cpp:275-399): new face velocities into
+ * u_out/v_out/w_out (must not alias u/v/w; the reference swaps u2_ in).
+ * wind may be NULL.  Leak faces keep their velocity. */
+int phg_momentum(phg_ctx *ctx, int grid, float dt, const phg_wind *wind, const float *u, const float *v, const float *w,
+                 const float *T, const float *Pb, const float *nu, const float *p, float *u_out, float *v_out,
+                 float *w_out);
+/* The start of Solver::step (solver.cpp:231-234): Q, Qr, pilot, massSrc cleared. */
+int phg_step_clear(phg_ctx *ctx, int grid, const phg_gas *gas);
+/* Solver::addGasSpecies (solver.cpp:213-224) on each of ncells device-listed
+ * cells (distinct; solid ones skipped), pilot set when set_pilot: the
+ * burners' injection (injectBurners, gasphase.cpp:92-105). */
+int phg_add_gas(phg_ctx *ctx, int grid, int ncells, const int *cells, float fuel_gen, float aerosol, float soot_pot,
+                float ait, int set_pilot, const phg_gas *gas);
+/* Solver::combustion (solver.cpp:284-344); hrr_total (host, may be NULL):
+ * the summed heat release rate, W (FP64 over fixed-order block sums). */
+int phg_combustion(phg_ctx *ctx, int grid, float dt, int radiation_on, const phg_gas *gas, double *hrr_total);
+/* Closed zones' integral of S dV (leakage, gasphase.cpp:405-409), FP64, into
+ * dint (host, one per zone; 0 for open zones). */
+int phg_zone_integrals(phg_ctx *ctx, int grid, const float *S, double *dint);
+/* A value per zone (host, one per zone) applied to the cells of the closed
+ * zones: x -= value (subtract = 1, leakage's shift of S, gasphase.cpp:494-498)
+ * or x = value (subtract = 0, Pb from zoneP, solver.cpp:262-263). */
+int phg_zone_values(phg_ctx *ctx, int grid, const float *values, float *x, int subtract);
+
 /* ---- measurement aid: a plain device copy (16-byte vector loads/stores),
  * the calibration kernel for HBM byte counters; bytes must be a multiple of 16 */
 int phg_copy(phg_ctx *ctx, void *dst, const void *src, size_t bytes);

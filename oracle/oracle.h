@@ -134,6 +134,38 @@ void orc_advect_scalars(int nx, int ny, int nz, float dx, float dt, const float 
                         const float *fx, const float *fy, const float *fz, const uint8_t *blocked, const float *Pb,
                         const float *nu, const float *const *fields, float *const *out);
 
+/* ---- the rest of the gas step (oracle_step.c) ------------------------------ */
+typedef struct {
+    int nx, ny, nz;
+    float dx;
+    const float *fx, *fy, *fz;     /* face open fractions */
+    const uint8_t *blocked;
+    const uint8_t *lx, *ly, *lz;   /* crack (leak) faces, or NULL */
+} orc_grid;
+
+typedef struct {
+    float speed, from_deg, exponent;   /* Solver::wind (solver.h:106-111) */
+    float oz;                          /* height of the grid's floor, m */
+} orc_wind;
+
+typedef struct {
+    float *T, *Pb;
+    float *Yo, *Yf, *Ys, *Ya, *Yp, *Yc, *Yw, *Yco, *Yai;
+    float *massSrc;
+} orc_species;
+
+float orc_wind_at(const orc_wind *wd, int axis, int side, float z);
+float orc_stable_dt(float dt_max, float profile_dt_max, float cfl, float dx, float max_vel, float max_nu,
+                    float max_div);
+float orc_turbulence(const orc_grid *g, const float *u, const float *v, const float *w, const float *T, float *nu);
+void orc_heat_diffusion(const orc_grid *g, float dt, const float *T, const float *nu, const float *Pb, float *Q);
+void orc_momentum(const orc_grid *g, const orc_wind *wd, float dt, const float *u, const float *v, const float *w,
+                  const float *T, const float *Pb, const float *nu, const float *p, float *u2, float *v2, float *w2);
+void orc_add_gas(int ncell, const int *cells, float dx, float fuel_gen, float aerosol, float soot_pot, float ait,
+                 int set_pilot, const uint8_t *blocked, const orc_species *sp, uint8_t *pilot);
+double orc_combustion(const orc_grid *g, float dt, int radiation_on, const orc_species *sp, float *Q, float *Qr,
+                      float *hrr, const uint8_t *pilot, uint8_t *lit);
+
 #ifdef __cplusplus
 }
 #endif

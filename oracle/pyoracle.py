@@ -77,8 +77,38 @@ def lib():
         L.orc_energy_expansion.argtypes = [C.c_int, C.c_float, C.c_float, _u8p, _f32p, _f32p, _f32p, _f32p, _f32p]
         L.orc_advect_scalars.argtypes = [C.c_int, C.c_int, C.c_int, C.c_float, C.c_float, _f32p, _f32p, _f32p,
                                          _f32p, _f32p, _f32p, _u8p, _f32p, _f32p, C.c_void_p, C.c_void_p]
+        L.orc_wind_at.restype = C.c_float
+        L.orc_wind_at.argtypes = [C.c_void_p, C.c_int, C.c_int, C.c_float]
+        L.orc_stable_dt.restype = C.c_float
+        L.orc_stable_dt.argtypes = [C.c_float] * 7
+        L.orc_turbulence.restype = C.c_float
+        L.orc_turbulence.argtypes = [C.POINTER(OrcGrid), _f32p, _f32p, _f32p, _f32p, _f32p]
+        L.orc_heat_diffusion.argtypes = [C.POINTER(OrcGrid), C.c_float, _f32p, _f32p, _f32p, _f32p]
+        L.orc_momentum.argtypes = [C.POINTER(OrcGrid), C.c_void_p, C.c_float] + [_f32p] * 10
+        L.orc_add_gas.argtypes = [C.c_int, _i32p, C.c_float, C.c_float, C.c_float, C.c_float, C.c_float, C.c_int,
+                                  _u8p, C.POINTER(OrcSpecies), _u8p]
+        L.orc_combustion.restype = C.c_double
+        L.orc_combustion.argtypes = [C.POINTER(OrcGrid), C.c_float, C.c_int, C.POINTER(OrcSpecies), _f32p, _f32p,
+                                     _f32p, _u8p, _u8p]
         _lib = L
     return _lib
+
+
+class OrcGrid(C.Structure):
+    _fields_ = [("nx", C.c_int), ("ny", C.c_int), ("nz", C.c_int), ("dx", C.c_float),
+                ("fx", C.c_void_p), ("fy", C.c_void_p), ("fz", C.c_void_p), ("blocked", C.c_void_p),
+                ("lx", C.c_void_p), ("ly", C.c_void_p), ("lz", C.c_void_p)]
+
+
+class OrcWind(C.Structure):
+    _fields_ = [("speed", C.c_float), ("from_deg", C.c_float), ("exponent", C.c_float), ("oz", C.c_float)]
+
+
+SPECIES = ("T", "Pb", "Yo", "Yf", "Ys", "Ya", "Yp", "Yc", "Yw", "Yco", "Yai", "massSrc")
+
+
+class OrcSpecies(C.Structure):
+    _fields_ = [(n, C.c_void_p) for n in SPECIES]
 
 
 def _ptr(a):
@@ -206,6 +236,67 @@ def project(hier, grid, dt, u, v, w, p, S, **kw):
     return dict(iters=out.iters, residual=out.residual, max_vel=out.max_vel, relres=out.relres, b=b, diag=d)
 
 
+def orc_grid(grid, leaks=None):
+    """The oracle's view of a scenario grid (kept alive with the grid)."""
+    og = OrcGrid(grid.nx, grid.ny, grid.nz, grid.dx, _ptr(grid.fx), _ptr(grid.fy), _ptr(grid.fz),
+                 _ptr(grid.blocked), None, None, None)
+    if leaks is not None:
+        og._leaks = [np.ascontiguousarray(a, np.uint8) for a in leaks]
+        og.lx, og.ly, og.lz = (_ptr(a) for a in og._leaks)
+    return og
+
+
+def wind(speed, from_deg, exponent=0.25, oz=0.0):
+    return OrcWind(speed, from_deg, exponent, oz)
+
+
+def wind_at(wd, axis, side, z):
+    return lib().orc_wind_at(C.byref(wd) if wd is not None else None, axis, side, z)
+
+
+def stable_dt(dt_max, profile_dt_max, cfl, dx, max_vel, max_nu, max_div):
+    return lib().orc_stable_dt(dt_max, profile_dt_max, cfl, dx, max_vel, max_nu, max_div)
+
+
+def turbulence(grid, u, v, w, T):
+    nu = np.zeros(grid.n, np.float32)
+    m = lib().orc_turbulence(C.byref(orc_grid(grid)), *[np.ascontiguousarray(a, np.float32) for a in (u, v, w, T)],
+                             nu)
+    return nu, m
+
+
+def heat_diffusion(grid, dt, T, nu, Pb, Q):
+    """Q is updated in place."""
+    lib().orc_heat_diffusion(C.byref(orc_grid(grid)), dt, *[np.ascontiguousarray(a, np.float32) for a in (T, nu, Pb)],
+                             Q)
+
+
+def momentum(grid, dt, u, v, w, T, Pb, nu, p, wd=None, leaks=None):
+    """Returns the new (u, v, w)."""
+    outs = [np.zeros(len(a), np.float32) for a in (u, v, w)]
+    ins = [np.ascontiguousarray(a, np.float32) for a in (u, v, w, T, Pb, nu, p)]
+    lib().orc_momentum(C.byref(orc_grid(grid, leaks)), C.byref(wd) if wd is not None else None, dt, *ins, *outs)
+    return outs
+
+
+def species(f):
+    """An OrcSpecies over a dict of float32 cell arrays (updated in place)."""
+    sp = OrcSpecies(*[_ptr(f[n]) for n in SPECIES])
+    sp._keep = f
+    return sp
+
+
+def add_gas(grid, cells, fuel_gen, aerosol, soot_pot, ait, set_pilot, f, pilot):
+    cells = np.ascontiguousarray(cells, np.int32)
+    lib().orc_add_gas(len(cells), cells, grid.dx, fuel_gen, aerosol, soot_pot, ait, 1 if set_pilot else 0,
+                      grid.blocked, C.byref(species(f)), pilot)
+
+
+def combustion(grid, dt, radiation_on, f, Q, Qr, hrr, pilot, lit):
+    return lib().orc_combustion(C.byref(orc_grid(grid)), dt, 1 if radiation_on else 0, C.byref(species(f)), Q, Qr,
+                                hrr, pilot, lit)
+
+
 def rhs(grid, dt, u, v, w, S):
     a = project_args(grid, dt)
     b = np.zeros(grid.n, np.float32)
@@ -288,6 +379,20 @@ class RefSolver:
             L.ref_advect.argtypes = [C.c_void_p, C.c_float]
             L.ref_threads.argtypes = [C.c_void_p]
             L.ref_set_wind.argtypes = [C.c_void_p, C.c_float, C.c_float, C.c_float]
+            L.ref_set_bytes.argtypes = [C.c_void_p, C.c_char_p, _u8p]
+            L.ref_get_bytes.argtypes = [C.c_void_p, C.c_char_p, _u8p]
+            L.ref_turbulence.restype = C.c_float
+            L.ref_turbulence.argtypes = [C.c_void_p]
+            for nm in ("ref_heat_diffusion", "ref_momentum", "ref_inject_burners", "ref_combustion", "ref_leakage"):
+                getattr(L, nm).argtypes = [C.c_void_p, C.c_float]
+            L.ref_stable_dt.restype = C.c_float
+            L.ref_stable_dt.argtypes = [C.c_void_p] + [C.c_float] * 6
+            L.ref_add_burner.argtypes = [C.c_void_p] + [C.c_float] * 7
+            L.ref_burner_cells.argtypes = [C.c_void_p, C.c_int, C.c_void_p]
+            L.ref_step.restype = C.c_float
+            L.ref_step.argtypes = [C.c_void_p, C.c_float, C.c_float, C.c_int]
+            L.ref_scalars.argtypes = [C.c_void_p, _f32p]
+            L.ref_zone_pressure.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p]
             L.ref_wind_at.restype = C.c_float
             L.ref_wind_at.argtypes = [C.c_void_p, C.c_int, C.c_int, C.c_float]
             cls._libs[path] = L
@@ -349,6 +454,59 @@ class RefSolver:
                                _ptr(d["diag"]), _ptr(d["agg"]))
             out.append(d)
         return out
+
+    def set_bytes(self, name, arr):
+        if self._L.ref_set_bytes(self.h, name.encode(), np.ascontiguousarray(arr, np.uint8)) < 0:
+            raise KeyError(name)
+
+    def get_bytes(self, name, size):
+        out = np.zeros(size, np.uint8)
+        if self._L.ref_get_bytes(self.h, name.encode(), out) < 0:
+            raise KeyError(name)
+        return out
+
+    def turbulence(self):
+        return self._L.ref_turbulence(self.h)
+
+    def heat_diffusion(self, dt):
+        self._L.ref_heat_diffusion(self.h, dt)
+
+    def momentum(self, dt):
+        self._L.ref_momentum(self.h, dt)
+
+    def inject_burners(self, dt):
+        self._L.ref_inject_burners(self.h, dt)
+
+    def combustion(self, dt):
+        self._L.ref_combustion(self.h, dt)
+
+    def leakage(self, dt):
+        self._L.ref_leakage(self.h, dt)
+
+    def stable_dt(self, dt_max, max_vel, max_nu, max_div, cfl=0.8, profile_dt_max=0.05):
+        return self._L.ref_stable_dt(self.h, dt_max, max_vel, max_nu, max_div, cfl, profile_dt_max)
+
+    def add_burner(self, x0, y0, x1, y1, z, peak_w, ramp_s=0.0):
+        b = self._L.ref_add_burner(self.h, x0, y0, x1, y1, z, peak_w, ramp_s)
+        n = self._L.ref_burner_cells(self.h, b, None)
+        cells = np.zeros(n, np.int32)
+        self._L.ref_burner_cells(self.h, b, cells.ctypes.data)
+        return b, cells
+
+    def step(self, dt_max, tol, max_iters):
+        return self._L.ref_step(self.h, dt_max, tol, max_iters)
+
+    def scalars(self):
+        out = np.zeros(6, np.float32)
+        it = self._L.ref_scalars(self.h, out)
+        return dict(time=out[0], max_vel=out[1], max_nu=out[2], max_div=out[3], residual=out[4], hrr=out[5],
+                    iters=it)
+
+    def zone_pressure(self):
+        n = self._L.ref_zone_pressure(self.h, None, None)
+        zp, zdp = np.zeros(max(n, 1), np.float32), np.zeros(max(n, 1), np.float32)
+        self._L.ref_zone_pressure(self.h, zp.ctypes.data, zdp.ctypes.data)
+        return zp[:n], zdp[:n]
 
     def set_wind(self, speed, from_deg, exponent=0.25):
         self._L.ref_set_wind(self.h, speed, from_deg, exponent)

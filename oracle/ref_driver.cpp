@@ -51,6 +51,17 @@ std::vector<float>* field(Solver* s, const char* name) {
     if (n == "massSrc") return &s->massSrc;
     if (n == "rhs") return &s->rhs_;
     if (n == "diag") return &s->tmp_[9];
+    if (n == "Qr") return &s->Qr;
+    if (n == "hrr") return &s->hrr;
+    return nullptr;
+}
+std::vector<uint8_t>* bytes(Solver* s, const char* name) {
+    std::string n(name);
+    if (n == "lit") return &s->lit;
+    if (n == "pilot") return &s->pilot;
+    if (n == "lx") return &s->lx_;
+    if (n == "ly") return &s->ly_;
+    if (n == "lz") return &s->lz_;
     return nullptr;
 }
 }  // namespace
@@ -85,9 +96,92 @@ int ref_setup(void* h, int nx, int ny, int nz, float dx, const float* fx, const 
     s->rhs_.assign(N, 0.0f);
     for (auto& t : s->tmp_) t.assign(N, 0.0f);
     s->cgR_.assign(N, 0.0f); s->cgZ_.assign(N, 0.0f); s->cgP_.assign(N, 0.0f); s->cgQ_.assign(N, 0.0f);
+    s->Qr.assign(N, 0.0f); s->hrr.assign(N, 0.0f); s->lit.assign(N, 0); s->pilot.assign(N, 0);
+    s->u2_.assign(nfx, 0.0f); s->v2_.assign(nfy, 0.0f); s->w2_.assign(nfz, 0.0f);
+    // a gas-only world: no radiation, no surface exchange, no solids or barriers
+    // (the Phase 3 comparison switches, as buildRoom in test_main.cpp:420-433)
+    s->radiationOn = false;
+    s->adiabaticSurfaces = true;
     s->zone.assign(N, -1);
     s->computeZones();
     s->mgDirty_ = true;
+    return (int)s->zoneP.size();
+}
+
+int ref_set_bytes(void* h, const char* name, const uint8_t* data) {
+    std::vector<uint8_t>* f = bytes(static_cast<Solver*>(h), name);
+    if (!f) return -1;
+    std::copy(data, data + f->size(), f->begin());
+    return (int)f->size();
+}
+
+int ref_get_bytes(void* h, const char* name, uint8_t* out) {
+    std::vector<uint8_t>* f = bytes(static_cast<Solver*>(h), name);
+    if (!f) return -1;
+    std::copy(f->begin(), f->end(), out);
+    return (int)f->size();
+}
+
+// the private phases of Solver::step (solver.cpp:226-269), one at a time
+float ref_turbulence(void* h) {
+    Solver* s = static_cast<Solver*>(h);
+    s->turbulence();
+    return s->maxNu;
+}
+void ref_heat_diffusion(void* h, float dt) { static_cast<Solver*>(h)->heatDiffusion(dt); }
+void ref_momentum(void* h, float dt) { static_cast<Solver*>(h)->momentum(dt); }
+void ref_inject_burners(void* h, float dt) { static_cast<Solver*>(h)->injectBurners(dt); }
+void ref_combustion(void* h, float dt) { static_cast<Solver*>(h)->combustion(dt); }
+void ref_leakage(void* h, float dt) { static_cast<Solver*>(h)->leakage(dt); }
+
+// stableDt (gasphase.cpp:63-70) from the given step statistics and profile
+float ref_stable_dt(void* h, float dtMax, float maxVel, float maxNu, float maxDiv, float cfl, float profileDtMax) {
+    Solver* s = static_cast<Solver*>(h);
+    s->maxVel = maxVel;
+    s->maxNu = maxNu;
+    s->maxDiv = maxDiv;
+    s->profile.cfl = cfl;
+    s->profile.dtMax = profileDtMax;
+    return s->stableDt(dtMax);
+}
+
+// a burner over a floor rectangle (Solver::addBurner, gasphase.cpp:72-90)
+int ref_add_burner(void* h, float x0, float y0, float x1, float y1, float z, float peakW, float rampS) {
+    photon::Burner b;
+    b.x0 = x0; b.y0 = y0; b.x1 = x1; b.y1 = y1; b.z = z; b.peakW = peakW; b.rampS = rampS;
+    b.label = "burner";
+    return static_cast<Solver*>(h)->addBurner(b);
+}
+int ref_burner_cells(void* h, int b, int* cells) {
+    const std::vector<int>& c = static_cast<Solver*>(h)->burners[b].cells;
+    if (cells) std::copy(c.begin(), c.end(), cells);
+    return (int)c.size();
+}
+
+// the whole step with the solver's stop rule (profile); returns dt
+float ref_step(void* h, float dtMax, float tol, int maxIters) {
+    Solver* s = static_cast<Solver*>(h);
+    s->profile.pressureTol = tol;
+    s->profile.pressureMaxIters = maxIters;
+    return s->step(dtMax);
+}
+
+// scalars of the last step: time, maxVel, maxNu, maxDiv, pressureResidual,
+// hrr; pressureIters as the return value
+int ref_scalars(void* h, float* out6) {
+    Solver* s = static_cast<Solver*>(h);
+    out6[0] = s->time; out6[1] = s->maxVel; out6[2] = s->maxNu; out6[3] = s->maxDiv;
+    out6[4] = s->pressureResidual; out6[5] = s->stats.hrr;
+    return s->pressureIters;
+}
+
+// zone background pressures and their rise rates of the last step
+int ref_zone_pressure(void* h, float* zoneP, float* zoneDP) {
+    Solver* s = static_cast<Solver*>(h);
+    if (zoneP) std::copy(s->zoneP.begin(), s->zoneP.end(), zoneP);
+    if (zoneDP) {
+        for (size_t z = 0; z < s->zoneP.size(); ++z) zoneDP[z] = z < s->zoneDP_.size() ? s->zoneDP_[z] : 0.0f;
+    }
     return (int)s->zoneP.size();
 }
 

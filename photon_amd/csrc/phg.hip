@@ -13,6 +13,7 @@
 #include <new>
 #include <stdexcept>
 #include <string>
+#include <limits>
 #include <vector>
 
 #include "../../include/photon_gas.h"
@@ -21,6 +22,7 @@
 #include "kernels_pcg.hpp"
 #include "kernels_step.hpp"
 #include "kernels_transport.hpp"
+#include "kernels_flow.hpp"
 
 // Device workspace of phg_zone_mean_noalloc, the entry point that allocates
 // nothing: the closed-zone partial sums (one per task) and then the means
@@ -39,13 +41,14 @@ namespace phg {
 enum KClass {
     K_RESID_INIT, K_DIR_APPLY, K_PRE_RED, K_RB_BLACK, K_RESID0, K_POST_BLACK, K_POST_RED, K_FIN, K_VC_RESTRICT,
     K_VC_JAC, K_VC_RESID, K_VC_PROLONG, K_VC_COARSE, K_XFLUSH, K_ZONE, K_RHS, K_CORRECT, K_ENERGY, K_ADVECT,
-    K_COUNT, K_PRE_SMOOTH, K_POST_SMOOTH, K_NCLASS
+    K_COUNT, K_PRE_SMOOTH, K_POST_SMOOTH, K_TURB, K_HEAT, K_MOMENTUM, K_COMBUST, K_GASMISC, K_NCLASS
 };
 static const char* kKName[K_NCLASS] = {"resid_init",  "dir_apply",  "pre_red",    "rb_black",  "resid0",
                                        "post_black",  "post_red",   "finalize",   "vc_restrict", "vc_jacobi",
                                        "vc_resid",    "vc_prolong", "vc_coarse",  "x_flush",   "zone_mean",
                                        "rhs",         "correct",    "energy",     "advect",    "count_active",
-                                       "pre_smooth",  "post_smooth"};
+                                       "pre_smooth",  "post_smooth", "turbulence", "heat_diffusion",
+                                       "momentum",    "combustion",  "gas_misc"};
 
 struct GridDev {
     int nx = 0, ny = 0, nz = 0, n = 0;
@@ -82,6 +85,15 @@ struct GridDev {
     int4* ztask = nullptr;
     float* scratch_b = nullptr;   // RHS of phg_project
     float* vmax = nullptr;        // one float for max reductions
+    // the rest of the gas step
+    uint8_t* lit_new = nullptr;   // combustion's new flame flags (N)
+    double* cpart = nullptr;      // combustion's per-block heat release sums
+    double* zsum = nullptr;       // closed zones: task sums, then zone sums
+    float* zval = nullptr;        // closed zones: one value each (shift or pressure)
+    float* wind = nullptr;        // [axis 0..1][side 0..1][k] boundary inflow (NaN: none)
+    bool wind_set = false;
+    float wind_key[4] = {0, 0, 0, 0};
+    uint8_t *lx = nullptr, *ly = nullptr, *lz = nullptr;   // crack (leak) faces, or none
     std::vector<void*> allocs;
     int single_batch = -1;
     DGrid dg() const {
@@ -922,6 +934,11 @@ int phg_grid_create_planned(phg_ctx* ctx, int nx, int ny, int nz, float dx, cons
         }
         if (!rc) rc = dalloc(ctx, G->allocs, &G->scratch_b, (size_t)G->n);
         if (!rc) rc = dalloc(ctx, G->allocs, &G->vmax, 1);
+        if (!rc) rc = dalloc(ctx, G->allocs, &G->lit_new, (size_t)G->n);
+        if (!rc) rc = dalloc(ctx, G->allocs, &G->cpart, (size_t)tiles(G->n));
+        if (!rc) rc = dalloc(ctx, G->allocs, &G->zsum, (size_t)(G->nztask + G->nclosed + 1));
+        if (!rc) rc = dalloc(ctx, G->allocs, &G->zval, (size_t)(G->nclosed + 1));
+        if (!rc) rc = dalloc(ctx, G->allocs, &G->wind, (size_t)4 * nz);
     #undef UP
         if (rc) {
             destroy_grid(G);
@@ -1672,6 +1689,233 @@ int phg_advect_scalars(phg_ctx* ctx, int grid, float dt, const float* u, const f
     A.kchunk = std::max(4, (G->nz + want - 1) / want);
     const int nchunk = (G->nz + A.kchunk - 1) / A.kchunk;
     launch_nt(ctx, K_ADVECT, k_transport, ntile * nchunk, TR_THREADS, 0, A);
+    HIP_OK(hipGetLastError());
+    return PHG_OK;
+}
+
+// ---------------------------------------------------------------------------
+// the rest of the gas step
+// ---------------------------------------------------------------------------
+int phg_stable_dt(float dt_max, float profile_dt_max, float cfl, float dx, float max_vel, float max_nu, float max_div,
+                  float* dt_out) {
+    if (!dt_out) return PHG_EINVAL;
+    // stableDt (gasphase.cpp:63-70), on the host with the reference's float arithmetic
+    const float PR_T = 0.5f;
+    float dt = std::min(dt_max, profile_dt_max);
+    dt = std::min(dt, cfl * dx / std::max(max_vel, 0.1f));
+    const float alpha = max_nu * (1.0f + 1.0f / PR_T);
+    if (alpha > 0) dt = std::min(dt, 0.9f * dx * dx / (6.0f * alpha));
+    if (max_div > 0) dt = std::min(dt, 0.25f / max_div);
+    *dt_out = std::max(dt, 1e-3f);
+    return PHG_OK;
+}
+
+int phg_turbulence(phg_ctx* ctx, int grid, const float* u, const float* v, const float* w, const float* T, float* nu,
+                   float* max_nu) {
+    if (!ctx || !u || !v || !w || !T || !nu) return PHG_EINVAL;
+    GridDev* G = get_grid(ctx, grid);
+    if (!G) return fail(ctx, PHG_ESTATE, "unknown grid %d", grid);
+    TurbArgs A;
+    A.nx = G->nx;
+    A.ny = G->ny;
+    A.nz = G->nz;
+    A.dx = G->dx;
+    A.u = u;
+    A.v = v;
+    A.w = w;
+    A.T = T;
+    A.blocked = G->blocked;
+    A.nu = nu;
+    A.max_nu = G->vmax;
+    A.tiles_x = (G->nx + TU_TX - 1) / TU_TX;
+    A.tiles_y = (G->ny + TU_TY - 1) / TU_TY;
+    const int ntile = A.tiles_x * A.tiles_y;
+    const int want = std::max(1, (1024 + ntile - 1) / ntile);
+    A.kchunk = std::max(4, (G->nz + want - 1) / want);
+    const int nchunk = (G->nz + A.kchunk - 1) / A.kchunk;
+    hipLaunchKernelGGL(k_fill_f32, dim3(1), dim3(64), 0, ctx->stream, G->vmax, 1, 0.0f);
+    launch_nt(ctx, K_TURB, k_turbulence, ntile * nchunk, TU_THREADS, 0, A);
+    if (max_nu) {
+        HIP_OK(hipMemcpyAsync(max_nu, G->vmax, sizeof(float), hipMemcpyDeviceToHost, ctx->stream));
+        HIP_OK(hipStreamSynchronize(ctx->stream));
+    }
+    HIP_OK(hipGetLastError());
+    return PHG_OK;
+}
+
+int phg_heat_diffusion(phg_ctx* ctx, int grid, float dt, const float* T, const float* nu, const float* Pb, float* Q) {
+    if (!ctx || !T || !nu || !Pb || !Q) return PHG_EINVAL;
+    GridDev* G = get_grid(ctx, grid);
+    if (!G) return fail(ctx, PHG_ESTATE, "unknown grid %d", grid);
+    const CellGrid cg{G->nx, G->ny, G->nz, G->dx, G->fx, G->fy, G->fz, G->blocked};
+    launch(ctx, K_HEAT, k_heat_diffusion, tiles(G->n), 0, cg, dt, T, nu, Pb, Q);
+    HIP_OK(hipGetLastError());
+    return PHG_OK;
+}
+
+namespace {
+// windAt (gasphase.cpp:52-61) for every boundary level, on the host with the
+// host C library, so the device applies the reference's own values
+int wind_table(phg_ctx* ctx, GridDev* G, const phg_wind* wd) {
+    const float key[4] = {wd->speed, wd->from_deg, wd->exponent, wd->oz};
+    if (G->wind_set && std::memcmp(key, G->wind_key, sizeof key) == 0) return 0;
+    std::vector<float> t((size_t)4 * G->nz, std::numeric_limits<float>::quiet_NaN());
+    if (wd->speed > 0) {
+        const float th = wd->from_deg * 3.14159265f / 180.0f;
+        for (int axis = 0; axis < 2; ++axis)
+            for (int k = 0; k < G->nz; ++k) {
+                const float z = wd->oz + (k + 0.5f) * G->dx;
+                const float U = wd->speed * std::pow(std::max(z, 0.05f) / 10.0f, wd->exponent);
+                const float comp = axis == 0 ? -std::sin(th) * U : -std::cos(th) * U;
+                if (comp > 0) t[(size_t)(axis * 2 + 0) * G->nz + k] = comp;   // enters through the low side
+                if (comp < 0) t[(size_t)(axis * 2 + 1) * G->nz + k] = comp;   // through the high side
+            }
+    }
+    HIP_OK(hipMemcpy(G->wind, t.data(), t.size() * sizeof(float), hipMemcpyHostToDevice));
+    std::memcpy(G->wind_key, key, sizeof key);
+    G->wind_set = true;
+    return 0;
+}
+}  // namespace
+
+int phg_momentum(phg_ctx* ctx, int grid, float dt, const phg_wind* wind, const float* u, const float* v,
+                 const float* w, const float* T, const float* Pb, const float* nu, const float* p, float* u_out,
+                 float* v_out, float* w_out) {
+    if (!ctx || !u || !v || !w || !T || !Pb || !nu || !p || !u_out || !v_out || !w_out) return PHG_EINVAL;
+    GridDev* G = get_grid(ctx, grid);
+    if (!G) return fail(ctx, PHG_ESTATE, "unknown grid %d", grid);
+    if (u_out == u || v_out == v || w_out == w) return fail(ctx, PHG_EINVAL, "momentum output aliases its input");
+    const bool windy = wind && wind->speed > 0;
+    if (windy) {
+        int rc = wind_table(ctx, G, wind);
+        if (rc) return rc;
+    }
+    FlowArgs A;
+    A.nx = G->nx;
+    A.ny = G->ny;
+    A.nz = G->nz;
+    A.dx = G->dx;
+    A.dt = dt;
+    A.u = u;
+    A.v = v;
+    A.w = w;
+    A.fx = G->fx;
+    A.fy = G->fy;
+    A.fz = G->fz;
+    A.lx = G->lx;
+    A.ly = G->ly;
+    A.lz = G->lz;
+    A.T = T;
+    A.Pb = Pb;
+    A.nu = nu;
+    A.p = p;
+    A.wind = windy ? G->wind : nullptr;
+    A.u2 = u_out;
+    A.v2 = v_out;
+    A.w2 = w_out;
+    const int nx = G->nx, ny = G->ny, nz = G->nz;
+    launch(ctx, K_MOMENTUM, k_momentum<0>, tiles((nx + 1) * ny * nz), 0, A);
+    launch(ctx, K_MOMENTUM, k_momentum<1>, tiles(nx * (ny + 1) * nz), 0, A);
+    launch(ctx, K_MOMENTUM, k_momentum<2>, tiles(nx * ny * (nz + 1)), 0, A);
+    launch(ctx, K_MOMENTUM, k_momentum_edge<0>, tiles(2 * ny * nz), 0, A);
+    launch(ctx, K_MOMENTUM, k_momentum_edge<1>, tiles(2 * nz * nx), 0, A);
+    launch(ctx, K_MOMENTUM, k_momentum_edge<2>, tiles(2 * nx * ny), 0, A);
+    HIP_OK(hipGetLastError());
+    return PHG_OK;
+}
+
+namespace {
+bool gas_ok(const phg_gas* g) {
+    return g && g->T && g->Yo && g->Yf && g->Ys && g->Ya && g->Yp && g->Yc && g->Yw && g->Yco && g->Yai && g->Pb &&
+           g->massSrc && g->Q && g->Qr && g->hrr && g->pilot && g->lit;
+}
+GasFields gas_fields(const phg_gas* g) {
+    return GasFields{g->T,  g->Yo, g->Yf,      g->Ys, g->Ya, g->Yp,  g->Yc,    g->Yw,  g->Yco,
+                     g->Yai, g->Pb, g->massSrc, g->Q,  g->Qr, g->hrr, g->pilot, g->lit};
+}
+}  // namespace
+
+int phg_step_clear(phg_ctx* ctx, int grid, const phg_gas* gas) {
+    if (!ctx || !gas_ok(gas)) return PHG_EINVAL;
+    GridDev* G = get_grid(ctx, grid);
+    if (!G) return fail(ctx, PHG_ESTATE, "unknown grid %d", grid);
+    launch(ctx, K_GASMISC, k_step_clear, tiles(G->n), 0, G->n, gas->Q, gas->Qr, gas->pilot, gas->massSrc);
+    HIP_OK(hipGetLastError());
+    return PHG_OK;
+}
+
+int phg_add_gas(phg_ctx* ctx, int grid, int ncells, const int* cells, float fuel_gen, float aerosol, float soot_pot,
+                float ait, int set_pilot, const phg_gas* gas) {
+    if (!ctx || ncells < 0 || (ncells > 0 && !cells) || !gas_ok(gas)) return PHG_EINVAL;
+    GridDev* G = get_grid(ctx, grid);
+    if (!G) return fail(ctx, PHG_ESTATE, "unknown grid %d", grid);
+    if (ncells == 0) return PHG_OK;
+    launch(ctx, K_GASMISC, k_add_gas, (ncells + TB - 1) / TB, 0, ncells, cells, (const uint8_t*)G->blocked, G->dx,
+           fuel_gen, aerosol, soot_pot, ait, set_pilot, gas_fields(gas));
+    HIP_OK(hipGetLastError());
+    return PHG_OK;
+}
+
+int phg_combustion(phg_ctx* ctx, int grid, float dt, int radiation_on, const phg_gas* gas, double* hrr_total) {
+    if (!ctx || !gas_ok(gas)) return PHG_EINVAL;
+    GridDev* G = get_grid(ctx, grid);
+    if (!G) return fail(ctx, PHG_ESTATE, "unknown grid %d", grid);
+    if (!(dt > 0)) return fail(ctx, PHG_EINVAL, "dt must be positive");
+    // mixing-limited fraction and radiant share of the flame heat (solver.cpp:287-289)
+    const float frac = 1.0f - std::exp(-dt / 0.1f);
+    const float chiR = radiation_on ? 0.35f : 0.0f;
+    const int nb = tiles(G->n);
+    launch(ctx, K_COMBUST, k_combustion, nb, 0, G->nx, G->ny, G->nz, G->dx, dt, frac, chiR,
+           (const uint8_t*)G->blocked, gas_fields(gas), G->lit_new, G->cpart);
+    launch_nt(ctx, K_COMBUST, k_copy_u8, (G->n + TB - 1) / TB, TB, 0, (const uint8_t*)G->lit_new, gas->lit, G->n);
+    if (hrr_total) {
+        std::vector<double> part(nb);
+        HIP_OK(hipMemcpyAsync(part.data(), G->cpart, nb * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
+        HIP_OK(hipStreamSynchronize(ctx->stream));
+        double t = 0.0;
+        for (double x : part) t += x;
+        *hrr_total = t;
+    }
+    HIP_OK(hipGetLastError());
+    return PHG_OK;
+}
+
+int phg_zone_integrals(phg_ctx* ctx, int grid, const float* S, double* dint) {
+    if (!ctx || !S || !dint) return PHG_EINVAL;
+    GridDev* G = get_grid(ctx, grid);
+    if (!G) return fail(ctx, PHG_ESTATE, "unknown grid %d", grid);
+    const size_t nzn = G->zone_open.size();
+    std::fill(dint, dint + nzn, 0.0);
+    if (G->nclosed == 0) return PHG_OK;
+    const float V = G->dx * G->dx * G->dx;
+    launch(ctx, K_ZONE, k_zone_sv, G->nztask, 0, (const int4*)G->ztask, (const int*)G->zcells, S, V, G->zsum);
+    launch(ctx, K_ZONE, k_zone_fin_sum, 1, 0, G->nclosed, (const int*)G->ztptr, (const double*)G->zsum,
+           G->zsum + G->nztask);
+    std::vector<double> zs(G->nclosed);
+    HIP_OK(hipMemcpyAsync(zs.data(), G->zsum + G->nztask, G->nclosed * sizeof(double), hipMemcpyDeviceToHost,
+                          ctx->stream));
+    HIP_OK(hipStreamSynchronize(ctx->stream));
+    int k = 0;
+    for (size_t z = 0; z < nzn; ++z)
+        if (!G->zone_open[z]) dint[z] = zs[k++];
+    HIP_OK(hipGetLastError());
+    return PHG_OK;
+}
+
+int phg_zone_values(phg_ctx* ctx, int grid, const float* values, float* x, int subtract) {
+    if (!ctx || !values || !x) return PHG_EINVAL;
+    GridDev* G = get_grid(ctx, grid);
+    if (!G) return fail(ctx, PHG_ESTATE, "unknown grid %d", grid);
+    if (G->nclosed == 0) return PHG_OK;
+    std::vector<float> cv;
+    cv.reserve(G->nclosed);
+    for (size_t z = 0; z < G->zone_open.size(); ++z)
+        if (!G->zone_open[z]) cv.push_back(values[z]);
+    HIP_OK(hipMemcpyAsync(G->zval, cv.data(), cv.size() * sizeof(float), hipMemcpyHostToDevice, ctx->stream));
+    launch(ctx, K_ZONE, k_zone_apply, tiles(G->n), 0, G->n, (const int*)G->zclosed, (const float*)G->zval, x,
+           subtract ? 1 : 0);
+    // the staging vector is pageable: wait before it goes out of scope
+    HIP_OK(hipStreamSynchronize(ctx->stream));
     HIP_OK(hipGetLastError());
     return PHG_OK;
 }

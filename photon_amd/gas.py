@@ -33,6 +33,8 @@ EXPORTS = (
     "phg_hierarchy_shape", "phg_batch_create_ex", "phg_copy", "phg_leakage",
     "phg_grid_plan_build", "phg_grid_plan_free", "phg_grid_create_planned", "phg_zone_mean_noalloc",
     "phg_grid_set_coarse", "phg_grid_get_coarse",
+    "phg_stable_dt", "phg_turbulence", "phg_heat_diffusion", "phg_momentum", "phg_step_clear", "phg_add_gas",
+    "phg_combustion", "phg_zone_integrals", "phg_zone_values",
 )
 
 
@@ -49,6 +51,19 @@ class SolveParams(C.Structure):
 class BatchOpts(C.Structure):
     _fields_ = [("kchunk", C.c_int), ("level0_blocks", C.c_int), ("unfused", C.c_int), ("sweeps0", C.c_int),
                 ("sys_kchunk", C.c_void_p), ("sys_sweeps0", C.c_void_p), ("fused_threads", C.c_int)]
+
+
+class Wind(C.Structure):
+    """Solver::wind (src/sim/solver.h:106-111) and the grid's floor height."""
+    _fields_ = [("speed", C.c_float), ("from_deg", C.c_float), ("exponent", C.c_float), ("oz", C.c_float)]
+
+
+GAS_FIELDS = ("T", "Yo", "Yf", "Ys", "Ya", "Yp", "Yc", "Yw", "Yco", "Yai", "Pb", "massSrc", "Q", "Qr", "hrr",
+              "pilot", "lit")
+
+
+class GasPtrs(C.Structure):
+    _fields_ = [(n, C.c_void_p) for n in GAS_FIELDS]
 
 
 class SolveStats(C.Structure):
@@ -100,6 +115,15 @@ def load_library():
     L.phg_zone_mean_noalloc.argtypes = [vp, i32, vp]
     L.phg_grid_set_coarse.argtypes = [vp, i32, i32]
     L.phg_grid_get_coarse.argtypes = [vp, i32, C.POINTER(i32)]
+    L.phg_stable_dt.argtypes = [f32] * 7 + [C.POINTER(f32)]
+    L.phg_turbulence.argtypes = [vp, i32, vp, vp, vp, vp, vp, C.POINTER(f32)]
+    L.phg_heat_diffusion.argtypes = [vp, i32, f32, vp, vp, vp, vp]
+    L.phg_momentum.argtypes = [vp, i32, f32, C.POINTER(Wind)] + [vp] * 10
+    L.phg_step_clear.argtypes = [vp, i32, C.POINTER(GasPtrs)]
+    L.phg_add_gas.argtypes = [vp, i32, i32, vp, f32, f32, f32, f32, i32, C.POINTER(GasPtrs)]
+    L.phg_combustion.argtypes = [vp, i32, f32, i32, C.POINTER(GasPtrs), C.POINTER(f64)]
+    L.phg_zone_integrals.argtypes = [vp, i32, vp, vp]
+    L.phg_zone_values.argtypes = [vp, i32, vp, vp, i32]
     L.phg_leakage.argtypes = [vp, i32, vp, vp]
     L.phg_correct.argtypes = [vp, i32, f32, vp, vp, vp, vp, C.POINTER(f32)]
     L.phg_energy_expansion.argtypes = [vp, i32, f32, vp, vp, vp, vp, vp, C.POINTER(f32)]
@@ -417,10 +441,12 @@ class GasSolver:
     similar counts, so the host waits about once per solve.  Replays give
     the same results bit for bit as direct launches."""
 
-    CELL_FIELDS = ("p", "S", "Pb", "Q", "massSrc", "nu") + FIELD_NAMES
+    CELL_FIELDS = ("p", "S", "Pb", "Q", "Qr", "hrr", "massSrc", "nu") + FIELD_NAMES
+    BYTE_FIELDS = ("pilot", "lit")
 
     def __init__(self, ctx: Context, scen, precision=PREC_FP32, wind_closed=None, tol=1e-6, max_iters=400,
-                 stop_rule=STOP_REL_L2, plan: GridPlan | None = None, graph_steps=True, coarse=None):
+                 stop_rule=STOP_REL_L2, plan: GridPlan | None = None, graph_steps=True, coarse=None, wind=None,
+                 radiation_on=False, dt_max_profile=0.05, cfl=0.8):
         self.ctx = ctx
         self.scen = scen
         self.graph_steps = graph_steps
@@ -433,9 +459,29 @@ class GasSolver:
             size = {"u": len(scen.fx), "v": len(scen.fy), "w": len(scen.fz)}[name]
             self.d[name] = ctx.array(size)
             self.d[name].zero()
+            self.d[name + "2"] = ctx.array(size)      # momentum writes here, then the two swap
+            self.d[name + "2"].zero()
         init = dict(T=293.15, Yo=0.232, Yc=0.0006, Yw=0.0073, Pb=101325.0)
         for name in self.CELL_FIELDS:
             self.d[name] = ctx.to_device(np.full(n, init.get(name, 0.0), np.float32))
+        for name in self.BYTE_FIELDS:
+            self.d[name] = ctx.to_device(np.zeros(n, np.uint8))
+        # Solver::wind, radiationOn and the profile's dtMax and cfl (solver.h:82-111,184-187)
+        self.wind = Wind(*(wind if wind is not None else (0.0, 0.0, 0.25, 0.0)))
+        self.radiation_on = radiation_on
+        self.dt_max_profile, self.cfl = dt_max_profile, cfl
+        self.time = 0.0
+        self.burners = []
+        self.maxNu = 0.0
+        self.hrr_total = 0.0
+        # zones: background pressure per zone and volumes (computeZones, gasphase.cpp:111-158)
+        zone, zopen = self.grid.zones()
+        self.zone_open = zopen
+        self.zoneP = np.full(len(zopen), 101325.0, np.float32)
+        self.zoneDP = np.zeros(len(zopen), np.float32)
+        cnt = np.bincount(zone[zone >= 0], minlength=len(zopen)).astype(np.float64)
+        dxd = np.float64(np.float32(scen.dx))
+        self.zone_vol = (cnt * dxd * dxd * dxd).astype(np.float32)
         self._tmp = [ctx.array(n) for _ in range(10)]
         self.d["b"] = ctx.array(n)       # right-hand side of the split projection (rhs / correct)
         self.d["diag"] = ctx.array(n)
@@ -447,7 +493,7 @@ class GasSolver:
         self.maxDiv = 0.0
 
     def set(self, name, arr):
-        self.d[name].upload(np.asarray(arr, np.float32))
+        self.d[name].upload(np.asarray(arr, self.d[name].dtype))
 
     def get(self, name):
         return self.d[name].download()
@@ -507,6 +553,151 @@ class GasSolver:
         if report:
             self.maxDiv = md.value
         return self.maxDiv
+
+    # ---- the rest of the gas step (Solver::step, src/sim/solver.cpp:226-269)
+    def _gas(self):
+        return GasPtrs(*[self.d[n].ptr for n in GAS_FIELDS])
+
+    def turbulence(self):
+        """Solver::turbulence: nu and maxNu (gasphase.cpp:163-218)."""
+        m = C.c_float()
+        d = self.d
+        self.ctx._check(self.ctx.L.phg_turbulence(self.ctx.h, self.grid.id, d["u"].ptr, d["v"].ptr, d["w"].ptr,
+                                                  d["T"].ptr, d["nu"].ptr, C.byref(m)))
+        self.maxNu = m.value
+        return self.maxNu
+
+    def heat_diffusion(self, dt):
+        d = self.d
+        self.ctx._check(self.ctx.L.phg_heat_diffusion(self.ctx.h, self.grid.id, dt, d["T"].ptr, d["nu"].ptr,
+                                                      d["Pb"].ptr, d["Q"].ptr))
+
+    def momentum(self, dt):
+        """Solver::momentum (gasphase.cpp:275-399): new face velocities, swapped in."""
+        d = self.d
+        self.ctx._check(self.ctx.L.phg_momentum(self.ctx.h, self.grid.id, dt, C.byref(self.wind), d["u"].ptr,
+                                                d["v"].ptr, d["w"].ptr, d["T"].ptr, d["Pb"].ptr, d["nu"].ptr,
+                                                d["p"].ptr, d["u2"].ptr, d["v2"].ptr, d["w2"].ptr))
+        for f in ("u", "v", "w"):
+            d[f], d[f + "2"] = d[f + "2"], d[f]
+
+    def add_burner(self, x0, y0, x1, y1, z, peak_w, ramp_s=0.0, start=0.0, duration=1e9, ox=0.0, oy=0.0, oz=0.0):
+        """Solver::addBurner (gasphase.cpp:72-90): the gas cells under a floor
+        rectangle at height z; returns its index, or -1 if it covers none."""
+        g = self.scen
+        f32 = np.float32
+        dx = f32(g.dx)
+
+        def cell_at(x, y, zz):
+            i, j, k = (int(np.floor((f32(a) - f32(o)) / dx)) for a, o in ((x, ox), (y, oy), (zz, oz)))
+            if i < 0 or j < 0 or k < 0 or i >= g.nx or j >= g.ny or k >= g.nz:
+                return -1
+            return i + g.nx * (j + g.ny * k)
+
+        cells = []
+        zc = f32(z) + f32(0.5) * dx
+        for j in range(g.ny):
+            for i in range(g.nx):
+                x, y = f32(ox) + (f32(i) + f32(0.5)) * dx, f32(oy) + (f32(j) + f32(0.5)) * dx
+                if x < f32(x0) or x > f32(x1) or y < f32(y0) or y > f32(y1):
+                    continue
+                c = cell_at(x, y, zc)
+                if c >= 0 and not g.blocked[c]:
+                    cells.append(c)
+        if not cells:
+            c = cell_at(f32(0.5) * (f32(x0) + f32(x1)), f32(0.5) * (f32(y0) + f32(y1)), zc)
+            if c < 0 or g.blocked[c]:
+                return -1
+            cells.append(c)
+        arr = np.array(cells, np.int32)
+        self.burners.append(dict(cells=arr, dcells=self.ctx.to_device(arr), peak=f32(peak_w), ramp=f32(ramp_s),
+                                 start=f32(start), duration=f32(duration)))
+        return len(self.burners) - 1
+
+    def inject_burners(self, dt):
+        """Solver::injectBurners (gasphase.cpp:92-105)."""
+        f32 = np.float32
+        t = f32(self.time)
+        for b in self.burners:
+            if t < b["start"] or t > b["start"] + b["duration"]:
+                continue
+            r = min(f32(1.0), (t - b["start"]) / b["ramp"]) if b["ramp"] > 0 else f32(1.0)
+            P = b["peak"] * r * r
+            if P <= 0:
+                continue
+            mf = P * f32(dt) / f32(25.0e6) / f32(len(b["cells"]))
+            self.ctx._check(self.ctx.L.phg_add_gas(self.ctx.h, self.grid.id, len(b["cells"]), b["dcells"].ptr,
+                                                   mf, 0.0, mf * f32(0.01), 800.0, 1, C.byref(self._gas())))
+
+    def combustion(self, dt):
+        tot = C.c_double()
+        self.ctx._check(self.ctx.L.phg_combustion(self.ctx.h, self.grid.id, dt, 1 if self.radiation_on else 0,
+                                                  C.byref(self._gas()), C.byref(tot)))
+        self.hrr_total = tot.value
+        return self.hrr_total
+
+    def step_clear(self):
+        self.ctx._check(self.ctx.L.phg_step_clear(self.ctx.h, self.grid.id, C.byref(self._gas())))
+
+    def leakage_zones(self, dt):
+        """Solver::leakage for grids without cracks (gasphase.cpp:402-498): each
+        closed zone's pressure rise rate from its integral of S, and the
+        compatible shift of S."""
+        f32 = np.float32
+        nz = len(self.zone_open)
+        dint = np.zeros(max(nz, 1), np.float64)
+        self.ctx._check(self.ctx.L.phg_zone_integrals(self.ctx.h, self.grid.id, self.d["S"].ptr,
+                                                      dint.ctypes.data))
+        gamma = f32(1100.0) / (f32(1100.0) - f32(287.0))
+        shift = np.zeros(max(nz, 1), np.float32)
+        self.zoneDP[:] = 0
+        for z in range(nz):
+            if self.zone_open[z]:
+                continue
+            Vz = self.zone_vol[z]
+            dI = f32(dint[z])
+            self.zoneDP[z] = gamma * self.zoneP[z] * dI / Vz
+            shift[z] = dI / Vz
+        if nz and not self.zone_open.all():
+            self.ctx._check(self.ctx.L.phg_zone_values(self.ctx.h, self.grid.id, shift.ctypes.data,
+                                                       self.d["S"].ptr, 1))
+        return shift[:nz]
+
+    def zone_pressure(self, dt):
+        """Closed zones' background pressure follows its rise rate (solver.cpp:258-263)."""
+        f32 = np.float32
+        if not len(self.zone_open) or self.zone_open.all():
+            return
+        for z in range(len(self.zone_open)):
+            if not self.zone_open[z]:
+                v = self.zoneP[z] + f32(dt) * self.zoneDP[z]
+                self.zoneP[z] = min(max(v, f32(101325.0 - 20000.0)), f32(101325.0 + 50000.0))
+        self.ctx._check(self.ctx.L.phg_zone_values(self.ctx.h, self.grid.id, self.zoneP.ctypes.data,
+                                                   self.d["Pb"].ptr, 0))
+
+    def stable_dt(self, dt_max):
+        out = C.c_float()
+        self.ctx._check(self.ctx.L.phg_stable_dt(dt_max, self.dt_max_profile, self.cfl, self.scen.dx, self.maxVel,
+                                                 self.maxNu, self.maxDiv, C.byref(out)))
+        return out.value
+
+    def step(self, dt_max):
+        """Solver::step (src/sim/solver.cpp:226-269) for a gas-only world (no
+        radiation, solids or barriers): returns dt."""
+        dt = self.stable_dt(dt_max)
+        self.step_clear()
+        self.inject_burners(dt)
+        self.combustion(dt)
+        self.turbulence()
+        self.heat_diffusion(dt)
+        self.energy_and_expansion(dt)
+        self.momentum(dt)
+        self.leakage_zones(dt)
+        self.project(dt)
+        self.advect_scalars(dt)
+        self.zone_pressure(dt)
+        self.time = float(np.float32(self.time) + np.float32(dt))
+        return dt
 
     def advect_scalars(self, dt):
         d = self.d

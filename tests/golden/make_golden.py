@@ -5,10 +5,14 @@ oracle/_ref/libphoton_ref_pin.so (oracle/Makefile, -ffp-contract=off).
 Run in a container that has the reference checkout:
     make -C oracle && python tests/golden/make_golden.py
 Each fixture is an .npz of inputs and the reference's outputs on a seeded
-grid: the multigrid hierarchy (level sizes, level-1 operator), one V-cycle
-z = M^-1 r (pressurePrecondition), project(dt) after a fixed number of CG
-iterations (pressure, face velocities, pressureIters, pressureResidual,
-maxVel), energyAndExpansion and advectScalars on seeded fields.
+grid: ref_<case>.npz the multigrid hierarchy (level sizes, level-1
+operator), one V-cycle z = M^-1 r (pressurePrecondition), project(dt) after a
+fixed number of CG iterations (pressure, face velocities, pressureIters,
+pressureResidual, maxVel), energyAndExpansion and advectScalars on seeded
+fields; ref_<case>_phases.npz turbulence, heatDiffusion, momentum, burner
+injection + combustion and stableDt on seeded fields; ref_<case>_step.npz a
+whole-step trajectory (Solver::step) from still air with a burner.
+    python tests/golden/make_golden.py [room walls wind_phases room_step ...]
 """
 import os
 import sys
@@ -108,9 +112,122 @@ def make(name, g, K=6):
     print(name, g.shape, "levels", out["level_n"], "project iters", it)
 
 
+STEP_CASES = {
+    # (grid, wind, burner rectangle x0 y0 x1 y1 at the floor, peak W, steps)
+    "room": (lambda: sc.room(), None, (1.8, 1.2, 2.2, 1.6), 100e3, 20),
+    "walls": (lambda: sc.house(7, 24, 16, 12), None, (2.0, 1.0, 2.6, 1.4), 20e3, 20),
+    "wind": (lambda: sc.room(window=True), (3.0, 30.0, 0.25), (1.8, 1.2, 2.2, 1.6), 100e3, 20),
+}
+STEP_DTMAX, STEP_TOL, STEP_MAXIT = 0.01, 0.02, 80
+SPECIES = ("Yo", "Yf", "Ys", "Ya", "Yp", "Yc", "Yw", "Yco", "Yai")
+
+
+def phase_inputs(g, seed):
+    rng = np.random.default_rng(seed)
+    u = (0.5 * rng.standard_normal(len(g.fx))).astype(np.float32) * g.fx
+    v = (0.5 * rng.standard_normal(len(g.fy))).astype(np.float32) * g.fy
+    w = (0.5 * rng.standard_normal(len(g.fz))).astype(np.float32) * g.fz
+    T = (293.15 + 900 * rng.random(g.n)).astype(np.float32)
+    Pb = (101325 + 30 * rng.standard_normal(g.n)).astype(np.float32)
+    p = rng.standard_normal(g.n).astype(np.float32)
+    Q0 = (10 * rng.random(g.n)).astype(np.float32)
+    sp = {}
+    for n, b in (("Yo", 0.232), ("Yf", 0.0), ("Ys", 0), ("Ya", 0), ("Yp", 0), ("Yc", 0.0006), ("Yw", 0.0073),
+                 ("Yco", 0), ("Yai", 0)):
+        sp[n] = (b + 0.02 * rng.random(g.n)).astype(np.float32)
+    sp["Yf"][rng.random(g.n) < 0.5] = 0
+    sp["Yai"] = (sp["Yf"] * (600 + 400 * rng.random(g.n))).astype(np.float32)
+    lit = (rng.random(g.n) < 0.1).astype(np.uint8)
+    pilot = (rng.random(g.n) < 0.05).astype(np.uint8)
+    return u.astype(np.float32), v.astype(np.float32), w.astype(np.float32), T, Pb, p, Q0, sp, lit, pilot
+
+
+def make_phases(name, g, K=0):
+    """turbulence, heatDiffusion, momentum, burner injection + combustion and
+    stableDt of the reference on seeded inputs."""
+    out = {}
+    R = po.RefSolver(g, threads=1, pin=True)
+    if name in STEP_CASES and STEP_CASES[name][1] is not None:
+        R.set_wind(*STEP_CASES[name][1])
+        out["wind"] = np.array(STEP_CASES[name][1], np.float32)
+    u, v, w, T, Pb, p, Q0, sp, lit, pilot = phase_inputs(g, 91)
+    for f, a in (("u", u), ("v", v), ("w", w), ("T", T), ("Pb", Pb), ("p", p)):
+        R.set(f, a)
+        out["in_" + f] = a
+    out["tu_maxnu"] = np.float32(R.turbulence())
+    out["tu_nu"] = nu = R.get("nu", g.n)
+    R.set("Q", Q0)
+    out["in_Q"] = Q0
+    R.heat_diffusion(DT)
+    out["hd_Q"] = R.get("Q", g.n)
+    R.momentum(DT)
+    out["mo_u"], out["mo_v"], out["mo_w"] = R.get("u", len(g.fx)), R.get("v", len(g.fy)), R.get("w", len(g.fz))
+    # burner gas and combustion on hot, partly fuelled gas
+    for n, a in sp.items():
+        R.set(n, a)
+        out["in_" + n] = a
+    R.set_bytes("lit", lit)
+    R.set_bytes("pilot", pilot)
+    out["in_lit"], out["in_pilot"] = lit, pilot
+    for n in ("Q", "Qr", "hrr", "massSrc"):
+        R.set(n, np.zeros(g.n, np.float32))
+    x0, y0, x1, y1 = STEP_CASES[name][2]
+    b, cells = R.add_burner(x0, y0, x1, y1, 0.0, 50e3, 0.0)
+    out["cb_cells"] = cells
+    R.inject_burners(DT)
+    R.combustion(DT)
+    for n in ("T",) + SPECIES:
+        out["cb_" + n] = R.get(n, g.n)
+    for n in ("Q", "Qr", "hrr", "massSrc"):
+        out["cb_" + n] = R.get(n, g.n)
+    out["cb_lit"], out["cb_pilot"] = R.get_bytes("lit", g.n), R.get_bytes("pilot", g.n)
+    out["sd_in"] = np.array([0.05, 1.7, 0.004, 3.0, 0.8, 0.05], np.float32)
+    out["sd_dt"] = np.float32(R.stable_dt(*[float(x) for x in out["sd_in"]]))
+    np.savez_compressed(os.path.join(HERE, f"ref_{name}_phases.npz"), **out)
+    print(name, "phases: maxNu", out["tu_maxnu"], "burner cells", len(cells))
+
+
+def make_step(name):
+    """A short whole-step trajectory of Solver::step from still air with a burner."""
+    gf, wind, rect, peak, nsteps = STEP_CASES[name]
+    g = gf()
+    R = po.RefSolver(g, threads=1, pin=True)
+    out = {}
+    if wind is not None:
+        R.set_wind(*wind)
+        out["wind"] = np.array(wind, np.float32)
+    b, cells = R.add_burner(*rect, 0.0, peak, 0.0)
+    out["burner"] = np.array(list(rect) + [peak], np.float32)
+    out["cells"] = cells
+    rows = []
+    for _ in range(nsteps):
+        dt = R.step(STEP_DTMAX, STEP_TOL, STEP_MAXIT)
+        sc_ = R.scalars()
+        zp, zdp = R.zone_pressure()
+        rows.append([dt, sc_["iters"], sc_["residual"], sc_["max_vel"], sc_["max_nu"], sc_["max_div"], sc_["hrr"],
+                     float(zp.min()), float(zp.max())])
+    out["st_rows"] = np.array(rows, np.float64)
+    out["st_cols"] = np.array(["dt", "iters", "residual", "maxVel", "maxNu", "maxDiv", "hrr", "zoneP_min",
+                               "zoneP_max"])
+    for f, n in (("u", len(g.fx)), ("v", len(g.fy)), ("w", len(g.fz))):
+        out["st_" + f] = R.get(f, n)
+    for f in ("p", "T", "nu", "Pb", "S") + SPECIES:
+        out["st_" + f] = R.get(f, g.n)
+    out["st_zoneP"], out["st_zoneDP"] = R.zone_pressure()
+    out["st_lit"] = R.get_bytes("lit", g.n)
+    np.savez_compressed(os.path.join(HERE, f"ref_{name}_step.npz"), **out)
+    print(name, "step:", nsteps, "steps, iters", out["st_rows"][:, 1].astype(int).tolist(), "T max",
+          float(out["st_T"].max()), "zoneP", out["st_zoneP"])
+
+
 if __name__ == "__main__":
     if not po.ref_available(pin=True):
         sys.exit("oracle/_ref/libphoton_ref_pin.so not built (needs the reference checkout): make -C oracle")
-    names = sys.argv[1:] or list(CASES)
+    names = sys.argv[1:] or list(CASES) + [n + "_phases" for n in STEP_CASES] + [n + "_step" for n in STEP_CASES]
     for name in names:
-        make(name, CASES[name]())
+        if name.endswith("_phases"):
+            make_phases(name[:-7], CASES[name[:-7]]())
+        elif name.endswith("_step"):
+            make_step(name[:-5])
+        else:
+            make(name, CASES[name]())

@@ -84,3 +84,114 @@ def test_transport_matches_oracle_bitwise(ctx, name):
     out = run_transport(ctx, g, u, v, w, Pb, nu, fields)
     for f, o, r in zip(gas.FIELD_NAMES, out, ref):
         assert bits_equal(o, r), (shape, f, int(np.sum(o != r)), float(np.max(np.abs(o - r))))
+
+
+# ---------------------------------------------------------------------------
+# the rest of the gas step against the reference's golden vectors
+# (tests/golden/ref_<case>_phases.npz, ref_<case>_step.npz)
+# ---------------------------------------------------------------------------
+SPECIES = ("Yo", "Yf", "Ys", "Ya", "Yp", "Yc", "Yw", "Yco", "Yai")
+
+
+def phases(name):
+    return np.load(os.path.join(HERE, "golden", f"ref_{name}_phases.npz"))
+
+
+def wind_args(G, g):
+    """(wind tuple for GasSolver, wind_closed flags) of a fixture."""
+    if "wind" not in G.files:
+        return None, None
+    speed, frm, ex = (float(x) for x in G["wind"])
+    wd = po.wind(speed, frm, ex, 0.0)
+    wc = np.array([0 if np.isnan(po.wind_at(wd, a, sd, 1.0)) else 1 for a in (0, 1) for sd in (0, 1)], np.uint8)
+    return (speed, frm, ex, 0.0), wc
+
+
+def rel(a, b):
+    a, b = np.asarray(a, np.float64), np.asarray(b, np.float64)
+    return float(np.linalg.norm(a - b) / max(np.linalg.norm(b), 1e-300))
+
+
+@pytest.mark.parametrize("name", list(GOLDEN))
+def test_turbulence_heat_momentum_match_reference(ctx, name):
+    g, G = GOLDEN[name](), phases(name)
+    wind, wc = wind_args(G, g)
+    s = gas.GasSolver(ctx, g, wind=wind, wind_closed=wc)
+    for f in ("u", "v", "w", "T", "Pb", "p"):
+        s.set(f, G["in_" + f])
+    max_nu = s.turbulence()
+    nu = s.get("nu")
+    # nu_mol = 1.5e-5 (T/T0)^1.7 uses the device pow: within 2 ulp of the host's
+    assert np.allclose(nu, G["tu_nu"], rtol=3e-7, atol=0), float(np.max(np.abs(nu / G["tu_nu"] - 1)))
+    assert abs(max_nu - float(G["tu_maxnu"])) <= 3e-7 * float(G["tu_maxnu"])
+    # conduction and momentum from the reference's nu, so the comparison is of these phases alone
+    s.set("nu", G["tu_nu"])
+    s.set("Q", G["in_Q"])
+    s.heat_diffusion(DT)
+    Q = s.get("Q")
+    assert np.allclose(Q, G["hd_Q"], rtol=2e-6, atol=1e-6), float(np.max(np.abs(Q - G["hd_Q"])))
+    s.momentum(DT)
+    for f in ("u", "v", "w"):
+        assert bits_equal(s.get(f), G["mo_" + f]), (f, int(np.sum(s.get(f) != G["mo_" + f])))
+    s.grid.destroy()
+
+
+@pytest.mark.parametrize("name", list(GOLDEN))
+def test_burner_and_combustion_match_reference_bitwise(ctx, name):
+    g, G = GOLDEN[name](), phases(name)
+    s = gas.GasSolver(ctx, g)
+    for f in ("T", "Pb") + SPECIES:
+        s.set(f, G["in_" + f])
+    s.set("lit", G["in_lit"])
+    s.set("pilot", G["in_pilot"])
+    cells = G["cb_cells"]
+    f32 = np.float32
+    mf = f32(50e3) * f32(DT) / f32(25e6) / f32(len(cells))
+    dc = ctx.to_device(cells.astype(np.int32))
+    ctx._check(ctx.L.phg_add_gas(ctx.h, s.grid.id, len(cells), dc.ptr, mf, 0.0, mf * f32(0.01), 800.0, 1,
+                                 gas.C.byref(s._gas())))
+    total = s.combustion(DT)
+    for f in ("T",) + SPECIES + ("Q", "Qr", "hrr", "massSrc", "lit", "pilot"):
+        assert bits_equal(s.get(f), G["cb_" + f]) if s.get(f).dtype == np.float32 else \
+            np.array_equal(s.get(f), G["cb_" + f]), f
+    V = np.float32(g.dx) * np.float32(g.dx) * np.float32(g.dx)
+    expect = float(np.sum((G["cb_hrr"] * V).astype(np.float64)))   # FP64 sum of the FP32 cell terms
+    assert abs(total - expect) <= 1e-12 * expect
+    dt_max, max_vel, max_nu, max_div, cfl, pdt = (float(x) for x in G["sd_in"])
+    out = gas.C.c_float()
+    ctx._check(ctx.L.phg_stable_dt(dt_max, pdt, cfl, g.dx, max_vel, max_nu, max_div, gas.C.byref(out)))
+    assert np.float32(out.value) == G["sd_dt"]
+    s.grid.destroy()
+
+
+@pytest.mark.parametrize("name", list(GOLDEN))
+def test_whole_step_follows_reference_trajectory(ctx, name):
+    """Solver::step from still air with a burner, 20 steps at dtMax 0.01 s
+    under the reference's realtime stop rule (max-norm 0.02 1/s, 80
+    iterations) and its coarsest sweeps, against the reference's own run.
+    Every step's dt and pressureIters must be the reference's; the fields
+    after 20 steps within 1e-5 relative L2 (T and species 1e-6), the pressure
+    within 1e-4 of its scale, zone pressures within 1e-4 Pa."""
+    g, G = GOLDEN[name](), np.load(os.path.join(HERE, "golden", f"ref_{name}_step.npz"))
+    wind, wc = wind_args(G, g)
+    s = gas.GasSolver(ctx, g, wind=wind, wind_closed=wc, tol=0.02, max_iters=80, stop_rule=gas.STOP_MAXNORM,
+                      coarse=gas.COARSE_GS20)
+    x0, y0, x1, y1, peak = (float(x) for x in G["burner"])
+    b = s.add_burner(x0, y0, x1, y1, 0.0, peak)
+    assert b == 0 and np.array_equal(s.burners[0]["cells"], G["cells"])
+    rows = G["st_rows"]
+    for k in range(len(rows)):
+        dt = s.step(0.01)
+        assert np.float32(dt) == np.float32(rows[k, 0]), (k, dt, rows[k, 0])
+        assert s.pressureIters == int(rows[k, 1]), (k, s.pressureIters, int(rows[k, 1]))
+    print(f"{name}: max|dT| {np.abs(s.get('T') - G['st_T']).max():.3g} K, u {rel(s.get('u'), G['st_u']):.2e}, "
+          f"p {np.abs(s.get('p') - G['st_p']).max() / np.abs(G['st_p']).max():.2e}, zoneP {s.zoneP} / {G['st_zoneP']}")
+    for f in ("u", "v", "w"):
+        assert rel(s.get(f), G["st_" + f]) < 1e-5, f
+    for f in ("T",) + SPECIES:
+        assert rel(s.get(f), G["st_" + f]) < 1e-6, f
+    assert np.abs(s.get("p") - G["st_p"]).max() <= 1e-4 * np.abs(G["st_p"]).max()
+    assert rel(s.get("nu"), G["st_nu"]) < 1e-5
+    assert np.abs(s.zoneP - G["st_zoneP"]).max() <= 1e-4 * 101325
+    assert np.array_equal(s.get("lit"), G["st_lit"])
+    s.grid.destroy()

@@ -1,0 +1,145 @@
+"""CPU tests of the oracle's restatement of the rest of the gas step
+(oracle/oracle_step.c): eddy viscosity, heat conduction, momentum with wind,
+burner gas and combustion, stableDt.
+
+Pins: bit-exact agreement with the golden vectors written by the reference's
+own Solver (tests/golden/ref_<case>_phases.npz, room / walled house / windy
+room), and properties the physics fixes independently of any restatement:
+still air stays still, a uniform flow has no sub-grid energy, conduction in a
+closed box conserves heat, burning conserves element mass shares."""
+import os
+
+import numpy as np
+import pytest
+
+from oracle import pyoracle as po
+from photon_amd import scenarios as sc
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+DT = 0.01
+CASES = {"room": sc.room, "walls": lambda: sc.house(7, 24, 16, 12), "wind": lambda: sc.room(window=True)}
+SPECIES = ("Yo", "Yf", "Ys", "Ya", "Yp", "Yc", "Yw", "Yco", "Yai")
+
+
+def golden(name):
+    return np.load(os.path.join(HERE, "golden", f"ref_{name}_phases.npz"))
+
+
+def bits(a, b):
+    a, b = np.asarray(a), np.asarray(b)
+    if a.dtype == np.float32:
+        return np.array_equal(a.view(np.uint32), np.asarray(b, np.float32).view(np.uint32))
+    return np.array_equal(a, b)
+
+
+@pytest.fixture(scope="module", params=list(CASES))
+def case(request):
+    return request.param, CASES[request.param](), golden(request.param)
+
+
+def wind_of(G):
+    return po.wind(*G["wind"].tolist()) if "wind" in G.files else None
+
+
+def test_turbulence_matches_golden(case):
+    _, g, G = case
+    nu, m = po.turbulence(g, G["in_u"], G["in_v"], G["in_w"], G["in_T"])
+    assert bits(nu, G["tu_nu"]) and np.float32(m) == G["tu_maxnu"]
+
+
+def test_heat_diffusion_matches_golden(case):
+    _, g, G = case
+    Q = G["in_Q"].copy()
+    po.heat_diffusion(g, DT, G["in_T"], G["tu_nu"], G["in_Pb"], Q)
+    assert bits(Q, G["hd_Q"])
+
+
+def test_momentum_matches_golden(case):
+    _, g, G = case
+    u, v, w = po.momentum(g, DT, G["in_u"], G["in_v"], G["in_w"], G["in_T"], G["in_Pb"], G["tu_nu"], G["in_p"],
+                          wind_of(G))
+    assert bits(u, G["mo_u"]) and bits(v, G["mo_v"]) and bits(w, G["mo_w"])
+
+
+def test_burner_and_combustion_match_golden(case):
+    _, g, G = case
+    f = {n: G["in_" + n].copy() for n in ("T", "Pb") + SPECIES}
+    f["massSrc"] = np.zeros(g.n, np.float32)
+    Q, Qr, hrr = (np.zeros(g.n, np.float32) for _ in range(3))
+    lit, pilot = G["in_lit"].copy(), G["in_pilot"].copy()
+    cells = G["cb_cells"]
+    f32 = np.float32
+    mf = f32(50e3) * f32(DT) / f32(25e6) / f32(len(cells))
+    po.add_gas(g, cells, mf, 0.0, mf * f32(0.01), 800.0, True, f, pilot)
+    po.combustion(g, DT, False, f, Q, Qr, hrr, pilot, lit)
+    for n in ("T",) + SPECIES:
+        assert bits(f[n], G["cb_" + n]), n
+    for n, a in (("Q", Q), ("Qr", Qr), ("hrr", hrr), ("massSrc", f["massSrc"])):
+        assert bits(a, G["cb_" + n]), n
+    assert bits(lit, G["cb_lit"]) and bits(pilot, G["cb_pilot"])
+    assert (G["cb_hrr"] > 0).sum() > 10   # the fixture burns
+
+
+def test_stable_dt_matches_golden(case):
+    _, g, G = case
+    dt_max, max_vel, max_nu, max_div, cfl, pdt = (float(x) for x in G["sd_in"])
+    assert np.float32(po.stable_dt(dt_max, pdt, cfl, g.dx, max_vel, max_nu, max_div)) == G["sd_dt"]
+
+
+# ----------------------------------------------------------- physics pins --
+def test_still_air_stays_still():
+    # uniform T and Pb, zero velocity and pressure: no force anywhere (the
+    # reference's still_air_stays_still, test_main.cpp:437-445, for one step)
+    g = sc.house(3, 20, 14, 10, n_walls=4, n_open=2)
+    z = [np.zeros(len(a), np.float32) for a in (g.fx, g.fy, g.fz)]
+    T = np.full(g.n, 293.15, np.float32)
+    Pb = np.full(g.n, 101325.0, np.float32)
+    nu, m = po.turbulence(g, *z, T)
+    u, v, w = po.momentum(g, DT, *z, T, Pb, nu, np.zeros(g.n, np.float32))
+    assert not u.any() and not v.any() and not w.any()
+    # nu is the molecular viscosity of air at T0 in every gas cell
+    assert np.allclose(nu[g.blocked == 0], 1.5e-5, rtol=1e-6) and np.all(nu[g.blocked == 1] == 0)
+
+
+def test_uniform_flow_has_no_subgrid_energy():
+    g = sc.room()
+    u = np.full(len(g.fx), 0.7, np.float32)
+    v = np.full(len(g.fy), -0.3, np.float32)
+    w = np.full(len(g.fz), 0.2, np.float32)
+    nu, m = po.turbulence(g, u, v, w, np.full(g.n, 400.0, np.float32))
+    assert np.allclose(nu, 1.5e-5 * (400.0 / 293.15) ** 1.7, rtol=1e-5)
+
+
+def test_conduction_in_a_closed_box_conserves_heat():
+    g = sc.house(5, 16, 12, 8, n_walls=3, n_open=1)    # closed domain
+    rng = np.random.default_rng(3)
+    T = (293.15 + 500 * rng.random(g.n)).astype(np.float32)
+    nu = (1e-3 * rng.random(g.n)).astype(np.float32)
+    Pb = np.full(g.n, 101325.0, np.float32)
+    Q = np.zeros(g.n, np.float32)
+    po.heat_diffusion(g, DT, T, nu, Pb, Q)
+    assert abs(Q.astype(np.float64).sum()) <= 1e-5 * np.abs(Q).astype(np.float64).sum()
+    assert np.abs(Q).sum() > 0
+
+
+def test_burning_conserves_element_shares():
+    # one cell's burn: fuel lost equals CO2/soot/CO carbon plus O2 balance
+    g = sc.room()
+    f = {n: np.zeros(g.n, np.float32) for n in ("T", "Pb") + SPECIES + ("massSrc",)}
+    f["T"][:] = 1200.0
+    f["Pb"][:] = 101325.0
+    f["Yo"][:] = 0.2
+    f["Yf"][:] = 0.05
+    f["Yai"][:] = 0.05 * 800.0
+    Q, Qr, hrr = (np.zeros(g.n, np.float32) for _ in range(3))
+    lit = np.zeros(g.n, np.uint8)
+    pilot = np.zeros(g.n, np.uint8)
+    y0 = {n: f[n].astype(np.float64).copy() for n in SPECIES}
+    tot = po.combustion(g, DT, False, f, Q, Qr, hrr, pilot, lit)
+    dfuel = y0["Yf"] - f["Yf"]
+    frac = 1 - np.exp(-DT / 0.1)
+    assert np.allclose(dfuel, 0.05 * frac, rtol=1e-5)             # mixing-limited share of the fuel
+    carbon = (f["Yc"] - y0["Yc"]) * 12 / 44 + (f["Yco"] - y0["Yco"]) * 12 / 28 + (f["Ys"] - y0["Ys"])
+    assert np.allclose(carbon, 0.597 * dfuel, rtol=1e-4)
+    assert np.allclose(Q, dfuel * 25e6 * (101325.0 / (287.0 * 1200.0)) * 0.2 ** 3, rtol=1e-4)
+    assert tot > 0 and lit.all()
