@@ -35,13 +35,32 @@ def hipcc() -> str:
     return "hipcc"
 
 
-def build(verbose: bool = True) -> str:
-    cmd = [hipcc(), *FLAGS, "-I", os.path.join(HERE, "..", "include"), "-o", OUT + ".tmp",
-           *[os.path.join(CSRC, s) for s in SOURCES]]
+# the C++ host class over the C ABI (no HIP headers: plain g++)
+HOST_SRC = os.path.join(HERE, "host", "gas_phase.cpp")
+HOST_OUT = os.path.join(HERE, "libphoton_gasphase.so")
+# a C++ caller of the host class, run by the GPU tests
+CHECK_SRC = os.path.join(HERE, "..", "tests", "cpp", "gas_step_check.cpp")
+CHECK_OUT = os.path.join(HERE, "..", "tests", "cpp", "gas_step_check")
+CXX = ["g++", "-std=c++20", "-O2", "-Wall", "-Wno-unused-result"]
+
+
+def _run(cmd, verbose):
     if verbose:
         print(" ".join(cmd), flush=True)
     subprocess.run(cmd, check=True)
+
+
+def build(verbose: bool = True) -> str:
+    cmd = [hipcc(), *FLAGS, "-I", os.path.join(HERE, "..", "include"), "-o", OUT + ".tmp",
+           *[os.path.join(CSRC, s) for s in SOURCES]]
+    _run(cmd, verbose)
     os.replace(OUT + ".tmp", OUT)
+    _run([*CXX, "-fPIC", "-shared", "-o", HOST_OUT + ".tmp", HOST_SRC, "-L" + HERE, "-lphoton_gas",
+          "-Wl,-rpath,$ORIGIN"], verbose)
+    os.replace(HOST_OUT + ".tmp", HOST_OUT)
+    _run([*CXX, "-o", CHECK_OUT + ".tmp", CHECK_SRC, "-L" + HERE, "-lphoton_gasphase", "-lphoton_gas",
+          "-Wl,-rpath,$ORIGIN/../../photon_amd"], verbose)
+    os.replace(CHECK_OUT + ".tmp", CHECK_OUT)
     return OUT
 
 

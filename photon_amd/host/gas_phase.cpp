@@ -1,0 +1,333 @@
+// gas_phase.cpp -- photon_gpu::GasPhase over the C ABI of libphoton_gas.so.
+// Each member names the reference function it stands for.
+#include "gas_phase.hpp"
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+
+namespace photon_gpu {
+
+namespace {
+constexpr float T0 = 293.15f, P0 = 101325.0f, RS = 287.0f, CP = 1100.0f;
+constexpr float GAMMA = CP / (CP - RS);   // solver.h:175
+constexpr float DHC_FUEL = 25.0e6f;       // solver.h:165
+constexpr float BURNER_SOOT = 0.01f;      // gasphase.cpp:93
+constexpr float T_AUTO = 800.0f;          // solver.h:179
+}  // namespace
+
+float Burner::powerAt(float t) const {   // solver.h:121-125
+    if (t < start || t > start + duration) return 0;
+    float r = rampS > 0 ? std::min(1.0f, (t - start) / rampS) : 1.0f;
+    return peakW * r * r;
+}
+
+struct GasPhase::Copy {
+    void* dev;
+    void* host;
+    size_t bytes;
+};
+
+GasPhase::~GasPhase() {
+    if (ctx_) {
+        for (void* p : dmem_) phg_free(ctx_, p);
+        if (grid_ >= 0) phg_grid_destroy(ctx_, grid_);
+        phg_destroy(ctx_);
+    }
+}
+
+bool GasPhase::check(int rc, const char* what) {
+    if (rc == PHG_OK) return true;
+    if (err_.empty()) err_ = std::string(what) + ": " + (ctx_ ? phg_last_error(ctx_) : "no context");
+    return false;
+}
+
+template <class T>
+T* GasPhase::alloc(size_t n) {
+    void* p = nullptr;
+    if (!check(phg_malloc(ctx_, &p, std::max<size_t>(n, 1) * sizeof(T)), "phg_malloc")) return nullptr;
+    dmem_.push_back(p);
+    return static_cast<T*>(p);
+}
+
+phg_gas GasPhase::gas() const {
+    phg_gas g;
+    g.T = dF_[0];
+    g.Yo = dF_[1];
+    g.Yf = dF_[2];
+    g.Ys = dF_[3];
+    g.Ya = dF_[4];
+    g.Yp = dF_[5];
+    g.Yc = dF_[6];
+    g.Yw = dF_[7];
+    g.Yco = dF_[8];
+    g.Yai = dF_[9];
+    g.Pb = dPb_;
+    g.massSrc = dMs_;
+    g.Q = dQ_;
+    g.Qr = dQr_;
+    g.hrr = dHrr_;
+    g.pilot = dPilot_;
+    g.lit = dLit_;
+    return g;
+}
+
+// the host vectors and their device copies, for syncToHost / syncToDevice
+std::vector<GasPhase::Copy> GasPhase::copies() const {
+    auto* self = const_cast<GasPhase*>(this);
+    std::vector<float>* F[10] = {&self->T, &self->Yo, &self->Yf, &self->Ys, &self->Ya,
+                                 &self->Yp, &self->Yc, &self->Yw, &self->Yco, &self->Yai};
+    std::vector<Copy> c = {{dU_, self->u.data(), u.size() * 4},   {dV_, self->v.data(), v.size() * 4},
+                           {dW_, self->w.data(), w.size() * 4},   {dP_, self->p.data(), p.size() * 4},
+                           {dS_, self->S.data(), S.size() * 4},   {dNu_, self->nu.data(), nu.size() * 4},
+                           {dPb_, self->Pb.data(), Pb.size() * 4}, {dQ_, self->Q.data(), Q.size() * 4},
+                           {dQr_, self->Qr.data(), Qr.size() * 4}, {dMs_, self->massSrc.data(), massSrc.size() * 4},
+                           {dHrr_, self->hrr.data(), hrr.size() * 4}, {dPilot_, self->pilot.data(), pilot.size()},
+                           {dLit_, self->lit.data(), lit.size()}};
+    for (int q = 0; q < 10; ++q) c.push_back({dF_[q], F[q]->data(), F[q]->size() * 4});
+    return c;
+}
+
+void GasPhase::syncToHost() {
+    for (const Copy& c : copies()) check(phg_memcpy_d2h(ctx_, c.host, c.dev, c.bytes), "syncToHost");
+}
+void GasPhase::syncToDevice() {
+    for (const Copy& c : copies()) check(phg_memcpy_h2d(ctx_, c.dev, c.host, c.bytes), "syncToDevice");
+}
+
+bool GasPhase::build(int nx_, int ny_, int nz_, float dx_, const std::vector<float>& fx_,
+                     const std::vector<float>& fy_, const std::vector<float>& fz_, const std::vector<float>& ax,
+                     const std::vector<float>& ay, const std::vector<float>& az, const std::vector<uint8_t>& blocked_,
+                     std::string& err, int device, bool refCoarse, int precision) {
+    nx = nx_;
+    ny = ny_;
+    nz = nz_;
+    N = nx * ny * nz;
+    dx = dx_;
+    const size_t nfx = (size_t)(nx + 1) * ny * nz, nfy = (size_t)nx * (ny + 1) * nz, nfz = (size_t)nx * ny * (nz + 1);
+    if (fx_.size() != nfx || fy_.size() != nfy || fz_.size() != nfz || ax.size() != nfx || ay.size() != nfy ||
+        az.size() != nfz || blocked_.size() != (size_t)N) {
+        err = "geometry arrays do not match the grid";
+        return false;
+    }
+    fx = fx_;
+    fy = fy_;
+    fz = fz_;
+    blocked = blocked_;
+    if (phg_create(&ctx_, device) != PHG_OK) {
+        err = "phg_create failed: no usable GPU";
+        return false;
+    }
+    // sides where windAt prescribes inflow close for the pressure (gasphase.cpp:510)
+    uint8_t wc[4] = {0, 0, 0, 0};
+    if (wind.speed > 0) {
+        const float th = wind.fromDeg * 3.14159265f / 180.0f;
+        const float sx = -std::sin(th), sy = -std::cos(th);
+        wc[0] = sx > 0;
+        wc[1] = sx < 0;
+        wc[2] = sy > 0;
+        wc[3] = sy < 0;
+    }
+    if (!check(phg_grid_create(ctx_, nx, ny, nz, dx, fx.data(), fy.data(), fz.data(), ax.data(), ay.data(), az.data(),
+                               blocked.data(), wind.speed > 0 ? wc : nullptr, precision, &grid_),
+               "phg_grid_create") ||
+        (refCoarse && !check(phg_grid_set_coarse(ctx_, grid_, PHG_COARSE_GS20), "phg_grid_set_coarse"))) {
+        err = err_;
+        return false;
+    }
+    // zones (computeZones, gasphase.cpp:111-158)
+    int nl = 0, nzn = 0, rl = 0;
+    int lvn[16];
+    check(phg_grid_info(ctx_, grid_, &nl, lvn, &nzn, &rl), "phg_grid_info");
+    zone.assign(N, -1);
+    zoneOpen.assign(std::max(nzn, 1), 0);
+    check(phg_grid_zones(ctx_, grid_, zone.data(), zoneOpen.data()), "phg_grid_zones");
+    zoneOpen.resize(nzn);
+    zoneP.assign(nzn, P0);
+    zoneDP.assign(nzn, 0.0f);
+    zoneVol_.assign(nzn, 0.0);
+    for (int c = 0; c < N; ++c)
+        if (zone[c] >= 0) zoneVol_[zone[c]] += 1;
+    for (double& v : zoneVol_) v = v * dx * dx * dx;
+    // state at ambient, still air (as Solver::build leaves the gas)
+    u.assign(nfx, 0.0f);
+    v.assign(nfy, 0.0f);
+    w.assign(nfz, 0.0f);
+    for (auto* f : {&p, &S, &nu, &Q, &Qr, &massSrc, &hrr, &Yf, &Ys, &Ya, &Yp, &Yco, &Yai}) f->assign(N, 0.0f);
+    T.assign(N, T0);
+    Yo.assign(N, 0.232f);
+    Yc.assign(N, 0.0006f);
+    Yw.assign(N, 0.0073f);
+    Pb.assign(N, P0);
+    pilot.assign(N, 0);
+    lit.assign(N, 0);
+    dU_ = alloc<float>(nfx);
+    dV_ = alloc<float>(nfy);
+    dW_ = alloc<float>(nfz);
+    dU2_ = alloc<float>(nfx);
+    dV2_ = alloc<float>(nfy);
+    dW2_ = alloc<float>(nfz);
+    for (float** d : {&dP_, &dS_, &dNu_, &dPb_, &dQ_, &dQr_, &dMs_, &dHrr_}) *d = alloc<float>(N);
+    for (int q = 0; q < 10; ++q) {
+        dF_[q] = alloc<float>(N);
+        dFt_[q] = alloc<float>(N);
+    }
+    dPilot_ = alloc<uint8_t>(N);
+    dLit_ = alloc<uint8_t>(N);
+    if (!err_.empty()) {
+        err = err_;
+        return false;
+    }
+    syncToDevice();
+    err = err_;
+    return err_.empty();
+}
+
+int GasPhase::addBurner(const Burner& b0) {   // gasphase.cpp:72-90
+    Burner b = b0;
+    b.cells.clear();
+    auto cellAt = [&](float x, float y, float z) {
+        int i = (int)std::floor((x - ox) / dx), j = (int)std::floor((y - oy) / dx), k = (int)std::floor((z - oz) / dx);
+        if (i < 0 || j < 0 || k < 0 || i >= nx || j >= ny || k >= nz) return -1;
+        return i + nx * (j + ny * k);
+    };
+    for (int j = 0; j < ny; ++j)
+        for (int i = 0; i < nx; ++i) {
+            float x = ox + (i + 0.5f) * dx, y = oy + (j + 0.5f) * dx;
+            if (x < b.x0 || x > b.x1 || y < b.y0 || y > b.y1) continue;
+            int c = cellAt(x, y, b.z + 0.5f * dx);
+            if (c >= 0 && !blocked[c]) b.cells.push_back(c);
+        }
+    if (b.cells.empty()) {
+        int c = cellAt(0.5f * (b.x0 + b.x1), 0.5f * (b.y0 + b.y1), b.z + 0.5f * dx);
+        if (c < 0 || blocked[c]) return -1;
+        b.cells.push_back(c);
+    }
+    int* d = alloc<int>(b.cells.size());
+    if (!d || !check(phg_memcpy_h2d(ctx_, d, b.cells.data(), b.cells.size() * sizeof(int)), "burner cells")) return -1;
+    dBurnerCells_.push_back(d);
+    burners.push_back(b);
+    return (int)burners.size() - 1;
+}
+
+float GasPhase::stableDt(float dtMax) const {
+    float dt = dtMax;
+    phg_stable_dt(dtMax, profile.dtMax, profile.cfl, dx, maxVel, maxNu, maxDiv, &dt);
+    return dt;
+}
+
+void GasPhase::stepClear() {
+    const phg_gas g = gas();
+    check(phg_step_clear(ctx_, grid_, &g), "phg_step_clear");
+}
+
+void GasPhase::injectBurners(float dt) {   // gasphase.cpp:92-105
+    const phg_gas g = gas();
+    for (size_t bi = 0; bi < burners.size(); ++bi) {
+        const Burner& b = burners[bi];
+        float P = b.powerAt(time);
+        if (P <= 0) continue;
+        float mf = P * dt / DHC_FUEL / b.cells.size();
+        check(phg_add_gas(ctx_, grid_, (int)b.cells.size(), dBurnerCells_[bi], mf, 0.0f, mf * BURNER_SOOT, T_AUTO, 1,
+                          &g),
+              "phg_add_gas");
+    }
+}
+
+void GasPhase::combustion(float dt) {
+    const phg_gas g = gas();
+    check(phg_combustion(ctx_, grid_, dt, radiationOn ? 1 : 0, &g, &hrrTotal), "phg_combustion");
+}
+
+void GasPhase::turbulence() { check(phg_turbulence(ctx_, grid_, dU_, dV_, dW_, dF_[0], dNu_, &maxNu), "phg_turbulence"); }
+
+void GasPhase::heatDiffusion(float dt) {
+    check(phg_heat_diffusion(ctx_, grid_, dt, dF_[0], dNu_, dPb_, dQ_), "phg_heat_diffusion");
+}
+
+void GasPhase::energyAndExpansion(float dt) {
+    check(phg_energy_expansion(ctx_, grid_, dt, dF_[0], dS_, dPb_, dQ_, dMs_, &maxDiv), "phg_energy_expansion");
+}
+
+void GasPhase::momentum(float dt) {
+    const phg_wind wd{wind.speed, wind.fromDeg, wind.exponent, oz};
+    if (check(phg_momentum(ctx_, grid_, dt, &wd, dU_, dV_, dW_, dF_[0], dPb_, dNu_, dP_, dU2_, dV2_, dW2_),
+              "phg_momentum")) {
+        std::swap(dU_, dU2_);
+        std::swap(dV_, dV2_);
+        std::swap(dW_, dW2_);
+    }
+}
+
+void GasPhase::leakage(float dt) {   // gasphase.cpp:402-498, grids without crack faces
+    (void)dt;
+    const size_t nZ = zoneP.size();
+    std::vector<double> dInt(std::max<size_t>(nZ, 1), 0.0);
+    if (!check(phg_zone_integrals(ctx_, grid_, dS_, dInt.data()), "phg_zone_integrals")) return;
+    std::vector<float> shift(std::max<size_t>(nZ, 1), 0.0f);
+    bool any = false;
+    for (size_t z = 0; z < nZ; ++z) {
+        zoneDP[z] = 0.0f;
+        if (zoneOpen[z]) continue;
+        const float Vz = (float)zoneVol_[z];
+        zoneDP[z] = GAMMA * zoneP[z] * (float)dInt[z] / Vz;
+        shift[z] = (float)dInt[z] / Vz;
+        any = true;
+    }
+    if (any) check(phg_zone_values(ctx_, grid_, shift.data(), dS_, 1), "phg_zone_values");
+}
+
+void GasPhase::project(float dt) {   // gasphase.cpp:505-677
+    phg_solve_params prm{};
+    prm.tol = profile.pressureTol;
+    prm.max_iters = profile.pressureMaxIters;
+    prm.stop_rule = PHG_STOP_MAXNORM;
+    prm.demean = 1;
+    // the set-up and iterations as one graph sized from the last step's count
+    prm.use_graph = 2;
+    prm.check_every = pressureIters > 0 ? (pressureIters + 4) / 4 * 4 : 8;
+    phg_solve_stats st{};
+    float vmax = 0;
+    if (!check(phg_project(ctx_, grid_, dt, dU_, dV_, dW_, dP_, dS_, &prm, &st, &vmax), "phg_project")) {
+        pressureIters = -1;   // step() has no error path: flag it
+        pressureResidual = NAN;
+        return;
+    }
+    pressureIters = st.iters;
+    pressureResidual = (float)st.maxres;
+    maxVel = vmax;
+}
+
+void GasPhase::advectScalars(float dt) {   // gasphase.cpp:685-784; tmp_ swapped in
+    if (check(phg_advect_scalars(ctx_, grid_, dt, dU_, dV_, dW_, dPb_, dNu_, dF_, dFt_), "phg_advect_scalars"))
+        for (int q = 0; q < 10; ++q) std::swap(dF_[q], dFt_[q]);
+}
+
+void GasPhase::zonePressure(float dt) {   // solver.cpp:258-263
+    bool any = false;
+    for (size_t z = 0; z < zoneP.size(); ++z)
+        if (!zoneOpen[z]) {
+            zoneP[z] = std::clamp(zoneP[z] + dt * zoneDP[z], P0 - 20000.0f, P0 + 50000.0f);
+            any = true;
+        }
+    if (any) check(phg_zone_values(ctx_, grid_, zoneP.data(), dPb_, 0), "phg_zone_values");
+}
+
+float GasPhase::step(float dtMax) {   // solver.cpp:226-269, gas-only
+    float dt = stableDt(dtMax);
+    stepClear();
+    injectBurners(dt);
+    combustion(dt);
+    turbulence();
+    heatDiffusion(dt);
+    energyAndExpansion(dt);
+    momentum(dt);
+    leakage(dt);
+    project(dt);
+    advectScalars(dt);
+    zonePressure(dt);
+    time += dt;
+    return dt;
+}
+
+}  // namespace photon_gpu

@@ -1,0 +1,109 @@
+"""The compiled C++ drop-in on the GPU: tests/cpp/gas_step_check (built by
+__graft_entry__.build() with g++) drives photon_gpu::GasPhase, the
+Solver-shaped host class over the C ABI, and its results are compared with
+the fixtures the reference's own Solver wrote.
+
+phases: project after the fixture's K iterations under the max-norm rule
+with the reference's coarsest sweeps (pressureIters, pressureResidual,
+maxVel, p, u, v, w), energyAndExpansion (T, S, maxDiv) and advectScalars (ten
+fields, bit for bit).  step: 20 steps of Solver::step from still air with a
+burner: every step's dt and pressureIters, then the fields."""
+import os
+import subprocess
+
+import numpy as np
+import pytest
+
+from photon_amd import scenarios as sc
+
+pytestmark = pytest.mark.gpu
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+EXE = os.path.join(HERE, "cpp", "gas_step_check")
+CASES = {"room": sc.room, "walls": lambda: sc.house(7, 24, 16, 12), "wind": lambda: sc.room(window=True)}
+SPECIES = ("Yo", "Yf", "Ys", "Ya", "Yp", "Yc", "Yw", "Yco", "Yai")
+
+
+def rel(a, b):
+    a, b = np.asarray(a, np.float64), np.asarray(b, np.float64)
+    return float(np.linalg.norm(a - b) / max(np.linalg.norm(b), 1e-300))
+
+
+def write_case(d, g, meta, arrays):
+    os.makedirs(d, exist_ok=True)
+    base = dict(nx=g.nx, ny=g.ny, nz=g.nz, dx=g.dx)
+    base.update(meta)
+    with open(os.path.join(d, "meta.txt"), "w") as f:
+        for k, v in base.items():
+            f.write(f"{k}={v}\n")
+    geo = dict(fx=g.fx, fy=g.fy, fz=g.fz, ax=g.ax, ay=g.ay, az=g.az)
+    for k, a in geo.items():
+        np.ascontiguousarray(a, np.float32).tofile(os.path.join(d, k + ".bin"))
+    np.ascontiguousarray(g.blocked, np.uint8).tofile(os.path.join(d, "blocked.bin"))
+    for k, a in arrays.items():
+        np.ascontiguousarray(a).tofile(os.path.join(d, k + ".bin"))
+
+
+def run(mode, d):
+    assert os.path.exists(EXE), "tests/cpp/gas_step_check not built: run __graft_entry__.build()"
+    print(f"gas_step_check {mode} {d}", flush=True)
+    r = subprocess.run([EXE, mode, d], capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stderr
+    with open(os.path.join(d, "out_log.txt")) as f:
+        return f.read().split("\n")
+
+
+def out(d, name, dtype=np.float32):
+    return np.fromfile(os.path.join(d, f"out_{name}.bin"), dtype)
+
+
+def wind_meta(G):
+    if "wind" not in G.files:
+        return {}
+    s, fr, ex = (float(x) for x in G["wind"])
+    return dict(wind_speed=s, wind_from=fr, wind_exponent=ex)
+
+
+@pytest.mark.parametrize("name", list(CASES))
+def test_cpp_dropin_phases_match_reference(tmp_path, name):
+    g = CASES[name]()
+    G = np.load(os.path.join(HERE, "golden", f"ref_{name}.npz"))
+    d = str(tmp_path / name)
+    arrays = {k: G[k] for k in ("pj_u0", "pj_v0", "pj_w0", "pj_S", "en_T0", "en_Q", "en_ms", "ad_nu", "pj_u", "pj_v",
+                                "pj_w")}
+    arrays.update({"ad_in_" + n: G["ad_in_" + n] for n in SPECIES})
+    write_case(d, g, dict(dt=0.01, pj_K=int(G["pj_K"]), **wind_meta(G)), arrays)
+    log = run("phases", d)
+    it, res, vmax = log[0].split()[1:]
+    assert int(it) == int(G["pj_iters"])
+    assert abs(float(res) - float(G["pj_residual"])) <= 1e-4 * float(G["pj_residual"])
+    assert abs(float(vmax) - float(G["pj_maxvel"])) <= 1e-4 * max(float(G["pj_maxvel"]), 1e-30)
+    for f in ("p", "u", "v", "w"):
+        assert rel(out(d, f), G["pj_" + f]) < 1e-5, f
+    assert np.array_equal(out(d, "en_T"), G["en_T"]) and np.array_equal(out(d, "en_S"), G["en_S"])
+    assert np.float32(float(log[1].split()[1])) == G["en_maxdiv"]
+    assert np.array_equal(out(d, "ad_T").view(np.uint32), G["ad_out_T"].view(np.uint32))
+    for n in SPECIES:
+        assert np.array_equal(out(d, "ad_" + n).view(np.uint32), G["ad_out_" + n].view(np.uint32)), n
+
+
+@pytest.mark.parametrize("name", list(CASES))
+def test_cpp_dropin_step_follows_reference(tmp_path, name):
+    g = CASES[name]()
+    G = np.load(os.path.join(HERE, "golden", f"ref_{name}_step.npz"))
+    x0, y0, x1, y1, peak = (float(x) for x in G["burner"])
+    rows = G["st_rows"]
+    d = str(tmp_path / name)
+    write_case(d, g, dict(bx0=x0, by0=y0, bx1=x1, by1=y1, peak=peak, nsteps=len(rows), dt_max=0.01, tol=0.02,
+                          max_iters=80, **wind_meta(G)), {})
+    log = [line.split() for line in run("step", d) if line.startswith("step")]
+    assert len(log) == len(rows)
+    for k, ln in enumerate(log):
+        assert np.float32(float(ln[2])) == np.float32(rows[k, 0]), (k, ln)
+        assert int(ln[3]) == int(rows[k, 1]), (k, ln, rows[k])
+    for f in ("u", "v", "w"):
+        assert rel(out(d, f), G["st_" + f]) < 1e-5, f
+    assert rel(out(d, "T"), G["st_T"]) < 1e-6 and rel(out(d, "Yo"), G["st_Yo"]) < 1e-6
+    p = out(d, "p")
+    assert np.abs(p - G["st_p"]).max() <= 1e-4 * np.abs(G["st_p"]).max()
+    assert np.abs(out(d, "zoneP") - G["st_zoneP"]).max() <= 1e-4 * 101325
