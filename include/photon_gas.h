@@ -302,6 +302,11 @@ int phg_add_gas(phg_ctx *ctx, int grid, int ncells, const int *cells, float fuel
 /* Solver::combustion (solver.cpp:284-344); hrr_total (host, may be NULL):
  * the summed heat release rate, W (FP64 over fixed-order block sums). */
 int phg_combustion(phg_ctx *ctx, int grid, float dt, int radiation_on, const phg_gas *gas, double *hrr_total);
+/* The maxima the last phg_correct / phg_project, phg_turbulence and
+ * phg_energy_expansion calls on the grid left on the device (maxVel, maxNu,
+ * maxDiv), read in one wait, for callers that queue those calls without their
+ * host outputs.  Any pointer may be NULL. */
+int phg_grid_stats(phg_ctx *ctx, int grid, float *max_vel, float *max_nu, float *max_div);
 /* Closed zones' integral of S dV (leakage, gasphase.cpp:405-409), FP64, into
  * dint (host, one per zone; 0 for open zones). */
 int phg_zone_integrals(phg_ctx *ctx, int grid, const float *S, double *dint);

@@ -21,6 +21,9 @@ struct DGrid {
 
 __device__ __forceinline__ float h_out(float vel, bool inflow) { return inflow ? -0.5f * vel * vel : 0.0f; }
 
+// std::clamp's semantics (a NaN passes through, as in the reference)
+__device__ __forceinline__ float clamp_ref(float v, float lo, float hi) { return v < lo ? lo : (hi < v ? hi : v); }
+
 // b = (S - div(f u*)) dx^2/dt plus the known outside H on open boundary
 // faces, diag = sum of the masked coefficients, p = 0 in solids (gasphase.cpp:528-549)
 __global__ void __launch_bounds__(TB) k_rhs(DGrid g, float dx, float inv, const float* __restrict__ u,
@@ -77,7 +80,7 @@ __global__ void __launch_bounds__(TB) k_correct(DGrid g, float sc, const float* 
                 const int c = i + nx * (j + ny * k);
                 const float hr = i < nx ? p[c] : h_out(uf, uf < 0);
                 const float hl = i > 0 ? p[c - 1] : h_out(uf, uf > 0);
-                uf = fminf(fmaxf(uf - sc * (hr - hl), -VMAX), VMAX);
+                uf = clamp_ref(uf - sc * (hr - hl), -VMAX, VMAX);
                 u[f] = uf;
             }
             vm = fabsf(uf) * g.fx[f];
@@ -92,7 +95,7 @@ __global__ void __launch_bounds__(TB) k_correct(DGrid g, float sc, const float* 
                 const int c = i + nx * (j + ny * k);
                 const float hr = j < ny ? p[c] : h_out(vf, vf < 0);
                 const float hl = j > 0 ? p[c - nx] : h_out(vf, vf > 0);
-                vf = fminf(fmaxf(vf - sc * (hr - hl), -VMAX), VMAX);
+                vf = clamp_ref(vf - sc * (hr - hl), -VMAX, VMAX);
                 v[f] = vf;
             }
             vm = fabsf(vf) * g.fy[f];
@@ -107,7 +110,7 @@ __global__ void __launch_bounds__(TB) k_correct(DGrid g, float sc, const float* 
             float wf = w[f];
             if (g.az[f] > 0) {
                 const float hu = k + 1 < nz ? p[f] : h_out(wf, wf < 0);
-                wf = fminf(fmaxf(wf - sc * (hu - p[c]), -VMAX), VMAX);
+                wf = clamp_ref(wf - sc * (hu - p[c]), -VMAX, VMAX);
                 w[f] = wf;
             }
             vm = fabsf(wf) * g.fz[f];
@@ -133,11 +136,11 @@ __global__ void __launch_bounds__(TB) k_energy(int n, float dx, float dt, const 
             const float t = T[c], pb = Pb[c], q = Q[c];
             const float rho = pb / (RS * t);
             const float m = rho * V;
-            const float tn = fminf(fmaxf(t + q / (m * CP), T_MIN), T_MAX);
+            const float tn = clamp_ref(t + q / (m * CP), T_MIN, T_MAX);
             const float rhon = pb / (RS * tn);
             const float D = q / (m * CP * t * dt) + massSrc[c] / (rhon * V * dt);
             T[c] = tn;
-            const float sv = fminf(fmaxf(D, -D_MAX), D_MAX);
+            const float sv = clamp_ref(D, -D_MAX, D_MAX);
             S[c] = sv;
             md = fabsf(sv);
         }

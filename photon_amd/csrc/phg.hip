@@ -84,7 +84,7 @@ struct GridDev {
     int *zcells = nullptr, *zclosed = nullptr, *ztptr = nullptr;
     int4* ztask = nullptr;
     float* scratch_b = nullptr;   // RHS of phg_project
-    float* vmax = nullptr;        // one float for max reductions
+    float* vmax = nullptr;        // max reductions of the last call: maxVel, maxNu, maxDiv
     // the rest of the gas step
     uint8_t* lit_new = nullptr;   // combustion's new flame flags (N)
     double* cpart = nullptr;      // combustion's per-block heat release sums
@@ -933,7 +933,7 @@ int phg_grid_create_planned(phg_ctx* ctx, int nx, int ny, int nz, float dx, cons
             UP(G->zclosed, zclosed.data(), zclosed.size())
         }
         if (!rc) rc = dalloc(ctx, G->allocs, &G->scratch_b, (size_t)G->n);
-        if (!rc) rc = dalloc(ctx, G->allocs, &G->vmax, 1);
+        if (!rc) rc = dalloc(ctx, G->allocs, &G->vmax, 4);
         if (!rc) rc = dalloc(ctx, G->allocs, &G->lit_new, (size_t)G->n);
         if (!rc) rc = dalloc(ctx, G->allocs, &G->cpart, (size_t)tiles(G->n));
         if (!rc) rc = dalloc(ctx, G->allocs, &G->zsum, (size_t)(G->nztask + G->nclosed + 1));
@@ -1642,11 +1642,11 @@ int phg_energy_expansion(phg_ctx* ctx, int grid, float dt, float* T, float* S, c
     if (!ctx || !T || !S || !Pb || !Q || !massSrc) return PHG_EINVAL;
     GridDev* G = get_grid(ctx, grid);
     if (!G) return fail(ctx, PHG_ESTATE, "unknown grid %d", grid);
-    hipLaunchKernelGGL(k_fill_f32, dim3(1), dim3(64), 0, ctx->stream, G->vmax, 1, 0.0f);
+    hipLaunchKernelGGL(k_fill_f32, dim3(1), dim3(64), 0, ctx->stream, G->vmax + 2, 1, 0.0f);
     launch(ctx, K_ENERGY, k_energy, tiles(G->n), 0, G->n, G->dx, dt, (const uint8_t*)G->blocked, Pb, Q, massSrc, T, S,
-           G->vmax);
+           G->vmax + 2);
     if (max_div) {
-        HIP_OK(hipMemcpyAsync(max_div, G->vmax, sizeof(float), hipMemcpyDeviceToHost, ctx->stream));
+        HIP_OK(hipMemcpyAsync(max_div, G->vmax + 2, sizeof(float), hipMemcpyDeviceToHost, ctx->stream));
         HIP_OK(hipStreamSynchronize(ctx->stream));
     }
     HIP_OK(hipGetLastError());
@@ -1726,17 +1726,17 @@ int phg_turbulence(phg_ctx* ctx, int grid, const float* u, const float* v, const
     A.T = T;
     A.blocked = G->blocked;
     A.nu = nu;
-    A.max_nu = G->vmax;
+    A.max_nu = G->vmax + 1;
     A.tiles_x = (G->nx + TU_TX - 1) / TU_TX;
     A.tiles_y = (G->ny + TU_TY - 1) / TU_TY;
     const int ntile = A.tiles_x * A.tiles_y;
     const int want = std::max(1, (1024 + ntile - 1) / ntile);
     A.kchunk = std::max(4, (G->nz + want - 1) / want);
     const int nchunk = (G->nz + A.kchunk - 1) / A.kchunk;
-    hipLaunchKernelGGL(k_fill_f32, dim3(1), dim3(64), 0, ctx->stream, G->vmax, 1, 0.0f);
+    hipLaunchKernelGGL(k_fill_f32, dim3(1), dim3(64), 0, ctx->stream, G->vmax + 1, 1, 0.0f);
     launch_nt(ctx, K_TURB, k_turbulence, ntile * nchunk, TU_THREADS, 0, A);
     if (max_nu) {
-        HIP_OK(hipMemcpyAsync(max_nu, G->vmax, sizeof(float), hipMemcpyDeviceToHost, ctx->stream));
+        HIP_OK(hipMemcpyAsync(max_nu, G->vmax + 1, sizeof(float), hipMemcpyDeviceToHost, ctx->stream));
         HIP_OK(hipStreamSynchronize(ctx->stream));
     }
     HIP_OK(hipGetLastError());
@@ -1877,6 +1877,19 @@ int phg_combustion(phg_ctx* ctx, int grid, float dt, int radiation_on, const phg
         *hrr_total = t;
     }
     HIP_OK(hipGetLastError());
+    return PHG_OK;
+}
+
+int phg_grid_stats(phg_ctx* ctx, int grid, float* max_vel, float* max_nu, float* max_div) {
+    if (!ctx) return PHG_EINVAL;
+    GridDev* G = get_grid(ctx, grid);
+    if (!G) return fail(ctx, PHG_ESTATE, "unknown grid %d", grid);
+    float v[3];
+    HIP_OK(hipMemcpyAsync(v, G->vmax, sizeof v, hipMemcpyDeviceToHost, ctx->stream));
+    HIP_OK(hipStreamSynchronize(ctx->stream));
+    if (max_vel) *max_vel = v[0];
+    if (max_nu) *max_nu = v[1];
+    if (max_div) *max_div = v[2];
     return PHG_OK;
 }
 

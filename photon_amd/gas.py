@@ -34,7 +34,7 @@ EXPORTS = (
     "phg_grid_plan_build", "phg_grid_plan_free", "phg_grid_create_planned", "phg_zone_mean_noalloc",
     "phg_grid_set_coarse", "phg_grid_get_coarse",
     "phg_stable_dt", "phg_turbulence", "phg_heat_diffusion", "phg_momentum", "phg_step_clear", "phg_add_gas",
-    "phg_combustion", "phg_zone_integrals", "phg_zone_values",
+    "phg_combustion", "phg_zone_integrals", "phg_zone_values", "phg_grid_stats",
 )
 
 
@@ -124,6 +124,7 @@ def load_library():
     L.phg_combustion.argtypes = [vp, i32, f32, i32, C.POINTER(GasPtrs), C.POINTER(f64)]
     L.phg_zone_integrals.argtypes = [vp, i32, vp, vp]
     L.phg_zone_values.argtypes = [vp, i32, vp, vp, i32]
+    L.phg_grid_stats.argtypes = [vp, i32, C.POINTER(f32), C.POINTER(f32), C.POINTER(f32)]
     L.phg_leakage.argtypes = [vp, i32, vp, vp]
     L.phg_correct.argtypes = [vp, i32, f32, vp, vp, vp, vp, C.POINTER(f32)]
     L.phg_energy_expansion.argtypes = [vp, i32, f32, vp, vp, vp, vp, vp, C.POINTER(f32)]
@@ -411,6 +412,38 @@ class Batch:
 FIELD_NAMES = ("T", "Yo", "Yf", "Ys", "Ya", "Yp", "Yc", "Yw", "Yco", "Yai")
 
 
+def burner_cells(g, x0, y0, x1, y1, z, ox=0.0, oy=0.0, oz=0.0):
+    """The gas cells a burner over the floor rectangle (x0, y0)-(x1, y1) at
+    height z releases into, by Solver::addBurner's rule (gasphase.cpp:72-90:
+    cell centres inside the rectangle, else the cell under its centre); None
+    when it covers no gas cell."""
+    f32 = np.float32
+    dx = f32(g.dx)
+
+    def cell_at(x, y, zz):
+        i, j, k = (int(np.floor((f32(a) - f32(o)) / dx)) for a, o in ((x, ox), (y, oy), (zz, oz)))
+        if i < 0 or j < 0 or k < 0 or i >= g.nx or j >= g.ny or k >= g.nz:
+            return -1
+        return i + g.nx * (j + g.ny * k)
+
+    cells = []
+    zc = f32(z) + f32(0.5) * dx
+    for j in range(g.ny):
+        for i in range(g.nx):
+            x, y = f32(ox) + (f32(i) + f32(0.5)) * dx, f32(oy) + (f32(j) + f32(0.5)) * dx
+            if x < f32(x0) or x > f32(x1) or y < f32(y0) or y > f32(y1):
+                continue
+            c = cell_at(x, y, zc)
+            if c >= 0 and not g.blocked[c]:
+                cells.append(c)
+    if not cells:
+        c = cell_at(f32(0.5) * (f32(x0) + f32(x1)), f32(0.5) * (f32(y0) + f32(y1)), zc)
+        if c < 0 or g.blocked[c]:
+            return None
+        cells.append(c)
+    return np.array(cells, np.int32)
+
+
 def hierarchy_shape(scen, wind_closed=None):
     """Level sizes and level-0 aggregates the library builds for a geometry
     (host only, no device needed)."""
@@ -558,14 +591,24 @@ class GasSolver:
     def _gas(self):
         return GasPtrs(*[self.d[n].ptr for n in GAS_FIELDS])
 
-    def turbulence(self):
-        """Solver::turbulence: nu and maxNu (gasphase.cpp:163-218)."""
+    def turbulence(self, report=True):
+        """Solver::turbulence: nu and maxNu (gasphase.cpp:163-218); maxNu is
+        read back when `report` (which waits for the device)."""
         m = C.c_float()
         d = self.d
         self.ctx._check(self.ctx.L.phg_turbulence(self.ctx.h, self.grid.id, d["u"].ptr, d["v"].ptr, d["w"].ptr,
-                                                  d["T"].ptr, d["nu"].ptr, C.byref(m)))
-        self.maxNu = m.value
+                                                  d["T"].ptr, d["nu"].ptr, C.byref(m) if report else None))
+        if report:
+            self.maxNu = m.value
         return self.maxNu
+
+    def read_stats(self):
+        """maxVel, maxNu and maxDiv of the last queued correction, turbulence
+        and energy calls, in one wait (phg_grid_stats)."""
+        a, b, c = C.c_float(), C.c_float(), C.c_float()
+        self.ctx._check(self.ctx.L.phg_grid_stats(self.ctx.h, self.grid.id, C.byref(a), C.byref(b), C.byref(c)))
+        self.maxVel, self.maxNu, self.maxDiv = a.value, b.value, c.value
+        return self.maxVel, self.maxNu, self.maxDiv
 
     def heat_diffusion(self, dt):
         d = self.d
@@ -582,34 +625,12 @@ class GasSolver:
             d[f], d[f + "2"] = d[f + "2"], d[f]
 
     def add_burner(self, x0, y0, x1, y1, z, peak_w, ramp_s=0.0, start=0.0, duration=1e9, ox=0.0, oy=0.0, oz=0.0):
-        """Solver::addBurner (gasphase.cpp:72-90): the gas cells under a floor
-        rectangle at height z; returns its index, or -1 if it covers none."""
-        g = self.scen
+        """Solver::addBurner (gasphase.cpp:72-90): returns its index, or -1 if
+        it covers no gas cell."""
+        arr = burner_cells(self.scen, x0, y0, x1, y1, z, ox, oy, oz)
+        if arr is None:
+            return -1
         f32 = np.float32
-        dx = f32(g.dx)
-
-        def cell_at(x, y, zz):
-            i, j, k = (int(np.floor((f32(a) - f32(o)) / dx)) for a, o in ((x, ox), (y, oy), (zz, oz)))
-            if i < 0 or j < 0 or k < 0 or i >= g.nx or j >= g.ny or k >= g.nz:
-                return -1
-            return i + g.nx * (j + g.ny * k)
-
-        cells = []
-        zc = f32(z) + f32(0.5) * dx
-        for j in range(g.ny):
-            for i in range(g.nx):
-                x, y = f32(ox) + (f32(i) + f32(0.5)) * dx, f32(oy) + (f32(j) + f32(0.5)) * dx
-                if x < f32(x0) or x > f32(x1) or y < f32(y0) or y > f32(y1):
-                    continue
-                c = cell_at(x, y, zc)
-                if c >= 0 and not g.blocked[c]:
-                    cells.append(c)
-        if not cells:
-            c = cell_at(f32(0.5) * (f32(x0) + f32(x1)), f32(0.5) * (f32(y0) + f32(y1)), zc)
-            if c < 0 or g.blocked[c]:
-                return -1
-            cells.append(c)
-        arr = np.array(cells, np.int32)
         self.burners.append(dict(cells=arr, dcells=self.ctx.to_device(arr), peak=f32(peak_w), ramp=f32(ramp_s),
                                  start=f32(start), duration=f32(duration)))
         return len(self.burners) - 1
@@ -629,11 +650,12 @@ class GasSolver:
             self.ctx._check(self.ctx.L.phg_add_gas(self.ctx.h, self.grid.id, len(b["cells"]), b["dcells"].ptr,
                                                    mf, 0.0, mf * f32(0.01), 800.0, 1, C.byref(self._gas())))
 
-    def combustion(self, dt):
+    def combustion(self, dt, report=True):
         tot = C.c_double()
         self.ctx._check(self.ctx.L.phg_combustion(self.ctx.h, self.grid.id, dt, 1 if self.radiation_on else 0,
-                                                  C.byref(self._gas()), C.byref(tot)))
-        self.hrr_total = tot.value
+                                                  C.byref(self._gas()), C.byref(tot) if report else None))
+        if report:
+            self.hrr_total = tot.value
         return self.hrr_total
 
     def step_clear(self):
@@ -674,6 +696,81 @@ class GasSolver:
                 self.zoneP[z] = min(max(v, f32(101325.0 - 20000.0)), f32(101325.0 + 50000.0))
         self.ctx._check(self.ctx.L.phg_zone_values(self.ctx.h, self.grid.id, self.zoneP.ctypes.data,
                                                    self.d["Pb"].ptr, 0))
+
+    # ---- checkpoint and resume (the role of Solver::saveState / loadState,
+    # src/sim/state.cpp:62-195): every field a later step reads, the zone
+    # pressures, the step statistics stableDt uses, time and burners
+    STATE_MAGIC = "PHGGAS01"
+    STATE_FIELDS = ("u", "v", "w", "p", "S", "nu", "Pb", "hrr", "lit") + FIELD_NAMES
+
+    def fingerprint(self):
+        """Hash of the geometry and wind a state belongs to."""
+        import hashlib
+        h = hashlib.sha256()
+        g = self.scen
+        h.update(np.array([g.nx, g.ny, g.nz], np.int64).tobytes())
+        h.update(np.float32(g.dx).tobytes())
+        for a in (g.fx, g.fy, g.fz, g.ax, g.ay, g.az):
+            h.update(np.ascontiguousarray(a, np.float32).tobytes())
+        h.update(np.ascontiguousarray(g.blocked, np.uint8).tobytes())
+        h.update(np.array([self.wind.speed, self.wind.from_deg, self.wind.exponent, self.wind.oz], np.float32).tobytes())
+        return h.hexdigest()
+
+    def snapshot(self):
+        """The full state as host arrays (a dict)."""
+        st = {"magic": self.STATE_MAGIC, "fingerprint": self.fingerprint(), "time": np.float32(self.time),
+              "zoneP": self.zoneP.copy(),
+              "stats": np.array([self.maxVel, self.maxNu, self.maxDiv], np.float32),
+              "pressureIters": self.pressureIters,
+              "burners": [{k: (v.copy() if isinstance(v, np.ndarray) else v) for k, v in b.items() if k != "dcells"}
+                          for b in self.burners]}
+        for f in self.STATE_FIELDS:
+            st["f_" + f] = self.get(f)
+        return st
+
+    def restore(self, st):
+        """Load a snapshot taken from a solver of the same geometry and wind
+        (refused otherwise, with a message)."""
+        if st.get("magic") != self.STATE_MAGIC:
+            raise PhgError("not a gas state")
+        if st.get("fingerprint") != self.fingerprint():
+            raise PhgError("the state belongs to another geometry or wind")
+        for f in self.STATE_FIELDS:
+            self.set(f, st["f_" + f])
+        self.time = float(st["time"])
+        self.zoneP[:] = st["zoneP"]
+        self.maxVel, self.maxNu, self.maxDiv = (float(x) for x in st["stats"])
+        self.pressureIters = int(st["pressureIters"])
+        self.burners = []
+        for b in st["burners"]:
+            nb = dict(b)
+            nb["dcells"] = self.ctx.to_device(np.asarray(b["cells"], np.int32))
+            self.burners.append(nb)
+
+    def save_state(self, path):
+        """Write the snapshot as an .npz of plain arrays (no pickled objects)."""
+        st = self.snapshot()
+        flat = {k: v for k, v in st.items() if k != "burners"}
+        flat["magic"] = np.array(st["magic"])
+        flat["fingerprint"] = np.array(st["fingerprint"])
+        flat["pressureIters"] = np.int32(st["pressureIters"])
+        for i, b in enumerate(st["burners"]):
+            flat[f"burner{i}_cells"] = b["cells"]
+            flat[f"burner{i}_params"] = np.array([b["peak"], b["ramp"], b["start"], b["duration"]], np.float32)
+        with open(path, "wb") as f:
+            np.savez(f, **flat)
+
+    def load_state(self, path):
+        with np.load(path, allow_pickle=False) as z:
+            st = {k: z[k] for k in z.files}
+        st["magic"], st["fingerprint"] = str(st["magic"]), str(st["fingerprint"])
+        burners, i = [], 0
+        while f"burner{i}_cells" in st:
+            peak, ramp, start, dur = (np.float32(x) for x in st.pop(f"burner{i}_params"))
+            burners.append(dict(cells=st.pop(f"burner{i}_cells"), peak=peak, ramp=ramp, start=start, duration=dur))
+            i += 1
+        st["burners"] = burners
+        self.restore(st)
 
     def stable_dt(self, dt_max):
         out = C.c_float()

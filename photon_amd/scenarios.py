@@ -203,18 +203,19 @@ def _finish(blocked3, dx, open_sides=(), door=None) -> Grid:
     return g
 
 
-def room(dx: float = 0.2, window: bool = False) -> Grid:
+def room(dx: float = 0.2, window: bool = False, door: bool = True) -> Grid:
     """Single 4.0 x 3.0 x 2.4 m room: walls closed, a 0.8 x 2.0 m door in the
     x = 0 wall (j = 6..9 and k = 0..9 at 0.2 m) open to ambient, burner cells
-    (0.4 x 0.4 m) at the floor centre.  With `window`, also a 1.2 x 1.0 m
+    (0.4 x 0.4 m) at the floor centre; `door` = False seals it.  With
+    `window`, also a 1.2 x 1.0 m
     window open to ambient in the y = 3.0 m wall (x 1.4..2.6 m, z 1.0..2.0 m;
     i = 7..12, k = 5..9 at 0.2 m)."""
     nx, ny, nz = round(4.0 / dx), round(3.0 / dx), round(2.4 / dx)
     blocked = np.zeros((nz, ny, nx), np.uint8)
     yc = (np.arange(ny) + 0.5) * dx
     zc = (np.arange(nz) + 0.5) * dx
-    door = (np.abs(zc[:, None] - 1.0) < 1.0) & (yc[None, :] > 1.2) & (yc[None, :] < 2.0)
-    g = _finish(blocked, dx, door=door)
+    opening = (np.abs(zc[:, None] - 1.0) < 1.0) & (yc[None, :] > 1.2) & (yc[None, :] < 2.0)
+    g = _finish(blocked, dx, door=opening if door else None)
     xc = (np.arange(nx) + 0.5) * dx
     if window:
         win = (np.abs(zc[:, None] - 1.5) < 0.5) & (np.abs(xc[None, :] - 2.0) < 0.6)

@@ -195,3 +195,68 @@ def test_whole_step_follows_reference_trajectory(ctx, name):
     assert np.abs(s.zoneP - G["st_zoneP"]).max() <= 1e-4 * 101325
     assert np.array_equal(s.get("lit"), G["st_lit"])
     s.grid.destroy()
+
+
+# ---------------------------------------------------------------------------
+# invariants of the reference's own tests (src/tests/test_main.cpp:437-546),
+# through the device step
+# ---------------------------------------------------------------------------
+def test_still_air_stays_still(ctx):
+    # test_main.cpp:437-445: no fire, 40 steps of up to 0.05 s
+    s = gas.GasSolver(ctx, sc.room())
+    for _ in range(40):
+        s.step(0.05)
+    s.read_stats()
+    dT = np.abs(s.get("T") - 293.15).max()
+    assert s.maxVel < 1e-3 and dT < 1e-3, (s.maxVel, dT)
+    s.grid.destroy()
+
+
+def test_sealed_room_pressure_rise(ctx):
+    # test_main.cpp:450-470: a sealed adiabatic room with a 20 kW burner; the
+    # background pressure rises by (gamma - 1) Q / V within 8 %, nothing leaves
+    g = sc.room(door=False)
+    s = gas.GasSolver(ctx, g)
+    assert s.add_burner(1.6, 1.0, 2.0, 1.4, 0.0, 20e3) >= 0
+    heat = 0.0
+    for _ in range(60):
+        dt = s.step(0.05)
+        heat += s.hrr_total * dt
+    z = int(s.grid.zones()[0][g.n // 2])
+    assert not s.zone_open[z]
+    V = float(s.zone_vol[z])
+    gamma = 1100.0 / (1100.0 - 287.0)
+    dp_expect = (gamma - 1) * heat / V
+    dp = float(s.get("Pb")[g.n // 2]) - 101325.0
+    print(f"sealed room: dp {dp:.3f} Pa, expected {dp_expect:.3f} Pa from {heat / 1e3:.2f} kJ in {V:.2f} m3")
+    assert heat > 0 and abs(dp - dp_expect) < 0.08 * dp_expect
+    s.grid.destroy()
+
+
+def test_resume_matches_uninterrupted(ctx, tmp_path):
+    # test_main.cpp:519-546: save at t, continue; a fresh solver loaded from
+    # the file follows the same path bit for bit; another grid refuses it
+    def make():
+        s = gas.GasSolver(ctx, sc.room())
+        assert s.add_burner(3.0, 1.0, 3.4, 1.4, 0.0, 60e3, ramp_s=5.0) >= 0
+        return s
+
+    a = make()
+    for _ in range(60):
+        a.step(0.05)
+    path = str(tmp_path / "resume.npz")
+    a.save_state(path)
+    for _ in range(60):
+        a.step(0.05)
+    r = make()
+    r.load_state(path)
+    for _ in range(60):
+        r.step(0.05)
+    assert a.time == r.time
+    for f in ("T", "Yf", "Ys", "p", "u", "v", "w", "nu", "Pb"):
+        assert bits_equal(a.get(f), r.get(f)), f
+    other = gas.GasSolver(ctx, sc.room(0.25))
+    with pytest.raises(gas.PhgError, match="another geometry"):
+        other.load_state(path)
+    for s in (a, r, other):
+        s.grid.destroy()
