@@ -102,7 +102,31 @@ def algorithmic_bytes(name, iters, n0, n1, vt, flex):
     return per_sys, shared
 
 
-def run_batch(ctx, gas, grid_scen, prec, bs_host, steps, warmup, profile):
+def whole_solve_traffic(ms_per_step):
+    """Counter-measured HBM bytes of every kernel of one solve step (the
+    committed FETCH_SIZE/WRITE_SIZE runs: dispatches x calibrated bytes per
+    launch, over the solves of that run) against this run's step time: the
+    whole solve's fraction of the HBM peak."""
+    path = os.path.join(ROOT, TRAFFIC_FILE)
+    total, nsolve = 0.0, None
+    try:
+        with open(path) as f:
+            for line in f:
+                parts = line.split()
+                if len(parts) == 3 and parts[0] == "dispatched" and parts[1] != "k_copy16":
+                    total += float(parts[2])
+                elif len(parts) == 2 and parts[0] == "solves":
+                    nsolve = int(parts[1])
+    except OSError:
+        return None
+    if not nsolve or total <= 0:
+        return None
+    per_step = total / nsolve
+    return {"bytes_per_step": round(per_step), "GBps": round(per_step / (ms_per_step * 1e-3) / 1e9, 1),
+            "frac": round(per_step / (ms_per_step * 1e-3) / HBM_PEAK, 4), "source": TRAFFIC_FILE}
+
+
+def run_batch(ctx, gas, grid_scen, prec, bs_host, steps, warmup, profile, sample=()):
     n = grid_scen.n
     nsys = len(bs_host)
     G = gas.Grid(ctx, grid_scen, precision=prec)
@@ -144,11 +168,32 @@ def run_batch(ctx, gas, grid_scen, prec, bs_host, steps, warmup, profile):
         for s in st:
             key = {1: "converged", 2: "max_iters", 3: "breakdown"}.get(s["status"], "running")
             status[key] = status.get(key, 0) + 1
-    x0 = x_all.download()[0] if nsys else None
+    xs = x_all.download()[list(sample)] if len(sample) else None
     B.destroy()
     G.destroy()
     del b_all, x_all
-    return dict(seconds=t1 - t0, iters=iters, ok=ok, prof=prof, prof_all=prof_all, x0=x0, status=status)
+    return dict(seconds=t1 - t0, iters=iters, ok=ok, prof=prof, prof_all=prof_all, xs=xs, status=status)
+
+
+def fp64_errors(grid_scen, bs_host, xs, threads=8):
+    """Error of GPU solutions against the FP64 oracle (solved to 1e-10,
+    closed-zone means removed) and their true relative residual ||b - A x|| /
+    ||b|| in FP64, for the sampled systems; the oracle runs on host threads."""
+    import concurrent.futures
+
+    from oracle import pyoracle as po
+    H = po.Hierarchy(grid_scen)
+
+    def one(k):
+        x64, _, _ = H.pcg64(bs_host[k], tol=1e-10)
+        x = np.asarray(xs[k], np.float64)
+        err = float(np.linalg.norm(x - x64) / np.linalg.norm(x64))
+        return err, float(H.true_relres(bs_host[k], xs[k]))
+
+    with concurrent.futures.ThreadPoolExecutor(threads) as pool:
+        res = list(pool.map(one, range(len(xs))))
+    return {"err_fp64_max": max(r[0] for r in res), "true_relres_max": max(r[1] for r in res),
+            "sampled_systems": len(res)}
 
 
 def run_plain(ctx, gas, grid_scen, bs_host, nsolve):
@@ -181,36 +226,45 @@ def run_precision(ctx, gas, sc, nrhs=16):
     g = sc.house(1, 160, 100, 80, 0.1)
     bs = [sc.rhs_from_divergence(g, sc.divergence_field(g, 5000 + s), DT) for s in range(nrhs)]
     out = {}
+    sample = [0, nrhs - 1]
     for name, prec in (("fp32", gas.PREC_FP32), ("mixed_f16", gas.PREC_MIXED_F16), ("mixed_bf16", gas.PREC_MIXED_BF16)):
-        r = run_batch(ctx, gas, g, prec, bs, 1, 1, profile=False)
+        r = run_batch(ctx, gas, g, prec, bs, 1, 1, profile=False, sample=sample)
         out[name] = {"solves_per_s": round(nrhs / r["seconds"], 2), "iters_mean": float(r["iters"].mean()),
                      "iters_max": int(r["iters"].max()), "all_converged": bool(r["ok"])}
-    out["workload"] = f"house 160x100x80 cells at 0.1 m (seed 1), {nrhs} RHS as one batch, ||r||/||b|| <= 1e-6"
+        out[name].update(fp64_errors(g, [bs[k] for k in sample], r["xs"], threads=len(sample)))
+    out["mixed_f16_vs_fp32"] = round(out["mixed_f16"]["solves_per_s"] / out["fp32"]["solves_per_s"], 3)
+    out["workload"] = (f"house 160x100x80 cells at 0.1 m (seed 1), {nrhs} RHS as one batch, ||r||/||b|| <= 1e-6; "
+                       f"errors against the FP64 oracle on RHS {sample}")
     return out
 
 
 def run_batching(ctx, gas, sc, nscen=64, steps=5):
     """The batching workload: `nscen` scenarios with grids uniform between
     20x15x12 and 80x50x40 (independent walls), each with a seeded 10-500 kW
-    burner; one timestep's pressure solves (FP16 V-cycle) as one batch whose
-    set-up and iterations replay as one captured graph.  Timed with the
-    library's default block shape and with the autotuner's per-size-class
-    block depth and smoother sweeps."""
+    burner; one timestep's pressure solves (FP16 V-cycle) as one batch, with
+    up to two restarts from the FP64 true residual (refine = 2, the setting
+    its parity tests use).  The set-up and every iteration of the first round
+    replay as ONE captured graph, sized from the previous timestep's
+    iteration count (the first timestep takes 16 iterations and checks).
+    Timed with the library's default block shape and with the autotuner's
+    per-size-class block depth and smoother sweeps; every system's solution
+    is then checked against the FP64 oracle."""
     from photon_amd import autotune as at
     grids = [sc.batch_grid(s) for s in range(nscen)]
     rhs = [sc.rhs_from_divergence(g, sc.burner_field(g, s), DT) for s, g in enumerate(grids)]
     Gs = [gas.Grid(ctx, g, precision=gas.PREC_MIXED_F16) for g in grids]
     dbs = [ctx.to_device(b) for b in rhs]
     dxs = [ctx.array(g.n) for g in grids]
-    # the set-up and the first 16 iterations as one graph, then a check every 4
-    params = gas.solve_params(tol=TOL, max_iters=400, zero_x=True, use_graph=2, check_every=16)
+    first = gas.solve_params(tol=TOL, max_iters=400, zero_x=True, use_graph=2, check_every=16, refine=2)
 
     def timed(kcs, sws):
         B = gas.Batch(ctx, Gs, sys_kchunk=kcs, sys_sweeps0=sws)
-        B.solve(dbs, dxs, params)
+        st = B.solve(dbs, dxs, first)       # the first timestep
+        n = max(4, -(-max(s["iters"] for s in st) // 4) * 4)
+        params = gas.solve_params(tol=TOL, max_iters=400, zero_x=True, use_graph=2, check_every=n, refine=2)
+        B.solve(dbs, dxs, params)           # graph capture at the new size
         ctx.synchronize()
         lat = []
-        st = None
         for _ in range(steps):
             t0 = time.perf_counter()
             st = B.solve(dbs, dxs, params)
@@ -220,105 +274,111 @@ def run_batching(ctx, gas, sc, nscen=64, steps=5):
         it = np.array([s["iters"] for s in st])
         ok = all(s["status"] == gas.CONVERGED for s in st)
         return {"solves_per_s": round(nscen / float(np.median(lat)), 1), "step_ms_median": round(1e3 * float(np.median(lat)), 3),
-                "iters_mean": float(it.mean()), "iters_max": int(it.max()), "all_converged": bool(ok)}
+                "iters_mean": float(it.mean()), "iters_max": int(it.max()), "all_converged": bool(ok),
+                "graph_iterations": n}
 
     default = timed(None, None)
     t0 = time.perf_counter()
-    table = at.tune(ctx, Gs, dbs, dxs, params, repeats=2)
+    table = at.tune(ctx, Gs, dbs, dxs, first, repeats=2)
     tune_s = time.perf_counter() - t0
     kcs, sws = at.settings(Gs, table)
     tuned = timed(kcs, sws)
+    errs = fp64_errors_multi(grids, rhs, [d.download() for d in dxs])
     for G in Gs:
         G.destroy()
     sizes = np.array([g.n for g in grids])
     return {"workload": f"{nscen} scenarios, grids uniform 20x15x12..80x50x40 ({sizes.min()}..{sizes.max()} cells, "
-                        f"{sizes.sum()} total), 10-500 kW burner each, one timestep's solves as one batch, FP16 V-cycle",
+                        f"{sizes.sum()} total), 10-500 kW burner each, one timestep's solves as one batch, FP16 V-cycle, "
+                        "refine = 2, ||r||/||b|| <= 1e-6",
             "default": default, "autotuned": tuned, "autotune_table": at.table_to_json(table),
-            "autotune_seconds": round(tune_s, 2)}
+            "autotune_seconds": round(tune_s, 2), "fp64_check_all": errs}
 
 
-def run_serving(ctx, window_s=2.0, steps=30):
+def fp64_errors_multi(grids, bs, xs, threads=8):
+    """fp64_errors over systems on different grids (one oracle hierarchy each)."""
+    import concurrent.futures
+
+    from oracle import pyoracle as po
+
+    def one(k):
+        H = po.Hierarchy(grids[k])
+        x64, _, _ = H.pcg64(bs[k], tol=1e-10)
+        x64 = po.closed_zone_demean(grids[k], x64)
+        x = po.closed_zone_demean(grids[k], np.asarray(xs[k], np.float64))
+        return (float(np.linalg.norm(x - x64) / np.linalg.norm(x64)), float(H.true_relres(bs[k], xs[k])))
+
+    with concurrent.futures.ThreadPoolExecutor(threads) as pool:
+        res = list(pool.map(one, range(len(grids))))
+    return {"err_fp64_max": max(r[0] for r in res), "true_relres_max": max(r[1] for r in res),
+            "systems": len(res)}
+
+
+def run_serving(ctx, window_s=2.0, steps=300):
     """The serving workload: Poisson arrivals at 20 scenarios/s over
     `window_s` seconds, sizes log-uniform 3.6k..1.28M cells (room aspect),
-    10-500 kW burners, each scenario run for `steps` steps of dt = 0.01 s
-    (energyAndExpansion, projection, advectScalars) with its state resident in
-    HBM; continuous batching under a 64 GB footprint budget."""
+    10-500 kW burners, each scenario run for `steps` whole gas steps
+    (Solver::step at dt <= 0.01 s) with its state resident in HBM and
+    FP16-storage MG-PCG (||r||/||b|| <= 1e-6, one restart allowed); continuous
+    batching under a 64 GB footprint budget."""
     from photon_amd import serving as sv
     specs = sv.poisson_stream(7, rate_per_s=20.0, window_s=window_s, steps=steps)
     out = sv.serve(ctx, specs, hbm_budget_bytes=64e9, max_active=64)
     out["workload"] = (f"Poisson 20/s over {window_s} s ({len(specs)} scenarios), 3.6k..1.28M cells log-uniform, "
-                       f"{steps} steps each at dt = 0.01 s, FP32, 64 GB admission budget")
+                       f"{steps} whole gas steps each at dt <= 0.01 s ({steps / 100:g} s simulated of the config's 30 s, "
+                       "to keep the run near a minute), FP16-storage MG-PCG, 64 GB admission budget")
     return out
 
 
-def run_room(ctx, gas, sc, steps=300, warmup=20):
+def run_room(ctx, gas, sc, steps=6000, warmup=20):
     """The room case (20x15x12 at 0.2 m, door open to ambient) with a 100 kW
-    2x2-cell burner at the floor centre, stepped at dt = 0.01 s through the
-    phases this library implements, in Solver::step's order: energyAndExpansion,
-    the closed-zone source shift, project (||r||/||b|| <= 1e-6, warm start from
-    the previous pressure, iteration count read back every step as the
-    reference reads pressureIters), advectScalars."""
+    0.4 x 0.4 m burner at the floor centre: Solver::step on the device (burner
+    gas, combustion, eddy viscosity, conduction, energy and expansion,
+    momentum, leakage, project to ||r||/||b|| <= 1e-6 warm-started from the
+    last pressure, transport, zone pressure) at dt = min(0.01 s, stableDt)."""
     g = sc.room()
-    dt, hrr = 0.01, 100e3
-    Q = np.zeros(g.n, np.float32)
-    i0, j0 = g.nx // 2 - 1, g.ny // 2 - 1
-    for j in (j0, j0 + 1):
-        for i in (i0, i0 + 1):
-            Q[i + g.nx * j] = hrr * dt / 4.0
     s = gas.GasSolver(ctx, g, tol=1e-6, max_iters=400)
-    s.set("Q", Q)
-    iters = []
-
-    def step():
-        s.energy_and_expansion(dt, report=False)
-        s.leakage()
-        iters.append(s.project(dt))
-        s.advect_scalars(dt)
-
+    s.add_burner(1.8, 1.2, 2.2, 1.6, 0.0, 100e3)
+    iters, dts = [], []
     for _ in range(warmup):
-        step()
+        s.step(0.01)
+        iters.append(s.pressureIters)
     ctx.synchronize()
-    iters.clear()
     t0 = time.perf_counter()
     for _ in range(steps):
-        step()
+        dts.append(s.step(0.01))
+        iters.append(s.pressureIters)
     ctx.synchronize()
     sec = time.perf_counter() - t0
     T = s.get("T")
-    return {"workload": "room 20x15x12 at 0.2 m, door open, 100 kW burner, dt = 0.01 s, FP32, "
-                        f"{warmup} warm-up steps then {steps} timed",
+    s.read_stats()
+    return {"workload": f"room 20x15x12 at 0.2 m, door open, 100 kW burner, Solver::step at dt <= 0.01 s, "
+                        f"FP32, ||r||/||b|| <= 1e-6, {warmup} warm-up steps then {steps} timed "
+                        f"({sum(dts):.1f} s simulated)",
             "steps_per_s": round(steps / sec, 1), "ms_per_step": round(1e3 * sec / steps, 3),
-            "pcg_iters_mean": round(float(np.mean(iters)), 2), "pcg_iters_max": int(np.max(iters)),
-            "T_max_K": round(float(T.max()), 2), "_iters": list(iters), "_Q": s.get("Q")}
+            "dt_min": round(float(min(dts)), 5), "pcg_iters_mean": round(float(np.mean(iters[warmup:])), 2),
+            "pcg_iters_max": int(np.max(iters[warmup:])), "T_max_K": round(float(T.max()), 2),
+            "max_vel": round(s.maxVel, 3), "_iters": list(iters)}
 
 
-def cpu_room(po, iters, Q, budget_s):
-    """The room case through the reference's own Solver phases on the host
-    (energyAndExpansion, project, advectScalars; the door's open boundary
-    leaves leakage nothing to shift), each step's solve run for the iteration
-    count the GPU run needed on that step.  One thread: with more, the
-    reference's job pool has stalled on this 12-plane grid (in project and
-    advectScalars)."""
+def cpu_room(po, iters, budget_s):
+    """The room case through the reference's own Solver::step on the host
+    (oracle/_ref, its GCC flags): the same burner from still air, each step's
+    pressure solve run for the iteration count the GPU run needed on that
+    step.  One thread: with more, the reference's job pool has stalled on
+    this 12-plane grid (in project and advectScalars)."""
     from photon_amd import scenarios as sc
     g = sc.room()
     R = po.RefSolver(g, threads=1, pin=False)
-    R.set("Q", Q)
-    dt = 0.01
-    for k in range(20):   # the GPU run's warm-up
-        R.energy(dt)
-        R.project(dt, 0.0, int(iters[0]))
-        R.advect(dt)
+    R.add_burner(1.8, 1.2, 2.2, 1.6, 0.0, 100e3)
     done, used = 0, 0.0
     while used < budget_s and done < len(iters):
         t0 = time.perf_counter()
-        R.energy(dt)
-        R.project(dt, 0.0, int(iters[done]))
-        R.advect(dt)
+        R.step(0.01, 0.0, int(iters[done]))
         used += time.perf_counter() - t0
         done += 1
     return {"steps_per_s": round(done / used, 1), "threads": R.threads,
-            "sample": f"{done} steps after 20 warm-up steps, each solve run for the GPU run's iteration count on that "
-                      f"step (mean {np.mean(iters[:done]):.2f})"}
+            "sample": f"the first {done} steps from still air (Solver::step, burner, gas-only switches), each "
+                      f"solve run for the GPU run's iteration count on that step (mean {np.mean(iters[:done]):.2f})"}
 
 
 def cpu_baseline(scen, bs_host, iters_ref, budget_s=15.0, room=None):
@@ -383,7 +443,7 @@ def cpu_baseline(scen, bs_host, iters_ref, budget_s=15.0, room=None):
     if by_threads:
         out["by_threads"] = by_threads
     if room is not None and kind == "reference":
-        out["room_c1"] = cpu_room(po, room["_iters"], room["_Q"], min(budget_s, 10.0))
+        out["room_c1"] = cpu_room(po, room["_iters"], min(budget_s, 10.0))
     return out
 
 
@@ -415,7 +475,8 @@ def main():
     bs = [sc.rhs_from_divergence(scen, S, DT) for S in S_list]
     ctx = gas.Context(0)
 
-    mixed = run_batch(ctx, gas, scen, gas.PREC_MIXED_F16, bs, args.steps, args.warmup, profile=True)
+    sample = sorted({0, args.nsys // 3, 2 * args.nsys // 3, args.nsys - 1})
+    mixed = run_batch(ctx, gas, scen, gas.PREC_MIXED_F16, bs, args.steps, args.warmup, profile=True, sample=sample)
     if args.copy_gib > 0:
         copy_bytes = int(args.copy_gib * (1 << 30)) // 16 * 16
         src, dst = ctx.array(copy_bytes // 4), ctx.array(copy_bytes // 4)
@@ -428,7 +489,7 @@ def main():
         fp32 = plain = None
         v_fp32 = v_plain = float("nan")
     else:
-        fp32 = run_batch(ctx, gas, scen, gas.PREC_FP32, bs, args.steps, args.warmup, profile=False)
+        fp32 = run_batch(ctx, gas, scen, gas.PREC_FP32, bs, args.steps, args.warmup, profile=False, sample=sample)
         plain = run_plain(ctx, gas, scen, bs, args.plain)
         v_fp32 = solves / fp32["seconds"]
         v_plain = args.plain / plain["seconds"]
@@ -465,6 +526,10 @@ def main():
             "algorithmic_bytes_per_launch": round(bpl), "avg_launch_us": round(1e3 * ms / max(nlaunch, 1), 2),
             "launches": nlaunch, "level0_algorithmic_GBps": {k: round(v, 1) for k, v in level0_gbs.items()}}
     time_share = {k: round(v[0] / sum(x[0] for x in prof_all.values()), 4) for k, v in prof_all.items() if v[0] > 0}
+    ms_step = 1e3 * mixed["seconds"] / args.steps
+    whole = whole_solve_traffic(ms_step)
+    errors = None if args.mixed_only else {"mixed_f16": fp64_errors(scen, [bs[k] for k in sample], mixed["xs"]),
+                                          "fp32": fp64_errors(scen, [bs[k] for k in sample], fp32["xs"])}
 
     cpu = None
     if not args.no_cpu:
@@ -482,7 +547,6 @@ def main():
 
     if room is not None:   # the GPU run's per-step data served the CPU baseline only
         room.pop("_iters", None)
-        room.pop("_Q", None)
     out = {
         "metric": METRIC,
         "value": round(v_mixed, 2),
@@ -494,7 +558,7 @@ def main():
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f32",
+        "dtype": "mixed: f16 V-cycle storage, f32 CG vectors, f64 dots and coarsest solve",
         "data": "synthetic",
         "config": {"workload": "house 80x50x40 cells at 0.2 m, 40 walls + 10 openings (seed 1), "
                                f"{args.nsys} independent RHS per step, ||r||/||b|| <= 1e-6, x0 = 0",
@@ -511,6 +575,8 @@ def main():
         "status_mixed": mixed["status"],
         "status_fp32": None if fp32 is None else fp32["status"],
         "roofline": roof,
+        "whole_solve_hbm": whole,
+        "error_vs_fp64_oracle": errors,
         "dominant_kernel_by_time": dom,
         "kernel_time_share": time_share,
         "precision_1p28M": precision,
