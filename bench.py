@@ -313,6 +313,53 @@ def fp64_errors_multi(grids, bs, xs, threads=8):
             "systems": len(res)}
 
 
+# algorithmic bytes per cell of the gas-step kernels (one pass over each array
+# a launch touches, 4 B per float, 1 B per flag): see each kernel's comment
+STEP_BYTES = {"rhs": 49, "correct": 52, "energy": 25, "advect": 113, "turbulence": 21, "heat_diffusion": 33,
+              "momentum": 108}
+
+
+def run_step_kernels(ctx, gas, sc, reps=5):
+    """The gas-step kernels on the 160x100x80 house at 0.1 m (1.28M cells),
+    seeded fields: average launch time from HIP events on the library's
+    stream and algorithmic GB/s against the HBM peak."""
+    g = sc.house(1, 160, 100, 80, 0.1)
+    rng = np.random.default_rng(4)
+    s = gas.GasSolver(ctx, g)
+    for f, a in (("u", g.fx), ("v", g.fy), ("w", g.fz)):
+        s.set(f, (0.5 * rng.standard_normal(len(a))).astype(np.float32) * a)
+    s.set("T", (293.15 + 600 * rng.random(g.n)).astype(np.float32))
+    s.set("nu", (1e-3 * rng.random(g.n)).astype(np.float32))
+    s.set("Q", (10 * rng.random(g.n)).astype(np.float32))
+    s.set("S", sc.divergence_field(g, 3))
+    ctx.synchronize()
+    ctx.profile_reset()
+    ctx.profile(True, classes=list(STEP_BYTES))
+    for _ in range(reps):
+        s.rhs(DT)
+        s.correct(DT, report=False)
+        s.energy_and_expansion(DT, report=False)
+        s.advect_scalars(DT)
+        s.turbulence(report=False)
+        s.heat_diffusion(DT)
+        s.momentum(DT)
+    ctx.synchronize()
+    prof = ctx.profile_read()
+    ctx.profile(False)
+    out = {}
+    for k, bpc in STEP_BYTES.items():
+        ms, n = prof.get(k, (0.0, 0))
+        if n == 0:
+            continue
+        per = ms / reps                       # one call (momentum: 3 component + 3 edge launches)
+        gbs = bpc * g.n / (per * 1e-3) / 1e9
+        out[k] = {"B_per_cell": bpc, "us_per_call": round(1e3 * per, 2), "launches_per_call": n // reps,
+                  "GBps": round(gbs, 1), "frac": round(gbs * 1e9 / HBM_PEAK, 3)}
+    out["workload"] = "house 160x100x80 at 0.1 m (1.28M cells), seeded fields, 5 calls of each"
+    s.grid.destroy()
+    return out
+
+
 def run_serving(ctx, window_s=2.0, steps=300):
     """The serving workload: Poisson arrivals at 20 scenarios/s over
     `window_s` seconds, sizes log-uniform 3.6k..1.28M cells (room aspect),
@@ -458,6 +505,10 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--mixed-only", action="store_true",
                     help="time the mixed-precision batch only (for counter runs under rocprofv3)")
+    ap.add_argument("--step-kernels-only", action="store_true",
+                    help="time only the gas-step kernels (for kernel traces under rocprofv3)")
+    ap.add_argument("--batching-only", action="store_true",
+                    help="run only the 64-scenario batching workload (for kernel traces under rocprofv3)")
     ap.add_argument("--copy-gib", type=float, default=0.0,
                     help="also run the plain copy kernel over this many GiB (counter calibration)")
     args = ap.parse_args()
@@ -468,6 +519,13 @@ def main():
 
     from photon_amd import gas
     from photon_amd import scenarios as sc
+
+    if args.step_kernels_only or args.batching_only:
+        ctx = gas.Context(0)
+        out = run_step_kernels(ctx, gas, sc) if args.step_kernels_only else run_batching(ctx, gas, sc)
+        ctx.close()
+        print(json.dumps(out))
+        return
 
     scen = sc.house(1)
     n = scen.n
@@ -497,6 +555,7 @@ def main():
     batching = None if args.mixed_only else run_batching(ctx, gas, sc)
     serving = None if args.mixed_only else run_serving(ctx)
     room = None if args.mixed_only else run_room(ctx, gas, sc)
+    step_kernels = None if args.mixed_only else run_step_kernels(ctx, gas, sc)
 
     # roofline of the dominant kernel, from HIP events over the timed steps
     # (recorded on the library's stream, around each level-0 launch); the
@@ -583,6 +642,7 @@ def main():
         "batching_64_scenarios": batching,
         "serving_poisson": serving,
         "room_c1": room,
+        "step_kernels_1p28M": step_kernels,
         "cpu_baseline": cpu,
     }
     ctx.close()

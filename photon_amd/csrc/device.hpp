@@ -222,8 +222,14 @@ __device__ __forceinline__ float block_max(float v) {
 }
 
 // max of non-negative floats through their bit patterns (order independent)
+// Max of non-negative floats (their bit patterns order as unsigned ints).  A
+// block whose value does not exceed the one already stored skips the atomic:
+// thousands of blocks updating one address would otherwise queue at the L2
+// (a stale read only costs an unneeded atomic).
 __device__ __forceinline__ void atomic_max_nonneg(float* addr, float v) {
-    atomicMax(reinterpret_cast<unsigned int*>(addr), __float_as_uint(v));
+    unsigned int* a = reinterpret_cast<unsigned int*>(addr);
+    const unsigned int bits = __float_as_uint(v);
+    if (bits > __hip_atomic_load(a, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) atomicMax(a, bits);
 }
 
 }  // namespace phg
