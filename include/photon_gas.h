@@ -199,6 +199,15 @@ int phg_batch_destroy(phg_ctx *ctx, int batch);
  * stats: nsys entries (may be NULL). */
 int phg_batch_solve(phg_ctx *ctx, int batch, const float *const *b, float *const *x, const phg_solve_params *params,
                     phg_solve_stats *stats);
+/* Captures (without launching) the whole-solve graph phg_batch_solve replays
+ * for use_graph = 2 (the set-up and `iterations` CG iterations, refine as in
+ * phg_solve_params) and checks every node against the device's launch
+ * limits: a kernel node, a non-empty grid, threads per block, LDS, grid
+ * size, no scratch.  nodes_out: the node count; msg: a summary (may be NULL).
+ * Returns PHG_EINVAL naming the first node outside the limits.  No
+ * counterpart in the reference (a check on this library's own launches). */
+int phg_batch_graph_check(phg_ctx *ctx, int batch, int iterations, int refine, int *nodes_out, char *msg,
+                          size_t msg_len);
 /* one system: convenience over a cached one-system batch of `grid` */
 int phg_solve(phg_ctx *ctx, int grid, const float *b, float *x, const phg_solve_params *params,
               phg_solve_stats *stats);

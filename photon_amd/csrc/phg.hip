@@ -547,6 +547,19 @@ void demean_batch(phg_ctx* ctx, BatchDev* B, int which = 0) {
     launch(ctx, K_ZONE, k_zone_sub, B->tab_len[0], 0, S, (const int2*)B->tab[0], which);
 }
 
+// r_0 = b - A x_0, the stop test at iteration 0, z_0 = M^-1 r_0, r.z; a
+// restart projects its residual onto the range of A first.  hi: residuals
+// formed from x accumulate in FP64.
+void solve_setup(phg_ctx* ctx, BatchDev* B, bool restart, int hi) {
+    const DSys* S = B->d_sys;
+    if (B->binary) launch(ctx, K_RESID_INIT, k_resid_init<Mask01>, B->tab_len[0], 0, S, B->tab[0], hi);
+    else launch(ctx, K_RESID_INIT, k_resid_init<float>, B->tab_len[0], 0, S, B->tab[0], hi);
+    if (restart) demean_batch(ctx, B, 1);
+    launch(ctx, K_FIN, k_fin_init, B->nsys, 0, S, B->tab_sys);
+    vcycle(ctx, B, 1);
+    launch(ctx, K_FIN, k_fin_rz0, B->nsys, 0, S, B->tab_sys);
+}
+
 int upload_sys(phg_ctx* ctx, BatchDev* B) {
     HIP_OK(hipMemcpyAsync(B->d_sys, B->h_sys.data(), sizeof(DSys) * B->nsys, hipMemcpyHostToDevice, ctx->stream));
     return 0;
@@ -1376,20 +1389,10 @@ int phg_batch_solve(phg_ctx* ctx, int batch, const float* const* b, float* const
         if (!rc) rc = reset_state(ctx, B, P);
         if (rc) return rc;
         const DSys* S = B->d_sys;
-        // r_0 = b - A x_0, the stop test at iteration 0, z_0 = M^-1 r_0, r.z;
-        // a restart projects its residual onto the range of A first
-        auto setup = [&](bool restart) {
-            // with refinement on, every residual formed from x (a warm start's
-            // first one as well as a restart's) accumulates in FP64: an FP32 one
-            // is noise once it nears eps |A| |x|, where a warm start begins
-            const int hi = restart || P->refine > 0 ? 1 : 0;
-            if (B->binary) launch(ctx, K_RESID_INIT, k_resid_init<Mask01>, B->tab_len[0], 0, S, B->tab[0], hi);
-            else launch(ctx, K_RESID_INIT, k_resid_init<float>, B->tab_len[0], 0, S, B->tab[0], hi);
-            if (restart) demean_batch(ctx, B, 1);
-            launch(ctx, K_FIN, k_fin_init, B->nsys, 0, S, B->tab_sys);
-            vcycle(ctx, B, 1);
-            launch(ctx, K_FIN, k_fin_rz0, B->nsys, 0, S, B->tab_sys);
-        };
+        // with refinement on, every residual formed from x (a warm start's
+        // first one as well as a restart's) accumulates in FP64: an FP32 one
+        // is noise once it nears eps |A| |x|, where a warm start begins
+        auto setup = [&](bool restart) { solve_setup(ctx, B, restart, restart || P->refine > 0 ? 1 : 0); };
         const int check = P->check_every > 0 ? P->check_every : 4;
         const bool graph = P->use_graph && !ctx->prof;
         const bool whole = P->use_graph == 2 && !ctx->prof;
@@ -1480,6 +1483,77 @@ int phg_batch_solve(phg_ctx* ctx, int batch, const float* const* b, float* const
         if (ctx->prof) prof_drain(ctx);
         return PHG_OK;
     });
+}
+
+int phg_batch_graph_check(phg_ctx* ctx, int batch, int iterations, int refine, int* nodes_out, char* msg,
+                          size_t msg_len) {
+    if (!ctx || iterations < 0) return PHG_EINVAL;
+    BatchDev* B = get_batch(ctx, batch);
+    if (!B) return fail(ctx, PHG_ESTATE, "unknown batch %d", batch);
+    if (ctx->prof) return fail(ctx, PHG_EINVAL, "profiling is on: launches are not captured then");
+    HIP_OK(hipSetDevice(ctx->device));
+    // the whole-solve graph phg_batch_solve captures for use_graph = 2
+    hipGraph_t g = nullptr;
+    HIP_OK(hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal));
+    solve_setup(ctx, B, false, refine > 0 ? 1 : 0);
+    for (int i = 0; i < iterations; ++i) cg_iteration(ctx, B);
+    HIP_OK(hipStreamEndCapture(ctx->stream, &g));
+    size_t n = 0;
+    hipError_t e = hipGraphGetNodes(g, nullptr, &n);
+    std::vector<hipGraphNode_t> nodes(n);
+    if (e == hipSuccess && n) e = hipGraphGetNodes(g, nodes.data(), &n);
+    hipDeviceProp_t prop;
+    if (e == hipSuccess) e = hipGetDeviceProperties(&prop, ctx->device);
+    std::string bad;
+    size_t kernels = 0, max_lds = 0, max_threads = 0;
+    for (size_t k = 0; e == hipSuccess && k < n && bad.empty(); ++k) {
+        hipGraphNodeType t;
+        e = hipGraphNodeGetType(nodes[k], &t);
+        if (e != hipSuccess) break;
+        if (t != hipGraphNodeTypeKernel) {
+            bad = "node " + std::to_string(k) + " is not a kernel node (type " + std::to_string((int)t) + ")";
+            break;
+        }
+        hipKernelNodeParams kp;
+        e = hipGraphKernelNodeGetParams(nodes[k], &kp);
+        if (e != hipSuccess) break;
+        hipFuncAttributes fa;
+        e = hipFuncGetAttributes(&fa, kp.func);
+        if (e != hipSuccess) break;
+        ++kernels;
+        const size_t threads = (size_t)kp.blockDim.x * kp.blockDim.y * kp.blockDim.z;
+        const size_t lds = fa.sharedSizeBytes + kp.sharedMemBytes;
+        const unsigned long long work = (unsigned long long)kp.gridDim.x * kp.blockDim.x;
+        max_lds = std::max(max_lds, lds);
+        max_threads = std::max(max_threads, threads);
+        char buf[256];
+        if (threads == 0 || kp.gridDim.x == 0 || kp.gridDim.y == 0 || kp.gridDim.z == 0)
+            snprintf(buf, sizeof buf, "node %zu: empty launch (grid %u, block %zu)", k, kp.gridDim.x, threads);
+        else if ((int)threads > fa.maxThreadsPerBlock || (int)threads > prop.maxThreadsPerBlock)
+            snprintf(buf, sizeof buf, "node %zu: %zu threads per block, limit %d", k, threads,
+                     std::min(fa.maxThreadsPerBlock, prop.maxThreadsPerBlock));
+        else if (lds > (size_t)prop.maxSharedMemoryPerMultiProcessor)
+            snprintf(buf, sizeof buf, "node %zu: %zu B of LDS, limit %zu", k, lds,
+                     (size_t)prop.maxSharedMemoryPerMultiProcessor);
+        else if (work > 0xFFFFFFFFull)
+            snprintf(buf, sizeof buf, "node %zu: grid of %llu work-items exceeds 2^32", k, work);
+        else if (fa.localSizeBytes > 0)
+            snprintf(buf, sizeof buf, "node %zu: %zu B of scratch per lane", k, fa.localSizeBytes);
+        else
+            buf[0] = 0;
+        bad = buf;
+    }
+    (void)hipGraphDestroy(g);
+    if (e != hipSuccess) return fail(ctx, PHG_EHIP, "graph inspection: %s", hipGetErrorString(e));
+    if (nodes_out) *nodes_out = (int)n;
+    if (msg && msg_len) {
+        snprintf(msg, msg_len, "%zu kernel nodes of %zu; largest block %zu threads, largest LDS %zu B (device: %d "
+                 "threads, %zu B per CU)%s%s",
+                 kernels, n, max_threads, max_lds, prop.maxThreadsPerBlock, (size_t)prop.maxSharedMemoryPerMultiProcessor,
+                 bad.empty() ? "" : "; ", bad.c_str());
+    }
+    if (!bad.empty()) return fail(ctx, PHG_EINVAL, "%s", bad.c_str());
+    return PHG_OK;
 }
 
 int phg_solve(phg_ctx* ctx, int grid, const float* b, float* x, const phg_solve_params* P, phg_solve_stats* stats) {

@@ -34,7 +34,7 @@ EXPORTS = (
     "phg_grid_plan_build", "phg_grid_plan_free", "phg_grid_create_planned", "phg_zone_mean_noalloc",
     "phg_grid_set_coarse", "phg_grid_get_coarse",
     "phg_stable_dt", "phg_turbulence", "phg_heat_diffusion", "phg_momentum", "phg_step_clear", "phg_add_gas",
-    "phg_combustion", "phg_zone_integrals", "phg_zone_values", "phg_grid_stats",
+    "phg_combustion", "phg_zone_integrals", "phg_zone_values", "phg_grid_stats", "phg_batch_graph_check",
 )
 
 
@@ -125,6 +125,7 @@ def load_library():
     L.phg_zone_integrals.argtypes = [vp, i32, vp, vp]
     L.phg_zone_values.argtypes = [vp, i32, vp, vp, i32]
     L.phg_grid_stats.argtypes = [vp, i32, C.POINTER(f32), C.POINTER(f32), C.POINTER(f32)]
+    L.phg_batch_graph_check.argtypes = [vp, i32, i32, i32, C.POINTER(i32), C.c_char_p, sz]
     L.phg_leakage.argtypes = [vp, i32, vp, vp]
     L.phg_correct.argtypes = [vp, i32, f32, vp, vp, vp, vp, C.POINTER(f32)]
     L.phg_energy_expansion.argtypes = [vp, i32, f32, vp, vp, vp, vp, vp, C.POINTER(f32)]
@@ -402,6 +403,15 @@ class Batch:
         st = (SolveStats * n)()
         self.ctx._check(self.ctx.L.phg_batch_solve(self.ctx.h, self.id, bp, xp, C.byref(params), st))
         return [dict(iters=s.iters, status=s.status, relres=s.relres, maxres=s.maxres) for s in st]
+
+    def graph_check(self, iterations, refine=0):
+        """Capture the whole-solve graph and check its nodes against the
+        device's launch limits (phg_batch_graph_check): (nodes, summary)."""
+        n = C.c_int()
+        buf = C.create_string_buffer(512)
+        self.ctx._check(self.ctx.L.phg_batch_graph_check(self.ctx.h, self.id, iterations, refine, C.byref(n), buf,
+                                                         len(buf)))
+        return n.value, buf.value.decode()
 
     def destroy(self):
         if self.id is not None and self.ctx.h:

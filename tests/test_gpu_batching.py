@@ -169,3 +169,28 @@ def test_autotuner_picks_converging_settings(ctx):
     B.destroy()
     for G in Gs:
         G.destroy()
+
+
+def test_whole_solve_graphs_stay_within_launch_limits(ctx):
+    """Every node of the whole-solve graphs of the 64-scenario batch (the
+    library's default shape and every autotuner candidate, 16 and 40
+    iterations, with and without refinement) is a kernel node within the
+    device's limits on threads per block, LDS, grid size, and uses no
+    scratch (phg_batch_graph_check, captured and inspected, never launched)."""
+    from photon_amd import autotune as at
+    grids = [sc.batch_grid(s) for s in range(64)]
+    Gs = [gas.Grid(ctx, g, precision=gas.PREC_MIXED_F16) for g in grids]
+    shapes = [(None, None)]
+    for kc in at.KCHUNKS:
+        for nu in at.SWEEPS:
+            shapes.append(([at._kc(kc, g.nz) for g in grids], [nu] * len(grids)))
+    for kcs, sws in shapes:
+        B = gas.Batch(ctx, Gs, sys_kchunk=kcs, sys_sweeps0=sws)
+        for iters, refine in ((16, 0), (40, 2)):
+            n, msg = B.graph_check(iters, refine)
+            print(f"kchunk {None if kcs is None else sorted(set(kcs))[:3]} sweeps {sws and sws[0]} "
+                  f"iters {iters}: {msg}")
+            assert n > 20 * iters
+        B.destroy()
+    for G in Gs:
+        G.destroy()
