@@ -166,6 +166,12 @@ void orc_add_gas(int ncell, const int *cells, float dx, float fuel_gen, float ae
 double orc_combustion(const orc_grid *g, float dt, int radiation_on, const orc_species *sp, float *Q, float *Qr,
                       float *hrr, const uint8_t *pilot, uint8_t *lit);
 
+/* leakage (gasphase.cpp:402-498): crack flow, closed zones' background
+ * pressure, zoneDP and the shift of S; returns the number of crack faces */
+int orc_leakage(const orc_grid *g, float dt, const int *zone, int nzones, const uint8_t *zone_open,
+                const float *zone_vol, const float *zoneP, const float *T, const float *Pb, const float *p, float *u,
+                float *v, float *w, float *S, float *zoneDP);
+
 #ifdef __cplusplus
 }
 #endif

@@ -342,3 +342,138 @@ double orc_combustion(const orc_grid *g, float dt, int radiation_on, const orc_s
     free(litNew);
     return total;
 }
+
+/* ---- crack flow and closed zones' background pressure ---------------------
+ * leakage (gasphase.cpp:402-498).  Crack faces (lx/ly/lz set, neither side
+ * solid) carry the orifice flow cd*sqrt(2|dp|/rho_up), capped at 40 m/s.  For
+ * each closed zone the end-of-step background pressure Pt solves
+ *   Pt - P = dt*gamma*P/Vz * (integral of S dV - net crack outflow(Pt))
+ * by 40 bisection steps on [P - 20 kPa, P + 20 kPa] (left at P when the
+ * bracket does not hold).  Each crack velocity is then relaxed halfway to the
+ * orifice velocity of the zone pressures plus the dynamic-pressure term, and
+ * every closed zone gets its rise rate zoneDP and the matching shift of S.
+ * All sums are FP64, in the reference's order: cells ascending, crack faces
+ * x then y then z, each in face order.  zone_vol: each zone's volume, m^3, as
+ * a float.  Returns the number of crack faces. */
+typedef struct { int axis, f, a, b; } orc_crack;
+
+static float crack_speed(float dp, float rho_up) {   /* gasphase.cpp:431-434 */
+    const float sgn = dp >= 0 ? 1.0f : -1.0f;
+    return sgn * fminr(40.0f, 1.0f * sqrtf(2.0f * fabsf(dp) / fmaxr(rho_up, 0.05f)));
+}
+
+int orc_leakage(const orc_grid *g, float dt, const int *zone, int nzones, const uint8_t *zone_open,
+                const float *zone_vol, const float *zoneP, const float *T, const float *Pb, const float *p, float *u,
+                float *v, float *w, float *S, float *zoneDP) {
+    const int nx = g->nx, ny = g->ny, nz = g->nz, N = nx * ny * nz;
+    const float dx = g->dx, V = dx * dx * dx, A = dx * dx;
+    const float gamma = CP / (CP - RS), rho_amb = P0 / (RS * T0);
+    double *dint = (double *)calloc((size_t)nzones + 1, sizeof(double));
+    for (int c = 0; c < N; ++c)
+        if (zone[c] >= 0) dint[zone[c]] += S[c] * V;
+    /* the crack faces */
+    const int nf[3] = {(nx + 1) * ny * nz, nx * (ny + 1) * nz, nx * ny * (nz + 1)};
+    const uint8_t *lk[3] = {g->lx, g->ly, g->lz};
+    const float *fo[3] = {g->fx, g->fy, g->fz};
+    float *vel[3] = {u, v, w};
+    int ncr = 0;
+    for (int ax = 0; ax < 3; ++ax)
+        if (lk[ax])
+            for (int f = 0; f < nf[ax]; ++f) ncr += lk[ax][f] != 0;
+    orc_crack *cr = (orc_crack *)malloc(sizeof(orc_crack) * (size_t)(ncr + 1));
+    int m = 0;
+    for (int ax = 0; ax < 3; ++ax) {
+        if (!lk[ax]) continue;
+        const int ex = nx + (ax == 0), ey = ny + (ax == 1);
+        const int lim = ax == 0 ? nx : ax == 1 ? ny : nz;
+        for (int f = 0; f < nf[ax]; ++f) {
+            if (!lk[ax][f]) continue;
+            const int i = f % ex, j = (f / ex) % ey, k = f / (ex * ey);
+            const int pos = ax == 0 ? i : ax == 1 ? j : k;
+            const int step = ax == 0 ? 1 : ax == 1 ? nx : nx * ny;
+            const int cb = pos < lim ? i + nx * (j + ny * k) : -1;
+            const int ca = pos > 0 ? i + nx * (j + ny * k) - step : -1;
+            if ((ca >= 0 && g->blocked[ca]) || (cb >= 0 && g->blocked[cb])) continue;
+            cr[m].axis = ax; cr[m].f = f; cr[m].a = ca; cr[m].b = cb;
+            ++m;
+        }
+    }
+    ncr = m;
+#define ZP(c, P) ((c) >= 0 && zone[c] >= 0 ? (P)[zone[c]] : P0)
+#define RHO(c) ((c) >= 0 ? Pb[c] / (RS * T[c]) : rho_amb)
+    /* each zone's cracks to another zone or outside, in crack order */
+    int *cnt = (int *)calloc((size_t)nzones + 1, sizeof(int));
+    for (int q = 0; q < ncr; ++q) {
+        const int za = cr[q].a >= 0 ? zone[cr[q].a] : -1, zb = cr[q].b >= 0 ? zone[cr[q].b] : -1;
+        if (za == zb) continue;
+        if (za >= 0) cnt[za]++;
+        if (zb >= 0) cnt[zb]++;
+    }
+    int *ptr = (int *)calloc((size_t)nzones + 2, sizeof(int));
+    for (int z = 0; z < nzones; ++z) ptr[z + 1] = ptr[z] + cnt[z];
+    int *lst = (int *)malloc(sizeof(int) * (size_t)(ptr[nzones] + 1));
+    memset(cnt, 0, sizeof(int) * (size_t)(nzones + 1));
+    for (int q = 0; q < ncr; ++q) {
+        const int za = cr[q].a >= 0 ? zone[cr[q].a] : -1, zb = cr[q].b >= 0 ? zone[cr[q].b] : -1;
+        if (za == zb) continue;
+        if (za >= 0) lst[ptr[za] + cnt[za]++] = q;
+        if (zb >= 0) lst[ptr[zb] + cnt[zb]++] = q;
+    }
+    float *Pnew = (float *)malloc(sizeof(float) * (size_t)(nzones + 1));
+    for (int z = 0; z < nzones; ++z) Pnew[z] = zoneP[z];
+    for (int z = 0; z < nzones; ++z) {
+        if (zone_open[z]) continue;
+        const float P = zoneP[z], Vz = zone_vol[z];
+        float lo = P - 20000.0f, hi = P + 20000.0f;
+        float rl = 0, rh = 0;
+        for (int pass = 0; pass < 42; ++pass) {
+            const float Pt = pass == 0 ? lo : pass == 1 ? hi : 0.5f * (lo + hi);
+            double outflow = 0;
+            for (int e = ptr[z]; e < ptr[z + 1]; ++e) {
+                const orc_crack *c = &cr[lst[e]];
+                const int mine_a = c->a >= 0 && zone[c->a] == z;
+                const int other = mine_a ? c->b : c->a;
+                const float dp = Pt - ZP(other, zoneP);
+                const int up = dp >= 0 ? (mine_a ? c->a : c->b) : other;
+                outflow += fo[c->axis][c->f] * A * fabsf(crack_speed(dp, RHO(up))) * (dp >= 0 ? 1 : -1);
+            }
+            const float r = Pt - P - dt * gamma * P / Vz * (float)(dint[z] - outflow);
+            if (pass == 0) rl = r;
+            else if (pass == 1) {
+                rh = r;
+                if (rl > 0 || rh < 0) break;
+            } else if (r > 0) hi = Pt;
+            else lo = Pt;
+        }
+        Pnew[z] = (rl > 0 || rh < 0) ? P : 0.5f * (lo + hi);
+    }
+    /* relaxed crack velocities and each zone's net outflow */
+    double *out = (double *)calloc((size_t)nzones + 1, sizeof(double));
+    for (int q = 0; q < ncr; ++q) {
+        const orc_crack *c = &cr[q];
+        const float pa = ZP(c->a, Pnew), pb = ZP(c->b, Pnew);
+        const float ra = RHO(c->a), rb = RHO(c->b);
+        const float Ha = c->a >= 0 ? p[c->a] : 0, Hb = c->b >= 0 ? p[c->b] : 0;
+        const float dp = (pa - pb) + 0.5f * (ra + rb) * (Ha - Hb);
+        const float un = crack_speed(dp, dp >= 0 ? ra : rb);
+        float *vv = vel[c->axis];
+        vv[c->f] = 0.5f * (vv[c->f] + un);
+        const float flow = fo[c->axis][c->f] * A * vv[c->f];
+        if (c->a >= 0 && zone[c->a] >= 0) out[zone[c->a]] += flow;
+        if (c->b >= 0 && zone[c->b] >= 0) out[zone[c->b]] -= flow;
+    }
+#undef ZP
+#undef RHO
+    float *shift = (float *)calloc((size_t)nzones + 1, sizeof(float));
+    for (int z = 0; z < nzones; ++z) {
+        zoneDP[z] = 0;
+        if (zone_open[z]) continue;
+        const float Vz = zone_vol[z];
+        zoneDP[z] = gamma * zoneP[z] * (float)(dint[z] - out[z]) / Vz;
+        shift[z] = (float)(dint[z] - out[z]) / Vz;
+    }
+    for (int c = 0; c < N; ++c)
+        if (zone[c] >= 0 && !zone_open[zone[c]]) S[c] -= shift[zone[c]];
+    free(shift); free(out); free(Pnew); free(lst); free(ptr); free(cnt); free(cr); free(dint);
+    return ncr;
+}

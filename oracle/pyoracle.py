@@ -90,6 +90,8 @@ def lib():
         L.orc_combustion.restype = C.c_double
         L.orc_combustion.argtypes = [C.POINTER(OrcGrid), C.c_float, C.c_int, C.POINTER(OrcSpecies), _f32p, _f32p,
                                      _f32p, _u8p, _u8p]
+        L.orc_leakage.argtypes = [C.POINTER(OrcGrid), C.c_float, _i32p, C.c_int, _u8p, _f32p, _f32p, _f32p, _f32p,
+                                  _f32p, _f32p, _f32p, _f32p, _f32p, _f32p]
         _lib = L
     return _lib
 
@@ -279,6 +281,27 @@ def momentum(grid, dt, u, v, w, T, Pb, nu, p, wd=None, leaks=None):
     return outs
 
 
+def zone_volumes(grid):
+    """Each zone's volume as the reference keeps it (gasphase.cpp:155-156):
+    the cell count times dx^3 in FP64, then read as a float."""
+    zone = grid.zone
+    cnt = np.bincount(zone[zone >= 0], minlength=len(grid.zone_open)).astype(np.float64)
+    dxd = np.float64(np.float32(grid.dx))
+    return (cnt * dxd * dxd * dxd).astype(np.float32)
+
+
+def leakage(grid, dt, leaks, zoneP, T, Pb, p, u, v, w, S):
+    """leakage(dt) restated; u, v, w and S are updated in place.  Returns
+    (zoneDP, number of crack faces)."""
+    zoneP = np.ascontiguousarray(zoneP, np.float32)
+    zdp = np.zeros(max(len(zoneP), 1), np.float32)
+    zo = np.ascontiguousarray(grid.zone_open, np.uint8)
+    n = lib().orc_leakage(C.byref(orc_grid(grid, leaks)), dt, np.ascontiguousarray(grid.zone, np.int32),
+                          len(zoneP), zo, zone_volumes(grid), zoneP,
+                          *[np.ascontiguousarray(a, np.float32) for a in (T, Pb, p)], u, v, w, S, zdp)
+    return zdp[:len(zoneP)], n
+
+
 def species(f):
     """An OrcSpecies over a dict of float32 cell arrays (updated in place)."""
     sp = OrcSpecies(*[_ptr(f[n]) for n in SPECIES])
@@ -393,6 +416,8 @@ class RefSolver:
             L.ref_step.argtypes = [C.c_void_p, C.c_float, C.c_float, C.c_int]
             L.ref_scalars.argtypes = [C.c_void_p, _f32p]
             L.ref_zone_pressure.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p]
+            L.ref_set_zone_pressure.argtypes = [C.c_void_p, _f32p]
+            L.ref_leak_faces.argtypes = [C.c_void_p]
             L.ref_wind_at.restype = C.c_float
             L.ref_wind_at.argtypes = [C.c_void_p, C.c_int, C.c_int, C.c_float]
             cls._libs[path] = L
@@ -507,6 +532,12 @@ class RefSolver:
         zp, zdp = np.zeros(max(n, 1), np.float32), np.zeros(max(n, 1), np.float32)
         self._L.ref_zone_pressure(self.h, zp.ctypes.data, zdp.ctypes.data)
         return zp[:n], zdp[:n]
+
+    def set_zone_pressure(self, zoneP):
+        self._L.ref_set_zone_pressure(self.h, np.ascontiguousarray(zoneP, np.float32))
+
+    def leak_faces(self):
+        return self._L.ref_leak_faces(self.h)
 
     def set_wind(self, speed, from_deg, exponent=0.25):
         self._L.ref_set_wind(self.h, speed, from_deg, exponent)

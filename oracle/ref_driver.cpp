@@ -185,6 +185,19 @@ int ref_zone_pressure(void* h, float* zoneP, float* zoneDP) {
     return (int)s->zoneP.size();
 }
 
+// set the zone background pressures (and each zone's Pb), as a step leaves
+// them (solver.cpp:258-263); returns the zone count
+int ref_set_zone_pressure(void* h, const float* zoneP) {
+    Solver* s = static_cast<Solver*>(h);
+    std::copy(zoneP, zoneP + s->zoneP.size(), s->zoneP.begin());
+    for (int c = 0; c < s->N; ++c)
+        if (s->zone[c] >= 0) s->Pb[c] = s->zoneP[s->zone[c]];
+    return (int)s->zoneP.size();
+}
+
+// crack faces the last leakage handled (Solver::leakFaces, solver.h:279)
+int ref_leak_faces(void* h) { return static_cast<Solver*>(h)->leakFaces; }
+
 // Wind at the open boundaries (Solver::wind, solver.h:106-111); the
 // hierarchy is rebuilt on the next project (its level 0 masks wind sides,
 // pressure_mg.cpp:31-46).

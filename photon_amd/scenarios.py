@@ -20,6 +20,12 @@ Workloads (sizes in cells at the stated cell size):
             openings 0.8-1.6 m wide and 2.0 m high cut into them.
   batch     grids uniform between 20x15x12 and 80x50x40 cells, walls scaled to
             size, for the batched solves.
+  cracked   the room's 20x15x12 box split at x = 2.0 m by a thin partition:
+            every wall, the ceiling and the partition are intact barriers
+            with crack leakage, the partition's door is closed (undercut
+            leakage), the right-hand room has an open window and a
+            free-standing leaky panel; the left room is a closed zone whose
+            only exits are cracks.
 """
 from __future__ import annotations
 
@@ -92,6 +98,10 @@ class Grid:
     ay: np.ndarray
     az: np.ndarray
     burner_cells: np.ndarray | None = None
+    # crack (leak) faces (uint8 per face: lx_, ly_, lz_), or None for none
+    lx: np.ndarray | None = None
+    ly: np.ndarray | None = None
+    lz: np.ndarray | None = None
     # (zone int32 [N], zone_open uint8 [nzones]), formed on first use
     _zones: tuple | None = dataclasses.field(default=None, init=False, repr=False, compare=False)
 
@@ -110,6 +120,13 @@ class Grid:
     @property
     def n(self) -> int:
         return self.nx * self.ny * self.nz
+
+    @property
+    def leaks(self):
+        """(lx, ly, lz) crack flags, or None when the grid has no cracks."""
+        if self.lx is None:
+            return None
+        return self.lx, self.ly, self.lz
 
     @property
     def shape(self):
@@ -278,6 +295,83 @@ def house(seed: int, nx: int = 80, ny: int = 50, nz: int = 40, dx: float = 0.2,
         sz = zc < 2.0
         blocked &= ~(sz[:, None, None] & (sy[:, None] & sx[None, :])[None, :, :])
     return _finish(blocked.astype(np.uint8), dx)
+
+
+LEAK_OPEN = 0.05   # open fraction at or above which an intact barrier is an opening (solver.cpp:20)
+
+
+def barrier_face(leak: float, intact: bool = True):
+    """(open fraction, matrix coefficient, crack flag) of a gas-gas face that
+    carries a barrier whose open fraction while intact is `leak`
+    (Solver::updateFace, src/sim/solver.cpp:82-100): a failed or opened
+    barrier is fully open; an intact one is open by `leak`, and below
+    LEAK_OPEN that opening is a crack whose flow is the orifice equation's,
+    so it leaves the pressure matrix (coefficient 0)."""
+    f = 1.0 if not intact else float(np.float32(leak))
+    crack = intact and 0.0 < f < LEAK_OPEN
+    a = f if f > 0 and not crack else 0.0
+    return f, a, crack
+
+
+def cracked(dx: float = 0.2, wall_leak: float = 0.0003, ceiling_leak: float = 0.0002, door_leak: float = 0.01,
+            panel_leak: float = 0.0005) -> Grid:
+    """Two rooms joined only by cracks, one of them open to ambient.
+
+    Box 4.0 x 3.0 x 2.4 m; the ground is closed.  Walls (x = 0, x = 4, y = 0,
+    y = 3) and ceiling are intact barriers of leakage `wall_leak` and
+    `ceiling_leak`; a partition at x = 2.0 m (leakage `wall_leak`) has a
+    closed 0.8 x 2.0 m door (y 1.2..2.0 m, z 0..2.0 m, leakage `door_leak`).
+    The right room has a 1.0 x 1.0 m window open to ambient in the y = 3 wall
+    (x 2.6..3.6 m, z 1.0..2.0 m) and a free-standing 0.8 x 1.0 m panel on the
+    y = 0.8 m plane (x 2.6..3.4 m, leakage `panel_leak`) whose two sides are
+    in the same zone.  Burner cells: 0.4 x 0.4 m at the left room's floor
+    centre."""
+    nx, ny, nz = round(4.0 / dx), round(3.0 / dx), round(2.4 / dx)
+    n = nx * ny * nz
+    fx = np.ones((nz, ny, nx + 1), np.float32)
+    fy = np.ones((nz, ny + 1, nx), np.float32)
+    fz = np.ones((nz + 1, ny, nx), np.float32)
+    ax, ay, az = fx.copy(), fy.copy(), fz.copy()
+    lx = np.zeros(fx.shape, np.uint8)
+    ly = np.zeros(fy.shape, np.uint8)
+    lz = np.zeros(fz.shape, np.uint8)
+
+    def put(f, a, l, sel, leak):
+        fv, av, cv = barrier_face(leak)
+        f[sel], a[sel], l[sel] = fv, av, cv
+
+    xc = (np.arange(nx) + 0.5) * dx
+    yc = (np.arange(ny) + 0.5) * dx
+    zc = (np.arange(nz) + 0.5) * dx
+    put(fx, ax, lx, (slice(None), slice(None), 0), wall_leak)
+    put(fx, ax, lx, (slice(None), slice(None), nx), wall_leak)
+    put(fy, ay, ly, (slice(None), 0, slice(None)), wall_leak)
+    put(fy, ay, ly, (slice(None), ny, slice(None)), wall_leak)
+    put(fz, az, lz, (nz,), ceiling_leak)
+    fz[0], az[0], lz[0] = 0.0, 0.0, 0                       # ground
+    ip = round(2.0 / dx)
+    put(fx, ax, lx, (slice(None), slice(None), ip), wall_leak)
+    door = (zc[:, None] < 2.0) & (yc[None, :] > 1.2) & (yc[None, :] < 2.0)
+    fv, av, cv = barrier_face(door_leak)
+    fx[:, :, ip] = np.where(door, fv, fx[:, :, ip])
+    ax[:, :, ip] = np.where(door, av, ax[:, :, ip])
+    lx[:, :, ip] = np.where(door, np.uint8(cv), lx[:, :, ip])
+    win = (np.abs(zc[:, None] - 1.5) < 0.5) & (np.abs(xc[None, :] - 3.1) < 0.5)
+    fy[:, ny, :] = np.where(win, 1.0, fy[:, ny, :])
+    ay[:, ny, :] = np.where(win, 1.0, ay[:, ny, :])
+    ly[:, ny, :] = np.where(win, np.uint8(0), ly[:, ny, :])
+    jp = round(0.8 / dx)
+    panel = (zc[:, None] < 1.0) & (np.abs(xc[None, :] - 3.0) < 0.4)
+    fv, av, cv = barrier_face(panel_leak)
+    fy[:, jp, :] = np.where(panel, fv, fy[:, jp, :])
+    ay[:, jp, :] = np.where(panel, av, ay[:, jp, :])
+    ly[:, jp, :] = np.where(panel, np.uint8(cv), ly[:, jp, :])
+    g = Grid(nx, ny, nz, float(dx), np.zeros(n, np.uint8), fx.ravel(), fy.ravel(), fz.ravel(), ax.ravel(),
+             ay.ravel(), az.ravel(), lx=lx.ravel(), ly=ly.ravel(), lz=lz.ravel())
+    bi = np.nonzero(np.abs(xc - 1.0) < 0.2)[0]
+    bj = np.nonzero(np.abs(yc - 1.4) < 0.2)[0]
+    g.burner_cells = np.array([i + nx * j for j in bj for i in bi], np.int32)
+    return g
 
 
 def batch_grid(seed: int) -> Grid:

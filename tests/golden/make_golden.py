@@ -11,7 +11,9 @@ fixed number of CG iterations (pressure, face velocities, pressureIters,
 pressureResidual, maxVel), energyAndExpansion and advectScalars on seeded
 fields; ref_<case>_phases.npz turbulence, heatDiffusion, momentum, burner
 injection + combustion and stableDt on seeded fields; ref_<case>_step.npz a
-whole-step trajectory (Solver::step) from still air with a burner.
+whole-step trajectory (Solver::step) from still air with a burner;
+ref_cracked_leakage.npz leakage (crack flow and closed-zone pressure) on
+seeded fields of the cracked two-room grid.
     python tests/golden/make_golden.py [room walls wind_phases room_step ...]
 """
 import os
@@ -27,12 +29,15 @@ from oracle import pyoracle as po  # noqa: E402
 from photon_amd import scenarios as sc  # noqa: E402
 
 DT = 0.01
+# two rooms joined only by cracks (every wall leaks), one closed zone
+CRACKED = lambda: sc.cracked()  # noqa: E731
 CASES = {
     "room": lambda: sc.room(),
     "walls": lambda: sc.house(7, 24, 16, 12),
     # the room with a window in the y-high wall, wind from the north: the
     # window side takes inflow, so windAt closes it for the pressure matrix
     "wind": lambda: sc.room(window=True),
+    "cracked": lambda: CRACKED(),
 }
 # Solver::wind (speed m/s at 10 m, degrees it blows from, profile exponent)
 WIND = {"wind": (3.0, 0.0, 0.25)}
@@ -64,7 +69,7 @@ def scalar_inputs(g, seed):
 
 def make(name, g, K=6):
     out = {}
-    R = po.RefSolver(g, threads=1, pin=True)
+    R = ref_solver(g)
     if name in WIND:
         R.set_wind(*WIND[name])
         out["wind"] = np.array(WIND[name], np.float32)
@@ -117,6 +122,7 @@ STEP_CASES = {
     "room": (lambda: sc.room(), None, (1.8, 1.2, 2.2, 1.6), 100e3, 20),
     "walls": (lambda: sc.house(7, 24, 16, 12), None, (2.0, 1.0, 2.6, 1.4), 20e3, 20),
     "wind": (lambda: sc.room(window=True), (3.0, 30.0, 0.25), (1.8, 1.2, 2.2, 1.6), 100e3, 20),
+    "cracked": (CRACKED, None, (0.8, 1.2, 1.2, 1.6), 50e3, 20),
 }
 STEP_DTMAX, STEP_TOL, STEP_MAXIT = 0.01, 0.02, 80
 SPECIES = ("Yo", "Yf", "Ys", "Ya", "Yp", "Yc", "Yw", "Yco", "Yai")
@@ -142,11 +148,59 @@ def phase_inputs(g, seed):
     return u.astype(np.float32), v.astype(np.float32), w.astype(np.float32), T, Pb, p, Q0, sp, lit, pilot
 
 
+def ref_solver(g):
+    """The reference solver on grid g, with the grid's crack faces."""
+    R = po.RefSolver(g, threads=1, pin=True)
+    if g.leaks is not None:
+        for nm, a in zip(("lx", "ly", "lz"), g.leaks):
+            R.set_bytes(nm, a)
+    return R
+
+
+LEAK_CASES = {
+    # (seed, closed-zone pressure offset Pa, scale of S in the closed zone 1/s)
+    "lk": (17, 15.0, 0.25),
+    # large source: the bisection bracket fails and zoneP stays; crack
+    # speeds reach the 40 m/s cap
+    "lk2": (23, 2500.0, 20.0),
+}
+
+
+def make_leakage(name, g):
+    """leakage(dt) of the reference: crack flow, closed zones' background
+    pressure, zoneDP and the shift of S, on seeded fields."""
+    out = {}
+    for key, (seed, dP, sS) in LEAK_CASES.items():
+        R = ref_solver(g)
+        rng = np.random.default_rng(seed)
+        zone, zopen = g.zone, g.zone_open
+        zp = np.where(zopen == 0, np.float32(101325.0 + dP), np.float32(101325.0)).astype(np.float32)
+        R.set_zone_pressure(zp)
+        T = (293.15 + 400 * rng.random(g.n)).astype(np.float32)
+        S = np.where(zone >= 0, sS * rng.random(g.n), 0).astype(np.float32)
+        p = (2 * rng.standard_normal(g.n)).astype(np.float32)
+        u = (0.3 * rng.standard_normal(len(g.fx))).astype(np.float32) * g.fx
+        v = (0.3 * rng.standard_normal(len(g.fy))).astype(np.float32) * g.fy
+        w = (0.3 * rng.standard_normal(len(g.fz))).astype(np.float32) * g.fz
+        for f, a in (("T", T), ("S", S), ("p", p), ("u", u), ("v", v), ("w", w)):
+            R.set(f, a)
+            out[f"{key}_in_{f}"] = a
+        out[f"{key}_in_Pb"] = R.get("Pb", g.n)
+        out[f"{key}_in_zoneP"] = zp
+        R.leakage(DT)
+        for f, n in (("u", len(g.fx)), ("v", len(g.fy)), ("w", len(g.fz)), ("S", g.n)):
+            out[f"{key}_{f}"] = R.get(f, n)
+        out[f"{key}_zoneDP"] = R.zone_pressure()[1]
+        out[f"{key}_faces"] = np.int32(R.leak_faces())
+        print(name, key, "crack faces", int(out[f"{key}_faces"]), "zoneDP", out[f"{key}_zoneDP"])
+    np.savez_compressed(os.path.join(HERE, f"ref_{name}_leakage.npz"), **out)
+
+
 def make_phases(name, g, K=0):
     """turbulence, heatDiffusion, momentum, burner injection + combustion and
     stableDt of the reference on seeded inputs."""
     out = {}
-    R = po.RefSolver(g, threads=1, pin=True)
+    R = ref_solver(g)
     if name in STEP_CASES and STEP_CASES[name][1] is not None:
         R.set_wind(*STEP_CASES[name][1])
         out["wind"] = np.array(STEP_CASES[name][1], np.float32)
@@ -191,7 +245,7 @@ def make_step(name):
     """A short whole-step trajectory of Solver::step from still air with a burner."""
     gf, wind, rect, peak, nsteps = STEP_CASES[name]
     g = gf()
-    R = po.RefSolver(g, threads=1, pin=True)
+    R = ref_solver(g)
     out = {}
     if wind is not None:
         R.set_wind(*wind)
@@ -224,9 +278,12 @@ if __name__ == "__main__":
     if not po.ref_available(pin=True):
         sys.exit("oracle/_ref/libphoton_ref_pin.so not built (needs the reference checkout): make -C oracle")
     names = sys.argv[1:] or list(CASES) + [n + "_phases" for n in STEP_CASES] + [n + "_step" for n in STEP_CASES]
+    names = names + ([] if sys.argv[1:] else ["cracked_leakage"])
     for name in names:
         if name.endswith("_phases"):
             make_phases(name[:-7], CASES[name[:-7]]())
+        elif name.endswith("_leakage"):
+            make_leakage(name[:-8], STEP_CASES[name[:-8]][0]())
         elif name.endswith("_step"):
             make_step(name[:-5])
         else:

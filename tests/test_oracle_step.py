@@ -6,7 +6,9 @@ Pins: bit-exact agreement with the golden vectors written by the reference's
 own Solver (tests/golden/ref_<case>_phases.npz, room / walled house / windy
 room), and properties the physics fixes independently of any restatement:
 still air stays still, a uniform flow has no sub-grid energy, conduction in a
-closed box conserves heat, burning conserves element mass shares."""
+closed box conserves heat, burning conserves element mass shares; crack
+flow (leakage) against the reference's own leakage on the cracked two-room
+grid, and its direction and balance from the orifice law."""
 import os
 
 import numpy as np
@@ -17,7 +19,8 @@ from photon_amd import scenarios as sc
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 DT = 0.01
-CASES = {"room": sc.room, "walls": lambda: sc.house(7, 24, 16, 12), "wind": lambda: sc.room(window=True)}
+CASES = {"room": sc.room, "walls": lambda: sc.house(7, 24, 16, 12), "wind": lambda: sc.room(window=True),
+         "cracked": sc.cracked}
 SPECIES = ("Yo", "Yf", "Ys", "Ya", "Yp", "Yc", "Yw", "Yco", "Yai")
 
 
@@ -57,7 +60,7 @@ def test_heat_diffusion_matches_golden(case):
 def test_momentum_matches_golden(case):
     _, g, G = case
     u, v, w = po.momentum(g, DT, G["in_u"], G["in_v"], G["in_w"], G["in_T"], G["in_Pb"], G["tu_nu"], G["in_p"],
-                          wind_of(G))
+                          wind_of(G), leaks=g.leaks)
     assert bits(u, G["mo_u"]) and bits(v, G["mo_v"]) and bits(w, G["mo_w"])
 
 
@@ -143,3 +146,91 @@ def test_burning_conserves_element_shares():
     assert np.allclose(carbon, 0.597 * dfuel, rtol=1e-4)
     assert np.allclose(Q, dfuel * 25e6 * (101325.0 / (287.0 * 1200.0)) * 0.2 ** 3, rtol=1e-4)
     assert tot > 0 and lit.all()
+
+
+# ---- crack flow and closed-zone pressure (leakage, gasphase.cpp:402-498) ----
+LEAK = np.load(os.path.join(HERE, "golden", "ref_cracked_leakage.npz"))
+
+
+def leak_run(g, G, key):
+    ins = {f: G[f"{key}_in_{f}"].copy() for f in ("u", "v", "w", "S")}
+    zdp, nf = po.leakage(g, DT, g.leaks, G[f"{key}_in_zoneP"], G[f"{key}_in_T"], G[f"{key}_in_Pb"],
+                         G[f"{key}_in_p"], ins["u"], ins["v"], ins["w"], ins["S"])
+    return ins, zdp, nf
+
+
+@pytest.mark.parametrize("key", ["lk", "lk2"])
+def test_leakage_matches_golden(key):
+    g = sc.cracked()
+    out, zdp, nf = leak_run(g, LEAK, key)
+    assert nf == int(LEAK[f"{key}_faces"]) == sum(int(a.sum()) for a in g.leaks)
+    for f in ("u", "v", "w", "S"):
+        assert bits(out[f], LEAK[f"{key}_{f}"]), f
+    assert bits(zdp, LEAK[f"{key}_zoneDP"])
+    # only crack faces move
+    for f, lk in zip("uvw", g.leaks):
+        moved = out[f] != LEAK[f"{key}_in_{f}"]
+        assert not (moved & (lk == 0)).any()
+
+
+def face_cells(g, axis):
+    """(low-side cell, high-side cell) of every face of an axis, -1 outside."""
+    nx, ny, nz = g.shape
+    ex, ey, ez = nx + (axis == 0), ny + (axis == 1), nz + (axis == 2)
+    k, j, i = np.meshgrid(np.arange(ez), np.arange(ey), np.arange(ex), indexing="ij")
+    pos = (i, j, k)[axis].ravel()
+    lim = (nx, ny, nz)[axis]
+    step = (1, nx, nx * ny)[axis]
+    c = (i + nx * (j + ny * k)).ravel()
+    cb = np.where(pos < lim, c, -1)
+    ca = np.where(pos > 0, c - step, -1)
+    return ca, cb
+
+
+def test_leakage_pressurised_zone_vents_outward():
+    """A closed zone above ambient with still air, no source and no dynamic
+    pressure: every crack blows out of it (positive velocity is low side to
+    high side), its rise rate is negative, S in it rises uniformly by the
+    vented volume per unit volume, and the open zone is untouched."""
+    g = sc.cracked()
+    zone, zo = g.zone, g.zone_open
+    z = int(np.flatnonzero(zo == 0)[0])
+    zp = np.where(zo == 0, np.float32(101325.0 + 40.0), np.float32(101325.0)).astype(np.float32)
+    T = np.full(g.n, 293.15, np.float32)
+    Pb = np.where(zone >= 0, zp[np.maximum(zone, 0)], np.float32(101325.0)).astype(np.float32)
+    u, v, w = (np.zeros(len(a), np.float32) for a in (g.fx, g.fy, g.fz))
+    S = np.zeros(g.n, np.float32)
+    zdp, nf = po.leakage(g, DT, g.leaks, zp, T, Pb, np.zeros(g.n, np.float32), u, v, w, S)
+    assert nf > 0 and zdp[z] < 0 and all(zdp[k] == 0 for k in np.flatnonzero(zo))
+    for axis, (vel, lk) in enumerate(zip((u, v, w), g.leaks)):
+        ca, cb = face_cells(g, axis)
+        za = np.where(ca >= 0, zone[np.maximum(ca, 0)], -1)
+        zb = np.where(cb >= 0, zone[np.maximum(cb, 0)], -1)
+        crack = lk.astype(bool)
+        assert (vel[crack & (za == z) & (zb != z)] > 0).all()
+        assert (vel[crack & (zb == z) & (za != z)] < 0).all()
+        assert (vel[crack & (za != z) & (zb != z)] == 0).all()
+        assert (vel[~crack] == 0).all()
+    assert np.ptp(S[zone == z]) == 0 and S[zone == z][0] > 0
+    assert (S[zone != z] == 0).all()
+
+
+def test_leakage_without_cracks_is_the_mean_shift():
+    """With no crack faces the closed zones' S loses its volume mean and
+    zoneDP = gamma P0 * integral(S dV) / Vz (the no-crack source shift)."""
+    g = sc.room(door=False)
+    zo = g.zone_open
+    rng = np.random.default_rng(3)
+    S = rng.random(g.n).astype(np.float32) * (g.blocked == 0)
+    S0 = S.copy()
+    zp = np.full(len(zo), 101325.0, np.float32)
+    u, v, w = (np.zeros(len(a), np.float32) for a in (g.fx, g.fy, g.fz))
+    zdp, nf = po.leakage(g, DT, None, zp, np.full(g.n, 293.15, np.float32), np.full(g.n, 101325.0, np.float32),
+                         np.zeros(g.n, np.float32), u, v, w, S)
+    assert nf == 0 and not u.any() and not v.any() and not w.any()
+    z = g.zone == 0
+    Vz = po.zone_volumes(g)[0]
+    dint = np.sum(S0[z].astype(np.float64) * np.float64(np.float32(g.dx) ** 3))
+    assert abs(S[z].astype(np.float64).sum() * float(np.float32(g.dx) ** 3)) < 1e-6 * dint
+    gamma = np.float32(1100.0) / (np.float32(1100.0) - np.float32(287.0))
+    assert abs(zdp[0] - gamma * 101325.0 * dint / Vz) < 1e-5 * abs(zdp[0])
