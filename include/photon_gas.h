@@ -323,6 +323,24 @@ int phg_zone_integrals(phg_ctx *ctx, int grid, const float *S, double *dint);
  * zones: x -= value (subtract = 1, leakage's shift of S, gasphase.cpp:494-498)
  * or x = value (subtract = 0, Pb from zoneP, solver.cpp:262-263). */
 int phg_zone_values(phg_ctx *ctx, int grid, const float *values, float *x, int subtract);
+/* The grid's crack faces: Solver::lx_, ly_, lz_ (solver.h:297), which
+ * Solver::updateFace (solver.cpp:93-100) sets where an intact barrier is open
+ * by less than 0.05 (such a face has matrix coefficient 0).  One byte per
+ * face of each axis (host; NULL: none on that axis); all three NULL clears
+ * them.  A crack with a solid side is ignored (gasphase.cpp:424).  Momentum
+ * keeps crack velocities, which phg_zone_leakage drives. */
+int phg_grid_set_cracks(phg_ctx *ctx, int grid, const uint8_t *lx, const uint8_t *ly, const uint8_t *lz);
+/* Solver::leakage (gasphase.cpp:402-498): each closed zone's end-of-step
+ * background pressure against its cracks (orifice flow, 40-step bisection),
+ * every crack velocity relaxed halfway to its orifice velocity, each closed
+ * zone's rise rate and the matching shift of S.  zoneP (host, one per zone)
+ * the zones' background pressures; zoneDP (host, one per zone, may be NULL)
+ * receives the rise rates (0 for open zones); leak_faces (may be NULL) the
+ * number of crack faces (Solver::leakFaces).  T, Pb, p read; u, v, w (crack
+ * faces) and S updated.  Without cracks this is the closed zones' mean
+ * shift of S. */
+int phg_zone_leakage(phg_ctx *ctx, int grid, float dt, const float *zoneP, const float *T, const float *Pb,
+                     const float *p, float *u, float *v, float *w, float *S, float *zoneDP, int *leak_faces);
 
 /* ---- measurement aid: a plain device copy (16-byte vector loads/stores),
  * the calibration kernel for HBM byte counters; bytes must be a multiple of 16 */

@@ -12,7 +12,8 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
 OUT = os.path.join(HERE, "libphoton_gas.so")
 SOURCES = ["phg.hip", "hierarchy.cpp"]
-HEADERS = ["device.hpp", "kernels_pcg.hpp", "kernels_step.hpp", "kernels_transport.hpp", "hierarchy.hpp"]
+HEADERS = ["device.hpp", "kernels_pcg.hpp", "kernels_step.hpp", "kernels_transport.hpp", "kernels_flow.hpp",
+           "kernels_leak.hpp", "hierarchy.hpp"]
 
 FLAGS = [
     "--offload-arch=gfx950",

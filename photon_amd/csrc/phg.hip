@@ -23,6 +23,7 @@
 #include "kernels_step.hpp"
 #include "kernels_transport.hpp"
 #include "kernels_flow.hpp"
+#include "kernels_leak.hpp"
 
 // Device workspace of phg_zone_mean_noalloc, the entry point that allocates
 // nothing: the closed-zone partial sums (one per task) and then the means
@@ -41,14 +42,14 @@ namespace phg {
 enum KClass {
     K_RESID_INIT, K_DIR_APPLY, K_PRE_RED, K_RB_BLACK, K_RESID0, K_POST_BLACK, K_POST_RED, K_FIN, K_VC_RESTRICT,
     K_VC_JAC, K_VC_RESID, K_VC_PROLONG, K_VC_COARSE, K_XFLUSH, K_ZONE, K_RHS, K_CORRECT, K_ENERGY, K_ADVECT,
-    K_COUNT, K_PRE_SMOOTH, K_POST_SMOOTH, K_TURB, K_HEAT, K_MOMENTUM, K_COMBUST, K_GASMISC, K_NCLASS
+    K_COUNT, K_PRE_SMOOTH, K_POST_SMOOTH, K_TURB, K_HEAT, K_MOMENTUM, K_COMBUST, K_GASMISC, K_LEAK, K_NCLASS
 };
 static const char* kKName[K_NCLASS] = {"resid_init",  "dir_apply",  "pre_red",    "rb_black",  "resid0",
                                        "post_black",  "post_red",   "finalize",   "vc_restrict", "vc_jacobi",
                                        "vc_resid",    "vc_prolong", "vc_coarse",  "x_flush",   "zone_mean",
                                        "rhs",         "correct",    "energy",     "advect",    "count_active",
                                        "pre_smooth",  "post_smooth", "turbulence", "heat_diffusion",
-                                       "momentum",    "combustion",  "gas_misc"};
+                                       "momentum",    "combustion",  "gas_misc",   "leakage"};
 
 struct GridDev {
     int nx = 0, ny = 0, nz = 0, n = 0;
@@ -94,6 +95,17 @@ struct GridDev {
     bool wind_set = false;
     float wind_key[4] = {0, 0, 0, 0};
     uint8_t *lx = nullptr, *ly = nullptr, *lz = nullptr;   // crack (leak) faces, or none
+    // closed-zone leakage: zone k of the closed ones -> zone id, its volume
+    int* czone = nullptr;
+    float* zvol = nullptr;
+    float* zp = nullptr;          // zone background pressures as passed in (every zone)
+    float* zpn = nullptr;         // ... and their end-of-step values from the crack root search
+    float* zdp = nullptr;         // closed zones' rise rates
+    CrackDev crack;               // crack faces and the per-zone lists over them
+    int crack_nb = 0;             // bisection entries
+    float4* crack_pre = nullptr;  // per bisection entry: other side's pressure, densities, open area
+    float* crack_q = nullptr;     // per crack: volume flow a -> b
+    std::vector<void*> crack_allocs;
     std::vector<void*> allocs;
     int single_batch = -1;
     DGrid dg() const {
@@ -609,6 +621,7 @@ void destroy_batch(BatchDev* B) {
 }
 
 void destroy_grid(GridDev* G) {
+    free_all(G->crack_allocs);
     free_all(G->allocs);
     delete G;
 }
@@ -944,6 +957,17 @@ int phg_grid_create_planned(phg_ctx* ctx, int nx, int ny, int nz, float dx, cons
             UP(G->ztptr, ztptr.data(), ztptr.size())
             UP(G->zcells, zcells.data(), zcells.size())
             UP(G->zclosed, zclosed.data(), zclosed.size())
+            // volumes as the reference keeps them (cell count * dx^3 in FP64, gasphase.cpp:155-156)
+            std::vector<int> czone(std::max(nclosed, 1), 0);
+            std::vector<float> zvol(std::max(nclosed, 1), 0.0f);
+            for (size_t z = 0; z < G->zone_open.size(); ++z)
+                if (zid[z] >= 0) {
+                    const double d = (double)dx;
+                    czone[zid[z]] = (int)z;
+                    zvol[zid[z]] = (float)((double)(zptr[zid[z] + 1] - zptr[zid[z]]) * d * d * d);
+                }
+            UP(G->czone, czone.data(), czone.size())
+            UP(G->zvol, zvol.data(), zvol.size())
         }
         if (!rc) rc = dalloc(ctx, G->allocs, &G->scratch_b, (size_t)G->n);
         if (!rc) rc = dalloc(ctx, G->allocs, &G->vmax, 4);
@@ -952,6 +976,9 @@ int phg_grid_create_planned(phg_ctx* ctx, int nx, int ny, int nz, float dx, cons
         if (!rc) rc = dalloc(ctx, G->allocs, &G->zsum, (size_t)(G->nztask + G->nclosed + 1));
         if (!rc) rc = dalloc(ctx, G->allocs, &G->zval, (size_t)(G->nclosed + 1));
         if (!rc) rc = dalloc(ctx, G->allocs, &G->wind, (size_t)4 * nz);
+        if (!rc) rc = dalloc(ctx, G->allocs, &G->zp, G->zone_open.size() + 1);
+        if (!rc) rc = dalloc(ctx, G->allocs, &G->zpn, G->zone_open.size() + 1);
+        if (!rc) rc = dalloc(ctx, G->allocs, &G->zdp, (size_t)G->nclosed + 1);
     #undef UP
         if (rc) {
             destroy_grid(G);
@@ -1949,6 +1976,169 @@ int phg_combustion(phg_ctx* ctx, int grid, float dt, int radiation_on, const phg
         double t = 0.0;
         for (double x : part) t += x;
         *hrr_total = t;
+    }
+    HIP_OK(hipGetLastError());
+    return PHG_OK;
+}
+
+int phg_grid_set_cracks(phg_ctx* ctx, int grid, const uint8_t* lx, const uint8_t* ly, const uint8_t* lz) {
+    return guarded(ctx, [&]() -> int {
+        if (!ctx) return PHG_EINVAL;
+        GridDev* G = get_grid(ctx, grid);
+        if (!G) return fail(ctx, PHG_ESTATE, "unknown grid %d", grid);
+        HIP_OK(hipStreamSynchronize(ctx->stream));
+        free_all(G->crack_allocs);
+        G->crack = CrackDev();
+        G->crack_nb = 0;
+        G->crack_pre = nullptr;
+        G->crack_q = nullptr;
+        G->lx = G->ly = G->lz = nullptr;
+        if (!lx && !ly && !lz) return PHG_OK;
+        const int nx = G->nx, ny = G->ny, nz = G->nz, n = G->n;
+        const size_t nf[3] = {(size_t)(nx + 1) * ny * nz, (size_t)nx * (ny + 1) * nz, (size_t)nx * ny * (nz + 1)};
+        const uint8_t* lk[3] = {lx, ly, lz};
+        const float* dfo[3] = {G->fx, G->fy, G->fz};
+        std::vector<uint8_t> blocked(n);
+        HIP_OK(hipMemcpy(blocked.data(), G->blocked, n, hipMemcpyDeviceToHost));
+        const float A = G->dx * G->dx;
+        std::vector<int4> face;
+        std::vector<int2> fzone;
+        std::vector<float> oa;
+        for (int ax = 0; ax < 3; ++ax) {
+            if (!lk[ax]) continue;
+            std::vector<float> fo(nf[ax]);
+            HIP_OK(hipMemcpy(fo.data(), dfo[ax], nf[ax] * sizeof(float), hipMemcpyDeviceToHost));
+            const int ex = nx + (ax == 0), ey = ny + (ax == 1);
+            const int lim = ax == 0 ? nx : ax == 1 ? ny : nz, step = ax == 0 ? 1 : ax == 1 ? nx : nx * ny;
+            for (size_t f = 0; f < nf[ax]; ++f) {
+                if (!lk[ax][f]) continue;
+                const int i = (int)(f % ex), j = (int)((f / ex) % ey), k = (int)(f / ((size_t)ex * ey));
+                const int pos = ax == 0 ? i : ax == 1 ? j : k;
+                const int cb = pos < lim ? i + nx * (j + ny * k) : -1;
+                const int ca = pos > 0 ? i + nx * (j + ny * k) - step : -1;
+                if ((ca >= 0 && blocked[ca]) || (cb >= 0 && blocked[cb])) continue;   // gasphase.cpp:424
+                face.push_back(make_int4(ax, (int)f, ca, cb));
+                fzone.push_back(make_int2(ca >= 0 ? G->zone[ca] : -1, cb >= 0 ? G->zone[cb] : -1));
+                oa.push_back(fo[f] * A);
+            }
+        }
+        const int ncr = (int)face.size();
+        // per closed zone: the cracks of its root search (to another zone or
+        // outside) and every crack that adds to its outflow, in crack order
+        std::vector<int> kid(G->zone_open.size(), -1);
+        int nclosed = 0;
+        for (size_t z = 0; z < G->zone_open.size(); ++z)
+            if (!G->zone_open[z]) kid[z] = nclosed++;
+        std::vector<std::vector<int>> bl(nclosed), ol(nclosed);
+        for (int q = 0; q < ncr; ++q) {
+            const int za = fzone[q].x, zb = fzone[q].y;
+            const int ka = za >= 0 ? kid[za] : -1, kb = zb >= 0 ? kid[zb] : -1;
+            if (za != zb) {
+                if (ka >= 0) bl[ka].push_back(q << 1 | 1);
+                if (kb >= 0) bl[kb].push_back(q << 1);
+            }
+            if (ka >= 0) ol[ka].push_back(q << 1);
+            if (kb >= 0) ol[kb].push_back(q << 1 | 1);
+        }
+        std::vector<int> bptr(nclosed + 1, 0), optr(nclosed + 1, 0), bent, oent;
+        for (int k = 0; k < nclosed; ++k) {
+            bent.insert(bent.end(), bl[k].begin(), bl[k].end());
+            oent.insert(oent.end(), ol[k].begin(), ol[k].end());
+            bptr[k + 1] = (int)bent.size();
+            optr[k + 1] = (int)oent.size();
+        }
+        int rc = 0;
+        auto up = [&](auto** dst, const auto* src, size_t cnt) {
+            if (!rc) rc = upload(ctx, G->crack_allocs, dst, src, cnt);
+        };
+        uint8_t* dl[3] = {nullptr, nullptr, nullptr};
+        for (int ax = 0; ax < 3; ++ax)
+            if (lk[ax]) up(&dl[ax], lk[ax], nf[ax]);
+        int4* dface = nullptr;
+        int2* dfz = nullptr;
+        float* doa = nullptr;
+        int *dbp = nullptr, *dbe = nullptr, *dop = nullptr, *doe = nullptr;
+        if (face.empty()) {
+            face.push_back(make_int4(0, 0, -1, -1));
+            fzone.push_back(make_int2(-1, -1));
+            oa.push_back(0.0f);
+        }
+        if (bent.empty()) bent.push_back(0);
+        if (oent.empty()) oent.push_back(0);
+        up(&dface, face.data(), face.size());
+        up(&dfz, fzone.data(), fzone.size());
+        up(&doa, oa.data(), oa.size());
+        up(&dbp, bptr.data(), bptr.size());
+        up(&dbe, bent.data(), bent.size());
+        up(&dop, optr.data(), optr.size());
+        up(&doe, oent.data(), oent.size());
+        if (!rc) rc = dalloc(ctx, G->crack_allocs, &G->crack_pre, (size_t)bptr[nclosed] + 1);
+        if (!rc) rc = dalloc(ctx, G->crack_allocs, &G->crack_q, (size_t)ncr + 1);
+        if (rc) {
+            free_all(G->crack_allocs);
+            return rc;
+        }
+        G->lx = dl[0];
+        G->ly = dl[1];
+        G->lz = dl[2];
+        CrackDev& cd = G->crack;
+        cd.ncr = ncr;
+        cd.face = dface;
+        cd.fzone = dfz;
+        cd.oa = doa;
+        cd.bptr = dbp;
+        cd.bent = dbe;
+        cd.optr = dop;
+        cd.oent = doe;
+        G->crack_nb = bptr[nclosed];
+        return PHG_OK;
+    });
+}
+
+int phg_zone_leakage(phg_ctx* ctx, int grid, float dt, const float* zoneP, const float* T, const float* Pb,
+                     const float* p, float* u, float* v, float* w, float* S, float* zoneDP, int* leak_faces) {
+    if (!ctx || !zoneP || !T || !Pb || !p || !u || !v || !w || !S) return PHG_EINVAL;
+    GridDev* G = get_grid(ctx, grid);
+    if (!G) return fail(ctx, PHG_ESTATE, "unknown grid %d", grid);
+    if (!(dt > 0)) return fail(ctx, PHG_EINVAL, "dt must be positive");
+    const size_t nzn = G->zone_open.size();
+    const CrackDev& cd = G->crack;
+    if (leak_faces) *leak_faces = cd.ncr;
+    if (zoneDP) std::fill(zoneDP, zoneDP + nzn, 0.0f);
+    if (nzn == 0) return PHG_OK;
+    HIP_OK(hipMemcpyAsync(G->zp, zoneP, nzn * sizeof(float), hipMemcpyHostToDevice, ctx->stream));
+    const int nc = G->nclosed;
+    double* dint = G->zsum + G->nztask;
+    if (nc) {
+        const float V = G->dx * G->dx * G->dx;
+        launch(ctx, K_LEAK, k_zone_sv, G->nztask, 0, (const int4*)G->ztask, (const int*)G->zcells, S, V, G->zsum);
+        launch(ctx, K_LEAK, k_zone_fin_sum, 1, 0, nc, (const int*)G->ztptr, (const double*)G->zsum, dint);
+    }
+    if (cd.ncr) {
+        HIP_OK(hipMemcpyAsync(G->zpn, G->zp, nzn * sizeof(float), hipMemcpyDeviceToDevice, ctx->stream));
+        if (nc) {
+            if (G->crack_nb)
+                launch(ctx, K_LEAK, k_crack_prep, tiles(G->crack_nb), 0, cd, G->crack_nb, (const float*)G->zp, Pb, T,
+                       G->crack_pre);
+            launch(ctx, K_LEAK, k_crack_bisect, nc, 0, cd, (const int*)G->czone, (const float*)G->zvol,
+                   (const double*)dint, (const float*)G->zp, (const float4*)G->crack_pre, dt, G->zpn);
+        }
+        launch(ctx, K_LEAK, k_crack_velocity, tiles(cd.ncr), 0, cd, (const float*)G->zpn, Pb, T, p, u, v, w,
+               G->crack_q);
+    }
+    if (nc) {
+        launch(ctx, K_LEAK, k_crack_zone_out, nc, 0, cd, (const int*)G->czone, (const float*)G->zvol,
+               (const double*)dint, (const float*)G->zp, (const float*)G->crack_q, G->zdp, G->zval);
+        launch(ctx, K_LEAK, k_zone_apply, tiles(G->n), 0, G->n, (const int*)G->zclosed, (const float*)G->zval, S, 1);
+        std::vector<float> zd(nc);
+        HIP_OK(hipMemcpyAsync(zd.data(), G->zdp, nc * sizeof(float), hipMemcpyDeviceToHost, ctx->stream));
+        HIP_OK(hipStreamSynchronize(ctx->stream));
+        if (zoneDP)
+            for (int k = 0, z = 0; z < (int)nzn; ++z)
+                if (!G->zone_open[z]) zoneDP[z] = zd[k++];
+    } else {
+        // zoneP's staging copy must land before the caller may reuse it
+        HIP_OK(hipStreamSynchronize(ctx->stream));
     }
     HIP_OK(hipGetLastError());
     return PHG_OK;

@@ -35,6 +35,7 @@ EXPORTS = (
     "phg_grid_set_coarse", "phg_grid_get_coarse",
     "phg_stable_dt", "phg_turbulence", "phg_heat_diffusion", "phg_momentum", "phg_step_clear", "phg_add_gas",
     "phg_combustion", "phg_zone_integrals", "phg_zone_values", "phg_grid_stats", "phg_batch_graph_check",
+    "phg_grid_set_cracks", "phg_zone_leakage",
 )
 
 
@@ -127,6 +128,8 @@ def load_library():
     L.phg_grid_stats.argtypes = [vp, i32, C.POINTER(f32), C.POINTER(f32), C.POINTER(f32)]
     L.phg_batch_graph_check.argtypes = [vp, i32, i32, i32, C.POINTER(i32), C.c_char_p, sz]
     L.phg_leakage.argtypes = [vp, i32, vp, vp]
+    L.phg_grid_set_cracks.argtypes = [vp, i32, vp, vp, vp]
+    L.phg_zone_leakage.argtypes = [vp, i32, f32] + [vp] * 9 + [vp]
     L.phg_correct.argtypes = [vp, i32, f32, vp, vp, vp, vp, C.POINTER(f32)]
     L.phg_energy_expansion.argtypes = [vp, i32, f32, vp, vp, vp, vp, vp, C.POINTER(f32)]
     L.phg_advect_scalars.argtypes = [vp, i32, f32, vp, vp, vp, vp, vp, vp, vp]
@@ -330,6 +333,18 @@ class Grid:
                                                      C.byref(gid)))
             plan.free()
         self.id = gid.value
+        self.leak_faces = 0
+        if getattr(scen, "leaks", None) is not None:
+            self.set_cracks(scen.leaks)
+
+    def set_cracks(self, leaks):
+        """Crack faces (lx, ly, lz: one byte per face, None for none) of the
+        grid (phg_grid_set_cracks); None clears them."""
+        lk = [None, None, None] if leaks is None else [None if a is None else np.ascontiguousarray(a, np.uint8)
+                                                        for a in leaks]
+        self._leaks = lk
+        self.ctx._check(self.ctx.L.phg_grid_set_cracks(self.ctx.h, self.id,
+                                                       *[None if a is None else a.ctypes.data for a in lk]))
 
     def info(self):
         nl, nz, rl = C.c_int(), C.c_int(), C.c_int()
@@ -672,28 +687,20 @@ class GasSolver:
         self.ctx._check(self.ctx.L.phg_step_clear(self.ctx.h, self.grid.id, C.byref(self._gas())))
 
     def leakage_zones(self, dt):
-        """Solver::leakage for grids without cracks (gasphase.cpp:402-498): each
-        closed zone's pressure rise rate from its integral of S, and the
-        compatible shift of S."""
-        f32 = np.float32
+        """Solver::leakage (gasphase.cpp:402-498) on the device
+        (phg_zone_leakage): crack flow, each closed zone's background pressure
+        root, the relaxed crack velocities, zoneDP and the shift of S."""
         nz = len(self.zone_open)
-        dint = np.zeros(max(nz, 1), np.float64)
-        self.ctx._check(self.ctx.L.phg_zone_integrals(self.ctx.h, self.grid.id, self.d["S"].ptr,
-                                                      dint.ctypes.data))
-        gamma = f32(1100.0) / (f32(1100.0) - f32(287.0))
-        shift = np.zeros(max(nz, 1), np.float32)
-        self.zoneDP[:] = 0
-        for z in range(nz):
-            if self.zone_open[z]:
-                continue
-            Vz = self.zone_vol[z]
-            dI = f32(dint[z])
-            self.zoneDP[z] = gamma * self.zoneP[z] * dI / Vz
-            shift[z] = dI / Vz
-        if nz and not self.zone_open.all():
-            self.ctx._check(self.ctx.L.phg_zone_values(self.ctx.h, self.grid.id, shift.ctypes.data,
-                                                       self.d["S"].ptr, 1))
-        return shift[:nz]
+        zdp = np.zeros(max(nz, 1), np.float32)
+        zp = np.ascontiguousarray(self.zoneP, np.float32)
+        nf = C.c_int()
+        d = self.d
+        self.ctx._check(self.ctx.L.phg_zone_leakage(self.ctx.h, self.grid.id, dt, zp.ctypes.data, d["T"].ptr,
+                                                    d["Pb"].ptr, d["p"].ptr, d["u"].ptr, d["v"].ptr, d["w"].ptr,
+                                                    d["S"].ptr, zdp.ctypes.data, C.byref(nf)))
+        self.zoneDP[:] = zdp[:nz]
+        self.leakFaces = nf.value
+        return self.zoneDP
 
     def zone_pressure(self, dt):
         """Closed zones' background pressure follows its rise rate (solver.cpp:258-263)."""

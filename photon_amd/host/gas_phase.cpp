@@ -10,7 +10,6 @@ namespace photon_gpu {
 
 namespace {
 constexpr float T0 = 293.15f, P0 = 101325.0f, RS = 287.0f, CP = 1100.0f;
-constexpr float GAMMA = CP / (CP - RS);   // solver.h:175
 constexpr float DHC_FUEL = 25.0e6f;       // solver.h:165
 constexpr float BURNER_SOOT = 0.01f;      // gasphase.cpp:93
 constexpr float T_AUTO = 800.0f;          // solver.h:179
@@ -145,10 +144,6 @@ bool GasPhase::build(int nx_, int ny_, int nz_, float dx_, const std::vector<flo
     zoneOpen.resize(nzn);
     zoneP.assign(nzn, P0);
     zoneDP.assign(nzn, 0.0f);
-    zoneVol_.assign(nzn, 0.0);
-    for (int c = 0; c < N; ++c)
-        if (zone[c] >= 0) zoneVol_[zone[c]] += 1;
-    for (double& v : zoneVol_) v = v * dx * dx * dx;
     // state at ambient, still air (as Solver::build leaves the gas)
     u.assign(nfx, 0.0f);
     v.assign(nfy, 0.0f);
@@ -259,22 +254,24 @@ void GasPhase::momentum(float dt) {
     }
 }
 
-void GasPhase::leakage(float dt) {   // gasphase.cpp:402-498, grids without crack faces
-    (void)dt;
-    const size_t nZ = zoneP.size();
-    std::vector<double> dInt(std::max<size_t>(nZ, 1), 0.0);
-    if (!check(phg_zone_integrals(ctx_, grid_, dS_, dInt.data()), "phg_zone_integrals")) return;
-    std::vector<float> shift(std::max<size_t>(nZ, 1), 0.0f);
-    bool any = false;
-    for (size_t z = 0; z < nZ; ++z) {
-        zoneDP[z] = 0.0f;
-        if (zoneOpen[z]) continue;
-        const float Vz = (float)zoneVol_[z];
-        zoneDP[z] = GAMMA * zoneP[z] * (float)dInt[z] / Vz;
-        shift[z] = (float)dInt[z] / Vz;
-        any = true;
+bool GasPhase::setCracks(const std::vector<uint8_t>& lx, const std::vector<uint8_t>& ly,
+                         const std::vector<uint8_t>& lz) {
+    const size_t nfx = (size_t)(nx + 1) * ny * nz, nfy = (size_t)nx * (ny + 1) * nz, nfz = (size_t)nx * ny * (nz + 1);
+    if ((!lx.empty() && lx.size() != nfx) || (!ly.empty() && ly.size() != nfy) || (!lz.empty() && lz.size() != nfz)) {
+        err_ = "crack arrays do not match the grid";
+        return false;
     }
-    if (any) check(phg_zone_values(ctx_, grid_, shift.data(), dS_, 1), "phg_zone_values");
+    return check(phg_grid_set_cracks(ctx_, grid_, lx.empty() ? nullptr : lx.data(), ly.empty() ? nullptr : ly.data(),
+                                     lz.empty() ? nullptr : lz.data()),
+                 "phg_grid_set_cracks");
+}
+
+void GasPhase::leakage(float dt) {   // gasphase.cpp:402-498
+    zoneDP.assign(zoneP.size(), 0.0f);
+    if (zoneP.empty()) return;
+    check(phg_zone_leakage(ctx_, grid_, dt, zoneP.data(), dF_[0], dPb_, dP_, dU_, dV_, dW_, dS_, zoneDP.data(),
+                           &leakFaces),
+          "phg_zone_leakage");
 }
 
 void GasPhase::project(float dt) {   // gasphase.cpp:505-677

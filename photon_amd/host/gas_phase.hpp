@@ -56,6 +56,10 @@ public:
                const std::vector<float>& az, const std::vector<uint8_t>& blocked, std::string& err, int device = 0,
                bool refCoarse = true, int precision = PHG_PREC_FP32);
 
+    // Crack faces (Solver::lx_, ly_, lz_, solver.h:297): one byte per face of
+    // each axis, empty for none on that axis.  Returns false on error.
+    bool setCracks(const std::vector<uint8_t>& lx, const std::vector<uint8_t>& ly, const std::vector<uint8_t>& lz);
+
     float step(float dtMax);                 // Solver::step, returns dt
     int addBurner(const Burner& b);          // Solver::addBurner, -1 if it covers no gas cell
 
@@ -92,6 +96,7 @@ public:
     Profile profile;
     bool radiationOn = false;
     float maxVel = 0, maxNu = 0, maxDiv = 0;
+    int leakFaces = 0;                       // crack faces handled by the last leakage
     int pressureIters = 0;
     float pressureResidual = 0;
     double hrrTotal = 0;
@@ -101,7 +106,6 @@ private:
     int grid_ = -1;
     std::string err_;
     std::vector<void*> dmem_;
-    std::vector<double> zoneVol_;
     float *dU_ = nullptr, *dV_ = nullptr, *dW_ = nullptr, *dU2_ = nullptr, *dV2_ = nullptr, *dW2_ = nullptr;
     float *dP_ = nullptr, *dS_ = nullptr, *dNu_ = nullptr;
     float* dF_[10] = {};      // T, Yo, Yf, Ys, Ya, Yp, Yc, Yw, Yco, Yai

@@ -43,6 +43,11 @@ std::vector<T> read_raw(const std::string& dir, const std::string& name) {
     return v;
 }
 
+bool exists(const std::string& dir, const std::string& name) {
+    std::ifstream in(dir + "/" + name + ".bin", std::ios::binary);
+    return (bool)in;
+}
+
 template <class T>
 void write_raw(const std::string& dir, const std::string& name, const std::vector<T>& v) {
     std::ofstream out(dir + "/out_" + name + ".bin", std::ios::binary);
@@ -72,6 +77,11 @@ int main(int argc, char** argv) {
                   read_raw<float>(dir, "ax"), read_raw<float>(dir, "ay"), read_raw<float>(dir, "az"),
                   read_raw<uint8_t>(dir, "blocked"), err)) {
         std::fprintf(stderr, "build: %s\n", err.c_str());
+        return 1;
+    }
+    if (exists(dir, "lx") &&
+        !gp.setCracks(read_raw<uint8_t>(dir, "lx"), read_raw<uint8_t>(dir, "ly"), read_raw<uint8_t>(dir, "lz"))) {
+        std::fprintf(stderr, "setCracks: %s\n", gp.error().c_str());
         return 1;
     }
     std::FILE* log = std::fopen((dir + "/out_log.txt").c_str(), "w");

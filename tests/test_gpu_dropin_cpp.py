@@ -20,7 +20,8 @@ pytestmark = pytest.mark.gpu
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 EXE = os.path.join(HERE, "cpp", "gas_step_check")
-CASES = {"room": sc.room, "walls": lambda: sc.house(7, 24, 16, 12), "wind": lambda: sc.room(window=True)}
+CASES = {"room": sc.room, "walls": lambda: sc.house(7, 24, 16, 12), "wind": lambda: sc.room(window=True),
+         "cracked": sc.cracked}
 SPECIES = ("Yo", "Yf", "Ys", "Ya", "Yp", "Yc", "Yw", "Yco", "Yai")
 
 
@@ -40,6 +41,9 @@ def write_case(d, g, meta, arrays):
     for k, a in geo.items():
         np.ascontiguousarray(a, np.float32).tofile(os.path.join(d, k + ".bin"))
     np.ascontiguousarray(g.blocked, np.uint8).tofile(os.path.join(d, "blocked.bin"))
+    if g.leaks is not None:
+        for k, a in zip(("lx", "ly", "lz"), g.leaks):
+            np.ascontiguousarray(a, np.uint8).tofile(os.path.join(d, k + ".bin"))
     for k, a in arrays.items():
         np.ascontiguousarray(a).tofile(os.path.join(d, k + ".bin"))
 

@@ -21,7 +21,8 @@ pytestmark = pytest.mark.gpu
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 DT = 0.01
-GOLDEN = {"room": sc.room, "walls": lambda: sc.house(7, 24, 16, 12), "wind": lambda: sc.room(window=True)}
+GOLDEN = {"room": sc.room, "walls": lambda: sc.house(7, 24, 16, 12), "wind": lambda: sc.room(window=True),
+          "cracked": sc.cracked}
 
 
 def golden(name):
@@ -194,6 +195,61 @@ def test_whole_step_follows_reference_trajectory(ctx, name):
     assert rel(s.get("nu"), G["st_nu"]) < 1e-5
     assert np.abs(s.zoneP - G["st_zoneP"]).max() <= 1e-4 * 101325
     assert np.array_equal(s.get("lit"), G["st_lit"])
+    s.grid.destroy()
+
+
+# ---------------------------------------------------------------------------
+# crack flow and closed-zone pressure (leakage, gasphase.cpp:402-498) against
+# the reference's own leakage on the cracked two-room grid
+# (tests/golden/ref_cracked_leakage.npz: "lk" a working root search, "lk2"
+# a source so large that the bracket fails and crack speeds reach the cap)
+# ---------------------------------------------------------------------------
+@pytest.mark.parametrize("key", ["lk", "lk2"])
+def test_crack_leakage_matches_reference(ctx, key):
+    """Crack velocities, S and zoneDP against the reference.  The device sums
+    in FP64 trees where the reference sums sequentially, so a float formed
+    from a sum may differ by one rounding: S and zoneDP within 1e-6
+    relative, crack velocities bit for bit (they depend on the zone
+    pressure's root, whose float is the same unless a sum lands within
+    2^-50 of a rounding boundary), everything else untouched."""
+    g, G = sc.cracked(), np.load(os.path.join(HERE, "golden", "ref_cracked_leakage.npz"))
+    s = gas.GasSolver(ctx, g)
+    for f in ("u", "v", "w", "S", "T", "Pb", "p"):
+        s.set(f, G[f"{key}_in_{f}"])
+    s.zoneP[:] = G[f"{key}_in_zoneP"]
+    zdp = s.leakage_zones(DT).copy()
+    assert s.leakFaces == int(G[f"{key}_faces"]) == sum(int(a.sum()) for a in g.leaks)
+    for f in ("u", "v", "w"):
+        o, r = s.get(f), G[f"{key}_{f}"]
+        assert bits_equal(o, r), (f, int(np.sum(o != r)), float(np.max(np.abs(o - r))))
+    S, Sr = s.get("S"), G[f"{key}_S"]
+    assert np.abs(S - Sr).max() <= 1e-6 * np.abs(Sr).max(), float(np.abs(S - Sr).max())
+    zr = G[f"{key}_zoneDP"]
+    assert np.all(np.abs(zdp - zr) <= 1e-6 * np.abs(zr)), (zdp, zr)
+    assert np.all(zdp[s.zone_open == 1] == 0)
+    s.grid.destroy()
+
+
+def test_crack_leakage_without_cracks_is_the_zone_mean_shift(ctx):
+    """On a sealed room with no crack faces the device leakage is the
+    closed-zone mean shift: S keeps no mean in the zone, zoneDP =
+    gamma P integral(S dV) / Vz, and no face velocity moves."""
+    g = sc.room(door=False)
+    s = gas.GasSolver(ctx, g)
+    rng = np.random.default_rng(9)
+    S0 = (rng.random(g.n) * (g.blocked == 0)).astype(np.float32)
+    s.set("S", S0)
+    u0 = (0.1 * rng.standard_normal(len(g.fx))).astype(np.float32) * g.fx
+    s.set("u", u0)
+    zdp = s.leakage_zones(DT).copy()
+    assert s.leakFaces == 0 and bits_equal(s.get("u"), u0)
+    z = s.grid.zones()[0] == 0
+    Vz = float(s.zone_vol[0])
+    dint = float(np.sum(S0[z].astype(np.float64)) * np.float64(np.float32(g.dx) ** 3))
+    S = s.get("S")
+    assert abs(float(np.sum(S[z].astype(np.float64)))) < 1e-6 * float(np.sum(S0[z]))
+    gamma = 1100.0 / (1100.0 - 287.0)
+    assert abs(zdp[0] - gamma * 101325.0 * dint / Vz) <= 1e-5 * abs(zdp[0])
     s.grid.destroy()
 
 
