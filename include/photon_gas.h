@@ -330,6 +330,20 @@ int phg_zone_values(phg_ctx *ctx, int grid, const float *values, float *x, int s
  * them.  A crack with a solid side is ignored (gasphase.cpp:424).  Momentum
  * keeps crack velocities, which phg_zone_leakage drives. */
 int phg_grid_set_cracks(phg_ctx *ctx, int grid, const uint8_t *lx, const uint8_t *ly, const uint8_t *lz);
+/* Face changes of a grid (barrier failure, a door opened or closed): the
+ * results of Solver::updateFace (solver.cpp:66-101) for n faces (host
+ * arrays): axis (0..2), face index, open fraction f, matrix coefficient a,
+ * crack flag.  Velocities (device, may be NULL) of faces that close (f = 0)
+ * are zeroed, as updateFace does.  When a matrix coefficient changed, the
+ * zones and the multigrid hierarchy are set up again under the same id,
+ * precision, wind sides and coarsest-solve mode (computeZones and
+ * buildPressureLevels on the reference's zonesDirty_ / mgDirty_,
+ * gasphase.cpp:111-158, pressure_mg.cpp:28-191); batches made earlier over
+ * this grid are stale then and their solves return PHG_ESTATE.  Otherwise
+ * only the open fractions and the crack faces change.  rebuilt (may be
+ * NULL): 1 when the set-up was redone. */
+int phg_grid_update_faces(phg_ctx *ctx, int grid, int n, const int *axis, const int *face, const float *f,
+                          const float *a, const uint8_t *crack, float *u, float *v, float *w, int *rebuilt);
 /* Solver::leakage (gasphase.cpp:402-498): each closed zone's end-of-step
  * background pressure against its cracks (orifice flow, 40-step bisection),
  * every crack velocity relaxed halfway to its orifice velocity, each closed

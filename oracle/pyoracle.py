@@ -418,6 +418,7 @@ class RefSolver:
             L.ref_zone_pressure.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p]
             L.ref_set_zone_pressure.argtypes = [C.c_void_p, _f32p]
             L.ref_leak_faces.argtypes = [C.c_void_p]
+            L.ref_update_faces.argtypes = [C.c_void_p, C.c_int, _i32p, _i32p, _f32p, _f32p, _u8p]
             L.ref_wind_at.restype = C.c_float
             L.ref_wind_at.argtypes = [C.c_void_p, C.c_int, C.c_int, C.c_float]
             cls._libs[path] = L
@@ -438,6 +439,7 @@ class RefSolver:
             self.h = None
 
     def zones(self):
+        self.nzones = self._L.ref_zone_pressure(self.h, None, None)   # the current zone count
         zone = np.zeros(self.grid.n, np.int32)
         zo = np.zeros(max(self.nzones, 1), np.uint8)
         self._L.ref_get_zones(self.h, zone, zo)
@@ -535,6 +537,11 @@ class RefSolver:
 
     def set_zone_pressure(self, zoneP):
         self._L.ref_set_zone_pressure(self.h, np.ascontiguousarray(zoneP, np.float32))
+
+    def update_faces(self, axis, face, f, a, crack):
+        self._L.ref_update_faces(self.h, len(axis), np.ascontiguousarray(axis, np.int32),
+                                 np.ascontiguousarray(face, np.int32), np.ascontiguousarray(f, np.float32),
+                                 np.ascontiguousarray(a, np.float32), np.ascontiguousarray(crack, np.uint8))
 
     def leak_faces(self):
         return self._L.ref_leak_faces(self.h)

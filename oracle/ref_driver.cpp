@@ -195,6 +195,27 @@ int ref_set_zone_pressure(void* h, const float* zoneP) {
     return (int)s->zoneP.size();
 }
 
+// Face changes as Solver::updateFace leaves them (solver.cpp:88-100): open
+// fraction, matrix coefficient and crack flag written, the velocity of a
+// closed face zeroed, zonesDirty_ set when the coefficient or the flag
+// changed (the next step recomputes the zones, which marks mgDirty_).
+void ref_update_faces(void* h, int n, const int* axis, const int* face, const float* f, const float* a,
+                      const uint8_t* crack) {
+    Solver* s = static_cast<Solver*>(h);
+    for (int q = 0; q < n; ++q) {
+        const int d = axis[q], e = face[q];
+        std::vector<float>& fo = d == 0 ? s->fx : d == 1 ? s->fy : s->fz;
+        std::vector<float>& av = d == 0 ? s->ax_ : d == 1 ? s->ay_ : s->az_;
+        std::vector<uint8_t>& lv = d == 0 ? s->lx_ : d == 1 ? s->ly_ : s->lz_;
+        std::vector<float>& vel = d == 0 ? s->u : d == 1 ? s->v : s->w;
+        fo[e] = f[q];
+        if (f[q] == 0) vel[e] = 0;
+        if (av[e] != a[q] || lv[e] != crack[q]) s->zonesDirty_ = true;
+        av[e] = a[q];
+        lv[e] = crack[q];
+    }
+}
+
 // crack faces the last leakage handled (Solver::leakFaces, solver.h:279)
 int ref_leak_faces(void* h) { return static_cast<Solver*>(h)->leakFaces; }
 

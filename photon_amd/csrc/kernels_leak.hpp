@@ -167,4 +167,14 @@ __global__ void __launch_bounds__(TB) k_crack_zone_out(CrackDev cd, const int* _
     }
 }
 
+// face velocities set to zero where a face update closes the face
+// (Solver::updateFace, solver.cpp:92); list: (axis, face)
+__global__ void __launch_bounds__(TB) k_face_zero(const int2* __restrict__ list, int n, float* __restrict__ u,
+                                                  float* __restrict__ v, float* __restrict__ w) {
+    const int q = blockIdx.x * TB + threadIdx.x;
+    if (q >= n) return;
+    const int2 e = list[q];
+    (e.x == 0 ? u : e.x == 1 ? v : w)[e.y] = 0.0f;
+}
+
 }  // namespace phg

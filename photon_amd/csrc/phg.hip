@@ -106,6 +106,12 @@ struct GridDev {
     float4* crack_pre = nullptr;  // per bisection entry: other side's pressure, densities, open area
     float* crack_q = nullptr;     // per crack: volume flow a -> b
     std::vector<void*> crack_allocs;
+    // face updates: the set-up's generation (batches made before a rebuild
+    // are stale), the wind-closed sides and the caller's coefficients there
+    int gen = 0;
+    bool has_wc = false;
+    uint8_t wc[4] = {0, 0, 0, 0};
+    std::vector<float> side_a[4];
     std::vector<void*> allocs;
     int single_batch = -1;
     DGrid dg() const {
@@ -145,6 +151,7 @@ struct BatchDev {
     int maxnu0_u = 0;              // most sweeps among the others
     size_t lds0() const { return 2 * (size_t)(TB + 2 * maxnx) * sizeof(float); }
     std::vector<int> grids;
+    std::vector<int> gens;         // each grid's set-up generation when the batch was made
     std::vector<DSys> h_sys;
     DSys* d_sys = nullptr;
     std::vector<int2*> tab;        // level 0: (system, tile); tab[0] only
@@ -610,6 +617,20 @@ BatchDev* get_batch(phg_ctx* ctx, int id) {
     return it == ctx->batches.end() ? nullptr : it->second;
 }
 
+// 0, or PHG_ESTATE when a grid of the batch was rebuilt by a face update
+// after the batch was made (the batch holds the old set-up's buffers)
+int batch_fresh(phg_ctx* ctx, const BatchDev* B, int batch) {
+    for (int s = 0; s < B->nsys; ++s) {
+        auto it = ctx->grids.find(B->grids[s]);
+        if (it == ctx->grids.end())
+            return fail(ctx, PHG_ESTATE, "batch %d: grid %d no longer exists", batch, B->grids[s]);
+        if (it->second->gen != B->gens[s])
+            return fail(ctx, PHG_ESTATE, "batch %d: grid %d was rebuilt by a face update; create the batch again",
+                        batch, B->grids[s]);
+    }
+    return 0;
+}
+
 void destroy_batch(BatchDev* B) {
     if (B->graph_exec) (void)hipGraphExecDestroy(B->graph_exec);
     if (B->graph) (void)hipGraphDestroy(B->graph);
@@ -796,12 +817,16 @@ int phg_grid_create(phg_ctx* ctx, int nx, int ny, int nz, float dx, const float*
                                    nullptr, grid_out);
 }
 
-int phg_grid_create_planned(phg_ctx* ctx, int nx, int ny, int nz, float dx, const float* fx, const float* fy,
-                            const float* fz, const float* ax, const float* ay, const float* az,
-                            const uint8_t* blocked, const uint8_t* wind_closed, int precision, phg_grid_plan* plan,
-                            int* grid_out) {
-    return guarded(ctx, [&]() -> int {
-        if (!ctx || !grid_out) return PHG_EINVAL;
+}  // extern "C"
+
+namespace {
+
+// A grid's device set-up from host geometry (the work of phg_grid_create),
+// not yet registered with the context.
+int build_grid_dev(phg_ctx* ctx, int nx, int ny, int nz, float dx, const float* fx, const float* fy, const float* fz,
+                   const float* ax, const float* ay, const float* az, const uint8_t* blocked,
+                   const uint8_t* wind_closed, int precision, phg_grid_plan* plan, GridDev** out) {
+        *out = nullptr;
         const std::string bad = check_shape(nx, ny, nz);
         if (!bad.empty()) return fail(ctx, PHG_EINVAL, "%s", bad.c_str());
         if (!(dx > 0)) return fail(ctx, PHG_EINVAL, "bad cell size dx=%g", dx);
@@ -984,6 +1009,39 @@ int phg_grid_create_planned(phg_ctx* ctx, int nx, int ny, int nz, float dx, cons
             destroy_grid(G);
             return rc;
         }
+        // what a later face update needs to redo the set-up: the wind-closed
+        // sides and the caller's coefficients on them (level 0 holds zeros there)
+        G->has_wc = wind_closed != nullptr;
+        for (int q = 0; q < 4; ++q) {
+            G->wc[q] = wind_closed ? wind_closed[q] : 0;
+            G->side_a[q].clear();
+            if (!G->wc[q]) continue;
+            const int axis = q / 2, side = q % 2;
+            if (axis == 0)
+                for (int k = 0; k < nz; ++k)
+                    for (int j = 0; j < ny; ++j) G->side_a[q].push_back(ax[(side ? nx : 0) + (nx + 1) * (j + ny * k)]);
+            else
+                for (int k = 0; k < nz; ++k)
+                    for (int i = 0; i < nx; ++i) G->side_a[q].push_back(ay[i + nx * ((side ? ny : 0) + (ny + 1) * k)]);
+        }
+        *out = G;
+        return PHG_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int phg_grid_create_planned(phg_ctx* ctx, int nx, int ny, int nz, float dx, const float* fx, const float* fy,
+                            const float* fz, const float* ax, const float* ay, const float* az,
+                            const uint8_t* blocked, const uint8_t* wind_closed, int precision, phg_grid_plan* plan,
+                            int* grid_out) {
+    return guarded(ctx, [&]() -> int {
+        if (!ctx || !grid_out) return PHG_EINVAL;
+        GridDev* G = nullptr;
+        const int rc = build_grid_dev(ctx, nx, ny, nz, dx, fx, fy, fz, ax, ay, az, blocked, wind_closed, precision,
+                                      plan, &G);
+        if (rc) return rc;
         const int id = ctx->next_id++;
         ctx->grids[id] = G;
         *grid_out = id;
@@ -1087,6 +1145,7 @@ int phg_batch_create_ex(phg_ctx* ctx, int nsys, const int* grids, const phg_batc
         BatchDev* B = new BatchDev();
         B->nsys = nsys;
         B->grids.assign(grids, grids + nsys);
+        B->gens.assign(nsys, 0);
         B->unfused = opts && opts->unfused;
         B->fuse_nt = opts ? opts->fused_threads : 0;
         B->h_sys.resize(nsys);
@@ -1098,6 +1157,7 @@ int phg_batch_create_ex(phg_ctx* ctx, int nsys, const int* grids, const phg_batc
                 rc = fail(ctx, PHG_ESTATE, "unknown grid %d", grids[s]);
                 break;
             }
+            B->gens[s] = G->gen;
             if (s == 0) B->prec = G->prec;
             else if (G->prec != B->prec) {
                 rc = fail(ctx, PHG_EINVAL, "all grids of a batch must share one precision");
@@ -1401,6 +1461,7 @@ int phg_batch_solve(phg_ctx* ctx, int batch, const float* const* b, float* const
         if (!ctx || !b || !x || !P) return PHG_EINVAL;
         BatchDev* B = get_batch(ctx, batch);
         if (!B) return fail(ctx, PHG_ESTATE, "unknown batch %d", batch);
+        if (int rs = batch_fresh(ctx, B, batch)) return rs;
         if (!(P->tol >= 0) || P->max_iters < 0) return fail(ctx, PHG_EINVAL, "bad tolerance or iteration cap");
         if (P->stop_rule != PHG_STOP_REL_L2 && P->stop_rule != PHG_STOP_MAXNORM)
             return fail(ctx, PHG_EINVAL, "unknown stop rule %d", P->stop_rule);
@@ -1517,6 +1578,7 @@ int phg_batch_graph_check(phg_ctx* ctx, int batch, int iterations, int refine, i
     if (!ctx || iterations < 0) return PHG_EINVAL;
     BatchDev* B = get_batch(ctx, batch);
     if (!B) return fail(ctx, PHG_ESTATE, "unknown batch %d", batch);
+    if (int rs = batch_fresh(ctx, B, batch)) return rs;
     if (ctx->prof) return fail(ctx, PHG_EINVAL, "profiling is on: launches are not captured then");
     HIP_OK(hipSetDevice(ctx->device));
     // the whole-solve graph phg_batch_solve captures for use_graph = 2
@@ -2092,6 +2154,125 @@ int phg_grid_set_cracks(phg_ctx* ctx, int grid, const uint8_t* lx, const uint8_t
         cd.oent = doe;
         G->crack_nb = bptr[nclosed];
         return PHG_OK;
+    });
+}
+
+int phg_grid_update_faces(phg_ctx* ctx, int grid, int n, const int* axis, const int* face, const float* f,
+                          const float* a, const uint8_t* crack, float* u, float* v, float* w, int* rebuilt) {
+    return guarded(ctx, [&]() -> int {
+        if (!ctx || n < 0 || (n > 0 && (!axis || !face || !f || !a || !crack))) return PHG_EINVAL;
+        GridDev* G = get_grid(ctx, grid);
+        if (!G) return fail(ctx, PHG_ESTATE, "unknown grid %d", grid);
+        if (rebuilt) *rebuilt = 0;
+        const int nx = G->nx, ny = G->ny, nz = G->nz;
+        const size_t nf[3] = {(size_t)(nx + 1) * ny * nz, (size_t)nx * (ny + 1) * nz, (size_t)nx * ny * (nz + 1)};
+        for (int q = 0; q < n; ++q) {
+            if (axis[q] < 0 || axis[q] > 2 || face[q] < 0 || (size_t)face[q] >= nf[axis[q]])
+                return fail(ctx, PHG_EINVAL, "update %d: no face %d on axis %d", q, face[q], axis[q]);
+            if (!(f[q] >= 0 && f[q] <= 1) || !(a[q] >= 0 && std::isfinite(a[q])))
+                return fail(ctx, PHG_EINVAL, "update %d: bad open fraction %g or coefficient %g", q, (double)f[q],
+                            (double)a[q]);
+        }
+        if (n == 0) return PHG_OK;
+        HIP_OK(hipSetDevice(ctx->device));
+        HIP_OK(hipStreamSynchronize(ctx->stream));
+        // the geometry as the caller gave it: open fractions and solid mask
+        // from the device, level 0's coefficients with the caller's values
+        // restored on wind-closed sides, the crack flags
+        std::vector<float> fo[3], co[3];
+        std::vector<uint8_t> lk[3];
+        const float* dfo[3] = {G->fx, G->fy, G->fz};
+        const float* dco[3] = {G->ax, G->ay, G->az};
+        const uint8_t* dlk[3] = {G->lx, G->ly, G->lz};
+        bool had_cracks = false;
+        for (int d = 0; d < 3; ++d) {
+            fo[d].resize(nf[d]);
+            co[d].resize(nf[d]);
+            lk[d].assign(nf[d], 0);
+            HIP_OK(hipMemcpy(fo[d].data(), dfo[d], nf[d] * sizeof(float), hipMemcpyDeviceToHost));
+            HIP_OK(hipMemcpy(co[d].data(), dco[d], nf[d] * sizeof(float), hipMemcpyDeviceToHost));
+            if (dlk[d]) {
+                HIP_OK(hipMemcpy(lk[d].data(), dlk[d], nf[d], hipMemcpyDeviceToHost));
+                had_cracks = true;
+            }
+        }
+        for (int q = 0; q < 4; ++q) {
+            if (!G->wc[q]) continue;
+            size_t e = 0;
+            const int side = q % 2;
+            if (q < 2)
+                for (int k = 0; k < nz; ++k)
+                    for (int j = 0; j < ny; ++j) co[0][(side ? nx : 0) + (nx + 1) * (j + ny * k)] = G->side_a[q][e++];
+            else
+                for (int k = 0; k < nz; ++k)
+                    for (int i = 0; i < nx; ++i) co[1][i + nx * ((side ? ny : 0) + (ny + 1) * k)] = G->side_a[q][e++];
+        }
+        // apply; the matrix or the crack set changing is what marks the
+        // reference's zones dirty (solver.cpp:98)
+        bool matrix = false, cracks = false;
+        std::vector<int2> closing;
+        for (int q = 0; q < n; ++q) {
+            const int d = axis[q];
+            const size_t e = (size_t)face[q];
+            const uint8_t c = crack[q] ? 1 : 0;
+            matrix |= co[d][e] != a[q];
+            cracks |= lk[d][e] != c || (c && fo[d][e] != f[q]);
+            fo[d][e] = f[q];
+            co[d][e] = a[q];
+            lk[d][e] = c;
+            if (f[q] == 0.0f) closing.push_back(make_int2(d, face[q]));
+        }
+        if (!closing.empty() && u && v && w) {
+            int2* dl = nullptr;
+            HIP_OK(hipMalloc(&dl, closing.size() * sizeof(int2)));
+            HIP_OK(hipMemcpy(dl, closing.data(), closing.size() * sizeof(int2), hipMemcpyHostToDevice));
+            hipLaunchKernelGGL(k_face_zero, dim3(((int)closing.size() + TB - 1) / TB), dim3(TB), 0, ctx->stream,
+                               (const int2*)dl, (int)closing.size(), u, v, w);
+            HIP_OK(hipStreamSynchronize(ctx->stream));
+            HIP_OK(hipFree(dl));
+        }
+        bool any_crack = false;
+        for (int d = 0; d < 3; ++d)
+            for (uint8_t c : lk[d]) any_crack |= c != 0;
+        auto set_cracks = [&](int id) -> int {
+            if (!any_crack && !had_cracks && !cracks) return PHG_OK;
+            if (!any_crack) return phg_grid_set_cracks(ctx, id, nullptr, nullptr, nullptr);
+            return phg_grid_set_cracks(ctx, id, lk[0].data(), lk[1].data(), lk[2].data());
+        };
+        if (!matrix) {
+            // zones and hierarchy stand: the open fractions and cracks only
+            for (int d = 0; d < 3; ++d)
+                HIP_OK(hipMemcpy((void*)dfo[d], fo[d].data(), nf[d] * sizeof(float), hipMemcpyHostToDevice));
+            return cracks ? set_cracks(grid) : PHG_OK;
+        }
+        // the matrix changed: zones and the hierarchy are set up again
+        // (computeZones, buildPressureLevels: gasphase.cpp:111-158,
+        // pressure_mg.cpp:28-191) under the same id, precision, wind sides
+        // and coarsest-solve choice
+        std::vector<uint8_t> blocked(G->n);
+        HIP_OK(hipMemcpy(blocked.data(), G->blocked, G->n, hipMemcpyDeviceToHost));
+        GridDev* NG = nullptr;
+        int rc = build_grid_dev(ctx, nx, ny, nz, G->dx, fo[0].data(), fo[1].data(), fo[2].data(), co[0].data(),
+                                co[1].data(), co[2].data(), blocked.data(), G->has_wc ? G->wc : nullptr, G->prec,
+                                nullptr, &NG);
+        if (rc) return rc;
+        if (G->single_batch >= 0) {
+            rc = phg_batch_destroy(ctx, G->single_batch);
+            if (rc) {
+                destroy_grid(NG);
+                return rc;
+            }
+        }
+        const int coarse = G->coarse, gen = G->gen;
+        free_all(G->crack_allocs);
+        free_all(G->allocs);
+        *G = std::move(*NG);
+        delete NG;
+        G->gen = gen + 1;
+        G->single_batch = -1;
+        if (coarse == PHG_COARSE_GS20 && G->H.L.size() >= 2) G->coarse = PHG_COARSE_GS20;
+        if (rebuilt) *rebuilt = 1;
+        return set_cracks(grid);
     });
 }
 

@@ -35,7 +35,7 @@ EXPORTS = (
     "phg_grid_set_coarse", "phg_grid_get_coarse",
     "phg_stable_dt", "phg_turbulence", "phg_heat_diffusion", "phg_momentum", "phg_step_clear", "phg_add_gas",
     "phg_combustion", "phg_zone_integrals", "phg_zone_values", "phg_grid_stats", "phg_batch_graph_check",
-    "phg_grid_set_cracks", "phg_zone_leakage",
+    "phg_grid_set_cracks", "phg_zone_leakage", "phg_grid_update_faces",
 )
 
 
@@ -130,6 +130,7 @@ def load_library():
     L.phg_leakage.argtypes = [vp, i32, vp, vp]
     L.phg_grid_set_cracks.argtypes = [vp, i32, vp, vp, vp]
     L.phg_zone_leakage.argtypes = [vp, i32, f32] + [vp] * 9 + [vp]
+    L.phg_grid_update_faces.argtypes = [vp, i32, i32] + [vp] * 9
     L.phg_correct.argtypes = [vp, i32, f32, vp, vp, vp, vp, C.POINTER(f32)]
     L.phg_energy_expansion.argtypes = [vp, i32, f32, vp, vp, vp, vp, vp, C.POINTER(f32)]
     L.phg_advect_scalars.argtypes = [vp, i32, f32, vp, vp, vp, vp, vp, vp, vp]
@@ -336,6 +337,21 @@ class Grid:
         self.leak_faces = 0
         if getattr(scen, "leaks", None) is not None:
             self.set_cracks(scen.leaks)
+
+    def update_faces(self, upd, u=None, v=None, w=None):
+        """Face changes (axis, face, open fraction, coefficient, crack flag
+        arrays; phg_grid_update_faces); the velocities of closing faces are
+        zeroed in u, v, w (device arrays) when given.  Returns True when the
+        zones and hierarchy were set up again."""
+        axis, face, f, a, c = upd
+        arrs = [np.ascontiguousarray(axis, np.int32), np.ascontiguousarray(face, np.int32),
+                np.ascontiguousarray(f, np.float32), np.ascontiguousarray(a, np.float32),
+                np.ascontiguousarray(c, np.uint8)]
+        reb = C.c_int()
+        self.ctx._check(self.ctx.L.phg_grid_update_faces(
+            self.ctx.h, self.id, len(arrs[0]), *[x.ctypes.data for x in arrs],
+            *[None if x is None else x.ptr for x in (u, v, w)], C.byref(reb)))
+        return bool(reb.value)
 
     def set_cracks(self, leaks):
         """Crack faces (lx, ly, lz: one byte per face, None for none) of the
@@ -685,6 +701,36 @@ class GasSolver:
 
     def step_clear(self):
         self.ctx._check(self.ctx.L.phg_step_clear(self.ctx.h, self.grid.id, C.byref(self._gas())))
+
+    def update_faces(self, upd):
+        """Face changes between steps (barrier failure, a door opened or
+        shut): the results of Solver::updateFace (solver.cpp:66-101) for a
+        list of faces, as (axis, face, open fraction, coefficient, crack
+        flag) arrays.  When the pressure matrix changes, the zones are
+        computed again and each closed zone's background pressure is its
+        cells' mean Pb, every cell's Pb its zone's (computeZones,
+        gasphase.cpp:111-158); the hierarchy is rebuilt on the device side."""
+        from . import scenarios as sc
+        d = self.d
+        rebuilt = self.grid.update_faces(upd, d["u"], d["v"], d["w"])
+        self.scen = sc.apply_face_updates(self.scen, upd)
+        self.grid.scen = self.scen
+        if rebuilt:
+            zone, zopen = self.grid.zones()
+            Pb = self.get("Pb")
+            nzn = len(zopen)
+            g = zone >= 0
+            cnt = np.bincount(zone[g], minlength=nzn).astype(np.float64)
+            psum = np.bincount(zone[g], weights=Pb[g].astype(np.float64), minlength=nzn)
+            zp = np.where(zopen != 0, np.float32(101325.0),
+                          (psum / np.maximum(cnt, 1.0)).astype(np.float32)).astype(np.float32)
+            self.zone_open = zopen
+            self.zoneP = zp
+            self.zoneDP = np.zeros(nzn, np.float32)
+            dxd = np.float64(np.float32(self.scen.dx))
+            self.zone_vol = (cnt * dxd * dxd * dxd).astype(np.float32)
+            self.set("Pb", np.where(g, zp[np.maximum(zone, 0)], np.float32(101325.0)).astype(np.float32))
+        return rebuilt
 
     def leakage_zones(self, dt):
         """Solver::leakage (gasphase.cpp:402-498) on the device

@@ -266,6 +266,52 @@ bool GasPhase::setCracks(const std::vector<uint8_t>& lx, const std::vector<uint8
                  "phg_grid_set_cracks");
 }
 
+bool GasPhase::updateFaces(const std::vector<FaceUpdate>& faces) {
+    const int n = (int)faces.size();
+    std::vector<int> axis(n), face(n);
+    std::vector<float> fo(n), co(n);
+    std::vector<uint8_t> cr(n);
+    for (int q = 0; q < n; ++q) {
+        axis[q] = faces[q].axis;
+        face[q] = faces[q].face;
+        fo[q] = faces[q].open;
+        co[q] = faces[q].coef;
+        cr[q] = faces[q].crack ? 1 : 0;
+    }
+    int rebuilt = 0;
+    if (!check(phg_grid_update_faces(ctx_, grid_, n, axis.data(), face.data(), fo.data(), co.data(), cr.data(), dU_,
+                                     dV_, dW_, &rebuilt),
+               "phg_grid_update_faces"))
+        return false;
+    for (int q = 0; q < n; ++q) {
+        std::vector<float>& f = axis[q] == 0 ? fx : axis[q] == 1 ? fy : fz;
+        f[face[q]] = fo[q];
+    }
+    if (!rebuilt) return true;
+    // computeZones: closed zones at their cells' mean background pressure
+    int nl = 0, nzn = 0, rl = 0;
+    int lvn[16];
+    if (!check(phg_grid_info(ctx_, grid_, &nl, lvn, &nzn, &rl), "phg_grid_info")) return false;
+    zoneOpen.assign(std::max(nzn, 1), 0);
+    if (!check(phg_grid_zones(ctx_, grid_, zone.data(), zoneOpen.data()), "phg_grid_zones")) return false;
+    zoneOpen.resize(nzn);
+    std::vector<float> pb(N);
+    if (!check(phg_memcpy_d2h(ctx_, pb.data(), dPb_, N * sizeof(float)), "phg_memcpy_d2h")) return false;
+    std::vector<double> pSum(nzn, 0.0), vSum(nzn, 0.0);
+    for (int c = 0; c < N; ++c)
+        if (zone[c] >= 0) {
+            pSum[zone[c]] += pb[c];
+            vSum[zone[c]] += 1;
+        }
+    zoneP.assign(nzn, P0);
+    zoneDP.assign(nzn, 0.0f);
+    for (int z = 0; z < nzn; ++z)
+        if (!zoneOpen[z]) zoneP[z] = (float)(pSum[z] / std::max(1.0, vSum[z]));
+    for (int c = 0; c < N; ++c) pb[c] = zone[c] >= 0 ? zoneP[zone[c]] : P0;
+    Pb = pb;
+    return check(phg_memcpy_h2d(ctx_, dPb_, pb.data(), N * sizeof(float)), "phg_memcpy_h2d");
+}
+
 void GasPhase::leakage(float dt) {   // gasphase.cpp:402-498
     zoneDP.assign(zoneP.size(), 0.0f);
     if (zoneP.empty()) return;

@@ -24,6 +24,14 @@ struct Wind {
     float exponent = 0.25f;
 };
 
+// One face's state as Solver::updateFace leaves it (solver.cpp:66-101)
+struct FaceUpdate {
+    int axis = 0, face = 0;
+    float open = 1.0f;        // open fraction (fx/fy/fz)
+    float coef = 1.0f;        // pressure-matrix coefficient (ax_/ay_/az_)
+    bool crack = false;       // lx_/ly_/lz_
+};
+
 // Solver's Burner (solver.h:113-127): fuel gas over a floor rectangle
 struct Burner {
     float x0 = 0, y0 = 0, x1 = 0, y1 = 0, z = 0;
@@ -59,6 +67,13 @@ public:
     // Crack faces (Solver::lx_, ly_, lz_, solver.h:297): one byte per face of
     // each axis, empty for none on that axis.  Returns false on error.
     bool setCracks(const std::vector<uint8_t>& lx, const std::vector<uint8_t>& ly, const std::vector<uint8_t>& lz);
+
+    // Face changes between steps (barrier failure, door toggles): the grid's
+    // faces and cracks are updated; when the pressure matrix changes the
+    // zones are computed again (closed zones take their cells' mean Pb,
+    // computeZones, gasphase.cpp:111-158) and the hierarchy is rebuilt.
+    // Returns false on error.
+    bool updateFaces(const std::vector<FaceUpdate>& faces);
 
     float step(float dtMax);                 // Solver::step, returns dt
     int addBurner(const Burner& b);          // Solver::addBurner, -1 if it covers no gas cell

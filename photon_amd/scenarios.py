@@ -374,6 +374,64 @@ def cracked(dx: float = 0.2, wall_leak: float = 0.0003, ceiling_leak: float = 0.
     return g
 
 
+def face_updates(grid: Grid, axis: int, mask, leak: float, intact: bool = True):
+    """Face changes (axis, face, open fraction, coefficient, crack flag) for
+    every face of an axis selected by a boolean mask of that axis's face
+    array shape ([nz, ny, nx+1], [nz, ny+1, nx] or [nz+1, ny, nx]), each now
+    carrying a barrier of open fraction `leak` (barrier_face); leak = 0 with
+    intact closes the faces."""
+    face = np.flatnonzero(np.asarray(mask, bool).ravel()).astype(np.int32)
+    f, a, c = barrier_face(leak, intact)
+    n = len(face)
+    return (np.full(n, axis, np.int32), face, np.full(n, f, np.float32), np.full(n, a, np.float32),
+            np.full(n, 1 if c else 0, np.uint8))
+
+
+def cracked_events(grid: Grid):
+    """The face changes of the cracked two-room grid's fire: (1) the
+    partition door fails (fully open), joining the closed left room to the
+    open right one; (2) the window shuts to a crack (leakage 0.002) and the
+    panel becomes solid (open fraction 0), which leaves one closed zone."""
+    nx, ny, nz = grid.shape
+    dx = grid.dx
+    yc = (np.arange(ny) + 0.5) * dx
+    zc = (np.arange(nz) + 0.5) * dx
+    xc = (np.arange(nx) + 0.5) * dx
+    ip = round(2.0 / dx)
+    door = np.zeros((nz, ny, nx + 1), bool)
+    door[:, :, ip] = (zc[:, None] < 2.0) & (yc[None, :] > 1.2) & (yc[None, :] < 2.0)
+    first = face_updates(grid, 0, door, 1.0, intact=False)
+    win = np.zeros((nz, ny + 1, nx), bool)
+    win[:, ny, :] = (np.abs(zc[:, None] - 1.5) < 0.5) & (np.abs(xc[None, :] - 3.1) < 0.5)
+    panel = np.zeros((nz, ny + 1, nx), bool)
+    panel[:, round(0.8 / dx), :] = (zc[:, None] < 1.0) & (np.abs(xc[None, :] - 3.0) < 0.4)
+    w_ = face_updates(grid, 1, win, 0.002)
+    p_ = face_updates(grid, 1, panel, 0.0)
+    second = tuple(np.concatenate([x, y]) for x, y in zip(w_, p_))
+    return [first, second]
+
+
+def apply_face_updates(grid: Grid, upd) -> Grid:
+    """A copy of grid with face changes applied (as Solver::updateFace)."""
+    g = dataclasses.replace(grid, fx=grid.fx.copy(), fy=grid.fy.copy(), fz=grid.fz.copy(), ax=grid.ax.copy(),
+                            ay=grid.ay.copy(), az=grid.az.copy(),
+                            lx=None if grid.lx is None else grid.lx.copy(),
+                            ly=None if grid.ly is None else grid.ly.copy(),
+                            lz=None if grid.lz is None else grid.lz.copy())
+    axis, face, f, a, c = upd
+    nf = (len(g.fx), len(g.fy), len(g.fz))
+    if g.lx is None and np.any(c):
+        g.lx, g.ly, g.lz = (np.zeros(k, np.uint8) for k in nf)
+    for d, F, A, L in ((0, g.fx, g.ax, g.lx), (1, g.fy, g.ay, g.ly), (2, g.fz, g.az, g.lz)):
+        sel = axis == d
+        F[face[sel]] = f[sel]
+        A[face[sel]] = a[sel]
+        if L is not None:
+            L[face[sel]] = c[sel]
+    g._zones = None
+    return g
+
+
 def batch_grid(seed: int) -> Grid:
     """One scenario of the batched workload: grid uniform between 20x15x12
     and 80x50x40 cells, walls and openings scaled to its floor area."""

@@ -7,6 +7,7 @@
 //   gas_step_check phases <dir>   project (fixed iterations, max-norm rule),
 //                                 energyAndExpansion, advectScalars
 //   gas_step_check step <dir>     Solver::step from still air with a burner
+//   gas_step_check faces <dir>    the same through face changes between steps
 #include <cstdio>
 #include <cstdlib>
 #include <fstream>
@@ -151,6 +152,50 @@ int main(int argc, char** argv) {
         write_raw(dir, "p", gp.p);
         write_raw(dir, "T", gp.T);
         write_raw(dir, "Yo", gp.Yo);
+        write_raw(dir, "zoneP", gp.zoneP);
+    } else if (mode == "faces") {
+        // the step with face changes: event e's arrays evE_{axis,face,f,a,crack}
+        // are applied before step evE_at
+        photon_gpu::Burner b;
+        b.x0 = num(meta, "bx0");
+        b.y0 = num(meta, "by0");
+        b.x1 = num(meta, "bx1");
+        b.y1 = num(meta, "by1");
+        b.peakW = num(meta, "peak");
+        if (gp.addBurner(b) < 0) {
+            std::fprintf(stderr, "burner covers no gas cell\n");
+            return 1;
+        }
+        gp.profile.pressureTol = num(meta, "tol");
+        gp.profile.pressureMaxIters = (int)num(meta, "max_iters");
+        const int nsteps = (int)num(meta, "nsteps"), nev = (int)num(meta, "nevents");
+        for (int k = 0; k < nsteps; ++k) {
+            for (int e = 0; e < nev; ++e) {
+                const std::string pre = "ev" + std::to_string(e) + "_";
+                if ((int)num(meta, (pre + "at").c_str()) != k) continue;
+                const auto ax = read_raw<int>(dir, pre + "axis"), fc = read_raw<int>(dir, pre + "face");
+                const auto fo = read_raw<float>(dir, pre + "f"), co = read_raw<float>(dir, pre + "a");
+                const auto cr = read_raw<uint8_t>(dir, pre + "crack");
+                std::vector<photon_gpu::FaceUpdate> up(ax.size());
+                for (size_t q = 0; q < ax.size(); ++q) {
+                    up[q].axis = ax[q];
+                    up[q].face = fc[q];
+                    up[q].open = fo[q];
+                    up[q].coef = co[q];
+                    up[q].crack = cr[q] != 0;
+                }
+                if (!gp.updateFaces(up)) {
+                    std::fprintf(stderr, "updateFaces: %s\n", gp.error().c_str());
+                    return 1;
+                }
+            }
+            const float dt = gp.step(num(meta, "dt_max"));
+            std::fprintf(log, "step %d %.9g %d %d\n", k, dt, gp.pressureIters, (int)gp.zoneP.size());
+        }
+        gp.syncToHost();
+        write_raw(dir, "u", gp.u);
+        write_raw(dir, "T", gp.T);
+        write_raw(dir, "p", gp.p);
         write_raw(dir, "zoneP", gp.zoneP);
     } else {
         std::fprintf(stderr, "unknown mode %s\n", mode.c_str());

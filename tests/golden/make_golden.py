@@ -13,7 +13,8 @@ fields; ref_<case>_phases.npz turbulence, heatDiffusion, momentum, burner
 injection + combustion and stableDt on seeded fields; ref_<case>_step.npz a
 whole-step trajectory (Solver::step) from still air with a burner;
 ref_cracked_leakage.npz leakage (crack flow and closed-zone pressure) on
-seeded fields of the cracked two-room grid.
+seeded fields of the cracked two-room grid; ref_cracked_faces.npz a
+whole-step trajectory through two face changes (zones and hierarchy rebuilt).
     python tests/golden/make_golden.py [room walls wind_phases room_step ...]
 """
 import os
@@ -241,6 +242,42 @@ def make_phases(name, g, K=0):
     print(name, "phases: maxNu", out["tu_maxnu"], "burner cells", len(cells))
 
 
+def make_faces(name):
+    """Solver::step through face changes: the cracked grid's burner fire
+    with the partition door failing after 10 steps and the window shutting
+    to a crack (and the panel turning solid) after 20; zones and the
+    hierarchy are rebuilt by the reference at each change."""
+    gf, wind, rect, peak, _ = STEP_CASES[name]
+    g = gf()
+    R = ref_solver(g)
+    out = {}
+    b, cells = R.add_burner(*rect, 0.0, peak, 0.0)
+    out["burner"] = np.array(list(rect) + [peak], np.float32)
+    out["cells"] = cells
+    events = sc.cracked_events(g)
+    rows = []
+    for k in range(30):
+        if k in (10, 20):
+            R.update_faces(*events[k // 10 - 1])
+        dt = R.step(STEP_DTMAX, STEP_TOL, STEP_MAXIT)
+        sc_ = R.scalars()
+        zp, zdp = R.zone_pressure()
+        rows.append([dt, sc_["iters"], sc_["residual"], sc_["max_vel"], len(zp), float(zp.min()), float(zp.max())])
+        if k in (10, 20):
+            zone, zo = R.zones()
+            out[f"zone_{k}"], out[f"zone_open_{k}"] = zone, zo
+    out["st_rows"] = np.array(rows, np.float64)
+    out["st_cols"] = np.array(["dt", "iters", "residual", "maxVel", "nzones", "zoneP_min", "zoneP_max"])
+    for f, n in (("u", len(g.fx)), ("v", len(g.fy)), ("w", len(g.fz))):
+        out["st_" + f] = R.get(f, n)
+    for f in ("p", "T", "Pb", "Yo"):
+        out["st_" + f] = R.get(f, g.n)
+    out["st_zoneP"], out["st_zoneDP"] = R.zone_pressure()
+    np.savez_compressed(os.path.join(HERE, f"ref_{name}_faces.npz"), **out)
+    print(name, "faces: iters", out["st_rows"][:, 1].astype(int).tolist(), "zones", out["st_rows"][:, 4].astype(int).tolist(),
+          "zoneP", out["st_zoneP"])
+
+
 def make_step(name):
     """A short whole-step trajectory of Solver::step from still air with a burner."""
     gf, wind, rect, peak, nsteps = STEP_CASES[name]
@@ -278,10 +315,12 @@ if __name__ == "__main__":
     if not po.ref_available(pin=True):
         sys.exit("oracle/_ref/libphoton_ref_pin.so not built (needs the reference checkout): make -C oracle")
     names = sys.argv[1:] or list(CASES) + [n + "_phases" for n in STEP_CASES] + [n + "_step" for n in STEP_CASES]
-    names = names + ([] if sys.argv[1:] else ["cracked_leakage"])
+    names = names + ([] if sys.argv[1:] else ["cracked_leakage", "cracked_faces"])
     for name in names:
         if name.endswith("_phases"):
             make_phases(name[:-7], CASES[name[:-7]]())
+        elif name.endswith("_faces"):
+            make_faces(name[:-6])
         elif name.endswith("_leakage"):
             make_leakage(name[:-8], STEP_CASES[name[:-8]][0]())
         elif name.endswith("_step"):

@@ -111,3 +111,30 @@ def test_cpp_dropin_step_follows_reference(tmp_path, name):
     p = out(d, "p")
     assert np.abs(p - G["st_p"]).max() <= 1e-4 * np.abs(G["st_p"]).max()
     assert np.abs(out(d, "zoneP") - G["st_zoneP"]).max() <= 1e-4 * 101325
+
+
+def test_cpp_dropin_face_changes_follow_reference(tmp_path):
+    """GasPhase::updateFaces between steps (the partition door fails after
+    10 steps, the window shuts to a crack and the panel turns solid after
+    20): every step's dt, pressureIters and zone count, then the fields, as
+    the reference's own run (tests/golden/ref_cracked_faces.npz)."""
+    g = sc.cracked()
+    G = np.load(os.path.join(HERE, "golden", "ref_cracked_faces.npz"))
+    x0, y0, x1, y1, peak = (float(x) for x in G["burner"])
+    rows = G["st_rows"]
+    d = str(tmp_path / "faces")
+    arrays, meta = {}, dict(bx0=x0, by0=y0, bx1=x1, by1=y1, peak=peak, nsteps=len(rows), dt_max=0.01, tol=0.02,
+                           max_iters=80, nevents=2)
+    for e, (at, upd) in enumerate(zip((10, 20), sc.cracked_events(g))):
+        meta[f"ev{e}_at"] = at
+        for k, a in zip(("axis", "face", "f", "a", "crack"), upd):
+            arrays[f"ev{e}_{k}"] = a
+    write_case(d, g, meta, arrays)
+    log = [line.split() for line in run("faces", d) if line.startswith("step")]
+    assert len(log) == len(rows)
+    for k, ln in enumerate(log):
+        assert np.float32(float(ln[2])) == np.float32(rows[k, 0]), (k, ln)
+        assert int(ln[3]) == int(rows[k, 1]) and int(ln[4]) == int(rows[k, 4]), (k, ln, rows[k])
+    assert rel(out(d, "u"), G["st_u"]) < 1e-5
+    assert rel(out(d, "T"), G["st_T"]) < 1e-6
+    assert np.abs(out(d, "zoneP") - G["st_zoneP"]).max() <= 1e-4 * 101325

@@ -253,6 +253,74 @@ def test_crack_leakage_without_cracks_is_the_zone_mean_shift(ctx):
     s.grid.destroy()
 
 
+def test_face_changes_follow_reference_trajectory(ctx):
+    """Solver::step on the cracked grid with a burner, through two face
+    changes between steps (after 10 steps the partition door fails and the
+    closed room joins the open one; after 20 the window shuts to a crack and
+    the panel turns solid, which leaves one closed zone): the zones are
+    recomputed and the hierarchy rebuilt in place each time.  Every step's
+    dt, pressureIters and zone count must be the reference's; the fields
+    after 30 steps as in the trajectory test."""
+    g, G = sc.cracked(), np.load(os.path.join(HERE, "golden", "ref_cracked_faces.npz"))
+    s = gas.GasSolver(ctx, g, tol=0.02, max_iters=80, stop_rule=gas.STOP_MAXNORM, coarse=gas.COARSE_GS20)
+    x0, y0, x1, y1, peak = (float(x) for x in G["burner"])
+    assert s.add_burner(x0, y0, x1, y1, 0.0, peak) == 0
+    events = sc.cracked_events(g)
+    rows = G["st_rows"]
+    stale = None
+    for k in range(len(rows)):
+        if k == 5:
+            stale = gas.Batch(ctx, [s.grid])
+        if k in (10, 20):
+            assert s.update_faces(events[k // 10 - 1])
+            assert s.grid.coarse == gas.COARSE_GS20
+            zone, zo = s.grid.zones()
+            assert np.array_equal(zone, G[f"zone_{k}"]) and np.array_equal(zo, G[f"zone_open_{k}"]), k
+        dt = s.step(0.01)
+        assert np.float32(dt) == np.float32(rows[k, 0]), (k, dt, rows[k, 0])
+        assert s.pressureIters == int(rows[k, 1]), (k, s.pressureIters, int(rows[k, 1]))
+        assert len(s.zone_open) == int(rows[k, 4])
+    print(f"faces: u {rel(s.get('u'), G['st_u']):.2e} T {rel(s.get('T'), G['st_T']):.2e} zoneP {s.zoneP} / "
+          f"{G['st_zoneP']}")
+    for f in ("u", "v", "w"):
+        assert rel(s.get(f), G["st_" + f]) < 1e-5, f
+    assert rel(s.get("T"), G["st_T"]) < 1e-6 and rel(s.get("Yo"), G["st_Yo"]) < 1e-6
+    assert np.abs(s.get("p") - G["st_p"]).max() <= 1e-4 * np.abs(G["st_p"]).max()
+    assert np.abs(s.zoneP - G["st_zoneP"]).max() <= 1e-4 * 101325
+    # a batch made before the rebuild refuses to solve
+    x = ctx.array(g.n)
+    b = ctx.to_device(np.zeros(g.n, np.float32))
+    with pytest.raises(gas.PhgError, match="rebuilt by a face update"):
+        stale.solve([b], [x], gas.solve_params(tol=1e-6, max_iters=4))
+    stale.destroy()
+    s.grid.destroy()
+
+
+def test_face_change_without_matrix_change_keeps_the_setup(ctx):
+    """A crack's open fraction changing (the matrix stays) updates the crack
+    flow in place: no rebuild, and the leakage follows the new fraction as
+    a grid built with it from scratch does."""
+    g = sc.cracked()
+    s = gas.GasSolver(ctx, g)
+    nx, ny, nz = g.shape
+    wall = np.zeros((nz, ny, nx + 1), bool)
+    wall[:, :, 0] = True
+    upd = sc.face_updates(g, 0, wall, 0.001)
+    assert not s.update_faces(upd)
+    g2 = sc.apply_face_updates(g, upd)
+    s2 = gas.GasSolver(ctx, g2)
+    rng = np.random.default_rng(4)
+    for t in (s, s2):
+        t.zoneP[:] = np.where(t.zone_open == 0, 101325.0 + 30.0, 101325.0).astype(np.float32)
+        t.set("S", (0.2 * rng.random(g.n)).astype(np.float32) if t is s else s.get("S"))
+    z1, z2 = s.leakage_zones(DT).copy(), s2.leakage_zones(DT).copy()
+    assert np.array_equal(z1, z2)
+    for f in ("u", "v", "w", "S"):
+        assert bits_equal(s.get(f), s2.get(f)), f
+    s.grid.destroy()
+    s2.grid.destroy()
+
+
 # ---------------------------------------------------------------------------
 # invariants of the reference's own tests (src/tests/test_main.cpp:437-546),
 # through the device step

@@ -234,3 +234,19 @@ def test_leakage_without_cracks_is_the_mean_shift():
     assert abs(S[z].astype(np.float64).sum() * float(np.float32(g.dx) ** 3)) < 1e-6 * dint
     gamma = np.float32(1100.0) / (np.float32(1100.0) - np.float32(287.0))
     assert abs(zdp[0] - gamma * 101325.0 * dint / Vz) < 1e-5 * abs(zdp[0])
+
+
+def test_face_change_zones_match_reference():
+    """The cracked grid's two face changes (door fails, window shuts to a
+    crack and the panel turns solid) applied as updateFace does: the zones
+    and their openness after each equal the reference's own recomputed
+    zones (computeZones after zonesDirty_), and the oracle's hierarchy of the
+    final geometry has the level sizes the reference rebuilds."""
+    G = np.load(os.path.join(HERE, "golden", "ref_cracked_faces.npz"))
+    g = sc.cracked()
+    for k, upd in zip((10, 20), sc.cracked_events(g)):
+        g = sc.apply_face_updates(g, upd)
+        zone, zo = po.zones(g)
+        assert np.array_equal(zone, G[f"zone_{k}"]) and np.array_equal(zo, G[f"zone_open_{k}"]), k
+        assert np.array_equal(g.zone, zone) and np.array_equal(g.zone_open, zo)
+    assert len(g.zone_open) == 1 and g.zone_open[0] == 0
