@@ -5,6 +5,7 @@
 #include <algorithm>
 #include <cmath>
 #include <cstring>
+#include <fstream>
 
 namespace photon_gpu {
 
@@ -92,6 +93,151 @@ void GasPhase::syncToHost() {
 }
 void GasPhase::syncToDevice() {
     for (const Copy& c : copies()) check(phg_memcpy_h2d(ctx_, c.dev, c.host, c.bytes), "syncToDevice");
+}
+
+namespace {
+constexpr char kStateMagic[8] = {'P', 'H', 'G', 'S', 'T', 'A', 'T', '1'};
+
+// FNV-1a over the geometry a state belongs to
+uint64_t geometry_hash(const std::vector<float>& fx, const std::vector<float>& fy, const std::vector<float>& fz,
+                       const std::vector<uint8_t>& blocked, const std::vector<int>& zone) {
+    uint64_t h = 1469598103934665603ull;
+    auto mix = [&](const void* p, size_t n) {
+        const unsigned char* b = static_cast<const unsigned char*>(p);
+        for (size_t i = 0; i < n; ++i) h = (h ^ b[i]) * 1099511628211ull;
+    };
+    mix(fx.data(), fx.size() * 4);
+    mix(fy.data(), fy.size() * 4);
+    mix(fz.data(), fz.size() * 4);
+    mix(blocked.data(), blocked.size());
+    mix(zone.data(), zone.size() * 4);
+    return h;
+}
+
+struct Writer {
+    std::ofstream out;
+    template <class T>
+    void pod(const T& v) { out.write(reinterpret_cast<const char*>(&v), sizeof(T)); }
+    template <class T>
+    void vec(const std::vector<T>& v) {
+        const uint64_t n = v.size();
+        pod(n);
+        out.write(reinterpret_cast<const char*>(v.data()), (std::streamsize)(n * sizeof(T)));
+    }
+};
+
+struct Reader {
+    std::ifstream in;
+    template <class T>
+    bool pod(T& v) { return (bool)in.read(reinterpret_cast<char*>(&v), sizeof(T)); }
+    template <class T>
+    bool vec(std::vector<T>& v, size_t expect) {
+        uint64_t n = 0;
+        if (!pod(n) || (expect != (size_t)-1 && n != expect) || n > (1ull << 34)) return false;
+        v.resize(n);
+        return (bool)in.read(reinterpret_cast<char*>(v.data()), (std::streamsize)(n * sizeof(T)));
+    }
+};
+}  // namespace
+
+bool GasPhase::saveState(const std::string& path, std::string& err) {
+    syncToHost();
+    if (!ok()) {
+        err = err_;
+        return false;
+    }
+    Writer wr;
+    wr.out.open(path, std::ios::binary);
+    if (!wr.out) {
+        err = "cannot write " + path;
+        return false;
+    }
+    wr.out.write(kStateMagic, 8);
+    wr.pod(nx);
+    wr.pod(ny);
+    wr.pod(nz);
+    wr.pod(dx);
+    wr.pod(geometry_hash(fx, fy, fz, blocked, zone));
+    wr.pod(time);
+    wr.pod(maxVel);
+    wr.pod(maxNu);
+    wr.pod(maxDiv);
+    wr.pod(pressureIters);
+    wr.pod(pressureResidual);
+    wr.vec(zoneP);
+    wr.vec(zoneDP);
+    for (const std::vector<float>* f : {&u, &v, &w, &p, &S, &nu, &Pb, &Q, &Qr, &massSrc, &hrr, &T, &Yo, &Yf, &Ys, &Ya,
+                                        &Yp, &Yc, &Yw, &Yco, &Yai})
+        wr.vec(*f);
+    wr.vec(pilot);
+    wr.vec(lit);
+    wr.pod((uint64_t)burners.size());
+    for (const Burner& b : burners) {
+        for (float x : {b.x0, b.y0, b.x1, b.y1, b.z, b.peakW, b.rampS, b.start, b.duration}) wr.pod(x);
+        wr.vec(b.cells);
+    }
+    if (!wr.out) {
+        err = "write failed: " + path;
+        return false;
+    }
+    return true;
+}
+
+bool GasPhase::loadState(const std::string& path, std::string& err) {
+    Reader r;
+    r.in.open(path, std::ios::binary);
+    char magic[8];
+    if (!r.in || !r.in.read(magic, 8) || std::memcmp(magic, kStateMagic, 8) != 0) {
+        err = "not a gas state file: " + path;
+        return false;
+    }
+    int sx = 0, sy = 0, sz = 0;
+    float sdx = 0;
+    uint64_t hash = 0;
+    r.pod(sx);
+    r.pod(sy);
+    r.pod(sz);
+    r.pod(sdx);
+    r.pod(hash);
+    if (sx != nx || sy != ny || sz != nz || sdx != dx || hash != geometry_hash(fx, fy, fz, blocked, zone)) {
+        err = "the state was saved from another geometry";
+        return false;
+    }
+    const size_t nfx = u.size(), nfy = v.size(), nfz = w.size(), n = (size_t)N, nzn = zoneP.size();
+    bool good = r.pod(time) && r.pod(maxVel) && r.pod(maxNu) && r.pod(maxDiv) && r.pod(pressureIters) &&
+                r.pod(pressureResidual) && r.vec(zoneP, nzn) && r.vec(zoneDP, nzn) && r.vec(u, nfx) && r.vec(v, nfy) &&
+                r.vec(w, nfz);
+    for (std::vector<float>* f : {&p, &S, &nu, &Pb, &Q, &Qr, &massSrc, &hrr, &T, &Yo, &Yf, &Ys, &Ya, &Yp, &Yc, &Yw,
+                                  &Yco, &Yai})
+        good = good && r.vec(*f, n);
+    good = good && r.vec(pilot, n) && r.vec(lit, n);
+    uint64_t nb = 0;
+    good = good && r.pod(nb) && nb < (1u << 20);
+    if (!good) {
+        err = "truncated or malformed state file: " + path;
+        return false;
+    }
+    for (int* d : dBurnerCells_) phg_free(ctx_, d), dmem_.erase(std::find(dmem_.begin(), dmem_.end(), (void*)d));
+    dBurnerCells_.clear();
+    burners.clear();
+    for (uint64_t i = 0; i < nb; ++i) {
+        Burner b;
+        float* fields[9] = {&b.x0, &b.y0, &b.x1, &b.y1, &b.z, &b.peakW, &b.rampS, &b.start, &b.duration};
+        for (float* x : fields) good = good && r.pod(*x);
+        good = good && r.vec(b.cells, (size_t)-1);
+        for (int c : b.cells) good = good && c >= 0 && c < N;
+        if (!good) {
+            err = "malformed burner in " + path;
+            return false;
+        }
+        int* d = alloc<int>(b.cells.size());
+        if (!d || !check(phg_memcpy_h2d(ctx_, d, b.cells.data(), b.cells.size() * sizeof(int)), "burner cells")) break;
+        dBurnerCells_.push_back(d);
+        burners.push_back(b);
+    }
+    syncToDevice();
+    err = err_;
+    return ok();
 }
 
 bool GasPhase::build(int nx_, int ny_, int nz_, float dx_, const std::vector<float>& fx_,

@@ -92,6 +92,15 @@ public:
     void advectScalars(float dt);
     void zonePressure(float dt);
 
+    // Checkpoint and resume (the role of Solver::saveState / loadState,
+    // src/sim/state.cpp:62-195) for the gas state: every field a later step
+    // reads, zone pressures, the statistics stableDt uses, time and burners.
+    // A state belongs to the geometry it was saved from (checked by a hash
+    // of the open fractions, coefficients' zones and solid mask); resuming
+    // continues bit for bit as if uninterrupted.
+    bool saveState(const std::string& path, std::string& err);
+    bool loadState(const std::string& path, std::string& err);
+
     void syncToHost();     // device state -> the public vectors
     void syncToDevice();   // the public vectors -> device state
     const std::string& error() const { return err_; }

@@ -8,6 +8,8 @@
 //                                 energyAndExpansion, advectScalars
 //   gas_step_check step <dir>     Solver::step from still air with a burner
 //   gas_step_check faces <dir>    the same through face changes between steps
+//   gas_step_check resume <dir>   saveState after some steps, loadState in a
+//                                 fresh object, compared with an uninterrupted run
 #include <cstdio>
 #include <cstdlib>
 #include <fstream>
@@ -68,23 +70,40 @@ int main(int argc, char** argv) {
     }
     const std::string mode = argv[1], dir = argv[2];
     const auto meta = read_meta(dir);
+    // a GasPhase on the case's geometry, with its cracks and profile
+    auto make = [&](photon_gpu::GasPhase& g) {
+        g.wind.speed = num(meta, "wind_speed");
+        g.wind.fromDeg = num(meta, "wind_from");
+        g.wind.exponent = num(meta, "wind_exponent", 0.25f);
+        std::string err;
+        if (!g.build((int)num(meta, "nx"), (int)num(meta, "ny"), (int)num(meta, "nz"), num(meta, "dx"),
+                     read_raw<float>(dir, "fx"), read_raw<float>(dir, "fy"), read_raw<float>(dir, "fz"),
+                     read_raw<float>(dir, "ax"), read_raw<float>(dir, "ay"), read_raw<float>(dir, "az"),
+                     read_raw<uint8_t>(dir, "blocked"), err)) {
+            std::fprintf(stderr, "build: %s\n", err.c_str());
+            return false;
+        }
+        if (exists(dir, "lx") &&
+            !g.setCracks(read_raw<uint8_t>(dir, "lx"), read_raw<uint8_t>(dir, "ly"), read_raw<uint8_t>(dir, "lz"))) {
+            std::fprintf(stderr, "setCracks: %s\n", g.error().c_str());
+            return false;
+        }
+        g.profile.pressureTol = num(meta, "tol", 0.02f);
+        g.profile.pressureMaxIters = (int)num(meta, "max_iters", 80);
+        return true;
+    };
+    auto burner = [&](photon_gpu::GasPhase& g) {
+        photon_gpu::Burner b;
+        b.x0 = num(meta, "bx0");
+        b.y0 = num(meta, "by0");
+        b.x1 = num(meta, "bx1");
+        b.y1 = num(meta, "by1");
+        b.peakW = num(meta, "peak");
+        b.rampS = num(meta, "ramp");
+        return g.addBurner(b) >= 0;
+    };
     photon_gpu::GasPhase gp;
-    gp.wind.speed = num(meta, "wind_speed");
-    gp.wind.fromDeg = num(meta, "wind_from");
-    gp.wind.exponent = num(meta, "wind_exponent", 0.25f);
-    std::string err;
-    if (!gp.build((int)num(meta, "nx"), (int)num(meta, "ny"), (int)num(meta, "nz"), num(meta, "dx"),
-                  read_raw<float>(dir, "fx"), read_raw<float>(dir, "fy"), read_raw<float>(dir, "fz"),
-                  read_raw<float>(dir, "ax"), read_raw<float>(dir, "ay"), read_raw<float>(dir, "az"),
-                  read_raw<uint8_t>(dir, "blocked"), err)) {
-        std::fprintf(stderr, "build: %s\n", err.c_str());
-        return 1;
-    }
-    if (exists(dir, "lx") &&
-        !gp.setCracks(read_raw<uint8_t>(dir, "lx"), read_raw<uint8_t>(dir, "ly"), read_raw<uint8_t>(dir, "lz"))) {
-        std::fprintf(stderr, "setCracks: %s\n", gp.error().c_str());
-        return 1;
-    }
+    if (!make(gp)) return 1;
     std::FILE* log = std::fopen((dir + "/out_log.txt").c_str(), "w");
     if (mode == "phases") {
         const float dt = num(meta, "dt");
@@ -153,6 +172,56 @@ int main(int argc, char** argv) {
         write_raw(dir, "T", gp.T);
         write_raw(dir, "Yo", gp.Yo);
         write_raw(dir, "zoneP", gp.zoneP);
+    } else if (mode == "resume") {
+        // Solver's resume test (test_main.cpp:519-546): nsteps uninterrupted
+        // (out_a_*), and ksave steps, saveState, a fresh object, loadState,
+        // the remaining steps (out_b_*)
+        const int nsteps = (int)num(meta, "nsteps"), ksave = (int)num(meta, "ksave");
+        const float dtMax = num(meta, "dt_max");
+        const std::string state = dir + "/state.bin";
+        if (!burner(gp)) return 1;
+        for (int k = 0; k < nsteps; ++k) gp.step(dtMax);
+        gp.syncToHost();
+        {
+            photon_gpu::GasPhase g1;
+            if (!make(g1) || !burner(g1)) return 1;
+            for (int k = 0; k < ksave; ++k) g1.step(dtMax);
+            std::string err;
+            if (!g1.saveState(state, err)) {
+                std::fprintf(stderr, "saveState: %s\n", err.c_str());
+                return 1;
+            }
+        }
+        photon_gpu::GasPhase g2;
+        if (!make(g2)) return 1;
+        std::string err;
+        if (!g2.loadState(state, err)) {
+            std::fprintf(stderr, "loadState: %s\n", err.c_str());
+            return 1;
+        }
+        for (int k = ksave; k < nsteps; ++k) g2.step(dtMax);
+        g2.syncToHost();
+        std::fprintf(log, "time %.9g %.9g iters %d %d\n", gp.time, g2.time, gp.pressureIters, g2.pressureIters);
+        for (auto* pr : {&gp, &g2}) {
+            const std::string tag = pr == &gp ? "a_" : "b_";
+            write_raw(dir, tag + "u", pr->u);
+            write_raw(dir, tag + "w", pr->w);
+            write_raw(dir, tag + "p", pr->p);
+            write_raw(dir, tag + "T", pr->T);
+            write_raw(dir, tag + "Yf", pr->Yf);
+            write_raw(dir, tag + "lit", pr->lit);
+            write_raw(dir, tag + "zoneP", pr->zoneP);
+        }
+        // a state from another geometry is refused
+        photon_gpu::GasPhase g3;
+        std::vector<float> fx3 = read_raw<float>(dir, "fx");
+        fx3[fx3.size() / 2] = fx3[fx3.size() / 2] > 0 ? 0.0f : 1.0f;
+        if (g3.build((int)num(meta, "nx"), (int)num(meta, "ny"), (int)num(meta, "nz"), num(meta, "dx"), fx3,
+                     read_raw<float>(dir, "fy"), read_raw<float>(dir, "fz"), read_raw<float>(dir, "ax"),
+                     read_raw<float>(dir, "ay"), read_raw<float>(dir, "az"), read_raw<uint8_t>(dir, "blocked"), err)) {
+            const bool refused = !g3.loadState(state, err);
+            std::fprintf(log, "other_geometry_refused %d\n", refused ? 1 : 0);
+        }
     } else if (mode == "faces") {
         // the step with face changes: event e's arrays evE_{axis,face,f,a,crack}
         // are applied before step evE_at

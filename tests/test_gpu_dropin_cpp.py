@@ -138,3 +138,23 @@ def test_cpp_dropin_face_changes_follow_reference(tmp_path):
     assert rel(out(d, "u"), G["st_u"]) < 1e-5
     assert rel(out(d, "T"), G["st_T"]) < 1e-6
     assert np.abs(out(d, "zoneP") - G["st_zoneP"]).max() <= 1e-4 * 101325
+
+
+def test_cpp_dropin_resume_matches_uninterrupted(tmp_path):
+    """GasPhase::saveState after 15 steps of the room with a ramped burner,
+    loadState in a fresh object, 15 more steps: bit for bit the uninterrupted
+    30 steps (the reference's resume test, test_main.cpp:519-546); a state
+    saved from another geometry is refused."""
+    g = sc.room()
+    d = str(tmp_path / "resume")
+    write_case(d, g, dict(bx0=1.8, by0=1.2, bx1=2.2, by1=1.6, peak=100e3, ramp=0.2, nsteps=30, ksave=15, dt_max=0.01,
+                          tol=0.02, max_iters=80), {})
+    log = run("resume", d)
+    assert "other_geometry_refused 1" in log
+    t = log[0].split()
+    assert t[1] == t[2] and t[4] == t[5], log[0]
+    for f, dt in (("u", np.float32), ("w", np.float32), ("p", np.float32), ("T", np.float32), ("Yf", np.float32),
+                  ("lit", np.uint8), ("zoneP", np.float32)):
+        a, b = out(d, "a_" + f, dt), out(d, "b_" + f, dt)
+        assert np.array_equal(a.view(np.uint8), b.view(np.uint8)), f
+    assert out(d, "a_T").max() > 300.0
