@@ -87,23 +87,25 @@ __device__ __forceinline__ float tr_superbee(float r) {
     return x;
 }
 
+// one face's record (its open flag is kept apart, as a byte, so that the
+// face records of a tile plane and the plane itself fit three blocks per CU)
 struct TrFace {
     float v[TR_NF];   // limited upwind face value of each field
     float flux;       // f u, positive along +axis
     float rup;        // upwind density at the face
     float dw, dr;     // f D and f rho_f D of the species diffusion
-    int open;         // f > 0 and neither cell solid
 };
 
 // One face between cell a (low side) and b (high side); aa lies beyond a and
 // bb beyond b along the axis.  Cells outside the domain hold ambient values.
-__device__ __forceinline__ void tr_face(float fo, float vel, const float (&va)[TR_NV], const float (&vb)[TR_NV],
+// Returns whether the face is open (f > 0 and neither cell solid); the
+// record is written only then.
+__device__ __forceinline__ bool tr_face(float fo, float vel, const float (&va)[TR_NV], const float (&vb)[TR_NV],
                                         const float (&vaa)[TR_NV], const float (&vbb)[TR_NV], int sa, int sb, int saa,
                                         int sbb, TrFace& f) {
 #pragma clang fp contract(off)
     const float RS = 287.0f, T_MIN = 200.0f, SC_T = 0.5f;
-    f.open = fo > 0.0f && sa != TR_SOLID && sb != TR_SOLID;
-    if (!f.open) return;
+    if (!(fo > 0.0f && sa != TR_SOLID && sb != TR_SOLID)) return false;
     const float flux = fo * vel;
     const bool fwd = flux > 0.0f;                     // a is upwind (with no flow, b: unused then)
     const bool far_solid = (fwd ? saa : sbb) == TR_SOLID;
@@ -134,6 +136,7 @@ __device__ __forceinline__ void tr_face(float fo, float vel, const float (&va)[T
     f.flux = flux;
     f.dw = fo * D;
     f.dr = fo * rf * D;
+    return true;
 }
 
 __device__ __forceinline__ void tr_load_cell(const TransportArgs& A, int i, int j, int k, float (&v)[TR_NV], int& st) {
@@ -154,9 +157,10 @@ __device__ __forceinline__ void tr_load_cell(const TransportArgs& A, int i, int 
 __global__ void __launch_bounds__(TR_THREADS) k_transport(TransportArgs A) {
 #pragma clang fp contract(off)
     __shared__ float pl[TR_NV][TR_LY][TR_LX];
-    __shared__ int pst[TR_LY][TR_LX];
+    __shared__ uint8_t pst[TR_LY][TR_LX];
     __shared__ TrFace xf[TR_NXF];
     __shared__ TrFace yf[TR_NYF];
+    __shared__ uint8_t xo[TR_NXF], yo[TR_NYF];   // open flags of the face records
     const float RS = 287.0f;
     const int tx = (int)threadIdx.x % TR_TX, ty = (int)threadIdx.x / TR_TX;
     const int ntile = A.tiles_x * A.tiles_y;
@@ -185,6 +189,7 @@ __global__ void __launch_bounds__(TR_THREADS) k_transport(TransportArgs A) {
     float W[4][TR_NV];
     int WS[4];
     TrFace zlo;   // the face below plane k
+    bool zlo_o;
     {
         float P[TR_NV];
         int PS;
@@ -197,21 +202,21 @@ __global__ void __launch_bounds__(TR_THREADS) k_transport(TransportArgs A) {
             fo = A.fz[f];
             wv = A.w[f];
         }
-        tr_face(fo, wv, W[0], W[1], P, W[2], WS[0], WS[1], PS, WS[2], zlo);
+        zlo_o = tr_face(fo, wv, W[0], W[1], P, W[2], WS[0], WS[1], PS, WS[2], zlo);
         tr_load_cell(A, i, j, k0 + 2, W[3], WS[3]);
     }
     for (int k = k0; k < k1; ++k) {
         // 1. the plane into LDS: own columns from the window, halo from memory
 #pragma unroll
         for (int q = 0; q < TR_NV; ++q) pl[q][ty + 2][tx + 2] = W[1][q];
-        pst[ty + 2][tx + 2] = WS[1];
+        pst[ty + 2][tx + 2] = (uint8_t)WS[1];
         if (hlx >= 0) {
             float h[TR_NV];
             int hs;
             tr_load_cell(A, i0 + hlx - 2, j0 + hly - 2, k, h, hs);
 #pragma unroll
             for (int q = 0; q < TR_NV; ++q) pl[q][hly][hlx] = h[q];
-            pst[hly][hlx] = hs;
+            pst[hly][hlx] = (uint8_t)hs;
         }
         __syncthreads();
         // 2. every x and y face of the tile plane, once
@@ -241,11 +246,13 @@ __global__ void __launch_bounds__(TR_THREADS) k_transport(TransportArgs A) {
                 vbb[q] = pl[q][by + dy_][bx + dx_];
             }
             TrFace& out = isx ? xf[r] : yf[r];
-            tr_face(fo, vel, va, vb, vaa, vbb, pst[by - dy_][bx - dx_], pst[by][bx], pst[by - 2 * dy_][bx - 2 * dx_],
-                    pst[by + dy_][bx + dx_], out);
+            const bool o = tr_face(fo, vel, va, vb, vaa, vbb, pst[by - dy_][bx - dx_], pst[by][bx],
+                                   pst[by - 2 * dy_][bx - 2 * dx_], pst[by + dy_][bx + dx_], out);
+            (isx ? xo[r] : yo[r]) = o ? 1 : 0;
         }
         // the z face above the own cell, from the column window
         TrFace zhi;
+        bool zhi_o;
         {
             float fo = 0.0f, wv = 0.0f;
             if (own) {
@@ -253,7 +260,7 @@ __global__ void __launch_bounds__(TR_THREADS) k_transport(TransportArgs A) {
                 fo = A.fz[f];
                 wv = A.w[f];
             }
-            tr_face(fo, wv, W[1], W[2], W[0], W[3], WS[1], WS[2], WS[0], WS[3], zhi);
+            zhi_o = tr_face(fo, wv, W[1], W[2], W[0], W[3], WS[1], WS[2], WS[0], WS[3], zhi);
         }
         __syncthreads();
         // 3. the own cell from its six faces, in the reference's order
@@ -279,7 +286,12 @@ __global__ void __launch_bounds__(TR_THREADS) k_transport(TransportArgs A) {
                 int op[6];
 #pragma unroll
                 for (int d = 0; d < 6; ++d) {
-                    op[d] = pick(d, [](const TrFace& f) { return f.open; });
+                    op[d] = d == 0   ? xo[ty * (TR_TX + 1) + tx]
+                            : d == 1 ? xo[ty * (TR_TX + 1) + tx + 1]
+                            : d == 2 ? yo[ty * TR_TX + tx]
+                            : d == 3 ? yo[(ty + 1) * TR_TX + tx]
+                            : d == 4 ? (int)zlo_o
+                                     : (int)zhi_o;
                     un[d] = 0.0f;
                     ru[d] = dw[d] = dr[d] = 0.0f;
                     if (!op[d]) continue;
@@ -321,6 +333,7 @@ __global__ void __launch_bounds__(TR_THREADS) k_transport(TransportArgs A) {
         // advance the window and carry the face; the LDS plane and face
         // records are rewritten only after every thread has read them
         zlo = zhi;
+        zlo_o = zhi_o;
 #pragma unroll
         for (int p = 0; p < 3; ++p) {
 #pragma unroll

@@ -25,6 +25,16 @@
 #include "kernels_flow.hpp"
 #include "kernels_leak.hpp"
 
+// scalar transport: blocks aimed at per launch (tile columns x k-chunks):
+// three resident blocks on each of the 256 CUs (its LDS allows three), and
+// chunks down to one plane, since a block's planes are a serial chain
+#ifndef PHG_TR_BLOCKS
+#define PHG_TR_BLOCKS 768
+#endif
+#ifndef PHG_TR_KMIN
+#define PHG_TR_KMIN 1
+#endif
+
 // Device workspace of phg_zone_mean_noalloc, the entry point that allocates
 // nothing: the closed-zone partial sums (one per task) and then the means
 // (one per closed zone) of the call in progress.  Each device holds its own
@@ -1844,12 +1854,12 @@ int phg_advect_scalars(phg_ctx* ctx, int grid, float dt, const float* u, const f
     A.blocked = G->blocked;
     A.Pb = Pb;
     A.nu = nu;
-    // column tiles x k-chunks of at least 4 planes, about 1024 blocks
+    // column tiles x k-chunks of at least PHG_TR_KMIN planes, about PHG_TR_BLOCKS blocks
     A.tiles_x = (G->nx + TR_TX - 1) / TR_TX;
     A.tiles_y = (G->ny + TR_TY - 1) / TR_TY;
     const int ntile = A.tiles_x * A.tiles_y;
-    const int want = std::max(1, (1024 + ntile - 1) / ntile);
-    A.kchunk = std::max(4, (G->nz + want - 1) / want);
+    const int want = std::max(1, (PHG_TR_BLOCKS + ntile - 1) / ntile);
+    A.kchunk = std::max(PHG_TR_KMIN, (G->nz + want - 1) / want);
     const int nchunk = (G->nz + A.kchunk - 1) / A.kchunk;
     launch_nt(ctx, K_ADVECT, k_transport, ntile * nchunk, TR_THREADS, 0, A);
     HIP_OK(hipGetLastError());
@@ -1893,8 +1903,10 @@ int phg_turbulence(phg_ctx* ctx, int grid, const float* u, const float* v, const
     A.tiles_x = (G->nx + TU_TX - 1) / TU_TX;
     A.tiles_y = (G->ny + TU_TY - 1) / TU_TY;
     const int ntile = A.tiles_x * A.tiles_y;
-    const int want = std::max(1, (1024 + ntile - 1) / ntile);
-    A.kchunk = std::max(4, (G->nz + want - 1) / want);
+    // about eight resident blocks per CU, chunks down to one plane (a block's
+    // planes are a serial chain)
+    const int want = std::max(1, (2048 + ntile - 1) / ntile);
+    A.kchunk = std::max(1, (G->nz + want - 1) / want);
     const int nchunk = (G->nz + A.kchunk - 1) / A.kchunk;
     hipLaunchKernelGGL(k_fill_f32, dim3(1), dim3(64), 0, ctx->stream, G->vmax + 1, 1, 0.0f);
     launch_nt(ctx, K_TURB, k_turbulence, ntile * nchunk, TU_THREADS, 0, A);
