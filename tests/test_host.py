@@ -81,6 +81,35 @@ def test_grid_plan_builds_without_a_device():
     assert not h.value
 
 
+def pocket_grid(n):
+    """n^3 cells cut into sealed 2x2x2 pockets: open faces only inside a
+    pocket, so coarsening stalls at level 1 with one unknown per pocket."""
+    fx = np.zeros((n, n, n + 1), np.float32)
+    fx[:, :, 1:n:2] = 1.0
+    fy = np.zeros((n, n + 1, n), np.float32)
+    fy[:, 1:n:2, :] = 1.0
+    fz = np.zeros((n + 1, n, n), np.float32)
+    fz[1:n:2] = 1.0
+    return sc.Grid(n, n, n, 0.2, np.zeros(n ** 3, np.uint8), fx.ravel(), fy.ravel(), fz.ravel(), fx.ravel().copy(),
+                   fy.ravel().copy(), fz.ravel().copy())
+
+
+def test_stalled_coarsening_plans_promptly_or_refuses_cleanly():
+    """Thousands of sealed pockets stall the coarsening (C.n > 0.8 F.n,
+    pressure_mg.cpp:176): with 13,824 coarsest unknowns the plan takes the
+    reference's Gauss-Seidel sweeps (no dense inverse, built at once); with
+    32,768, beyond what the sweeps' level schedule keeps, the host set-up
+    refuses with PHG_EINVAL, promptly, instead of an O(n^3) inverse or an
+    exception through the C ABI."""
+    import time
+    t = time.perf_counter()
+    p = gas.GridPlan(pocket_grid(48))
+    p.free()
+    with pytest.raises(gas.PhgError, match="-1"):
+        gas.GridPlan(pocket_grid(64))
+    assert time.perf_counter() - t < 10.0
+
+
 def test_abi_exports_every_declared_symbol():
     header = open(os.path.join(ROOT, "include", "photon_gas.h")).read()
     declared = set(re.findall(r"\b(phg_\w+)\s*\(", header))
