@@ -2298,7 +2298,7 @@ int phg_zone_leakage(phg_ctx* ctx, int grid, float dt, const float* zoneP, const
     const CrackDev& cd = G->crack;
     if (leak_faces) *leak_faces = cd.ncr;
     if (zoneDP) std::fill(zoneDP, zoneDP + nzn, 0.0f);
-    if (nzn == 0) return PHG_OK;
+    if (nzn == 0 || (G->nclosed == 0 && cd.ncr == 0)) return PHG_OK;   // nothing to shift, no crack flow
     HIP_OK(hipMemcpyAsync(G->zp, zoneP, nzn * sizeof(float), hipMemcpyHostToDevice, ctx->stream));
     const int nc = G->nclosed;
     double* dint = G->zsum + G->nztask;

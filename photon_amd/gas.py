@@ -848,12 +848,14 @@ class GasSolver:
         self.step_clear()
         self.inject_burners(dt)
         self.combustion(dt)
-        self.turbulence()
+        # maxNu and maxDiv stay on the device until the solve has waited
+        self.turbulence(report=False)
         self.heat_diffusion(dt)
-        self.energy_and_expansion(dt)
+        self.energy_and_expansion(dt, report=False)
         self.momentum(dt)
         self.leakage_zones(dt)
         self.project(dt)
+        self.read_stats()
         self.advect_scalars(dt)
         self.zone_pressure(dt)
         self.time = float(np.float32(self.time) + np.float32(dt))

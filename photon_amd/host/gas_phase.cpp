@@ -380,14 +380,17 @@ void GasPhase::combustion(float dt) {
     check(phg_combustion(ctx_, grid_, dt, radiationOn ? 1 : 0, &g, &hrrTotal), "phg_combustion");
 }
 
-void GasPhase::turbulence() { check(phg_turbulence(ctx_, grid_, dU_, dV_, dW_, dF_[0], dNu_, &maxNu), "phg_turbulence"); }
+void GasPhase::turbulence() {
+    check(phg_turbulence(ctx_, grid_, dU_, dV_, dW_, dF_[0], dNu_, deferStats_ ? nullptr : &maxNu), "phg_turbulence");
+}
 
 void GasPhase::heatDiffusion(float dt) {
     check(phg_heat_diffusion(ctx_, grid_, dt, dF_[0], dNu_, dPb_, dQ_), "phg_heat_diffusion");
 }
 
 void GasPhase::energyAndExpansion(float dt) {
-    check(phg_energy_expansion(ctx_, grid_, dt, dF_[0], dS_, dPb_, dQ_, dMs_, &maxDiv), "phg_energy_expansion");
+    check(phg_energy_expansion(ctx_, grid_, dt, dF_[0], dS_, dPb_, dQ_, dMs_, deferStats_ ? nullptr : &maxDiv),
+          "phg_energy_expansion");
 }
 
 void GasPhase::momentum(float dt) {
@@ -507,12 +510,16 @@ float GasPhase::step(float dtMax) {   // solver.cpp:226-269, gas-only
     stepClear();
     injectBurners(dt);
     combustion(dt);
+    // maxNu and maxDiv stay on the device until the solve has waited
+    deferStats_ = true;
     turbulence();
     heatDiffusion(dt);
     energyAndExpansion(dt);
+    deferStats_ = false;
     momentum(dt);
     leakage(dt);
     project(dt);
+    check(phg_grid_stats(ctx_, grid_, nullptr, &maxNu, &maxDiv), "phg_grid_stats");
     advectScalars(dt);
     zonePressure(dt);
     time += dt;

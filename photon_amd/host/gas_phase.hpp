@@ -129,6 +129,7 @@ private:
     phg_ctx* ctx_ = nullptr;
     int grid_ = -1;
     std::string err_;
+    bool deferStats_ = false;   // step(): maxNu and maxDiv read after the solve, in one wait
     std::vector<void*> dmem_;
     float *dU_ = nullptr, *dV_ = nullptr, *dW_ = nullptr, *dU2_ = nullptr, *dV2_ = nullptr, *dW2_ = nullptr;
     float *dP_ = nullptr, *dS_ = nullptr, *dNu_ = nullptr;
