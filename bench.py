@@ -65,11 +65,12 @@ def measured_traffic(kernel):
     return None
 
 
-def algorithmic_bytes(name, iters, n0, n1, vt, flex):
+def algorithmic_bytes(name, iters, n0, n1, vt, flex, zt=4):
     """Algorithmic HBM bytes of all launches of a level-0 kernel class for a
     batch of systems of one grid that stopped after `iters` iterations each:
     one pass over every array the kernel touches.  Outer CG vectors
-    (x r q d) and the smoother iterate z are FP32; the V-cycle residual
+    (x r q d) are FP32; the smoother iterate z (and the fused passes' z2) is
+    `zt` bytes (FP32, or FP16 in the mixed precisions); the V-cycle residual
     scratch and level-1 correction are `vt` bytes; a half sweep writes half
     of z.  Per launch, the grid's 0/1 face mask (1 B/cell) and, for the
     prolongation, its level-0 aggregate map (4 B/cell) are read once, being
@@ -80,26 +81,27 @@ def algorithmic_bytes(name, iters, n0, n1, vt, flex):
     nvc = int((it + 1).sum())                        # system-launches of a V-cycle kernel
     nd = int(np.maximum(it, 0).sum())                # system-launches of dir_apply
     nfirst = int((it > 0).sum())
+    half = zt / 2
     if name == "dir_apply":                          # read z d_old x, write x d_new q; first: read z, write d q
-        per_sys = nfirst * 12 * n0 + (nd - nfirst) * 24 * n0
-    elif name == "pre_smooth":                       # read r q, write r z r0; first: read r, write z r0
-        per_sys = len(it) * (8 + vt) * n0 + (nvc - len(it)) * (16 + vt) * n0
+        per_sys = nfirst * (8 + zt) * n0 + (nd - nfirst) * (20 + zt) * n0
+    elif name == "pre_smooth":                       # read r q, write r z2 r0; first: read r, write z2 r0
+        per_sys = len(it) * (4 + zt + vt) * n0 + (nvc - len(it)) * (12 + zt + vt) * n0
     elif name == "pre_red":                          # read r q, write r z; first: read r, write z
-        per_sys = len(it) * 8 * n0 + (nvc - len(it)) * 16 * n0
+        per_sys = len(it) * (4 + zt) * n0 + (nvc - len(it)) * (12 + zt) * n0
     elif name == "rb_black":                         # read z r, write half of z
-        per_sys = nvc * 10 * n0
+        per_sys = nvc * (zt + 4 + half) * n0
     elif name == "resid0":                           # read z r, write r0
-        per_sys = nvc * (8 + vt) * n0
+        per_sys = nvc * (zt + 4 + vt) * n0
     elif name == "post_black":                       # read z r and t1 (level 1), write half of z
-        per_sys = nvc * (10 * n0 + vt * n1)
-    elif name == "post_smooth":                      # read z r t1 (and q when flexible), write z
-        per_sys = nvc * (12 * n0 + vt * n1) + (nvc - len(it) if flex else 0) * 4 * n0
+        per_sys = nvc * ((zt + 4 + half) * n0 + vt * n1)
+    elif name == "post_smooth":                      # read z2 r t1 (and q when flexible), write z
+        per_sys = nvc * ((4 + 2 * zt) * n0 + vt * n1) + (nvc - len(it) if flex else 0) * 4 * n0
     elif name == "post_red":                         # read z r (and q when flexible), write half of z
-        per_sys = nvc * 10 * n0 + (nvc - len(it) if flex else 0) * 4 * n0
+        per_sys = nvc * (zt + 4 + half) * n0 + (nvc - len(it) if flex else 0) * 4 * n0
     else:
         raise KeyError(name)
     shared = {"post_black": 5, "post_smooth": 5}.get(name, 1) * n0   # per launch
-    return per_sys, shared
+    return int(per_sys), shared
 
 
 def whole_solve_traffic(ms_per_step):
@@ -596,10 +598,10 @@ def main():
     for k in LEVEL0:
         if prof.get(k, (0, 0))[1] == 0:
             continue
-        per_sys, shared = algorithmic_bytes(k, mixed["iters"], n, n1, 2, flex=True)
+        per_sys, shared = algorithmic_bytes(k, mixed["iters"], n, n1, 2, flex=True, zt=2)
         level0_gbs[k] = (per_sys + prof[k][1] * shared) / (prof[k][0] * 1e-3) / 1e9
     dom0 = max(LEVEL0, key=lambda k: prof.get(k, (0, 0))[0])
-    per_sys, shared = algorithmic_bytes(dom0, mixed["iters"], n, n1, 2, flex=True)
+    per_sys, shared = algorithmic_bytes(dom0, mixed["iters"], n, n1, 2, flex=True, zt=2)
     nlaunch = prof[dom0][1]
     total_bytes = per_sys + nlaunch * shared
     ms = prof[dom0][0]

@@ -32,6 +32,9 @@ template <> __device__ __forceinline__ bf16_t from_f<bf16_t>(float v) {
     return r;
 }
 
+// a value as it reads back after a store in T (FP32: itself)
+template <class T> __device__ __forceinline__ float zq(float v) { return to_f(from_f<T>(v)); }
+
 // Global-memory access.  Pointers read from a descriptor are generic, and a
 // generic access compiles to a flat instruction, which counts against both the
 // vector-memory and the LDS wait counters: every LDS wait would then also wait
@@ -52,8 +55,12 @@ template <class T> __device__ __forceinline__ PHG_GLOBAL T* gptr(T* p, unsigned 
 }
 template <class T> __device__ __forceinline__ T gld(const T* p, unsigned i) { return *gptr(p, i); }
 template <class T> __device__ __forceinline__ void gst(T* p, unsigned i, T v) { *gptr(p, i) = v; }
-// two adjacent floats in one 8-byte access
+// two adjacent floats in one 8-byte access; two halves in one 4-byte access
 typedef float f32x2 __attribute__((ext_vector_type(2)));
+typedef _Float16 f16x2 __attribute__((ext_vector_type(2)));
+template <class T> struct pair_of;
+template <> struct pair_of<float> { using type = f32x2; };
+template <> struct pair_of<_Float16> { using type = f16x2; };
 // BF16 is a struct of raw bits: access the bits
 template <> __device__ __forceinline__ bf16_t gld(const bf16_t* p, unsigned i) {
     bf16_t r;
@@ -147,7 +154,7 @@ struct DSys {
     float* d0;                    // direction, double-buffered: d_k lives in d[k & 1]
     float* d1;
     void* z;                      // preconditioned residual, V-cycle vector type
-    float* z2;                    // the pre-smoothed iterate of the fused passes: the fused
+    void* z2;                     // the pre-smoothed iterate of the fused passes (z's type): the fused
                                   // post-smoothing reads it in its halo rows while it writes z
     const float* invd;            // 1 / diag of level 0 (0 where diag <= 0), FP32
     double* scal;

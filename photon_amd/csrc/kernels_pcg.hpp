@@ -888,7 +888,7 @@ __global__ void __launch_bounds__(TB) k_post_black(const DSys* __restrict__ sys,
 
 // red post-smoothing half sweep (gsColor(0)) fused with the partial r.z
 // (slot 3) and, for flexible CG, q.z (slot 0)
-template <class CT, class VT>
+template <class CT, class ZT>
 __global__ void __launch_bounds__(TB) k_post_red(const DSys* __restrict__ sys, const int2* __restrict__ tab, int sweep) {
     PHG_SYS_PROLOGUE
     if (sweep >= s.nu0) return;   // sweep of a system that smooths fewer times
@@ -912,7 +912,7 @@ __global__ void __launch_bounds__(TB) k_post_red(const DSys* __restrict__ sys, c
         o.q = with_q ? gld(s.q, q.c) : 0.0f;
         return o;
     };
-    auto raw = [&](unsigned n) { return ld<VT>(z, n); };
+    auto raw = [&](unsigned n) { return ld<ZT>(z, n); };
     march(s, t, lds, raw, [](auto&) {}, [](float v) { return v; }, pre, [&](int k, float vp, float vc, float vn, const float* p, const Op& o) {
         const Cell q{t.c2 + sxy * k, t.i, t.j, k};
         const float r = o.r;
@@ -920,8 +920,8 @@ __global__ void __launch_bounds__(TB) k_post_red(const DSys* __restrict__ sys, c
         if (((t.i + t.j + k) & 1) == 0) {
             const float dg = cell_diag(o.cf);
             zc = 0.0f;
-            if (dg > 0) zc = cell_div<CT>(r * zs + cell_lat(o.cf, q, L.nx, L.ny, L.nz, p, vp, vn), dg);
-            st_nt<VT>(s.z, q.c, zc);
+            if (dg > 0) zc = zq<ZT>(cell_div<CT>(r * zs + cell_lat(o.cf, q, L.nx, L.ny, L.nz, p, vp, vn), dg));
+            st_nt<ZT>(s.z, q.c, zc);   // the dots below use the value as stored
         }
         v[0] += (double)r * zc;
         if (with_q) vq += (double)o.q * zc;
@@ -1008,7 +1008,7 @@ __host__ __device__ constexpr size_t fuse_lds(int nt, int p2, int p1, bool post)
 
 // P2, P1: staging and smoothing pairs per thread, for NT/2 + 2 nx and at
 // most NT/2 + nx + 1 pairs
-template <class VT, bool FIRST, int NT, int P2, int P1>
+template <class VT, class ZT, bool FIRST, int NT, int P2, int P1>
 __global__ void __launch_bounds__(NT) k_pre_fused(const DSys* __restrict__ sys, const int2* __restrict__ tab) {
     PHG_SYS_PROLOGUE
     const DLevel& L = s.L[0];
@@ -1111,7 +1111,8 @@ __global__ void __launch_bounds__(NT) k_pre_fused(const DSys* __restrict__ sys, 
             const bool ared = ((par2[m] + p) & 1) == 0;   // the first cell is red
             const float rsa = rna * zs, rsb = rnb * zs;
             const int dg = __popc(ared ? ma : mb);
-            const float vr = dg > 0 ? div_count(ared ? rsa : rsb, (float)dg) : 0.0f;
+            // the red value from zero, as the separate red pass stores it in z
+            const float vr = dg > 0 ? zq<ZT>(div_count(ared ? rsa : rsb, (float)dg)) : 0.0f;
             *reinterpret_cast<float2*>(R + 2 * u) =
                 float2{oka ? (ared ? vr : rsa) : 0.0f, okb ? (ared ? rsb : vr) : 0.0f};
             *reinterpret_cast<uint16_t*>(M + 2 * u) = (uint16_t)((oka ? ma : 0u) | ((okb ? mb : 0u) << 8));
@@ -1140,8 +1141,9 @@ __global__ void __launch_bounds__(NT) k_pre_fused(const DSys* __restrict__ sys, 
             const int dg = __popc(mk);
             float zb = 0.0f;
             if (pin && (ared ? inb1[m] : ina1[m]) && dg > 0)
-                zb = div_count(R[hb] + lat_bits(mk & ((ared ? ijb[m] : ija[m]) | kb), R + hb, nx, Rm[hb], Rp[hb]),
-                               (float)dg);
+                zb = zq<ZT>(div_count(R[hb] + lat_bits(mk & ((ared ? ijb[m] : ija[m]) | kb), R + hb, nx, Rm[hb],
+                                                       Rp[hb]),
+                                      (float)dg));
             const float zr = pin && (ared ? ina1[m] : inb1[m]) ? (ared ? xm : xp) : 0.0f;
             *reinterpret_cast<float2*>(Z + ha1[m]) = ared ? float2{zr, zb} : float2{zb, zr};
         }
@@ -1173,7 +1175,7 @@ __global__ void __launch_bounds__(NT) k_pre_fused(const DSys* __restrict__ sys, 
             r0 = rn * zs - dg * zc +
                  cell_lat(cf, Cell{c, t.i, t.j, k}, nx, ny, nz, Z, Zb[((sp + 2) % 3) * SZ + hz],
                           Zb[((sp + 1) % 3) * SZ + hz]);
-        gst_nt(s.z2, c, zc);   // read by the fused post-smoothing
+        st_nt<ZT>(s.z2, c, zc);   // read by the fused post-smoothing (exact: zc is a ZT value)
         st_nt<VT>(L.r, c, r0);
         if (!FIRST) {
             gst_nt(rout, c, rn);
@@ -1261,8 +1263,9 @@ __global__ void __launch_bounds__(NT) k_pre_fused(const DSys* __restrict__ sys, 
 // results are bitwise the same; the pass reads z and r once and writes z once
 // (12 B/cell, 16 with q, in place of about 27).
 
-template <class VT, int NT, int P2, int P1>
+template <class VT, class ZT, int NT, int P2, int P1>
 __global__ void __launch_bounds__(NT) k_post_fused(const DSys* __restrict__ sys, const int2* __restrict__ tab) {
+    using ZP = typename pair_of<ZT>::type;
     PHG_SYS_PROLOGUE
     const DLevel& L = s.L[0];
     const ColTile t = col_tile(s, e.y);
@@ -1274,7 +1277,7 @@ __global__ void __launch_bounds__(NT) k_post_fused(const DSys* __restrict__ sys,
     const float zs = (float)s.scal[S_ZS];
     const bool with_q = st_uni(s, ST_FLEX) != 0 && st_uni(s, ST_VC) != 0;   // q exists once an iteration has run
     const float* __restrict__ rv = r_vc(s);
-    const float* __restrict__ z = s.z2;   // the pre-smoothed iterate, never written here
+    const ZT* __restrict__ z = static_cast<const ZT*>(s.z2);   // the pre-smoothed iterate, never written here
     const VT* __restrict__ t1 = reinterpret_cast<const VT*>(s.L[1].t);
     constexpr int NXM = fuse_nxmax(NT, P2, P1), SR = fuse_sr(NT, NXM), SZ = fuse_sz(NT, NXM);
     extern __shared__ float lds[];
@@ -1324,7 +1327,8 @@ __global__ void __launch_bounds__(NT) k_post_fused(const DSys* __restrict__ sys,
     // flight (pairs outside the grid load a clamped pair that is not used).
     typedef int i32x2 __attribute__((ext_vector_type(2)));
     struct Raw {
-        f32x2 z[P2], r[P2];
+        ZP z[P2];
+        f32x2 r[P2];
         i32x2 a[P2];      // level-1 aggregates of the pair; a.x becomes the red cell's once gathered
         uint16_t m[P2];   // mask bytes of the pair: first cell in bits 0-7
         VT c[P2];         // level-1 value of the red cell's aggregate
@@ -1335,7 +1339,7 @@ __global__ void __launch_bounds__(NT) k_post_fused(const DSys* __restrict__ sys,
 #pragma unroll
         for (int m = 0; m < P2; ++m) {
             const unsigned i2 = (ca[m] + base) >> 1;
-            X.z[m] = gld(reinterpret_cast<const f32x2*>(z), i2);
+            X.z[m] = gld(reinterpret_cast<const ZP*>(z), i2);
             X.r[m] = gld(reinterpret_cast<const f32x2*>(rv), i2);
             X.a[m] = gld(reinterpret_cast<const i32x2*>(L.agg), i2);
             X.m[m] = gld(reinterpret_cast<const uint16_t*>(L.mask), i2);
@@ -1364,7 +1368,8 @@ __global__ void __launch_bounds__(NT) k_post_fused(const DSys* __restrict__ sys,
             if (u >= NP2) break;
             const bool oka = pin && ina[m], okb = pin && inb[m];
             const bool ared = ((par2[m] + p) & 1) == 0;   // the first cell is red
-            const float zr = (ared ? X.z[m].x : X.z[m].y) + (X.a[m].x >= 0 ? VC_LEVEL_SCALE * to_f(X.c[m]) : 0.0f);
+            const float zr = (float)(ared ? X.z[m].x : X.z[m].y) +
+                             (X.a[m].x >= 0 ? VC_LEVEL_SCALE * to_f(X.c[m]) : 0.0f);
             const float va = ared ? zr : X.r[m].x * zs, vb = ared ? X.r[m].y * zs : zr;
             *reinterpret_cast<float2*>(R + 2 * u) = float2{oka ? va : 0.0f, okb ? vb : 0.0f};
             *reinterpret_cast<uint16_t*>(M + 2 * u) =
@@ -1397,9 +1402,10 @@ __global__ void __launch_bounds__(NT) k_post_fused(const DSys* __restrict__ sys,
             const unsigned mk = M[hb];
             const int dg = __popc(mk);
             float zb = 0.0f;
-            if (pin && (ared ? inb1[m] : ina1[m]) && dg > 0)
-                zb = div_count(R[hb] + lat_bits(mk & ((ared ? ijb[m] : ija[m]) | kb), R + hb, nx, Rm[hb], Rp[hb]),
-                               (float)dg);
+            if (pin && (ared ? inb1[m] : ina1[m]) && dg > 0)   // as the separate black pass stores it in z
+                zb = zq<ZT>(div_count(R[hb] + lat_bits(mk & ((ared ? ijb[m] : ija[m]) | kb), R + hb, nx, Rm[hb],
+                                                       Rp[hb]),
+                                      (float)dg));
             *reinterpret_cast<float2*>(Z + ha1[m]) = ared ? float2{0.0f, zb} : float2{zb, 0.0f};
         }
     };
@@ -1427,11 +1433,12 @@ __global__ void __launch_bounds__(NT) k_post_fused(const DSys* __restrict__ sys,
             const float dg = cell_diag(cf);
             zc = 0.0f;
             if (dg > 0)
-                zc = cell_div<Mask01>(r * zs + cell_lat(cf, Cell{c, t.i, t.j, k}, nx, ny, nz, Z,
-                                                        Zb[((sp + 2) % 3) * SZ + hz], Zb[((sp + 1) % 3) * SZ + hz]),
-                                      dg);
+                zc = zq<ZT>(cell_div<Mask01>(r * zs + cell_lat(cf, Cell{c, t.i, t.j, k}, nx, ny, nz, Z,
+                                                               Zb[((sp + 2) % 3) * SZ + hz],
+                                                               Zb[((sp + 1) % 3) * SZ + hz]),
+                                             dg));
         }
-        gst_nt(reinterpret_cast<float*>(s.z), c, zc);
+        st_nt<ZT>(s.z, c, zc);   // exact; the dots use the value as stored
         acc += (double)r * zc;
         if (with_q) accq += (double)o.q * zc;
     };

@@ -390,7 +390,7 @@ constexpr int kMaxSweeps0 = 4;
 // ---------------------------------------------------------------------------
 // C0: level-0 coefficient type of the V-cycle (FP32/FP16 faces or the 0/1
 // mask), CC: coarse operator storage, VT: V-cycle vector storage
-template <class C0, class CC, class VT>
+template <class C0, class CC, class VT, class ZT>
 void vcycle_t(phg_ctx* ctx, BatchDev* B, int first) {
     const DSys* S = B->d_sys;
     // 0/1 grids of moderate width and even plane size smoothed once take the
@@ -415,33 +415,33 @@ void vcycle_t(phg_ctx* ctx, BatchDev* B, int first) {
         if (B->fuse_nt != TB) {
             if (B->tabwf_len > 0) {
                 if (mx <= 128)
-                    go(k_pre_fused<VT, true, 2 * TB, 1, 1>, k_pre_fused<VT, false, 2 * TB, 1, 1>, 2 * TB, 1, 1);
+                    go(k_pre_fused<VT, ZT, true, 2 * TB, 1, 1>, k_pre_fused<VT, ZT, false, 2 * TB, 1, 1>, 2 * TB, 1, 1);
                 else
-                    go(k_pre_fused<VT, true, 2 * TB, 2, 1>, k_pre_fused<VT, false, 2 * TB, 2, 1>, 2 * TB, 2, 1);
+                    go(k_pre_fused<VT, ZT, true, 2 * TB, 2, 1>, k_pre_fused<VT, ZT, false, 2 * TB, 2, 1>, 2 * TB, 2, 1);
             }
             if (B->tabqf_len)
                 launch_nt(ctx, K_PRE_SMOOTH,
-                          first ? k_pre_fused<VT, true, 4 * TB, 1, 1> : k_pre_fused<VT, false, 4 * TB, 1, 1>,
+                          first ? k_pre_fused<VT, ZT, true, 4 * TB, 1, 1> : k_pre_fused<VT, ZT, false, 4 * TB, 1, 1>,
                           B->tabqf_len, 4 * TB, fuse_lds(4 * TB, 1, 1, false), S, (const int2*)B->tabqf);
         } else {
             if (mx <= 64)
-                go(k_pre_fused<VT, true, TB, 1, 1>, k_pre_fused<VT, false, TB, 1, 1>, TB, 1, 1);
+                go(k_pre_fused<VT, ZT, true, TB, 1, 1>, k_pre_fused<VT, ZT, false, TB, 1, 1>, TB, 1, 1);
             else if (mx <= 127)
-                go(k_pre_fused<VT, true, TB, 2, 1>, k_pre_fused<VT, false, TB, 2, 1>, TB, 2, 1);
+                go(k_pre_fused<VT, ZT, true, TB, 2, 1>, k_pre_fused<VT, ZT, false, TB, 2, 1>, TB, 2, 1);
             else
-                go(k_pre_fused<VT, true, TB, 2, 2>, k_pre_fused<VT, false, TB, 2, 2>, TB, 2, 2);
+                go(k_pre_fused<VT, ZT, true, TB, 2, 2>, k_pre_fused<VT, ZT, false, TB, 2, 2>, TB, 2, 2);
         }
     }
     if (nu > 0) {
         // red from zero, black, then (red, black) for each further sweep of
         // the systems that make it, then the residual
-        launch(ctx, K_PRE_RED, k_pre_red<C0, float>, nu, 0, S, TU, first);
-        launch(ctx, K_RB_BLACK, k_rb_black<C0, float>, nu, B->lds0(), S, TU, 0);
+        launch(ctx, K_PRE_RED, k_pre_red<C0, ZT>, nu, 0, S, TU, first);
+        launch(ctx, K_RB_BLACK, k_rb_black<C0, ZT>, nu, B->lds0(), S, TU, 0);
         for (int v = 1; v < B->maxnu0_u; ++v) {
-            launch(ctx, K_POST_RED, k_post_red<C0, float>, nu, B->lds0(), S, TU, v);
-            launch(ctx, K_RB_BLACK, k_rb_black<C0, float>, nu, B->lds0(), S, TU, v);
+            launch(ctx, K_POST_RED, k_post_red<C0, ZT>, nu, B->lds0(), S, TU, v);
+            launch(ctx, K_RB_BLACK, k_rb_black<C0, ZT>, nu, B->lds0(), S, TU, v);
         }
-        launch(ctx, K_RESID0, k_resid0<C0, float, VT>, nu, B->lds0(), S, TU);
+        launch(ctx, K_RESID0, k_resid0<C0, ZT, VT>, nu, B->lds0(), S, TU);
     }
     const std::vector<int4*>& TG = B->tabg;
     const std::vector<int>& NG = B->tabg_len;
@@ -495,62 +495,70 @@ void vcycle_t(phg_ctx* ctx, BatchDev* B, int first) {
         if (B->fuse_nt != TB) {
             if (B->tabwf_len > 0) {
                 if (mx <= 128)
-                    go(k_post_fused<VT, 2 * TB, 1, 1>, 2 * TB, 1, 1);
+                    go(k_post_fused<VT, ZT, 2 * TB, 1, 1>, 2 * TB, 1, 1);
                 else
-                    go(k_post_fused<VT, 2 * TB, 2, 1>, 2 * TB, 2, 1);
+                    go(k_post_fused<VT, ZT, 2 * TB, 2, 1>, 2 * TB, 2, 1);
             }
             if (B->tabqf_len)
-                launch_nt(ctx, K_POST_SMOOTH, k_post_fused<VT, 4 * TB, 1, 1>, B->tabqf_len, 4 * TB,
+                launch_nt(ctx, K_POST_SMOOTH, k_post_fused<VT, ZT, 4 * TB, 1, 1>, B->tabqf_len, 4 * TB,
                           fuse_lds(4 * TB, 1, 1, true), S, (const int2*)B->tabqf);
         } else {
             if (mx <= 64)
-                go(k_post_fused<VT, TB, 1, 1>, TB, 1, 1);
+                go(k_post_fused<VT, ZT, TB, 1, 1>, TB, 1, 1);
             else if (mx <= 127)
-                go(k_post_fused<VT, TB, 2, 1>, TB, 2, 1);
+                go(k_post_fused<VT, ZT, TB, 2, 1>, TB, 2, 1);
             else
-                go(k_post_fused<VT, TB, 2, 2>, TB, 2, 2);
+                go(k_post_fused<VT, ZT, TB, 2, 2>, TB, 2, 2);
         }
     }
     if (nu > 0) {
-        launch(ctx, K_POST_BLACK, k_post_black<C0, float, VT>, nu, B->lds0(), S, TU);
-        launch(ctx, K_POST_RED, k_post_red<C0, float>, nu, B->lds0(), S, TU, 0);
+        launch(ctx, K_POST_BLACK, k_post_black<C0, ZT, VT>, nu, B->lds0(), S, TU);
+        launch(ctx, K_POST_RED, k_post_red<C0, ZT>, nu, B->lds0(), S, TU, 0);
         for (int v = 1; v < B->maxnu0_u; ++v) {
-            launch(ctx, K_RB_BLACK, k_rb_black<C0, float>, nu, B->lds0(), S, TU, v);
-            launch(ctx, K_POST_RED, k_post_red<C0, float>, nu, B->lds0(), S, TU, v);
+            launch(ctx, K_RB_BLACK, k_rb_black<C0, ZT>, nu, B->lds0(), S, TU, v);
+            launch(ctx, K_POST_RED, k_post_red<C0, ZT>, nu, B->lds0(), S, TU, v);
         }
     }
 }
 
 void vcycle(phg_ctx* ctx, BatchDev* B, int first) {
+    // the mixed precisions keep z in FP16 (scaled to an RMS near 1, so the
+    // range of FP16 is ample and its 11 bits beat BF16's 8)
     if (B->prec == PHG_PREC_FP32) {
-        if (B->binary) vcycle_t<Mask01, float, float>(ctx, B, first);
-        else vcycle_t<float, float, float>(ctx, B, first);
+        if (B->binary) vcycle_t<Mask01, float, float, float>(ctx, B, first);
+        else vcycle_t<float, float, float, float>(ctx, B, first);
     } else if (B->prec == PHG_PREC_MIXED_F16) {
-        if (B->binary) vcycle_t<Mask01, _Float16, _Float16>(ctx, B, first);
-        else vcycle_t<_Float16, _Float16, _Float16>(ctx, B, first);
+        if (B->binary) vcycle_t<Mask01, _Float16, _Float16, _Float16>(ctx, B, first);
+        else vcycle_t<_Float16, _Float16, _Float16, _Float16>(ctx, B, first);
     } else {
-        if (B->binary) vcycle_t<Mask01, _Float16, bf16_t>(ctx, B, first);
-        else vcycle_t<_Float16, _Float16, bf16_t>(ctx, B, first);
+        if (B->binary) vcycle_t<Mask01, _Float16, bf16_t, _Float16>(ctx, B, first);
+        else vcycle_t<_Float16, _Float16, bf16_t, _Float16>(ctx, B, first);
     }
 }
 
 // d = z + beta d_old, q = A d: on 2 TB-column blocks (the even column tiles)
 // unless the batch asks for TB; one halo slot per thread when the rows allow
-template <class OT>
-void dir_apply(phg_ctx* ctx, BatchDev* B) {
+template <class OT, class ZT>
+void dir_apply_t(phg_ctx* ctx, BatchDev* B) {
     const DSys* S = B->d_sys;
     if (B->fuse_nt != TB) {
         const size_t lds = 2 * (size_t)(2 * TB + 2 * B->maxnx) * sizeof(float);
         const int2* TW = B->tabw;
         if (2 * B->maxnx <= 2 * TB)
-            launch_nt(ctx, K_DIR_APPLY, k_dir_apply<OT, float, 2 * TB, 1>, B->tabw_len, 2 * TB, lds, S, TW);
+            launch_nt(ctx, K_DIR_APPLY, k_dir_apply<OT, ZT, 2 * TB, 1>, B->tabw_len, 2 * TB, lds, S, TW);
         else
-            launch_nt(ctx, K_DIR_APPLY, k_dir_apply<OT, float, 2 * TB, MARCH_HALO>, B->tabw_len, 2 * TB, lds, S, TW);
+            launch_nt(ctx, K_DIR_APPLY, k_dir_apply<OT, ZT, 2 * TB, MARCH_HALO>, B->tabw_len, 2 * TB, lds, S, TW);
     } else if (2 * B->maxnx <= TB) {
-        launch(ctx, K_DIR_APPLY, k_dir_apply<OT, float, TB, 1>, B->tab_len[0], B->lds0(), S, B->tab[0]);
+        launch(ctx, K_DIR_APPLY, k_dir_apply<OT, ZT, TB, 1>, B->tab_len[0], B->lds0(), S, B->tab[0]);
     } else {
-        launch(ctx, K_DIR_APPLY, k_dir_apply<OT, float>, B->tab_len[0], B->lds0(), S, B->tab[0]);
+        launch(ctx, K_DIR_APPLY, k_dir_apply<OT, ZT>, B->tab_len[0], B->lds0(), S, B->tab[0]);
     }
+}
+
+template <class OT>
+void dir_apply(phg_ctx* ctx, BatchDev* B) {
+    if (B->prec == PHG_PREC_FP32) dir_apply_t<OT, float>(ctx, B);
+    else dir_apply_t<OT, _Float16>(ctx, B);
 }
 
 void cg_iteration(phg_ctx* ctx, BatchDev* B) {
@@ -1233,7 +1241,7 @@ int phg_batch_create_ex(phg_ctx* ctx, int nsys, const int* grids, const phg_batc
             S.q = v4 + n0;
             S.d0 = v4 + 2 * n0;
             S.d1 = v4 + 3 * n0;
-            S.z = v4 + 4 * n0;   // the level-0 iterate stays FP32 in every precision: it feeds d directly
+            S.z = v4 + 4 * n0;   // the level-0 iterate: FP32, or FP16 in the mixed precisions
             S.r1 = v4 + 5 * n0;
             S.z2 = v4 + 6 * n0;
             rc = dalloc(ctx, B->allocs, &S.part, (size_t)S.tiles0 * NPART);
@@ -1695,7 +1703,8 @@ int phg_precondition(phg_ctx* ctx, int grid, const float* r, float* z) {
     if (rc) return rc;
     vcycle(ctx, B, 1);
     const DSys* S = B->d_sys;
-    launch(ctx, K_POST_RED, k_z_out<float>, B->tab_len[0], 0, S, B->tab[0], z);
+    if (B->prec == PHG_PREC_FP32) launch(ctx, K_POST_RED, k_z_out<float>, B->tab_len[0], 0, S, B->tab[0], z);
+    else launch(ctx, K_POST_RED, k_z_out<_Float16>, B->tab_len[0], 0, S, B->tab[0], z);
     HIP_OK(hipStreamSynchronize(ctx->stream));
     HIP_OK(hipGetLastError());
     if (ctx->prof) prof_drain(ctx);
@@ -2235,13 +2244,17 @@ int phg_grid_update_faces(phg_ctx* ctx, int grid, int n, const int* axis, const 
             if (f[q] == 0.0f) closing.push_back(make_int2(d, face[q]));
         }
         if (!closing.empty() && u && v && w) {
+            std::vector<void*> tmp;   // freed on every path out of this block
             int2* dl = nullptr;
-            HIP_OK(hipMalloc(&dl, closing.size() * sizeof(int2)));
-            HIP_OK(hipMemcpy(dl, closing.data(), closing.size() * sizeof(int2), hipMemcpyHostToDevice));
-            hipLaunchKernelGGL(k_face_zero, dim3(((int)closing.size() + TB - 1) / TB), dim3(TB), 0, ctx->stream,
-                               (const int2*)dl, (int)closing.size(), u, v, w);
-            HIP_OK(hipStreamSynchronize(ctx->stream));
-            HIP_OK(hipFree(dl));
+            int rz = upload(ctx, tmp, &dl, closing.data(), closing.size());
+            if (!rz) {
+                hipLaunchKernelGGL(k_face_zero, dim3(((int)closing.size() + TB - 1) / TB), dim3(TB), 0, ctx->stream,
+                                   (const int2*)dl, (int)closing.size(), u, v, w);
+                const hipError_t e = hipStreamSynchronize(ctx->stream);
+                if (e != hipSuccess) rz = fail(ctx, PHG_EHIP, "zeroing closed faces: %s", hipGetErrorString(e));
+            }
+            free_all(tmp);
+            if (rz) return rz;
         }
         bool any_crack = false;
         for (int d = 0; d < 3; ++d)
