@@ -340,8 +340,13 @@ int phg_grid_set_cracks(phg_ctx *ctx, int grid, const uint8_t *lx, const uint8_t
  * buildPressureLevels on the reference's zonesDirty_ / mgDirty_,
  * gasphase.cpp:111-158, pressure_mg.cpp:28-191); batches made earlier over
  * this grid are stale then and their solves return PHG_ESTATE.  Otherwise
- * only the open fractions and the crack faces change.  rebuilt (may be
- * NULL): 1 when the set-up was redone. */
+ * only the open fractions and the crack faces change.  A rebuild forms
+ * levels 0 and 1 of the hierarchy (the work that grows with the cells) on the
+ * device, bit for bit as the host set-up does, and the coarser levels, the
+ * coarsest solve and the zones on the host.  rebuilt (may be NULL): 0 when
+ * the set-up stood, 2 when it was redone with the device-formed levels, 1
+ * when it was redone wholly on the host (a grid beyond the device build's
+ * limits). */
 int phg_grid_update_faces(phg_ctx *ctx, int grid, int n, const int *axis, const int *face, const float *f,
                           const float *a, const uint8_t *crack, float *u, float *v, float *w, int *rebuilt);
 /* Solver::leakage (gasphase.cpp:402-498): each closed zone's end-of-step

@@ -11,7 +11,7 @@ import sys
 HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
 OUT = os.path.join(HERE, "libphoton_gas.so")
-SOURCES = ["phg.hip", "hierarchy.cpp"]
+SOURCES = ["phg.hip", "hierarchy.cpp", "hierarchy_dev.hip"]
 HEADERS = ["device.hpp", "kernels_pcg.hpp", "kernels_step.hpp", "kernels_transport.hpp", "kernels_flow.hpp",
            "kernels_leak.hpp", "hierarchy.hpp"]
 

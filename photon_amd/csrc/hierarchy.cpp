@@ -527,6 +527,10 @@ bool build_hierarchy(int nx, int ny, int nz, const float* ax, const float* ay, c
             if (L0.agg[c] >= 0 && L0.leak[c]) C.leak[L0.agg[c]] = 1;
         H.L.push_back(std::move(C));
     }
+    return finish_hierarchy(H, dense_max, err);
+}
+
+bool finish_hierarchy(HostHierarchy& H, int dense_max, std::string* err) {
     // ---- coarser levels (reference rule), then optional extension for the dense solve
     bool reference_phase = true;
     while (true) {
@@ -547,7 +551,8 @@ bool build_hierarchy(int nx, int ny, int nz, const float* ax, const float* ay, c
     }
     if (H.ref_levels == 0) H.ref_levels = (int)H.L.size();
     for (size_t l = 1; l < H.L.size(); ++l) make_ell(H.L[l]);
-    for (size_t l = 0; l + 1 < H.L.size(); ++l) make_children(H.L[l], H.L[l + 1].n);
+    for (size_t l = 0; l + 1 < H.L.size(); ++l)   // level 0's lists may come with a device-built level 1
+        if (!(l == 0 && H.L[0].cptr.size() == (size_t)H.L[1].n + 1)) make_children(H.L[l], H.L[l + 1].n);
     // coarsening can stall (many small sealed pockets, each its own unknown):
     // such a coarsest level is solved by the reference's Gauss-Seidel sweeps
     // only, without the O(n^2) memory and O(n^3) work of the dense inverse

@@ -66,6 +66,22 @@ bool build_hierarchy(int nx, int ny, int nz, const float* ax, const float* ay, c
                      const uint8_t* blocked, const uint8_t* wind_closed, int dense_max, HostHierarchy& H,
                      std::string* err);
 
+// The rest of build_hierarchy once H.L holds levels 0 and 1 (level 0's child
+// lists may be there already): the coarser levels, ELL layouts, child lists,
+// the coarsest level's Gauss-Seidel schedule and dense inverse.
+bool finish_hierarchy(HostHierarchy& H, int dense_max, std::string* err);
+
+// Levels 0 and 1 of build_hierarchy formed on the device, with the same
+// results bit for bit: d_ax/d_ay/d_az are the wind-masked level-0
+// coefficients and d_blocked the solid mask, in device memory; host copies
+// of the masked coefficients (ax/ay/az) go to H.L[0].  Fills H.L[0] (diag,
+// leak, agg, child lists) and H.L[1] (operator, diagonal, leak flags, block
+// coordinates).  stream: the hipStream_t to work on.  Returns "" or why it
+// could not (sizes beyond its key packing); the caller then builds on the host.
+std::string device_levels01(void* stream, int nx, int ny, int nz, const float* d_ax, const float* d_ay,
+                            const float* d_az, const uint8_t* d_blocked, const float* ax, const float* ay,
+                            const float* az, HostHierarchy& H);
+
 // Pressure zones: gas cells joined through faces with a > 0; open when a
 // side or top boundary face with a > 0 is reached (src/sim/gasphase.cpp:111-158).
 int compute_zones(int nx, int ny, int nz, const float* ax, const float* ay, const float* az, const uint8_t* blocked,

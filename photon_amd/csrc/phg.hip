@@ -2276,10 +2276,48 @@ int phg_grid_update_faces(phg_ctx* ctx, int grid, int n, const int* axis, const 
         // and coarsest-solve choice
         std::vector<uint8_t> blocked(G->n);
         HIP_OK(hipMemcpy(blocked.data(), G->blocked, G->n, hipMemcpyDeviceToHost));
+        // levels 0 and 1 (the work that grows with the cells) are formed on the
+        // device from the wind-masked coefficients; the coarser levels, the
+        // coarsest solve and the zones follow on the host.  If the device build
+        // cannot take the grid, the whole set-up runs on the host.
+        phg_grid_plan plan;
+        plan.nx = nx;
+        plan.ny = ny;
+        plan.nz = nz;
+        bool on_device = false;
+        {
+            std::vector<float> m[3] = {co[0], co[1], co[2]};
+            for (int k = 0; k < nz; ++k) {
+                for (int j = 0; j < ny; ++j) {
+                    if (G->wc[0]) m[0][(nx + 1) * (j + ny * k)] = 0.0f;
+                    if (G->wc[1]) m[0][nx + (nx + 1) * (j + ny * k)] = 0.0f;
+                }
+                for (int i = 0; i < nx; ++i) {
+                    if (G->wc[2]) m[1][i + nx * ((ny + 1) * k)] = 0.0f;
+                    if (G->wc[3]) m[1][i + nx * (ny + (ny + 1) * k)] = 0.0f;
+                }
+            }
+            std::vector<void*> tmp;
+            float* dm[3] = {nullptr, nullptr, nullptr};
+            int ru = 0;
+            for (int d = 0; d < 3 && !ru; ++d) ru = upload(ctx, tmp, &dm[d], m[d].data(), m[d].size());
+            std::string derr = ru ? std::string("upload failed")
+                                  : device_levels01(ctx->stream, nx, ny, nz, dm[0], dm[1], dm[2], G->blocked,
+                                                    m[0].data(), m[1].data(), m[2].data(), plan.H);
+            free_all(tmp);
+            std::string herr;
+            if (derr.empty() && finish_hierarchy(plan.H, kDenseMax, &herr)) {
+                compute_zones(nx, ny, nz, co[0].data(), co[1].data(), co[2].data(), blocked.data(), plan.zone,
+                              plan.zone_open);
+                on_device = true;
+            } else if (derr.empty()) {
+                return fail(ctx, PHG_EINVAL, "%s", herr.c_str());
+            }
+        }
         GridDev* NG = nullptr;
         int rc = build_grid_dev(ctx, nx, ny, nz, G->dx, fo[0].data(), fo[1].data(), fo[2].data(), co[0].data(),
                                 co[1].data(), co[2].data(), blocked.data(), G->has_wc ? G->wc : nullptr, G->prec,
-                                nullptr, &NG);
+                                on_device ? &plan : nullptr, &NG);
         if (rc) return rc;
         if (G->single_batch >= 0) {
             rc = phg_batch_destroy(ctx, G->single_batch);
@@ -2296,7 +2334,7 @@ int phg_grid_update_faces(phg_ctx* ctx, int grid, int n, const int* axis, const 
         G->gen = gen + 1;
         G->single_batch = -1;
         if (coarse == PHG_COARSE_GS20 && G->H.L.size() >= 2) G->coarse = PHG_COARSE_GS20;
-        if (rebuilt) *rebuilt = 1;
+        if (rebuilt) *rebuilt = on_device ? 2 : 1;
         return set_cracks(grid);
     });
 }

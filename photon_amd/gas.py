@@ -341,8 +341,9 @@ class Grid:
     def update_faces(self, upd, u=None, v=None, w=None):
         """Face changes (axis, face, open fraction, coefficient, crack flag
         arrays; phg_grid_update_faces); the velocities of closing faces are
-        zeroed in u, v, w (device arrays) when given.  Returns True when the
-        zones and hierarchy were set up again."""
+        zeroed in u, v, w (device arrays) when given.  Returns 0 when the set-up
+        stood, 2 when the zones and hierarchy were set up again with levels 0
+        and 1 formed on the device, 1 when wholly on the host."""
         axis, face, f, a, c = upd
         arrs = [np.ascontiguousarray(axis, np.int32), np.ascontiguousarray(face, np.int32),
                 np.ascontiguousarray(f, np.float32), np.ascontiguousarray(a, np.float32),
@@ -351,7 +352,7 @@ class Grid:
         self.ctx._check(self.ctx.L.phg_grid_update_faces(
             self.ctx.h, self.id, len(arrs[0]), *[x.ctypes.data for x in arrs],
             *[None if x is None else x.ptr for x in (u, v, w)], C.byref(reb)))
-        return bool(reb.value)
+        return reb.value
 
     def set_cracks(self, leaks):
         """Crack faces (lx, ly, lz: one byte per face, None for none) of the
