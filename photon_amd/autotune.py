@@ -41,7 +41,7 @@ def size_class(n_cells: int) -> str:
 class Choice:
     kchunk: int      # 0 = whole columns, -1 = the library's own choice
     sweeps0: int
-    seconds: float   # time to solution of the whole batch with the table's settings
+    seconds: float   # the whole batch's time once this class was tuned (classes are tuned largest first)
 
 
 def _kc(kchunk: int, nz: int) -> int:
@@ -88,8 +88,9 @@ def tune(ctx, grids, bs, xs, params, repeats: int = 2, kchunks=KCHUNKS, sweeps=S
     the library's defaults, each class in turn, largest first, tries every
     candidate with the other classes held at their current choice, timing the
     whole batch, and keeps one that converges and is faster by more than
-    `margin`.  Returns class name -> Choice (seconds: the batch's time with the
-    final table)."""
+    `margin`.  Returns class name -> Choice (seconds: the batch's time after
+    that class's turn), and under the key "_default" the batch's time with the
+    library's defaults (Choice(-1, 1, seconds))."""
     present = [name for _, name in SIZE_CLASSES if any(size_class(G.scen.n) == name for G in grids)]
     table = {name: Choice(-1, 1, float("nan")) for name in present}
 
@@ -100,6 +101,8 @@ def tune(ctx, grids, bs, xs, params, repeats: int = 2, kchunks=KCHUNKS, sweeps=S
     best, ok = run(table)
     if not ok:
         best = float("inf")
+    base = best
+    after = {}
     for name in reversed(present):
         for kc in kchunks:
             for nu in sweeps:
@@ -108,7 +111,10 @@ def tune(ctx, grids, bs, xs, params, repeats: int = 2, kchunks=KCHUNKS, sweeps=S
                 t, ok = run(trial)
                 if ok and t < best * (1.0 - margin):
                     best, table = t, trial
-    return {name: Choice(c.kchunk, c.sweeps0, best) for name, c in table.items()}
+        after[name] = best
+    out = {name: Choice(c.kchunk, c.sweeps0, after[name]) for name, c in table.items()}
+    out["_default"] = Choice(-1, 1, base)
+    return out
 
 
 def default_table() -> dict:
@@ -119,6 +125,9 @@ def default_table() -> dict:
 def table_to_json(table: dict) -> dict:
     def kc(v):
         return "whole column" if v == 0 else "library default" if v < 0 else v
-    return {k: {"kchunk": kc(v.kchunk), "sweeps0": v.sweeps0,
-                "batch_seconds": None if np.isnan(v.seconds) else round(v.seconds, 6)}
-            for k, v in table.items()}
+    out = {k: {"kchunk": kc(v.kchunk), "sweeps0": v.sweeps0,
+               "batch_seconds_after_its_turn": None if np.isnan(v.seconds) else round(v.seconds, 6)}
+           for k, v in table.items() if not k.startswith("_")}
+    if "_default" in table:
+        out["batch_seconds_with_defaults"] = round(table["_default"].seconds, 6)
+    return out

@@ -158,7 +158,10 @@ def test_autotuner_picks_converging_settings(ctx):
     p = gas.solve_params(tol=1e-6, max_iters=300, zero_x=True, use_graph=2, check_every=40)
     table = at.tune(ctx, Gs, dbs, dxs, p, repeats=1, kchunks=(8, 0), sweeps=(1, 2))
     classes = {at.size_class(g.n) for g in grids}
-    assert set(table) == classes
+    assert set(table) == classes | {"_default"}
+    base = table["_default"].seconds
+    after = [table[c].seconds for _, c in at.SIZE_CLASSES if c in table]   # tuned largest first
+    assert all(t <= base for t in after) and all(a >= b for a, b in zip(after[::-1], after[::-1][1:]))
     for c in table.values():   # a tried candidate, or the library's defaults (-1, 1) when none beat them
         assert (c.kchunk in (8, 0) and c.sweeps0 in (1, 2)) or (c.kchunk, c.sweeps0) == (-1, 1)
         assert np.isfinite(c.seconds)
