@@ -191,6 +191,10 @@ typedef struct {
                             0 = 512 (1024 for the smoothing of grids wider than 128 cells), or
                             256, 512 or 1024 for every such pass (1024: the direction update
                             keeps 512); every choice gives the same results bit for bit */
+    int coarse_split;    /* 0: from the first coarse level whose two vectors (and the next
+                            level's) fit in one block's LDS, a system's coarse levels run as one
+                            down and one up launch, one block per system; 1: every coarse level
+                            as its own launches (same results bit for bit; for comparison) */
 } phg_batch_opts;
 int phg_batch_create_ex(phg_ctx *ctx, int nsys, const int *grids, const phg_batch_opts *opts, int *batch_out);
 int phg_batch_destroy(phg_ctx *ctx, int batch);

@@ -90,6 +90,20 @@ template <class T> __device__ __forceinline__ void st_nt(void* p, unsigned i, fl
     gst_nt(static_cast<T*>(p), i, from_f<T>(v));
 }
 
+// ELL column and value of a coarse level in one load: (column << 16 | FP16
+// value bits) for the FP16 coefficient type, (column, FP32 value bits) for FP32
+template <class CT> struct CVPack;
+template <> struct CVPack<_Float16> {
+    using T = uint32_t;
+    static __device__ __forceinline__ int col(T p) { return (int)(p >> 16); }
+    static __device__ __forceinline__ float val(T p) { return (float)__builtin_bit_cast(_Float16, (uint16_t)(p & 0xFFFFu)); }
+};
+template <> struct CVPack<float> {
+    using T = uint64_t;   // column in the low word, value bits in the high word
+    static __device__ __forceinline__ int col(T p) { return (int)(uint32_t)p; }
+    static __device__ __forceinline__ float val(T p) { return __uint_as_float((uint32_t)(p >> 32)); }
+};
+
 // ---- descriptors -----------------------------------------------------------
 // One multigrid level of one system.  Operator arrays belong to the grid and
 // are shared by every system on it; x, b, r, t are the system's own.
@@ -110,6 +124,10 @@ struct DLevel {
     const int* agg;               // unknown -> next-level unknown (-1: none)
     const int* cptr;              // children of next-level unknowns into this level
     const int* cidx;
+    const void* cv;               // coarse levels under 65536 rows: ELL column and value in one word
+                                  // (CVPack), or null
+    const uint32_t* cblk;         // level 0: per level-1 aggregate, its 2x2x2 block's corner cell |
+                                  // child mask << 24 (bit dx | dy << 1 | dz << 2), or null
     void* x;                      // work vectors in the V-cycle vector type
     void* b;
     void* r;

@@ -1731,6 +1731,297 @@ __global__ void __launch_bounds__(TB) k_vc_prolong(const DSys* __restrict__ sys,
         if ((G.active >> g) & 1u) st<VT>(sys[G.first + g].L[l].t, I, t[g]);
 }
 
+// ---------------------------------------------------------------------------
+// V-cycle, the coarse levels that fit in one block's LDS
+// ---------------------------------------------------------------------------
+// From its first coarse level s whose vectors fit in one block's LDS, a
+// system's levels above the coarsest are swept by one block of CT_NT threads
+// in two launches instead of five or six per level:
+//   k_vc_down  the restriction into s, then on each level s .. nl-2 the first
+//              Jacobi sweep from zero, one more sweep, the residual, and the
+//              restriction into the next level, ending with the coarsest
+//              right-hand side;
+//   k_vc_up    after the coarsest solve, on each level nl-2 .. s the
+//              prolongation and two sweeps.
+// A level's two vectors (iterate and second Jacobi buffer) live in LDS and a
+// barrier separates the passes.  Rows are dealt to the threads round-robin,
+// CT_U at a time: every load of a thread's CT_U rows is issued before the
+// first is used, since a block's time is its chain of dependent loads.
+// LDS holds two regions A and B of n_s values each.  Level s keeps its
+// iterate in A and its second buffer in B; a level l > s keeps both in A for
+// even l - s and in B for odd (2 n_l <= n_s), so a restriction or
+// prolongation reads one level while it writes the other.  Each value is
+// formed by the same operations as in k_vc_restrict, k_vc_jac, k_vc_resid and
+// k_vc_prolong and rounded to VT where they store it, so the results are
+// theirs bit for bit; what other launches read (b and t of levels s .. nl-2,
+// the coarsest b, t of level s) is stored to the level's vectors as they
+// store it.
+constexpr int CT_NT = 1024;
+constexpr int CT_U = 4;
+constexpr int CT_EC = 6;   // ELL entries loaded together (a coarse level's usual row width)
+
+// LDS bytes of the down and up blocks of a system whose LDS levels start at s
+__host__ __device__ constexpr size_t tail_lds(int n_s, size_t vts) { return 2 * (size_t)n_s * vts; }
+
+// The CT_U rows I0 + u CT_NT of a pass; rows past n repeat row n - 1 (their
+// loads stay in bounds; nothing is stored for them)
+template <int U = CT_U>
+struct TailRows {
+    int I[U];
+    bool ok[U];
+    __device__ __forceinline__ TailRows(int I0, int n) {
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+            ok[u] = I0 + u * CT_NT < n;
+            I[u] = ok[u] ? I0 + u * CT_NT : n - 1;
+        }
+    }
+};
+
+// acc[u] + the sum over row I[u]'s ELL entries, in entry order, of
+// val * src[col] (src in LDS): ell_accum for one system and CT_U rows, each
+// entry's column and value in one load (the level's packed ELL)
+template <class CT, class VT>
+__device__ __forceinline__ void ell_rows(const DLevel& L, const TailRows<>& R, const VT* src, float (&acc)[CT_U]) {
+    using P = CVPack<CT>;
+    const typename P::T* cv = static_cast<const typename P::T*>(L.cv);
+    const int K = L.ellk;   // >= 1
+    int p0 = 0;
+    do {
+        typename P::T e[CT_U][CT_EC];
+#pragma unroll
+        for (int q = 0; q < CT_EC; ++q)
+#pragma unroll
+            for (int u = 0; u < CT_U; ++u)
+                e[u][q] = gld(cv, (unsigned)min(p0 + q, K - 1) * (unsigned)L.n + (unsigned)R.I[u]);
+        float x[CT_U][CT_EC];
+#pragma unroll
+        for (int q = 0; q < CT_EC; ++q)
+#pragma unroll
+            for (int u = 0; u < CT_U; ++u) x[u][q] = to_f(src[P::col(e[u][q])]);
+#pragma unroll
+        for (int q = 0; q < CT_EC; ++q) {
+            if (p0 + q >= K) break;
+#pragma unroll
+            for (int u = 0; u < CT_U; ++u) {
+                const float vq = P::val(e[u][q]);
+                acc[u] += vq * x[u][q];
+            }
+        }
+        p0 += CT_EC;
+    } while (p0 < K);
+}
+
+// restriction from level 0 into level 1 (k_vc_restrict) by the level-1
+// aggregates' 2x2x2 blocks: b of rows R, the children's r0 summed in
+// ascending cell order from 0 and scaled (the children list's order)
+template <class VT>
+__device__ __forceinline__ void block_rows(const DLevel& L0, const TailRows<>& R, float (&b)[CT_U]) {
+    const unsigned nx = (unsigned)L0.nx, sxy = nx * (unsigned)L0.ny;
+    const unsigned off[8] = {0u, 1u, nx, nx + 1u, sxy, sxy + 1u, sxy + nx, sxy + nx + 1u};
+    uint32_t w[CT_U];
+#pragma unroll
+    for (int u = 0; u < CT_U; ++u) w[u] = gld(L0.cblk, (unsigned)R.I[u]);
+    float x[CT_U][8];
+#pragma unroll
+    for (int u = 0; u < CT_U; ++u) {
+        const unsigned base = w[u] & 0xFFFFFFu;
+#pragma unroll
+        for (int q = 0; q < 8; ++q) x[u][q] = ld<VT>(L0.r, (w[u] >> (24 + q)) & 1u ? base + off[q] : base);
+    }
+#pragma unroll
+    for (int u = 0; u < CT_U; ++u) {
+        float acc = 0.0f;
+#pragma unroll
+        for (int q = 0; q < 8; ++q)
+            if ((w[u] >> (24 + q)) & 1u) acc += x[u][q];
+        b[u] = to_f(from_f<VT>(acc * (1.0f / VC_LEVEL_SCALE)));
+    }
+}
+
+// restriction (k_vc_restrict): b of rows R of the next level, the children of
+// each in level `from` summed in list order from 0 and scaled; val(c) reads
+// child c
+template <class VT, int U, class F>
+__device__ __forceinline__ void child_rows(const DLevel& from, const TailRows<U>& R, F val, float (&b)[U]) {
+    int p0[U], p1[U], m = 0;
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+        p0[u] = gld(from.cptr, (unsigned)R.I[u]);
+        p1[u] = gld(from.cptr, (unsigned)R.I[u] + 1u);   // p1 > p0: an aggregate has a child
+    }
+    float acc[U];
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+        acc[u] = 0.0f;
+        m = max(m, p1[u] - p0[u]);
+    }
+    for (int j = 0; j < m; j += 8) {
+        float x[U][8];
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+            int c[8];
+#pragma unroll
+            for (int q = 0; q < 8; ++q) c[q] = gld(from.cidx, (unsigned)min(p0[u] + j + q, p1[u] - 1));
+#pragma unroll
+            for (int q = 0; q < 8; ++q) x[u][q] = val(c[q]);
+        }
+#pragma unroll
+        for (int u = 0; u < U; ++u)
+#pragma unroll
+            for (int q = 0; q < 8; ++q)
+                if (p0[u] + j + q < p1[u]) acc[u] += x[u][q];
+    }
+#pragma unroll
+    for (int u = 0; u < U; ++u) b[u] = to_f(from_f<VT>(acc[u] * (1.0f / VC_LEVEL_SCALE)));
+}
+
+// one damped Jacobi sweep src -> dst (k_vc_jac); the new values also go to
+// `out` (a level vector, or null)
+template <class CT, class VT>
+__device__ __forceinline__ void tail_jac(const DLevel& L, const VT* src, VT* dst, void* out) {
+    for (int I0 = (int)threadIdx.x; I0 < L.n; I0 += CT_U * CT_NT) {
+        const TailRows<> R(I0, L.n);
+        float dg[CT_U], acc[CT_U], self[CT_U];
+#pragma unroll
+        for (int u = 0; u < CT_U; ++u) {
+            dg[u] = ld<CT>(L.diag, (unsigned)R.I[u]);
+            acc[u] = ld<VT>(L.b, (unsigned)R.I[u]);
+            self[u] = to_f(src[R.I[u]]);
+        }
+        ell_rows<CT, VT>(L, R, src, acc);
+#pragma unroll
+        for (int u = 0; u < CT_U; ++u) {
+            if (!R.ok[u]) continue;
+            float o = 0.0f;
+            if (dg[u] > 0) o = (1.0f - 0.7f) * self[u] + (0.7f * acc[u]) / dg[u];
+            dst[R.I[u]] = from_f<VT>(o);
+            if (out) st<VT>(out, (unsigned)R.I[u], o);
+        }
+    }
+}
+
+// a level's two LDS vectors: level s in A and B, a level l > s in A (even
+// l - s) or B (odd)
+template <class VT>
+__device__ __forceinline__ VT* tail_region(VT* lds, int ns, int l, int s) {
+    return lds + (((l - s) & 1) ? ns : 0);
+}
+
+template <class CT, class VT>
+__global__ void __launch_bounds__(CT_NT) __attribute__((amdgpu_waves_per_eu(8))) k_vc_down(const DSys* __restrict__ sys, const int* __restrict__ states,
+                                                   const int2* __restrict__ tab) {
+    const int2 e = tab[blockIdx.x];   // (system, first LDS level)
+    if (!states[e.x * ST_NSTATE + ST_ACTIVE]) return;
+    const DSys& S = sys[e.x];
+    const int nl = S.nl, s = e.y, ns = S.L[s].n, tid = (int)threadIdx.x;
+    extern __shared__ float tail_sh[];
+    VT* const lds = reinterpret_cast<VT*>(tail_sh);
+    for (int l = s; l < nl; ++l) {
+        const DLevel& F = S.L[l - 1];
+        const DLevel& L = S.L[l];
+        const int n = L.n;
+        const VT* Rl = tail_region(lds, ns, l - 1, s);   // level l-1's residual (l > s)
+        VT* X = tail_region(lds, ns, l, s);
+        VT* T = l == s ? lds + ns : X + n;
+        const bool top = l == nl - 1;
+        // restriction into level l, and below the coarsest the first Jacobi
+        // sweep from zero into X
+        auto restrict_rows = [&](auto rows_tag, auto sum) {
+            constexpr int U = decltype(rows_tag)::value;
+            for (int I0 = tid; I0 < n; I0 += U * CT_NT) {
+                const TailRows<U> R(I0, n);
+                float b[U], dg[U];
+                sum(R, b);
+#pragma unroll
+                for (int u = 0; u < U; ++u) dg[u] = top ? 0.0f : ld<CT>(L.diag, (unsigned)R.I[u]);
+#pragma unroll
+                for (int u = 0; u < U; ++u) {
+                    if (!R.ok[u]) continue;
+                    st<VT>(L.b, (unsigned)R.I[u], b[u]);
+                    if (!top) X[R.I[u]] = from_f<VT>(dg[u] > 0 ? (0.7f * b[u]) / dg[u] : 0.0f);
+                }
+            }
+        };
+        using U2 = std::integral_constant<int, 2>;
+        if (l > s)
+            restrict_rows(U2{}, [&](const TailRows<2>& R, float (&b)[2]) {
+                child_rows<VT>(F, R, [&](int c) { return to_f(Rl[c]); }, b);
+            });
+        else if (F.cblk)
+            restrict_rows(std::integral_constant<int, CT_U>{},
+                          [&](const TailRows<>& R, float (&b)[CT_U]) { block_rows<VT>(F, R, b); });
+        else
+            restrict_rows(U2{}, [&](const TailRows<2>& R, float (&b)[2]) {
+                child_rows<VT>(F, R, [&](int c) { return ld<VT>(F.r, (unsigned)c); }, b);
+            });
+        if (top) break;
+        __syncthreads();
+        // one sweep x -> t; t is the iterate the up leg corrects
+        tail_jac<CT, VT>(L, X, T, L.t);
+        __syncthreads();
+        // the residual of t into X (k_vc_resid)
+        for (int I0 = tid; I0 < n; I0 += CT_U * CT_NT) {
+            const TailRows<> R(I0, n);
+            float dg[CT_U], r[CT_U];
+#pragma unroll
+            for (int u = 0; u < CT_U; ++u) {
+                dg[u] = ld<CT>(L.diag, (unsigned)R.I[u]);
+                r[u] = ld<VT>(L.b, (unsigned)R.I[u]) - dg[u] * to_f(T[R.I[u]]);
+            }
+            ell_rows<CT, VT>(L, R, T, r);
+#pragma unroll
+            for (int u = 0; u < CT_U; ++u)
+                if (R.ok[u]) X[R.I[u]] = from_f<VT>(dg[u] > 0 ? r[u] : 0.0f);
+        }
+        __syncthreads();
+    }
+}
+
+template <class CT, class VT>
+__global__ void __launch_bounds__(CT_NT) __attribute__((amdgpu_waves_per_eu(8))) k_vc_up(const DSys* __restrict__ sys, const int* __restrict__ states,
+                                                 const int2* __restrict__ tab) {
+    const int2 e = tab[blockIdx.x];
+    if (!states[e.x * ST_NSTATE + ST_ACTIVE]) return;
+    const DSys& S = sys[e.x];
+    const int nl = S.nl, s = e.y, ns = S.L[s].n, tid = (int)threadIdx.x;
+    extern __shared__ float tail_sh[];
+    VT* const lds = reinterpret_cast<VT*>(tail_sh);
+    for (int l = nl - 2; l >= s; --l) {
+        const DLevel& L = S.L[l];
+        const DLevel& Cl = S.L[l + 1];
+        const int n = L.n;
+        // the prolonged iterate P and the middle buffer Q: level s prolongs
+        // into A while it reads level s+1 from B
+        VT* P = l == s ? lds : tail_region(lds, ns, l, s) + n;
+        VT* Q = l == s ? lds + ns : tail_region(lds, ns, l, s);
+        const VT* Ct = tail_region(lds, ns, l + 1, s) + Cl.n;   // t of level l+1 below the coarsest
+        const bool top = l + 1 == nl - 1;
+        // t_l += t_{l+1}[agg] (k_vc_prolong)
+        for (int I0 = tid; I0 < n; I0 += CT_U * CT_NT) {
+            const TailRows<> R(I0, n);
+            int a[CT_U];
+            float t[CT_U], tc[CT_U];
+#pragma unroll
+            for (int u = 0; u < CT_U; ++u) {
+                a[u] = gld(L.agg, (unsigned)R.I[u]);
+                t[u] = ld<VT>(L.t, (unsigned)R.I[u]);
+            }
+#pragma unroll
+            for (int u = 0; u < CT_U; ++u) tc[u] = top ? ld<VT>(Cl.t, (unsigned)a[u]) : to_f(Ct[a[u]]);
+#pragma unroll
+            for (int u = 0; u < CT_U; ++u)
+                if (R.ok[u]) P[R.I[u]] = from_f<VT>(t[u] + VC_LEVEL_SCALE * tc[u]);
+        }
+        __syncthreads();
+        tail_jac<CT, VT>(L, P, Q, nullptr);
+        __syncthreads();
+        tail_jac<CT, VT>(L, Q, P, l == s ? L.t : nullptr);
+        __syncthreads();
+    }
+}
+
+// ---------------------------------------------------------------------------
 // coarsest level: t = B b in FP64, B the projected inverse of the grounded
 // coarsest operator (symmetric, so column i is read as row i).  A block takes
 // up to CGS systems that share one grid, so each entry of B is read once for

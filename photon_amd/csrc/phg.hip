@@ -52,14 +52,16 @@ namespace phg {
 enum KClass {
     K_RESID_INIT, K_DIR_APPLY, K_PRE_RED, K_RB_BLACK, K_RESID0, K_POST_BLACK, K_POST_RED, K_FIN, K_VC_RESTRICT,
     K_VC_JAC, K_VC_RESID, K_VC_PROLONG, K_VC_COARSE, K_XFLUSH, K_ZONE, K_RHS, K_CORRECT, K_ENERGY, K_ADVECT,
-    K_COUNT, K_PRE_SMOOTH, K_POST_SMOOTH, K_TURB, K_HEAT, K_MOMENTUM, K_COMBUST, K_GASMISC, K_LEAK, K_NCLASS
+    K_COUNT, K_PRE_SMOOTH, K_POST_SMOOTH, K_TURB, K_HEAT, K_MOMENTUM, K_COMBUST, K_GASMISC, K_LEAK, K_VC_DOWN,
+    K_VC_UP, K_NCLASS
 };
 static const char* kKName[K_NCLASS] = {"resid_init",  "dir_apply",  "pre_red",    "rb_black",  "resid0",
                                        "post_black",  "post_red",   "finalize",   "vc_restrict", "vc_jacobi",
                                        "vc_resid",    "vc_prolong", "vc_coarse",  "x_flush",   "zone_mean",
                                        "rhs",         "correct",    "energy",     "advect",    "count_active",
                                        "pre_smooth",  "post_smooth", "turbulence", "heat_diffusion",
-                                       "momentum",    "combustion",  "gas_misc",   "leakage"};
+                                       "momentum",    "combustion",  "gas_misc",   "leakage",   "vc_down",
+                                       "vc_up"};
 
 struct GridDev {
     int nx = 0, ny = 0, nz = 0, n = 0;
@@ -79,6 +81,8 @@ struct GridDev {
         int* agg = nullptr;
         int* cptr = nullptr;
         int* cidx = nullptr;
+        void* cv = nullptr;          // coarse levels under 65536 rows: ELL column and value packed
+        uint32_t* cblk = nullptr;    // level 0: each level-1 aggregate as its 2x2x2 block
     };
     std::vector<Lev> lv;
     double* cinv = nullptr;       // the coarsest inverse whole (nc <= CP_MIN) ...
@@ -185,6 +189,13 @@ struct BatchDev {
     int2* tab_gs = nullptr;        // systems whose coarsest level takes the Gauss-Seidel sweeps
     int tab_gs_len = 0;
     int maxnc_gs = 0;              // their largest coarsest level (LDS of k_vc_coarse_gs)
+    // coarse levels in LDS: (system, first LDS level) of the systems whose levels
+    // from that one down take k_vc_down / k_vc_up (one block each); tail_s per
+    // system (nl: every coarse level as its own launches)
+    int2* tab_tail = nullptr;
+    int tab_tail_len = 0;
+    size_t tail_lds = 0;
+    std::vector<int> tail_s;
     double* scal = nullptr;
     int* state = nullptr;
     int* d_count = nullptr;
@@ -220,6 +231,7 @@ struct phg_ctx {
     double prof_ms[K_NCLASS] = {};
     long long prof_n[K_NCLASS] = {};
     double* scratch_dev = nullptr;   // this device's copy of `scratch`
+    size_t lds_block = 65536;        // LDS one block may take on this device
 };
 
 namespace {
@@ -451,6 +463,9 @@ void vcycle_t(phg_ctx* ctx, BatchDev* B, int first) {
         launch(ctx, K_VC_RESID, k_vc_resid<CC, VT>, NG[l], 0, S, (const int*)B->state, TG[l], l);
         launch(ctx, K_VC_RESTRICT, k_vc_restrict<CC, VT>, NG[l + 1], 0, S, (const int*)B->state, TG[l + 1], l);
     }
+    if (B->tab_tail_len > 0)
+        launch_nt(ctx, K_VC_DOWN, k_vc_down<CC, VT>, B->tab_tail_len, CT_NT, B->tail_lds, S, (const int*)B->state,
+                  (const int2*)B->tab_tail);
     auto coarse = [&](auto kern, int cgs) {
         launch(ctx, K_VC_COARSE, kern, B->tab_coarse_len, coarse_lds(cgs, B->maxnc), S, (const int*)B->state,
                (const int4*)B->tab_coarse);
@@ -476,6 +491,9 @@ void vcycle_t(phg_ctx* ctx, BatchDev* B, int first) {
         }
         launch(ctx, K_VC_COARSE, k_vc_coarse_r<VT>, B->tab_cr_len, 0, S, (const int*)B->state, (const int4*)B->tab_cr);
     }
+    if (B->tab_tail_len > 0)
+        launch_nt(ctx, K_VC_UP, k_vc_up<CC, VT>, B->tab_tail_len, CT_NT, B->tail_lds, S, (const int*)B->state,
+                  (const int2*)B->tab_tail);
     for (int l = B->maxl - 2; l >= 1; --l) {
         launch(ctx, K_VC_PROLONG, k_vc_prolong<VT>, NG[l], 0, S, (const int*)B->state, TG[l], l);
         launch(ctx, K_VC_JAC, k_vc_jac<CC, VT>, NG[l], 0, S, (const int*)B->state, TG[l], l, 0);
@@ -705,6 +723,9 @@ int phg_create(phg_ctx** out, int device) {
         return PHG_EHIP;
     }
     ctx->scratch_dev = static_cast<double*>(sp);
+    hipDeviceProp_t prop;
+    if (hipGetDeviceProperties(&prop, device) == hipSuccess && prop.sharedMemPerBlock > 0)
+        ctx->lds_block = prop.sharedMemPerBlock;
     *out = ctx;
     return PHG_OK;
 }
@@ -916,6 +937,47 @@ int build_grid_dev(phg_ctx* ctx, int nx, int ny, int nz, float dx, const float* 
                 UP(D.agg, L.agg.data(), L.agg.size())
                 UP(D.cptr, L.cptr.data(), L.cptr.size())
                 UP(D.cidx, L.cidx.data(), L.cidx.size())
+            }
+            if (l > 0 && L.n < 65536 && !rc) {
+                // one load per ELL entry: (column << 16 | FP16 value bits), or
+                // (column, FP32 value bits) in the FP32 precision
+                const size_t m = L.ell_col.size();
+                std::vector<uint32_t> pk(precision == PHG_PREC_FP32 ? 2 * m : m);
+                for (size_t i = 0; i < m; ++i) {
+                    if (precision == PHG_PREC_FP32) {
+                        pk[2 * i] = (uint32_t)L.ell_col[i];
+                        std::memcpy(&pk[2 * i + 1], &L.ell_val[i], 4);
+                    } else {
+                        pk[i] = (uint32_t)L.ell_col[i] << 16 | f32_to_f16_bits(L.ell_val[i]);
+                    }
+                }
+                uint32_t* q = nullptr;
+                rc = upload(ctx, G->allocs, &q, pk.data(), pk.size());
+                D.cv = q;
+            }
+            if (l == 0 && nl > 1 && G->n < (1 << 24) && !rc) {
+                // each level-1 aggregate lies in one 2x2x2 block of cells: its
+                // corner cell | the mask of its children among the block's eight
+                // (bit dx | dy << 1 | dz << 2; ascending bits are ascending cells)
+                const int n1 = G->H.L[1].n, sxy = nx * ny;
+                std::vector<uint32_t> cb(n1);
+                bool ok = true;
+                for (int I = 0; I < n1 && ok; ++I) {
+                    int corner = -1, last = -1;
+                    uint32_t m = 0;
+                    for (int p = L.cptr[I]; p < L.cptr[I + 1] && ok; ++p) {
+                        const int c = L.cidx[p], k = c / sxy, j = (c - k * sxy) / nx, i = c - k * sxy - j * nx;
+                        const int cc = (i & ~1) + nx * ((j & ~1) + ny * (k & ~1));
+                        const int bit = (i & 1) | (j & 1) << 1 | (k & 1) << 2;
+                        if (corner < 0) corner = cc;
+                        ok = cc == corner && bit > last;
+                        last = bit;
+                        m |= 1u << bit;
+                    }
+                    ok = ok && corner >= 0;
+                    cb[I] = (uint32_t)corner | m << 24;
+                }
+                if (ok) UP(D.cblk, cb.data(), cb.size())
             }
         }
         G->dense = !G->H.cinv.empty();
@@ -1151,7 +1213,8 @@ int phg_batch_create_ex(phg_ctx* ctx, int nsys, const int* grids, const phg_batc
         if (opts && (opts->kchunk < 0 || opts->level0_blocks < 0 || opts->unfused < 0 || opts->unfused > 1 ||
                      opts->sweeps0 < 0 || opts->sweeps0 > kMaxSweeps0 ||
                      (opts->fused_threads != 0 && opts->fused_threads != TB && opts->fused_threads != 2 * TB &&
-                      opts->fused_threads != 4 * TB)))
+                      opts->fused_threads != 4 * TB) ||
+                     opts->coarse_split < 0 || opts->coarse_split > 1))
             return fail(ctx, PHG_EINVAL, "bad batch options");
         if (opts)
             for (int s = 0; s < nsys; ++s) {
@@ -1266,6 +1329,8 @@ int phg_batch_create_ex(phg_ctx* ctx, int nsys, const int* grids, const phg_batc
                 D.agg = GL.agg;
                 D.cptr = GL.cptr;
                 D.cidx = GL.cidx;
+                D.cv = GL.cv;
+                D.cblk = GL.cblk;
                 if (l == 0) {
                     D.nx = G->nx;
                     D.ny = G->ny;
@@ -1292,6 +1357,32 @@ int phg_batch_create_ex(phg_ctx* ctx, int nsys, const int* grids, const phg_batc
             }
         }
         if (!rc) rc = dalloc(ctx, B->allocs, &B->d_sys, (size_t)nsys);
+        // coarse levels in LDS: each system from its first level l (1 <= l <= nl-2)
+        // whose down and up blocks fit the device's LDS per block
+        B->tail_s.assign(nsys, 0);
+        if (!rc) {
+            std::vector<int2> tt;
+            for (int s = 0; s < nsys; ++s) {
+                const GridDev* G = get_grid(ctx, grids[s]);
+                const int nl = (int)G->lv.size();
+                int ts = nl;   // no LDS levels
+                if (!(opts && opts->coarse_split))
+                    for (int l = 1; l + 2 <= nl; ++l) {
+                        // every level below l but the coarsest must fit half the block's LDS
+                        const bool fits = (l + 2 == nl || 2 * G->lv[l + 1].n <= G->lv[l].n) && G->lv[l].cv;
+                        const size_t need = tail_lds(G->lv[l].n, vts);
+                        if (fits && need <= ctx->lds_block) {
+                            ts = l;
+                            B->tail_lds = std::max(B->tail_lds, need);
+                            break;
+                        }
+                    }
+                B->tail_s[s] = ts;
+                if (ts + 2 <= nl) tt.push_back(make_int2(s, ts));
+            }
+            B->tab_tail_len = (int)tt.size();
+            if (!tt.empty()) rc = upload(ctx, B->allocs, &B->tab_tail, tt.data(), tt.size());
+        }
         // tile tables: level 0 by column tiles x k-chunks; coarse levels by 256
         // unknowns of a group of up to CV_GRP consecutive systems on one grid
         B->tab.assign(1, nullptr);
@@ -1346,14 +1437,15 @@ int phg_batch_create_ex(phg_ctx* ctx, int nsys, const int* grids, const phg_batc
         for (int l = 1; l < B->maxl && !rc; ++l) {
             // groups of CV_GRP, or of 2 or 1 while the level's launches would offer
             // the chip fewer than kCoarseBlocks blocks (each system's arithmetic is
-            // the same in any group)
+            // the same in any group); systems whose LDS levels cover level l (and
+            // so the restriction into it) are left to k_vc_down / k_vc_up
             auto level_tiles = [&](int grp, std::vector<int4>* out) {
                 long nb = 0;
                 for (int s = 0; s < nsys;) {
                     GridDev* G = get_grid(ctx, grids[s]);
                     int cnt = 1;
                     while (s + cnt < nsys && cnt < grp && grids[s + cnt] == grids[s]) ++cnt;
-                    if ((int)G->lv.size() > l) {
+                    if ((int)G->lv.size() > l && B->tail_s[s] > l) {
                         nb += tiles(G->lv[l].n);
                         if (out)
                             for (int k = 0; k < tiles(G->lv[l].n); ++k)

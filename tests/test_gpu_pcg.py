@@ -196,6 +196,45 @@ def test_batch_splits_smoothing_per_system(ctx, prec, width):
         G.destroy()
 
 
+@pytest.mark.parametrize("prec", [gas.PREC_FP32, gas.PREC_MIXED_F16, gas.PREC_MIXED_BF16])
+def test_coarse_levels_in_lds_equal_split_launches(ctx, prec):
+    """The coarse levels that fit in one block's LDS run as one down and one
+    up launch per system (k_vc_down / k_vc_up); every system's solve is bit
+    for bit the one whose coarse levels each take their own launches.  The
+    mix holds the 80x50x40 house twice (its LDS levels start at level 1 in
+    FP16 and BF16, at level 2 in FP32), batch scenarios of four sizes (three
+    or four levels) and, in FP16, the 160x100x80 house (LDS levels from
+    level 2, level 1 on the separate launches)."""
+    house = sc.house(1)
+    gs = [house, house] + [sc.batch_grid(s) for s in (0, 3, 5, 10)]
+    if prec == gas.PREC_MIXED_F16:
+        gs.append(sc.house(1, 160, 100, 80, 0.1))
+    Gs = [gas.Grid(ctx, g, precision=prec) for g in gs[1:]]
+    Gs.insert(0, Gs[0])   # the house twice on one grid
+    bs = [sc.rhs_from_divergence(g, sc.divergence_field(g, 700 + s), DT) for s, g in enumerate(gs)]
+    p = gas.solve_params(tol=1e-6, max_iters=200, zero_x=True)
+    res = []
+    for split in (False, True):
+        B = gas.Batch(ctx, Gs, coarse_split=split)
+        dbs = [ctx.to_device(b) for b in bs]
+        dxs = [ctx.array(g.n) for g in gs]
+        ctx.profile(True, classes=["vc_down", "vc_up"])
+        ctx.profile_reset()
+        stats = B.solve(dbs, dxs, p)
+        prof = ctx.profile_read()
+        ctx.profile(False)
+        assert all(s["status"] == gas.CONVERGED for s in stats), stats
+        res.append(([d.download() for d in dxs], [s["iters"] for s in stats]))
+        # the LDS launches run exactly when the batch asks for them
+        assert (prof["vc_down"][1] > 0) == (not split) and (prof["vc_up"][1] > 0) == (not split)
+        B.destroy()
+    assert res[0][1] == res[1][1]
+    for k, (a, b) in enumerate(zip(res[0][0], res[1][0])):
+        assert np.array_equal(a, b), f"system {k}"
+    for G in Gs[1:]:
+        G.destroy()
+
+
 def test_graph_replay_matches_direct(ctx, small):
     g, H = small
     G = gas.Grid(ctx, g)
