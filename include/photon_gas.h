@@ -191,10 +191,11 @@ typedef struct {
                             0 = 512 (1024 for the smoothing of grids wider than 128 cells), or
                             256, 512 or 1024 for every such pass (1024: the direction update
                             keeps 512); every choice gives the same results bit for bit */
-    int coarse_split;    /* 0: from the first coarse level whose two vectors (and the next
-                            level's) fit in one block's LDS, a system's coarse levels run as one
-                            down and one up launch, one block per system; 1: every coarse level
-                            as its own launches (same results bit for bit; for comparison) */
+    int coarse_lds;      /* coarse levels in LDS: from its first coarse level whose vectors fit
+                            one block's LDS, a system's coarse levels run as one down and one up
+                            launch, one block per system, instead of several launches per level
+                            (same results bit for bit).  0: when at least two such blocks per CU
+                            would run (fewer leave the chip idle); 1: never; 2: always */
 } phg_batch_opts;
 int phg_batch_create_ex(phg_ctx *ctx, int nsys, const int *grids, const phg_batch_opts *opts, int *batch_out);
 int phg_batch_destroy(phg_ctx *ctx, int batch);

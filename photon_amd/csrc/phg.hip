@@ -232,6 +232,7 @@ struct phg_ctx {
     long long prof_n[K_NCLASS] = {};
     double* scratch_dev = nullptr;   // this device's copy of `scratch`
     size_t lds_block = 65536;        // LDS one block may take on this device
+    int cus = 256;                   // its compute units
 };
 
 namespace {
@@ -724,8 +725,10 @@ int phg_create(phg_ctx** out, int device) {
     }
     ctx->scratch_dev = static_cast<double*>(sp);
     hipDeviceProp_t prop;
-    if (hipGetDeviceProperties(&prop, device) == hipSuccess && prop.sharedMemPerBlock > 0)
-        ctx->lds_block = prop.sharedMemPerBlock;
+    if (hipGetDeviceProperties(&prop, device) == hipSuccess) {
+        if (prop.sharedMemPerBlock > 0) ctx->lds_block = prop.sharedMemPerBlock;
+        if (prop.multiProcessorCount > 0) ctx->cus = prop.multiProcessorCount;
+    }
     *out = ctx;
     return PHG_OK;
 }
@@ -1214,7 +1217,7 @@ int phg_batch_create_ex(phg_ctx* ctx, int nsys, const int* grids, const phg_batc
                      opts->sweeps0 < 0 || opts->sweeps0 > kMaxSweeps0 ||
                      (opts->fused_threads != 0 && opts->fused_threads != TB && opts->fused_threads != 2 * TB &&
                       opts->fused_threads != 4 * TB) ||
-                     opts->coarse_split < 0 || opts->coarse_split > 1))
+                     opts->coarse_lds < 0 || opts->coarse_lds > 2))
             return fail(ctx, PHG_EINVAL, "bad batch options");
         if (opts)
             for (int s = 0; s < nsys; ++s) {
@@ -1358,15 +1361,17 @@ int phg_batch_create_ex(phg_ctx* ctx, int nsys, const int* grids, const phg_batc
         }
         if (!rc) rc = dalloc(ctx, B->allocs, &B->d_sys, (size_t)nsys);
         // coarse levels in LDS: each system from its first level l (1 <= l <= nl-2)
-        // whose down and up blocks fit the device's LDS per block
+        // whose down and up blocks fit the device's LDS per block; by default only
+        // when at least two such blocks per CU would run
         B->tail_s.assign(nsys, 0);
         if (!rc) {
+            const int mode = opts ? opts->coarse_lds : 0;
             std::vector<int2> tt;
             for (int s = 0; s < nsys; ++s) {
                 const GridDev* G = get_grid(ctx, grids[s]);
                 const int nl = (int)G->lv.size();
                 int ts = nl;   // no LDS levels
-                if (!(opts && opts->coarse_split))
+                if (mode != 1)
                     for (int l = 1; l + 2 <= nl; ++l) {
                         // every level below l but the coarsest must fit half the block's LDS
                         const bool fits = (l + 2 == nl || 2 * G->lv[l + 1].n <= G->lv[l].n) && G->lv[l].cv;
@@ -1380,6 +1385,11 @@ int phg_batch_create_ex(phg_ctx* ctx, int nsys, const int* grids, const phg_batc
                 B->tail_s[s] = ts;
                 if (ts + 2 <= nl) tt.push_back(make_int2(s, ts));
             }
+            if (mode == 0 && (long)tt.size() < 2L * ctx->cus) {
+                tt.clear();
+                for (int s = 0; s < nsys; ++s) B->tail_s[s] = (int)get_grid(ctx, grids[s])->lv.size();
+            }
+            if (tt.empty()) B->tail_lds = 0;
             B->tab_tail_len = (int)tt.size();
             if (!tt.empty()) rc = upload(ctx, B->allocs, &B->tab_tail, tt.data(), tt.size());
         }

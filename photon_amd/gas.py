@@ -52,7 +52,7 @@ class SolveParams(C.Structure):
 class BatchOpts(C.Structure):
     _fields_ = [("kchunk", C.c_int), ("level0_blocks", C.c_int), ("unfused", C.c_int), ("sweeps0", C.c_int),
                 ("sys_kchunk", C.c_void_p), ("sys_sweeps0", C.c_void_p), ("fused_threads", C.c_int),
-                ("coarse_split", C.c_int)]
+                ("coarse_lds", C.c_int)]
 
 
 class Wind(C.Structure):
@@ -413,7 +413,7 @@ class Batch:
     """A fixed set of systems solved together (phg_batch_create / _solve)."""
 
     def __init__(self, ctx: Context, grids, kchunk=0, level0_blocks=0, unfused=False, sweeps0=0, sys_kchunk=None,
-                 sys_sweeps0=None, fused_threads=0, coarse_split=False):
+                 sys_sweeps0=None, fused_threads=0, coarse_lds=0):
         self.ctx = ctx
         self.grids = list(grids)
         ids = np.array([g.id for g in self.grids], np.int32)
@@ -424,7 +424,7 @@ class Batch:
             assert a is None or len(a) == len(ids)
         opts = BatchOpts(kchunk, level0_blocks, 1 if unfused else 0, sweeps0,
                          None if kc is None else kc.ctypes.data, None if sw is None else sw.ctypes.data,
-                         fused_threads, 1 if coarse_split else 0)
+                         fused_threads, coarse_lds)
         ctx._check(ctx.L.phg_batch_create_ex(ctx.h, len(ids), ids.ctypes.data, C.byref(opts), C.byref(bid)))
         self.id = bid.value
 

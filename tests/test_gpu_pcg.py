@@ -199,7 +199,8 @@ def test_batch_splits_smoothing_per_system(ctx, prec, width):
 @pytest.mark.parametrize("prec", [gas.PREC_FP32, gas.PREC_MIXED_F16, gas.PREC_MIXED_BF16])
 def test_coarse_levels_in_lds_equal_split_launches(ctx, prec):
     """The coarse levels that fit in one block's LDS run as one down and one
-    up launch per system (k_vc_down / k_vc_up); every system's solve is bit
+    up launch per system (k_vc_down / k_vc_up; forced here, by default only for
+    batches of at least two such blocks per CU); every system's solve is bit
     for bit the one whose coarse levels each take their own launches.  The
     mix holds the 80x50x40 house twice (its LDS levels start at level 1 in
     FP16 and BF16, at level 2 in FP32), batch scenarios of four sizes (three
@@ -215,7 +216,7 @@ def test_coarse_levels_in_lds_equal_split_launches(ctx, prec):
     p = gas.solve_params(tol=1e-6, max_iters=200, zero_x=True)
     res = []
     for split in (False, True):
-        B = gas.Batch(ctx, Gs, coarse_split=split)
+        B = gas.Batch(ctx, Gs, coarse_lds=1 if split else 2)
         dbs = [ctx.to_device(b) for b in bs]
         dxs = [ctx.array(g.n) for g in gs]
         ctx.profile(True, classes=["vc_down", "vc_up"])
