@@ -140,7 +140,9 @@ struct DLevel {
 // S_RPREV: ||r|| at the start of the current round (restarts must make progress)
 // S_PNEXT: the ||r||^2 / ||b||^2 below which the residual's closed-zone means
 // are removed next (see k_proj_r)
-enum { S_RZ = 0, S_DQ, S_ALPHA, S_BETA, S_RR, S_BB, S_TOL, S_MAXR, S_INV, S_ZS, S_ZSN, S_RPREV, S_PNEXT, S_NSCAL = 16 };
+// S_DS: the power-of-two scale the last direction d is stored with (the S_ZS of
+// its z; in FP32 d is stored unscaled)
+enum { S_RZ = 0, S_DQ, S_ALPHA, S_BETA, S_RR, S_BB, S_TOL, S_MAXR, S_INV, S_ZS, S_ZSN, S_RPREV, S_PNEXT, S_DS, S_NSCAL = 16 };
 // ST_FLEX: flexible beta (mixed precision); ST_VC: a CG iteration has run (q is defined)
 // ST_ITOT: iterations of earlier rounds (restarts from the true residual)
 // ST_RESTART: this round restarts from the recomputed residual (see k_fin_init)
@@ -169,8 +171,8 @@ struct DSys {
     float* r;                     // residual, double-buffered: r_k lives in r (k even) or r1 (k odd),
     float* r1;                    // so a pass that updates it never overwrites what a neighbour reads
     float* q;
-    float* d0;                    // direction, double-buffered: d_k lives in d[k & 1]
-    float* d1;
+    void* d0;                     // direction, double-buffered: d_k lives in d[k & 1]; z's type, in
+    void* d1;                     // the mixed precisions scaled by S_DS
     void* z;                      // preconditioned residual, V-cycle vector type
     void* z2;                     // the pre-smoothed iterate of the fused passes (z's type): the fused
                                   // post-smoothing reads it in its halo rows while it writes z

@@ -464,6 +464,11 @@ __global__ void __launch_bounds__(TB) k_resid_init(const DSys* __restrict__ sys,
 // x += alpha d_{k-1} (gasphase.cpp:612), deferred to this pass which reads
 // d_{k-1} anyway.  d is double-buffered because neighbours read d_{k-1}
 // while d_k is written.
+// d is stored in z's type: FP32, or in the mixed precisions FP16 scaled by
+// the power of two of its z (S_DS once stored), so its values stay near an
+// RMS of 1.  q = A d and x += alpha d use d as stored, so x and r stay
+// consistent; only the direction itself is rounded, which the flexible beta
+// of the mixed precisions absorbs like the rounding of z.
 template <class OT, class ZT, int NT = TB, int HS = MARCH_HALO>
 __global__ void __launch_bounds__(NT) k_dir_apply(const DSys* __restrict__ sys, const int2* __restrict__ tab) {
     PHG_SYS_PROLOGUE
@@ -472,14 +477,21 @@ __global__ void __launch_bounds__(NT) k_dir_apply(const DSys* __restrict__ sys, 
     const int sxy = L.nx * L.ny;
     const int it = st_uni(s, ST_ITERS);
     const bool first = it == 0;
-    float* __restrict__ dnew = (it & 1) ? s.d1 : s.d0;
-    const float* __restrict__ dold = (it & 1) ? s.d0 : s.d1;
+    constexpr bool scaled = !std::is_same<ZT, float>::value;
+    ZT* __restrict__ dnew = static_cast<ZT*>((it & 1) ? s.d1 : s.d0);
+    const ZT* __restrict__ dold = static_cast<const ZT*>((it & 1) ? s.d0 : s.d1);
     const float beta = (float)s.scal[S_BETA], alpha = (float)s.scal[S_ALPHA];
     const void* z = s.z;
     const float izs = (float)(1.0 / s.scal[S_ZS]);   // z is stored scaled by S_ZS (a power of two)
-    // d = z + beta d_old (d = z on the first iteration, where d_old is not read)
-    auto raw = [&](unsigned n) { return float2{ld<ZT>(z, n), gld(dold, n)}; };
-    auto fin = [&](const float2& v) { return first ? v.x * izs : fmaf(beta, v.y, v.x * izs); };
+    const float ds = scaled ? (float)s.scal[S_ZS] : 1.0f, ids = scaled ? izs : 1.0f;   // d_k's scale
+    const float idso = scaled && !first ? (float)(1.0 / s.scal[S_DS]) : 1.0f;         // d_{k-1}'s
+    // d = z + beta d_old (d = z on the first iteration, where d_old is not read),
+    // as it reads back once stored
+    auto raw = [&](unsigned n) { return float2{ld<ZT>(z, n), ld<ZT>(dold, n)}; };
+    auto fin = [&](const float2& v) {
+        const float d = first ? v.x * izs : fmaf(beta, v.y * idso, v.x * izs);
+        return scaled ? zq<ZT>(d * ds) * ids : d;
+    };
     extern __shared__ float lds[];
     double acc = 0.0;
     struct Op {
@@ -490,7 +502,7 @@ __global__ void __launch_bounds__(NT) k_dir_apply(const DSys* __restrict__ sys, 
         Op o;
         o.cf.load(L, L.ax32, L.ay32, L.az32, q);
         o.x = gld(s.x, q.c);
-        o.dold = gld(dold, q.c);
+        o.dold = ld<ZT>(dold, q.c) * idso;
         return o;
     };
     march<NT, HS>(s, t, lds, raw, [](auto&) {}, fin, pre, [&](int k, float vp, float dc, float vn, const float* p, const Op& o) {
@@ -499,7 +511,7 @@ __global__ void __launch_bounds__(NT) k_dir_apply(const DSys* __restrict__ sys, 
         float y = 0.0f;
         if (dg > 0) y = dg * dc - cell_lat(o.cf, q, L.nx, L.ny, L.nz, p, vp, vn);
         if (!first) gst_nt(s.x, q.c, fmaf(alpha, o.dold, o.x));
-        gst_nt(dnew, q.c, dc);
+        st_nt<ZT>(dnew, q.c, dc * ds);   // exact: dc * ds is a ZT value
         gst_nt(s.q, q.c, y);
         acc += (double)y * dc;
     });
@@ -522,6 +534,7 @@ __global__ void __launch_bounds__(NT) k_dir_apply(const DSys* __restrict__ sys, 
 }
 
 // after the loop: the last x += alpha d of systems that stopped (ST_XPEND)
+template <class ZT>
 __global__ void __launch_bounds__(TB) k_x_flush(const DSys* __restrict__ sys, const int2* __restrict__ tab) {
     const int2 e = tab[blockIdx.x];
     const DSys& s = sys[e.x];
@@ -530,11 +543,12 @@ __global__ void __launch_bounds__(TB) k_x_flush(const DSys* __restrict__ sys, co
     if (!t.valid) return;
     const int sxy = s.L[0].nx * s.L[0].ny;
     const int it = st_uni(s, ST_ITERS);
-    const float* __restrict__ dk = ((it - 1) & 1) ? s.d1 : s.d0;
+    const void* dk = ((it - 1) & 1) ? s.d1 : s.d0;
+    const float ids = std::is_same<ZT, float>::value ? 1.0f : (float)(1.0 / s.scal[S_DS]);
     const float alpha = (float)s.scal[S_ALPHA];
     for (int k = t.k0; k < t.k1; ++k) {
         const int c = t.c2 + sxy * k;
-        s.x[c] = fmaf(alpha, dk[c], s.x[c]);
+        s.x[c] = fmaf(alpha, ld<ZT>(dk, c) * ids, s.x[c]);
     }
 }
 
@@ -653,6 +667,7 @@ __global__ void __launch_bounds__(TB) k_fin_alpha(const DSys* __restrict__ sys, 
     if (threadIdx.x == 0) {
         const double dq = res[0];
         s.scal[S_DQ] = dq;
+        s.scal[S_DS] = s.scal[S_ZS];   // the scale k_dir_apply stored d with
         if (!(dq > 0)) {
             s.state[ST_STATUS] = 3;
             s.state[ST_ACTIVE] = 0;

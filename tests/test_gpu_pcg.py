@@ -200,8 +200,9 @@ def test_batch_splits_smoothing_per_system(ctx, prec, width):
 def test_coarse_levels_in_lds_equal_split_launches(ctx, prec):
     """The coarse levels that fit in one block's LDS run as one down and one
     up launch per system (k_vc_down / k_vc_up; forced here, by default only for
-    batches of at least two such blocks per CU); every system's solve is bit
-    for bit the one whose coarse levels each take their own launches.  The
+    batches of at least two such blocks per CU, so not for this one); every
+    system's solve is bit for bit the one whose coarse levels each take their
+    own launches.  The
     mix holds the 80x50x40 house twice (its LDS levels start at level 1 in
     FP16 and BF16, at level 2 in FP32), batch scenarios of four sizes (three
     or four levels) and, in FP16, the 160x100x80 house (LDS levels from
@@ -215,8 +216,10 @@ def test_coarse_levels_in_lds_equal_split_launches(ctx, prec):
     bs = [sc.rhs_from_divergence(g, sc.divergence_field(g, 700 + s), DT) for s, g in enumerate(gs)]
     p = gas.solve_params(tol=1e-6, max_iters=200, zero_x=True)
     res = []
-    for split in (False, True):
-        B = gas.Batch(ctx, Gs, coarse_lds=1 if split else 2)
+    # 1: per-level launches; 2: LDS levels; 0 (default): per-level launches for
+    # a batch this small
+    for mode in (1, 2, 0):
+        B = gas.Batch(ctx, Gs, coarse_lds=mode)
         dbs = [ctx.to_device(b) for b in bs]
         dxs = [ctx.array(g.n) for g in gs]
         ctx.profile(True, classes=["vc_down", "vc_up"])
@@ -227,11 +230,12 @@ def test_coarse_levels_in_lds_equal_split_launches(ctx, prec):
         assert all(s["status"] == gas.CONVERGED for s in stats), stats
         res.append(([d.download() for d in dxs], [s["iters"] for s in stats]))
         # the LDS launches run exactly when the batch asks for them
-        assert (prof["vc_down"][1] > 0) == (not split) and (prof["vc_up"][1] > 0) == (not split)
+        assert (prof["vc_down"][1] > 0) == (mode == 2) and (prof["vc_up"][1] > 0) == (mode == 2)
         B.destroy()
-    assert res[0][1] == res[1][1]
-    for k, (a, b) in enumerate(zip(res[0][0], res[1][0])):
-        assert np.array_equal(a, b), f"system {k}"
+    for other in res[1:]:
+        assert other[1] == res[0][1]
+        for k, (a, b) in enumerate(zip(res[0][0], other[0])):
+            assert np.array_equal(a, b), f"system {k}"
     for G in Gs[1:]:
         G.destroy()
 
