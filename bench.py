@@ -69,21 +69,24 @@ def algorithmic_bytes(name, iters, n0, n1, vt, flex, zt=4):
     """Algorithmic HBM bytes of all launches of a level-0 kernel class for a
     batch of systems of one grid that stopped after `iters` iterations each:
     one pass over every array the kernel touches.  Outer CG vectors
-    (x r q d) are FP32; the smoother iterate z (and the fused passes' z2) is
-    `zt` bytes (FP32, or FP16 in the mixed precisions); the V-cycle residual
-    scratch and level-1 correction are `vt` bytes; a half sweep writes half
-    of z.  Per launch, the grid's 0/1 face mask (1 B/cell) and, for the
-    prolongation, its level-0 aggregate map (4 B/cell) are read once, being
-    shared by every system of the batch.  One solve with k iterations runs
-    dir_apply k times (the first from d = z alone) and every V-cycle kernel
-    k + 1 times (the first with r unchanged)."""
+    (x r q) are FP32; the smoother iterate z (and the fused passes' z2) and
+    the direction d are `zt` bytes (FP32, or FP16 in the mixed precisions);
+    the V-cycle residual scratch and level-1 correction are `vt` bytes; a
+    half sweep writes half of z.  Per launch, the grid's 0/1 face mask
+    (1 B/cell) and, for the prolongation, its level-0 aggregate map (4 B/cell)
+    are read once, being shared by every system of the batch.  One solve with
+    k iterations runs dir_apply k times (the first from d = z alone; x is
+    read and written on the even ones from the third on, with the direction
+    two back) and every V-cycle kernel k + 1 times (the first with r
+    unchanged)."""
     it = np.asarray(iters, np.int64).ravel()
     nvc = int((it + 1).sum())                        # system-launches of a V-cycle kernel
-    nd = int(np.maximum(it, 0).sum())                # system-launches of dir_apply
     nfirst = int((it > 0).sum())
+    nodd = int((np.maximum(it, 0) // 2).sum())       # iterations 1, 3, 5, ...
+    neven = int((np.maximum(it - 1, 0) // 2).sum())  # iterations 2, 4, ...
     half = zt / 2
-    if name == "dir_apply":                          # read z d_old x, write x d_new q; first: read z, write d q
-        per_sys = nfirst * (8 + zt) * n0 + (nd - nfirst) * (20 + zt) * n0
+    if name == "dir_apply":   # first: read z, write d q; odd: + read d_old; even: + read x d_{k-2}, write x
+        per_sys = (nfirst * (4 + 2 * zt) + nodd * (4 + 3 * zt) + neven * (12 + 4 * zt)) * n0
     elif name == "pre_smooth":                       # read r q, write r z2 r0; first: read r, write z2 r0
         per_sys = len(it) * (4 + zt + vt) * n0 + (nvc - len(it)) * (12 + zt + vt) * n0
     elif name == "pre_red":                          # read r q, write r z; first: read r, write z

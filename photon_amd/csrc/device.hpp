@@ -141,8 +141,12 @@ struct DLevel {
 // S_PNEXT: the ||r||^2 / ||b||^2 below which the residual's closed-zone means
 // are removed next (see k_proj_r)
 // S_DS: the power-of-two scale the last direction d is stored with (the S_ZS of
-// its z; in FP32 d is stored unscaled)
-enum { S_RZ = 0, S_DQ, S_ALPHA, S_BETA, S_RR, S_BB, S_TOL, S_MAXR, S_INV, S_ZS, S_ZSN, S_RPREV, S_PNEXT, S_DS, S_NSCAL = 16 };
+// its z; in FP32 d is stored unscaled); S_ALPHA2, S_DS2: alpha and the scale of
+// the direction before it
+enum { S_RZ = 0, S_DQ, S_ALPHA, S_BETA, S_RR, S_BB, S_TOL, S_MAXR, S_INV, S_ZS, S_ZSN, S_RPREV, S_PNEXT, S_DS, S_ALPHA2, S_DS2,
+       S_NSCAL = 16 };
+// ST_XPEND: x += alpha d updates still owed by a stopped system (0, 1 or 2:
+// k_dir_apply applies them in pairs, on even iterations)
 // ST_FLEX: flexible beta (mixed precision); ST_VC: a CG iteration has run (q is defined)
 // ST_ITOT: iterations of earlier rounds (restarts from the true residual)
 // ST_RESTART: this round restarts from the recomputed residual (see k_fin_init)

@@ -460,10 +460,13 @@ __global__ void __launch_bounds__(TB) k_resid_init(const DSys* __restrict__ sys,
 }
 
 // d_k = z + beta d_{k-1} (d_0 = z), q = A d_k with the partial d.q (slot 0)
-// (gasphase.cpp:620-622 and the apply of :606), and the previous iteration's
-// x += alpha d_{k-1} (gasphase.cpp:612), deferred to this pass which reads
-// d_{k-1} anyway.  d is double-buffered because neighbours read d_{k-1}
-// while d_k is written.
+// (gasphase.cpp:620-622 and the apply of :606), and on even k >= 2 the two
+// previous iterations' x += alpha d (gasphase.cpp:612), deferred to this pass:
+// x = (x + alpha_{k-2} d_{k-2}) + alpha_{k-1} d_{k-1}, the same two FMAs in the
+// same order as one update per iteration, with x read and written every
+// second iteration only.  d is double-buffered because neighbours read
+// d_{k-1} while d_k is written; d_{k-2}, in the buffer d_k goes to, is read
+// at the cell before d_k is stored there.
 // d is stored in z's type: FP32, or in the mixed precisions FP16 scaled by
 // the power of two of its z (S_DS once stored), so its values stay near an
 // RMS of 1.  q = A d and x += alpha d use d as stored, so x and r stay
@@ -480,11 +483,13 @@ __global__ void __launch_bounds__(NT) k_dir_apply(const DSys* __restrict__ sys, 
     constexpr bool scaled = !std::is_same<ZT, float>::value;
     ZT* __restrict__ dnew = static_cast<ZT*>((it & 1) ? s.d1 : s.d0);
     const ZT* __restrict__ dold = static_cast<const ZT*>((it & 1) ? s.d0 : s.d1);
-    const float beta = (float)s.scal[S_BETA], alpha = (float)s.scal[S_ALPHA];
+    const bool xup = it >= 2 && !(it & 1);   // the two owed x updates are applied here
+    const float beta = (float)s.scal[S_BETA], alpha = (float)s.scal[S_ALPHA], alpha2 = (float)s.scal[S_ALPHA2];
     const void* z = s.z;
     const float izs = (float)(1.0 / s.scal[S_ZS]);   // z is stored scaled by S_ZS (a power of two)
     const float ds = scaled ? (float)s.scal[S_ZS] : 1.0f, ids = scaled ? izs : 1.0f;   // d_k's scale
     const float idso = scaled && !first ? (float)(1.0 / s.scal[S_DS]) : 1.0f;         // d_{k-1}'s
+    const float idso2 = scaled && xup ? (float)(1.0 / s.scal[S_DS2]) : 1.0f;          // d_{k-2}'s
     // d = z + beta d_old (d = z on the first iteration, where d_old is not read),
     // as it reads back once stored
     auto raw = [&](unsigned n) { return float2{ld<ZT>(z, n), ld<ZT>(dold, n)}; };
@@ -496,13 +501,16 @@ __global__ void __launch_bounds__(NT) k_dir_apply(const DSys* __restrict__ sys, 
     double acc = 0.0;
     struct Op {
         CoefRegs<OT> cf;
-        float x, dold;
+        float x, dold, dold2;
     };
     auto pre = [&](const Cell& q) {
         Op o;
         o.cf.load(L, L.ax32, L.ay32, L.az32, q);
-        o.x = gld(s.x, q.c);
-        o.dold = ld<ZT>(dold, q.c) * idso;
+        if (xup) {
+            o.x = gld(s.x, q.c);
+            o.dold = ld<ZT>(dold, q.c) * idso;
+            o.dold2 = ld<ZT>(dnew, q.c) * idso2;
+        }
         return o;
     };
     march<NT, HS>(s, t, lds, raw, [](auto&) {}, fin, pre, [&](int k, float vp, float dc, float vn, const float* p, const Op& o) {
@@ -510,7 +518,7 @@ __global__ void __launch_bounds__(NT) k_dir_apply(const DSys* __restrict__ sys, 
         const float dg = cell_diag(o.cf);
         float y = 0.0f;
         if (dg > 0) y = dg * dc - cell_lat(o.cf, q, L.nx, L.ny, L.nz, p, vp, vn);
-        if (!first) gst_nt(s.x, q.c, fmaf(alpha, o.dold, o.x));
+        if (xup) gst_nt(s.x, q.c, fmaf(alpha, o.dold, fmaf(alpha2, o.dold2, o.x)));
         st_nt<ZT>(dnew, q.c, dc * ds);   // exact: dc * ds is a ZT value
         gst_nt(s.q, q.c, y);
         acc += (double)y * dc;
@@ -533,22 +541,30 @@ __global__ void __launch_bounds__(NT) k_dir_apply(const DSys* __restrict__ sys, 
     }
 }
 
-// after the loop: the last x += alpha d of systems that stopped (ST_XPEND)
+// after the loop: the x += alpha d updates systems that stopped still owe
+// (ST_XPEND: the last direction, d in buffer (it - 1) & 1 with S_ALPHA and
+// S_DS, and for 2 first the one before it, with S_ALPHA2 and S_DS2)
 template <class ZT>
 __global__ void __launch_bounds__(TB) k_x_flush(const DSys* __restrict__ sys, const int2* __restrict__ tab) {
     const int2 e = tab[blockIdx.x];
     const DSys& s = sys[e.x];
-    if (s.state[ST_XPEND] == 0) return;
+    const int owed = s.state[ST_XPEND];
+    if (owed == 0) return;
     const ColTile t = col_tile(s, e.y);
     if (!t.valid) return;
     const int sxy = s.L[0].nx * s.L[0].ny;
     const int it = st_uni(s, ST_ITERS);
     const void* dk = ((it - 1) & 1) ? s.d1 : s.d0;
-    const float ids = std::is_same<ZT, float>::value ? 1.0f : (float)(1.0 / s.scal[S_DS]);
-    const float alpha = (float)s.scal[S_ALPHA];
+    const void* dk2 = (it & 1) ? s.d1 : s.d0;
+    constexpr bool scaled = !std::is_same<ZT, float>::value;
+    const float ids = scaled ? (float)(1.0 / s.scal[S_DS]) : 1.0f;
+    const float ids2 = scaled && owed == 2 ? (float)(1.0 / s.scal[S_DS2]) : 1.0f;
+    const float alpha = (float)s.scal[S_ALPHA], alpha2 = (float)s.scal[S_ALPHA2];
     for (int k = t.k0; k < t.k1; ++k) {
         const int c = t.c2 + sxy * k;
-        s.x[c] = fmaf(alpha, ld<ZT>(dk, c) * ids, s.x[c]);
+        float x = s.x[c];
+        if (owed == 2) x = fmaf(alpha2, ld<ZT>(dk2, c) * ids2, x);
+        s.x[c] = fmaf(alpha, ld<ZT>(dk, c) * ids, x);
     }
 }
 
@@ -667,11 +683,15 @@ __global__ void __launch_bounds__(TB) k_fin_alpha(const DSys* __restrict__ sys, 
     if (threadIdx.x == 0) {
         const double dq = res[0];
         s.scal[S_DQ] = dq;
-        s.scal[S_DS] = s.scal[S_ZS];   // the scale k_dir_apply stored d with
         if (!(dq > 0)) {
             s.state[ST_STATUS] = 3;
             s.state[ST_ACTIVE] = 0;
+            // on odd iterations k_dir_apply left alpha_{k-1} d_{k-1} owed
+            s.state[ST_XPEND] = s.state[ST_ITERS] & 1;
         } else {
+            s.scal[S_ALPHA2] = s.scal[S_ALPHA];
+            s.scal[S_DS2] = s.scal[S_DS];
+            s.scal[S_DS] = s.scal[S_ZS];   // the scale k_dir_apply stored d with
             s.scal[S_ALPHA] = s.scal[S_RZ] / dq;
             s.scal[S_ZS] = s.scal[S_ZSN];   // z of the previous V-cycle has been consumed
             // k_proj: sums taken -> subtract this iteration; subtracted -> done
@@ -721,7 +741,9 @@ __global__ void __launch_bounds__(TB) k_fin_beta(const DSys* __restrict__ sys, c
         if (status) {
             s.state[ST_STATUS] = status;
             s.state[ST_ACTIVE] = 0;
-            s.state[ST_XPEND] = 1;   // this iteration's x += alpha d is still owed
+            // this iteration's x += alpha d is owed, and on odd iterations the
+            // one before it (k_dir_apply applies them in pairs)
+            s.state[ST_XPEND] = (it & 1) ? 1 : 2;
         }
     }
 }
