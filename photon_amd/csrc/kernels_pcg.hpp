@@ -1849,13 +1849,31 @@ __device__ __forceinline__ void ell_rows(const DLevel& L, const TailRows<>& R, c
     } while (p0 < K);
 }
 
+// two adjacent cells 2 i2, 2 i2 + 1 of a V-cycle vector in one load
+template <class VT> __device__ __forceinline__ float2 ld_pair(const void* p, unsigned i2);
+template <> __device__ __forceinline__ float2 ld_pair<float>(const void* p, unsigned i2) {
+    const f32x2 v = gld(static_cast<const f32x2*>(p), i2);
+    return float2{v.x, v.y};
+}
+template <> __device__ __forceinline__ float2 ld_pair<_Float16>(const void* p, unsigned i2) {
+    const f16x2 v = gld(static_cast<const f16x2*>(p), i2);
+    return float2{(float)v.x, (float)v.y};
+}
+template <> __device__ __forceinline__ float2 ld_pair<bf16_t>(const void* p, unsigned i2) {
+    const uint32_t v = gld(static_cast<const uint32_t*>(p), i2);
+    return float2{__uint_as_float(v << 16), __uint_as_float(v & 0xFFFF0000u)};
+}
+
 // restriction from level 0 into level 1 (k_vc_restrict) by the level-1
 // aggregates' 2x2x2 blocks: b of rows R, the children's r0 summed in
-// ascending cell order from 0 and scaled (the children list's order)
+// ascending cell order from 0 and scaled (the children list's order).  The
+// block words exist for grids of even nx only, where a block's corner cell is
+// even, so each of its four x-pairs is one aligned load (a pair without
+// children loads the corner's, in bounds).
 template <class VT>
 __device__ __forceinline__ void block_rows(const DLevel& L0, const TailRows<>& R, float (&b)[CT_U]) {
     const unsigned nx = (unsigned)L0.nx, sxy = nx * (unsigned)L0.ny;
-    const unsigned off[8] = {0u, 1u, nx, nx + 1u, sxy, sxy + 1u, sxy + nx, sxy + nx + 1u};
+    const unsigned off[4] = {0u, nx, sxy, sxy + nx};
     uint32_t w[CT_U];
 #pragma unroll
     for (int u = 0; u < CT_U; ++u) w[u] = gld(L0.cblk, (unsigned)R.I[u]);
@@ -1864,7 +1882,11 @@ __device__ __forceinline__ void block_rows(const DLevel& L0, const TailRows<>& R
     for (int u = 0; u < CT_U; ++u) {
         const unsigned base = w[u] & 0xFFFFFFu;
 #pragma unroll
-        for (int q = 0; q < 8; ++q) x[u][q] = ld<VT>(L0.r, (w[u] >> (24 + q)) & 1u ? base + off[q] : base);
+        for (int h = 0; h < 4; ++h) {
+            const float2 v = ld_pair<VT>(L0.r, ((w[u] >> (24 + 2 * h)) & 3u ? base + off[h] : base) >> 1);
+            x[u][2 * h] = v.x;
+            x[u][2 * h + 1] = v.y;
+        }
     }
 #pragma unroll
     for (int u = 0; u < CT_U; ++u) {

@@ -958,10 +958,12 @@ int build_grid_dev(phg_ctx* ctx, int nx, int ny, int nz, float dx, const float* 
                 rc = upload(ctx, G->allocs, &q, pk.data(), pk.size());
                 D.cv = q;
             }
-            if (l == 0 && nl > 1 && G->n < (1 << 24) && !rc) {
+            if (l == 0 && nl > 1 && G->n < (1 << 24) && nx % 2 == 0 && !rc) {
                 // each level-1 aggregate lies in one 2x2x2 block of cells: its
                 // corner cell | the mask of its children among the block's eight
-                // (bit dx | dy << 1 | dz << 2; ascending bits are ascending cells)
+                // (bit dx | dy << 1 | dz << 2; ascending bits are ascending cells);
+                // for even nx only, where every corner cell is even (the kernels
+                // read a block's x-pairs as aligned pairs)
                 const int n1 = G->H.L[1].n, sxy = nx * ny;
                 std::vector<uint32_t> cb(n1);
                 bool ok = true;
