@@ -393,7 +393,7 @@ def run_step_kernels(ctx, gas, sc, reps=20):
     return out
 
 
-def run_serving(ctx, window_s=2.0, steps=300):
+def run_serving(ctx, window_s=1.0, steps=3000):
     """The serving workload: Poisson arrivals at 20 scenarios/s over
     `window_s` seconds, sizes log-uniform 3.6k..1.28M cells (room aspect),
     10-500 kW burners, each scenario run for `steps` whole gas steps
@@ -402,10 +402,14 @@ def run_serving(ctx, window_s=2.0, steps=300):
     batching under a 64 GB footprint budget."""
     from photon_amd import serving as sv
     specs = sv.poisson_stream(7, rate_per_s=20.0, window_s=window_s, steps=steps)
-    out = sv.serve(ctx, specs, hbm_budget_bytes=64e9, max_active=64)
-    out["workload"] = (f"Poisson 20/s over {window_s} s ({len(specs)} scenarios), 3.6k..1.28M cells log-uniform, "
-                       f"{steps} whole gas steps each at dt <= 0.01 s ({steps / 100:g} s simulated of the config's 30 s, "
-                       "to keep the run near a minute), FP16-storage MG-PCG, 64 GB admission budget")
+    def progress(loops, job_steps, done):
+        print(f"serving: {loops} loop steps, {job_steps} job-steps, {done}/{len(specs)} jobs done", file=sys.stderr,
+              flush=True)
+
+    out = sv.serve(ctx, specs, progress=progress, hbm_budget_bytes=64e9, max_active=64)
+    out["workload"] = (f"Poisson 20/s over {window_s} s ({len(specs)} scenarios; the arrival window keeps the run near "
+                       f"a minute), 3.6k..1.28M cells log-uniform, {steps} whole gas steps each at dt <= 0.01 s "
+                       f"({steps / 100:g} s simulated), FP16-storage MG-PCG, 64 GB admission budget")
     return out
 
 
@@ -542,6 +546,9 @@ def main():
                     help="time only the gas-step kernels (for kernel traces under rocprofv3)")
     ap.add_argument("--batching-only", action="store_true",
                     help="run only the 64-scenario batching workload (for kernel traces under rocprofv3)")
+    ap.add_argument("--serving-only", action="store_true", help="run only the serving workload")
+    ap.add_argument("--serving-window", type=float, default=1.0, help="seconds of Poisson arrivals at 20/s")
+    ap.add_argument("--serving-steps", type=int, default=3000, help="whole gas steps per served scenario")
     ap.add_argument("--copy-gib", type=float, default=0.0,
                     help="also run the plain copy kernel over this many GiB (counter calibration)")
     args = ap.parse_args()
@@ -553,9 +560,12 @@ def main():
     from photon_amd import gas
     from photon_amd import scenarios as sc
 
-    if args.step_kernels_only or args.batching_only:
+    if args.step_kernels_only or args.batching_only or args.serving_only:
         ctx = gas.Context(0)
-        out = run_step_kernels(ctx, gas, sc) if args.step_kernels_only else run_batching(ctx, gas, sc)
+        if args.serving_only:
+            out = run_serving(ctx, args.serving_window, args.serving_steps)
+        else:
+            out = run_step_kernels(ctx, gas, sc) if args.step_kernels_only else run_batching(ctx, gas, sc)
         ctx.close()
         print(json.dumps(out))
         return
@@ -586,7 +596,7 @@ def main():
         v_plain = args.plain / plain["seconds"]
     precision = None if args.mixed_only else run_precision(ctx, gas, sc)
     batching = None if args.mixed_only else run_batching(ctx, gas, sc)
-    serving = None if args.mixed_only else run_serving(ctx)
+    serving = None if args.mixed_only else run_serving(ctx, args.serving_window, args.serving_steps)
     room = None if args.mixed_only else run_room(ctx, gas, sc)
     step_kernels = None if args.mixed_only else run_step_kernels(ctx, gas, sc)
 

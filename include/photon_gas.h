@@ -365,6 +365,25 @@ int phg_grid_update_faces(phg_ctx *ctx, int grid, int n, const int *axis, const 
  * shift of S. */
 int phg_zone_leakage(phg_ctx *ctx, int grid, float dt, const float *zoneP, const float *T, const float *Pb,
                      const float *p, float *u, float *v, float *w, float *S, float *zoneDP, int *leak_faces);
+/* Zone background pressures kept on the device.  The reference keeps
+ * Solver::zoneP / zoneDP_ in host vectors (solver.h:243,294) and the calls
+ * above take them from the host, which makes every step wait for the device
+ * twice; a caller that keeps a scenario's whole state on the device (the
+ * serving loop) keeps them in the grid instead, and these calls queue work
+ * without waiting.  phg_zone_pressures_set loads them (host, one per zone;
+ * rise rates reset to 0) and phg_zone_pressures_get reads them back (and the
+ * rise rates, 0 for open zones; zoneDP may be NULL), waiting.
+ * phg_zone_leakage_resident is phg_zone_leakage on them (the rise rates stay
+ * in the grid); phg_zone_pressure_resident is the end of Solver::step
+ * (solver.cpp:258-263): each closed zone's pressure follows its rise rate,
+ * clamped to [P0 - 20 kPa, P0 + 50 kPa], and Pb of its cells takes it.  The
+ * results equal the host path's bit for bit.  phg_zone_leakage (host
+ * pressures) and a face change that rebuilds the zones replace them. */
+int phg_zone_pressures_set(phg_ctx *ctx, int grid, const float *zoneP);
+int phg_zone_pressures_get(phg_ctx *ctx, int grid, float *zoneP, float *zoneDP);
+int phg_zone_leakage_resident(phg_ctx *ctx, int grid, float dt, const float *T, const float *Pb, const float *p,
+                              float *u, float *v, float *w, float *S);
+int phg_zone_pressure_resident(phg_ctx *ctx, int grid, float dt, float *Pb);
 
 /* ---- measurement aid: a plain device copy (16-byte vector loads/stores),
  * the calibration kernel for HBM byte counters; bytes must be a multiple of 16 */

@@ -167,6 +167,22 @@ __global__ void __launch_bounds__(TB) k_crack_zone_out(CrackDev cd, const int* _
     }
 }
 
+// the closed zones' background pressure follows its rise rate
+// (solver.cpp:258-263): zoneP[z] = clamp(zoneP[z] + dt zdp[k], P0 - 20 kPa,
+// P0 + 50 kPa), in the reference's float operations; val[k] takes the new
+// pressure for k_zone_apply
+__global__ void __launch_bounds__(TB) k_zone_pressure(int nclosed, const int* __restrict__ czone,
+                                                      const float* __restrict__ zdp, float dt,
+                                                      float* __restrict__ zoneP, float* __restrict__ val) {
+#pragma clang fp contract(off)
+    const int k = blockIdx.x * TB + threadIdx.x;
+    if (k >= nclosed) return;
+    const int z = czone[k];
+    const float v = fl_clamp(zoneP[z] + dt * zdp[k], 101325.0f - 20000.0f, 101325.0f + 50000.0f);
+    zoneP[z] = v;
+    val[k] = v;
+}
+
 // face velocities set to zero where a face update closes the face
 // (Solver::updateFace, solver.cpp:92); list: (axis, face)
 __global__ void __launch_bounds__(TB) k_face_zero(const int2* __restrict__ list, int n, float* __restrict__ u,

@@ -2448,18 +2448,12 @@ int phg_grid_update_faces(phg_ctx* ctx, int grid, int n, const int* axis, const 
     });
 }
 
-int phg_zone_leakage(phg_ctx* ctx, int grid, float dt, const float* zoneP, const float* T, const float* Pb,
-                     const float* p, float* u, float* v, float* w, float* S, float* zoneDP, int* leak_faces) {
-    if (!ctx || !zoneP || !T || !Pb || !p || !u || !v || !w || !S) return PHG_EINVAL;
-    GridDev* G = get_grid(ctx, grid);
-    if (!G) return fail(ctx, PHG_ESTATE, "unknown grid %d", grid);
-    if (!(dt > 0)) return fail(ctx, PHG_EINVAL, "dt must be positive");
+// the leakage launches on the zone pressures in G->zp; the closed zones' rise
+// rates are left in G->zdp
+static int leakage_launch(phg_ctx* ctx, GridDev* G, float dt, const float* T, const float* Pb, const float* p, float* u,
+                          float* v, float* w, float* S) {
     const size_t nzn = G->zone_open.size();
     const CrackDev& cd = G->crack;
-    if (leak_faces) *leak_faces = cd.ncr;
-    if (zoneDP) std::fill(zoneDP, zoneDP + nzn, 0.0f);
-    if (nzn == 0 || (G->nclosed == 0 && cd.ncr == 0)) return PHG_OK;   // nothing to shift, no crack flow
-    HIP_OK(hipMemcpyAsync(G->zp, zoneP, nzn * sizeof(float), hipMemcpyHostToDevice, ctx->stream));
     const int nc = G->nclosed;
     double* dint = G->zsum + G->nztask;
     if (nc) {
@@ -2483,6 +2477,26 @@ int phg_zone_leakage(phg_ctx* ctx, int grid, float dt, const float* zoneP, const
         launch(ctx, K_LEAK, k_crack_zone_out, nc, 0, cd, (const int*)G->czone, (const float*)G->zvol,
                (const double*)dint, (const float*)G->zp, (const float*)G->crack_q, G->zdp, G->zval);
         launch(ctx, K_LEAK, k_zone_apply, tiles(G->n), 0, G->n, (const int*)G->zclosed, (const float*)G->zval, S, 1);
+    }
+    (void)nzn;
+    return PHG_OK;
+}
+
+int phg_zone_leakage(phg_ctx* ctx, int grid, float dt, const float* zoneP, const float* T, const float* Pb,
+                     const float* p, float* u, float* v, float* w, float* S, float* zoneDP, int* leak_faces) {
+    if (!ctx || !zoneP || !T || !Pb || !p || !u || !v || !w || !S) return PHG_EINVAL;
+    GridDev* G = get_grid(ctx, grid);
+    if (!G) return fail(ctx, PHG_ESTATE, "unknown grid %d", grid);
+    if (!(dt > 0)) return fail(ctx, PHG_EINVAL, "dt must be positive");
+    const size_t nzn = G->zone_open.size();
+    const CrackDev& cd = G->crack;
+    if (leak_faces) *leak_faces = cd.ncr;
+    if (zoneDP) std::fill(zoneDP, zoneDP + nzn, 0.0f);
+    if (nzn == 0 || (G->nclosed == 0 && cd.ncr == 0)) return PHG_OK;   // nothing to shift, no crack flow
+    HIP_OK(hipMemcpyAsync(G->zp, zoneP, nzn * sizeof(float), hipMemcpyHostToDevice, ctx->stream));
+    const int nc = G->nclosed;
+    if (int rc = leakage_launch(ctx, G, dt, T, Pb, p, u, v, w, S)) return rc;
+    if (nc) {
         std::vector<float> zd(nc);
         HIP_OK(hipMemcpyAsync(zd.data(), G->zdp, nc * sizeof(float), hipMemcpyDeviceToHost, ctx->stream));
         HIP_OK(hipStreamSynchronize(ctx->stream));
@@ -2493,6 +2507,58 @@ int phg_zone_leakage(phg_ctx* ctx, int grid, float dt, const float* zoneP, const
         // zoneP's staging copy must land before the caller may reuse it
         HIP_OK(hipStreamSynchronize(ctx->stream));
     }
+    HIP_OK(hipGetLastError());
+    return PHG_OK;
+}
+
+int phg_zone_pressures_set(phg_ctx* ctx, int grid, const float* zoneP) {
+    if (!ctx || !zoneP) return PHG_EINVAL;
+    GridDev* G = get_grid(ctx, grid);
+    if (!G) return fail(ctx, PHG_ESTATE, "unknown grid %d", grid);
+    const size_t nzn = G->zone_open.size();
+    if (nzn == 0) return PHG_OK;
+    HIP_OK(hipMemcpyAsync(G->zp, zoneP, nzn * sizeof(float), hipMemcpyHostToDevice, ctx->stream));
+    HIP_OK(hipMemsetAsync(G->zdp, 0, ((size_t)G->nclosed + 1) * sizeof(float), ctx->stream));
+    HIP_OK(hipStreamSynchronize(ctx->stream));   // the caller's array may go at once
+    return PHG_OK;
+}
+
+int phg_zone_pressures_get(phg_ctx* ctx, int grid, float* zoneP, float* zoneDP) {
+    if (!ctx || !zoneP) return PHG_EINVAL;
+    GridDev* G = get_grid(ctx, grid);
+    if (!G) return fail(ctx, PHG_ESTATE, "unknown grid %d", grid);
+    const size_t nzn = G->zone_open.size();
+    if (nzn == 0) return PHG_OK;
+    std::vector<float> zd((size_t)G->nclosed + 1);
+    HIP_OK(hipMemcpyAsync(zoneP, G->zp, nzn * sizeof(float), hipMemcpyDeviceToHost, ctx->stream));
+    HIP_OK(hipMemcpyAsync(zd.data(), G->zdp, zd.size() * sizeof(float), hipMemcpyDeviceToHost, ctx->stream));
+    HIP_OK(hipStreamSynchronize(ctx->stream));
+    if (zoneDP)
+        for (size_t k = 0, z = 0; z < nzn; ++z) zoneDP[z] = G->zone_open[z] ? 0.0f : zd[k++];
+    return PHG_OK;
+}
+
+int phg_zone_leakage_resident(phg_ctx* ctx, int grid, float dt, const float* T, const float* Pb, const float* p,
+                              float* u, float* v, float* w, float* S) {
+    if (!ctx || !T || !Pb || !p || !u || !v || !w || !S) return PHG_EINVAL;
+    GridDev* G = get_grid(ctx, grid);
+    if (!G) return fail(ctx, PHG_ESTATE, "unknown grid %d", grid);
+    if (!(dt > 0)) return fail(ctx, PHG_EINVAL, "dt must be positive");
+    if (G->zone_open.empty() || (G->nclosed == 0 && G->crack.ncr == 0)) return PHG_OK;
+    if (int rc = leakage_launch(ctx, G, dt, T, Pb, p, u, v, w, S)) return rc;
+    HIP_OK(hipGetLastError());
+    return PHG_OK;
+}
+
+int phg_zone_pressure_resident(phg_ctx* ctx, int grid, float dt, float* Pb) {
+    if (!ctx || !Pb) return PHG_EINVAL;
+    GridDev* G = get_grid(ctx, grid);
+    if (!G) return fail(ctx, PHG_ESTATE, "unknown grid %d", grid);
+    const int nc = G->nclosed;
+    if (nc == 0) return PHG_OK;
+    launch(ctx, K_LEAK, k_zone_pressure, (nc + TB - 1) / TB, 0, nc, (const int*)G->czone, (const float*)G->zdp, dt,
+           G->zp, G->zval);
+    launch(ctx, K_ZONE, k_zone_apply, tiles(G->n), 0, G->n, (const int*)G->zclosed, (const float*)G->zval, Pb, 0);
     HIP_OK(hipGetLastError());
     return PHG_OK;
 }

@@ -35,7 +35,8 @@ EXPORTS = (
     "phg_grid_set_coarse", "phg_grid_get_coarse",
     "phg_stable_dt", "phg_turbulence", "phg_heat_diffusion", "phg_momentum", "phg_step_clear", "phg_add_gas",
     "phg_combustion", "phg_zone_integrals", "phg_zone_values", "phg_grid_stats", "phg_batch_graph_check",
-    "phg_grid_set_cracks", "phg_zone_leakage", "phg_grid_update_faces",
+    "phg_grid_set_cracks", "phg_zone_leakage", "phg_grid_update_faces", "phg_zone_pressures_set",
+    "phg_zone_pressures_get", "phg_zone_leakage_resident", "phg_zone_pressure_resident",
 )
 
 
@@ -131,6 +132,10 @@ def load_library():
     L.phg_leakage.argtypes = [vp, i32, vp, vp]
     L.phg_grid_set_cracks.argtypes = [vp, i32, vp, vp, vp]
     L.phg_zone_leakage.argtypes = [vp, i32, f32] + [vp] * 9 + [vp]
+    L.phg_zone_pressures_set.argtypes = [vp, i32, vp]
+    L.phg_zone_pressures_get.argtypes = [vp, i32, vp, vp]
+    L.phg_zone_leakage_resident.argtypes = [vp, i32, f32] + [vp] * 7
+    L.phg_zone_pressure_resident.argtypes = [vp, i32, f32, vp]
     L.phg_grid_update_faces.argtypes = [vp, i32, i32] + [vp] * 9
     L.phg_correct.argtypes = [vp, i32, f32, vp, vp, vp, vp, C.POINTER(f32)]
     L.phg_energy_expansion.argtypes = [vp, i32, f32, vp, vp, vp, vp, vp, C.POINTER(f32)]
@@ -515,14 +520,19 @@ class GasSolver:
     iterations as one captured graph covering the previous step's iteration
     count rounded up to a multiple of 4: consecutive steps' solves need
     similar counts, so the host waits about once per solve.  Replays give
-    the same results bit for bit as direct launches."""
+    the same results bit for bit as direct launches.
+
+    With `resident_zones`, the zones' background pressures and rise rates stay
+    on the device (phg_zone_*_resident), so a step queues its work without
+    waiting for the device; zoneP and zoneDP on the host are refreshed by
+    zones_from_device() (snapshots do so themselves)."""
 
     CELL_FIELDS = ("p", "S", "Pb", "Q", "Qr", "hrr", "massSrc", "nu") + FIELD_NAMES
     BYTE_FIELDS = ("pilot", "lit")
 
     def __init__(self, ctx: Context, scen, precision=PREC_FP32, wind_closed=None, tol=1e-6, max_iters=400,
                  stop_rule=STOP_REL_L2, plan: GridPlan | None = None, graph_steps=True, coarse=None, wind=None,
-                 radiation_on=False, dt_max_profile=0.05, cfl=0.8):
+                 radiation_on=False, dt_max_profile=0.05, cfl=0.8, resident_zones=False):
         self.ctx = ctx
         self.scen = scen
         self.graph_steps = graph_steps
@@ -558,6 +568,8 @@ class GasSolver:
         cnt = np.bincount(zone[zone >= 0], minlength=len(zopen)).astype(np.float64)
         dxd = np.float64(np.float32(scen.dx))
         self.zone_vol = (cnt * dxd * dxd * dxd).astype(np.float32)
+        self.resident_zones = resident_zones
+        self._zones_to_device()
         self._tmp = [ctx.array(n) for _ in range(10)]
         self.d["b"] = ctx.array(n)       # right-hand side of the split projection (rhs / correct)
         self.d["diag"] = ctx.array(n)
@@ -732,17 +744,40 @@ class GasSolver:
             dxd = np.float64(np.float32(self.scen.dx))
             self.zone_vol = (cnt * dxd * dxd * dxd).astype(np.float32)
             self.set("Pb", np.where(g, zp[np.maximum(zone, 0)], np.float32(101325.0)).astype(np.float32))
+            self._zones_to_device()
         return rebuilt
+
+    def _zones_to_device(self):
+        if self.resident_zones and len(self.zone_open):
+            zp = np.ascontiguousarray(self.zoneP, np.float32)
+            self.ctx._check(self.ctx.L.phg_zone_pressures_set(self.ctx.h, self.grid.id, zp.ctypes.data))
+
+    def zones_from_device(self):
+        """zoneP and zoneDP from the device (resident zones; waits)."""
+        if self.resident_zones and len(self.zone_open):
+            zp = np.zeros(len(self.zone_open), np.float32)
+            zd = np.zeros(len(self.zone_open), np.float32)
+            self.ctx._check(self.ctx.L.phg_zone_pressures_get(self.ctx.h, self.grid.id, zp.ctypes.data,
+                                                              zd.ctypes.data))
+            self.zoneP[:] = zp
+            self.zoneDP[:] = zd
+        return self.zoneP
 
     def leakage_zones(self, dt):
         """Solver::leakage (gasphase.cpp:402-498) on the device
         (phg_zone_leakage): crack flow, each closed zone's background pressure
-        root, the relaxed crack velocities, zoneDP and the shift of S."""
+        root, the relaxed crack velocities, zoneDP and the shift of S.  With
+        resident zones nothing comes back to the host (returns None)."""
+        d = self.d
+        if self.resident_zones:
+            self.ctx._check(self.ctx.L.phg_zone_leakage_resident(self.ctx.h, self.grid.id, dt, d["T"].ptr,
+                                                                 d["Pb"].ptr, d["p"].ptr, d["u"].ptr, d["v"].ptr,
+                                                                 d["w"].ptr, d["S"].ptr))
+            return None
         nz = len(self.zone_open)
         zdp = np.zeros(max(nz, 1), np.float32)
         zp = np.ascontiguousarray(self.zoneP, np.float32)
         nf = C.c_int()
-        d = self.d
         self.ctx._check(self.ctx.L.phg_zone_leakage(self.ctx.h, self.grid.id, dt, zp.ctypes.data, d["T"].ptr,
                                                     d["Pb"].ptr, d["p"].ptr, d["u"].ptr, d["v"].ptr, d["w"].ptr,
                                                     d["S"].ptr, zdp.ctypes.data, C.byref(nf)))
@@ -754,6 +789,9 @@ class GasSolver:
         """Closed zones' background pressure follows its rise rate (solver.cpp:258-263)."""
         f32 = np.float32
         if not len(self.zone_open) or self.zone_open.all():
+            return
+        if self.resident_zones:
+            self.ctx._check(self.ctx.L.phg_zone_pressure_resident(self.ctx.h, self.grid.id, dt, self.d["Pb"].ptr))
             return
         for z in range(len(self.zone_open)):
             if not self.zone_open[z]:
@@ -783,6 +821,7 @@ class GasSolver:
 
     def snapshot(self):
         """The full state as host arrays (a dict)."""
+        self.zones_from_device()
         st = {"magic": self.STATE_MAGIC, "fingerprint": self.fingerprint(), "time": np.float32(self.time),
               "zoneP": self.zoneP.copy(),
               "stats": np.array([self.maxVel, self.maxNu, self.maxDiv], np.float32),
@@ -804,6 +843,7 @@ class GasSolver:
             self.set(f, st["f_" + f])
         self.time = float(st["time"])
         self.zoneP[:] = st["zoneP"]
+        self._zones_to_device()
         self.maxVel, self.maxNu, self.maxDiv = (float(x) for x in st["stats"])
         self.pressureIters = int(st["pressureIters"])
         self.burners = []

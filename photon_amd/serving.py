@@ -12,8 +12,10 @@ scenarios:
 - a request's host set-up (geometry, multigrid hierarchy with its dense
   coarsest inverse, zones: gas.GridPlan) starts on a worker thread when it
   arrives, while the device steps the admitted scenarios;
-- every scenario keeps its whole gas state resident on the device.  A loop
-  step advances every running scenario by one Solver::step
+- every scenario keeps its whole gas state resident on the device, its zone
+  background pressures included, so a step queues its work without waiting
+  for the device until the solve.  A loop step advances every running
+  scenario by one Solver::step
   (src/sim/solver.cpp:226-269) at its own stable dt: burners, combustion,
   eddy viscosity, conduction, energy and expansion, momentum, the closed
   zones' pressure rise and source shift, the right-hand side, then ONE batched
@@ -218,7 +220,7 @@ class Server:
         return g, gas.GridPlan(g)
 
     def _new_solver(self, g, plan, spec):
-        s = gas.GasSolver(self.ctx, g, precision=self.precision, plan=plan, graph_steps=False)
+        s = gas.GasSolver(self.ctx, g, precision=self.precision, plan=plan, graph_steps=False, resident_zones=True)
         x0, y0, x1, y1 = burner_rect(g, spec)
         s.add_burner(x0, y0, x1, y1, 0.0, spec.hrr_kw * 1e3)
         return s
@@ -381,10 +383,11 @@ def poisson_stream(seed: int, rate_per_s: float, window_s: float, steps: int, ce
     return out
 
 
-def serve(ctx, specs: list[JobSpec], **server_kw) -> dict:
+def serve(ctx, specs: list[JobSpec], progress=None, **server_kw) -> dict:
     """Run the stream to completion against the wall clock: a request joins
     the queue when its arrival time has passed.  Reports throughput (job-steps
-    per second) and per-step latency percentiles."""
+    per second) and per-step latency percentiles.  progress(loop steps,
+    job-steps, jobs finished), if given, is called every 500 loop steps."""
     srv = Server(ctx, **server_kw)
     pending = sorted(specs, key=lambda s: s.arrival_s)
     t0 = time.perf_counter()
@@ -407,6 +410,8 @@ def serve(ctx, specs: list[JobSpec], **server_kw) -> dict:
             iters_all.append(j.iters)
             states[j.state] += 1
             srv.release(j)
+        if progress is not None and srv.steps and srv.steps % 500 == 0:
+            progress(srv.steps, srv.steps_run, sum(states.values()))
     wall = time.perf_counter() - t0
     its = np.concatenate([np.asarray(v, np.int64) for v in iters_all]) if iters_all else np.zeros(1, np.int64)
     lat = np.array(srv.step_latencies) if srv.step_latencies else np.zeros(1)
