@@ -547,6 +547,8 @@ def main():
     ap.add_argument("--batching-only", action="store_true",
                     help="run only the 64-scenario batching workload (for kernel traces under rocprofv3)")
     ap.add_argument("--serving-only", action="store_true", help="run only the serving workload")
+    ap.add_argument("--precision-only", action="store_true",
+                    help="run only the 1.28M-cell precision workload (for kernel traces under rocprofv3)")
     ap.add_argument("--serving-window", type=float, default=1.0, help="seconds of Poisson arrivals at 20/s")
     ap.add_argument("--serving-steps", type=int, default=3000, help="whole gas steps per served scenario")
     ap.add_argument("--copy-gib", type=float, default=0.0,
@@ -560,10 +562,12 @@ def main():
     from photon_amd import gas
     from photon_amd import scenarios as sc
 
-    if args.step_kernels_only or args.batching_only or args.serving_only:
+    if args.step_kernels_only or args.batching_only or args.serving_only or args.precision_only:
         ctx = gas.Context(0)
         if args.serving_only:
             out = run_serving(ctx, args.serving_window, args.serving_steps)
+        elif args.precision_only:
+            out = run_precision(ctx, gas, sc)
         else:
             out = run_step_kernels(ctx, gas, sc) if args.step_kernels_only else run_batching(ctx, gas, sc)
         ctx.close()
