@@ -179,7 +179,9 @@ int phg_batch_create(phg_ctx *ctx, int nsys, const int *grids, int *batch_out);
 /* tile shape of a batch (chosen per size class by the autotuner) */
 typedef struct {
     int kchunk;          /* level-0 planes per block along z (0 = from level0_blocks) */
-    int level0_blocks;   /* level-0 blocks per launch to aim for when kchunk = 0 (0 = 4096) */
+    int level0_blocks;   /* level-0 blocks per launch to aim for when kchunk = 0 (0 = 4096, with chunks
+                            of at least 32 planes when the batch's 256-column tiles alone number 512
+                            or more) */
     int unfused;         /* 1: level-0 smoothing as separate half sweeps instead of the two fused
                             passes (same results bit for bit; for comparison) */
     int sweeps0;         /* level-0 red-black sweeps before and after the coarse correction
