@@ -2197,6 +2197,127 @@ __global__ void __launch_bounds__(TB) k_vc_coarse(const DSys* __restrict__ sys, 
     }
 }
 
+// Coarsest level on the FP64 matrix cores for grids that many systems of the
+// batch share: t_j = B b_j for all of them is one GEMM (nc x nc times
+// nc x m).  A block takes CM_ROWS = 16 rows and 16 ST systems of one grid;
+// its four waves split the columns, wave w taking the w-th quarter of every
+// chunk of CM_KC.  For each group of four columns a wave issues ST
+// v_mfma_f64_16x16x4f64: lane l holds B[row l % 16][col l / 16] (B is
+// symmetric, so that is column row of the stored matrix, read coalesced) and
+// b_{l % 16}[col l / 16], and accumulates t of rows l / 16 + 4 r and system
+// l % 16 of the tile.  Wave 0 then adds the other waves' partial sums in wave
+// order.  The right-hand sides go through LDS CM_KC columns at a time in their
+// storage type, zero past nc, in two buffers: while a chunk's MFMAs run, the
+// next chunk's right-hand sides and matrix columns are already on their way
+// into registers, and each chunk costs one load round trip, so chunks are
+// long.  Small tiles give the chip several
+// waves per SIMD, which the MFMA chains need to issue back to back.  Each
+// output comes from its own row of B and column of b only, so a system's
+// result does not depend on the other systems of its block; it differs from
+// the vector kernel's column-ordered sums by FP64 rounding only.  The inverse
+// is read once per 16 ST systems instead of once per 16.
+constexpr int CM_ROWS = 16, CM_KC = 128;
+typedef double f64x4 __attribute__((ext_vector_type(4)));
+
+template <class VT, int ST>
+__global__ void __launch_bounds__(TB) k_vc_coarse_mfma(const DSys* __restrict__ sys, const int* __restrict__ states,
+                                                       const int4* __restrict__ tab) {
+    constexpr int NS = 16 * ST;
+    constexpr int BL = CM_KC * NS / TB;     // right-hand-side values per thread per chunk
+    constexpr int AW = CM_KC / 4 / 4;      // MFMA column groups per wave per chunk (a quarter of it)
+    static_assert(TB == 4 * 64, "four waves");
+    static_assert(CM_KC * NS % TB == 0 && TB % CM_KC == 0, "whole staging rounds");
+    const int4 e = tab[blockIdx.x];   // first system, count (<= NS), row tile
+    const unsigned long long live = live_of(states, e.x, e.y);
+    if (!live) return;
+    const DSys& s0 = sys[e.x];
+    const int n = s0.nc, top = s0.nl - 1;
+    const double* __restrict__ A = s0.cinv;
+    // two staging buffers, padded so the transposing store hits distinct
+    // banks; after the last chunk they hold the partial sums of waves 1..3
+    constexpr int SB = 2 * CM_KC * (NS + 1);
+    static_assert(SB * sizeof(VT) >= 3 * 64 * 4 * ST * sizeof(double), "partials fit the staging buffers");
+    __shared__ __align__(16) VT bsh[SB];
+    __shared__ const void* bp[NS];
+    __shared__ void* tp[NS];
+    const int tid = (int)threadIdx.x, lane = tid & 63, w = tid >> 6;
+    if (tid < NS) {
+        const DSys& sg = sys[e.x + min(tid, e.y - 1)];
+        bp[tid] = sg.L[top].b;
+        tp[tid] = sg.L[top].t;
+    }
+    __syncthreads();
+    const unsigned arow = (unsigned)min(e.z * CM_ROWS + (lane & 15), n - 1);
+    const int kq = lane >> 4;
+    // staging: thread tid takes column k0 + tid % CM_KC of systems
+    // tid / CM_KC + u TB / CM_KC, consecutive threads on consecutive columns
+    const int skk = tid % CM_KC, sj = tid / CM_KC;
+    // the raw values stay in their storage type until stash: converting them
+    // where they are loaded would wait for the loads before the MFMAs
+    VT bn[BL];
+    double an[AW];
+    auto fetch = [&](int k0) {
+        const unsigned k = (unsigned)min(k0 + skk, n - 1);
+#pragma unroll
+        for (int u = 0; u < BL; ++u) bn[u] = gld(static_cast<const VT*>(bp[sj + u * (TB / CM_KC)]), k);
+#pragma unroll
+        for (int u = 0; u < AW; ++u)
+            an[u] = gld(A, (unsigned)min(k0 + CM_KC / 4 * w + 4 * u + kq, n - 1) * (unsigned)n + arow);
+    };
+    const VT zero = from_f<VT>(0.0f);
+    auto stash = [&](int buf, int k0) {
+        const bool kin = k0 + skk < n;
+#pragma unroll
+        for (int u = 0; u < BL; ++u) {
+            const int j = sj + u * (TB / CM_KC);
+            bsh[(buf * CM_KC + skk) * (NS + 1) + j] = kin && j < e.y ? bn[u] : zero;
+        }
+    };
+    f64x4 acc[ST];
+#pragma unroll
+    for (int t = 0; t < ST; ++t) acc[t] = f64x4{0.0, 0.0, 0.0, 0.0};
+    fetch(0);
+    stash(0, 0);
+    __syncthreads();
+    for (int k0 = 0, buf = 0; k0 < n; k0 += CM_KC, buf ^= 1) {
+        double a[AW];
+#pragma unroll
+        for (int u = 0; u < AW; ++u) a[u] = an[u];
+        const bool more = k0 + CM_KC < n;
+        if (more) fetch(k0 + CM_KC);
+#pragma unroll
+        for (int u = 0; u < AW; ++u) {
+            const VT* brow = bsh + (buf * CM_KC + CM_KC / 4 * w + 4 * u + kq) * (NS + 1) + (lane & 15);
+#pragma unroll
+            for (int t = 0; t < ST; ++t)
+                acc[t] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[u], (double)to_f(brow[16 * t]), acc[t], 0, 0, 0);
+        }
+        if (more) stash(buf ^ 1, k0 + CM_KC);
+        __syncthreads();
+    }
+    double* part = reinterpret_cast<double*>(bsh);   // [3][4 ST][64]
+    if (w > 0) {
+#pragma unroll
+        for (int t = 0; t < ST; ++t)
+#pragma unroll
+            for (int r = 0; r < 4; ++r) part[((w - 1) * 4 * ST + 4 * t + r) * 64 + lane] = acc[t][r];
+    }
+    __syncthreads();
+    if (w > 0) return;
+#pragma unroll
+    for (int t = 0; t < ST; ++t) {
+        const int j = 16 * t + (lane & 15);
+        if (j >= e.y || !((live >> j) & 1u)) continue;
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+            const int i = e.z * CM_ROWS + kq + 4 * r;
+            const double v = acc[t][r] + part[(4 * t + r) * 64 + lane] + part[((4 * ST) + 4 * t + r) * 64 + lane] +
+                             part[((8 * ST) + 4 * t + r) * 64 + lane];
+            if (i < n) st<VT>(tp[j], (unsigned)i, (float)v);
+        }
+    }
+}
+
 // Coarsest level by the reference's own method (pressure_mg.cpp:258-281):
 // from x = 0, 20 times a forward then a backward Gauss-Seidel sweep,
 // x_I = (b_I + sum val x_col) / diag_I over the row's entries in order,

@@ -180,6 +180,8 @@ struct BatchDev {
     int tab_z_len = 0;
     int4* tab_coarse = nullptr;    // (first system, count, row tile): coarsest-solve blocks
     int tab_coarse_len = 0;
+    int4* tab_cm = nullptr;        // the same for the MFMA coarsest product (runs of >= kMfmaMinSys systems)
+    int tab_cm_len = 0;
     int cg_sys = 1;                // systems per coarsest-solve block (64 KB of LDS)
     int4* tab_cp = nullptr;        // packed inverses: (first system, count, strip I, strip J >= I)
     int tab_cp_len = 0;
@@ -399,6 +401,10 @@ constexpr int kColTilesFull = 512;
 constexpr int kMinChunk = 32;
 // whole-inverse coarsest-solve blocks below which its groups are made smaller
 constexpr int kCoarseBlocks = 256;
+// systems on one grid from which the coarsest product runs on the matrix cores,
+// and the 16-system tiles of an MFMA block
+constexpr int kMfmaMinSys = 32;
+constexpr int kMfmaTiles = 4;
 // most level-0 red-black sweeps per side of the V-cycle
 constexpr int kMaxSweeps0 = 4;
 
@@ -475,6 +481,9 @@ void vcycle_t(phg_ctx* ctx, BatchDev* B, int first) {
         launch(ctx, K_VC_COARSE, kern, B->tab_coarse_len, coarse_lds(cgs, B->maxnc), S, (const int*)B->state,
                (const int4*)B->tab_coarse);
     };
+    if (B->tab_cm_len > 0)
+        launch(ctx, K_VC_COARSE, k_vc_coarse_mfma<VT, kMfmaTiles>, B->tab_cm_len, 0, S, (const int*)B->state,
+               (const int4*)B->tab_cm);
     switch (B->cg_sys) {
         case 1: coarse(k_vc_coarse<VT, 1>, 1); break;
         case 4: coarse(k_vc_coarse<VT, 4>, 4); break;
@@ -1510,11 +1519,16 @@ int phg_batch_create_ex(phg_ctx* ctx, int nsys, const int* grids, const phg_batc
             // (systems whose grid takes the Gauss-Seidel sweeps are left out: one
             // block each, k_vc_coarse_gs)
             auto gs_of = [&](int s) { return get_grid(ctx, grids[s])->coarse == PHG_COARSE_GS20; };
+            // a run of at least kMfmaMinSys systems on one grid whose inverse is
+            // read whole takes the MFMA product (k_vc_coarse_mfma)
+            auto mfma_run = [&](int s, int cnt) {
+                return cnt >= kMfmaMinSys && !gs_of(s) && !get_grid(ctx, grids[s])->cnt;
+            };
             int maxrun = 1, maxrun_p = 1;
             for (int s = 0; s < nsys;) {
                 int cnt = 1;
                 while (s + cnt < nsys && grids[s + cnt] == grids[s]) ++cnt;
-                if (gs_of(s)) {
+                if (gs_of(s) || mfma_run(s, cnt)) {
                 } else if (get_grid(ctx, grids[s])->cnt) {
                     maxrun_p = std::max(maxrun_p, cnt);
                 } else {
@@ -1530,7 +1544,8 @@ int phg_batch_create_ex(phg_ctx* ctx, int nsys, const int* grids, const phg_batc
                     int cnt = 1;
                     while (s + cnt < nsys && grids[s + cnt] == grids[s]) ++cnt;
                     const GridDev* G = get_grid(ctx, grids[s]);
-                    if (!G->cnt && !gs_of(s)) nb += (long)((cnt + cgs - 1) / cgs) * ((G->H.nc + CG_ROWS - 1) / CG_ROWS);
+                    if (!G->cnt && !gs_of(s) && !mfma_run(s, cnt))
+                        nb += (long)((cnt + cgs - 1) / cgs) * ((G->H.nc + CG_ROWS - 1) / CG_ROWS);
                     s += cnt;
                 }
                 return nb;
@@ -1538,10 +1553,21 @@ int phg_batch_create_ex(phg_ctx* ctx, int nsys, const int* grids, const phg_batc
             while (B->cg_sys > 1 && coarse_blocks(B->cg_sys) < kCoarseBlocks)
                 B->cg_sys = B->cg_sys > CG_SYS_MAX / 2 ? CG_SYS_MAX / 2 : B->cg_sys > 4 ? 4 : 1;
             B->cp_sys = maxrun_p <= 1 ? 1 : maxrun_p <= 4 ? 4 : 8;
-            std::vector<int4> tp, tr;
+            std::vector<int4> tp, tr, tm;
             std::vector<int2> tg;
             for (int s = 0; s < nsys;) {
                 const GridDev* G = get_grid(ctx, grids[s]);
+                int run = 1;
+                while (s + run < nsys && grids[s + run] == grids[s]) ++run;
+                if (mfma_run(s, run)) {
+                    // row tile outer: after xcd_order each XCD reads its own
+                    // eighth of the inverse for every system group
+                    for (int t = 0; t < (G->H.nc + CM_ROWS - 1) / CM_ROWS; ++t)
+                        for (int g0 = 0; g0 < run; g0 += 16 * kMfmaTiles)
+                            tm.push_back(make_int4(s + g0, std::min(16 * kMfmaTiles, run - g0), t, 0));
+                    s += run;
+                    continue;
+                }
                 const int lim = G->cnt ? B->cp_sys : B->cg_sys;
                 int cnt = 1;
                 while (s + cnt < nsys && cnt < lim && grids[s + cnt] == grids[s]) ++cnt;
@@ -1558,6 +1584,9 @@ int phg_batch_create_ex(phg_ctx* ctx, int nsys, const int* grids, const phg_batc
                 s += cnt;
             }
             B->tab_coarse_len = (int)tc.size();
+            B->tab_cm_len = (int)tm.size();
+            tm = xcd_order(tm);
+            if (!rc && !tm.empty()) rc = upload(ctx, B->allocs, &B->tab_cm, tm.data(), tm.size());
             B->tab_cp_len = (int)tp.size();
             B->tab_cr_len = (int)tr.size();
             B->tab_gs_len = (int)tg.size();
