@@ -223,20 +223,41 @@ def run_plain(ctx, gas, grid_scen, bs_host, nsolve):
     return dict(seconds=t1 - t0, iters=np.array(iters))
 
 
+def level0_rates(r, n, n1, vt, zt, flex):
+    """Algorithmic GB/s of each level-0 kernel class over the timed steps of a
+    run_batch(profile=True) result (HIP events around each launch), and the
+    HBM roofline fraction of the one with the most time."""
+    prof = r["prof"]
+    gbs = {}
+    for k in LEVEL0:
+        if prof.get(k, (0, 0))[1] == 0:
+            continue
+        per_sys, shared = algorithmic_bytes(k, r["iters"], n, n1, vt, flex=flex, zt=zt)
+        gbs[k] = (per_sys + prof[k][1] * shared) / (prof[k][0] * 1e-3) / 1e9
+    dom = max(gbs, key=lambda k: prof[k][0])
+    return {"kernel": dom, "achieved_GBps": round(gbs[dom], 1), "frac": round(gbs[dom] * 1e9 / HBM_PEAK, 4),
+            "avg_launch_us": round(1e3 * prof[dom][0] / prof[dom][1], 2),
+            "level0_algorithmic_GBps": {k: round(v, 1) for k, v in gbs.items()}}
+
+
 def run_precision(ctx, gas, sc, nrhs=16):
     """The precision workload: the house refined to 0.1 m (160x100x80 =
     1.28M cells), `nrhs` right-hand sides as one batch, per storage
-    precision: solves/s and iterations (errors against the FP64 oracle are
-    measured by the GPU tests)."""
+    precision: solves/s, iterations, errors against the FP64 oracle on two
+    of the systems, and the level-0 kernels' algorithmic GB/s (their launches
+    timed with HIP events inside the timed step)."""
     g = sc.house(1, 160, 100, 80, 0.1)
     bs = [sc.rhs_from_divergence(g, sc.divergence_field(g, 5000 + s), DT) for s in range(nrhs)]
+    n1 = int(gas.hierarchy_shape(g)[0][1])
     out = {}
     sample = [0, nrhs - 1]
-    for name, prec in (("fp32", gas.PREC_FP32), ("mixed_f16", gas.PREC_MIXED_F16), ("mixed_bf16", gas.PREC_MIXED_BF16)):
-        r = run_batch(ctx, gas, g, prec, bs, 1, 1, profile=False, sample=sample)
+    for name, prec, vt, flex in (("fp32", gas.PREC_FP32, 4, False), ("mixed_f16", gas.PREC_MIXED_F16, 2, True),
+                                 ("mixed_bf16", gas.PREC_MIXED_BF16, 2, True)):
+        r = run_batch(ctx, gas, g, prec, bs, 1, 1, profile=True, sample=sample)
         out[name] = {"solves_per_s": round(nrhs / r["seconds"], 2), "iters_mean": float(r["iters"].mean()),
                      "iters_max": int(r["iters"].max()), "all_converged": bool(r["ok"])}
         out[name].update(fp64_errors(g, [bs[k] for k in sample], r["xs"], threads=len(sample)))
+        out[name]["level0_roofline"] = level0_rates(r, g.n, n1, vt, vt, flex)
     out["mixed_f16_vs_fp32"] = round(out["mixed_f16"]["solves_per_s"] / out["fp32"]["solves_per_s"], 3)
     out["workload"] = (f"house 160x100x80 cells at 0.1 m (seed 1), {nrhs} RHS as one batch, ||r||/||b|| <= 1e-6; "
                        f"errors against the FP64 oracle on RHS {sample}")
