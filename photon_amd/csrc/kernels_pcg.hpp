@@ -346,6 +346,46 @@ __device__ __forceinline__ float div_count(float x, float d) {
     return fmaf(fmaf(-q0, d, x), r, q0);
 }
 
+// Lookup tables of the fused passes, built in LDS by each block.  For a
+// six-bit face set (face mask and edge bits combined), the 0/1 weights of
+// lat_bits as bytes, faces 0-3 in .x and 4-5 in .y, which convert to floats
+// without a bit extract each; for a face-mask byte, its face count d and
+// RN(1/d) exactly as div_count forms them.  The weights, d and 1/d are the
+// same floats, and the sums run in the same order, so results are bitwise
+// those of lat_bits and div_count.
+struct FuseLut {
+    uint2 w[64];
+    float2 d[256];
+};
+__device__ __forceinline__ void fuse_lut_init(FuseLut& T, int tid, int nt) {
+    for (int q = tid; q < 64; q += nt) {
+        unsigned lo = 0, hi = 0;
+#pragma unroll
+        for (int f = 0; f < 4; ++f) lo |= ((q >> f) & 1u) << (8 * f);
+        hi = ((q >> 4) & 1u) | (((q >> 5) & 1u) << 8);
+        T.w[q] = make_uint2(lo, hi);
+    }
+    for (int q = tid; q < 256; q += nt) {
+        const float d = (float)__popc(q);
+        float r = 0.0f;
+        if (q != 0) {
+            const float r0 = __builtin_amdgcn_rcpf(d);
+            r = fmaf(r0, fmaf(-d, r0, 1.0f), r0);
+        }
+        T.d[q] = make_float2(d, r);
+    }
+}
+__device__ __forceinline__ float lat_lut(const FuseLut& T, unsigned b, const float* p, int nx, float vp, float vn) {
+    const uint2 w = T.w[b];
+    return (float)(w.x & 255u) * p[-1] + (float)((w.x >> 8) & 255u) * p[1] + (float)((w.x >> 16) & 255u) * p[-nx] +
+           (float)(w.x >> 24) * p[nx] + (float)(w.y & 255u) * vp + (float)(w.y >> 8) * vn;
+}
+// div_count(x, d) given r = RN(1/d)
+__device__ __forceinline__ float div_rn(float x, float d, float r) {
+    const float q0 = x * r;
+    return fmaf(fmaf(-q0, d, x), r, q0);
+}
+
 // a cell's diagonal, neighbour sum and division by the diagonal, from its
 // coefficient registers: the six values in general, the mask bits for a
 // 0/1 grid (same results, fewer instructions).  The division stays
@@ -1070,6 +1110,9 @@ __global__ void __launch_bounds__(NT) k_pre_fused(const DSys* __restrict__ sys, 
         const int j = c / nx, i = c - j * nx;
         return (i > 0 ? 1u : 0u) | (i < nx - 1 ? 2u : 0u) | (j > 0 ? 4u : 0u) | (j < ny - 1 ? 8u : 0u);
     };
+    __shared__ FuseLut lut;
+    fuse_lut_init(lut, tid, NT);
+    __syncthreads();
 
     // staging pairs (W2 indices 2u, 2u+1): clamped column of the first
     // cell, in-grid flags, colour (parity of i + j) of the first cell.  The
@@ -1147,9 +1190,9 @@ __global__ void __launch_bounds__(NT) k_pre_fused(const DSys* __restrict__ sys, 
             const bool oka = pin && ina[m], okb = pin && inb[m];
             const bool ared = ((par2[m] + p) & 1) == 0;   // the first cell is red
             const float rsa = rna * zs, rsb = rnb * zs;
-            const int dg = __popc(ared ? ma : mb);
+            const float2 D = lut.d[ared ? ma : mb];   // face count, its reciprocal
             // the red value from zero, as the separate red pass stores it in z
-            const float vr = dg > 0 ? zq<ZT>(div_count(ared ? rsa : rsb, (float)dg)) : 0.0f;
+            const float vr = D.x > 0 ? zq<ZT>(div_rn(ared ? rsa : rsb, D.x, D.y)) : 0.0f;
             *reinterpret_cast<float2*>(R + 2 * u) =
                 float2{oka ? (ared ? vr : rsa) : 0.0f, okb ? (ared ? rsb : vr) : 0.0f};
             *reinterpret_cast<uint16_t*>(M + 2 * u) = (uint16_t)((oka ? ma : 0u) | ((okb ? mb : 0u) << 8));
@@ -1175,12 +1218,12 @@ __global__ void __launch_bounds__(NT) k_pre_fused(const DSys* __restrict__ sys, 
             const int hb = ha1[m] + (ared ? 1 : 0);    // the black cell of the pair
             const float xm = R[hb - 1], xp = R[hb + 1];
             const unsigned mk = M[hb];
-            const int dg = __popc(mk);
+            const float2 D = lut.d[mk];
             float zb = 0.0f;
-            if (pin && (ared ? inb1[m] : ina1[m]) && dg > 0)
-                zb = zq<ZT>(div_count(R[hb] + lat_bits(mk & ((ared ? ijb[m] : ija[m]) | kb), R + hb, nx, Rm[hb],
-                                                       Rp[hb]),
-                                      (float)dg));
+            if (pin && (ared ? inb1[m] : ina1[m]) && D.x > 0)
+                zb = zq<ZT>(div_rn(R[hb] + lat_lut(lut, mk & ((ared ? ijb[m] : ija[m]) | kb), R + hb, nx, Rm[hb],
+                                                   Rp[hb]),
+                                   D.x, D.y));
             const float zr = pin && (ared ? ina1[m] : inb1[m]) ? (ared ? xm : xp) : 0.0f;
             *reinterpret_cast<float2*>(Z + ha1[m]) = ared ? float2{zr, zb} : float2{zb, zr};
         }
@@ -1205,13 +1248,12 @@ __global__ void __launch_bounds__(NT) k_pre_fused(const DSys* __restrict__ sys, 
         const float rn = o.rn;
         const float* Z = Zb + sp * SZ + hz;
         const float zc = *Z;
-        const CoefRegs<Mask01> cf{o.m};
-        const float dg = cell_diag(cf);
+        const float dg = lut.d[o.m].x;
         float r0 = 0.0f;
         if (dg > 0)
             r0 = rn * zs - dg * zc +
-                 cell_lat(cf, Cell{c, t.i, t.j, k}, nx, ny, nz, Z, Zb[((sp + 2) % 3) * SZ + hz],
-                          Zb[((sp + 1) % 3) * SZ + hz]);
+                 lat_lut(lut, o.m & edge_bits(t.i, t.j, k, nx, ny, nz), Z, nx, Zb[((sp + 2) % 3) * SZ + hz],
+                         Zb[((sp + 1) % 3) * SZ + hz]);
         st_nt<ZT>(s.z2, c, zc);   // read by the fused post-smoothing (exact: zc is a ZT value)
         st_nt<VT>(L.r, c, r0);
         if (!FIRST) {
@@ -1327,6 +1369,9 @@ __global__ void __launch_bounds__(NT) k_post_fused(const DSys* __restrict__ sys,
         const int j = c / nx, i = c - j * nx;
         return (i > 0 ? 1u : 0u) | (i < nx - 1 ? 2u : 0u) | (j > 0 ? 4u : 0u) | (j < ny - 1 ? 8u : 0u);
     };
+    __shared__ FuseLut lut;
+    fuse_lut_init(lut, tid, NT);
+    __syncthreads();
     // staging pairs (as in k_pre_fused)
     const int NP2 = W2 / 2;
     int ca[P2], par2[P2];
@@ -1437,12 +1482,12 @@ __global__ void __launch_bounds__(NT) k_post_fused(const DSys* __restrict__ sys,
             const bool ared = ((par1[m] + p) & 1) == 0;
             const int hb = ha1[m] + (ared ? 1 : 0);    // the black cell of the pair
             const unsigned mk = M[hb];
-            const int dg = __popc(mk);
+            const float2 D = lut.d[mk];   // face count, its reciprocal
             float zb = 0.0f;
-            if (pin && (ared ? inb1[m] : ina1[m]) && dg > 0)   // as the separate black pass stores it in z
-                zb = zq<ZT>(div_count(R[hb] + lat_bits(mk & ((ared ? ijb[m] : ija[m]) | kb), R + hb, nx, Rm[hb],
-                                                       Rp[hb]),
-                                      (float)dg));
+            if (pin && (ared ? inb1[m] : ina1[m]) && D.x > 0)   // as the separate black pass stores it in z
+                zb = zq<ZT>(div_rn(R[hb] + lat_lut(lut, mk & ((ared ? ijb[m] : ija[m]) | kb), R + hb, nx, Rm[hb],
+                                                   Rp[hb]),
+                                   D.x, D.y));
             *reinterpret_cast<float2*>(Z + ha1[m]) = ared ? float2{0.0f, zb} : float2{zb, 0.0f};
         }
     };
@@ -1466,14 +1511,12 @@ __global__ void __launch_bounds__(NT) k_post_fused(const DSys* __restrict__ sys,
         const float* Z = Zb + sp * SZ + hz;
         float zc = *Z;   // black cells: the new value
         if (((t.i + t.j + k) & 1) == 0) {
-            const CoefRegs<Mask01> cf{o.m};
-            const float dg = cell_diag(cf);
+            const float2 D = lut.d[o.m];
             zc = 0.0f;
-            if (dg > 0)
-                zc = zq<ZT>(cell_div<Mask01>(r * zs + cell_lat(cf, Cell{c, t.i, t.j, k}, nx, ny, nz, Z,
-                                                               Zb[((sp + 2) % 3) * SZ + hz],
-                                                               Zb[((sp + 1) % 3) * SZ + hz]),
-                                             dg));
+            if (D.x > 0)
+                zc = zq<ZT>(div_rn(r * zs + lat_lut(lut, o.m & edge_bits(t.i, t.j, k, nx, ny, nz), Z, nx,
+                                                    Zb[((sp + 2) % 3) * SZ + hz], Zb[((sp + 1) % 3) * SZ + hz]),
+                                   D.x, D.y));
         }
         st_nt<ZT>(s.z, c, zc);   // exact; the dots use the value as stored
         acc += (double)r * zc;
