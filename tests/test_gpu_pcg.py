@@ -4,8 +4,9 @@ Bars: the level-0 operator and dot product match the oracle to FP32 rounding;
 the V-cycle matches the oracle's V-cycle with the same (exact, FP64) coarsest
 solve to 1e-5; solutions at ||r||/||b|| <= 1e-6 match the FP64 oracle and the
 reference-method FP32 oracle within 1e-5 relative L2 (the north-star bar);
-batched results equal single-system results bit for bit; runs repeat bit for
-bit.
+batched results equal single-system results bit for bit (runs of 32+ systems
+on one grid, whose coarsest product runs on the FP64 matrix cores, to FP64
+rounding of it); runs repeat bit for bit.
 """
 import numpy as np
 import pytest
