@@ -53,7 +53,7 @@ enum KClass {
     K_RESID_INIT, K_DIR_APPLY, K_PRE_RED, K_RB_BLACK, K_RESID0, K_POST_BLACK, K_POST_RED, K_FIN, K_VC_RESTRICT,
     K_VC_JAC, K_VC_RESID, K_VC_PROLONG, K_VC_COARSE, K_XFLUSH, K_ZONE, K_RHS, K_CORRECT, K_ENERGY, K_ADVECT,
     K_COUNT, K_PRE_SMOOTH, K_POST_SMOOTH, K_TURB, K_HEAT, K_MOMENTUM, K_COMBUST, K_GASMISC, K_LEAK, K_VC_DOWN,
-    K_VC_UP, K_NCLASS
+    K_VC_UP, K_VC_COARSE_MFMA, K_NCLASS
 };
 static const char* kKName[K_NCLASS] = {"resid_init",  "dir_apply",  "pre_red",    "rb_black",  "resid0",
                                        "post_black",  "post_red",   "finalize",   "vc_restrict", "vc_jacobi",
@@ -61,7 +61,7 @@ static const char* kKName[K_NCLASS] = {"resid_init",  "dir_apply",  "pre_red",  
                                        "rhs",         "correct",    "energy",     "advect",    "count_active",
                                        "pre_smooth",  "post_smooth", "turbulence", "heat_diffusion",
                                        "momentum",    "combustion",  "gas_misc",   "leakage",   "vc_down",
-                                       "vc_up"};
+                                       "vc_up",       "vc_coarse_mfma"};
 
 struct GridDev {
     int nx = 0, ny = 0, nz = 0, n = 0;
@@ -482,7 +482,7 @@ void vcycle_t(phg_ctx* ctx, BatchDev* B, int first) {
                (const int4*)B->tab_coarse);
     };
     if (B->tab_cm_len > 0)
-        launch(ctx, K_VC_COARSE, k_vc_coarse_mfma<VT, kMfmaTiles>, B->tab_cm_len, 0, S, (const int*)B->state,
+        launch(ctx, K_VC_COARSE_MFMA, k_vc_coarse_mfma<VT, kMfmaTiles>, B->tab_cm_len, 0, S, (const int*)B->state,
                (const int4*)B->tab_cm);
     switch (B->cg_sys) {
         case 1: coarse(k_vc_coarse<VT, 1>, 1); break;

@@ -323,15 +323,18 @@ def test_precision_config_1p28M_cells(ctx):
         assert its[gas.PREC_MIXED_BF16] <= its[gas.PREC_FP32] + 3
 
 
-@pytest.mark.parametrize("prec", [gas.PREC_FP32, gas.PREC_MIXED_F16])
-def test_mfma_coarsest_product_matches_vector_path(ctx, small, prec):
+@pytest.mark.parametrize("grid,prec", [("small", gas.PREC_FP32), ("small", gas.PREC_MIXED_F16),
+                                       ("small", gas.PREC_MIXED_BF16), ("house", gas.PREC_MIXED_F16)])
+def test_mfma_coarsest_product_matches_vector_path(ctx, small, house, grid, prec):
     """A batch of 40 systems on one grid takes the coarsest product on the FP64
     matrix cores (k_vc_coarse_mfma); each system solved alone takes the vector
     kernel.  The two sum the product in different orders, so they agree to
     FP64 rounding of the coarsest solve, not bit for bit: the same iteration
     counts within one, solutions within 1e-6 of each other, and the FP64
-    oracle within the 1e-5 bar."""
-    g, H = small
+    oracle within the 1e-5 bar.  The small house's coarsest level has 455
+    unknowns, the 80x50x40 house's 712: a partial last row tile and column
+    chunk in both."""
+    g, H = small if grid == "small" else house
     G = gas.Grid(ctx, g, precision=prec)
     nsys = 40
     bs = [sc.rhs_from_divergence(g, sc.divergence_field(g, 900 + s), DT) for s in range(nsys)]
@@ -339,21 +342,24 @@ def test_mfma_coarsest_product_matches_vector_path(ctx, small, prec):
     B = gas.Batch(ctx, [G] * nsys)
     dbs = [ctx.to_device(b) for b in bs]
     dxs = [ctx.array(g.n) for _ in bs]
-    ctx.profile(True, classes=["vc_coarse"])
+    ctx.profile(True, classes=["vc_coarse", "vc_coarse_mfma"])
     ctx.profile_reset()
     stats = B.solve(dbs, dxs, p)
     prof = ctx.profile_read()
-    ctx.profile(False)
-    assert prof["vc_coarse"][1] > 0
+    assert prof["vc_coarse_mfma"][1] > 0 and prof["vc_coarse"][1] == 0   # the whole batch on the matrix cores
     assert all(s["status"] == gas.CONVERGED for s in stats)
     xb = [d.download() for d in dxs]
     B.destroy()
     for k in range(0, nsys, 8):
         db, dx = ctx.to_device(bs[k]), ctx.array(g.n)
+        ctx.profile_reset()
         st = G.solve(db, dx, p)
+        prof = ctx.profile_read()
+        assert prof["vc_coarse_mfma"][1] == 0 and prof["vc_coarse"][1] > 0   # alone: the vector kernel
         xa = dx.download()
         assert abs(st["iters"] - stats[k]["iters"]) <= 1, (k, st["iters"], stats[k]["iters"])
         assert rel(xb[k], xa) < 1e-6, k
+    ctx.profile(False)
     for k in (0, nsys - 1):
         x64, _, _ = H.pcg64(bs[k])
         assert rel(xb[k], x64) < 1e-5, k
