@@ -122,6 +122,7 @@ struct DLevel {
     const uint8_t* mask;          // level 0, grids whose coefficients are all 0 or 1: bit d set when
                                   // face d (-x,+x,-y,+y,-z,+z) has coefficient 1; else null
     const int* agg;               // unknown -> next-level unknown (-1: none)
+    const uint32_t* aggm;         // level 0 of a 0/1 grid: agg (0xFFFFFF: none) | face mask << 24
     const int* cptr;              // children of next-level unknowns into this level
     const int* cidx;
     const void* cv;               // coarse levels under 65536 rows: ELL column and value in one word

@@ -1411,8 +1411,8 @@ __global__ void __launch_bounds__(NT) k_post_fused(const DSys* __restrict__ sys,
     struct Raw {
         ZP z[P2];
         f32x2 r[P2];
-        i32x2 a[P2];      // level-1 aggregates of the pair; a.x becomes the red cell's once gathered
-        uint16_t m[P2];   // mask bytes of the pair: first cell in bits 0-7
+        i32x2 a[P2];      // the pair's level-1 aggregates (0xFFFFFF: none) | face masks << 24
+        int g[P2];        // the red cell's aggregate, once gathered
         VT c[P2];         // level-1 value of the red cell's aggregate
         float q;          // q of the own column
     };
@@ -1423,8 +1423,7 @@ __global__ void __launch_bounds__(NT) k_post_fused(const DSys* __restrict__ sys,
             const unsigned i2 = (ca[m] + base) >> 1;
             X.z[m] = gld(reinterpret_cast<const ZP*>(z), i2);
             X.r[m] = gld(reinterpret_cast<const f32x2*>(rv), i2);
-            X.a[m] = gld(reinterpret_cast<const i32x2*>(L.agg), i2);
-            X.m[m] = gld(reinterpret_cast<const uint16_t*>(L.mask), i2);
+            X.a[m] = gld(reinterpret_cast<const i32x2*>(L.aggm), i2);
         }
         X.q = gld(s.q, cown + base);
     };
@@ -1432,9 +1431,9 @@ __global__ void __launch_bounds__(NT) k_post_fused(const DSys* __restrict__ sys,
 #pragma unroll
         for (int m = 0; m < P2; ++m) {
             const bool ared = ((par2[m] + p) & 1) == 0;
-            const int g = ared ? X.a[m].x : X.a[m].y;
-            X.a[m].x = g;
-            X.c[m] = gld(t1, (unsigned)max(g, 0));
+            const int g = (ared ? X.a[m].x : X.a[m].y) & 0xFFFFFF;
+            X.g[m] = g;
+            X.c[m] = gld(t1, g == 0xFFFFFF ? 0u : (unsigned)g);
         }
     };
     // stage plane p: red cells z + t1[agg] (the prolongation), black cells r zs
@@ -1451,11 +1450,11 @@ __global__ void __launch_bounds__(NT) k_post_fused(const DSys* __restrict__ sys,
             const bool oka = pin && ina[m], okb = pin && inb[m];
             const bool ared = ((par2[m] + p) & 1) == 0;   // the first cell is red
             const float zr = (float)(ared ? X.z[m].x : X.z[m].y) +
-                             (X.a[m].x >= 0 ? VC_LEVEL_SCALE * to_f(X.c[m]) : 0.0f);
+                             (X.g[m] != 0xFFFFFF ? VC_LEVEL_SCALE * to_f(X.c[m]) : 0.0f);
             const float va = ared ? zr : X.r[m].x * zs, vb = ared ? X.r[m].y * zs : zr;
             *reinterpret_cast<float2*>(R + 2 * u) = float2{oka ? va : 0.0f, okb ? vb : 0.0f};
             *reinterpret_cast<uint16_t*>(M + 2 * u) =
-                (uint16_t)((oka ? (X.m[m] & 255u) : 0u) | ((okb ? ((unsigned)X.m[m] >> 8) : 0u) << 8));
+                (uint16_t)((oka ? ((unsigned)X.a[m].x >> 24) : 0u) | ((okb ? ((unsigned)X.a[m].y >> 24) : 0u) << 8));
             const int o = 2 * u - 2 * nx;
             if (o >= 0 && o < NT) *reinterpret_cast<float2*>(O + o) = float2{X.r[m].x, X.r[m].y};
         }

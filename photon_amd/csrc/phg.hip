@@ -94,6 +94,7 @@ struct GridDev {
     int* gs_lptr = nullptr;
     float* invd = nullptr;        // 1/diag of level 0, 0 where diag <= 0
     uint8_t* mask = nullptr;      // level-0 face mask when every coefficient is 0 or 1
+    uint32_t* aggm = nullptr;     // with it, each cell's level-1 aggregate | mask << 24 (one load for both)
     bool binary = false;
     int nclosed = 0, nztask = 0;
     int *zcells = nullptr, *zclosed = nullptr, *ztptr = nullptr;
@@ -1047,6 +1048,15 @@ int build_grid_dev(phg_ctx* ctx, int nx, int ny, int nz, float dx, const float* 
                         }
                 UP(G->mask, m.data(), m.size())
                 G->binary = true;
+                // the fused post-smoothing reads a cell's aggregate and mask
+                // in one load: aggregates under 2^24 - 1 (0xFFFFFF: none)
+                if (G->H.L.size() > 1 && G->H.L[1].n < 0xFFFFFF) {
+                    const std::vector<int>& agg = G->H.L[0].agg;
+                    std::vector<uint32_t> am(G->n);
+                    for (int c = 0; c < G->n; ++c)
+                        am[c] = (agg[c] < 0 ? 0xFFFFFFu : (uint32_t)agg[c]) | (uint32_t)m[c] << 24;
+                    UP(G->aggm, am.data(), am.size())
+                }
             }
         }
         // closed zones: cell lists, cut into chunks, for the deterministic mean removal
@@ -1366,6 +1376,7 @@ int phg_batch_create_ex(phg_ctx* ctx, int nsys, const int* grids, const phg_batc
                     D.ayc = G->ayc;
                     D.azc = G->azc;
                     D.mask = G->mask;
+                    D.aggm = G->aggm;
                     D.r = slab;
                     slab += n0 * vts;
                 } else {
@@ -1444,7 +1455,7 @@ int phg_batch_create_ex(phg_ctx* ctx, int nsys, const int* grids, const phg_batc
             for (const int2& e : logical) {
                 const GridDev* G = get_grid(ctx, grids[e.x]);
                 const bool f = B->binary && !B->unfused && G->nx <= FUSE_MAX_NX && (G->nx * G->ny) % 2 == 0 &&
-                               B->h_sys[e.x].nu0 == 1;
+                               B->h_sys[e.x].nu0 == 1 && G->aggm;
                 if (f) {
                     lf.push_back(e);
                     const int h = e.y % tiles(G->nx * G->ny);
