@@ -581,6 +581,193 @@ __global__ void __launch_bounds__(NT) k_dir_apply(const DSys* __restrict__ sys, 
     }
 }
 
+// k_dir_apply for 0/1 grids with an even plane size, on column pairs: a
+// block of TB threads owns the 2 TB columns [c0, c0 + 2 TB) of one k-chunk
+// (the even column tiles, like the 2 TB-thread form) and each lane the two
+// adjacent columns c0 + 2 tid and c0 + 2 tid + 1.  A pair starts at an even
+// cell index, so z, d_old and d_new are one 4-byte access per lane in FP16
+// (8 bytes in FP32), q one 8-byte access and the face masks one 2-byte
+// access: half the memory instructions of one column per lane, each covering
+// twice the bytes.  A march over FP16 vectors with one 2-byte access per lane
+// moves bytes at about half the rate of 4-byte accesses, so the pairing is
+// what lets the FP16 direction pay.
+// The staged plane holds the own columns and nxe = nx rounded up to even
+// columns on either side (pairs stay aligned for an odd nx); the halo needs
+// nxe <= TB.  The d.q terms of each column are summed along k as in
+// k_dir_apply, then handed through LDS to the lane that owns the column in
+// the 2 TB-thread form, whose wave and tile sums they follow, so the partials,
+// and with them every result, are bitwise those of k_dir_apply.  x is read and
+// written one cell per lane (the caller's vectors need only 4-byte alignment).
+template <class ZT>
+__global__ void __launch_bounds__(TB) k_dir_apply_pair(const DSys* __restrict__ sys, const int2* __restrict__ tab) {
+    PHG_SYS_PROLOGUE
+    using P2 = typename pair_of<ZT>::type;
+    const DLevel& L = s.L[0];
+    const int nx = L.nx, ny = L.ny, nz = L.nz, sxy = nx * ny;
+    const int nct = (sxy + TB - 1) / TB;
+    const int tid = (int)threadIdx.x;
+    const int c0 = (e.y % nct) * TB;   // an even column tile: a multiple of 2 TB
+    const int kc = e.y / nct;
+    const int k0 = kc * s.kchunk, k1 = min(nz, k0 + s.kchunk);
+    const int nxe = (nx + 1) & ~1;
+    const int W = 2 * TB + 2 * nxe;
+    const int ca = c0 + 2 * tid;                 // the lane's first column
+    const bool valid = ca < sxy;                 // sxy is even: both columns or neither
+    const int cown = valid ? ca : sxy - 2;
+    // the lane's halo pair: left of the own columns for hh < nxe / 2, right after
+    const int hh = tid;
+    const int hu = hh < nxe / 2 ? hh : hh + TB;
+    const int hcol = c0 - nxe + 2 * hu;
+    const bool hin = hh < nxe && hcol >= 0 && hcol < sxy;
+    const int hcl = hin ? hcol : cown;
+    const int hslot = 2 * hu;                    // its W index
+    // (i, j) edge bits of the two own cells (the second may start the next row)
+    unsigned ijb[2];
+    int ci[2], cj[2];
+#pragma unroll
+    for (int h = 0; h < 2; ++h) {
+        const int c = cown + h;
+        cj[h] = c / nx;
+        ci[h] = c - cj[h] * nx;
+        ijb[h] = (ci[h] > 0 ? 1u : 0u) | (ci[h] < nx - 1 ? 2u : 0u) | (cj[h] > 0 ? 4u : 0u) | (cj[h] < ny - 1 ? 8u : 0u);
+    }
+
+    const int it = st_uni(s, ST_ITERS);
+    const bool first = it == 0;
+    constexpr bool scaled = !std::is_same<ZT, float>::value;
+    ZT* __restrict__ dnew = static_cast<ZT*>((it & 1) ? s.d1 : s.d0);
+    const ZT* __restrict__ dold = static_cast<const ZT*>((it & 1) ? s.d0 : s.d1);
+    const bool xup = it >= 2 && !(it & 1);
+    const float beta = (float)s.scal[S_BETA], alpha = (float)s.scal[S_ALPHA], alpha2 = (float)s.scal[S_ALPHA2];
+    const ZT* __restrict__ z = static_cast<const ZT*>(s.z);
+    const float izs = (float)(1.0 / s.scal[S_ZS]);
+    const float ds = scaled ? (float)s.scal[S_ZS] : 1.0f, ids = scaled ? izs : 1.0f;
+    const float idso = scaled && !first ? (float)(1.0 / s.scal[S_DS]) : 1.0f;
+    const float idso2 = scaled && xup ? (float)(1.0 / s.scal[S_DS2]) : 1.0f;
+    auto fin1 = [&](float zv, float dv) {
+        const float d = first ? zv * izs : fmaf(beta, dv * idso, zv * izs);
+        return scaled ? zq<ZT>(d * ds) * ids : d;
+    };
+    struct Raw {
+        P2 z, d, hz, hd;
+    };
+    auto load = [&](int k, Raw& X) {
+        const unsigned base = (unsigned)sxy * (unsigned)min(max(k, 0), nz - 1);
+        const unsigned io = (cown + base) >> 1, ih = (hcl + base) >> 1;
+        X.z = gld(reinterpret_cast<const P2*>(z), io);
+        X.d = gld(reinterpret_cast<const P2*>(dold), io);
+        X.hz = gld(reinterpret_cast<const P2*>(z), ih);
+        X.hd = gld(reinterpret_cast<const P2*>(dold), ih);
+    };
+    auto fin_own = [&](int k, const Raw& X) {
+        const bool in = valid && k >= 0 && k < nz;
+        return float2{in ? fin1((float)X.z.x, (float)X.d.x) : 0.0f, in ? fin1((float)X.z.y, (float)X.d.y) : 0.0f};
+    };
+    auto fin_halo = [&](int k, const Raw& X) {
+        const bool in = hin && k < nz;
+        return float2{in ? fin1((float)X.hz.x, (float)X.hd.x) : 0.0f, in ? fin1((float)X.hz.y, (float)X.hd.y) : 0.0f};
+    };
+    struct Op {
+        uint16_t m;   // face masks of the pair, first cell in bits 0-7
+        float x[2], d2[2];
+    };
+    auto opnd = [&](int k) {
+        Op o;
+        const unsigned c = cown + (unsigned)sxy * (unsigned)min(max(k, 0), nz - 1);
+        o.m = gld(reinterpret_cast<const uint16_t*>(L.mask), c >> 1);
+        if (xup) {
+            const P2 d2 = gld(reinterpret_cast<const P2*>(dnew), c >> 1);
+            o.x[0] = gld(s.x, c);
+            o.x[1] = gld(s.x, c + 1);
+            o.d2[0] = (float)d2.x * idso2;
+            o.d2[1] = (float)d2.y * idso2;
+        }
+        return o;
+    };
+    extern __shared__ float lds[];
+    double acc[2] = {0.0, 0.0};
+    Raw X0, X1, X2;
+    float2 vprev, vcur, hcur;
+    P2 dcur;   // d_old of the own pair at plane k, as stored
+    {
+        Raw Z;
+        load(k0 - 1, Z);
+        vprev = fin_own(k0 - 1, Z);
+        load(k0, Z);
+        vcur = fin_own(k0, Z);
+        hcur = fin_halo(k0, Z);
+        dcur = Z.d;
+    }
+    load(k0 + 1, X0);
+    load(k0 + 2, X1);
+    Op P0 = opnd(k0), P1 = opnd(k0 + 1), P2r{};
+    auto step = [&](int k, const Raw& A, Raw& Cdst, const Op& o, Op& pdst) {
+        float* buf = lds + (k & 1) * W;
+        *reinterpret_cast<float2*>(buf + nxe + 2 * tid) = vcur;
+        if (hh < nxe) *reinterpret_cast<float2*>(buf + hslot) = hcur;
+        __syncthreads();
+        const float2 vnext = fin_own(k + 1, A), hnext = fin_halo(k + 1, A);
+        const P2 dnext = A.d;
+        load(min(k + 3, k1), Cdst);
+        pdst = opnd(min(k + 2, k1 - 1));
+        if (valid) {
+            const unsigned kb = (k > 0 ? 16u : 0u) | (k < nz - 1 ? 32u : 0u);
+            const float* p = buf + nxe + 2 * tid;
+            const float vc[2] = {vcur.x, vcur.y}, vp[2] = {vprev.x, vprev.y}, vn[2] = {vnext.x, vnext.y};
+            const float dd[2] = {(float)dcur.x * idso, (float)dcur.y * idso};
+            const unsigned c = (unsigned)cown + (unsigned)sxy * (unsigned)k;
+            float y[2];
+#pragma unroll
+            for (int h = 0; h < 2; ++h) {
+                const unsigned m = (h == 0 ? o.m : (unsigned)o.m >> 8) & 255u;
+                const float dg = (float)__popc(m);
+                y[h] = 0.0f;
+                if (dg > 0) y[h] = dg * vc[h] - lat_bits(m & (ijb[h] | kb), p + h, nx, vp[h], vn[h]);
+                if (xup) gst_nt(s.x, c + h, fmaf(alpha, dd[h], fmaf(alpha2, o.d2[h], o.x[h])));
+                acc[h] += (double)y[h] * vc[h];
+            }
+            P2 dst;
+            dst.x = (ZT)(vc[0] * ds);   // exact: vc * ds is a ZT value
+            dst.y = (ZT)(vc[1] * ds);
+            gst_nt(reinterpret_cast<P2*>(dnew), c >> 1, dst);
+            gst_nt(reinterpret_cast<f32x2*>(s.q), c >> 1, f32x2{y[0], y[1]});
+        }
+        vprev = vcur;
+        vcur = vnext;
+        hcur = hnext;
+        dcur = dnext;
+    };
+    int k = k0;
+    for (; k + 2 < k1; k += 3) {
+        step(k, X0, X2, P0, P2r);
+        step(k + 1, X1, X0, P1, P0);
+        step(k + 2, X2, X1, P2r, P1);
+    }
+    if (k < k1) step(k, X0, X2, P0, P2r);
+    if (k + 1 < k1) step(k + 1, X1, X0, P1, P0);
+    // partials: column c0 + m is thread m of the 2 TB-thread form; its wave
+    // sums are taken over the same lanes and added per TB-column tile in
+    // wave order
+    __shared__ double colacc[2 * TB];
+    __shared__ double sh[2 * TB / 64];
+    colacc[2 * tid] = acc[0];
+    colacc[2 * tid + 1] = acc[1];
+    __syncthreads();
+    const int lane = tid & 63, w = tid >> 6;
+    const double v0 = wave_sum(colacc[tid]), v1 = wave_sum(colacc[tid + TB]);
+    if (lane == 0) {
+        sh[w] = v0;
+        sh[TB / 64 + w] = v1;
+    }
+    __syncthreads();
+    if (lane == 0 && w < 2 && (w == 0 || c0 + TB < sxy)) {
+        double a = 0.0;
+#pragma unroll
+        for (int ww = 0; ww < TB / 64; ++ww) a += sh[w * (TB / 64) + ww];
+        s.part[(long)(e.y + w) * NPART] = a;
+    }
+}
+
 // after the loop: the x += alpha d updates systems that stopped still owe
 // (ST_XPEND: the last direction, d in buffer (it - 1) & 1 with S_ALPHA and
 // S_DS, and for 2 first the one before it, with S_ALPHA2 and S_DS2)

@@ -164,6 +164,47 @@ def test_fused_presmoothing_is_bitwise_the_separate_passes(ctx, small, house, pr
     G.destroy()
 
 
+@pytest.mark.parametrize("prec", [gas.PREC_MIXED_F16, gas.PREC_MIXED_BF16])
+@pytest.mark.parametrize("grid,kchunk,warm", [("small", 3, False), ("house", 7, True), ("house", 40, False),
+                                              ("odd37", 3, True), ("odd131", 4, False), ("wide", 5, False)])
+def test_direction_update_on_column_pairs_is_bitwise_one_column_per_thread(ctx, small, house, prec, grid, kchunk,
+                                                                           warm):
+    """In the mixed precisions (FP16 direction) the direction update runs on
+    column pairs by default for 0/1 grids with an even plane size up to 256
+    cells wide (k_dir_apply_pair: 256 threads, two adjacent cells per lane);
+    a batch that asks for 512-thread blocks keeps one column per thread.  The pair form sums each column's
+    d.q terms along k and reduces them over the lanes and tiles the
+    one-column form uses, so whole solves agree bit for bit: iteration
+    counts and solutions, with x read and written on every second iteration
+    (warm starts included), for pairs that straddle a row end (odd nx), a
+    grid 256 cells wide (the widest the pair form takes) and k-chunks that
+    cut the grid unevenly."""
+    if grid in ODD_NX:
+        g = sc.house(*ODD_NX[grid])
+    elif grid == "wide":
+        g = sc.house(14, 256, 6, 9)
+    else:
+        g, _ = small if grid == "small" else house
+    assert (g.nx * g.ny) % 2 == 0 and g.nx <= 256
+    G = gas.Grid(ctx, g, precision=prec)
+    bs = [sc.rhs_from_divergence(g, sc.divergence_field(g, 700 + s), DT) for s in range(4)]
+    x0 = [(0.01 * sc.divergence_field(g, 900 + s)).astype(np.float32) for s in range(4)]
+    p = gas.solve_params(tol=1e-6, max_iters=200, zero_x=not warm)
+    res = []
+    for width in (0, 512):
+        B = gas.Batch(ctx, [G] * len(bs), kchunk=kchunk, fused_threads=width)
+        dbs = [ctx.to_device(b) for b in bs]
+        dxs = [ctx.to_device(x) if warm else ctx.array(g.n) for x in x0]
+        stats = B.solve(dbs, dxs, p)
+        assert all(s["status"] == gas.CONVERGED for s in stats)
+        res.append(([d.download() for d in dxs], [s["iters"] for s in stats]))
+        B.destroy()
+    assert res[0][1] == res[1][1]
+    for k, (a, b) in enumerate(zip(res[0][0], res[1][0])):
+        assert np.array_equal(a, b), f"system {k}"
+    G.destroy()
+
+
 @pytest.mark.parametrize("prec", [gas.PREC_FP32, gas.PREC_MIXED_F16])
 @pytest.mark.parametrize("width", [0, 256])
 def test_batch_splits_smoothing_per_system(ctx, prec, width):
