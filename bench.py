@@ -35,6 +35,12 @@ METRIC = "mixed vs FP32 MG-PCG solves/s at 1e-6 residual; % HBM roofline"
 HBM_PEAK = 8.0e12   # B/s, MI355X
 DT = 0.01
 TOL = 1e-6
+# Every GPU solve of the bench (plain FP32, FP32 and mixed batches, the
+# 1.28M-cell, 64-scenario and serving workloads) weights the V-cycle's
+# coarse-grid corrections by the library's over-correction weight, so the
+# precision and batching ratios compare one method; the CPU baseline is the
+# reference's own Solver::project.
+from photon_amd.gas import COARSE_WEIGHT as CW  # noqa: E402
 
 
 LEVEL0 = ["dir_apply", "pre_smooth", "pre_red", "rb_black", "resid0", "post_smooth", "post_black", "post_red"]
@@ -140,7 +146,7 @@ def run_batch(ctx, gas, grid_scen, prec, bs_host, steps, warmup, profile, sample
     x_all = ctx.array((nsys, n))
     bp = [b_all.ptr + 4 * n * s for s in range(nsys)]
     xp = [x_all.ptr + 4 * n * s for s in range(nsys)]
-    params = gas.solve_params(tol=TOL, max_iters=400, demean=True, check_every=4, zero_x=True)
+    params = gas.solve_params(tol=TOL, max_iters=400, demean=True, check_every=4, zero_x=True, coarse_weight=CW)
     for _ in range(warmup):
         B.solve(bp, xp, params)
     ctx.synchronize()
@@ -207,7 +213,8 @@ def run_plain(ctx, gas, grid_scen, bs_host, nsolve):
     G = gas.Grid(ctx, grid_scen, precision=gas.PREC_FP32)
     db = ctx.array(n)
     dx = ctx.array(n)
-    params = gas.solve_params(tol=TOL, max_iters=400, demean=True, check_every=8, use_graph=True, zero_x=True)
+    params = gas.solve_params(tol=TOL, max_iters=400, demean=True, check_every=8, use_graph=True, zero_x=True,
+                              coarse_weight=CW)
     db.upload(bs_host[0])
     G.solve(db, dx, params)   # warm-up, graph capture
     dbs = [ctx.to_device(bs_host[k % len(bs_host)]) for k in range(min(nsolve, len(bs_host)))]
@@ -281,13 +288,15 @@ def run_batching(ctx, gas, sc, nscen=64, steps=5):
     Gs = [gas.Grid(ctx, g, precision=gas.PREC_MIXED_F16) for g in grids]
     dbs = [ctx.to_device(b) for b in rhs]
     dxs = [ctx.array(g.n) for g in grids]
-    first = gas.solve_params(tol=TOL, max_iters=400, zero_x=True, use_graph=2, check_every=16, refine=2)
+    first = gas.solve_params(tol=TOL, max_iters=400, zero_x=True, use_graph=2, check_every=16, refine=2,
+                             coarse_weight=CW)
 
     def timed(kcs, sws):
         B = gas.Batch(ctx, Gs, sys_kchunk=kcs, sys_sweeps0=sws)
         st = B.solve(dbs, dxs, first)       # the first timestep
         n = max(4, -(-max(s["iters"] for s in st) // 4) * 4)
-        params = gas.solve_params(tol=TOL, max_iters=400, zero_x=True, use_graph=2, check_every=n, refine=2)
+        params = gas.solve_params(tol=TOL, max_iters=400, zero_x=True, use_graph=2, check_every=n, refine=2,
+                                  coarse_weight=CW)
         B.solve(dbs, dxs, params)           # graph capture at the new size
         ctx.synchronize()
         lat = []
@@ -690,7 +699,8 @@ def main():
         "config": {"workload": "house 80x50x40 cells at 0.2 m, 40 walls + 10 openings (seed 1), "
                                f"{args.nsys} independent RHS per step, ||r||/||b|| <= 1e-6, x0 = 0",
                    "cells": n, "solves_per_step": args.nsys, "precision": "FP16 V-cycle storage, FP32 CG vectors, "
-                   "FP64 dots/alpha/beta and coarsest solve", "parallelism": "batch of independent solves"},
+                   "FP64 dots/alpha/beta and coarsest solve", "parallelism": "batch of independent solves",
+                   "coarse_weight": CW},
         "fp32_batched_solves_per_s": None if fp32 is None else round(v_fp32, 2),
         "plain_fp32_solves_per_s": None if plain is None else round(v_plain, 2),
         "mixed_vs_fp32_batched": None if fp32 is None else round(v_mixed / v_fp32, 3),

@@ -87,6 +87,13 @@ typedef struct {
                             converges or finds its residual at the FP32 noise floor.  With
                             refine > 0 every residual formed from x, a warm start's included, is
                             accumulated in FP64 */
+    float coarse_weight; /* weight of every coarse-grid correction of the V-cycle (the prolongation
+                            adds w times the coarse level's correction), 0 <= w < 2; 0 or 1 keeps
+                            the reference's cycle (pressure_mg.cpp:293-295) bit for bit.  Piecewise-
+                            constant aggregation under-corrects smooth errors: w near 1.6 cuts the
+                            iterations to 1e-6 on walled grids by a third (30 -> 20 on the 80x50x40
+                            house).  Not the reference's method, so its iteration counts differ;
+                            the solution meets the same stop rule */
 } phg_solve_params;
 
 typedef struct {

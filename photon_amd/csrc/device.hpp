@@ -144,8 +144,10 @@ struct DLevel {
 // S_DS: the power-of-two scale the last direction d is stored with (the S_ZS of
 // its z; in FP32 d is stored unscaled); S_ALPHA2, S_DS2: alpha and the scale of
 // the direction before it
+// S_PW: the factor every prolongation applies to the coarse level's correction:
+// the level scale times the solve's coarse-correction weight (4 for weight 1)
 enum { S_RZ = 0, S_DQ, S_ALPHA, S_BETA, S_RR, S_BB, S_TOL, S_MAXR, S_INV, S_ZS, S_ZSN, S_RPREV, S_PNEXT, S_DS, S_ALPHA2, S_DS2,
-       S_NSCAL = 16 };
+       S_PW, S_NSCAL = 18 };
 // ST_XPEND: x += alpha d updates still owed by a stopped system (0, 1 or 2:
 // k_dir_apply applies them in pairs, on even iterations)
 // ST_FLEX: flexible beta (mixed precision); ST_VC: a CG iteration has run (q is defined)

@@ -655,6 +655,7 @@ int reset_state(phg_ctx* ctx, BatchDev* B, const phg_solve_params* P) {
         sc[S_INV] = P ? P->inv : 1.0f;
         sc[S_RZ] = 1.0;
         sc[S_ZS] = sc[S_ZSN] = 1.0;
+        sc[S_PW] = (double)VC_LEVEL_SCALE * (P && P->coarse_weight > 0 ? (double)P->coarse_weight : 1.0);
         st[ST_ACTIVE] = 1;
         st[ST_MAXIT] = P ? P->max_iters : 0;
         st[ST_RULE] = P && P->stop_rule == PHG_STOP_MAXNORM ? 1 : 0;
@@ -1655,6 +1656,8 @@ int phg_batch_solve(phg_ctx* ctx, int batch, const float* const* b, float* const
             return fail(ctx, PHG_EINVAL, "unknown stop rule %d", P->stop_rule);
         if (P->stop_rule == PHG_STOP_MAXNORM && !(P->inv > 0 && std::isfinite(P->inv)))
             return fail(ctx, PHG_EINVAL, "PHG_STOP_MAXNORM needs inv = dx^2/dt > 0 (got %g)", (double)P->inv);
+        if (!(P->coarse_weight >= 0 && P->coarse_weight < 2))
+            return fail(ctx, PHG_EINVAL, "coarse_weight must lie in [0, 2) (got %g)", (double)P->coarse_weight);
         HIP_OK(hipSetDevice(ctx->device));
         for (int s = 0; s < B->nsys; ++s) {
             if (!b[s] || !x[s]) return fail(ctx, PHG_EINVAL, "null vector for system %d", s);

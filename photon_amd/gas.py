@@ -22,6 +22,8 @@ PREC_FP32, PREC_MIXED_F16, PREC_MIXED_BF16 = 0, 1, 2
 STOP_REL_L2, STOP_MAXNORM = 0, 1
 CONVERGED, MAXITER, BREAKDOWN, BREAKDOWN_RZ, NONFINITE = 1, 2, 3, 4, 5
 COARSE_EXACT, COARSE_GS20 = 0, 1
+# coarse-grid correction weight of the over-corrected V-cycle (solve_params)
+COARSE_WEIGHT = 1.6
 
 EXPORTS = (
     "phg_create", "phg_destroy", "phg_last_error", "phg_synchronize", "phg_stream", "phg_malloc", "phg_free",
@@ -47,7 +49,7 @@ class PhgError(RuntimeError):
 class SolveParams(C.Structure):
     _fields_ = [("tol", C.c_double), ("max_iters", C.c_int), ("stop_rule", C.c_int), ("inv", C.c_float),
                 ("demean", C.c_int), ("check_every", C.c_int), ("use_graph", C.c_int), ("zero_x", C.c_int),
-                ("refine", C.c_int)]
+                ("refine", C.c_int), ("coarse_weight", C.c_float)]
 
 
 class BatchOpts(C.Structure):
@@ -271,12 +273,15 @@ class Context:
 
 
 def solve_params(tol=1e-6, max_iters=200, stop_rule=STOP_REL_L2, inv=1.0, demean=True, check_every=4,
-                 use_graph=False, zero_x=False, refine=0):
+                 use_graph=False, zero_x=False, refine=0, coarse_weight=0.0):
     """use_graph: False/0 direct launches, True/1 one captured iteration
-    replayed, 2 the whole solve (set-up + check_every iterations) as one graph."""
+    replayed, 2 the whole solve (set-up + check_every iterations) as one graph.
+    coarse_weight: weight of the V-cycle's coarse-grid corrections (0 or 1:
+    the reference's cycle; COARSE_WEIGHT over-corrects, taking about a third
+    fewer iterations)."""
     g = int(use_graph) if not isinstance(use_graph, bool) else (1 if use_graph else 0)
     return SolveParams(tol, max_iters, stop_rule, inv, 1 if demean else 0, check_every, g, 1 if zero_x else 0,
-                       refine)
+                       refine, coarse_weight)
 
 
 class GridPlan:
@@ -525,14 +530,19 @@ class GasSolver:
     With `resident_zones`, the zones' background pressures and rise rates stay
     on the device (phg_zone_*_resident), so a step queues its work without
     waiting for the device; zoneP and zoneDP on the host are refreshed by
-    zones_from_device() (snapshots do so themselves)."""
+    zones_from_device() (snapshots do so themselves).
+
+    `coarse_weight` (0: the reference's V-cycle) weights the V-cycle's
+    coarse-grid corrections; COARSE_WEIGHT takes about a third fewer
+    iterations to the same stop rule, with iteration counts that then differ
+    from the reference's."""
 
     CELL_FIELDS = ("p", "S", "Pb", "Q", "Qr", "hrr", "massSrc", "nu") + FIELD_NAMES
     BYTE_FIELDS = ("pilot", "lit")
 
     def __init__(self, ctx: Context, scen, precision=PREC_FP32, wind_closed=None, tol=1e-6, max_iters=400,
                  stop_rule=STOP_REL_L2, plan: GridPlan | None = None, graph_steps=True, coarse=None, wind=None,
-                 radiation_on=False, dt_max_profile=0.05, cfl=0.8, resident_zones=False):
+                 radiation_on=False, dt_max_profile=0.05, cfl=0.8, resident_zones=False, coarse_weight=0.0):
         self.ctx = ctx
         self.scen = scen
         self.graph_steps = graph_steps
@@ -573,7 +583,7 @@ class GasSolver:
         self._tmp = [ctx.array(n) for _ in range(10)]
         self.d["b"] = ctx.array(n)       # right-hand side of the split projection (rhs / correct)
         self.d["diag"] = ctx.array(n)
-        self.params = solve_params(tol=tol, max_iters=max_iters, stop_rule=stop_rule)
+        self.params = solve_params(tol=tol, max_iters=max_iters, stop_rule=stop_rule, coarse_weight=coarse_weight)
         self.pressureIters = 0
         self.pressureResidual = 0.0
         self.pressureRelRes = 0.0

@@ -43,8 +43,13 @@ def solve(ctx, Gs, rhs, params, **opts):
     return out, stats
 
 
+@pytest.mark.parametrize("cw", [0.0, gas.COARSE_WEIGHT])
 @pytest.mark.parametrize("prec", [gas.PREC_FP32, gas.PREC_MIXED_F16])
-def test_mixed_settings_match_oracle_and_solo(ctx, scenarios, prec):
+def test_mixed_settings_match_oracle_and_solo(ctx, scenarios, prec, cw):
+    """Eight batch scenarios with mixed block depths and sweep counts, as one
+    whole-solve graph with two restarts: every solution within 1e-5 of the
+    FP64 oracle, and a system alone equal to it in the batch, with the
+    reference's V-cycle and with the over-corrected one (coarse_weight)."""
     grids, rhs = scenarios
     Gs = [gas.Grid(ctx, g, precision=prec) for g in grids]
     kcs = [[3, 0, 8, 5][k % 4] for k in range(len(grids))]
@@ -52,7 +57,8 @@ def test_mixed_settings_match_oracle_and_solo(ctx, scenarios, prec):
     # refine: FP32 CG can reach tol on its recursive residual while the true
     # residual of these point-source problems stalls near 1e-4; restarts from
     # b - A x bring the true residual, and the error, down
-    p = gas.solve_params(tol=1e-6, max_iters=300, zero_x=True, use_graph=2, check_every=40, refine=2)
+    p = gas.solve_params(tol=1e-6, max_iters=300, zero_x=True, use_graph=2, check_every=40, refine=2,
+                         coarse_weight=cw)
     xs, stats = solve(ctx, Gs, rhs, p, sys_kchunk=kcs, sys_sweeps0=sws)
     for k, g in enumerate(grids):
         x64, _, _ = po.Hierarchy(g).pcg64(rhs[k])

@@ -206,6 +206,106 @@ def test_direction_update_on_column_pairs_is_bitwise_one_column_per_thread(ctx, 
 
 
 @pytest.mark.parametrize("prec", [gas.PREC_FP32, gas.PREC_MIXED_F16])
+def test_coarse_weight_one_is_the_reference_cycle(ctx, house, prec):
+    """coarse_weight 1 and the default 0 both give the reference's V-cycle:
+    the prolongation then adds 4 t (the level scale), as before the weight
+    existed, so the solves agree bit for bit."""
+    g, _ = house
+    G = gas.Grid(ctx, g, precision=prec)
+    bs = [sc.rhs_from_divergence(g, sc.divergence_field(g, 40 + s), DT) for s in range(3)]
+    res = []
+    for cw in (0.0, 1.0):
+        B = gas.Batch(ctx, [G] * len(bs))
+        dbs = [ctx.to_device(b) for b in bs]
+        dxs = [ctx.array(g.n) for _ in bs]
+        st = B.solve(dbs, dxs, gas.solve_params(tol=1e-6, max_iters=200, zero_x=True, coarse_weight=cw))
+        res.append(([d.download() for d in dxs], [s["iters"] for s in st]))
+        B.destroy()
+    assert res[0][1] == res[1][1]
+    for a, b in zip(res[0][0], res[1][0]):
+        assert np.array_equal(a, b)
+    G.destroy()
+
+
+@pytest.mark.parametrize("prec", [gas.PREC_FP32, gas.PREC_MIXED_F16, gas.PREC_MIXED_BF16])
+def test_over_corrected_vcycle_meets_the_fp64_oracle_in_fewer_iterations(ctx, house, prec):
+    """With the coarse-grid corrections weighted by COARSE_WEIGHT the V-cycle
+    is no longer the reference's, so the bar is the solution: within 1e-5 of
+    the FP64 oracle (a direct-solve-pinned FP64 PCG, independent of the
+    preconditioner) at ||r||/||b|| <= 1e-6, with the true residual formed in
+    FP64 from the stored x at its FP32 floor, and in at most three quarters of
+    the reference cycle's iterations on the walled house."""
+    g, H = house
+    G = gas.Grid(ctx, g, precision=prec)
+    bs = [sc.rhs_from_divergence(g, sc.divergence_field(g, 123 + s), DT) for s in range(2)]
+    its = {}
+    for cw in (0.0, gas.COARSE_WEIGHT):
+        B = gas.Batch(ctx, [G] * len(bs))
+        dbs = [ctx.to_device(b) for b in bs]
+        dxs = [ctx.array(g.n) for _ in bs]
+        st = B.solve(dbs, dxs, gas.solve_params(tol=1e-6, max_iters=200, zero_x=True, coarse_weight=cw))
+        its[cw] = [s["iters"] for s in st]
+        assert all(s["status"] == gas.CONVERGED and s["relres"] <= 1e-6 for s in st)
+        if cw > 0:
+            for k, b in enumerate(bs):
+                x = dxs[k].download()
+                x64, _, _ = H.pcg64(b)
+                err, true = rel(x, x64), H.true_relres(b, x)
+                print(f"prec={prec} system {k}: iters {its[0.0][k]} -> {its[cw][k]}, err_fp64={err:.2e}, "
+                      f"true relres {true:.2e}")
+                assert err < 1e-5
+                assert true < 1e-4   # an FP32 x itself leaves ~1e-5 (so does the reference cycle)
+        B.destroy()
+    for a, b in zip(its[0.0], its[gas.COARSE_WEIGHT]):
+        assert b <= 0.75 * a
+    G.destroy()
+
+
+@pytest.mark.parametrize("prec", [gas.PREC_FP32, gas.PREC_MIXED_F16])
+def test_weighted_prolongations_agree_across_kernel_forms(ctx, small, house, prec):
+    """Every prolongation applies the weight: the fused post pass and the
+    separate black sweep at level 0, the LDS up pass and the per-level
+    launches on the coarse levels.  With coarse_weight = COARSE_WEIGHT the
+    fused and separate passes, and the LDS and per-level coarse forms, give
+    the same solves bit for bit, and a system alone equals it in a batch."""
+    for g in (small[0], house[0]):
+        G = gas.Grid(ctx, g, precision=prec)
+        bs = [sc.rhs_from_divergence(g, sc.divergence_field(g, 60 + s), DT) for s in range(3)]
+        p = gas.solve_params(tol=1e-6, max_iters=200, zero_x=True, coarse_weight=gas.COARSE_WEIGHT)
+        res = []
+        for opts in ({}, {"unfused": True}, {"coarse_lds": 2}, {"coarse_lds": 1}):
+            B = gas.Batch(ctx, [G] * len(bs), **opts)
+            dbs = [ctx.to_device(b) for b in bs]
+            dxs = [ctx.array(g.n) for _ in bs]
+            st = B.solve(dbs, dxs, p)
+            assert all(s["status"] == gas.CONVERGED for s in st)
+            res.append(([d.download() for d in dxs], [s["iters"] for s in st]))
+            B.destroy()
+        B = gas.Batch(ctx, [G])
+        db, dx = ctx.to_device(bs[1]), ctx.array(g.n)
+        st = B.solve([db], [dx], p)
+        res.append(([None, dx.download(), None], [None, st[0]["iters"], None]))
+        B.destroy()
+        for r in res[1:]:
+            for k in range(len(bs)):
+                if r[0][k] is None:
+                    continue
+                assert r[1][k] == res[0][1][k]
+                assert np.array_equal(r[0][k], res[0][0][k]), (g.nx, k)
+        G.destroy()
+
+
+@pytest.mark.parametrize("cw", [2.0, -0.5, float("nan")])
+def test_coarse_weight_outside_its_range_is_refused(ctx, small, cw):
+    g, _ = small
+    G = gas.Grid(ctx, g)
+    db, dx = ctx.to_device(np.ones(g.n, np.float32)), ctx.array(g.n)
+    with pytest.raises(gas.PhgError):
+        G.solve(db, dx, gas.solve_params(tol=1e-6, max_iters=10, coarse_weight=cw))
+    G.destroy()
+
+
+@pytest.mark.parametrize("prec", [gas.PREC_FP32, gas.PREC_MIXED_F16])
 @pytest.mark.parametrize("width", [0, 256])
 def test_batch_splits_smoothing_per_system(ctx, prec, width):
     """Fused passes are chosen per system: in one batch the grids of even
