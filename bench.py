@@ -581,9 +581,14 @@ def main():
                     help="run only the 1.28M-cell precision workload (for kernel traces under rocprofv3)")
     ap.add_argument("--serving-window", type=float, default=1.0, help="seconds of Poisson arrivals at 20/s")
     ap.add_argument("--serving-steps", type=int, default=3000, help="whole gas steps per served scenario")
+    ap.add_argument("--coarse-weight", type=float, default=None,
+                    help="weight of the V-cycle's coarse-grid corrections (default: the library's COARSE_WEIGHT)")
     ap.add_argument("--copy-gib", type=float, default=0.0,
                     help="also run the plain copy kernel over this many GiB (counter calibration)")
     args = ap.parse_args()
+    if args.coarse_weight is not None:
+        global CW
+        CW = args.coarse_weight
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for --gpus {args.gpus}); scenarios shard across GPUs "
               "with no exchange, so N GPUs would run N independent copies of this workload.", file=sys.stderr)

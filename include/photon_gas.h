@@ -90,9 +90,9 @@ typedef struct {
     float coarse_weight; /* weight of every coarse-grid correction of the V-cycle (the prolongation
                             adds w times the coarse level's correction), 0 <= w < 2; 0 or 1 keeps
                             the reference's cycle (pressure_mg.cpp:293-295) bit for bit.  Piecewise-
-                            constant aggregation under-corrects smooth errors: w near 1.6 cuts the
-                            iterations to 1e-6 on walled grids by a third (30 -> 20 on the 80x50x40
-                            house).  Not the reference's method, so its iteration counts differ;
+                            constant aggregation under-corrects smooth errors: w = 1.8 cuts the
+                            iterations to 1e-6 on walled grids by a third or more (29.5 -> 19 on the
+                            80x50x40 house, 38.6 -> 19.6 on the 160x100x80 one).  Not the reference's method, so its iteration counts differ;
                             the solution meets the same stop rule */
 } phg_solve_params;
 

@@ -164,8 +164,8 @@ __device__ __forceinline__ void tile_rows(const DSys& s, const int2& e, int c0, 
 // bitwise those of the unscaled cycle.
 // A solve may weight every coarse-grid correction by w (0 < w < 2): the
 // prolongation then adds 4 w times the coarse value (S_PW).  Piecewise-constant
-// aggregation under-corrects smooth errors, and a weight near 1.6 takes the
-// house workload from 30 CG iterations to 20; w = 1 is the reference's cycle,
+// aggregation under-corrects smooth errors, and a weight of 1.8 takes the
+// house workload from 29.5 CG iterations to 19; w = 1 is the reference's cycle,
 // bit for bit.  The symmetric cycle stays a symmetric positive definite
 // preconditioner for 0 < w < 2.
 constexpr float VC_LEVEL_SCALE = 4.0f;

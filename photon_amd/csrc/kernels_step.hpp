@@ -154,6 +154,7 @@ __global__ void __launch_bounds__(TB) k_energy(int n, float dx, float dt, const 
 namespace phg {
 // plain 16-byte-per-lane copy (calibration of the HBM byte counters)
 __global__ void __launch_bounds__(TB) k_copy16(const float4* __restrict__ src, float4* __restrict__ dst, long n) {
-    for (long i = (long)blockIdx.x * TB + threadIdx.x; i < n; i += (long)gridDim.x * TB) dst[i] = src[i];
+    const long i = (long)blockIdx.x * TB + threadIdx.x;   // one 16-byte element per thread
+    if (i < n) dst[i] = src[i];
 }
 }  // namespace phg

