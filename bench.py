@@ -51,7 +51,7 @@ TRAFFIC_FILE = "profiles/level0_house80x50x40_1000rhs_mixed_f16_hbm_traffic.txt"
 
 # kernel class (the library's profile name) -> the kernel that implements it
 # on the benchmarked grid
-KERNEL_OF_CLASS = {"pre_smooth": "k_pre_fused", "post_smooth": "k_post_fused"}
+KERNEL_OF_CLASS = {"pre_smooth": "k_pre_fused", "post_smooth": "k_post_fused", "dir_apply": "k_dir_apply_pair"}
 
 
 def measured_traffic(kernel):

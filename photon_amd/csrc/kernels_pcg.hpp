@@ -603,7 +603,8 @@ __global__ void __launch_bounds__(NT) k_dir_apply(const DSys* __restrict__ sys, 
 // k_dir_apply, then handed through LDS to the lane that owns the column in
 // the 2 TB-thread form, whose wave and tile sums they follow, so the partials,
 // and with them every result, are bitwise those of k_dir_apply.  x is read and
-// written one cell per lane (the caller's vectors need only 4-byte alignment).
+// written as a pair when the caller's vector is 8-byte aligned, else one cell
+// at a time (two 4-byte stores per lane write each line in two halves).
 template <class ZT>
 __global__ void __launch_bounds__(TB) k_dir_apply_pair(const DSys* __restrict__ sys, const int2* __restrict__ tab) {
     PHG_SYS_PROLOGUE
@@ -644,6 +645,7 @@ __global__ void __launch_bounds__(TB) k_dir_apply_pair(const DSys* __restrict__ 
     ZT* __restrict__ dnew = static_cast<ZT*>((it & 1) ? s.d1 : s.d0);
     const ZT* __restrict__ dold = static_cast<const ZT*>((it & 1) ? s.d0 : s.d1);
     const bool xup = it >= 2 && !(it & 1);
+    const bool xpair = (reinterpret_cast<uintptr_t>(s.x) & 7u) == 0;   // the same for the whole block
     const float beta = (float)s.scal[S_BETA], alpha = (float)s.scal[S_ALPHA], alpha2 = (float)s.scal[S_ALPHA2];
     const ZT* __restrict__ z = static_cast<const ZT*>(s.z);
     const float izs = (float)(1.0 / s.scal[S_ZS]);
@@ -683,8 +685,14 @@ __global__ void __launch_bounds__(TB) k_dir_apply_pair(const DSys* __restrict__ 
         o.m = gld(reinterpret_cast<const uint16_t*>(L.mask), c >> 1);
         if (xup) {
             const P2 d2 = gld(reinterpret_cast<const P2*>(dnew), c >> 1);
-            o.x[0] = gld(s.x, c);
-            o.x[1] = gld(s.x, c + 1);
+            if (xpair) {
+                const f32x2 xv = gld(reinterpret_cast<const f32x2*>(s.x), c >> 1);
+                o.x[0] = xv.x;
+                o.x[1] = xv.y;
+            } else {
+                o.x[0] = gld(s.x, c);
+                o.x[1] = gld(s.x, c + 1);
+            }
             o.d2[0] = (float)d2.x * idso2;
             o.d2[1] = (float)d2.y * idso2;
         }
@@ -722,15 +730,23 @@ __global__ void __launch_bounds__(TB) k_dir_apply_pair(const DSys* __restrict__ 
             const float vc[2] = {vcur.x, vcur.y}, vp[2] = {vprev.x, vprev.y}, vn[2] = {vnext.x, vnext.y};
             const float dd[2] = {(float)dcur.x * idso, (float)dcur.y * idso};
             const unsigned c = (unsigned)cown + (unsigned)sxy * (unsigned)k;
-            float y[2];
+            float y[2], xn[2];
 #pragma unroll
             for (int h = 0; h < 2; ++h) {
                 const unsigned m = (h == 0 ? o.m : (unsigned)o.m >> 8) & 255u;
                 const float dg = (float)__popc(m);
                 y[h] = 0.0f;
                 if (dg > 0) y[h] = dg * vc[h] - lat_bits(m & (ijb[h] | kb), p + h, nx, vp[h], vn[h]);
-                if (xup) gst_nt(s.x, c + h, fmaf(alpha, dd[h], fmaf(alpha2, o.d2[h], o.x[h])));
+                xn[h] = fmaf(alpha, dd[h], fmaf(alpha2, o.d2[h], o.x[h]));
                 acc[h] += (double)y[h] * vc[h];
+            }
+            if (xup) {
+                if (xpair) {
+                    gst_nt(reinterpret_cast<f32x2*>(s.x), c >> 1, f32x2{xn[0], xn[1]});
+                } else {
+                    gst_nt(s.x, c, xn[0]);
+                    gst_nt(s.x, c + 1, xn[1]);
+                }
             }
             P2 dst;
             dst.x = (ZT)(vc[0] * ds);   // exact: vc * ds is a ZT value

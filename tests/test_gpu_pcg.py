@@ -165,10 +165,11 @@ def test_fused_presmoothing_is_bitwise_the_separate_passes(ctx, small, house, pr
 
 
 @pytest.mark.parametrize("prec", [gas.PREC_MIXED_F16, gas.PREC_MIXED_BF16])
-@pytest.mark.parametrize("grid,kchunk,warm", [("small", 3, False), ("house", 7, True), ("house", 40, False),
-                                              ("odd37", 3, True), ("odd131", 4, False), ("wide", 5, False)])
+@pytest.mark.parametrize("grid,kchunk,warm,xoff", [("small", 3, False, 0), ("house", 7, True, 0),
+                                                   ("house", 40, False, 1), ("odd37", 3, True, 1),
+                                                   ("odd131", 4, False, 0), ("wide", 5, False, 0)])
 def test_direction_update_on_column_pairs_is_bitwise_one_column_per_thread(ctx, small, house, prec, grid, kchunk,
-                                                                           warm):
+                                                                           warm, xoff):
     """In the mixed precisions (FP16 direction) the direction update runs on
     column pairs by default for 0/1 grids with an even plane size up to 256
     cells wide (k_dir_apply_pair: 256 threads, two adjacent cells per lane);
@@ -176,9 +177,10 @@ def test_direction_update_on_column_pairs_is_bitwise_one_column_per_thread(ctx, 
     d.q terms along k and reduces them over the lanes and tiles the
     one-column form uses, so whole solves agree bit for bit: iteration
     counts and solutions, with x read and written on every second iteration
-    (warm starts included), for pairs that straddle a row end (odd nx), a
-    grid 256 cells wide (the widest the pair form takes) and k-chunks that
-    cut the grid unevenly."""
+    (warm starts included; as pairs, or one cell at a time when x is not
+    8-byte aligned), for pairs that straddle a row end (odd nx), a grid 256
+    cells wide (the widest the pair form takes) and k-chunks that cut the
+    grid unevenly."""
     if grid in ODD_NX:
         g = sc.house(*ODD_NX[grid])
     elif grid == "wide":
@@ -194,10 +196,12 @@ def test_direction_update_on_column_pairs_is_bitwise_one_column_per_thread(ctx, 
     for width in (0, 512):
         B = gas.Batch(ctx, [G] * len(bs), kchunk=kchunk, fused_threads=width)
         dbs = [ctx.to_device(b) for b in bs]
-        dxs = [ctx.to_device(x) if warm else ctx.array(g.n) for x in x0]
-        stats = B.solve(dbs, dxs, p)
+        # x at an offset of xoff floats inside a larger buffer (xoff = 1: 4-byte aligned only)
+        bufs = [ctx.to_device(np.concatenate([np.zeros(xoff, np.float32), x if warm else np.zeros_like(x)]))
+                for x in x0]
+        stats = B.solve(dbs, [d.ptr + 4 * xoff for d in bufs], p)
         assert all(s["status"] == gas.CONVERGED for s in stats)
-        res.append(([d.download() for d in dxs], [s["iters"] for s in stats]))
+        res.append(([d.download()[xoff:] for d in bufs], [s["iters"] for s in stats]))
         B.destroy()
     assert res[0][1] == res[1][1]
     for k, (a, b) in enumerate(zip(res[0][0], res[1][0])):
