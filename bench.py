@@ -207,6 +207,24 @@ def fp64_errors(grid_scen, bs_host, xs, threads=8):
             "sampled_systems": len(res)}
 
 
+def copy_rate(ctx, gib=1.0, reps=10):
+    """Bytes per second (read plus write) of the library's plain copy kernel,
+    one 16-byte element per lane over 1 GiB, by the host clock around `reps`
+    back-to-back launches after a warm-up one: the rate a plain stream reaches
+    on this GPU, beside which the kernels' fractions of the 8 TB/s peak read."""
+    nbytes = int(gib * (1 << 30)) // 16 * 16
+    src, dst = ctx.array(nbytes // 4), ctx.array(nbytes // 4)
+    ctx.copy(dst, src, nbytes)
+    ctx.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        ctx.copy(dst, src, nbytes)
+    ctx.synchronize()
+    dt = (time.perf_counter() - t0) / reps
+    del src, dst
+    return 2.0 * nbytes / dt
+
+
 def run_plain(ctx, gas, grid_scen, bs_host, nsolve):
     """Plain FP32 MG-PCG: one system at a time, iterations replayed from a hipGraph."""
     n = grid_scen.n
@@ -666,6 +684,9 @@ def main():
             "traffic_source": None if traffic is None else TRAFFIC_FILE,
             "algorithmic_bytes_per_launch": round(bpl), "avg_launch_us": round(1e3 * ms / max(nlaunch, 1), 2),
             "launches": nlaunch, "level0_algorithmic_GBps": {k: round(v, 1) for k, v in level0_gbs.items()}}
+    crate = copy_rate(ctx)
+    roof["copy_kernel_GBps"] = round(crate / 1e9, 1)    # measured here, 1 GiB each way
+    roof["frac_of_copy"] = round(achieved / crate, 4)
     time_share = {k: round(v[0] / sum(x[0] for x in prof_all.values()), 4) for k, v in prof_all.items() if v[0] > 0}
     ms_step = 1e3 * mixed["seconds"] / args.steps
     whole = whole_solve_traffic(ms_step)

@@ -475,6 +475,7 @@ void GasPhase::project(float dt) {   // gasphase.cpp:505-677
     prm.max_iters = profile.pressureMaxIters;
     prm.stop_rule = PHG_STOP_MAXNORM;
     prm.demean = 1;
+    prm.coarse_weight = profile.pressureCoarseWeight;
     // the set-up and iterations as one graph sized from the last step's count
     prm.use_graph = 2;
     prm.check_every = pressureIters > 0 ? (pressureIters + 4) / 4 * 4 : 8;

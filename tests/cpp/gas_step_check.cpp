@@ -115,6 +115,7 @@ int main(int argc, char** argv) {
         gp.syncToDevice();
         gp.profile.pressureTol = 0.0f;
         gp.profile.pressureMaxIters = (int)num(meta, "pj_K");
+        gp.profile.pressureCoarseWeight = num(meta, "coarse_weight", 0.0f);
         gp.project(dt);
         gp.syncToHost();
         std::fprintf(log, "project %d %.9g %.9g\n", gp.pressureIters, gp.pressureResidual, gp.maxVel);

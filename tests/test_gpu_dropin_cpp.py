@@ -14,6 +14,7 @@ import subprocess
 import numpy as np
 import pytest
 
+from photon_amd import gas
 from photon_amd import scenarios as sc
 
 pytestmark = pytest.mark.gpu
@@ -89,6 +90,37 @@ def test_cpp_dropin_phases_match_reference(tmp_path, name):
     assert np.array_equal(out(d, "ad_T").view(np.uint32), G["ad_out_T"].view(np.uint32))
     for n in SPECIES:
         assert np.array_equal(out(d, "ad_" + n).view(np.uint32), G["ad_out_" + n].view(np.uint32)), n
+
+
+@pytest.mark.parametrize("name", ["room", "walls"])
+def test_cpp_dropin_coarse_weight_reaches_the_solve(ctx, tmp_path, name):
+    """Profile::pressureCoarseWeight goes into the solve: GasPhase::project
+    with weight 1.8 (the reference's coarsest sweeps, K iterations) equals the
+    Python GasSolver's projection with the same weight bit for bit, and
+    differs from the weight-0 projection (the reference's cycle, which the
+    golden p follows)."""
+    g = CASES[name]()
+    G = np.load(os.path.join(HERE, "golden", f"ref_{name}.npz"))
+    K = int(G["pj_K"])
+    arrays = {k: G[k] for k in ("pj_u0", "pj_v0", "pj_w0", "pj_S", "en_T0", "en_Q", "en_ms", "ad_nu", "pj_u", "pj_v",
+                                "pj_w")}
+    arrays.update({"ad_in_" + n: G["ad_in_" + n] for n in SPECIES})
+    outs = {}
+    for cw in (0.0, 1.8):
+        d = str(tmp_path / f"{name}_{cw}")
+        write_case(d, g, dict(dt=0.01, pj_K=K, coarse_weight=cw), arrays)
+        run("phases", d)
+        outs[cw] = {f: out(d, f) for f in ("p", "u", "v", "w")}
+    s = gas.GasSolver(ctx, g, tol=0.0, max_iters=K, stop_rule=gas.STOP_MAXNORM, coarse=gas.COARSE_GS20,
+                      coarse_weight=1.8)
+    for f in ("u", "v", "w"):
+        s.set(f, G["pj_" + f + "0"])
+    s.set("S", G["pj_S"])
+    s.project(0.01)
+    for f in ("p", "u", "v", "w"):
+        assert np.array_equal(outs[1.8][f], s.get(f)), f
+    assert not np.array_equal(outs[1.8]["p"], outs[0.0]["p"])
+    assert rel(outs[0.0]["p"], G["pj_p"]) < 1e-5
 
 
 @pytest.mark.parametrize("name", list(CASES))
